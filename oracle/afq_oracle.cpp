@@ -1,0 +1,834 @@
+// afq_oracle.cpp -- FP64 CPU restatement of SAFIRE's AFQMC walker step.
+//
+// THIS IS TEST INFRASTRUCTURE. It is the parity oracle and the CPU baseline: only the test
+// suite, the smoke check and the benchmark's cpu_baseline leg may load it. The product
+// (safire_amd/libafq_hip.so) never links, loads or calls anything in this directory.
+//
+// Every routine restates the reference algorithm it cites (paths relative to the SAFIRE tree).
+// Dense linear algebra that the reference delegates to BLAS/LAPACK through nda (gemm, getrf,
+// getri, geqrf, ungqr, eigh) is restated here as plain textbook loops: unblocked LU with
+// partial pivoting (LAPACK zgetf2 semantics), inverse by LU solves on the identity, Householder
+// QR (zgeqr2/zung2r conventions, real R diagonal), and cyclic Jacobi for the symmetric
+// eigenproblem. They agree with the library routines to rounding.
+//
+// Conventions (all row-major, complex = interleaved re/im doubles):
+//   PsiT  [n][N]          trial, stored conjugate-transposed as in NOMSD (NOMSD.hpp:292-344)
+//   Phi   [W][N][n]       walker Slater matrices (WalkerSetBase.h:320-328)
+//   G     [W][n][N]       compact mixed density matrix (density_matrix.hpp:576)
+//   Likn  [N][N][M] real  Cholesky vectors L^m_{ik} (RealDenseHamiltonian.cpp:121-127)
+//   Lnak  [M][n][N]       half-rotated Cholesky vectors (RealDenseHamiltonian.cpp:141-163)
+//   haj   [n][N]          half-rotated one-body hamiltonian (one_body.hpp half_rotate_hamiltonian)
+//   X, v  [W][M]          fields / force bias
+//   V     [W][N][N]       vHS
+//
+// Parity pins (tests/): test_sdet_ops.cpp golden values for Log_Overlap, MixedDensityMatrix and
+// inverse_logdet; closed forms for vbias/vHS/energy (traces against the full G); truncated
+// Taylor series against an explicit matrix power sum and expm; QR invariants and numpy QR;
+// Acklam probit against scipy ndtri; Philox4x32-10 against the Random123 known-answer vectors.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef std::complex<double> cd;
+
+namespace {
+
+const double kPi = 3.14159265358979323846;
+
+// ---------------------------------------------------------------------------------------------
+// small dense helpers
+// ---------------------------------------------------------------------------------------------
+
+// C[m][n] = alpha * sum_k opA(A)[m][k] opB(B)[k][n] + beta*C ; row-major, explicit strides.
+// op = 'N' (as stored, element (r,c) at r*ld + c) or 'T' (transposed: element (r,c) at c*ld + r).
+void zgemm(char ta, char tb, int m, int n, int k, cd alpha, const cd* A, int lda, const cd* B,
+           int ldb, cd beta, cd* C, int ldc) {
+  std::vector<cd> acc(n);
+  for (int i = 0; i < m; ++i) {
+    std::fill(acc.begin(), acc.end(), cd(0.0));
+    for (int p = 0; p < k; ++p) {
+      cd a = (ta == 'N') ? A[(size_t)i * lda + p] : A[(size_t)p * lda + i];
+      if (a == cd(0.0)) continue;
+      if (tb == 'N') {
+        const cd* b = B + (size_t)p * ldb;
+        for (int j = 0; j < n; ++j) acc[j] += a * b[j];
+      } else {
+        for (int j = 0; j < n; ++j) acc[j] += a * B[(size_t)j * ldb + p];
+      }
+    }
+    for (int j = 0; j < n; ++j) {
+      cd c = (beta == cd(0.0)) ? cd(0.0) : beta * C[(size_t)i * ldc + j];
+      C[(size_t)i * ldc + j] = c + alpha * acc[j];
+    }
+  }
+}
+
+inline double cabs1(cd z) { return std::fabs(z.real()) + std::fabs(z.imag()); }
+
+// LU with partial pivoting, LAPACK zgetf2 semantics (pivot = first max of |re|+|im|), in place
+// on a row-major n x n matrix. piv[i] is the 1-based row swapped with row i.
+// Returns info: 0, or the 1-based index of the first exactly-zero pivot.
+int lu_factor(cd* a, int n, int* piv) {
+  int info = 0;
+  for (int j = 0; j < n; ++j) {
+    int p = j;
+    double best = cabs1(a[(size_t)j * n + j]);
+    for (int i = j + 1; i < n; ++i) {
+      double v = cabs1(a[(size_t)i * n + j]);
+      if (v > best) { best = v; p = i; }
+    }
+    piv[j] = p + 1;
+    if (a[(size_t)p * n + j] != cd(0.0)) {
+      if (p != j)
+        for (int c = 0; c < n; ++c) std::swap(a[(size_t)j * n + c], a[(size_t)p * n + c]);
+      cd r = cd(1.0) / a[(size_t)j * n + j];
+      for (int i = j + 1; i < n; ++i) a[(size_t)i * n + j] *= r;
+    } else if (info == 0) {
+      info = j + 1;
+    }
+    for (int i = j + 1; i < n; ++i) {
+      cd l = a[(size_t)i * n + j];
+      if (l == cd(0.0)) continue;
+      for (int c = j + 1; c < n; ++c) a[(size_t)i * n + c] -= l * a[(size_t)j * n + c];
+    }
+  }
+  return info;
+}
+
+// log det from getrf, determinants_impl.hpp:33-75: res += sum_i log(+-U_ii + DBL_MIN), the sign
+// flipped where row i was swapped, then the imaginary part of the running total is wrapped into
+// [-pi, pi].
+void logdet_from_getrf(const cd* a, const int* piv, int n, cd& res) {
+  const cd small(DBL_MIN, 0.0);
+  for (int i = 0; i < n; ++i) {
+    cd u = a[(size_t)i * n + i];
+    if (piv[i] == i + 1)
+      res += std::log(u + small);
+    else
+      res += std::log(-u + small);
+  }
+  if (res.imag() > kPi) {
+    while (res.imag() > kPi) res.imag(res.imag() - 2 * kPi);
+  } else if (res.imag() < -kPi) {
+    while (res.imag() < -kPi) res.imag(res.imag() + 2 * kPi);
+  }
+}
+
+// getri_or_zero (nda_functions.hpp:422-437): the inverse from the LU factors, or all zeros when
+// the factorization reported a singular matrix. Computed by solving LU x = P e_j per column.
+void inverse_from_lu(cd* a, const int* piv, int n, int info) {
+  if (info != 0) {
+    std::fill(a, a + (size_t)n * n, cd(0.0));
+    return;
+  }
+  std::vector<cd> inv((size_t)n * n, cd(0.0));
+  std::vector<int> perm(n);
+  for (int i = 0; i < n; ++i) perm[i] = i;
+  for (int i = 0; i < n; ++i) std::swap(perm[i], perm[piv[i] - 1]);
+  // row i of P*A is row perm[i] of A; solve L U X = P for X = A^{-1}
+  std::vector<cd> col(n);
+  for (int j = 0; j < n; ++j) {
+    for (int i = 0; i < n; ++i) col[i] = (perm[i] == j) ? cd(1.0) : cd(0.0);
+    for (int i = 0; i < n; ++i)
+      for (int k = 0; k < i; ++k) col[i] -= a[(size_t)i * n + k] * col[k];
+    for (int i = n - 1; i >= 0; --i) {
+      for (int k = i + 1; k < n; ++k) col[i] -= a[(size_t)i * n + k] * col[k];
+      col[i] /= a[(size_t)i * n + i];
+    }
+    for (int i = 0; i < n; ++i) inv[(size_t)i * n + j] = col[i];
+  }
+  std::copy(inv.begin(), inv.end(), a);
+}
+
+// T = PsiT * Phi_w, LU, log det accumulated into ovlp, optional inverse
+// (density_matrix.hpp:113-156, log_overlap_impl with herm = true).
+void log_overlap_one(const cd* A, const cd* B, int N, int n, cd* T, cd& ovlp, bool invert) {
+  zgemm('N', 'N', n, n, N, 1.0, A, N, B, n, 0.0, T, n);
+  std::vector<int> piv(n);
+  int info = lu_factor(T, n, piv.data());
+  logdet_from_getrf(T, piv.data(), n, ovlp);
+  if (invert) inverse_from_lu(T, piv.data(), n, info);
+}
+
+// Acklam's rational approximation of the inverse normal CDF (probit.h:58-137).
+double probit(double p) {
+  static const double a[] = {-3.969683028665376e+01, 2.209460984245205e+02,
+                             -2.759285104469687e+02, 1.383577518672690e+02,
+                             -3.066479806614716e+01, 2.506628277459239e+00};
+  static const double b[] = {-5.447609879822406e+01, 1.615858368580409e+02,
+                             -1.556989798598866e+02, 6.680131188771972e+01,
+                             -1.328068155288572e+01};
+  static const double c[] = {-7.784894002430293e-03, -3.223964580411365e-01,
+                             -2.400758277161838e+00, -2.549732539343734e+00,
+                             4.374664141464968e+00,  2.938163982698783e+00};
+  static const double d[] = {7.784695709041462e-03, 3.224671290700398e-01,
+                             2.445134137142996e+00, 3.754408661907416e+00};
+  const double plow = 0.02425, phigh = 0.97575;
+  if (p < 0 || p > 1) return 0.0;
+  if (p == 0) return -HUGE_VAL;
+  if (p == 1) return HUGE_VAL;
+  if (p < plow) {
+    double q = std::sqrt(-2 * std::log(p));
+    return (((((c[0] * q + c[1]) * q + c[2]) * q + c[3]) * q + c[4]) * q + c[5]) /
+           ((((d[0] * q + d[1]) * q + d[2]) * q + d[3]) * q + 1);
+  }
+  if (p > phigh) {
+    double q = std::sqrt(-2 * std::log(1 - p));
+    return -(((((c[0] * q + c[1]) * q + c[2]) * q + c[3]) * q + c[4]) * q + c[5]) /
+           ((((d[0] * q + d[1]) * q + d[2]) * q + d[3]) * q + 1);
+  }
+  double q = p - 0.5, r = q * q;
+  return (((((a[0] * r + a[1]) * r + a[2]) * r + a[3]) * r + a[4]) * r + a[5]) * q /
+         (((((b[0] * r + b[1]) * r + b[2]) * r + b[3]) * r + b[4]) * r + 1);
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11 "Parallel random numbers: as easy as 1, 2, 3").
+inline void philox4x32_10(uint32_t ctr[4], uint32_t key0, uint32_t key1) {
+  const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+  for (int r = 0; r < 10; ++r) {
+    uint64_t p0 = (uint64_t)M0 * ctr[0];
+    uint64_t p1 = (uint64_t)M1 * ctr[2];
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    uint32_t n0 = hi1 ^ ctr[1] ^ key0;
+    uint32_t n1 = lo1;
+    uint32_t n2 = hi0 ^ ctr[3] ^ key1;
+    uint32_t n3 = lo0;
+    ctr[0] = n0; ctr[1] = n1; ctr[2] = n2; ctr[3] = n3;
+    key0 += W0; key1 += W1;
+  }
+}
+
+// Field uniform u in (0,1) for (seed, step, walker w, field m): counter (m/2, w, step, stream),
+// key = seed; two doubles per Philox block from 53-bit mantissas.
+inline double field_uniform(uint64_t seed, uint32_t stream, uint32_t step, uint32_t w, uint32_t m) {
+  uint32_t ctr[4] = {m >> 1, w, step, stream};
+  philox4x32_10(ctr, (uint32_t)seed, (uint32_t)(seed >> 32));
+  uint64_t bits = (m & 1u) ? (((uint64_t)ctr[2] << 32) | ctr[3]) : (((uint64_t)ctr[0] << 32) | ctr[1]);
+  return ((double)(bits >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+// cyclic Jacobi for a real symmetric n x n matrix (row-major, destroyed); eigenvectors in Vout
+// as columns (Vout[i][j] = component i of eigenvector j).
+void jacobi_eigh(double* A, int n, double* w, double* Vout) {
+  std::vector<double> V((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) V[(size_t)i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        double v = A[(size_t)i * n + j] * A[(size_t)i * n + j];
+        tot += v;
+        if (i != j) off += v;
+      }
+    if (off <= 1e-30 * tot || off == 0.0) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        double apq = A[(size_t)p * n + q];
+        if (apq == 0.0) continue;
+        double app = A[(size_t)p * n + p], aqq = A[(size_t)q * n + q];
+        double theta = (aqq - app) / (2.0 * apq);
+        double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; ++k) {  // columns p,q
+          double akp = A[(size_t)k * n + p], akq = A[(size_t)k * n + q];
+          A[(size_t)k * n + p] = c * akp - s * akq;
+          A[(size_t)k * n + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {  // rows p,q
+          double apk = A[(size_t)p * n + k], aqk = A[(size_t)q * n + k];
+          A[(size_t)p * n + k] = c * apk - s * aqk;
+          A[(size_t)q * n + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          double vkp = V[(size_t)k * n + p], vkq = V[(size_t)k * n + q];
+          V[(size_t)k * n + p] = c * vkp - s * vkq;
+          V[(size_t)k * n + q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  for (int i = 0; i < n; ++i) w[i] = A[(size_t)i * n + i];
+  std::copy(V.begin(), V.end(), Vout);
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------------------------------------
+// Slater-determinant operations (density_matrix.hpp, propagate.hpp, orthogonalize.hpp)
+// ---------------------------------------------------------------------------------------------
+
+// det_ops::detail::inverse_logdet, batched form (density_matrix.hpp:82-111): Ainv = A^{-1},
+// ovlp[b] += log det A[b].
+void orc_inverse_logdet(const double* A_, int nb, int n, double* ovlp_, double* Ainv_) {
+  const cd* A = (const cd*)A_;
+  cd* Ainv = (cd*)Ainv_;
+  cd* ovlp = (cd*)ovlp_;
+  for (int b = 0; b < nb; ++b) {
+    cd* T = Ainv + (size_t)b * n * n;
+    std::copy(A + (size_t)b * n * n, A + (size_t)(b + 1) * n * n, T);
+    std::vector<int> piv(n);
+    int info = lu_factor(T, n, piv.data());
+    logdet_from_getrf(T, piv.data(), n, ovlp[b]);
+    inverse_from_lu(T, piv.data(), n, info);
+  }
+}
+
+// det_ops::Log_Overlap (density_matrix.hpp:431-447): ovlp[w] += log det(PsiT * Phi_w).
+void orc_log_overlap(const double* PsiT_, const double* Phi_, int W, int N, int n, double* ovlp_) {
+  const cd* A = (const cd*)PsiT_;
+  const cd* B = (const cd*)Phi_;
+  cd* ovlp = (cd*)ovlp_;
+  if (n == 0) return;
+#pragma omp parallel for schedule(static)
+  for (int w = 0; w < W; ++w) {
+    std::vector<cd> T((size_t)n * n);
+    log_overlap_one(A, B + (size_t)w * N * n, N, n, T.data(), ovlp[w], false);
+  }
+}
+
+// det_ops::MixedDensityMatrix (density_matrix.hpp:535-601), herm = true:
+//   T = PsiT*Phi, ovlp += log det T, T <- T^{-1};
+//   compact: G[a][k] = sum_b T[b][a] Phi[k][b]            (product<'T','T'>(TNN,B,C))
+//   full:    G[i][k] = sum_a PsiT[a][i] * Gc[a][k]        (product<'T'>(A,TNM,C))
+void orc_mixed_density_matrix(const double* PsiT_, const double* Phi_, int W, int N, int n,
+                              double* G_, double* ovlp_, int compact) {
+  const cd* A = (const cd*)PsiT_;
+  const cd* B = (const cd*)Phi_;
+  cd* G = (cd*)G_;
+  cd* ovlp = (cd*)ovlp_;
+  size_t gsz = compact ? (size_t)n * N : (size_t)N * N;
+  if (n == 0) {
+    if (!compact) std::fill(G, G + gsz * W, cd(0.0));
+    return;
+  }
+#pragma omp parallel for schedule(static)
+  for (int w = 0; w < W; ++w) {
+    std::vector<cd> T((size_t)n * n), Gc((size_t)n * N);
+    const cd* Bw = B + (size_t)w * N * n;
+    log_overlap_one(A, Bw, N, n, T.data(), ovlp[w], true);
+    zgemm('T', 'T', n, N, n, 1.0, T.data(), n, Bw, n, 0.0, Gc.data(), N);
+    if (compact)
+      std::copy(Gc.begin(), Gc.end(), G + gsz * w);
+    else
+      zgemm('T', 'N', N, N, n, 1.0, A, N, Gc.data(), N, 0.0, G + gsz * w, N);
+  }
+}
+
+// det_ops::detail::apply_expM (propagate.hpp:41-70) inside propagate_impl (:241-298) with a
+// dense (CSR-truncated) P1:  S <- P1 * S; T1 = S; for k=1..order: T2 = (i/k) op(V) T1, S += T2,
+// swap; S <- P1 * S.  TA = 'N' uses V and +i, 'H' uses V^H and -i, 'T' uses V^T and +i.
+void orc_propagate(const double* P1_, const double* V_, double* Phi_, int W, int N, int n,
+                   int order, char TA) {
+  const cd* P1 = (const cd*)P1_;
+  const cd* V = (const cd*)V_;
+  cd* Phi = (cd*)Phi_;
+  if (n == 0) return;
+  cd im = (TA == 'H') ? cd(0.0, -1.0) : cd(0.0, 1.0);
+#pragma omp parallel for schedule(static)
+  for (int w = 0; w < W; ++w) {
+    cd* S = Phi + (size_t)w * N * n;
+    const cd* Vw = V + (size_t)w * N * N;
+    std::vector<cd> TMN((size_t)N * n), T1((size_t)N * n), T2((size_t)N * n), Vop;
+    const cd* Vuse = Vw;
+    char tv = 'N';
+    if (TA == 'H') {
+      Vop.resize((size_t)N * N);
+      for (int i = 0; i < N; ++i)
+        for (int k = 0; k < N; ++k) Vop[(size_t)i * N + k] = std::conj(Vw[(size_t)k * N + i]);
+      Vuse = Vop.data();
+    } else if (TA == 'T') {
+      tv = 'T';
+    }
+    zgemm('N', 'N', N, n, N, 1.0, P1, N, S, n, 0.0, TMN.data(), n);
+    std::copy(TMN.begin(), TMN.end(), T1.begin());
+    for (int k = 1; k <= order; ++k) {
+      cd fact = im * (1.0 / (double)k);
+      zgemm(tv, 'N', N, n, N, fact, Vuse, N, T1.data(), n, 0.0, T2.data(), n);
+      for (size_t e = 0; e < TMN.size(); ++e) TMN[e] += T2[e];
+      std::swap(T1, T2);
+    }
+    zgemm('N', 'N', N, n, N, 1.0, P1, N, TMN.data(), n, 0.0, S, n);
+  }
+}
+
+// det_ops::orthogonalize (orthogonalize.hpp:131-169): Householder QR of each Phi_w (N x n)
+// with LAPACK zgeqr2/zlarfg conventions (real R diagonal), log det from geqrf
+// (determinants_impl.hpp:77-108: scl_i = sign Re R_ii, ldet += log(scl_i R_ii + DBL_MIN)),
+// Phi <- Q * diag(scl).
+void orc_orthogonalize(double* Phi_, int W, int N, int n, double* ldet_) {
+  cd* Phi = (cd*)Phi_;
+  cd* ldet = (cd*)ldet_;
+  if (n == 0) return;
+#pragma omp parallel for schedule(static)
+  for (int w = 0; w < W; ++w) {
+    cd* A = Phi + (size_t)w * N * n;  // A[i][j], column j is a length-N vector
+    std::vector<cd> R((size_t)N * n);
+    std::copy(A, A + (size_t)N * n, R.begin());
+    std::vector<cd> tau(n);
+    int kmax = std::min(N, n);
+    for (int j = 0; j < kmax; ++j) {
+      // zlarfg on (R[j][j], R[j+1:N][j])
+      cd alpha = R[(size_t)j * n + j];
+      double xnorm = 0.0;
+      for (int i = j + 1; i < N; ++i) xnorm = std::hypot(xnorm, std::abs(R[(size_t)i * n + j]));
+      if (xnorm == 0.0 && alpha.imag() == 0.0) {
+        tau[j] = 0.0;
+      } else {
+        double beta = -std::copysign(std::sqrt(alpha.real() * alpha.real() +
+                                               alpha.imag() * alpha.imag() + xnorm * xnorm),
+                                     alpha.real());
+        tau[j] = cd((beta - alpha.real()) / beta, -alpha.imag() / beta);
+        cd scal = cd(1.0) / (alpha - beta);
+        for (int i = j + 1; i < N; ++i) R[(size_t)i * n + j] *= scal;
+        R[(size_t)j * n + j] = beta;
+      }
+      // apply H^H = I - conj(tau) v v^H to R[j:N][j+1:n] (zgeqr2 applies conj(tau))
+      if (tau[j] != cd(0.0)) {
+        for (int c = j + 1; c < n; ++c) {
+          cd s = R[(size_t)j * n + c];  // v_j = 1
+          for (int i = j + 1; i < N; ++i) s += std::conj(R[(size_t)i * n + j]) * R[(size_t)i * n + c];
+          s *= std::conj(tau[j]);
+          R[(size_t)j * n + c] -= s;
+          for (int i = j + 1; i < N; ++i) R[(size_t)i * n + c] -= R[(size_t)i * n + j] * s;
+        }
+      }
+    }
+    // log det + signs
+    std::vector<cd> scl(n, cd(1.0));
+    const cd small(DBL_MIN, 0.0);
+    for (int i = 0; i < kmax; ++i) {
+      cd rii = R[(size_t)i * n + i];
+      scl[i] = (rii.real() < 0) ? cd(-1.0) : cd(1.0);
+      ldet[w] += std::log(scl[i] * rii + small);
+    }
+    // zung2r: Q = H_0 H_1 ... H_{k-1} [I_n; 0]
+    std::vector<cd> Q((size_t)N * n, cd(0.0));
+    for (int j = 0; j < n; ++j) Q[(size_t)j * n + j] = 1.0;
+    for (int j = kmax - 1; j >= 0; --j) {
+      if (tau[j] == cd(0.0)) continue;
+      for (int c = j; c < n; ++c) {
+        cd s = Q[(size_t)j * n + c];
+        for (int i = j + 1; i < N; ++i) s += std::conj(R[(size_t)i * n + j]) * Q[(size_t)i * n + c];
+        s *= tau[j];
+        Q[(size_t)j * n + c] -= s;
+        for (int i = j + 1; i < N; ++i) Q[(size_t)i * n + c] -= R[(size_t)i * n + j] * s;
+      }
+    }
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j < n; ++j) A[(size_t)i * n + j] = Q[(size_t)i * n + j] * scl[j];
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Cholesky hamiltonian operations (Real3IndexFactorization.hpp)
+// ---------------------------------------------------------------------------------------------
+
+// vbias (Real3IndexFactorization.hpp:328-428, ndet == 1 host branch :358-361):
+//   v[w][m] = sqrt(dt) * c * sum_{a,k} G[w][a][k] * Lnak[m][a][k],  c = 2 (CLOSED) or 1.
+// nel is the total number of electron rows in G (nup, or nup+ndown when COLLINEAR with Lnak
+// holding both spin blocks stacked along a).
+void orc_vbias(const double* Lnak_, const double* G_, int W, int M, int nel, int N, double dt,
+               double c, double* v_) {
+  const cd* L = (const cd*)Lnak_;
+  const cd* G = (const cd*)G_;
+  cd* v = (cd*)v_;
+  const size_t K = (size_t)nel * N;
+  const double a = std::sqrt(dt) * c;
+#pragma omp parallel for schedule(static)
+  for (int w = 0; w < W; ++w)
+    for (int m = 0; m < M; ++m) {
+      cd s = 0.0;
+      const cd* g = G + (size_t)w * K;
+      const cd* l = L + (size_t)m * K;
+      for (size_t e = 0; e < K; ++e) s += g[e] * l[e];
+      v[(size_t)w * M + m] = a * s;
+    }
+}
+
+// vHS (Real3IndexFactorization.hpp:282-326, host branch :300-313):
+//   V[w][i][k] = sqrt(dt) * sum_m L[i][k][m] X[w][m]
+void orc_vhs(const double* Likn, const double* X_, int W, int N, int M, double dt, double* V_) {
+  const cd* X = (const cd*)X_;
+  cd* V = (cd*)V_;
+  const double a = std::sqrt(dt);
+#pragma omp parallel for schedule(static)
+  for (int w = 0; w < W; ++w) {
+    const cd* x = X + (size_t)w * M;
+    for (size_t ik = 0; ik < (size_t)N * N; ++ik) {
+      const double* l = Likn + ik * M;
+      double sr = 0.0, si = 0.0;
+      for (int m = 0; m < M; ++m) {
+        sr += l[m] * x[m].real();
+        si += l[m] * x[m].imag();
+      }
+      V[(size_t)w * N * N + ik] = a * cd(sr, si);
+    }
+  }
+}
+
+// energy (Real3IndexFactorization.hpp:138-174 + energy_impl :476-590, host branch), one spin
+// block of nel rows:
+//   E[w][0] += c * sum_{a,j} G[a][j] haj[a][j]                      (+E0 added by the caller)
+//   T[b][m][a] = sum_k G[b][k] Lnak[m][a][k]
+//   E[w][1] += -0.5 c sum_{m,a,b} T[a][m][b] T[b][m][a]
+//   Kl[w][m] += sum_a T[a][m][a]
+// orc_energy adds E0 to column 0 and the Coulomb term E[w][2] = 0.5*c^2*sum_m Kl^2 (c^2 -> 1 for
+// COLLINEAR, where both spin blocks accumulate into the same Kl).
+void orc_energy_block(const double* haj_, const double* Lnak_, const double* G_, int W, int N,
+                      int nel, int M, double c, double* E_, double* Kl_) {
+  const cd* haj = (const cd*)haj_;
+  const cd* L = (const cd*)Lnak_;
+  const cd* G = (const cd*)G_;
+  cd* E = (cd*)E_;
+  cd* Kl = (cd*)Kl_;
+  if (nel == 0) return;
+#pragma omp parallel for schedule(dynamic)
+  for (int w = 0; w < W; ++w) {
+    const cd* g = G + (size_t)w * nel * N;
+    cd e1 = 0.0;
+    for (size_t e = 0; e < (size_t)nel * N; ++e) e1 += g[e] * haj[e];
+    E[(size_t)w * 3 + 0] += c * e1;
+    std::vector<cd> T((size_t)nel * nel);
+    cd exx = 0.0;
+    for (int m = 0; m < M; ++m) {
+      // T[b][a] = sum_k G[b][k] L[m][a][k]
+      zgemm('N', 'T', nel, nel, N, 1.0, g, N, L + (size_t)m * nel * N, N, 0.0, T.data(), nel);
+      cd s = 0.0, tr = 0.0;
+      for (int a = 0; a < nel; ++a) {
+        tr += T[(size_t)a * nel + a];
+        for (int b = 0; b < nel; ++b) s += T[(size_t)a * nel + b] * T[(size_t)b * nel + a];
+      }
+      exx += s;
+      Kl[(size_t)w * M + m] += tr;
+    }
+    E[(size_t)w * 3 + 1] += -0.5 * c * exx;
+  }
+}
+
+void orc_energy(const double* haj_, const double* Lnak_, const double* G_, int W, int N, int nup,
+                int ndown, int M, int closed, double E0, double* E_) {
+  cd* E = (cd*)E_;
+  std::vector<cd> Kl((size_t)W * M, cd(0.0));
+  for (int w = 0; w < W; ++w) E[(size_t)w * 3] = E0, E[(size_t)w * 3 + 1] = 0.0, E[(size_t)w * 3 + 2] = 0.0;
+  int nel_tot = closed ? nup : nup + ndown;
+  double c = closed ? 2.0 : 1.0;
+  // G rows: [nup | ndown]; Lnak / haj rows likewise stacked per spin block.
+  std::vector<cd> Gs;
+  const cd* G = (const cd*)G_;
+  const cd* haj = (const cd*)haj_;
+  const cd* L = (const cd*)Lnak_;
+  int nspin = closed ? 1 : 2;
+  for (int s = 0; s < nspin; ++s) {
+    int nel = (s == 0) ? nup : ndown;
+    int off = (s == 0) ? 0 : nup;
+    if (nel == 0) continue;
+    Gs.assign((size_t)W * nel * N, 0.0);
+    for (int w = 0; w < W; ++w)
+      std::copy(G + ((size_t)w * nel_tot + off) * N, G + ((size_t)w * nel_tot + off + nel) * N,
+                Gs.begin() + (size_t)w * nel * N);
+    // Lnak for spin s: [M][nel][N], stored after the alpha block when COLLINEAR
+    const cd* Ls = L + (s == 0 ? 0 : (size_t)M * nup * N);
+    orc_energy_block((const double*)(haj + (size_t)off * N), (const double*)Ls,
+                     (const double*)Gs.data(), W, N, nel, M, c, E_, (double*)Kl.data());
+  }
+  double scl = closed ? 4.0 : 1.0;
+  for (int w = 0; w < W; ++w) {
+    cd s = 0.0;
+    for (int m = 0; m < M; ++m) s += Kl[(size_t)w * M + m] * Kl[(size_t)w * M + m];
+    E[(size_t)w * 3 + 2] += 0.5 * scl * s;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// fields, weights
+// ---------------------------------------------------------------------------------------------
+
+double orc_probit(double p) { return probit(p); }
+
+void orc_philox4x32_10(const uint32_t* ctr_in, const uint32_t* key, uint32_t* out) {
+  uint32_t c[4] = {ctr_in[0], ctr_in[1], ctr_in[2], ctr_in[3]};
+  philox4x32_10(c, key[0], key[1]);
+  std::memcpy(out, c, sizeof(c));
+}
+
+void orc_field_uniforms(uint64_t seed, uint32_t stream, uint32_t step, int W, int M, double* u) {
+  for (int w = 0; w < W; ++w)
+    for (int m = 0; m < M; ++m) u[(size_t)w * M + m] = field_uniform(seed, stream, step, w, m);
+}
+
+// assemble_X (AFQMCBasePropagator.cpp:686-733) + construct_X_impl (construct_X_kernel.hpp:34-86)
+// for continuous charge fields, followed by the mean-field factor (AFQMCBasePropagator.cpp:335-336):
+//   vmf_t, vb_t = clamp(|.| <= bound); vdiff = i (vb_t - vmf_t) (0 in free projection);
+//   X = probit(u) + vdiff;  HW[w] = sum_m -vdiff (X - vdiff/2);
+//   mf[w] = i sum_m X[w][m] vMF[m] - 0.5 Re sum_m vMF[m]^2
+// X holds vbias on entry. project_vbias keeps only the real part of vb_t (charge fields).
+void orc_construct_x(const double* u, const double* vMF_, double bound, int project_vbias,
+                     int free_projection, int W, int M, double* X_, double* HW_, double* mf_) {
+  const cd* vMF = (const cd*)vMF_;
+  cd* X = (cd*)X_;
+  cd* HW = (cd*)HW_;
+  cd* mf = (cd*)mf_;
+  const cd im(0.0, 1.0);
+  cd vv = 0.0;
+  for (int m = 0; m < M; ++m) vv += vMF[m] * vMF[m];
+  for (int w = 0; w < W; ++w) {
+    cd hw = 0.0, xs = 0.0;
+    for (int m = 0; m < M; ++m) {
+      cd vmf0 = vMF[m];
+      cd vmf_t = std::abs(vmf0) > bound ? vmf0 / std::abs(vmf0) * bound : vmf0;
+      cd x = X[(size_t)w * M + m];
+      cd vb_t = std::abs(x) > bound ? x / std::abs(x) * bound : x;
+      if (project_vbias) vb_t = cd(vb_t.real(), 0.0);
+      cd vdiff = free_projection ? cd(0.0) : im * (vb_t - vmf_t);
+      cd xn = probit(u[(size_t)w * M + m]) + vdiff;
+      X[(size_t)w * M + m] = xn;
+      hw += -vdiff * (xn - 0.5 * vdiff);
+      xs += xn * vMF[m];
+    }
+    HW[w] = hw;
+    mf[w] = -0.5 * vv.real() + im * xs;
+  }
+}
+
+// walker_update (WalkerSetUpdate.hpp:70-191), hybrid branch, phaseless (no cp constraint
+// unless use_cp), followed by bound_walker_weights (:199-218).
+void orc_walker_update(int W, double dt, double Eshift, double E0, const double* new_ovlp_,
+                       const double* mf_, const double* HW_, double lower_cut, double upper_cut,
+                       int free_projection, int use_cp, double weight_floor, double weight_frac,
+                       double target_pop, double* weight_, double* phase_, double* eloc_,
+                       double* ovlp_, double* theta_) {
+  const cd* nov = (const cd*)new_ovlp_;
+  const cd* mf = (const cd*)mf_;
+  const cd* HW = (const cd*)HW_;
+  cd* weight = (cd*)weight_;
+  cd* phase = (cd*)phase_;
+  cd* eloc_p = (cd*)eloc_;
+  cd* ovlp = (cd*)ovlp_;
+  cd* theta = (cd*)theta_;
+  const double hi = Eshift + upper_cut * std::sqrt(2.0 / dt);
+  const double lo = Eshift - lower_cut * std::sqrt(2.0 / dt);
+  for (int i = 0; i < W; ++i) {
+    cd old_ovlp = ovlp[i], old_eloc = eloc_p[i];
+    double scale = 1.0, dth = 0.0;
+    if (!free_projection) {
+      dth = nov[i].imag() - old_ovlp.imag() - mf[i].imag();
+      scale = use_cp ? (std::cos(dth) > 0.0 ? 1.0 : 0.0) : std::max(0.0, std::cos(dth));
+    }
+    theta[i] = dth;
+    cd eloc = (mf[i] - HW[i] - (nov[i] - old_ovlp)) / dt + E0;
+    if (!std::isfinite(eloc.real())) {
+      weight[i] = 0.0;
+      phase[i] = 0.0;
+      eloc = old_eloc;
+    } else {
+      if (!free_projection) eloc = cd(std::min(std::max(eloc.real(), lo), hi), eloc.imag());
+      cd wf = std::exp(-dt * (eloc - Eshift));
+      weight[i] *= free_projection ? scale * wf : scale * std::abs(wf);
+      phase[i] *= wf / std::abs(wf);
+    }
+    eloc_p[i] = eloc;
+    ovlp[i] = nov[i];
+  }
+  const double maxw = std::max(weight_floor, weight_frac * target_pop);
+  for (int i = 0; i < W; ++i) {
+    double aw = std::abs(weight[i]);
+    if (aw > maxw) weight[i] *= maxw / aw;
+  }
+}
+
+// averagePseudoEnergy (average_energy.hpp:54-63): log(sum_w w e^{dt E} / sum_w w).real / dt
+double orc_average_pseudo_energy(int W, const double* weight_, const double* eloc_, double dt) {
+  const cd* weight = (const cd*)weight_;
+  const cd* eloc = (const cd*)eloc_;
+  cd s0 = 0.0, s1 = 0.0;
+  for (int i = 0; i < W; ++i) {
+    s0 += weight[i] * std::exp(dt * eloc[i]);
+    s1 += weight[i];
+  }
+  return std::log((s0 / s1).real()) / dt;
+}
+
+// ---------------------------------------------------------------------------------------------
+// setup: half rotation, vexx, one-body propagator (RealDenseHamiltonian.cpp:139-181,
+// one_body.hpp, AFQMCBasePropagator.cpp:43-214, exp.hpp:28-60)
+// ---------------------------------------------------------------------------------------------
+
+// Lnak[m][a][k] = sum_i PsiT[a][i] L[i][k][m];  haj[a][k] = sum_i PsiT[a][i] h[i][k]
+void orc_half_rotate(const double* PsiT_, const double* Likn, const double* h_, int N, int nel,
+                     int M, double* Lnak_, double* haj_) {
+  const cd* A = (const cd*)PsiT_;
+  const cd* h = (const cd*)h_;
+  cd* Lnak = (cd*)Lnak_;
+  cd* haj = (cd*)haj_;
+  zgemm('N', 'N', nel, N, N, 1.0, A, N, h, N, 0.0, haj, N);
+#pragma omp parallel for schedule(static)
+  for (int m = 0; m < M; ++m)
+    for (int a = 0; a < nel; ++a)
+      for (int k = 0; k < N; ++k) {
+        cd s = 0.0;
+        for (int i = 0; i < N; ++i) s += A[(size_t)a * N + i] * Likn[((size_t)i * N + k) * M + m];
+        Lnak[((size_t)m * nel + a) * N + k] = s;
+      }
+}
+
+// vexx[i][l] = -0.5 sum_{j,m} L[i][j][m] L[l][j][m]  (RealDenseHamiltonian.cpp:165-181)
+void orc_vexx(const double* Likn, int N, int M, double* vexx) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < N; ++i)
+    for (int l = 0; l < N; ++l) {
+      double s = 0.0;
+      for (int j = 0; j < N; ++j) {
+        const double* a = Likn + ((size_t)i * N + j) * M;
+        const double* b = Likn + ((size_t)l * N + j) * M;
+        for (int m = 0; m < M; ++m) s += a[m] * b[m];
+      }
+      vexx[(size_t)i * N + l] = -0.5 * s;
+    }
+}
+
+// exp_hermitian (exp.hpp:28-60) via eigh: exp(A) = U diag(e^lambda) U^H. A Hermitian N x N.
+// The Hermitian problem is solved as the real symmetric 2N x 2N embedding [[Ar,-Ai],[Ai,Ar]]
+// (or the N x N real problem when A is real).
+void orc_exp_hermitian(const double* A_, int N, double* out_) {
+  const cd* A = (const cd*)A_;
+  cd* out = (cd*)out_;
+  bool real = true;
+  for (size_t e = 0; e < (size_t)N * N; ++e)
+    if (A[e].imag() != 0.0) { real = false; break; }
+  int n2 = real ? N : 2 * N;
+  std::vector<double> E((size_t)n2 * n2), V((size_t)n2 * n2), w(n2);
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j) {
+      cd a = A[(size_t)i * N + j];
+      E[(size_t)i * n2 + j] = a.real();
+      if (!real) {
+        E[(size_t)i * n2 + j + N] = -a.imag();
+        E[(size_t)(i + N) * n2 + j] = a.imag();
+        E[(size_t)(i + N) * n2 + j + N] = a.real();
+      }
+    }
+  jacobi_eigh(E.data(), n2, w.data(), V.data());
+  std::vector<double> R((size_t)n2 * n2, 0.0);
+  for (int i = 0; i < n2; ++i)
+    for (int j = 0; j < n2; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < n2; ++k) s += V[(size_t)i * n2 + k] * std::exp(w[k]) * V[(size_t)j * n2 + k];
+      R[(size_t)i * n2 + j] = s;
+    }
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j)
+      out[(size_t)i * N + j] = real ? cd(R[(size_t)i * n2 + j], 0.0)
+                                    : cd(R[(size_t)i * n2 + j], R[(size_t)(i + N) * n2 + j]);
+}
+
+// generateP1 (AFQMCBasePropagator.cpp:43-214) for one spin block with a continuous charge
+// decomposition: H1 = dt*(h + vexx) + vHS(vMF) (one_body.hpp add_one_body_shifts), hermitize
+// (tensor.hpp:75-94), scale by -1/2, P1 = exp_hermitian(H1), then the CSR truncation
+// |P1_ij| <= 1e-8 -> 0 (csr_utils.hpp:43-66) kept as a dense matrix.
+void orc_generate_p1(const double* h_, const double* vexx, const double* Likn, const double* vMF_,
+                     int N, int M, double dt, double* P1_) {
+  const cd* h = (const cd*)h_;
+  std::vector<cd> H1((size_t)N * N), shift((size_t)N * N);
+  orc_vhs(Likn, vMF_, 1, N, M, dt, (double*)shift.data());
+  for (size_t e = 0; e < (size_t)N * N; ++e) H1[e] = dt * h[e] + dt * vexx[e] + shift[e];
+  for (int i = 0; i < N; ++i)
+    for (int j = i; j < N; ++j) {
+      if (i == j) {
+        H1[(size_t)i * N + i] = H1[(size_t)i * N + i].real();
+      } else {
+        H1[(size_t)i * N + j] = 0.5 * H1[(size_t)i * N + j] + 0.5 * std::conj(H1[(size_t)j * N + i]);
+        H1[(size_t)j * N + i] = std::conj(H1[(size_t)i * N + j]);
+      }
+    }
+  for (auto& x : H1) x *= -0.5;
+  orc_exp_hermitian((const double*)H1.data(), N, P1_);
+  cd* P1 = (cd*)P1_;
+  for (size_t e = 0; e < (size_t)N * N; ++e)
+    if (std::abs(P1[e]) <= 1e-8) P1[e] = 0.0;
+}
+
+// NOMSD::vMF (NOMSD.cpp:437-464) for a single determinant followed by the charge-decomposition
+// zero_imag (AFQMCBasePropagator.cpp:105-108): G_T = compact MixedDensityMatrix(PsiT, PsiT^H),
+// vMF = vbias(G_T) with its imaginary part set to zero.
+void orc_vmf(const double* PsiT_, const double* Lnak_, int N, int nel, int M, double dt, double c,
+             double* vMF_) {
+  const cd* A = (const cd*)PsiT_;
+  std::vector<cd> Phi((size_t)N * nel), G((size_t)nel * N);
+  for (int i = 0; i < N; ++i)
+    for (int a = 0; a < nel; ++a) Phi[(size_t)i * nel + a] = std::conj(A[(size_t)a * N + i]);
+  cd ov = 0.0;
+  orc_mixed_density_matrix(PsiT_, (const double*)Phi.data(), 1, N, nel, (double*)G.data(),
+                           (double*)&ov, 1);
+  orc_vbias(Lnak_, (const double*)G.data(), 1, M, nel, N, dt, c, vMF_);
+  cd* v = (cd*)vMF_;
+  for (int m = 0; m < M; ++m) v[m] = cd(v[m].real(), 0.0);
+}
+
+// ---------------------------------------------------------------------------------------------
+// one hybrid propagation step for a CLOSED single-determinant walker set
+// (AFQMCBasePropagator::Propagate, AFQMCBasePropagator.cpp:289-392), walker by walker.
+// ---------------------------------------------------------------------------------------------
+
+struct orc_problem {
+  int N, nel, M, order;
+  const double* Likn;   // [N][N][M]
+  const double* Lnak;   // [M][nel][N] complex
+  const double* PsiT;   // [nel][N] complex
+  const double* P1;     // [N][N] complex (truncated)
+  const double* vMF;    // [M] complex
+  double dt, E0, vbias_bound, lower_cut, upper_cut, weight_floor, weight_frac, target_pop;
+};
+
+// Advances every walker by one step. u: uniforms [W][M] (the field draws of this step).
+// Phi, weight, phase, eloc, ovlp, theta are updated in place. Returns nothing; walker-parallel.
+void orc_step(const orc_problem* P, int W, const double* u, double Eshift, double* Phi_,
+              double* weight_, double* phase_, double* eloc_, double* ovlp_, double* theta_) {
+  const int N = P->N, n = P->nel, M = P->M;
+  const cd* PsiT = (const cd*)P->PsiT;
+  std::vector<cd> new_ovlp(W), mf(W), HW(W);
+#pragma omp parallel
+  {
+    std::vector<cd> G((size_t)n * N), X(M), V((size_t)N * N), T((size_t)n * n);
+#pragma omp for schedule(dynamic)
+    for (int w = 0; w < W; ++w) {
+      cd* Phi = (cd*)Phi_ + (size_t)w * N * n;
+      cd ov = 0.0;
+      orc_mixed_density_matrix(P->PsiT, (const double*)Phi, 1, N, n, (double*)G.data(),
+                               (double*)&ov, 1);
+      orc_vbias(P->Lnak, (const double*)G.data(), 1, M, n, N, P->dt, 2.0, (double*)X.data());
+      cd hw, mfw;
+      orc_construct_x(u + (size_t)w * M, P->vMF, P->vbias_bound, 0, 0, 1, M, (double*)X.data(),
+                      (double*)&hw, (double*)&mfw);
+      HW[w] = hw;
+      mf[w] = mfw;
+      orc_vhs(P->Likn, (const double*)X.data(), 1, N, M, P->dt, (double*)V.data());
+      orc_propagate(P->P1, (const double*)V.data(), (double*)Phi, 1, N, n, P->order, 'N');
+      cd lo = 0.0;
+      log_overlap_one(PsiT, Phi, N, n, T.data(), lo, false);
+      new_ovlp[w] = 2.0 * lo;  // CLOSED (NOMSD.cpp:293-295); log(conj(ci0)) = 0 for ci0 = 1
+    }
+  }
+  orc_walker_update(W, P->dt, Eshift, P->E0, (const double*)new_ovlp.data(),
+                    (const double*)mf.data(), (const double*)HW.data(), P->lower_cut, P->upper_cut,
+                    0, 0, P->weight_floor, P->weight_frac, P->target_pop, weight_, phase_, eloc_,
+                    ovlp_, theta_);
+}
+
+int orc_num_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+}  // extern "C"

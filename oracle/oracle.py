@@ -1,0 +1,320 @@
+"""ctypes binding of the FP64 CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, always as the checker or the timed CPU baseline, never as a product path.
+Arrays are numpy complex128 / float64, C-contiguous, in the layouts documented in
+afq_oracle.cpp.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        sig = {
+            "orc_inverse_logdet": [_P, _I, _I, _P, _P],
+            "orc_log_overlap": [_P, _P, _I, _I, _I, _P],
+            "orc_mixed_density_matrix": [_P, _P, _I, _I, _I, _P, _P, _I],
+            "orc_propagate": [_P, _P, _P, _I, _I, _I, _I, ctypes.c_char],
+            "orc_orthogonalize": [_P, _I, _I, _I, _P],
+            "orc_vbias": [_P, _P, _I, _I, _I, _I, _D, _D, _P],
+            "orc_vhs": [_P, _P, _I, _I, _I, _D, _P],
+            "orc_energy": [_P, _P, _P, _I, _I, _I, _I, _I, _I, _D, _P],
+            "orc_probit": [_D],
+            "orc_philox4x32_10": [_P, _P, _P],
+            "orc_field_uniforms": [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _I, _I, _P],
+            "orc_construct_x": [_P, _P, _D, _I, _I, _I, _I, _P, _P, _P],
+            "orc_walker_update": [_I, _D, _D, _D, _P, _P, _P, _D, _D, _I, _I, _D, _D, _D,
+                                  _P, _P, _P, _P, _P],
+            "orc_average_pseudo_energy": [_I, _P, _P, _D],
+            "orc_half_rotate": [_P, _P, _P, _I, _I, _I, _P, _P],
+            "orc_vexx": [_P, _I, _I, _P],
+            "orc_exp_hermitian": [_P, _I, _P],
+            "orc_generate_p1": [_P, _P, _P, _P, _I, _I, _D, _P],
+            "orc_vmf": [_P, _P, _I, _I, _I, _D, _D, _P],
+            "orc_step": [_P, _I, _P, _D, _P, _P, _P, _P, _P, _P],
+            "orc_num_threads": [],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = None
+        L.orc_probit.restype = _D
+        L.orc_average_pseudo_energy.restype = _D
+        L.orc_num_threads.restype = _I
+        _lib = L
+    return _lib
+
+
+def _c(a, dtype):
+    a = np.ascontiguousarray(a, dtype=dtype)
+    return a
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+# --- Slater determinant operations --------------------------------------------------------
+
+def inverse_logdet(A):
+    A = _c(A, np.complex128)
+    nb, n, _ = A.shape
+    ov = np.zeros(nb, np.complex128)
+    Ainv = np.empty_like(A)
+    lib().orc_inverse_logdet(_p(A), nb, n, _p(ov), _p(Ainv))
+    return Ainv, ov
+
+
+def log_overlap(PsiT, Phi, ovlp=None):
+    PsiT = _c(PsiT, np.complex128)
+    Phi = _c(Phi, np.complex128)
+    W, N, n = Phi.shape
+    ov = np.zeros(W, np.complex128) if ovlp is None else _c(ovlp, np.complex128).copy()
+    lib().orc_log_overlap(_p(PsiT), _p(Phi), W, N, n, _p(ov))
+    return ov
+
+
+def mixed_density_matrix(PsiT, Phi, compact=True, ovlp=None):
+    PsiT = _c(PsiT, np.complex128)
+    Phi = _c(Phi, np.complex128)
+    W, N, n = Phi.shape
+    G = np.empty((W, n, N) if compact else (W, N, N), np.complex128)
+    ov = np.zeros(W, np.complex128) if ovlp is None else _c(ovlp, np.complex128).copy()
+    lib().orc_mixed_density_matrix(_p(PsiT), _p(Phi), W, N, n, _p(G), _p(ov), int(compact))
+    return G, ov
+
+
+def propagate(P1, V, Phi, order=6, TA="N"):
+    P1 = _c(P1, np.complex128)
+    V = _c(V, np.complex128)
+    Phi = _c(Phi, np.complex128).copy()
+    W, N, n = Phi.shape
+    lib().orc_propagate(_p(P1), _p(V), _p(Phi), W, N, n, order, TA.encode())
+    return Phi
+
+
+def orthogonalize(Phi):
+    Phi = _c(Phi, np.complex128).copy()
+    W, N, n = Phi.shape
+    ldet = np.zeros(W, np.complex128)
+    lib().orc_orthogonalize(_p(Phi), W, N, n, _p(ldet))
+    return Phi, ldet
+
+
+# --- Cholesky hamiltonian operations -------------------------------------------------------
+
+def vbias(Lnak, G, dt, c=2.0):
+    Lnak = _c(Lnak, np.complex128)
+    M, nel, N = Lnak.shape
+    G = _c(G, np.complex128).reshape(-1, nel * N)
+    W = G.shape[0]
+    v = np.empty((W, M), np.complex128)
+    lib().orc_vbias(_p(Lnak), _p(G), W, M, nel, N, dt, c, _p(v))
+    return v
+
+
+def vhs(Likn, X, dt):
+    Likn = _c(Likn, np.float64)
+    X = _c(X, np.complex128)
+    N, _, M = Likn.shape
+    W = X.shape[0]
+    V = np.empty((W, N, N), np.complex128)
+    lib().orc_vhs(_p(Likn), _p(X), W, N, M, dt, _p(V))
+    return V
+
+
+def energy(haj, Lnak, G, nup, ndown=0, closed=True, E0=0.0):
+    haj = _c(haj, np.complex128)
+    Lnak = _c(Lnak, np.complex128)
+    G = _c(G, np.complex128)
+    W = G.shape[0]
+    N = haj.shape[-1]
+    M = Lnak.shape[0]
+    E = np.empty((W, 3), np.complex128)
+    lib().orc_energy(_p(haj), _p(Lnak), _p(G), W, N, nup, ndown, M, int(closed), E0, _p(E))
+    return E
+
+
+# --- fields, weights -----------------------------------------------------------------------
+
+def probit(p):
+    return lib().orc_probit(float(p))
+
+
+def philox4x32_10(ctr, key):
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    out = np.empty(4, np.uint32)
+    lib().orc_philox4x32_10(_p(c), _p(k), _p(out))
+    return out
+
+
+def field_uniforms(seed, stream, step, W, M):
+    u = np.empty((W, M), np.float64)
+    lib().orc_field_uniforms(seed, stream, step, W, M, _p(u))
+    return u
+
+
+def construct_x(u, vbias_, vMF, bound=50.0, project_vbias=False, free_projection=False):
+    u = _c(u, np.float64)
+    X = _c(vbias_, np.complex128).copy()
+    vMF = _c(vMF, np.complex128)
+    W, M = X.shape
+    HW = np.empty(W, np.complex128)
+    mf = np.empty(W, np.complex128)
+    lib().orc_construct_x(_p(u), _p(vMF), bound, int(project_vbias), int(free_projection), W, M,
+                          _p(X), _p(HW), _p(mf))
+    return X, HW, mf
+
+
+def walker_update(state, dt, Eshift, E0, new_ovlp, mf, HW, lower_cut=1.0, upper_cut=10.0,
+                  free_projection=False, use_cp=False, weight_floor=100.0, weight_frac=0.1,
+                  target_pop=None):
+    """state: dict of complex128 arrays weight, phase, eloc, ovlp, theta (updated in place)."""
+    W = state["weight"].shape[0]
+    if target_pop is None:
+        target_pop = W
+    new_ovlp = _c(new_ovlp, np.complex128)
+    mf = _c(mf, np.complex128)
+    HW = _c(HW, np.complex128)
+    lib().orc_walker_update(W, dt, Eshift, E0, _p(new_ovlp), _p(mf), _p(HW), lower_cut, upper_cut,
+                            int(free_projection), int(use_cp), weight_floor, weight_frac,
+                            float(target_pop), _p(state["weight"]), _p(state["phase"]),
+                            _p(state["eloc"]), _p(state["ovlp"]), _p(state["theta"]))
+
+
+def average_pseudo_energy(weight, eloc, dt):
+    weight = _c(weight, np.complex128)
+    eloc = _c(eloc, np.complex128)
+    return lib().orc_average_pseudo_energy(weight.shape[0], _p(weight), _p(eloc), dt)
+
+
+# --- setup ---------------------------------------------------------------------------------
+
+def half_rotate(PsiT, Likn, h1):
+    PsiT = _c(PsiT, np.complex128)
+    Likn = _c(Likn, np.float64)
+    h = _c(h1, np.complex128)
+    nel, N = PsiT.shape
+    M = Likn.shape[2]
+    Lnak = np.empty((M, nel, N), np.complex128)
+    haj = np.empty((nel, N), np.complex128)
+    lib().orc_half_rotate(_p(PsiT), _p(Likn), _p(h), N, nel, M, _p(Lnak), _p(haj))
+    return Lnak, haj
+
+
+def vexx(Likn):
+    Likn = _c(Likn, np.float64)
+    N, _, M = Likn.shape
+    out = np.empty((N, N), np.float64)
+    lib().orc_vexx(_p(Likn), N, M, _p(out))
+    return out
+
+
+def exp_hermitian(A):
+    A = _c(A, np.complex128)
+    out = np.empty_like(A)
+    lib().orc_exp_hermitian(_p(A), A.shape[0], _p(out))
+    return out
+
+
+def vmf(PsiT, Lnak, dt, c=2.0):
+    PsiT = _c(PsiT, np.complex128)
+    Lnak = _c(Lnak, np.complex128)
+    M, nel, N = Lnak.shape
+    out = np.empty(M, np.complex128)
+    lib().orc_vmf(_p(PsiT), _p(Lnak), N, nel, M, dt, c, _p(out))
+    return out
+
+
+def generate_p1(h1, vexx_, Likn, vMF, dt):
+    h = _c(h1, np.complex128)
+    vx = _c(vexx_, np.float64)
+    Likn = _c(Likn, np.float64)
+    vMF = _c(vMF, np.complex128)
+    N = h.shape[0]
+    M = Likn.shape[2]
+    P1 = np.empty((N, N), np.complex128)
+    lib().orc_generate_p1(_p(h), _p(vx), _p(Likn), _p(vMF), N, M, dt, _p(P1))
+    return P1
+
+
+class _Problem(ctypes.Structure):
+    _fields_ = [("N", _I), ("nel", _I), ("M", _I), ("order", _I),
+                ("Likn", _P), ("Lnak", _P), ("PsiT", _P), ("P1", _P), ("vMF", _P),
+                ("dt", _D), ("E0", _D), ("vbias_bound", _D), ("lower_cut", _D), ("upper_cut", _D),
+                ("weight_floor", _D), ("weight_frac", _D), ("target_pop", _D)]
+
+
+class OracleSystem:
+    """Setup + stepping of a CLOSED single-determinant walker set, all in the oracle."""
+
+    def __init__(self, problem, dt=0.005, order=6, vbias_bound=50.0, target_pop=None):
+        self.problem = problem
+        self.dt = dt
+        self.order = order
+        self.Likn = _c(problem.Likn, np.float64)
+        self.PsiT = _c(problem.PsiT, np.complex128)
+        self.Lnak, self.haj = half_rotate(self.PsiT, self.Likn, problem.h1)
+        self.vexx = vexx(self.Likn)
+        self.vMF = vmf(self.PsiT, self.Lnak, dt)
+        self.P1 = generate_p1(problem.h1, self.vexx, self.Likn, self.vMF, dt)
+        self.E0 = problem.E0
+        self.vbias_bound = vbias_bound
+        self.target_pop = target_pop
+        self._keep = []
+
+    def struct(self, W):
+        s = _Problem(self.problem.N, self.problem.nup, self.problem.M, self.order,
+                     _p(self.Likn), _p(self.Lnak), _p(self.PsiT), _p(self.P1), _p(self.vMF),
+                     self.dt, self.E0, self.vbias_bound, 1.0, 10.0, 100.0, 0.1,
+                     float(self.target_pop if self.target_pop is not None else W))
+        return s
+
+    def init_walkers(self, Phi):
+        """Initial properties (execute.cpp:170-183): OVLP and E from Energy, PSEUDO_ELOC_ = E
+        (seedPseudoEnergy), WEIGHT = PHASE = 1; returns (state, Eshift)."""
+        Phi = _c(Phi, np.complex128).copy()
+        W = Phi.shape[0]
+        G, ov = mixed_density_matrix(self.PsiT, Phi, compact=True)
+        ov = 2.0 * ov
+        E = energy(self.haj, self.Lnak, G, self.problem.nup, closed=True, E0=self.E0)
+        eloc = E.sum(axis=1)
+        state = dict(Phi=Phi, weight=np.ones(W, np.complex128), phase=np.ones(W, np.complex128),
+                     eloc=eloc.copy(), ovlp=ov, theta=np.zeros(W, np.complex128))
+        Eshift = float((np.sum(state["weight"] * eloc) / np.sum(state["weight"])).real)
+        return state, Eshift
+
+    def step(self, state, u, Eshift):
+        W = state["Phi"].shape[0]
+        s = self.struct(W)
+        u = _c(u, np.float64)
+        lib().orc_step(ctypes.byref(s), W, _p(u), Eshift, _p(state["Phi"]), _p(state["weight"]),
+                       _p(state["phase"]), _p(state["eloc"]), _p(state["ovlp"]), _p(state["theta"]))
+
+    def orthogonalize(self, state):
+        Phi, _ = orthogonalize(state["Phi"])
+        state["Phi"][...] = Phi
+        state["ovlp"][...] = 2.0 * log_overlap(self.PsiT, Phi)

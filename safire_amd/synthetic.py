@@ -1,0 +1,101 @@
+"""Synthetic closed-shell AFQMC problems (the workloads the walker step is measured on).
+
+A problem is a bare one-body hamiltonian ``h1`` (real symmetric N x N), a real Cholesky
+factorization ``Likn[i][k][m]`` (each L^m symmetric), an RHF trial made of the ``nup`` lowest
+eigenvectors of h1, and ``W`` walkers initialised to the trial.  The layouts are the ones the
+reference reads from its HDF5 inputs (RealDenseHamiltonian.cpp:121-127: ``hcore`` [N][N],
+``DenseFactorized/L`` [N][N][M]) and stores its walkers in (WalkerSetBase.h:320-328:
+Phi [W][N][n] row-major).  Every draw uses ``numpy.random.default_rng(seed)``.
+
+Recipe (per workload):
+  h1  = sym(N(0, 0.1)) + diag(linspace(-2, 1, N))
+  L^m = sym(N(0, 1)) * 0.5 * exp(-m / 20),   m = 0..M-1
+where sym() keeps the upper triangle of a Gaussian matrix and mirrors it, so every entry is a
+draw of the stated distribution.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+
+@dataclasses.dataclass(frozen=True)
+class Workload:
+    name: str
+    N: int          # orbitals
+    nup: int        # electrons per spin (CLOSED: one block)
+    M: int          # Cholesky vectors
+    W: int          # walkers
+    steps: int
+    dt: float = 0.005
+    taylor_order: int = 6
+    ortho_interval: int = 5
+    seed: int = 0
+
+
+# The four fixed workloads; the serving loop draws its own (see safire_amd.serve).
+WORKLOADS = {
+    "cfg1": Workload("cfg1: FP64 N=24 n=5+5 M=96 W=64", 24, 5, 96, 64, 100, seed=1001),
+    "cfg2": Workload("cfg2: FP64 N=108 n=24+24 M=600 W=512", 108, 24, 600, 512, 200, seed=1002),
+    "cfg3": Workload("cfg3: mixed N=200 n=40+40 M=1600 W=1024", 200, 40, 1600, 1024, 300, seed=1003),
+}
+
+
+def _sym(rng: np.random.Generator, shape, scale: float) -> np.ndarray:
+    """Symmetric matrices (last two axes) whose entries are N(0, scale) draws."""
+    a = rng.normal(0.0, scale, size=shape)
+    n = shape[-1]
+    up = np.triu(np.ones((n, n), dtype=bool))
+    return np.where(up, a, np.swapaxes(a, -1, -2))
+
+
+@dataclasses.dataclass
+class Problem:
+    N: int
+    nup: int
+    M: int
+    h1: np.ndarray        # [N][N] float64
+    Likn: np.ndarray      # [N][N][M] float64
+    PsiT: np.ndarray      # [nup][N] complex128 (conjugate-transposed trial)
+    E0: float = 0.0
+
+    @property
+    def trial(self) -> np.ndarray:
+        """Trial Slater matrix Psi [N][nup]."""
+        return np.ascontiguousarray(self.PsiT.conj().T)
+
+    def walkers(self, W: int) -> np.ndarray:
+        """W walkers initialised to the trial: Phi [W][N][nup] complex128."""
+        return np.ascontiguousarray(np.broadcast_to(self.trial, (W, self.N, self.nup)))
+
+
+def make_problem(N: int, nup: int, M: int, seed: int) -> Problem:
+    rng = np.random.default_rng(seed)
+    h1 = _sym(rng, (N, N), 0.1) + np.diag(np.linspace(-2.0, 1.0, N))
+    decay = 0.5 * np.exp(-np.arange(M) / 20.0)
+    # generated as [M][N][N] (one symmetric L^m per vector), stored as Likn [N][N][M]
+    Lm = np.empty((M, N, N))
+    chunk = max(1, (1 << 24) // (N * N))
+    for m0 in range(0, M, chunk):
+        m1 = min(M, m0 + chunk)
+        Lm[m0:m1] = _sym(rng, (m1 - m0, N, N), 1.0) * decay[m0:m1, None, None]
+    Likn = np.ascontiguousarray(np.transpose(Lm, (1, 2, 0)))
+    del Lm
+    evals, evecs = np.linalg.eigh(h1)
+    psi = evecs[:, :nup]
+    PsiT = np.ascontiguousarray(psi.T.astype(np.complex128))
+    return Problem(N=N, nup=nup, M=M, h1=h1, Likn=Likn, PsiT=PsiT)
+
+
+def workload_problem(key: str) -> tuple[Workload, Problem]:
+    wl = WORKLOADS[key]
+    return wl, make_problem(wl.N, wl.nup, wl.M, wl.seed)
+
+
+def random_walkers(problem: Problem, W: int, seed: int, noise: float = 0.3) -> np.ndarray:
+    """Trial walkers with a complex perturbation: generic (non-trial) Slater matrices for
+    kernel-level parity tests, where walkers equal to the trial would hide index mistakes."""
+    rng = np.random.default_rng(seed)
+    pert = rng.normal(size=(W, problem.N, problem.nup)) + 1j * rng.normal(size=(W, problem.N, problem.nup))
+    return np.ascontiguousarray(problem.walkers(W) + noise * pert / np.sqrt(problem.N))

@@ -1,0 +1,315 @@
+"""Pins the FP64 CPU oracle before anything is checked against it.
+
+Each oracle function is checked against (a) the known-answer values SAFIRE's own tests hold
+(tests/test_sdet_ops.cpp, transcribed into tests/golden/sdet_ops.json), (b) closed forms of the
+same quantity computed independently here with numpy/scipy from the *full* Green's function, or
+(c) invariants and special cases, chosen so a dropped factor, a transposed operand or a wrong
+sign fails at least one check.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg
+import scipy.special
+
+import oracle as orc
+from safire_amd.synthetic import make_problem, random_walkers
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "sdet_ops.json")))
+
+
+def cplx(a):
+    a = np.asarray(a, dtype=np.float64)
+    return a[..., 0] + 1j * a[..., 1]
+
+
+def close(a, b, tol=1e-8):
+    """utils::Approx semantics (abs and rel tolerance) as in test_common.hpp:436-445."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    assert a.shape == b.shape
+    err = np.abs(a - b)
+    ok = err <= tol + tol * np.abs(b)
+    assert ok.all(), f"max err {err.max()}"
+
+
+# ---- golden values from test_sdet_ops.cpp -------------------------------------------------
+
+def _gold_AB():
+    A = cplx(GOLD["A"])
+    B = np.broadcast_to(cplx(GOLD["B"]), (GOLD["nwalk"], 4, 3)).copy()
+    return A, B
+
+
+def test_golden_log_overlap():
+    A, B = _gold_AB()
+    ov = orc.log_overlap(A, B)
+    close(ov, np.full(5, np.log(complex(*GOLD["ov_base_arg"]))))
+    ov2 = orc.log_overlap(A[0:2, 0:3], B[:, 0:3, 0:2])
+    close(ov2, np.full(5, np.log(complex(*GOLD["ov2_base_arg"]))))
+
+
+def test_golden_mixed_density_matrix():
+    A, B = _gold_AB()
+    G, ov = orc.mixed_density_matrix(A, B, compact=False)
+    close(G, np.broadcast_to(cplx(GOLD["v_ref"]), G.shape))
+    close(ov, np.full(5, np.log(complex(*GOLD["ov_base_arg"]))))
+    Gc, ov = orc.mixed_density_matrix(A, B, compact=True)
+    close(Gc, np.broadcast_to(cplx(GOLD["vc_ref"]), Gc.shape))
+    G2, _ = orc.mixed_density_matrix(A[0:2, 0:3], B[:, 0:3, 0:2], compact=False)
+    close(G2, np.broadcast_to(cplx(GOLD["v_ref_2"]), G2.shape))
+    Gc2, _ = orc.mixed_density_matrix(A[0:2, 0:3], B[:, 0:3, 0:2], compact=True)
+    close(Gc2, np.broadcast_to(cplx(GOLD["vc_ref_2"]), Gc2.shape))
+
+
+def test_golden_inverse_logdet():
+    t = GOLD["inverse_test"]
+    A = np.broadcast_to(cplx(t["A"]), (5, 4, 4)).copy()
+    Ainv, ld = orc.inverse_logdet(A)
+    close(Ainv, np.broadcast_to(cplx(t["inv_ref"]), Ainv.shape), tol=1e-7)  # quoted to 8 decimals
+    close(ld, np.full(5, complex(*t["logdet_ref"])))
+
+
+def test_log_overlap_accumulates_and_wraps():
+    """log det from getrf adds into ovlp and wraps the total's phase to [-pi, pi]."""
+    A, B = _gold_AB()
+    base = np.log(complex(*GOLD["ov_base_arg"]))
+    ov = orc.log_overlap(A, B, ovlp=np.full(5, 1.0 + 3.0j))
+    tot = base + 1.0 + 3.0j
+    wrapped = complex(tot.real, (tot.imag + np.pi) % (2 * np.pi) - np.pi)
+    close(ov, np.full(5, wrapped))
+
+
+# ---- closed forms on synthetic problems -----------------------------------------------------
+
+@pytest.fixture(scope="module")
+def small():
+    p = make_problem(N=12, nup=4, M=20, seed=7)
+    Phi = random_walkers(p, 6, seed=3)
+    return p, Phi
+
+
+def full_G_numpy(PsiT, Phi):
+    """G_full[i][k] = (Phi (PsiT Phi)^{-1} PsiT)^T[i][k], the textbook mixed density matrix in the
+    reference's index order (density_matrix.hpp:583-586)."""
+    out = []
+    for B in Phi:
+        T = PsiT @ B
+        out.append((B @ np.linalg.inv(T) @ PsiT).T)
+    return np.array(out)
+
+
+def test_mixed_density_matrix_closed_form(small):
+    p, Phi = small
+    G, ov = orc.mixed_density_matrix(p.PsiT, Phi, compact=False)
+    Gn = full_G_numpy(p.PsiT, Phi)
+    close(G, Gn, 1e-11)
+    for g in G:  # projector of rank n
+        assert abs(np.trace(g) - p.nup) < 1e-10
+        close(g @ g, g, 1e-10)
+    sign, lad = zip(*[np.linalg.slogdet(p.PsiT @ B) for B in Phi])
+    ref = np.log(np.array(sign)) + np.array(lad)
+    close(ov, ref, 1e-12)
+    Gc, ov2 = orc.mixed_density_matrix(p.PsiT, Phi, compact=True)
+    close(np.einsum("ai,wak->wik", p.PsiT, Gc), Gn, 1e-11)
+
+
+def test_vbias_closed_form(small):
+    p, Phi = small
+    dt = 0.01
+    Lnak, haj = orc.half_rotate(p.PsiT, p.Likn, p.h1)
+    Gc, _ = orc.mixed_density_matrix(p.PsiT, Phi, compact=True)
+    v = orc.vbias(Lnak, Gc, dt, c=2.0)
+    Gf = full_G_numpy(p.PsiT, Phi)
+    ref = 2.0 * np.sqrt(dt) * np.einsum("wik,ikm->wm", Gf, p.Likn)
+    close(v, ref, 1e-11)
+    # half rotation closed form
+    close(Lnak, np.einsum("ai,ikm->mak", p.PsiT, p.Likn), 1e-12)
+    close(haj, p.PsiT @ p.h1, 1e-12)
+
+
+def test_vhs_closed_form(small):
+    p, _ = small
+    rng = np.random.default_rng(5)
+    X = rng.normal(size=(3, p.M)) + 1j * rng.normal(size=(3, p.M))
+    V = orc.vhs(p.Likn, X, 0.02)
+    close(V, np.sqrt(0.02) * np.einsum("ikm,wm->wik", p.Likn, X), 1e-12)
+
+
+def test_energy_closed_form(small):
+    p, Phi = small
+    Lnak, haj = orc.half_rotate(p.PsiT, p.Likn, p.h1)
+    Gc, _ = orc.mixed_density_matrix(p.PsiT, Phi, compact=True)
+    E = orc.energy(haj, Lnak, Gc, p.nup, closed=True, E0=0.25)
+    Gf = full_G_numpy(p.PsiT, Phi)
+    L = np.transpose(p.Likn, (2, 0, 1))
+    c = 2.0
+    e1 = 0.25 + c * np.einsum("wik,ik->w", Gf, p.h1)
+    ej = 0.5 * c * c * np.einsum("wm->w", np.einsum("wik,mik->wm", Gf, L) ** 2)
+    exx = -0.5 * c * np.einsum("wij,mjk,wkl,mli->w", Gf, L, Gf, L)
+    close(E[:, 0], e1, 1e-11)
+    close(E[:, 1], exx, 1e-10)
+    close(E[:, 2], ej, 1e-10)
+
+
+def test_energy_eri_brute_force():
+    """Closed-shell energy from explicit ERIs V_ijkl = sum_m L^m_ik L^m_jl on a tiny system:
+    E = E0 + 2 sum G h + sum_ijkl V_ijkl (2 G_ik G_jl - G_il G_jk)."""
+    p = make_problem(N=6, nup=2, M=5, seed=11)
+    Phi = random_walkers(p, 2, seed=1)
+    Lnak, haj = orc.half_rotate(p.PsiT, p.Likn, p.h1)
+    Gc, _ = orc.mixed_density_matrix(p.PsiT, Phi, compact=True)
+    E = orc.energy(haj, Lnak, Gc, p.nup, closed=True)
+    Gf = full_G_numpy(p.PsiT, Phi)
+    V = np.einsum("ikm,jlm->ijkl", p.Likn, p.Likn)
+    for w in range(2):
+        g = Gf[w]
+        ref = 2 * np.sum(g * p.h1) + np.einsum("ijkl,ik,jl->", V, g, g) * 2 \
+            - np.einsum("ijkl,il,jk->", V, g, g)
+        close(E[w].sum(), ref, 1e-10)
+
+
+def test_propagate_explicit_series(small):
+    p, Phi = small
+    rng = np.random.default_rng(9)
+    N = p.N
+    V = 0.05 * (rng.normal(size=(Phi.shape[0], N, N)) + 1j * rng.normal(size=(Phi.shape[0], N, N)))
+    P1 = scipy.linalg.expm(-0.01 * (p.h1 + p.h1.T))
+    for TA, sgn in (("N", 1j), ("H", -1j)):
+        out = orc.propagate(P1, V, Phi, order=6, TA=TA)
+        for w in range(Phi.shape[0]):
+            Vw = V[w] if TA == "N" else V[w].conj().T
+            E = sum(np.linalg.matrix_power(sgn * Vw, k) / scipy.special.factorial(k) for k in range(7))
+            close(out[w], P1 @ E @ P1 @ Phi[w], 1e-12)
+    # the series converges to exp(iV) as the order grows
+    out = orc.propagate(P1, V, Phi, order=30)
+    close(out[0], P1 @ scipy.linalg.expm(1j * V[0]) @ P1 @ Phi[0], 1e-12)
+
+
+def test_orthogonalize_invariants(small):
+    p, Phi = small
+    Q, ldet = orc.orthogonalize(Phi)
+    for w in range(Phi.shape[0]):
+        q = Q[w]
+        close(q.conj().T @ q, np.eye(p.nup), 1e-12)
+        R = q.conj().T @ Phi[w]
+        close(np.tril(R, -1), np.zeros_like(R), 1e-12)
+        assert np.all(np.abs(np.imag(np.diag(R))) < 1e-12) and np.all(np.real(np.diag(R)) > 0)
+        # unique QR with positive diagonal = numpy's QR with signs fixed
+        qn, rn = np.linalg.qr(Phi[w])
+        d = np.diag(rn)
+        ph = d / np.abs(d)
+        close(q, qn * ph[None, :], 1e-12)
+        close(ldet[w], np.sum(np.log(np.abs(d))), 1e-12)
+    # log<Q|Q> = 0 (test_sdet_ops.cpp:295-304)
+    ov = orc.log_overlap(np.conj(np.swapaxes(Q[0], 0, 1)), Q)
+    close(ov[:1], np.zeros(1, complex), 1e-12)
+
+
+def test_probit_vs_ndtri():
+    ps = np.concatenate([np.linspace(1e-12, 0.02425, 200), np.linspace(0.02, 0.98, 500),
+                         np.linspace(0.97575, 1 - 1e-12, 200)])
+    got = np.array([orc.probit(x) for x in ps])
+    ref = scipy.special.ndtri(ps)
+    rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)
+    assert rel.max() < 1.15e-9
+    assert orc.probit(0.0) == -np.inf and orc.probit(1.0) == np.inf and orc.probit(1.5) == 0.0
+
+
+def test_philox_known_answers():
+    """Random123 philox4x32-10 known-answer vectors (kat_vectors)."""
+    kat = [((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+           ((0xffffffff,) * 4, (0xffffffff, 0xffffffff), (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+           ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+            (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1))]
+    for ctr, key, out in kat:
+        assert tuple(int(x) for x in orc.philox4x32_10(ctr, key)) == out
+    u = orc.field_uniforms(42, 1, 3, 64, 50)
+    assert u.min() > 0 and u.max() < 1
+    assert abs(u.mean() - 0.5) < 0.02
+    assert not np.array_equal(u, orc.field_uniforms(42, 1, 4, 64, 50))
+
+
+def test_exp_hermitian():
+    rng = np.random.default_rng(2)
+    a = rng.normal(size=(9, 9))
+    a = a + a.T
+    close(orc.exp_hermitian(a.astype(complex)), scipy.linalg.expm(a), 1e-12)
+    b = a + 1j * (lambda m: m - m.T)(rng.normal(size=(9, 9)))
+    close(orc.exp_hermitian(b), scipy.linalg.expm(b), 1e-12)
+
+
+def test_generate_p1_and_vmf(small):
+    p, _ = small
+    dt = 0.01
+    Lnak, _ = orc.half_rotate(p.PsiT, p.Likn, p.h1)
+    vmf = orc.vmf(p.PsiT, Lnak, dt)
+    GT = full_G_numpy(p.PsiT, p.walkers(1))[0]
+    ref = 2.0 * np.sqrt(dt) * np.einsum("ik,ikm->m", GT, p.Likn)
+    close(vmf, ref.real.astype(complex), 1e-12)
+    vx = orc.vexx(p.Likn)
+    close(vx, -0.5 * np.einsum("ijm,ljm->il", p.Likn, p.Likn), 1e-12)
+    P1 = orc.generate_p1(p.h1, vx, p.Likn, vmf, dt)
+    H1 = dt * (p.h1 + vx) + np.sqrt(dt) * np.einsum("ikm,m->ik", p.Likn, vmf)
+    H1 = 0.5 * (H1 + H1.conj().T)
+    ref = scipy.linalg.expm(-0.5 * H1)
+    ref[np.abs(ref) <= 1e-8] = 0.0
+    close(P1, ref, 1e-12)
+
+
+def test_construct_x_cases():
+    rng = np.random.default_rng(4)
+    W, M = 3, 7
+    u = rng.uniform(size=(W, M))
+    vb = rng.normal(size=(W, M)) + 1j * rng.normal(size=(W, M))
+    vb[0, 0] = 80.0 + 60.0j  # beyond the bound: clamped onto |v| = 50
+    vmf = rng.normal(size=M).astype(complex)
+    X, HW, mf = orc.construct_x(u, vb, vmf, bound=50.0)
+    g = scipy.special.ndtri(u)
+    vbt = vb.copy()
+    vbt[0, 0] = vb[0, 0] / abs(vb[0, 0]) * 50.0
+    vdiff = 1j * (vbt - vmf[None, :])
+    close(X, g + vdiff, 1e-8)
+    close(HW, np.sum(-vdiff * (X - 0.5 * vdiff), axis=1), 1e-8)
+    close(mf, 1j * X @ vmf - 0.5 * np.sum(vmf * vmf).real, 1e-8)
+    Xf, HWf, _ = orc.construct_x(u, vb, vmf, free_projection=True)
+    close(HWf, np.zeros(W, complex), 0)
+    close(Xf, g.astype(complex), 1e-8)
+
+
+def test_walker_update_cases():
+    W = 4
+    st = dict(weight=np.ones(W, complex), phase=np.ones(W, complex), eloc=np.full(W, -1.0 + 0j),
+              ovlp=np.zeros(W, complex), theta=np.zeros(W, complex))
+    dt, Eshift = 0.01, -1.0
+    new = np.array([0.01 + 0.0j, np.nan, 10.0 + 0.1j, 0.0 + 2.0j])
+    mf = np.zeros(W, complex)
+    HW = np.zeros(W, complex)
+    orc.walker_update(st, dt, Eshift, 0.0, new, mf, HW, target_pop=W)
+    # walker 0: eloc = -0.01/dt = -1 (inside the bound), phase 0 -> scale 1
+    assert abs(st["eloc"][0] - (-1.0)) < 1e-12 and abs(st["weight"][0] - 1.0) < 1e-12
+    # walker 1: non-finite -> dropped, keeps its last energy
+    assert st["weight"][1] == 0 and st["eloc"][1] == -1.0
+    # walker 2: eloc = -10/dt = -1000 clamped at Eshift - sqrt(2/dt)
+    assert abs(st["eloc"][2].real - (Eshift - np.sqrt(2 / dt))) < 1e-12
+    # weight cap max(100, 0.1*W) = 100 after the clamp
+    assert abs(st["weight"][2]) <= 100.0 + 1e-12
+    # walker 3: phase difference 2 rad -> cos < 0 -> weight 0
+    assert st["weight"][3] == 0 and abs(st["theta"][3] - 2.0) < 1e-12
+
+
+def test_oracle_steps_stay_finite():
+    p = make_problem(N=24, nup=5, M=96, seed=1001)
+    sysm = orc.OracleSystem(p, dt=0.005)
+    state, Eshift = sysm.init_walkers(p.walkers(8))
+    for step in range(6):
+        u = orc.field_uniforms(7, 1, step, 8, p.M)
+        sysm.step(state, u, Eshift)
+        if step % 5 == 0:
+            sysm.orthogonalize(state)
+    assert np.all(np.isfinite(state["Phi"]))
+    assert np.all(np.isfinite(state["eloc"]))
+    e = orc.average_pseudo_energy(state["weight"], state["eloc"], 0.005)
+    assert np.isfinite(e)
