@@ -1,0 +1,203 @@
+/* afq.h -- C ABI of the gfx950 AFQMC walker-step library (safire_amd/libafq_hip.so).
+ *
+ * Drop-in boundary for SAFIRE's Cholesky HamiltonianOperations and Slater-determinant
+ * operations.  Every entry point names the reference interface it replaces (paths relative to
+ * the SAFIRE source tree).
+ *
+ * Conventions
+ *  - All array arguments are DEVICE pointers (HBM of the context's GPU), row-major, complex
+ *    values interleaved (re, im) as double[2] (complex128) or float[2] (complex64).
+ *  - Every call is ordered on the context's stream (the one given to afq_ctx_create, or a
+ *    private non-blocking stream) and returns before the work completes; use
+ *    afq_ctx_synchronize() before reading results on the host.  Pass the caller's own stream
+ *    to order the calls with its other work.
+ *  - Return value: AFQ_OK (0) or a negative afq_status; afq_last_error() gives the text of the
+ *    last failure on the calling thread.  Shape errors play the role of the reference's
+ *    utils::check -> APP_ABORT (check.hpp:36-46); nothing is written when a check fails.
+ *  - Ownership: the caller owns every array it passes.  Hamiltonian tensors are copied into the
+ *    afq_ham object at creation and stay resident; scratch space belongs to the context.
+ *  - One context per host thread.  Calls on one context are not thread-safe.
+ *
+ * Layouts (N orbitals, M Cholesky vectors, W walkers, nup/ndown electrons, nel = nup (CLOSED)
+ * or nup + ndown (COLLINEAR)):
+ *   Likn  double [N][N][M]        DenseFactorized/L (RealDenseHamiltonian.cpp:121-127)
+ *   h1    double [N][N]           Hamiltonian/hcore
+ *   PsiT  complex [nel][N]        trial, conjugate-transposed (NOMSD.hpp:292-344); alpha rows first
+ *   Phi   complex [W][N][n]       walker Slater matrices of one spin block (WalkerSetBase.h:320-328)
+ *   G     complex [W][nel][N]     compact mixed density matrix (density_matrix.hpp:576)
+ *   X, v  complex [W][M]          fields / force bias
+ *   V     complex [W][N][N]       vHS (complex128 in FP64, complex64 in mixed precision)
+ *   E     complex [W][3]          (E1, EXX, EJ)
+ */
+#ifndef AFQ_H
+#define AFQ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  AFQ_OK = 0,
+  AFQ_ERR_ARG = -1,
+  AFQ_ERR_HIP = -2,
+  AFQ_ERR_SHAPE = -3,
+  AFQ_ERR_ALLOC = -4,
+  AFQ_ERR_STATE = -5
+} afq_status;
+
+enum { AFQ_CLOSED = 0, AFQ_COLLINEAR = 1 };          /* WALKER_TYPES (config.h) */
+enum { AFQ_FP64 = 0, AFQ_MIXED = 1 };                /* compute precision */
+
+typedef struct afq_ctx afq_ctx;
+typedef struct afq_ham afq_ham;
+typedef struct afq_prop afq_prop;
+
+/* ---- context ------------------------------------------------------------------------------ */
+
+/* Version of this ABI (major * 100 + minor). */
+int afq_abi_version(void);
+/* Text of the last error on this thread ("" when none). */
+const char* afq_last_error(void);
+/* Creates a context on `device`; `stream` is a hipStream_t or NULL for a private stream.
+ * Replaces arch::init's device selection (arch.cpp:60-69). */
+int afq_ctx_create(int device, void* stream, afq_ctx** out);
+int afq_ctx_destroy(afq_ctx* ctx);
+int afq_ctx_synchronize(afq_ctx* ctx);
+/* Bytes of scratch held by the context. */
+int afq_ctx_scratch_bytes(afq_ctx* ctx, uint64_t* bytes);
+
+/* ---- Cholesky hamiltonian operations (Real3IndexFactorization) ------------------------------ */
+
+/* RealDenseHamiltonian::getHamiltonianOperations (RealDenseHamiltonian.cpp:52-186) +
+ * Real3IndexFactorization ctor (Real3IndexFactorization.hpp:43-97): copies Likn, h1, PsiT and
+ * builds Lnak, haj and vexx on the device.  precision = AFQ_MIXED also keeps FP32 copies of the
+ * Cholesky tensors for the FP32 contractions. */
+int afq_ham_create(afq_ctx* ctx, int walker_type, int N, int nup, int ndown, int M,
+                   const double* Likn, const double* h1, const double* PsiT, double E0,
+                   int precision, afq_ham** out);
+int afq_ham_destroy(afq_ham* ham);
+/* Device views of the half-rotated tensors: Lnak complex [M][nup][N] (then [M][ndown][N] for
+ * COLLINEAR), haj complex [nel][N], vexx double [N][N]. number_of_cholesky_vectors() is M. */
+int afq_ham_tensors(afq_ham* ham, const double** Lnak, const double** haj, const double** vexx,
+                    int* real_trial);
+/* Real3IndexFactorization::vbias (Real3IndexFactorization.hpp:328-428):
+ * v[w][m] = sqrt(dt) * c * sum_{a,k} G[w][a][k] Lnak[m][a][k], c = 2 CLOSED / 1 COLLINEAR.
+ * precision AFQ_MIXED contracts in FP32 (FP32 MFMA, FP32 accumulation) and writes FP64 v. */
+int afq_vbias(afq_ham* ham, const double* G, int W, double dt, int precision, double* v);
+/* Real3IndexFactorization::vHS (Real3IndexFactorization.hpp:282-326):
+ * V[w][i][k] = sqrt(dt) * sum_m L[i][k][m] X[w][m].  FP64: X, V complex128.
+ * AFQ_MIXED: X, V complex64 (FP32 MFMA). */
+int afq_vhs(afq_ham* ham, const void* X, int W, double dt, int precision, void* V);
+/* Real3IndexFactorization::energy (Real3IndexFactorization.hpp:138-174, energy_impl :476-590):
+ * E[w] = (E0 + c sum G haj, -c/2 sum_{m,a,b} T_ab T_ba, c^2/2 sum_m (sum_a T_aa)^2).
+ * AFQ_MIXED runs the exchange contraction in FP32 with FP64 accumulation. */
+int afq_energy(afq_ham* ham, const double* G, int W, int addH1, int addEJ, int addEXX,
+               int precision, double* E);
+/* NOMSD::vMF (NOMSD.cpp:437-464) + zero_imag (AFQMCBasePropagator.cpp:105-108): vMF [M]. */
+int afq_vmf(afq_ham* ham, double dt, double* vMF);
+/* generateP1 + getOneBodyPropagatorMatrix (AFQMCBasePropagator.cpp:43-214,
+ * Real3IndexFactorization.hpp:99-124): P1 = exp(-1/2 hermitize(dt (h1 + vexx) + vHS(vMF))),
+ * entries with |x| <= 1e-8 set to zero (to_csr, csr_utils.hpp:43-66).  P1 complex [N][N]. */
+int afq_one_body_propagator(afq_ham* ham, const double* vMF, double dt, double* P1);
+
+/* ---- Slater-determinant operations (det_ops) --------------------------------------------- */
+
+/* det_ops::MixedDensityMatrix (density_matrix.hpp:535-601), herm = true.  ovlp [W] is
+ * accumulated: ovlp += log det(PsiT Phi), imaginary part wrapped to [-pi, pi]
+ * (determinants_impl.hpp:33-75).  compact != 0: G [W][n][N]; else full G [W][N][N]. */
+int afq_mixed_density_matrix(afq_ctx* ctx, const double* PsiT, int n, int N, const double* Phi,
+                             int W, double* G, double* ovlp, int compact);
+/* det_ops::Log_Overlap (density_matrix.hpp:431-447): ovlp += log det(PsiT Phi). */
+int afq_log_overlap(afq_ctx* ctx, const double* PsiT, int n, int N, const double* Phi, int W,
+                    double* ovlp);
+/* det_ops::detail::inverse_logdet (density_matrix.hpp:82-111): Ainv = A^{-1} (zero when
+ * singular), ovlp += log det A.  A, Ainv complex [nb][n][n]; A == Ainv is allowed. */
+int afq_inverse_logdet(afq_ctx* ctx, const double* A, int nb, int n, double* ovlp, double* Ainv);
+/* det_ops::detail::propagate_impl + apply_expM (propagate.hpp:41-70, :241-298):
+ * Phi <- P1 * (sum_{k<=order} (i op(V))^k / k!) * P1 * Phi, TA = 'N' or 'H' (op = V or V^H, -i).
+ * P1 complex128 [N][N]; V complex128 (AFQ_FP64) or complex64 (AFQ_MIXED) [W][N][N]. */
+int afq_propagate(afq_ctx* ctx, const double* P1, const void* V, double* Phi, int W, int N, int n,
+                  int order, char TA, int precision);
+/* det_ops::orthogonalize (orthogonalize.hpp:131-169): Phi = Q R with R upper triangular,
+ * positive real diagonal; Phi <- Q; ldet += log det R. */
+int afq_orthogonalize(afq_ctx* ctx, double* Phi, int W, int N, int n, double* ldet);
+
+/* ---- fields and weights ------------------------------------------------------------------- */
+
+/* Uniform field draws u[w][m] in (0,1) from Philox4x32-10 keyed by (seed, stream, step, w, m).
+ * Replaces RandomGenerator::sampleUniformFields (Random.hpp:37-85). */
+int afq_field_uniforms(afq_ctx* ctx, uint64_t seed, uint32_t stream, uint32_t step, int W, int M,
+                       double* u);
+/* assemble_X + construct_X (AFQMCBasePropagator.cpp:686-733, construct_X_kernel.hpp:34-86) and
+ * the mean-field factor (:335-336). X holds vbias on entry and the fields on exit; HW, mf [W]. */
+int afq_construct_x(afq_ctx* ctx, const double* u, const double* vMF, int W, int M, double bound,
+                    int project_vbias, int free_projection, double* X, double* HW, double* mf);
+
+typedef struct {
+  double dt, E0, lower_cut, upper_cut, weight_floor, weight_frac, target_pop;
+  int free_projection, use_cp;
+} afq_update_params;
+
+/* walker_update + bound_walker_weights (WalkerSetUpdate.hpp:70-218), hybrid branch.  Eshift is
+ * a device scalar.  weight, phase, eloc (PSEUDO_ELOC_), ovlp (OVLP), theta [W] updated. */
+int afq_walker_update(afq_ctx* ctx, int W, const afq_update_params* p, const double* Eshift,
+                      const double* new_ovlp, const double* mf, const double* HW, double* weight,
+                      double* phase, double* eloc, double* ovlp, double* theta);
+/* averagePseudoEnergy (average_energy.hpp:54-63) -> *out (device); when relax > 0 also
+ * *Eshift += relax * (*out - *Eshift) (run_afqmc.cpp:91). */
+int afq_average_pseudo_energy(afq_ctx* ctx, int W, const double* weight, const double* eloc,
+                              double dt, double relax, double* out, double* Eshift);
+
+/* ---- propagator (AFQMCBasePropagator, single-determinant trial) -------------------------- */
+
+typedef struct {
+  double dt;                 /* timestep */
+  int order;                 /* taylor_n (default 6) */
+  double vbias_bound;        /* default 50 */
+  double lower_cut, upper_cut;          /* lower/upper_cutoff_scale (1, 10) */
+  double weight_floor, weight_frac;     /* 100, 0.1 */
+  double target_pop;         /* global target population (<= 0: W) */
+  int free_projection, use_cp, project_force_bias;
+  uint64_t seed;             /* field RNG key */
+  uint32_t stream;           /* field RNG stream index */
+} afq_prop_params;
+
+typedef struct {
+  double* Phi;     /* complex [W][N][nup] (alpha / CLOSED) */
+  double* Phi_b;   /* complex [W][N][ndown] (COLLINEAR beta) or NULL */
+  double* weight;  /* complex [W] WEIGHT */
+  double* phase;   /* complex [W] PHASE */
+  double* eloc;    /* complex [W] PSEUDO_ELOC_ */
+  double* ovlp;    /* complex [W] OVLP (log overlap) */
+  double* theta;   /* complex [W] THETA */
+  int W;
+} afq_walkers;
+
+/* AFQMCBasePropagator constructor (AFQMCBasePropagator.h:45-171): builds vMF and P1. */
+int afq_prop_create(afq_ham* ham, int W, const afq_prop_params* p, afq_prop** out);
+int afq_prop_destroy(afq_prop* prop);
+/* Device views: vMF complex [M], P1 complex [N][N]. */
+int afq_prop_views(afq_prop* prop, const double** vMF, const double** P1);
+/* AFQMCBasePropagator::Propagate (AFQMCBasePropagator.cpp:289-392) for every walker.
+ * u: injected uniforms [W][M] (device) or NULL to draw them from Philox with the step counter
+ * held on the device (advanced by one per call).  Eshift: device scalar. */
+int afq_prop_step(afq_prop* prop, const afq_walkers* w, const double* u, const double* Eshift);
+/* AFQMCBasePropagator::Orthogonalize (AFQMCBasePropagator.cpp:735-741). */
+int afq_prop_orthogonalize(afq_prop* prop, const afq_walkers* w);
+/* Sets the device step counter used by the in-kernel RNG. */
+int afq_prop_set_step(afq_prop* prop, uint32_t step);
+/* NOMSD::Energy (NOMSD.cpp:159-217): E [W][3], ovlp [W] (log overlap). */
+int afq_prop_energy(afq_prop* prop, const afq_walkers* w, int precision, double* E, double* ovlp);
+/* One step + optional orthogonalization + growth estimator (-> *energy_out, device) + Eshift
+ * relaxation (relax > 0), captured in a HIP graph on first reuse and replayed afterwards
+ * (run_afqmc.cpp:69-100 without population control).  Uses the in-kernel RNG. */
+int afq_prop_step_graph(afq_prop* prop, const afq_walkers* w, double* Eshift, int ortho,
+                        double relax, double* energy_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AFQ_H */

@@ -1,0 +1,442 @@
+"""Python host mirror of the C ABI in include/afq.h (libafq_hip.so).
+
+The classes follow SAFIRE's interfaces for the walker step so tests read like the reference's:
+  HamiltonianOperations  ~ Real3IndexFactorization      vbias / vHS / energy / vMF / P1
+  det_ops                ~ det_ops free functions       MixedDensityMatrix / Log_Overlap /
+                                                        Propagate / orthogonalize / inverse_logdet
+  Propagator             ~ AFQMCBasePropagator          Propagate / Orthogonalize / Energy
+  WalkerSet              ~ WalkerSet (device-resident walker matrices and properties)
+
+Device memory and the stream come from PyTorch (plumbing only); every computation runs in the
+HIP library.  There is no CPU fallback: importing this module on a machine without the built
+library raises, and every call requires a GPU.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libafq_hip.so")
+
+AFQ_CLOSED, AFQ_COLLINEAR = 0, 1
+AFQ_FP64, AFQ_MIXED = 0, 1
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+
+
+class AfqError(RuntimeError):
+    pass
+
+
+class UpdateParams(ctypes.Structure):
+    _fields_ = [("dt", _D), ("E0", _D), ("lower_cut", _D), ("upper_cut", _D),
+                ("weight_floor", _D), ("weight_frac", _D), ("target_pop", _D),
+                ("free_projection", _I), ("use_cp", _I)]
+
+
+class PropParams(ctypes.Structure):
+    _fields_ = [("dt", _D), ("order", _I), ("vbias_bound", _D), ("lower_cut", _D),
+                ("upper_cut", _D), ("weight_floor", _D), ("weight_frac", _D), ("target_pop", _D),
+                ("free_projection", _I), ("use_cp", _I), ("project_force_bias", _I),
+                ("seed", ctypes.c_uint64), ("stream", ctypes.c_uint32)]
+
+
+class Walkers(ctypes.Structure):
+    _fields_ = [("Phi", _P), ("Phi_b", _P), ("weight", _P), ("phase", _P), ("eloc", _P),
+                ("ovlp", _P), ("theta", _P), ("W", _I)]
+
+
+# name -> argtypes (restype int unless noted)
+_SIGS = {
+    "afq_abi_version": [],
+    "afq_last_error": [],
+    "afq_ctx_create": [_I, _P, ctypes.POINTER(_P)],
+    "afq_ctx_destroy": [_P],
+    "afq_ctx_synchronize": [_P],
+    "afq_ctx_scratch_bytes": [_P, ctypes.POINTER(ctypes.c_uint64)],
+    "afq_ham_create": [_P, _I, _I, _I, _I, _I, _P, _P, _P, _D, _I, ctypes.POINTER(_P)],
+    "afq_ham_destroy": [_P],
+    "afq_ham_tensors": [_P, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_P),
+                        ctypes.POINTER(_I)],
+    "afq_vbias": [_P, _P, _I, _D, _I, _P],
+    "afq_vhs": [_P, _P, _I, _D, _I, _P],
+    "afq_energy": [_P, _P, _I, _I, _I, _I, _I, _P],
+    "afq_vmf": [_P, _D, _P],
+    "afq_one_body_propagator": [_P, _P, _D, _P],
+    "afq_mixed_density_matrix": [_P, _P, _I, _I, _P, _I, _P, _P, _I],
+    "afq_log_overlap": [_P, _P, _I, _I, _P, _I, _P],
+    "afq_inverse_logdet": [_P, _P, _I, _I, _P, _P],
+    "afq_propagate": [_P, _P, _P, _P, _I, _I, _I, _I, ctypes.c_char, _I],
+    "afq_orthogonalize": [_P, _P, _I, _I, _I, _P],
+    "afq_field_uniforms": [_P, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _I, _I, _P],
+    "afq_construct_x": [_P, _P, _P, _I, _I, _D, _I, _I, _P, _P, _P],
+    "afq_walker_update": [_P, _I, ctypes.POINTER(UpdateParams), _P, _P, _P, _P, _P, _P, _P, _P,
+                          _P],
+    "afq_average_pseudo_energy": [_P, _I, _P, _P, _D, _D, _P, _P],
+    "afq_prop_create": [_P, _I, ctypes.POINTER(PropParams), ctypes.POINTER(_P)],
+    "afq_prop_destroy": [_P],
+    "afq_prop_views": [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)],
+    "afq_prop_step": [_P, ctypes.POINTER(Walkers), _P, _P],
+    "afq_prop_orthogonalize": [_P, ctypes.POINTER(Walkers)],
+    "afq_prop_set_step": [_P, ctypes.c_uint32],
+    "afq_prop_energy": [_P, ctypes.POINTER(Walkers), _I, _P, _P],
+    "afq_prop_step_graph": [_P, ctypes.POINTER(Walkers), _P, _I, _D, _P],
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises when it has not been built (no fallback path exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise AfqError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                           "(make -C safire_amd)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _I
+        L.afq_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def _ck(rc):
+    if rc != 0:
+        raise AfqError(f"afq error {rc}: {lib().afq_last_error().decode()}")
+
+
+# ---------------------------------------------------------------------------------------------
+# device arrays (torch is plumbing: allocation, host<->device copies, the stream)
+# ---------------------------------------------------------------------------------------------
+
+def _torch():
+    import torch
+    return torch
+
+
+def device_tensor(a, dtype=None):
+    """Copy a numpy array to the GPU (complex stays complex)."""
+    torch = _torch()
+    a = np.ascontiguousarray(a if dtype is None else np.asarray(a, dtype=dtype))
+    return torch.from_numpy(a).to("cuda")
+
+
+def empty(shape, dtype):
+    torch = _torch()
+    return torch.empty(shape, dtype=dtype, device="cuda")
+
+
+def zeros(shape, dtype):
+    torch = _torch()
+    return torch.zeros(shape, dtype=dtype, device="cuda")
+
+
+def ptr(t):
+    if t is None:
+        return None
+    assert t.is_contiguous(), "arrays must be contiguous"
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def host(t):
+    _torch().cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+# ---------------------------------------------------------------------------------------------
+# context
+# ---------------------------------------------------------------------------------------------
+
+_STREAM = {}
+
+
+def shared_stream(device: int = 0):
+    """One non-default stream per device, made torch's current stream, on which every context
+    and every torch operation of this process run: library calls and torch copies are then in
+    program order, and the stream can be captured into HIP graphs."""
+    torch = _torch()
+    if device not in _STREAM:
+        _STREAM[device] = torch.cuda.Stream(device=device)
+    torch.cuda.set_stream(_STREAM[device])
+    return _STREAM[device]
+
+
+class Context:
+    """afq_ctx on the process-wide stream of `device` (see shared_stream)."""
+
+    def __init__(self, device: int = 0):
+        torch = _torch()
+        if not torch.cuda.is_available():
+            raise AfqError("no GPU available: the AFQMC step runs only on the device")
+        torch.cuda.set_device(device)
+        self.stream = shared_stream(device)
+        h = _P()
+        _ck(lib().afq_ctx_create(device, ctypes.c_void_p(self.stream.cuda_stream), ctypes.byref(h)))
+        self.h = h
+
+    def synchronize(self):
+        _ck(lib().afq_ctx_synchronize(self.h))
+
+    def scratch_bytes(self):
+        v = ctypes.c_uint64()
+        _ck(lib().afq_ctx_scratch_bytes(self.h, ctypes.byref(v)))
+        return v.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().afq_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------------------------
+# det_ops
+# ---------------------------------------------------------------------------------------------
+
+class det_ops:  # noqa: N801  (namespace mirroring sfqmc::afqmc::det_ops)
+    @staticmethod
+    def MixedDensityMatrix(ctx, PsiT, Phi, G, ovlp, compact=True):
+        n, N = PsiT.shape
+        W = Phi.shape[0]
+        _ck(lib().afq_mixed_density_matrix(ctx.h, ptr(PsiT), n, N, ptr(Phi), W, ptr(G), ptr(ovlp),
+                                           int(compact)))
+
+    @staticmethod
+    def Log_Overlap(ctx, PsiT, Phi, ovlp):
+        n, N = PsiT.shape
+        _ck(lib().afq_log_overlap(ctx.h, ptr(PsiT), n, N, ptr(Phi), Phi.shape[0], ptr(ovlp)))
+
+    @staticmethod
+    def inverse_logdet(ctx, A, ovlp, Ainv):
+        nb, n, _ = A.shape
+        _ck(lib().afq_inverse_logdet(ctx.h, ptr(A), nb, n, ptr(ovlp), ptr(Ainv)))
+
+    @staticmethod
+    def Propagate(ctx, P1, V, Phi, order=6, TA="N", precision=AFQ_FP64):
+        W, N, n = Phi.shape
+        _ck(lib().afq_propagate(ctx.h, ptr(P1), ptr(V), ptr(Phi), W, N, n, order, TA.encode(),
+                                precision))
+
+    @staticmethod
+    def orthogonalize(ctx, Phi, ldet):
+        W, N, n = Phi.shape
+        _ck(lib().afq_orthogonalize(ctx.h, ptr(Phi), W, N, n, ptr(ldet)))
+
+
+def field_uniforms(ctx, seed, stream, step, u):
+    W, M = u.shape
+    _ck(lib().afq_field_uniforms(ctx.h, seed, stream, step, W, M, ptr(u)))
+
+
+def construct_x(ctx, u, vMF, X, HW, mf, bound=50.0, project_vbias=False, free_projection=False):
+    W, M = X.shape
+    _ck(lib().afq_construct_x(ctx.h, ptr(u), ptr(vMF), W, M, bound, int(project_vbias),
+                              int(free_projection), ptr(X), ptr(HW), ptr(mf)))
+
+
+def walker_update(ctx, params: UpdateParams, Eshift, new_ovlp, mf, HW, weight, phase, eloc, ovlp,
+                  theta):
+    _ck(lib().afq_walker_update(ctx.h, weight.shape[0], ctypes.byref(params), ptr(Eshift),
+                                ptr(new_ovlp), ptr(mf), ptr(HW), ptr(weight), ptr(phase),
+                                ptr(eloc), ptr(ovlp), ptr(theta)))
+
+
+def average_pseudo_energy(ctx, weight, eloc, dt, out, relax=0.0, Eshift=None):
+    _ck(lib().afq_average_pseudo_energy(ctx.h, weight.shape[0], ptr(weight), ptr(eloc), dt, relax,
+                                        ptr(out), ptr(Eshift)))
+
+
+# ---------------------------------------------------------------------------------------------
+# HamiltonianOperations (Real3IndexFactorization)
+# ---------------------------------------------------------------------------------------------
+
+class HamiltonianOperations:
+    def __init__(self, ctx, h1, Likn, PsiT, nup, ndown=0, walker_type=AFQ_CLOSED, E0=0.0,
+                 precision=AFQ_FP64):
+        torch = _torch()
+        self.ctx = ctx
+        N = h1.shape[0]
+        M = Likn.shape[2]
+        self.N, self.M, self.nup, self.ndown = N, M, nup, (ndown if walker_type == AFQ_COLLINEAR else 0)
+        self.walker_type = walker_type
+        self.precision = precision
+        self.E0 = E0
+        dL = device_tensor(Likn, np.float64)
+        dh = device_tensor(h1, np.float64)
+        dP = device_tensor(PsiT, np.complex128)
+        h = _P()
+        _ck(lib().afq_ham_create(ctx.h, walker_type, N, nup, ndown, M, ptr(dL), ptr(dh), ptr(dP),
+                                 E0, precision, ctypes.byref(h)))
+        ctx.synchronize()
+        del dL, dh, dP
+        self.h = h
+        lp, hp, vp, rt = _P(), _P(), _P(), _I()
+        _ck(lib().afq_ham_tensors(h, ctypes.byref(lp), ctypes.byref(hp), ctypes.byref(vp),
+                                  ctypes.byref(rt)))
+        self.real_trial = bool(rt.value)
+        self._views = (lp.value, hp.value, vp.value)
+        self._torch = torch
+
+    @property
+    def nel(self):
+        return self.nup + (self.ndown if self.walker_type == AFQ_COLLINEAR else 0)
+
+    def number_of_cholesky_vectors(self):
+        return self.M
+
+    def energy_offset(self):
+        return self.E0
+
+    def tensors(self):
+        """Host copies of (Lnak [M][n][N] per spin block, haj [nel][N], vexx [N][N])."""
+        torch = self._torch
+        self.ctx.synchronize()
+        L = torch.empty(self.M * self.nel * self.N, dtype=torch.complex128, device="cuda")
+        H = torch.empty(self.nel * self.N, dtype=torch.complex128, device="cuda")
+        V = torch.empty(self.N * self.N, dtype=torch.float64, device="cuda")
+        for t, p in ((L, self._views[0]), (H, self._views[1]), (V, self._views[2])):
+            _copy_d2d(t.data_ptr(), p, t.numel() * t.element_size())
+        return host(L), host(H).reshape(self.nel, self.N), host(V).reshape(self.N, self.N)
+
+    def vbias(self, G, v, dt, precision=None):
+        W = G.shape[0]
+        _ck(lib().afq_vbias(self.h, ptr(G), W, dt, self.precision if precision is None else precision,
+                            ptr(v)))
+
+    def vHS(self, X, V, dt, precision=None):
+        W = X.shape[0]
+        _ck(lib().afq_vhs(self.h, ptr(X), W, dt, self.precision if precision is None else precision,
+                          ptr(V)))
+
+    def energy(self, E, G, addH1=True, addEJ=True, addEXX=True, precision=None):
+        W = G.shape[0]
+        _ck(lib().afq_energy(self.h, ptr(G), W, int(addH1), int(addEJ), int(addEXX),
+                             self.precision if precision is None else precision, ptr(E)))
+
+    def vMF(self, dt, v):
+        _ck(lib().afq_vmf(self.h, dt, ptr(v)))
+
+    def getOneBodyPropagator(self, vMF, dt, P1):
+        _ck(lib().afq_one_body_propagator(self.h, ptr(vMF), dt, ptr(P1)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().afq_ham_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _copy_d2d(dst, src, nbytes):
+    """Device-to-device copy of a library-owned view into a torch buffer."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [_P, _P, ctypes.c_size_t, _I]
+    hip.hipMemcpy.restype = _I
+    rc = hip.hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src), nbytes, 3)
+    if rc != 0:
+        raise AfqError(f"hipMemcpy failed ({rc})")
+
+
+# ---------------------------------------------------------------------------------------------
+# walker set + propagator
+# ---------------------------------------------------------------------------------------------
+
+class WalkerSet:
+    """Device-resident walkers: Slater matrices and the per-walker properties of
+    WalkerConfig.hpp:28-48 (WEIGHT, PHASE, PSEUDO_ELOC_, OVLP, THETA)."""
+
+    def __init__(self, Phi, Phi_b=None):
+        torch = _torch()
+        self.Phi = device_tensor(Phi, np.complex128)
+        self.Phi_b = None if Phi_b is None else device_tensor(Phi_b, np.complex128)
+        W = self.Phi.shape[0]
+        self.W = W
+        self.weight = torch.ones(W, dtype=torch.complex128, device="cuda")
+        self.phase = torch.ones(W, dtype=torch.complex128, device="cuda")
+        self.eloc = torch.zeros(W, dtype=torch.complex128, device="cuda")
+        self.ovlp = torch.zeros(W, dtype=torch.complex128, device="cuda")
+        self.theta = torch.zeros(W, dtype=torch.complex128, device="cuda")
+        self._c = None
+
+    def cstruct(self):
+        self._c = Walkers(ptr(self.Phi), ptr(self.Phi_b), ptr(self.weight), ptr(self.phase),
+                          ptr(self.eloc), ptr(self.ovlp), ptr(self.theta), self.W)
+        return self._c
+
+
+class Propagator:
+    def __init__(self, ham: HamiltonianOperations, W, dt=0.005, order=6, vbias_bound=50.0,
+                 seed=0, stream=1, target_pop=0.0, free_projection=False):
+        self.ham = ham
+        self.ctx = ham.ctx
+        self.dt = dt
+        p = PropParams(dt, order, vbias_bound, 1.0, 10.0, 100.0, 0.1, target_pop,
+                       int(free_projection), 0, 0, seed, stream)
+        h = _P()
+        _ck(lib().afq_prop_create(ham.h, W, ctypes.byref(p), ctypes.byref(h)))
+        self.h = h
+        v, P = _P(), _P()
+        _ck(lib().afq_prop_views(h, ctypes.byref(v), ctypes.byref(P)))
+        self._vmf, self._p1 = v.value, P.value
+
+    def vMF(self):
+        torch = _torch()
+        out = torch.empty(self.ham.M, dtype=torch.complex128, device="cuda")
+        self.ctx.synchronize()
+        _copy_d2d(out.data_ptr(), self._vmf, out.numel() * 16)
+        return out
+
+    def P1(self):
+        torch = _torch()
+        out = torch.empty((self.ham.N, self.ham.N), dtype=torch.complex128, device="cuda")
+        self.ctx.synchronize()
+        _copy_d2d(out.data_ptr(), self._p1, out.numel() * 16)
+        return out
+
+    def Propagate(self, wset: WalkerSet, Eshift, u=None):
+        _ck(lib().afq_prop_step(self.h, ctypes.byref(wset.cstruct()), ptr(u), ptr(Eshift)))
+
+    def Orthogonalize(self, wset: WalkerSet):
+        _ck(lib().afq_prop_orthogonalize(self.h, ctypes.byref(wset.cstruct())))
+
+    def Energy(self, wset: WalkerSet, E, ovlp, precision=AFQ_FP64):
+        _ck(lib().afq_prop_energy(self.h, ctypes.byref(wset.cstruct()), precision, ptr(E), ptr(ovlp)))
+
+    def set_step(self, step):
+        _ck(lib().afq_prop_set_step(self.h, step))
+
+    def step_graph(self, wset: WalkerSet, Eshift, ortho=False, relax=0.0, energy_out=None):
+        _ck(lib().afq_prop_step_graph(self.h, ctypes.byref(wset.cstruct()), ptr(Eshift), int(ortho),
+                                      relax, ptr(energy_out)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().afq_prop_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

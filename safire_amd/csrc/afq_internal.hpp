@@ -1,0 +1,70 @@
+// Internal launcher declarations shared by the kernel translation units and the C-ABI layer.
+#pragma once
+
+#include "afq_common.hpp"
+
+namespace afq {
+
+// ---- dense_small.hip -------------------------------------------------------------------------
+void lu_inverse_logdet(hipStream_t s, const z64* A, long a_bs, int nb, int n, z64* Ainv, long i_bs,
+                       z64* ovlp, double ov_scale, z64 ov_add, bool want_inverse);
+void chol_upper_inverse(hipStream_t s, const z64* S, int nb, int n, z64* Rinv, z64* ldet);
+
+// ---- fields.hip ------------------------------------------------------------------------------
+struct FieldParams {
+  double bound;          // vbias_bound
+  int project_vbias;     // use_cp_constraint || project_force_bias
+  int free_projection;
+  uint64_t seed;         // Philox key (used when u == nullptr)
+  uint32_t stream;
+  uint32_t step;
+  const uint32_t* step_ptr;  // when set, the step index is read from device memory
+};
+// X holds vbias on entry (complex [W][M]), fields on exit; HW, mf [W]. u == nullptr draws the
+// uniforms in-kernel from Philox4x32-10 keyed by (seed, stream, step, walker, field). X32, when
+// given, receives a complex64 copy of the fields.
+void construct_x(hipStream_t s, const double* u, const z64* vMF, int W, int M,
+                 const FieldParams& p, z64* X, z64* HW, z64* mf, c32* X32);
+void field_uniforms(hipStream_t s, uint64_t seed, uint32_t stream, uint32_t step, int W, int M,
+                    double* u);
+
+struct UpdateParams {
+  double dt, E0, lower_cut, upper_cut, weight_floor, weight_frac, target_pop;
+  int free_projection, use_cp;
+};
+// Eshift is read from device memory (so a captured step needs no host value).
+void walker_update(hipStream_t s, int W, const UpdateParams& p, const double* Eshift,
+                   const z64* new_ovlp, const z64* mf, const z64* HW, z64* weight, z64* phase,
+                   z64* eloc, z64* ovlp, z64* theta);
+void increment_counter(hipStream_t s, uint32_t* c);
+// growth estimator log(sum w e^{dt E} / sum w)/dt -> out[0]; Eshift[0] += f (out - Eshift) when
+// relax > 0.
+void average_pseudo_energy(hipStream_t s, int W, const z64* weight, const z64* eloc, double dt,
+                           double relax, double* out, double* Eshift);
+
+// ---- elementwise.hip -------------------------------------------------------------------------
+void scale_add_cplx(hipStream_t s, long n, const z64* x, double ar, double ai, z64* y);  // y = a x
+void cast_z64_c32(hipStream_t s, long n, const z64* x, c32* y);
+void cast_c32_z64(hipStream_t s, long n, const c32* x, z64* y);
+void cast_f64_f32(hipStream_t s, long n, const double* x, float* y);
+void fill_z64(hipStream_t s, long n, z64 v, z64* y);
+void zero_imag(hipStream_t s, long n, z64* y);
+void truncate_small(hipStream_t s, long n, double cut, z64* y);
+void hermitize_scale(hipStream_t s, int N, double scale, z64* A);
+void add_one_body(hipStream_t s, int N, double dt, const double* h, const double* vexx,
+                  const z64* shift, z64* H1);
+void identity(hipStream_t s, int N, z64* A);
+void axpy_z64(hipStream_t s, long n, double ar, double ai, const z64* x, z64* y);  // y += a x
+void permute_akm_mak(hipStream_t s, int A, int K, int M, const z64* in, z64* out);
+void conj_transpose(hipStream_t s, int R, int C, const z64* in, z64* out);
+void real_part(hipStream_t s, long n, const z64* x, double* y);
+void real_part_f32(hipStream_t s, long n, const z64* x, float* y);
+// out[w*stride] += c * sum_k G[w][k] H[k] + add
+void dot_rows(hipStream_t s, int W, long K, const z64* G, const z64* H, double cr, double ci,
+              double add, z64* out, long out_stride);
+void exchange_reduce(hipStream_t s, int W, int n, int MC, int m0, int M, const z64* T,
+                     z64* exx_part, int nchunk, int chunk_id, z64* Kl);
+void energy_finish(hipStream_t s, int W, int M, int nchunk, const z64* exx_part, const z64* Kl,
+                   double cx, double cj, int addEXX, int addEJ, z64* E);
+
+}  // namespace afq

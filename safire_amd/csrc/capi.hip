@@ -1,0 +1,349 @@
+// C ABI (include/afq.h) over the device classes.  Exceptions never cross the boundary: each
+// entry point converts afq::Error into its status code and keeps the message per thread.
+#include <string>
+
+#include "../../include/afq.h"
+#include "ops.hpp"
+
+using afq::z64;
+
+struct afq_ctx {
+  afq::Context c;
+  afq_ctx(int dev, hipStream_t s) : c(dev, s) {}
+};
+struct afq_ham {
+  afq_ctx* ctx;
+  afq::CholeskyOps* ops;
+};
+struct afq_prop {
+  afq_ham* ham;
+  afq::Propagator* p;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    g_err.clear();
+    return AFQ_OK;
+  } catch (const afq::Error& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return AFQ_ERR_STATE;
+  } catch (...) {
+    g_err = "unknown error";
+    return AFQ_ERR_STATE;
+  }
+}
+
+#define REQUIRE(cond, msg) AFQ_CHECK((cond), afq::kErrArg, (msg))
+
+inline const z64* Z(const double* p) { return reinterpret_cast<const z64*>(p); }
+inline z64* Z(double* p) { return reinterpret_cast<z64*>(p); }
+
+afq::WalkerState to_state(const afq_walkers* w) {
+  return afq::WalkerState{Z(w->Phi), Z(w->Phi_b), Z(w->weight), Z(w->phase), Z(w->eloc),
+                          Z(w->ovlp), Z(w->theta), w->W};
+}
+
+void check_walkers(const afq_prop* p, const afq_walkers* w) {
+  REQUIRE(w != nullptr, "walkers is NULL");
+  REQUIRE(w->Phi && w->weight && w->phase && w->eloc && w->ovlp && w->theta, "walker array is NULL");
+  REQUIRE(w->W > 0, "W must be positive");
+  if (p->ham->ops->walker_type() == afq::kCollinear && p->ham->ops->ndown() > 0)
+    REQUIRE(w->Phi_b != nullptr, "COLLINEAR walkers need Phi_b");
+}
+
+}  // namespace
+
+extern "C" {
+
+int afq_abi_version(void) { return 100; }
+
+const char* afq_last_error(void) { return g_err.c_str(); }
+
+int afq_ctx_create(int device, void* stream, afq_ctx** out) {
+  return guard([&] {
+    REQUIRE(out != nullptr, "out is NULL");
+    int n = 0;
+    AFQ_HIP(hipGetDeviceCount(&n));
+    AFQ_CHECK(device >= 0 && device < n, afq::kErrArg, "no such device");
+    *out = new afq_ctx(device, reinterpret_cast<hipStream_t>(stream));
+  });
+}
+
+int afq_ctx_destroy(afq_ctx* ctx) {
+  return guard([&] { delete ctx; });
+}
+
+int afq_ctx_synchronize(afq_ctx* ctx) {
+  return guard([&] {
+    REQUIRE(ctx, "ctx is NULL");
+    AFQ_HIP(hipStreamSynchronize(ctx->c.stream()));
+  });
+}
+
+int afq_ctx_scratch_bytes(afq_ctx* ctx, uint64_t* bytes) {
+  return guard([&] {
+    REQUIRE(ctx && bytes, "NULL argument");
+    *bytes = ctx->c.scratch_bytes();
+  });
+}
+
+int afq_ham_create(afq_ctx* ctx, int walker_type, int N, int nup, int ndown, int M,
+                   const double* Likn, const double* h1, const double* PsiT, double E0,
+                   int precision, afq_ham** out) {
+  return guard([&] {
+    REQUIRE(ctx && out && Likn && h1 && PsiT, "NULL argument");
+    auto* h = new afq_ham{ctx, nullptr};
+    try {
+      h->ops = new afq::CholeskyOps(ctx->c, walker_type, N, nup, ndown, M, Likn, h1, Z(PsiT), E0,
+                                    precision);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int afq_ham_destroy(afq_ham* ham) {
+  return guard([&] {
+    if (!ham) return;
+    delete ham->ops;
+    delete ham;
+  });
+}
+
+int afq_ham_tensors(afq_ham* ham, const double** Lnak, const double** haj, const double** vexx,
+                    int* real_trial) {
+  return guard([&] {
+    REQUIRE(ham, "ham is NULL");
+    if (Lnak) *Lnak = reinterpret_cast<const double*>(ham->ops->Lnak());
+    if (haj) *haj = reinterpret_cast<const double*>(ham->ops->haj());
+    if (vexx) *vexx = ham->ops->vexx();
+    if (real_trial) *real_trial = ham->ops->real_trial() ? 1 : 0;
+  });
+}
+
+int afq_vbias(afq_ham* ham, const double* G, int W, double dt, int precision, double* v) {
+  return guard([&] {
+    REQUIRE(ham && G && v, "NULL argument");
+    REQUIRE(W >= 0, "W");
+    ham->ops->vbias(Z(G), W, dt, Z(v), (afq::Precision)precision);
+  });
+}
+
+int afq_vhs(afq_ham* ham, const void* X, int W, double dt, int precision, void* V) {
+  return guard([&] {
+    REQUIRE(ham && X && V, "NULL argument");
+    REQUIRE(W >= 0, "W");
+    ham->ops->vHS(X, W, dt, V, (afq::Precision)precision);
+  });
+}
+
+int afq_energy(afq_ham* ham, const double* G, int W, int addH1, int addEJ, int addEXX,
+               int precision, double* E) {
+  return guard([&] {
+    REQUIRE(ham && G && E, "NULL argument");
+    ham->ops->energy(Z(G), W, Z(E), addH1 != 0, addEJ != 0, addEXX != 0, (afq::Precision)precision);
+  });
+}
+
+int afq_vmf(afq_ham* ham, double dt, double* vMF) {
+  return guard([&] {
+    REQUIRE(ham && vMF, "NULL argument");
+    ham->ops->vMF(dt, Z(vMF));
+  });
+}
+
+int afq_one_body_propagator(afq_ham* ham, const double* vMF, double dt, double* P1) {
+  return guard([&] {
+    REQUIRE(ham && vMF && P1, "NULL argument");
+    ham->ops->one_body_propagator(Z(vMF), dt, Z(P1));
+  });
+}
+
+int afq_mixed_density_matrix(afq_ctx* ctx, const double* PsiT, int n, int N, const double* Phi,
+                             int W, double* G, double* ovlp, int compact) {
+  return guard([&] {
+    REQUIRE(ctx && PsiT && Phi && G && ovlp, "NULL argument");
+    REQUIRE(n >= 0 && n <= 64 && N >= n && W >= 0, "invalid sizes");
+    long gbs = compact ? (long)n * N : (long)N * N;
+    afq::det_ops::mixed_density_matrix(ctx->c, Z(PsiT), n, N, Z(Phi), (long)N * n, W, Z(G), gbs,
+                                       Z(ovlp), compact != 0);
+  });
+}
+
+int afq_log_overlap(afq_ctx* ctx, const double* PsiT, int n, int N, const double* Phi, int W,
+                    double* ovlp) {
+  return guard([&] {
+    REQUIRE(ctx && PsiT && Phi && ovlp, "NULL argument");
+    REQUIRE(n >= 0 && n <= 64 && N >= n && W >= 0, "invalid sizes");
+    afq::det_ops::log_overlap(ctx->c, Z(PsiT), n, N, Z(Phi), (long)N * n, W, Z(ovlp));
+  });
+}
+
+int afq_inverse_logdet(afq_ctx* ctx, const double* A, int nb, int n, double* ovlp, double* Ainv) {
+  return guard([&] {
+    REQUIRE(ctx && A && ovlp && Ainv, "NULL argument");
+    REQUIRE(n >= 1 && n <= 64 && nb >= 0, "invalid sizes");
+    afq::det_ops::inverse_logdet(ctx->c, Z(A), nb, n, Z(ovlp), Z(Ainv));
+  });
+}
+
+int afq_propagate(afq_ctx* ctx, const double* P1, const void* V, double* Phi, int W, int N, int n,
+                  int order, char TA, int precision) {
+  return guard([&] {
+    REQUIRE(ctx && P1 && V && Phi, "NULL argument");
+    REQUIRE(W >= 0 && N > 0 && n >= 0 && order >= 0, "invalid sizes");
+    REQUIRE(TA == 'N' || TA == 'H', "TA must be 'N' or 'H'");
+    REQUIRE(precision == AFQ_FP64 || precision == AFQ_MIXED, "precision");
+    afq::det_ops::PropagateArgs a{W, N, n, order, TA, Z(P1), nullptr, nullptr, nullptr, V,
+                                  (long)N * N, Z(Phi), (long)N * n, (afq::Precision)precision};
+    if (precision == AFQ_MIXED) {
+      afq::c32* p32 = afq::scratch_as<afq::c32>(ctx->c, "capi_P1c32", (size_t)N * N);
+      afq::cast_z64_c32(ctx->c.stream(), (long)N * N, Z(P1), p32);
+      a.P1c32 = p32;
+    }
+    afq::det_ops::propagate(ctx->c, a);
+  });
+}
+
+int afq_orthogonalize(afq_ctx* ctx, double* Phi, int W, int N, int n, double* ldet) {
+  return guard([&] {
+    REQUIRE(ctx && Phi && ldet, "NULL argument");
+    REQUIRE(n >= 0 && n <= 64 && N >= n && W >= 0, "invalid sizes");
+    afq::det_ops::orthogonalize(ctx->c, Z(Phi), (long)N * n, W, N, n, Z(ldet));
+  });
+}
+
+int afq_field_uniforms(afq_ctx* ctx, uint64_t seed, uint32_t stream, uint32_t step, int W, int M,
+                       double* u) {
+  return guard([&] {
+    REQUIRE(ctx && u, "NULL argument");
+    afq::field_uniforms(ctx->c.stream(), seed, stream, step, W, M, u);
+  });
+}
+
+int afq_construct_x(afq_ctx* ctx, const double* u, const double* vMF, int W, int M, double bound,
+                    int project_vbias, int free_projection, double* X, double* HW, double* mf) {
+  return guard([&] {
+    REQUIRE(ctx && u && vMF && X && HW && mf, "NULL argument");
+    afq::FieldParams fp{bound, project_vbias, free_projection, 0, 0, 0, nullptr};
+    afq::construct_x(ctx->c.stream(), u, Z(vMF), W, M, fp, Z(X), Z(HW), Z(mf), nullptr);
+  });
+}
+
+int afq_walker_update(afq_ctx* ctx, int W, const afq_update_params* p, const double* Eshift,
+                      const double* new_ovlp, const double* mf, const double* HW, double* weight,
+                      double* phase, double* eloc, double* ovlp, double* theta) {
+  return guard([&] {
+    REQUIRE(ctx && p && Eshift && new_ovlp && mf && HW && weight && phase && eloc && ovlp && theta,
+            "NULL argument");
+    afq::UpdateParams up{p->dt, p->E0, p->lower_cut, p->upper_cut, p->weight_floor, p->weight_frac,
+                         p->target_pop, p->free_projection, p->use_cp};
+    afq::walker_update(ctx->c.stream(), W, up, Eshift, Z(new_ovlp), Z(mf), Z(HW), Z(weight),
+                       Z(phase), Z(eloc), Z(ovlp), Z(theta));
+  });
+}
+
+int afq_average_pseudo_energy(afq_ctx* ctx, int W, const double* weight, const double* eloc,
+                              double dt, double relax, double* out, double* Eshift) {
+  return guard([&] {
+    REQUIRE(ctx && weight && eloc, "NULL argument");
+    afq::average_pseudo_energy(ctx->c.stream(), W, Z(weight), Z(eloc), dt, relax, out, Eshift);
+  });
+}
+
+int afq_prop_create(afq_ham* ham, int W, const afq_prop_params* p, afq_prop** out) {
+  return guard([&] {
+    REQUIRE(ham && p && out, "NULL argument");
+    afq::PropagatorParams pp;
+    pp.dt = p->dt;
+    pp.order = p->order;
+    pp.vbias_bound = p->vbias_bound;
+    pp.lower_cut = p->lower_cut;
+    pp.upper_cut = p->upper_cut;
+    pp.weight_floor = p->weight_floor;
+    pp.weight_frac = p->weight_frac;
+    pp.target_pop = p->target_pop;
+    pp.free_projection = p->free_projection;
+    pp.use_cp = p->use_cp;
+    pp.project_force_bias = p->project_force_bias;
+    pp.seed = p->seed;
+    pp.stream = p->stream;
+    auto* pr = new afq_prop{ham, nullptr};
+    try {
+      pr->p = new afq::Propagator(*ham->ops, W, pp);
+    } catch (...) {
+      delete pr;
+      throw;
+    }
+    *out = pr;
+  });
+}
+
+int afq_prop_destroy(afq_prop* prop) {
+  return guard([&] {
+    if (!prop) return;
+    delete prop->p;
+    delete prop;
+  });
+}
+
+int afq_prop_views(afq_prop* prop, const double** vMF, const double** P1) {
+  return guard([&] {
+    REQUIRE(prop, "prop is NULL");
+    if (vMF) *vMF = reinterpret_cast<const double*>(prop->p->vMF());
+    if (P1) *P1 = reinterpret_cast<const double*>(prop->p->P1());
+  });
+}
+
+int afq_prop_step(afq_prop* prop, const afq_walkers* w, const double* u, const double* Eshift) {
+  return guard([&] {
+    REQUIRE(prop && Eshift, "NULL argument");
+    check_walkers(prop, w);
+    prop->p->step(to_state(w), u, Eshift);
+  });
+}
+
+int afq_prop_orthogonalize(afq_prop* prop, const afq_walkers* w) {
+  return guard([&] {
+    REQUIRE(prop, "prop is NULL");
+    check_walkers(prop, w);
+    prop->p->orthogonalize(to_state(w));
+  });
+}
+
+int afq_prop_set_step(afq_prop* prop, uint32_t step) {
+  return guard([&] {
+    REQUIRE(prop, "prop is NULL");
+    prop->p->set_step(step);
+  });
+}
+
+int afq_prop_energy(afq_prop* prop, const afq_walkers* w, int precision, double* E, double* ovlp) {
+  return guard([&] {
+    REQUIRE(prop && E && ovlp, "NULL argument");
+    check_walkers(prop, w);
+    prop->p->energy(to_state(w), Z(E), Z(ovlp), (afq::Precision)precision);
+  });
+}
+
+int afq_prop_step_graph(afq_prop* prop, const afq_walkers* w, double* Eshift, int ortho,
+                        double relax, double* energy_out) {
+  return guard([&] {
+    REQUIRE(prop && Eshift, "NULL argument");
+    check_walkers(prop, w);
+    prop->p->step_graph(to_state(w), Eshift, ortho != 0, relax, energy_out);
+  });
+}
+
+}  // extern "C"

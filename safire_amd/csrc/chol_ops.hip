@@ -1,0 +1,361 @@
+// Cholesky (real 3-index factorization) hamiltonian operations on the device.
+//
+//   setup        RealDenseHamiltonian::getHamiltonianOperations (RealDenseHamiltonian.cpp:139-181):
+//                Lnak[m][a][k] = sum_i PsiT[a][i] L[i][k][m], haj = PsiT h1,
+//                vexx[i][l] = -1/2 sum_{j,m} L[i][j][m] L[l][j][m]
+//   vbias        Real3IndexFactorization::vbias (Real3IndexFactorization.hpp:328-428)
+//   vHS          Real3IndexFactorization::vHS (:282-326)
+//   energy       Real3IndexFactorization::energy + energy_impl (:138-174, :476-590)
+//   vMF          NOMSD::vMF (NOMSD.cpp:437-464) + zero_imag (AFQMCBasePropagator.cpp:105-108)
+//   one-body P1  generateP1 (AFQMCBasePropagator.cpp:43-214), getOneBodyPropagatorMatrix (:99-124),
+//                add_one_body_shifts (one_body.hpp), exp_hermitian (exp.hpp:28-60)
+//
+// The large contractions are single MFMA GEMMs over the walker batch: vbias is
+// [W x nN] . [nN x M], vHS is [W x M] . [M x N^2] with a real right operand.  When the trial is
+// real the half-rotated tensor is real and vbias uses the real-operand (2-MFMA) form.  The
+// exchange energy follows the reference's chunking over Cholesky vectors: one GEMM per chunk
+// builds T[(w,b)][(m,a)] and a fixed-order reduction forms sum_ab T_ab T_ba and the Coulomb
+// traces.  The one-body exponential uses scaling and squaring of a degree-20 Taylor
+// polynomial on the device instead of the reference's host eigendecomposition; both agree to
+// rounding for the Hermitian H1.
+#include <cmath>
+#include <vector>
+
+#include "gemm.hpp"
+#include "ops.hpp"
+
+namespace afq {
+
+using gemm::EpiStore;
+using gemm::EpiStoreReal;
+using gemm::Opnd;
+
+namespace {
+
+template <typename T, class OA, class OB, class EPI>
+void launch_large(hipStream_t s, int batch, int M, int N, int K, const OA& A, const OB& B,
+                  const EPI& e) {
+  gemm::launch<T, 64, 64, 16, 2, 2>(s, batch, M, N, K, A, B, e);
+}
+template <typename T, class OA, class OB, class EPI>
+void launch_small(hipStream_t s, int batch, int M, int N, int K, const OA& A, const OB& B,
+                  const EPI& e) {
+  gemm::launch<T, 32, 32, 16, 1, 1>(s, batch, M, N, K, A, B, e);
+}
+
+template <typename T>
+T* dalloc(size_t count) {
+  void* p = nullptr;
+  if (count == 0) count = 1;
+  if (hipMalloc(&p, count * sizeof(T)) != hipSuccess)
+    throw Error(kErrAlloc, "device allocation of " + std::to_string(count * sizeof(T)) + " bytes failed");
+  return reinterpret_cast<T*>(p);
+}
+
+}  // namespace
+
+CholeskyOps::CholeskyOps(Context& c, int walker_type, int N, int nup, int ndown, int M,
+                         const double* Likn, const double* h1, const z64* PsiT, double E0,
+                         int precision)
+    : ctx_(c), walker_type_(walker_type), N_(N), nup_(nup), ndown_(ndown), M_(M), prec_(precision),
+      E0_(E0) {
+  AFQ_CHECK(walker_type == kClosed || walker_type == kCollinear, kErrArg, "walker_type");
+  AFQ_CHECK(N > 0 && M > 0 && nup >= 0 && ndown >= 0 && nup <= N && ndown <= N, kErrShape,
+            "invalid sizes");
+  AFQ_CHECK(nup <= 64 && ndown <= 64, kErrShape, "electrons per spin must be <= 64");
+  AFQ_CHECK(precision == kFP64 || precision == kMixed, kErrArg, "precision");
+  if (walker_type == kClosed) ndown_ = 0;
+  hipStream_t s = c.stream();
+  const int nel_ = nel();
+  try {
+    PsiT_ = dalloc<z64>((size_t)nel_ * N);
+    Likn_ = dalloc<double>((size_t)N * N * M);
+    h1_ = dalloc<double>((size_t)N * N);
+    vexx_ = dalloc<double>((size_t)N * N);
+    Lnak_ = dalloc<z64>((size_t)M * nel_ * N);
+    haj_ = dalloc<z64>((size_t)nel_ * N);
+    AFQ_HIP(hipMemcpyAsync(PsiT_, PsiT, sizeof(z64) * nel_ * N, hipMemcpyDeviceToDevice, s));
+    AFQ_HIP(hipMemcpyAsync(Likn_, Likn, sizeof(double) * N * N * M, hipMemcpyDeviceToDevice, s));
+    AFQ_HIP(hipMemcpyAsync(h1_, h1, sizeof(double) * N * N, hipMemcpyDeviceToDevice, s));
+
+    // real trial?  (NOMSD.hpp:70-86 makes the same test on the host)
+    std::vector<z64> hp((size_t)nel_ * N);
+    AFQ_HIP(hipMemcpyAsync(hp.data(), PsiT_, sizeof(z64) * hp.size(), hipMemcpyDeviceToHost, s));
+    AFQ_HIP(hipStreamSynchronize(s));
+    real_trial_ = true;
+    for (auto& v : hp)
+      if (v.im != 0.0) { real_trial_ = false; break; }
+
+    // half rotation per spin block: C[a][(k,m)] = sum_i PsiT[a][i] Likn[i][(k,m)], then
+    // permuted to Lnak[m][a][k]
+    z64* tmp = nullptr;
+    int nmax = std::max(nup_, ndown_);
+    if (nmax > 0) tmp = scratch_as<z64>(c, "setup_tmp", (size_t)nmax * N * M);
+    for (int sp = 0; sp < (walker_type_ == kCollinear ? 2 : 1); ++sp) {
+      int ns = sp == 0 ? nup_ : ndown_;
+      if (ns == 0) continue;
+      const z64* Ps = PsiT_ + (sp == 0 ? 0 : (size_t)nup_ * N);
+      z64* Ls = Lnak_ + (sp == 0 ? 0 : (size_t)M * nup_ * N);
+      Opnd<double, true, true> A{&Ps->re, N, 0};
+      Opnd<double, false, false> B{Likn_, (long)N * M, 0};
+      launch_large<double>(s, 1, ns, N * M, N, A, B,
+                           EpiStore<double, double>{tmp, (long)N * M, 0, 1, 0, 0});
+      permute_akm_mak(s, ns, N, M, tmp, Ls);
+    }
+    // haj = PsiT h1   (half_rotate_hamiltonian, one_body.hpp)
+    {
+      Opnd<double, true, true> A{&PsiT_->re, N, 0};
+      Opnd<double, false, false> B{h1_, N, 0};
+      launch_small<double>(s, 1, nel_, N, N, A, B, EpiStore<double, double>{haj_, N, 0, 1, 0, 0});
+    }
+    // vexx[i][l] = -1/2 sum_{(j,m)} Likn[i][(j,m)] Likn[l][(j,m)]
+    {
+      Opnd<double, false, true> A{Likn_, (long)N * M, 0};
+      Opnd<double, false, true> B{Likn_, (long)N * M, 0};
+      launch_large<double>(s, 1, N, N, N * M, A, B, EpiStoreReal<double>{vexx_, N, 0, -0.5});
+    }
+    if (real_trial_) {
+      Lnak_r_ = dalloc<double>((size_t)M * nel_ * N);
+      real_part(s, (long)M * nel_ * N, Lnak_, Lnak_r_);
+    }
+    if (prec_ == kMixed) {
+      Likn32_ = dalloc<float>((size_t)N * N * M);
+      cast_f64_f32(s, (long)N * N * M, Likn_, Likn32_);
+      if (real_trial_) {
+        Lnak_r32_ = dalloc<float>((size_t)M * nel_ * N);
+        real_part_f32(s, (long)M * nel_ * N, Lnak_, Lnak_r32_);
+      } else {
+        Lnak32_ = dalloc<c32>((size_t)M * nel_ * N);
+        cast_z64_c32(s, (long)M * nel_ * N, Lnak_, Lnak32_);
+      }
+    }
+    AFQ_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    free_all();
+    throw;
+  }
+}
+
+void CholeskyOps::free_all() {
+  for (void* p : {(void*)PsiT_, (void*)Likn_, (void*)Likn32_, (void*)h1_, (void*)vexx_,
+                  (void*)Lnak_, (void*)Lnak_r_, (void*)Lnak_r32_, (void*)Lnak32_, (void*)haj_})
+    if (p) (void)hipFree(p);
+  PsiT_ = nullptr;
+  Likn_ = nullptr;
+  Likn32_ = nullptr;
+  h1_ = vexx_ = nullptr;
+  Lnak_ = nullptr;
+  Lnak_r_ = nullptr;
+  Lnak_r32_ = nullptr;
+  Lnak32_ = nullptr;
+  haj_ = nullptr;
+}
+
+CholeskyOps::~CholeskyOps() {
+  (void)hipStreamSynchronize(ctx_.stream());
+  free_all();
+}
+
+void CholeskyOps::vbias(const z64* G, int W, double dt, z64* v, Precision prec) const {
+  if (W == 0) return;
+  hipStream_t s = ctx_.stream();
+  const double a = std::sqrt(dt) * (walker_type_ == kClosed ? 2.0 : 1.0);
+  const int nel_ = nel();
+  // spin blocks: G[w][(off+a)][k] with Lnak block s [M][ns][N]
+  for (int sp = 0; sp < (walker_type_ == kCollinear ? 2 : 1); ++sp) {
+    int ns = sp == 0 ? nup_ : ndown_;
+    int acc = (sp == 1 && nup_ > 0) ? 1 : 0;
+    if (ns == 0) {
+      if (sp == 0) fill_z64(s, (long)W * M_, {0, 0}, v);
+      continue;
+    }
+    size_t goff = sp == 0 ? 0 : (size_t)nup_ * N_;
+    size_t loff = sp == 0 ? 0 : (size_t)M_ * nup_ * N_;
+    const long K = (long)ns * N_;
+    if (prec == kFP64) {
+      // A(w, (a,k)) = G[w][off + a][k]: rows of length nel*N, K = ns*N contiguous
+      Opnd<double, true, true> A{&(G + goff)->re, (long)nel_ * N_, 0};
+      EpiStore<double, double> e{v, M_, 0, a, 0, acc};
+      if (real_trial_) {
+        Opnd<double, false, true> B{Lnak_r_ + loff, K, 0};
+        launch_large<double>(s, 1, W, M_, (int)K, A, B, e);
+      } else {
+        Opnd<double, true, true> B{&(Lnak_ + loff)->re, K, 0};
+        launch_large<double>(s, 1, W, M_, (int)K, A, B, e);
+      }
+    } else {
+      AFQ_CHECK(prec_ == kMixed, kErrState, "FP32 tensors were not built (precision FP64)");
+      Opnd<double, true, true> A{&(G + goff)->re, (long)nel_ * N_, 0};
+      EpiStore<float, double> e{v, M_, 0, (float)a, 0.f, acc};
+      if (real_trial_) {
+        Opnd<float, false, true> B{Lnak_r32_ + loff, K, 0};
+        launch_large<float>(s, 1, W, M_, (int)K, A, B, e);
+      } else {
+        Opnd<float, true, true> B{&(Lnak32_ + loff)->re, K, 0};
+        launch_large<float>(s, 1, W, M_, (int)K, A, B, e);
+      }
+    }
+  }
+}
+
+void CholeskyOps::vHS(const void* X, int W, double dt, void* V, Precision prec) const {
+  if (W == 0) return;
+  hipStream_t s = ctx_.stream();
+  const double a = std::sqrt(dt);
+  const long NN = (long)N_ * N_;
+  if (prec == kFP64) {
+    Opnd<double, true, true> A{reinterpret_cast<const double*>(X), M_, 0};
+    Opnd<double, false, true> B{Likn_, M_, 0};
+    launch_large<double>(s, 1, W, (int)NN, M_, A, B,
+                         EpiStore<double, double>{reinterpret_cast<z64*>(V), NN, 0, a, 0, 0});
+  } else {
+    AFQ_CHECK(prec_ == kMixed, kErrState, "FP32 tensors were not built (precision FP64)");
+    Opnd<float, true, true> A{reinterpret_cast<const float*>(X), M_, 0};
+    Opnd<float, false, true> B{Likn32_, M_, 0};
+    launch_large<float>(s, 1, W, (int)NN, M_, A, B,
+                        EpiStore<float, float>{reinterpret_cast<c32*>(V), NN, 0, (float)a, 0.f, 0});
+  }
+}
+
+void CholeskyOps::energy(const z64* G, int W, z64* E, bool addH1, bool addEJ, bool addEXX,
+                         Precision prec) const {
+  if (W == 0) return;
+  AFQ_CHECK(!addEJ || addEXX, kErrArg, "addEJ and not addEXX not yet implemented");
+  hipStream_t s = ctx_.stream();
+  const int nel_ = nel();
+  const double c = walker_type_ == kClosed ? 2.0 : 1.0;
+  // E = 0; E[:,0] = E0 (energy :156-157)
+  fill_z64(s, (long)W * 3, {0, 0}, E);
+  if (addH1) {
+    // E[w][0] = E0 + c sum G haj over both spin blocks (G rows and haj rows share the order)
+    dot_rows(s, W, (long)nel_ * N_, G, haj_, c, 0.0, E0_, E, 3);
+  }
+  if (!addEXX) return;
+  z64* Kl = scratch_as<z64>(ctx_, "en_Kl", (size_t)W * M_);
+  fill_z64(s, (long)W * M_, {0, 0}, Kl);
+  // chunk of Cholesky vectors: T chunk of W*ns*MC*ns complex within a 1 GiB budget
+  const int nspin = walker_type_ == kCollinear ? 2 : 1;
+  int nmax = std::max(nup_, ndown_);
+  long per_m = (long)W * nmax * nmax * (long)sizeof(z64);
+  int MC = (int)std::max<long>(1, std::min<long>(M_, (1L << 30) / std::max<long>(per_m, 1)));
+  const int nchunk = ceil_div(M_, MC);
+  z64* T = scratch_as<z64>(ctx_, "en_T", (size_t)W * nmax * nmax * MC);
+  z64* part = scratch_as<z64>(ctx_, "en_part", (size_t)W * nchunk * nspin);
+  z64* Gs = nullptr;
+  c32* G32 = nullptr;
+  for (int sp = 0; sp < nspin; ++sp) {
+    int ns = sp == 0 ? nup_ : ndown_;
+    if (ns == 0) {
+      fill_z64(s, (long)W * nchunk, {0, 0}, part + (size_t)sp * W * nchunk);
+      continue;
+    }
+    size_t goff = sp == 0 ? 0 : (size_t)nup_ * N_;
+    const z64* Lsp = Lnak_ + (sp == 0 ? 0 : (size_t)M_ * nup_ * N_);
+    // contiguous spin block of G: [W][ns][N]
+    const z64* Gblk = G;
+    if (nspin == 2) {
+      Gs = scratch_as<z64>(ctx_, "en_Gs", (size_t)W * ns * N_);
+      AFQ_HIP(hipMemcpy2DAsync(Gs, sizeof(z64) * ns * N_, G + goff, sizeof(z64) * nel_ * N_,
+                               sizeof(z64) * ns * N_, W, hipMemcpyDeviceToDevice, s));
+      Gblk = Gs;
+    }
+    for (int ch = 0; ch < nchunk; ++ch) {
+      int m0 = ch * MC;
+      int mc = std::min(MC, M_ - m0);
+      // T[(w,b)][(m,a)] = sum_k G[w][b][k] Lnak[m0+m][a][k]
+      EpiStore<double, double> e{T, (long)MC * ns, 0, 1, 0, 0};
+      if (prec == kFP64) {
+        Opnd<double, true, true> A{&Gblk->re, N_, 0};
+        Opnd<double, true, true> B{&(Lsp + (size_t)m0 * ns * N_)->re, N_, 0};
+        launch_large<double>(s, 1, W * ns, mc * ns, N_, A, B, e);
+      } else {
+        AFQ_CHECK(prec_ == kMixed, kErrState, "FP32 tensors were not built (precision FP64)");
+        if (!G32) G32 = scratch_as<c32>(ctx_, "en_G32", (size_t)W * ns * N_);
+        if (ch == 0) cast_z64_c32(s, (long)W * ns * N_, Gblk, G32);
+        EpiStore<float, double> ef{T, (long)MC * ns, 0, 1.f, 0.f, 0};
+        Opnd<float, true, true> A{&G32->re, N_, 0};
+        if (real_trial_) {
+          Opnd<float, false, true> B{Lnak_r32_ + (sp == 0 ? 0 : (size_t)M_ * nup_ * N_) +
+                                         (size_t)m0 * ns * N_, N_, 0};
+          launch_large<float>(s, 1, W * ns, mc * ns, N_, A, B, ef);
+        } else {
+          Opnd<float, true, true> B{&(Lnak32_ + (sp == 0 ? 0 : (size_t)M_ * nup_ * N_) +
+                                      (size_t)m0 * ns * N_)->re, N_, 0};
+          launch_large<float>(s, 1, W * ns, mc * ns, N_, A, B, ef);
+        }
+      }
+      exchange_reduce(s, W, ns, MC, m0, M_, T, part + (size_t)sp * W * nchunk, nchunk, ch,
+                      addEJ ? Kl : nullptr);
+    }
+    G32 = nullptr;
+  }
+  // E[w][1] = -1/2 c sum exx;  E[w][2] = 1/2 c^2 sum_m Kl^2  (energy :169-173)
+  for (int sp = 0; sp < nspin; ++sp)
+    energy_finish(s, W, M_, nchunk, part + (size_t)sp * W * nchunk, Kl, -0.5 * c, 0.5 * c * c,
+                  1, (addEJ && sp == nspin - 1) ? 1 : 0, E);
+}
+
+void CholeskyOps::vMF(double dt, z64* v) const {
+  hipStream_t s = ctx_.stream();
+  const int nel_ = nel();
+  z64* Psi = scratch_as<z64>(ctx_, "vmf_Psi", (size_t)N_ * std::max(nel_, 1));
+  z64* G = scratch_as<z64>(ctx_, "vmf_G", (size_t)nel_ * N_ + 1);
+  z64* ov = scratch_as<z64>(ctx_, "vmf_ov", 2);
+  fill_z64(s, 1, {0, 0}, ov);
+  for (int sp = 0; sp < (walker_type_ == kCollinear ? 2 : 1); ++sp) {
+    int ns = sp == 0 ? nup_ : ndown_;
+    if (ns == 0) continue;
+    const z64* Ps = PsiT_ + (sp == 0 ? 0 : (size_t)nup_ * N_);
+    conj_transpose(s, ns, N_, Ps, Psi);
+    det_ops::mixed_density_matrix(ctx_, Ps, ns, N_, Psi, (long)N_ * ns, 1,
+                                  G + (sp == 0 ? 0 : (size_t)nup_ * N_), (long)nel_ * N_, ov, true);
+  }
+  vbias(G, 1, dt, v, kFP64);
+  zero_imag(s, M_, v);
+}
+
+void CholeskyOps::one_body_propagator(const z64* vMF, double dt, z64* P1) const {
+  hipStream_t s = ctx_.stream();
+  const int N = N_;
+  const long NN = (long)N * N;
+  z64* shift = scratch_as<z64>(ctx_, "p1_shift", NN);
+  z64* H = scratch_as<z64>(ctx_, "p1_H", NN);
+  z64* E = scratch_as<z64>(ctx_, "p1_E", NN);
+  z64* F = scratch_as<z64>(ctx_, "p1_F", NN);
+  // meanfield shift = vHS(vMF) (includes sqrt(dt)); H1 = dt h + dt vexx + shift
+  vHS(vMF, 1, dt, shift, kFP64);
+  add_one_body(s, N, dt, h1_, vexx_, shift, H);
+  hermitize_scale(s, N, -0.5, H);  // hermitize(H1); H1 *= -1/2
+  // scaling and squaring: ||H||_1 on the host (setup path)
+  std::vector<z64> hH(NN);
+  AFQ_HIP(hipMemcpyAsync(hH.data(), H, sizeof(z64) * NN, hipMemcpyDeviceToHost, s));
+  AFQ_HIP(hipStreamSynchronize(s));
+  double norm1 = 0.0;
+  for (int j = 0; j < N; ++j) {
+    double cs = 0.0;
+    for (int i = 0; i < N; ++i) cs += std::hypot(hH[(size_t)i * N + j].re, hH[(size_t)i * N + j].im);
+    norm1 = std::max(norm1, cs);
+  }
+  int sq = 0;
+  while (norm1 / std::ldexp(1.0, sq) > 0.25) ++sq;
+  const double scl = std::ldexp(1.0, -sq);
+  // Horner: E = I; for k = 20..1: E = I + (B E) / k with B = H / 2^sq
+  identity(s, N, E);
+  for (int k = 20; k >= 1; --k) {
+    identity(s, N, F);
+    Opnd<double, true, true> A{&H->re, N, 0};
+    Opnd<double, true, false> B{&E->re, N, 0};
+    launch_small<double>(s, 1, N, N, N, A, B, EpiStore<double, double>{F, N, 0, scl / k, 0, 1});
+    std::swap(E, F);
+  }
+  for (int i = 0; i < sq; ++i) {
+    Opnd<double, true, true> A{&E->re, N, 0};
+    Opnd<double, true, false> B{&E->re, N, 0};
+    launch_small<double>(s, 1, N, N, N, A, B, EpiStore<double, double>{F, N, 0, 1, 0, 0});
+    std::swap(E, F);
+  }
+  AFQ_HIP(hipMemcpyAsync(P1, E, sizeof(z64) * NN, hipMemcpyDeviceToDevice, s));
+  truncate_small(s, NN, 1e-8, P1);  // to_csr(P1, 1e-8) (csr_utils.hpp:43-66)
+}
+
+}  // namespace afq

@@ -1,0 +1,258 @@
+// Batched small dense kernels: one wavefront per matrix, the matrix held in LDS.
+//
+//  * lu_inverse_logdet: Gauss-Jordan inversion with partial pivoting (pivot = first maximum of
+//    |re| + |im|, the getrf choice), producing the inverse and log det of each n x n complex
+//    matrix.  The pivots are exactly getrf's U diagonal, so the log det follows
+//    log_determinant_from_getrf (determinants_impl.hpp:33-75): sum_i log(+-U_ii + DBL_MIN) with
+//    the sign flipped on swapped rows, added to the running value, imaginary part wrapped into
+//    [-pi, pi].  A singular matrix yields a zero inverse (getri_or_zero,
+//    nda_functions.hpp:422-437).
+//  * chol_upper_inverse: Cholesky S = R^H R (R upper, positive real diagonal) of a Hermitian
+//    positive-definite Gram matrix and R^{-1}; the building block of the Cholesky-QR2
+//    orthogonalization.
+//
+// Lanes own columns (n <= 64).  All LDS traffic of one wavefront is program-ordered, and the
+// explicit barriers separate the phases that read another lane's column.
+#include "afq_internal.hpp"
+
+namespace afq {
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+__device__ inline double2 ld2(const double* p) { return *reinterpret_cast<const double2*>(p); }
+
+__device__ inline double cabs1(double re, double im) { return fabs(re) + fabs(im); }
+
+// log of a complex number (principal branch)
+__device__ inline void clog(double re, double im, double& lr, double& li) {
+  lr = log(hypot(re, im));
+  li = atan2(im, re);
+}
+
+__global__ void __launch_bounds__(64)
+    k_lu_inverse_logdet(const z64* __restrict__ A, long a_bs, int n, z64* __restrict__ Ainv,
+                        long i_bs, z64* __restrict__ ovlp, double ov_scale, double ov_add_re,
+                        double ov_add_im, int want_inverse) {
+  extern __shared__ double smem[];
+  const int ld = n + 1;  // complex elements per LDS row
+  double* a = smem;                 // [n][ld] complex, interleaved
+  double* fcol = smem + 2 * n * ld;  // [n] complex
+  int* piv = reinterpret_cast<int*>(fcol + 2 * n);
+  const int lane = threadIdx.x;
+  const long b = blockIdx.x;
+  const z64* Ab = A + b * a_bs;
+
+  for (int e = lane; e < n * n; e += 64) {
+    int r = e / n, c = e % n;
+    z64 v = Ab[e];
+    a[2 * (r * ld + c)] = v.re;
+    a[2 * (r * ld + c) + 1] = v.im;
+  }
+  __syncthreads();
+
+  double ldr = 0.0, ldi = 0.0;  // log det accumulated on every lane (uniform values)
+  bool singular = false;
+  for (int j = 0; j < n; ++j) {
+    // pivot: first maximum of |re|+|im| over rows j..n-1 of column j
+    double best = -1.0;
+    int p = n;
+    for (int r = j + lane; r < n; r += 64) {
+      double v = cabs1(a[2 * (r * ld + j)], a[2 * (r * ld + j) + 1]);
+      if (v > best) { best = v; p = r; }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      double ob = __shfl_xor(best, off, 64);
+      int op = __shfl_xor(p, off, 64);
+      if (ob > best || (ob == best && op < p)) { best = ob; p = op; }
+    }
+    if (lane == 0) piv[j] = p;
+    // swap rows j and p (lane = column)
+    if (p != j) {
+      for (int c = lane; c < n; c += 64) {
+        double2 x = ld2(a + 2 * (j * ld + c)), y = ld2(a + 2 * (p * ld + c));
+        a[2 * (j * ld + c)] = y.x; a[2 * (j * ld + c) + 1] = y.y;
+        a[2 * (p * ld + c)] = x.x; a[2 * (p * ld + c) + 1] = x.y;
+      }
+    }
+    __syncthreads();
+    double dr = a[2 * (j * ld + j)], di = a[2 * (j * ld + j) + 1];
+    {
+      double sr = (p == j) ? dr : -dr, si = (p == j) ? di : -di;
+      double lr, li;
+      clog(sr + 2.2250738585072014e-308, si, lr, li);
+      ldr += lr;
+      ldi += li;
+    }
+    if (dr == 0.0 && di == 0.0) {
+      singular = true;
+      continue;
+    }
+    // inverse of the pivot
+    double den = dr * dr + di * di;
+    double ir = dr / den, ii = -di / den;
+    __syncthreads();
+    // scale pivot row: a[j][c] *= 1/d (c != j), a[j][j] = 1/d
+    for (int c = lane; c < n; c += 64) {
+      if (c == j) {
+        a[2 * (j * ld + c)] = ir;
+        a[2 * (j * ld + c) + 1] = ii;
+      } else {
+        double xr = a[2 * (j * ld + c)], xi = a[2 * (j * ld + c) + 1];
+        a[2 * (j * ld + c)] = xr * ir - xi * ii;
+        a[2 * (j * ld + c) + 1] = xr * ii + xi * ir;
+      }
+    }
+    // column j multipliers (old values)
+    for (int r = lane; r < n; r += 64) {
+      fcol[2 * r] = a[2 * (r * ld + j)];
+      fcol[2 * r + 1] = a[2 * (r * ld + j) + 1];
+    }
+    __syncthreads();
+    for (int c = lane; c < n; c += 64) {
+      double pr = a[2 * (j * ld + c)], pi = a[2 * (j * ld + c) + 1];  // scaled pivot row
+      for (int r = 0; r < n; ++r) {
+        if (r == j) continue;
+        double fr = fcol[2 * r], fi = fcol[2 * r + 1];
+        if (c == j) {
+          // new a[r][j] = -f / d = -f * (1/d)
+          a[2 * (r * ld + c)] = -(fr * ir - fi * ii);
+          a[2 * (r * ld + c) + 1] = -(fr * ii + fi * ir);
+        } else {
+          a[2 * (r * ld + c)] -= fr * pr - fi * pi;
+          a[2 * (r * ld + c) + 1] -= fr * pi + fi * pr;
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  if (lane == 0 && ovlp != nullptr) {
+    z64 o = ovlp[b];
+    double re = o.re + ldr, im = o.im + ldi;
+    if (im > kPi) {
+      while (im > kPi) im -= 2 * kPi;
+    } else if (im < -kPi) {
+      while (im < -kPi) im += 2 * kPi;
+    }
+    ovlp[b] = {ov_scale * re + ov_add_re, ov_scale * im + ov_add_im};
+  }
+  if (!want_inverse) return;
+  __syncthreads();
+  // undo the row interchanges as column interchanges, last pivot first
+  if (!singular) {
+    for (int j = n - 1; j >= 0; --j) {
+      int p = piv[j];
+      if (p != j) {
+        for (int r = lane; r < n; r += 64) {
+          double2 x = ld2(a + 2 * (r * ld + j)), y = ld2(a + 2 * (r * ld + p));
+          a[2 * (r * ld + j)] = y.x; a[2 * (r * ld + j) + 1] = y.y;
+          a[2 * (r * ld + p)] = x.x; a[2 * (r * ld + p) + 1] = x.y;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  z64* Ib = Ainv + b * i_bs;
+  for (int e = lane; e < n * n; e += 64) {
+    int r = e / n, c = e % n;
+    Ib[e] = singular ? z64{0.0, 0.0} : z64{a[2 * (r * ld + c)], a[2 * (r * ld + c) + 1]};
+  }
+}
+
+// Cholesky of a Hermitian positive-definite S (upper R, S = R^H R) and the inverse of R.
+// Rinv is written as a full n x n matrix (zeros below the diagonal). ldet += sum_i log R_ii.
+__global__ void __launch_bounds__(64)
+    k_chol_upper_inverse(const z64* __restrict__ S, int n, z64* __restrict__ Rinv,
+                         z64* __restrict__ ldet) {
+  extern __shared__ double smem[];
+  const int ld = n + 1;
+  double* a = smem;  // [n][ld] complex
+  const int lane = threadIdx.x;
+  const long b = blockIdx.x;
+  const z64* Sb = S + b * (long)n * n;
+  for (int e = lane; e < n * n; e += 64) {
+    int r = e / n, c = e % n;
+    z64 v = Sb[e];
+    a[2 * (r * ld + c)] = v.re;
+    a[2 * (r * ld + c) + 1] = v.im;
+  }
+  __syncthreads();
+  double lsum = 0.0;
+  // right-looking upper Cholesky: row j of R
+  for (int j = 0; j < n; ++j) {
+    double d = sqrt(a[2 * (j * ld + j)]);
+    lsum += log(d);
+    __syncthreads();
+    for (int c = lane; c < n; c += 64) {
+      if (c == j) {
+        a[2 * (j * ld + c)] = d;
+        a[2 * (j * ld + c) + 1] = 0.0;
+      } else if (c > j) {
+        a[2 * (j * ld + c)] /= d;
+        a[2 * (j * ld + c) + 1] /= d;
+      }
+    }
+    __syncthreads();
+    // trailing update: A[i][c] -= conj(R[j][i]) R[j][c] for j < i <= c
+    for (int c = lane; c < n; c += 64) {
+      if (c <= j) continue;
+      double rcr = a[2 * (j * ld + c)], rci = a[2 * (j * ld + c) + 1];
+      for (int i = j + 1; i <= c; ++i) {
+        double rir = a[2 * (j * ld + i)], rii = -a[2 * (j * ld + i) + 1];  // conj(R[j][i])
+        a[2 * (i * ld + c)] -= rir * rcr - rii * rci;
+        a[2 * (i * ld + c) + 1] -= rir * rci + rii * rcr;
+      }
+    }
+    __syncthreads();
+  }
+  // inverse of the upper triangle, lane = column c: X[c][c] = 1/R[c][c];
+  // X[i][c] = -(sum_{k=i+1..c} R[i][k] X[k][c]) / R[i][i]
+  // (each lane writes and re-reads only its own column of Rinv in global memory)
+  z64* Rb = Rinv + b * (long)n * n;
+  for (int c = lane; c < n; c += 64) {
+    for (int i = c + 1; i < n; ++i) Rb[(long)i * n + c] = {0.0, 0.0};
+    Rb[(long)c * n + c] = {1.0 / a[2 * (c * ld + c)], 0.0};
+    for (int i = c - 1; i >= 0; --i) {
+      double sr = 0.0, si = 0.0;
+      for (int k = i + 1; k <= c; ++k) {
+        double rr = a[2 * (i * ld + k)], ri = a[2 * (i * ld + k) + 1];
+        z64 x = Rb[(long)k * n + c];
+        sr += rr * x.re - ri * x.im;
+        si += rr * x.im + ri * x.re;
+      }
+      double rd = a[2 * (i * ld + i)];
+      Rb[(long)i * n + c] = {-sr / rd, -si / rd};
+    }
+  }
+  if (lane == 0 && ldet != nullptr) {
+    z64 o = ldet[b];
+    ldet[b] = {o.re + lsum, o.im};
+  }
+}
+
+}  // namespace
+
+void lu_inverse_logdet(hipStream_t s, const z64* A, long a_bs, int nb, int n, z64* Ainv,
+                       long i_bs, z64* ovlp, double ov_scale, z64 ov_add, bool want_inverse) {
+  AFQ_CHECK(n >= 1 && n <= 64, kErrShape, "n must be in [1, 64]");
+  if (nb <= 0) return;
+  size_t lds = sizeof(double) * (2 * (size_t)n * (n + 1) + 2 * n) + sizeof(int) * n;
+  AFQ_CHECK(lds <= 65536, kErrShape, "matrix too large for one workgroup's LDS");
+  hipLaunchKernelGGL(k_lu_inverse_logdet, dim3(nb), dim3(64), lds, s, A, a_bs, n, Ainv, i_bs, ovlp,
+                     ov_scale, ov_add.re, ov_add.im, want_inverse ? 1 : 0);
+  AFQ_LAUNCH_CHECK();
+}
+
+void chol_upper_inverse(hipStream_t s, const z64* S, int nb, int n, z64* Rinv, z64* ldet) {
+  AFQ_CHECK(n >= 1 && n <= 64, kErrShape, "n must be in [1, 64]");
+  if (nb <= 0) return;
+  size_t lds = sizeof(double) * 2 * (size_t)n * (n + 1);
+  AFQ_CHECK(lds <= 65536, kErrShape, "matrix too large for one workgroup's LDS");
+  hipLaunchKernelGGL(k_chol_upper_inverse, dim3(nb), dim3(64), lds, s, S, n, Rinv, ldet);
+  AFQ_LAUNCH_CHECK();
+}
+
+}  // namespace afq

@@ -1,0 +1,258 @@
+// Auxiliary fields, walker weights and the growth estimator, all on the device.
+//
+//  * construct_x: assemble_X (AFQMCBasePropagator.cpp:686-733) with construct_X_impl
+//    (construct_X_kernel.hpp:34-86) for continuous charge fields, plus the mean-field factor
+//    (AFQMCBasePropagator.cpp:335-336).  One workgroup per walker; the per-walker sums HW and
+//    sum_m X vMF are block reductions in a fixed order (deterministic, unlike the reference's
+//    atomics).  Uniforms are either injected (parity runs, like the reference's
+//    DEVICE_RNG_FROM_HOST build) or drawn in-kernel from Philox4x32-10, a counter-based
+//    generator keyed by (seed, stream, step, walker, field) so every walker draws independently
+//    of launch shape.
+//  * walker_update: WalkerSetUpdate.hpp:70-191 (hybrid, phaseless) followed by
+//    bound_walker_weights (:199-218), one thread per walker; the reference runs this loop on the
+//    host after staging ten W-vectors across PCIe every step.
+//  * average_pseudo_energy: averagePseudoEnergy (average_energy.hpp:54-63) and the Eshift
+//    relaxation of run_afqmc.cpp:91, so the step never needs a host round trip.
+#include "afq_internal.hpp"
+
+namespace afq {
+
+namespace {
+
+__device__ inline double probit(double p) {
+  // Acklam's rational approximation of the inverse normal CDF (probit.h:58-137)
+  const double a0 = -3.969683028665376e+01, a1 = 2.209460984245205e+02, a2 = -2.759285104469687e+02,
+               a3 = 1.383577518672690e+02, a4 = -3.066479806614716e+01, a5 = 2.506628277459239e+00;
+  const double b0 = -5.447609879822406e+01, b1 = 1.615858368580409e+02, b2 = -1.556989798598866e+02,
+               b3 = 6.680131188771972e+01, b4 = -1.328068155288572e+01;
+  const double c0 = -7.784894002430293e-03, c1 = -3.223964580411365e-01, c2 = -2.400758277161838e+00,
+               c3 = -2.549732539343734e+00, c4 = 4.374664141464968e+00, c5 = 2.938163982698783e+00;
+  const double d0 = 7.784695709041462e-03, d1 = 3.224671290700398e-01, d2 = 2.445134137142996e+00,
+               d3 = 3.754408661907416e+00;
+  if (p < 0 || p > 1) return 0.0;
+  if (p == 0) return -HUGE_VAL;
+  if (p == 1) return HUGE_VAL;
+  if (p < 0.02425) {
+    double q = sqrt(-2 * log(p));
+    return (((((c0 * q + c1) * q + c2) * q + c3) * q + c4) * q + c5) /
+           ((((d0 * q + d1) * q + d2) * q + d3) * q + 1);
+  }
+  if (p > 0.97575) {
+    double q = sqrt(-2 * log(1 - p));
+    return -(((((c0 * q + c1) * q + c2) * q + c3) * q + c4) * q + c5) /
+           ((((d0 * q + d1) * q + d2) * q + d3) * q + 1);
+  }
+  double q = p - 0.5, r = q * q;
+  return (((((a0 * r + a1) * r + a2) * r + a3) * r + a4) * r + a5) * q /
+         (((((b0 * r + b1) * r + b2) * r + b3) * r + b4) * r + 1);
+}
+
+// Philox4x32-10 (Salmon et al., SC'11)
+__device__ inline void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint32_t lo0 = 0xD2511F53u * c[0], hi0 = __umulhi(0xD2511F53u, c[0]);
+    uint32_t lo1 = 0xCD9E8D57u * c[2], hi1 = __umulhi(0xCD9E8D57u, c[2]);
+    uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+__device__ inline double field_uniform(uint64_t seed, uint32_t stream, uint32_t step, uint32_t w,
+                                       uint32_t m) {
+  uint32_t c[4] = {m >> 1, w, step, stream};
+  philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  uint64_t bits = (m & 1u) ? (((uint64_t)c[2] << 32) | c[3]) : (((uint64_t)c[0] << 32) | c[1]);
+  return ((double)(bits >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+template <int NT>
+__device__ inline double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < NT / 64; ++i) t += red[i];
+  return t;  // valid on thread 0
+}
+
+constexpr int kXThreads = 256;
+
+__global__ void __launch_bounds__(kXThreads)
+    k_construct_x(const double* __restrict__ u, const z64* __restrict__ vMF, int M, FieldParams p,
+                  z64* __restrict__ X, z64* __restrict__ HW, z64* __restrict__ mf,
+                  c32* __restrict__ X32) {
+  __shared__ double red[kXThreads / 64];
+  const int w = blockIdx.x;
+  const uint32_t step = p.step_ptr ? *p.step_ptr : p.step;
+  double hwr = 0, hwi = 0, xsr = 0, xsi = 0, vv = 0;
+  for (int m = threadIdx.x; m < M; m += kXThreads) {
+    z64 vm = vMF[m];
+    double am = hypot(vm.re, vm.im);
+    z64 vmt = am > p.bound ? z64{vm.re / am * p.bound, vm.im / am * p.bound} : vm;
+    long idx = (long)w * M + m;
+    z64 x = X[idx];
+    double ax = hypot(x.re, x.im);
+    z64 vbt = ax > p.bound ? z64{x.re / ax * p.bound, x.im / ax * p.bound} : x;
+    if (p.project_vbias) vbt.im = 0.0;
+    // vdiff = i (vb_t - vmf_t)
+    z64 vd = p.free_projection ? z64{0.0, 0.0} : z64{-(vbt.im - vmt.im), vbt.re - vmt.re};
+    double uu = u ? u[idx] : field_uniform(p.seed, p.stream, step, (uint32_t)w, (uint32_t)m);
+    z64 xn = {probit(uu) + vd.re, vd.im};
+    X[idx] = xn;
+    if (X32) X32[idx] = {(float)xn.re, (float)xn.im};
+    // HW += -vdiff * (X - vdiff/2)
+    z64 t = {xn.re - 0.5 * vd.re, xn.im - 0.5 * vd.im};
+    hwr -= vd.re * t.re - vd.im * t.im;
+    hwi -= vd.re * t.im + vd.im * t.re;
+    // sum_m X vMF
+    xsr += xn.re * vm.re - xn.im * vm.im;
+    xsi += xn.re * vm.im + xn.im * vm.re;
+    vv += vm.re * vm.re - vm.im * vm.im;  // Re(vMF^2)
+  }
+  hwr = block_sum<kXThreads>(hwr, red);
+  hwi = block_sum<kXThreads>(hwi, red);
+  xsr = block_sum<kXThreads>(xsr, red);
+  xsi = block_sum<kXThreads>(xsi, red);
+  vv = block_sum<kXThreads>(vv, red);
+  if (threadIdx.x == 0) {
+    HW[w] = {hwr, hwi};
+    // mf = i * sum X vMF - 0.5 Re(sum vMF^2)
+    mf[w] = {-xsi - 0.5 * vv, xsr};
+  }
+}
+
+__global__ void k_field_uniforms(uint64_t seed, uint32_t stream, uint32_t step, int W, int M,
+                                 double* u) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)W * M) return;
+  int w = (int)(i / M), m = (int)(i % M);
+  u[i] = field_uniform(seed, stream, step, w, m);
+}
+
+__device__ inline z64 cexp(z64 z) {
+  double e = exp(z.re);
+  return {e * cos(z.im), e * sin(z.im)};
+}
+
+__global__ void k_walker_update(int W, UpdateParams p, const double* __restrict__ Eshift_p,
+                                const z64* __restrict__ nov, const z64* __restrict__ mf,
+                                const z64* __restrict__ HW, z64* __restrict__ weight,
+                                z64* __restrict__ phase, z64* __restrict__ eloc_p,
+                                z64* __restrict__ ovlp, z64* __restrict__ theta) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W) return;
+  const double Eshift = *Eshift_p;
+  const double hi = Eshift + p.upper_cut * sqrt(2.0 / p.dt);
+  const double lo = Eshift - p.lower_cut * sqrt(2.0 / p.dt);
+  z64 old_ov = ovlp[i], old_e = eloc_p[i], nv = nov[i], m = mf[i], hw = HW[i];
+  double scale = 1.0, dth = 0.0;
+  if (!p.free_projection) {
+    dth = nv.im - old_ov.im - m.im;
+    scale = p.use_cp ? (cos(dth) > 0.0 ? 1.0 : 0.0) : fmax(0.0, cos(dth));
+  }
+  theta[i] = {dth, 0.0};
+  // hybrid local energy: (mf - HW - (ov_new - ov_old)) / dt + E0
+  z64 e = {(m.re - hw.re - (nv.re - old_ov.re)) / p.dt + p.E0,
+           (m.im - hw.im - (nv.im - old_ov.im)) / p.dt};
+  z64 w = weight[i], ph = phase[i];
+  if (!isfinite(e.re)) {
+    w = {0.0, 0.0};
+    ph = {0.0, 0.0};
+    e = old_e;
+  } else {
+    if (!p.free_projection) e.re = fmin(fmax(e.re, lo), hi);
+    z64 wf = cexp({-p.dt * (e.re - Eshift), -p.dt * e.im});
+    double aw = hypot(wf.re, wf.im);
+    if (p.free_projection) {
+      w = cmul(w, z64{scale * wf.re, scale * wf.im});
+    } else {
+      w = {w.re * scale * aw, w.im * scale * aw};
+    }
+    ph = cmul(ph, z64{wf.re / aw, wf.im / aw});
+  }
+  // bound_walker_weights
+  double maxw = fmax(p.weight_floor, p.weight_frac * p.target_pop);
+  double a = hypot(w.re, w.im);
+  if (a > maxw) w = {w.re * maxw / a, w.im * maxw / a};
+  weight[i] = w;
+  phase[i] = ph;
+  eloc_p[i] = e;
+  ovlp[i] = nv;
+}
+
+constexpr int kAvgThreads = 1024;
+
+__global__ void __launch_bounds__(kAvgThreads)
+    k_average_pseudo_energy(int W, const z64* __restrict__ weight, const z64* __restrict__ eloc,
+                            double dt, double relax, double* out, double* Eshift) {
+  __shared__ double red[kAvgThreads / 64];
+  double s0r = 0, s0i = 0, s1r = 0, s1i = 0;
+  for (int i = threadIdx.x; i < W; i += kAvgThreads) {
+    z64 w = weight[i], e = eloc[i];
+    z64 x = cmul(w, cexp({dt * e.re, dt * e.im}));
+    s0r += x.re; s0i += x.im; s1r += w.re; s1i += w.im;
+  }
+  s0r = block_sum<kAvgThreads>(s0r, red);
+  s0i = block_sum<kAvgThreads>(s0i, red);
+  s1r = block_sum<kAvgThreads>(s1r, red);
+  s1i = block_sum<kAvgThreads>(s1i, red);
+  if (threadIdx.x == 0) {
+    // (s0 / s1).real
+    double den = s1r * s1r + s1i * s1i;
+    double q = (s0r * s1r + s0i * s1i) / den;
+    double e = log(q) / dt;
+    if (out) *out = e;
+    if (relax > 0.0 && Eshift) *Eshift += relax * (e - *Eshift);
+  }
+}
+
+__global__ void k_increment(uint32_t* c) {
+  if (threadIdx.x == 0) *c += 1u;
+}
+
+}  // namespace
+
+void increment_counter(hipStream_t s, uint32_t* c) {
+  hipLaunchKernelGGL(k_increment, dim3(1), dim3(64), 0, s, c);
+  AFQ_LAUNCH_CHECK();
+}
+
+void construct_x(hipStream_t s, const double* u, const z64* vMF, int W, int M,
+                 const FieldParams& p, z64* X, z64* HW, z64* mf, c32* X32) {
+  if (W <= 0) return;
+  hipLaunchKernelGGL(k_construct_x, dim3(W), dim3(kXThreads), 0, s, u, vMF, M, p, X, HW, mf, X32);
+  AFQ_LAUNCH_CHECK();
+}
+
+void field_uniforms(hipStream_t s, uint64_t seed, uint32_t stream, uint32_t step, int W, int M,
+                    double* u) {
+  long n = (long)W * M;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_field_uniforms, dim3(ceil_div(n, 256)), dim3(256), 0, s, seed, stream, step,
+                     W, M, u);
+  AFQ_LAUNCH_CHECK();
+}
+
+void walker_update(hipStream_t s, int W, const UpdateParams& p, const double* Eshift,
+                   const z64* new_ovlp, const z64* mf, const z64* HW, z64* weight, z64* phase,
+                   z64* eloc, z64* ovlp, z64* theta) {
+  if (W <= 0) return;
+  hipLaunchKernelGGL(k_walker_update, dim3(ceil_div(W, 256)), dim3(256), 0, s, W, p, Eshift,
+                     new_ovlp, mf, HW, weight, phase, eloc, ovlp, theta);
+  AFQ_LAUNCH_CHECK();
+}
+
+void average_pseudo_energy(hipStream_t s, int W, const z64* weight, const z64* eloc, double dt,
+                           double relax, double* out, double* Eshift) {
+  hipLaunchKernelGGL(k_average_pseudo_energy, dim3(1), dim3(kAvgThreads), 0, s, W, weight, eloc,
+                     dt, relax, out, Eshift);
+  AFQ_LAUNCH_CHECK();
+}
+
+}  // namespace afq

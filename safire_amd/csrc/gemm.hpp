@@ -1,0 +1,286 @@
+// Batched complex GEMM on CDNA4 matrix cores (gfx950).
+//
+//   C[b](m, n) = sum_k A[b](m, k) * Bt[b](n, k)         (both operands addressed as [rows][K])
+//
+// Complex products are issued as real MFMA blocks on split re/im planes: 4 MFMAs per complex x
+// complex block (Cr += Ar Br - Ai Bi, Ci += Ar Bi + Ai Br), 2 when one operand is real.  FP64
+// uses v_mfma_f64_16x16x4f64, FP32 uses v_mfma_f32_16x16x4f32; both produce a 16x16 block from
+// a 16x4 A fragment and a 4x16 B fragment with the lane layout
+//   A: lane l holds A[l % 16][l / 16]      B: lane l holds B[l / 16][l % 16]
+// and the accumulator register r (0..3) of lane l holding D[row][l % 16] with
+//   FP32: row = 4 * (l / 16) + r           FP64: row = 4 * r + l / 16
+// (measured on gfx950 with a one-hot lane probe of both builtins).
+//
+// A workgroup computes a BM x BN tile with WAVES_M x WAVES_N wavefronts; operands are staged
+// through LDS in k-major planes ([BK][BM + pad]) so the fragment reads of a 16-lane group are
+// consecutive words.  Global loads of the next k-slab are issued into registers before the
+// current slab is consumed (one LDS buffer, register double-buffering).  Operand element type
+// (TIn) may differ from the compute type T: FP64 walker matrices feed the FP32 path with the
+// conversion done while staging.  Edges are zero-filled, so any M, N, K is valid.
+#pragma once
+
+#include "afq_common.hpp"
+
+namespace afq {
+namespace gemm {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Acc;
+template <>
+struct Acc<double> {
+  using v4 = f64x4;
+  static __device__ inline v4 mfma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // row of accumulator register r in lane group g = lane / 16
+  static __device__ inline int row(int g, int r) { return 4 * r + g; }
+};
+template <>
+struct Acc<float> {
+  using v4 = f32x4;
+  static __device__ inline v4 mfma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int row(int g, int r) { return 4 * g + r; }
+};
+
+// Operand viewed as [rows][K]: element (r, k) of batch b lives at
+//   b * bs + (KCONTIG ? r * ld + k : k * ld + r)        (units of scalar or complex elements)
+// CONJ conjugates complex elements on load.
+template <typename TIn, bool CPLX, bool KCONTIG, bool CONJ = false>
+struct Opnd {
+  using in_t = TIn;
+  static constexpr bool is_cplx = CPLX;
+  static constexpr bool kcontig = KCONTIG;
+  static constexpr bool conj = CONJ;
+  const TIn* p;
+  long ld;
+  long bs;
+};
+
+// ---- epilogues: called once per output element with the accumulated (re, im) ----------------
+
+// C = alpha * acc (+ C when accumulate)
+template <typename T, typename TOut>
+struct EpiStore {
+  cplx<TOut>* c;
+  long ldc, bs;
+  T ar, ai;
+  int accumulate;
+  __device__ inline void operator()(int b, int m, int n, T re, T im) const {
+    long idx = (long)b * bs + (long)m * ldc + n;
+    T r = ar * re - ai * im, i = ar * im + ai * re;
+    cplx<TOut> out;
+    if (accumulate) {
+      cplx<TOut> o = c[idx];
+      out = {(TOut)(r + (T)o.re), (TOut)(i + (T)o.im)};
+    } else {
+      out = {(TOut)r, (TOut)i};
+    }
+    c[idx] = out;
+  }
+};
+
+// real output: C = alpha * Re(acc)   (real x real products)
+template <typename T>
+struct EpiStoreReal {
+  double* c;
+  long ldc, bs;
+  T alpha;
+  __device__ inline void operator()(int b, int m, int n, T re, T) const {
+    c[(long)b * bs + (long)m * ldc + n] = (double)(alpha * re);
+  }
+};
+
+// C = alpha * acc and D = alpha * acc (two copies: the first Taylor operand and the running sum)
+template <typename T>
+struct EpiStore2 {
+  cplx<T>* c;
+  cplx<T>* d;
+  long ldc, bs;
+  __device__ inline void operator()(int b, int m, int n, T re, T im) const {
+    long idx = (long)b * bs + (long)m * ldc + n;
+    c[idx] = {re, im};
+    d[idx] = {re, im};
+  }
+};
+
+// Taylor term: T_k = f * acc written to C, and S += T_k  (apply_expM, propagate.hpp:60-66)
+template <typename T>
+struct EpiTaylor {
+  cplx<T>* c;
+  cplx<T>* s;
+  long ldc, bs;
+  T fr, fi;
+  __device__ inline void operator()(int b, int m, int n, T re, T im) const {
+    long idx = (long)b * bs + (long)m * ldc + n;
+    T r = fr * re - fi * im, i = fr * im + fi * re;
+    c[idx] = {r, i};
+    cplx<T> o = s[idx];
+    s[idx] = {o.re + r, o.im + i};
+  }
+};
+
+// ---- operand staging -------------------------------------------------------------------------
+
+template <typename T, class OP, int ROWS, int BK, int NT, int LDP>
+struct Stager {
+  static constexpr int E = ROWS * BK / NT;
+  static_assert((ROWS * BK) % NT == 0, "tile must divide the thread count");
+  using TIn = typename OP::in_t;
+  TIn re[E];
+  TIn im[OP::is_cplx ? E : 1];
+
+  __device__ inline void load(const OP& op, int b, int r0, int k0, int R, int K, int tid) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      int idx = tid + e * NT;
+      int r, k;
+      if (OP::kcontig) {
+        r = idx / BK;
+        k = idx % BK;
+      } else {
+        r = idx % ROWS;
+        k = idx / ROWS;
+      }
+      int gr = r0 + r, gk = k0 + k;
+      bool in = (gr < R) && (gk < K);
+      long a = (long)b * op.bs + (OP::kcontig ? (long)gr * op.ld + gk : (long)gk * op.ld + gr);
+      if (OP::is_cplx) {
+        cplx<TIn> v = in ? reinterpret_cast<const cplx<TIn>*>(op.p)[a] : cplx<TIn>{0, 0};
+        re[e] = v.re;
+        im[e] = OP::conj ? -v.im : v.im;
+      } else {
+        re[e] = in ? op.p[a] : TIn(0);
+      }
+    }
+  }
+  __device__ inline void store(T* sre, T* sim, int tid) const {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      int idx = tid + e * NT;
+      int r, k;
+      if (OP::kcontig) {
+        r = idx / BK;
+        k = idx % BK;
+      } else {
+        r = idx % ROWS;
+        k = idx / ROWS;
+      }
+      sre[k * LDP + r] = (T)re[e];
+      if (OP::is_cplx) sim[k * LDP + r] = (T)im[e];
+    }
+  }
+};
+
+template <typename T, int BM, int BN, int BK, int WAVES_M, int WAVES_N, class OA, class OB, class EPI>
+__global__ void __launch_bounds__(WAVES_M * WAVES_N * 64)
+    kgemm(int M, int N, int K, int tiles_m, int tiles_n, OA A, OB B, EPI epi) {
+  constexpr int NT = WAVES_M * WAVES_N * 64;
+  constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
+  constexpr int RB = WM / 16, CB = WN / 16;
+  static_assert(WM % 16 == 0 && WN % 16 == 0 && BK % 4 == 0, "bad tile");
+  constexpr int LDA = BM + 4, LDB = BN + 4;
+  constexpr bool CA = OA::is_cplx, CBX = OB::is_cplx;
+  using V4 = typename Acc<T>::v4;
+
+  __shared__ T sAr[BK * LDA];
+  __shared__ T sAi[CA ? BK * LDA : 1];
+  __shared__ T sBr[BK * LDB];
+  __shared__ T sBi[CBX ? BK * LDB : 1];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm0 = (wave / WAVES_N) * WM, wn0 = (wave % WAVES_N) * WN;
+  long bid = blockIdx.x;
+  const int tn = bid % tiles_n;
+  const int tm = (bid / tiles_n) % tiles_m;
+  const int b = (int)(bid / ((long)tiles_n * tiles_m));
+  const int m0 = tm * BM, n0 = tn * BN;
+
+  V4 accr[RB][CB], acci[RB][CB];
+#pragma unroll
+  for (int i = 0; i < RB; ++i)
+#pragma unroll
+    for (int j = 0; j < CB; ++j) {
+      accr[i][j] = V4{0, 0, 0, 0};
+      acci[i][j] = V4{0, 0, 0, 0};
+    }
+
+  Stager<T, OA, BM, BK, NT, LDA> ga;
+  Stager<T, OB, BN, BK, NT, LDB> gb;
+  ga.load(A, b, m0, 0, M, K, tid);
+  gb.load(B, b, n0, 0, N, K, tid);
+
+  const int fr = lane & 15, fk = lane >> 4;
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    __syncthreads();
+    ga.store(sAr, sAi, tid);
+    gb.store(sBr, sBi, tid);
+    __syncthreads();
+    if (k0 + BK < K) {
+      ga.load(A, b, m0, k0 + BK, M, K, tid);
+      gb.load(B, b, n0, k0 + BK, N, K, tid);
+    }
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const int kl = kk + fk;
+      T ar[RB], ai[RB], br[CB], bi[CB];
+#pragma unroll
+      for (int i = 0; i < RB; ++i) {
+        ar[i] = sAr[kl * LDA + wm0 + i * 16 + fr];
+        if (CA) ai[i] = sAi[kl * LDA + wm0 + i * 16 + fr];
+      }
+#pragma unroll
+      for (int j = 0; j < CB; ++j) {
+        br[j] = sBr[kl * LDB + wn0 + j * 16 + fr];
+        if (CBX) bi[j] = sBi[kl * LDB + wn0 + j * 16 + fr];
+      }
+#pragma unroll
+      for (int i = 0; i < RB; ++i)
+#pragma unroll
+        for (int j = 0; j < CB; ++j) {
+          accr[i][j] = Acc<T>::mfma(ar[i], br[j], accr[i][j]);
+          if (CA && CBX) {
+            accr[i][j] = Acc<T>::mfma(-ai[i], bi[j], accr[i][j]);
+            acci[i][j] = Acc<T>::mfma(ar[i], bi[j], acci[i][j]);
+            acci[i][j] = Acc<T>::mfma(ai[i], br[j], acci[i][j]);
+          } else if (CA) {
+            acci[i][j] = Acc<T>::mfma(ai[i], br[j], acci[i][j]);
+          } else if (CBX) {
+            acci[i][j] = Acc<T>::mfma(ar[i], bi[j], acci[i][j]);
+          }
+        }
+    }
+  }
+
+#pragma unroll
+  for (int i = 0; i < RB; ++i)
+#pragma unroll
+    for (int j = 0; j < CB; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int m = m0 + wm0 + i * 16 + Acc<T>::row(fk, r);
+        int n = n0 + wn0 + j * 16 + fr;
+        if (m < M && n < N) epi(b, m, n, accr[i][j][r], acci[i][j][r]);
+      }
+}
+
+// Host launcher.  batch = number of independent products (grid = batch * tiles).
+template <typename T, int BM, int BN, int BK, int WAVES_M, int WAVES_N, class OA, class OB, class EPI>
+inline void launch(hipStream_t s, int batch, int M, int N, int K, const OA& A, const OB& B,
+                   const EPI& epi) {
+  if (batch <= 0 || M <= 0 || N <= 0) return;
+  int tm = ceil_div(M, BM), tn = ceil_div(N, BN);
+  long blocks = (long)batch * tm * tn;
+  AFQ_CHECK(blocks < (1L << 31), kErrShape, "grid too large");
+  hipLaunchKernelGGL((kgemm<T, BM, BN, BK, WAVES_M, WAVES_N, OA, OB, EPI>), dim3((unsigned)blocks),
+                     dim3(WAVES_M * WAVES_N * 64), 0, s, M, N, K, tm, tn, A, B, epi);
+  AFQ_LAUNCH_CHECK();
+}
+
+}  // namespace gemm
+}  // namespace afq

@@ -1,0 +1,221 @@
+// Host-side C++ classes that mirror SAFIRE's interfaces for the walker step:
+//   CholeskyOps  ~ Real3IndexFactorization (HamiltonianOperations alternative):
+//                  vbias, vHS, energy, getOneBodyPropagatorMatrix, vMF
+//   det_ops::*   ~ det_ops::{MixedDensityMatrix, Log_Overlap, Propagate, orthogonalize}
+//   Propagator   ~ AFQMCBasePropagator::{Propagate, Orthogonalize} for a CLOSED/COLLINEAR
+//                  single-determinant trial, with the whole step on the device.
+// All pointers are device pointers; complex data is interleaved (re, im), row-major.
+#pragma once
+
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "afq_internal.hpp"
+
+namespace afq {
+
+enum WalkerType : int { kClosed = 0, kCollinear = 1 };
+enum Precision : int { kFP64 = 0, kMixed = 1 };
+
+// Stream-ordered context: one device, one stream, named scratch buffers that grow on demand.
+class Context {
+ public:
+  Context(int device, hipStream_t stream);
+  ~Context();
+  int device() const { return device_; }
+  hipStream_t stream() const { return stream_; }
+  // Scratch buffer `name` of at least `bytes`; contents are not preserved across growth.
+  void* scratch(const std::string& name, size_t bytes);
+  size_t scratch_bytes() const;
+  bool capturing() const;
+
+ private:
+  int device_;
+  hipStream_t stream_;
+  bool own_stream_;
+  struct Buf {
+    void* p = nullptr;
+    size_t bytes = 0;
+  };
+  std::map<std::string, Buf> bufs_;
+};
+
+template <typename T>
+T* scratch_as(Context& c, const std::string& name, size_t count) {
+  return reinterpret_cast<T*>(c.scratch(name, count * sizeof(T)));
+}
+
+// ---------------------------------------------------------------------------------------------
+// Slater-determinant operations (density_matrix.hpp, propagate.hpp, orthogonalize.hpp)
+// ---------------------------------------------------------------------------------------------
+namespace det_ops {
+
+// T = PsiT * Phi_w; ovlp[w] = scale * wrap(ovlp[w] + log det T) + add; optionally T^{-1} in Tinv.
+void log_overlap(Context& c, const z64* PsiT, int n, int N, const z64* Phi, long phi_bs, int W,
+                 z64* ovlp, double scale = 1.0, z64 add = {0, 0});
+// MixedDensityMatrix: compact G [W][n][N] or full G [W][N][N]; ovlp accumulated as in
+// log_overlap.
+void mixed_density_matrix(Context& c, const z64* PsiT, int n, int N, const z64* Phi, long phi_bs,
+                          int W, z64* G, long g_bs, z64* ovlp, bool compact, double scale = 1.0);
+// inverse_logdet (density_matrix.hpp:82-111)
+void inverse_logdet(Context& c, const z64* A, int nb, int n, z64* ovlp, z64* Ainv);
+// Propagate (propagate.hpp:241-298): Phi <- P1 expM(V) P1 Phi, order-K Taylor series.
+// P1 [N][N] (real when P1r != nullptr), V [W][N][N] in the compute precision.
+struct PropagateArgs {
+  int W, N, n, order;
+  char TA;                 // 'N' or 'H'
+  const z64* P1;           // complex P1 (used when P1r == nullptr)
+  const double* P1r;       // real P1
+  const float* P1r32;      // real P1, FP32 copy (mixed)
+  const c32* P1c32;        // complex P1, FP32 copy (mixed)
+  const void* V;           // z64 or c32 [W][N][N]
+  long v_bs;
+  z64* Phi;                // [W][N][n] (FP64 master copy in both precisions)
+  long phi_bs;
+  Precision prec;
+};
+void propagate(Context& c, const PropagateArgs& a);
+// orthogonalize (orthogonalize.hpp:131-169): Phi <- Q with Phi = Q R, R upper triangular with a
+// positive real diagonal (Cholesky-QR2); ldet += log det R.
+void orthogonalize(Context& c, z64* Phi, long phi_bs, int W, int N, int n, z64* ldet);
+
+}  // namespace det_ops
+
+// ---------------------------------------------------------------------------------------------
+// Cholesky (real 3-index) hamiltonian operations
+// ---------------------------------------------------------------------------------------------
+class CholeskyOps {
+ public:
+  // Builds the half-rotated tensors on the device from the bare h1, Likn and the trial
+  // (RealDenseHamiltonian::getHamiltonianOperations, RealDenseHamiltonian.cpp:52-186).
+  // Likn [N][N][M] real, h1 [N][N] real, PsiT [nup+ndown][N] complex (alpha rows then beta).
+  CholeskyOps(Context& c, int walker_type, int N, int nup, int ndown, int M, const double* Likn,
+              const double* h1, const z64* PsiT, double E0, int precision);
+  ~CholeskyOps();
+
+  int N() const { return N_; }
+  int M() const { return M_; }
+  int nup() const { return nup_; }
+  int ndown() const { return ndown_; }
+  int nel() const { return walker_type_ == kCollinear ? nup_ + ndown_ : nup_; }
+  int walker_type() const { return walker_type_; }
+  int precision() const { return prec_; }
+  double E0() const { return E0_; }
+  const z64* PsiT() const { return PsiT_; }
+  const z64* Lnak() const { return Lnak_; }
+  const z64* haj() const { return haj_; }
+  const double* vexx() const { return vexx_; }
+  bool real_trial() const { return real_trial_; }
+
+  // v[w][m] = sqrt(dt) c sum_{a,k} G[w][a][k] Lnak[m][a][k]   (vbias, :328-428)
+  // prec == kMixed runs the contraction in FP32 and writes FP64 v.
+  void vbias(const z64* G, int W, double dt, z64* v, Precision prec) const;
+  // V[w][i][k] = sqrt(dt) sum_m L[i][k][m] X[w][m]   (vHS, :282-326); V is z64 (FP64) or c32
+  void vHS(const void* X, int W, double dt, void* V, Precision prec) const;
+  // E[w][0..2] = E1, EXX, EJ  (energy :138-174, energy_impl :476-590); G compact [W][nel][N]
+  void energy(const z64* G, int W, z64* E, bool addH1, bool addEJ, bool addEXX,
+              Precision prec) const;
+  // vMF (NOMSD.cpp:437-464) with the charge-decomposition zero_imag (AFQMCBasePropagator.cpp:105)
+  void vMF(double dt, z64* v) const;
+  // P1 = exp(-1/2 hermitize(dt (h + vexx) + vHS(vMF))) with |x| <= 1e-8 dropped
+  // (generateP1, AFQMCBasePropagator.cpp:43-214). P1 [N][N] complex.
+  void one_body_propagator(const z64* vMF, double dt, z64* P1) const;
+
+  Context& ctx() const { return ctx_; }
+
+ private:
+  Context& ctx_;
+  int walker_type_, N_, nup_, ndown_, M_, prec_;
+  double E0_;
+  bool real_trial_ = false;
+  z64* PsiT_ = nullptr;   // [nel][N]
+  double* Likn_ = nullptr;  // [N][N][M]
+  float* Likn32_ = nullptr;
+  double* h1_ = nullptr;    // [N][N]
+  double* vexx_ = nullptr;  // [N][N]
+  z64* Lnak_ = nullptr;     // [M][nel][N]: alpha block [M][nup][N] then beta [M][ndown][N]
+  double* Lnak_r_ = nullptr;  // real part when the trial is real
+  float* Lnak_r32_ = nullptr;
+  c32* Lnak32_ = nullptr;
+  z64* haj_ = nullptr;      // [nel][N]
+  void free_all();
+};
+
+// ---------------------------------------------------------------------------------------------
+// Propagator: one hybrid step of the whole walker set on the device
+// ---------------------------------------------------------------------------------------------
+struct PropagatorParams {
+  double dt = 0.005;
+  int order = 6;
+  double vbias_bound = 50.0;
+  double lower_cut = 1.0, upper_cut = 10.0;
+  double weight_floor = 100.0, weight_frac = 0.1;
+  double target_pop = 0.0;  // 0 -> W
+  int free_projection = 0, use_cp = 0, project_force_bias = 0;
+  uint64_t seed = 0;
+  uint32_t stream = 1;
+};
+
+struct WalkerState {
+  z64* Phi;      // [W][N][nel]: CLOSED [W][N][nup]; COLLINEAR alpha [W][N][nup] then beta
+                 //              [W][N][ndown] in separate blocks (Phi_b)
+  z64* Phi_b;    // beta block (COLLINEAR) or nullptr
+  z64* weight;
+  z64* phase;
+  z64* eloc;     // PSEUDO_ELOC_
+  z64* ovlp;     // OVLP (log overlap)
+  z64* theta;
+  int W;
+};
+
+class Propagator {
+ public:
+  Propagator(CholeskyOps& ham, int W, const PropagatorParams& p);
+  ~Propagator();
+  const z64* vMF() const { return vMF_; }
+  const z64* P1() const { return P1_; }
+  // One step (AFQMCBasePropagator::Propagate, AFQMCBasePropagator.cpp:289-392). Uniforms u
+  // [W][M] injected when given, else drawn from Philox with the step counter held on the device.
+  void step(const WalkerState& s, const double* u, const double* Eshift);
+  // Orthogonalize (:735-741): QR of every walker then Log_Overlap refresh of OVLP.
+  void orthogonalize(const WalkerState& s);
+  // Log_Overlap of the whole set into ovlp (NOMSD.cpp:280-333)
+  void log_overlap(const WalkerState& s, z64* ovlp);
+  // Energy of the set (NOMSD::Energy) into E [W][3] and ovlp [W]
+  void energy(const WalkerState& s, z64* E, z64* ovlp, Precision prec);
+  // Device step counter used by in-kernel RNG
+  uint32_t* step_counter() { return step_dev_; }
+  void set_step(uint32_t step);
+  // capture one step (+ optional orthogonalization) in a HIP graph and replay it
+  void step_graph(const WalkerState& s, const double* Eshift, bool ortho, double relax,
+                  double* energy_out);
+  Precision precision() const { return prec_; }
+
+ private:
+  CholeskyOps& ham_;
+  Context& ctx_;
+  int W_;
+  PropagatorParams p_;
+  Precision prec_;
+  z64* vMF_ = nullptr;
+  z64* P1_ = nullptr;
+  double* P1r_ = nullptr;
+  float* P1r32_ = nullptr;
+  c32* P1c32_ = nullptr;
+  bool real_P1_ = false;
+  uint32_t* step_dev_ = nullptr;
+  // graph cache, one executable graph per (walker buffers, ortho, relaxation) key
+  struct GraphEntry {
+    std::vector<const void*> key;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphEntry> graphs_;
+  bool warmed_ = false;
+  void step_impl(const WalkerState& s, const double* u, const double* Eshift);
+};
+
+}  // namespace afq

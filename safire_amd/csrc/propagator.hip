@@ -1,0 +1,229 @@
+// The walker step on the device: AFQMCBasePropagator::Propagate (AFQMCBasePropagator.cpp:289-392)
+// and Orthogonalize (:735-741) for a single-determinant trial (NOMSD, CLOSED or COLLINEAR).
+//
+//   G, log ovlp    MixedDensityMatrix         (NOMSD::vbias, NOMSD.cpp:39-61)
+//   v              vbias                      (Real3IndexFactorization.hpp:328-428)
+//   X, HW, mf      construct_X + mean field   (AFQMCBasePropagator.cpp:324-336, 686-733)
+//   V              vHS                        (:349)
+//   Phi            P1 expM(V) P1 Phi          (apply_propagators, :354)
+//   new ovlp       Log_Overlap                (:371)
+//   weights        walker_update + bound      (:380-386, WalkerSetUpdate.hpp)
+//
+// Nothing crosses to the host inside a step, so a step (with or without the periodic QR and
+// the Eshift relaxation) is captured once in a HIP graph and replayed.  In the mixed precision
+// mode vbias, vHS and the Taylor propagation run in FP32 on the matrix cores; G, overlaps, the
+// LU/QR, fields and weights stay FP64, and G is recomputed in FP64 from the FP64 walker matrices
+// after every propagation.
+#include <vector>
+
+#include "ops.hpp"
+
+namespace afq {
+
+namespace {
+
+template <typename T>
+T* dalloc(size_t count) {
+  void* p = nullptr;
+  if (hipMalloc(&p, (count ? count : 1) * sizeof(T)) != hipSuccess)
+    throw Error(kErrAlloc, "device allocation failed");
+  return reinterpret_cast<T*>(p);
+}
+
+}  // namespace
+
+Propagator::Propagator(CholeskyOps& ham, int W, const PropagatorParams& p)
+    : ham_(ham), ctx_(ham.ctx()), W_(W), p_(p), prec_((Precision)ham.precision()) {
+  AFQ_CHECK(W > 0, kErrArg, "W must be positive");
+  AFQ_CHECK(p.order >= 0, kErrArg, "order");
+  if (p_.target_pop <= 0) p_.target_pop = W;
+  hipStream_t s = ctx_.stream();
+  const int N = ham.N(), M = ham.M();
+  try {
+    vMF_ = dalloc<z64>(M);
+    P1_ = dalloc<z64>((size_t)N * N);
+    step_dev_ = dalloc<uint32_t>(1);
+    AFQ_HIP(hipMemsetAsync(step_dev_, 0, sizeof(uint32_t), s));
+    // generateP1: vMF (subtractMF = true), then H1 and P1
+    ham.vMF(p_.dt, vMF_);
+    ham.one_body_propagator(vMF_, p_.dt, P1_);
+    std::vector<z64> h((size_t)N * N);
+    AFQ_HIP(hipMemcpyAsync(h.data(), P1_, sizeof(z64) * h.size(), hipMemcpyDeviceToHost, s));
+    AFQ_HIP(hipStreamSynchronize(s));
+    real_P1_ = true;
+    for (auto& v : h)
+      if (v.im != 0.0) { real_P1_ = false; break; }
+    if (real_P1_) {
+      P1r_ = dalloc<double>((size_t)N * N);
+      real_part(s, (long)N * N, P1_, P1r_);
+      if (prec_ == kMixed) {
+        P1r32_ = dalloc<float>((size_t)N * N);
+        real_part_f32(s, (long)N * N, P1_, P1r32_);
+      }
+    } else if (prec_ == kMixed) {
+      P1c32_ = dalloc<c32>((size_t)N * N);
+      cast_z64_c32(s, (long)N * N, P1_, P1c32_);
+    }
+    AFQ_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    for (void* q : {(void*)vMF_, (void*)P1_, (void*)P1r_, (void*)P1r32_, (void*)P1c32_,
+                    (void*)step_dev_})
+      if (q) (void)hipFree(q);
+    throw;
+  }
+}
+
+Propagator::~Propagator() {
+  (void)hipStreamSynchronize(ctx_.stream());
+  for (auto& g : graphs_) {
+    (void)hipGraphExecDestroy(g.exec);
+    (void)hipGraphDestroy(g.graph);
+  }
+  for (void* q : {(void*)vMF_, (void*)P1_, (void*)P1r_, (void*)P1r32_, (void*)P1c32_,
+                  (void*)step_dev_})
+    if (q) hipFree(q);
+}
+
+void Propagator::set_step(uint32_t step) {
+  AFQ_HIP(hipMemcpyAsync(step_dev_, &step, sizeof(uint32_t), hipMemcpyHostToDevice, ctx_.stream()));
+  AFQ_HIP(hipStreamSynchronize(ctx_.stream()));
+}
+
+void Propagator::log_overlap(const WalkerState& st, z64* ovlp) {
+  hipStream_t s = ctx_.stream();
+  const int N = ham_.N(), nup = ham_.nup(), ndown = ham_.ndown();
+  const bool closed = ham_.walker_type() == kClosed;
+  fill_z64(s, st.W, {0, 0}, ovlp);
+  det_ops::log_overlap(ctx_, ham_.PsiT(), nup, N, st.Phi, (long)N * nup, st.W, ovlp,
+                       closed ? 2.0 : 1.0);
+  if (!closed && ndown > 0)
+    det_ops::log_overlap(ctx_, ham_.PsiT() + (size_t)nup * N, ndown, N, st.Phi_b, (long)N * ndown,
+                         st.W, ovlp, 1.0);
+  // + log(conj(ci0)) = 0 for the single determinant with unit coefficient
+}
+
+void Propagator::step_impl(const WalkerState& st, const double* u, const double* Eshift) {
+  hipStream_t s = ctx_.stream();
+  const int W = st.W, N = ham_.N(), M = ham_.M(), nup = ham_.nup(), ndown = ham_.ndown();
+  const int nel = ham_.nel();
+  const bool closed = ham_.walker_type() == kClosed;
+  AFQ_CHECK(W <= W_, kErrShape, "walker count exceeds the propagator's capacity");
+  z64* G = scratch_as<z64>(ctx_, "st_G", (size_t)W * nel * N);
+  z64* ov = scratch_as<z64>(ctx_, "st_ov", W);
+  z64* X = scratch_as<z64>(ctx_, "st_X", (size_t)W * M);
+  c32* X32 = prec_ == kMixed ? scratch_as<c32>(ctx_, "st_X32", (size_t)W * M) : nullptr;
+  z64* HW = scratch_as<z64>(ctx_, "st_HW", W);
+  z64* mf = scratch_as<z64>(ctx_, "st_mf", W);
+  z64* nov = scratch_as<z64>(ctx_, "st_nov", W);
+  void* V = ctx_.scratch("st_V", (size_t)W * N * N * (prec_ == kMixed ? sizeof(c32) : sizeof(z64)));
+
+  // 1. G and vbias (NOMSD::vbias)
+  fill_z64(s, W, {0, 0}, ov);
+  det_ops::mixed_density_matrix(ctx_, ham_.PsiT(), nup, N, st.Phi, (long)N * nup, W, G,
+                                (long)nel * N, ov, true, closed ? 2.0 : 1.0);
+  if (!closed && ndown > 0)
+    det_ops::mixed_density_matrix(ctx_, ham_.PsiT() + (size_t)nup * N, ndown, N, st.Phi_b,
+                                  (long)N * ndown, W, G + (size_t)nup * N, (long)nel * N, ov, true);
+  ham_.vbias(G, W, p_.dt, X, prec_);
+  // 2. fields, hybrid weight, mean-field factor
+  FieldParams fp{p_.vbias_bound, (p_.use_cp || p_.project_force_bias) ? 1 : 0, p_.free_projection,
+                 p_.seed, p_.stream, 0, u ? nullptr : step_dev_};
+  construct_x(s, u, vMF_, W, M, fp, X, HW, mf, X32);
+  // 3. vHS
+  ham_.vHS(prec_ == kMixed ? (const void*)X32 : (const void*)X, W, p_.dt, V, prec_);
+  // 4. propagate both spin blocks with the same V
+  det_ops::PropagateArgs pa{W, N, nup, p_.order, 'N', P1_, real_P1_ ? P1r_ : nullptr, P1r32_,
+                            P1c32_, V, (long)N * N, st.Phi, (long)N * nup, prec_};
+  det_ops::propagate(ctx_, pa);
+  if (!closed && ndown > 0) {
+    pa.n = ndown;
+    pa.Phi = st.Phi_b;
+    pa.phi_bs = (long)N * ndown;
+    det_ops::propagate(ctx_, pa);
+  }
+  // 5. new overlaps (hybrid)
+  log_overlap(st, nov);
+  // 6. weights
+  UpdateParams up{p_.dt, ham_.E0(), p_.lower_cut, p_.upper_cut, p_.weight_floor, p_.weight_frac,
+                  p_.target_pop, p_.free_projection, p_.use_cp};
+  walker_update(s, W, up, Eshift, nov, mf, HW, st.weight, st.phase, st.eloc, st.ovlp, st.theta);
+}
+
+void Propagator::step(const WalkerState& st, const double* u, const double* Eshift) {
+  step_impl(st, u, Eshift);
+  if (!u) increment_counter(ctx_.stream(), step_dev_);
+}
+
+void Propagator::orthogonalize(const WalkerState& st) {
+  hipStream_t s = ctx_.stream();
+  const int N = ham_.N(), nup = ham_.nup(), ndown = ham_.ndown();
+  z64* ldet = scratch_as<z64>(ctx_, "or_ldet", st.W);
+  fill_z64(s, st.W, {0, 0}, ldet);
+  det_ops::orthogonalize(ctx_, st.Phi, (long)N * nup, st.W, N, nup, ldet);
+  if (ham_.walker_type() == kCollinear && ndown > 0)
+    det_ops::orthogonalize(ctx_, st.Phi_b, (long)N * ndown, st.W, N, ndown, ldet);
+  log_overlap(st, st.ovlp);
+}
+
+void Propagator::energy(const WalkerState& st, z64* E, z64* ovlp, Precision prec) {
+  hipStream_t s = ctx_.stream();
+  const int W = st.W, N = ham_.N(), nup = ham_.nup(), ndown = ham_.ndown(), nel = ham_.nel();
+  const bool closed = ham_.walker_type() == kClosed;
+  z64* G = scratch_as<z64>(ctx_, "en_G", (size_t)W * nel * N);
+  fill_z64(s, W, {0, 0}, ovlp);
+  det_ops::mixed_density_matrix(ctx_, ham_.PsiT(), nup, N, st.Phi, (long)N * nup, W, G,
+                                (long)nel * N, ovlp, true, closed ? 2.0 : 1.0);
+  if (!closed && ndown > 0)
+    det_ops::mixed_density_matrix(ctx_, ham_.PsiT() + (size_t)nup * N, ndown, N, st.Phi_b,
+                                  (long)N * ndown, W, G + (size_t)nup * N, (long)nel * N, ovlp,
+                                  true);
+  ham_.energy(G, W, E, true, true, true, prec);
+}
+
+void Propagator::step_graph(const WalkerState& st, const double* Eshift, bool ortho, double relax,
+                            double* energy_out) {
+  hipStream_t s = ctx_.stream();
+  std::vector<const void*> key = {st.Phi, st.Phi_b, st.weight, st.phase, st.eloc, st.ovlp,
+                                  st.theta, Eshift, energy_out, (const void*)(long)st.W,
+                                  (const void*)(long)(ortho ? 1 : 0)};
+  long rbits;
+  static_assert(sizeof(rbits) == sizeof(relax), "");
+  std::memcpy(&rbits, &relax, sizeof(rbits));
+  key.push_back((const void*)rbits);
+  auto body = [&]() {
+    step_impl(st, nullptr, Eshift);
+    if (ortho) orthogonalize(st);
+    average_pseudo_energy(s, st.W, st.weight, st.eloc, p_.dt, relax, energy_out,
+                          const_cast<double*>(Eshift));
+    increment_counter(s, step_dev_);
+  };
+  bool have = false;
+  for (size_t i = 0; i < graphs_.size(); ++i)
+    if (graphs_[i].key == key) {
+      AFQ_HIP(hipGraphLaunch(graphs_[i].exec, s));
+      have = true;
+      break;
+    }
+  if (have) return;
+  if (!warmed_) {  // first use: run eagerly so every scratch buffer exists before capture
+    body();
+    warmed_ = true;
+    return;
+  }
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  AFQ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  try {
+    body();
+  } catch (...) {
+    (void)hipStreamEndCapture(s, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  AFQ_HIP(hipStreamEndCapture(s, &g));
+  AFQ_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+  graphs_.push_back({key, g, ge});
+  AFQ_HIP(hipGraphLaunch(ge, s));
+}
+
+}  // namespace afq
