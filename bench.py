@@ -1,0 +1,245 @@
+"""Walker-step throughput on one MI355X.
+
+Workload (the metric's configuration): N=200 orbitals, 40+40 electrons (closed shell), M=1600
+Cholesky vectors, 1024 walkers, dt=0.005, 6-term Taylor propagator, QR every 5 steps, synthetic
+hamiltonian (safire_amd.synthetic, seed 1003).  A step is one hybrid propagation of every walker
+(G, vbias, fields, vHS, P1 exp(iV) P1, overlap, weights) plus the growth estimator and Eshift
+relaxation, with the QR every fifth step, replayed from a HIP graph.  `value` is the mixed
+precision path (FP32 MFMA contractions, FP64 G/LU/QR/weights); the FP64 path is timed beside it.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu] [--only vhs|copy]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N, NUP, M, W = 200, 40, 1600, 1024
+DT, ORDER, ORTHO = 0.005, 6, 5
+SEED = 1003
+PEAK = {"fp32": 157.3e12, "fp64": 78.6e12}
+HBM_PEAK = 8.0e12
+
+
+def flops_per_walker_step(N, n, M, K=ORDER, k_ortho=ORTHO):
+    """Algorithmic FLOP of one hybrid step of one walker (complex MAC = 8 FLOP)."""
+    g = 16 * n * n * N + 16 * n ** 3
+    vb = 8 * M * n * N
+    prop = 8 * (K + 2) * N * N * n
+    ov = 8 * n * n * N + (8 / 3) * n ** 3
+    qr = 16 * n * n * N / k_ortho
+    vhs = 4 * N * N * M
+    return g + vb + prop + ov + qr + vhs
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def gpu_setup(problem, precision, W):
+    import torch
+    from safire_amd import afq
+    ctx = afq.Context(0)
+    ham = afq.HamiltonianOperations(ctx, problem.h1, problem.Likn, problem.PsiT, problem.nup,
+                                    precision=precision)
+    prop = afq.Propagator(ham, W, dt=DT, order=ORDER, seed=4242)
+    ws = afq.WalkerSet(problem.walkers(W))
+    E = afq.empty((W, 3), torch.complex128)
+    ov = afq.empty(W, torch.complex128)
+    prop.Energy(ws, E, ov)
+    ws.eloc.copy_(E.sum(1))
+    ws.ovlp.copy_(ov)
+    Esh = afq.device_tensor(np.array([float(afq.host(E.sum(1)).real.mean())]))
+    eout = afq.empty(1, torch.float64)
+    return ctx, ham, prop, ws, Esh, eout
+
+
+def time_steps(prop, ws, Esh, eout, warmup, steps):
+    import torch
+    relax = 1.0 - np.exp(-10.0 / 100.0)
+    for i in range(warmup):
+        prop.step_graph(ws, Esh, ortho=(i % ORTHO == 0), relax=relax, energy_out=eout)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(warmup, warmup + steps):
+        prop.step_graph(ws, Esh, ortho=(i % ORTHO == 0), relax=relax, energy_out=eout)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def time_vhs(ham, precision, W, reps=10):
+    """Average duration of one vHS GEMM launch (the dominant kernel), HIP events on the stream
+    the library launches on (the shared torch stream)."""
+    import torch
+    from safire_amd import afq
+    cdt = torch.complex64 if precision == afq.AFQ_MIXED else torch.complex128
+    X = (torch.randn(W, M, dtype=torch.float64, device="cuda") +
+         1j * torch.randn(W, M, dtype=torch.float64, device="cuda")).to(cdt).contiguous()
+    V = torch.empty((W, N, N), dtype=cdt, device="cuda")
+    ham.vHS(X, V, DT, precision=precision)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record()
+    for _ in range(reps):
+        ham.vHS(X, V, DT, precision=precision)
+    ev1.record()
+    ev1.synchronize()
+    return ev0.elapsed_time(ev1) / 1e3 / reps
+
+
+def energy_error(problem, W_e=64):
+    """Per-walker local energy of the mixed path vs the FP64 path after one step from the same
+    state and the same field draws (energies evaluated in FP64)."""
+    import torch
+    from safire_amd import afq
+    res = {}
+    for prec in (afq.AFQ_FP64, afq.AFQ_MIXED):
+        ctx, ham, prop, ws, Esh, eout = gpu_setup(problem, prec, W_e)
+        prop.set_step(0)
+        prop.Propagate(ws, Esh)
+        E = afq.empty((W_e, 3), torch.complex128)
+        ov = afq.empty(W_e, torch.complex128)
+        prop.Energy(ws, E, ov)
+        res[prec] = afq.host(E.sum(1))
+    d = np.abs(res[afq.AFQ_MIXED] - res[afq.AFQ_FP64])
+    return float(d.max()), float(np.median(d)), float((d / np.abs(res[afq.AFQ_FP64])).max()), \
+        float(np.median(np.abs(res[afq.AFQ_FP64])))
+
+
+def cpu_baseline(problem, budget_s=15.0):
+    """The FP64 oracle (CPU restatement of the reference step, OpenMP over walkers) on a
+    bounded sample: a few walkers of the same workload for one step, repeated to ~budget."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    lib = orc.lib()
+    threads = lib.orc_num_threads()
+    osys = orc.OracleSystem(problem, dt=DT)
+    Ws = max(threads, 8)
+    st, Eshift = osys.init_walkers(problem.walkers(Ws))
+    done, t_tot, step = 0, 0.0, 0
+    while t_tot < budget_s and step < 50:
+        u = orc.field_uniforms(7, 1, step, Ws, problem.M)
+        t0 = time.perf_counter()
+        osys.step(st, u, Eshift)
+        if step % ORTHO == 0:
+            osys.orthogonalize(st)
+        t_tot += time.perf_counter() - t0
+        done += Ws
+        step += 1
+    return {"value": done / t_tot, "unit": "walker-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{Ws} walkers x {step} steps of the same N=200/M=1600 workload, FP64 "
+                      f"oracle, {t_tot:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=6)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--only", choices=["vhs", "vhs64", "copy", "step"], default=None,
+                    help="profiling modes: launch only the named kernel(s)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(walkers shard across GPUs; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from safire_amd import afq
+    from safire_amd.synthetic import make_problem
+    log(f"generating workload N={N} n={NUP} M={M} W={W}")
+    problem = make_problem(N, NUP, M, SEED)
+
+    if args.only in ("vhs", "vhs64"):
+        prec = afq.AFQ_MIXED if args.only == "vhs" else afq.AFQ_FP64
+        ctx = afq.Context(0)
+        ham = afq.HamiltonianOperations(ctx, problem.h1, problem.Likn, problem.PsiT, NUP,
+                                        precision=prec)
+        t = time_vhs(ham, prec, W, reps=args.steps)
+        log(f"vHS launch {t * 1e3:.3f} ms")
+        return
+    if args.only == "copy":
+        # calibration kernel of known traffic for the FETCH/WRITE counters: copy 1 GiB
+        a = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(args.steps):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copied 1 GiB x %d" % args.steps)
+        return
+
+    out = {}
+    for prec, name in ((afq.AFQ_MIXED, "mixed"), (afq.AFQ_FP64, "fp64")):
+        if name == "fp64" and args.skip_fp64:
+            continue
+        log(f"[{name}] setup")
+        ctx, ham, prop, ws, Esh, eout = gpu_setup(problem, prec, W)
+        log(f"[{name}] timing {args.steps} steps after {args.warmup} warmup")
+        sec = time_steps(prop, ws, Esh, eout, args.warmup, args.steps)
+        tv = time_vhs(ham, prec, W)
+        out[name] = dict(sec=sec, vhs=tv)
+        log(f"[{name}] {sec * 1e3:.2f} ms/step, {W / sec:.4g} walker-steps/s, vHS {tv * 1e3:.3f} ms")
+        del ctx, ham, prop, ws
+        torch.cuda.empty_cache()
+        if args.only == "step":
+            break
+
+    emax = emed = erel = emag = None
+    if args.only is None:
+        log("energy error mixed vs fp64 (one step, 64 walkers)")
+        emax, emed, erel, emag = energy_error(problem)
+
+    cpu = None
+    if not args.skip_cpu and args.only is None:
+        log("cpu baseline (FP64 oracle)")
+        cpu = cpu_baseline(problem)
+
+    sec = out["mixed"]["sec"]
+    fstep = flops_per_walker_step(N, NUP, M)
+    vhs_flops = 4.0 * N * N * M * W
+    tv = out["mixed"]["vhs"]
+    achieved = vhs_flops / tv
+    line = {
+        "metric": "walker-steps/s @N=200,M=1600: mixed vs FP64 at <=0.1 mHa; serve p50/p99",
+        "value": W / sec,
+        "unit": "walker-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 FP32-MFMA contractions + c128 G/LU/QR/weights",
+        "data": "synthetic (seeded hamiltonian, trial, fields)",
+        "config": {"workload": "N=200 orbitals, 40+40 electrons, M=1600 Cholesky vectors, "
+                               "1024 walkers, dt=0.005, taylor 6, QR every 5 steps",
+                   "precision": "mixed", "graph": "one HIP graph per step"},
+        "roofline": {"bound": "mfma", "kernel": "vHS GEMM (FP32 MFMA, [W x M] . [M x N^2])",
+                     "achieved": achieved / 1e12, "peak": PEAK["fp32"] / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK["fp32"], "traffic": None},
+        "step_roofline_frac": (fstep * W / sec) / PEAK["fp32"],
+        "fp64": None if "fp64" not in out else {
+            "value": W / out["fp64"]["sec"], "ms_per_step": out["fp64"]["sec"] * 1e3,
+            "vhs_ms": out["fp64"]["vhs"] * 1e3,
+            "step_roofline_frac": (fstep * W / out["fp64"]["sec"]) / PEAK["fp64"]},
+        "mixed_over_fp64": None if "fp64" not in out else out["fp64"]["sec"] / sec,
+        "energy_error_mixed_vs_fp64_Ha": None if emax is None else
+            {"max": emax, "median": emed, "max_rel": erel, "median_abs_E": emag},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()
