@@ -57,6 +57,8 @@ void identity(hipStream_t s, int N, z64* A);
 void axpy_z64(hipStream_t s, long n, double ar, double ai, const z64* x, z64* y);  // y += a x
 void permute_akm_mak(hipStream_t s, int A, int K, int M, const z64* in, z64* out);
 void conj_transpose(hipStream_t s, int R, int C, const z64* in, z64* out);
+// true when every Cholesky vector L[.][.][m] of an [N][N][M] tensor is exactly symmetric
+bool symmetric_cholesky(hipStream_t s, int N, int M, const double* L);
 void real_part(hipStream_t s, long n, const z64* x, double* y);
 void real_part_f32(hipStream_t s, long n, const z64* x, float* y);
 // out[w*stride] += c * sum_k G[w][k] H[k] + add

@@ -78,6 +78,7 @@ CholeskyOps::CholeskyOps(Context& c, int walker_type, int N, int nup, int ndown,
     AFQ_HIP(hipMemcpyAsync(Likn_, Likn, sizeof(double) * N * N * M, hipMemcpyDeviceToDevice, s));
     AFQ_HIP(hipMemcpyAsync(h1_, h1, sizeof(double) * N * N, hipMemcpyDeviceToDevice, s));
 
+    sym_L_ = symmetric_cholesky(s, N, M, Likn_);
     // real trial?  (NOMSD.hpp:70-86 makes the same test on the host)
     std::vector<z64> hp((size_t)nel_ * N);
     AFQ_HIP(hipMemcpyAsync(hp.data(), PsiT_, sizeof(z64) * hp.size(), hipMemcpyDeviceToHost, s));
@@ -203,6 +204,27 @@ void CholeskyOps::vHS(const void* X, int W, double dt, void* V, Precision prec) 
   hipStream_t s = ctx_.stream();
   const double a = std::sqrt(dt);
   const long NN = (long)N_ * N_;
+  if (sym_L_) {
+    // V[w] is complex symmetric: compute the 8x8 blocks of the upper triangle of (i, k) pairs
+    // and write each value to (i, k) and (k, i)
+    const int nb = ceil_div(N_, 8);
+    const int npairs = nb * (nb + 1) / 2 * 64;
+    if (prec == kFP64) {
+      Opnd<double, true, true> A{reinterpret_cast<const double*>(X), M_, 0};
+      gemm::OpndSymPairs<double> B{Likn_, M_, 0, N_, nb};
+      launch_large<double>(s, 1, W, npairs, M_, A, B,
+                           gemm::EpiStoreSym<double, double>{reinterpret_cast<z64*>(V), NN, 0, a,
+                                                             0, N_, nb});
+    } else {
+      AFQ_CHECK(prec_ == kMixed, kErrState, "FP32 tensors were not built (precision FP64)");
+      Opnd<float, true, true> A{reinterpret_cast<const float*>(X), M_, 0};
+      gemm::OpndSymPairs<float> B{Likn32_, M_, 0, N_, nb};
+      launch_large<float>(s, 1, W, npairs, M_, A, B,
+                          gemm::EpiStoreSym<float, float>{reinterpret_cast<c32*>(V), NN, 0,
+                                                          (float)a, 0.f, N_, nb});
+    }
+    return;
+  }
   if (prec == kFP64) {
     Opnd<double, true, true> A{reinterpret_cast<const double*>(X), M_, 0};
     Opnd<double, false, true> B{Likn_, M_, 0};

@@ -162,6 +162,128 @@ __global__ void __launch_bounds__(64)
   }
 }
 
+// Register-resident variant of the same Gauss-Jordan inversion: lane c keeps column c of the
+// matrix in VGPRs (NMAX >= n rows), multipliers and pivots are broadcast with readlane, and the
+// row interchange is a predicated select over the (unrolled) rows.  Same pivots, same log det,
+// same inverse as k_lu_inverse_logdet, without LDS traffic in the elimination.
+__device__ inline double bcast(double v, int src) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __builtin_amdgcn_readlane(lo, src);
+  hi = __builtin_amdgcn_readlane(hi, src);
+  return __hiloint2double(hi, lo);
+}
+
+template <int NMAX>
+__global__ void __launch_bounds__(64)
+    k_lu_reg(const z64* __restrict__ A, long a_bs, int n, z64* __restrict__ Ainv, long i_bs,
+             z64* __restrict__ ovlp, double ov_scale, double ov_add_re, double ov_add_im,
+             int want_inverse) {
+  __shared__ int piv[NMAX];
+  const int lane = threadIdx.x;
+  const long b = blockIdx.x;
+  const z64* Ab = A + b * a_bs;
+  double xr[NMAX], xi[NMAX];
+#pragma unroll
+  for (int r = 0; r < NMAX; ++r) {
+    z64 v = (lane < n && r < n) ? Ab[r * n + lane] : z64{0.0, 0.0};
+    xr[r] = v.re;
+    xi[r] = v.im;
+  }
+  double ldr = 0.0, ldi = 0.0;
+  bool singular = false;
+  for (int j = 0; j < n; ++j) {
+    double best = -1.0;
+    int p = j;
+#pragma unroll
+    for (int r = 0; r < NMAX; ++r)
+      if (r >= j && r < n) {
+        double v = fabs(xr[r]) + fabs(xi[r]);
+        if (v > best) { best = v; p = r; }
+      }
+    p = __builtin_amdgcn_readlane(p, j);
+    if (lane == 0) piv[j] = p;
+    if (p != j) {
+      double tjr = 0, tji = 0, tpr = 0, tpi = 0;
+#pragma unroll
+      for (int r = 0; r < NMAX; ++r) {
+        if (r == j) { tjr = xr[r]; tji = xi[r]; }
+        if (r == p) { tpr = xr[r]; tpi = xi[r]; }
+      }
+#pragma unroll
+      for (int r = 0; r < NMAX; ++r) {
+        if (r == j) { xr[r] = tpr; xi[r] = tpi; }
+        if (r == p) { xr[r] = tjr; xi[r] = tji; }
+      }
+    }
+    double mjr = 0, mji = 0;  // my column's row-j entry
+#pragma unroll
+    for (int r = 0; r < NMAX; ++r)
+      if (r == j) { mjr = xr[r]; mji = xi[r]; }
+    const double dr = bcast(mjr, j), di = bcast(mji, j);
+    {
+      double sr = (p == j) ? dr : -dr, si = (p == j) ? di : -di;
+      double lr, li;
+      clog(sr + 2.2250738585072014e-308, si, lr, li);
+      ldr += lr;
+      ldi += li;
+    }
+    if (dr == 0.0 && di == 0.0) {
+      singular = true;
+      continue;
+    }
+    const double den = dr * dr + di * di;
+    const double ir = dr / den, ii = -di / den;
+    // scaled pivot-row entry of my column (1/d for column j itself)
+    const double pjr = (lane == j) ? ir : mjr * ir - mji * ii;
+    const double pji = (lane == j) ? ii : mjr * ii + mji * ir;
+#pragma unroll
+    for (int r = 0; r < NMAX; ++r) {
+      if (r >= n) continue;
+      const double fr = bcast(xr[r], j), fi = bcast(xi[r], j);
+      if (r == j) {
+        xr[r] = pjr;
+        xi[r] = pji;
+      } else if (lane == j) {
+        xr[r] = -(fr * ir - fi * ii);
+        xi[r] = -(fr * ii + fi * ir);
+      } else {
+        xr[r] -= fr * pjr - fi * pji;
+        xi[r] -= fr * pji + fi * pjr;
+      }
+    }
+  }
+  if (lane == 0 && ovlp != nullptr) {
+    z64 o = ovlp[b];
+    double re = o.re + ldr, im = o.im + ldi;
+    if (im > kPi) {
+      while (im > kPi) im -= 2 * kPi;
+    } else if (im < -kPi) {
+      while (im < -kPi) im += 2 * kPi;
+    }
+    ovlp[b] = {ov_scale * re + ov_add_re, ov_scale * im + ov_add_im};
+  }
+  if (!want_inverse) return;
+  __syncthreads();
+  if (!singular) {
+    for (int j = n - 1; j >= 0; --j) {
+      const int p = piv[j];
+      if (p == j) continue;
+      const int partner = (lane == j) ? p : ((lane == p) ? j : lane);
+#pragma unroll
+      for (int r = 0; r < NMAX; ++r) {
+        xr[r] = __shfl(xr[r], partner, 64);
+        xi[r] = __shfl(xi[r], partner, 64);
+      }
+    }
+  }
+  z64* Ib = Ainv + b * i_bs;
+  if (lane < n) {
+#pragma unroll
+    for (int r = 0; r < NMAX; ++r)
+      if (r < n) Ib[r * n + lane] = singular ? z64{0.0, 0.0} : z64{xr[r], xi[r]};
+  }
+}
+
 // Cholesky of a Hermitian positive-definite S (upper R, S = R^H R) and the inverse of R.
 // Rinv is written as a full n x n matrix (zeros below the diagonal). ldet += sum_i log R_ii.
 __global__ void __launch_bounds__(64)
@@ -239,10 +361,25 @@ void lu_inverse_logdet(hipStream_t s, const z64* A, long a_bs, int nb, int n, z6
                        long i_bs, z64* ovlp, double ov_scale, z64 ov_add, bool want_inverse) {
   AFQ_CHECK(n >= 1 && n <= 64, kErrShape, "n must be in [1, 64]");
   if (nb <= 0) return;
-  size_t lds = sizeof(double) * (2 * (size_t)n * (n + 1) + 2 * n) + sizeof(int) * n;
-  AFQ_CHECK(lds <= 65536, kErrShape, "matrix too large for one workgroup's LDS");
-  hipLaunchKernelGGL(k_lu_inverse_logdet, dim3(nb), dim3(64), lds, s, A, a_bs, n, Ainv, i_bs, ovlp,
-                     ov_scale, ov_add.re, ov_add.im, want_inverse ? 1 : 0);
+  const int wi = want_inverse ? 1 : 0;
+#define AFQ_LU_REG(NM)                                                                        \
+  hipLaunchKernelGGL(k_lu_reg<NM>, dim3(nb), dim3(64), 0, s, A, a_bs, n, Ainv, i_bs, ovlp,     \
+                     ov_scale, ov_add.re, ov_add.im, wi)
+  if (n <= 8)
+    AFQ_LU_REG(8);
+  else if (n <= 16)
+    AFQ_LU_REG(16);
+  else if (n <= 32)
+    AFQ_LU_REG(32);
+  else if (n <= 48)
+    AFQ_LU_REG(48);
+  else {
+    size_t lds = sizeof(double) * (2 * (size_t)n * (n + 1) + 2 * n) + sizeof(int) * n;
+    AFQ_CHECK(lds <= 65536, kErrShape, "matrix too large for one workgroup's LDS");
+    hipLaunchKernelGGL(k_lu_inverse_logdet, dim3(nb), dim3(64), lds, s, A, a_bs, n, Ainv, i_bs,
+                       ovlp, ov_scale, ov_add.re, ov_add.im, wi);
+  }
+#undef AFQ_LU_REG
   AFQ_LAUNCH_CHECK();
 }
 

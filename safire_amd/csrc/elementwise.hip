@@ -85,6 +85,15 @@ __global__ void k_permute_akm_mak(int A, int K, int M, const z64* in, z64* out) 
   int k = (int)(r % K), a = (int)(r / K);
   out[((long)m * A + a) * K + k] = in[t];
 }
+// flag = 1 when L[i][k][m] != L[k][i][m] for some i < k, m
+__global__ void k_check_sym(int N, int M, const double* L, int* flag) {
+  long t = (long)blockIdx.x * kT + threadIdx.x;
+  if (t >= (long)N * N * M) return;
+  int m = (int)(t % M);
+  long ik = t / M;
+  int i = (int)(ik / N), k = (int)(ik % N);
+  if (i < k && L[((long)i * N + k) * M + m] != L[((long)k * N + i) * M + m]) *flag = 1;
+}
 // Psi[i][a] = conj(PsiT[a][i])
 __global__ void k_conj_transpose(int R, int C, const z64* in, z64* out) {
   long t = (long)blockIdx.x * kT + threadIdx.x;
@@ -265,6 +274,18 @@ void permute_akm_mak(hipStream_t s, int A, int K, int M, const z64* in, z64* out
   if (n <= 0) return;
   hipLaunchKernelGGL(k_permute_akm_mak, grid1(n), dim3(kT), 0, s, A, K, M, in, out);
   AFQ_LAUNCH_CHECK();
+}
+bool symmetric_cholesky(hipStream_t s, int N, int M, const double* L) {
+  int* f = nullptr;
+  AFQ_HIP(hipMalloc(&f, sizeof(int)));
+  int h = 0;
+  AFQ_HIP(hipMemcpyAsync(f, &h, sizeof(int), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_check_sym, grid1((long)N * N * M), dim3(kT), 0, s, N, M, L, f);
+  AFQ_LAUNCH_CHECK();
+  AFQ_HIP(hipMemcpyAsync(&h, f, sizeof(int), hipMemcpyDeviceToHost, s));
+  AFQ_HIP(hipStreamSynchronize(s));
+  AFQ_HIP(hipFree(f));
+  return h == 0;
 }
 void conj_transpose(hipStream_t s, int R, int C, const z64* in, z64* out) {
   long n = (long)R * C;

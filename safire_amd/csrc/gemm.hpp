@@ -56,9 +56,49 @@ struct Opnd {
   static constexpr bool is_cplx = CPLX;
   static constexpr bool kcontig = KCONTIG;
   static constexpr bool conj = CONJ;
+  static constexpr bool mapped = false;
   const TIn* p;
   long ld;
   long bs;
+  __device__ long row_offset(int) const { return 0; }
+};
+
+// Upper-triangle 8x8 blocks of an N x N pair space (i, k): block (bi, bk), bi <= bk, enumerated
+// row by row, 64 pairs per block, pair t of a block = (8 bi + t / 8, 8 bk + t % 8).
+__device__ inline void sym_block(int id, int nb, int& bi, int& bk) {
+  float f = 0.5f * ((float)(2 * nb + 1) - sqrtf((float)((2 * nb + 1) * (2 * nb + 1) - 8 * id)));
+  int b = (int)f;
+  if (b < 0) b = 0;
+  while (b > 0 && b * nb - b * (b - 1) / 2 > id) --b;
+  while ((b + 1) * nb - (b + 1) * b / 2 <= id) ++b;
+  bi = b;
+  bk = b + (id - (b * nb - b * (b - 1) / 2));
+}
+__device__ inline bool sym_pair(int r, int nb, int N, int& i, int& k) {
+  int bi, bk;
+  sym_block(r >> 6, nb, bi, bk);
+  i = 8 * bi + ((r & 63) >> 3);
+  k = 8 * bk + (r & 7);
+  return i < N && k < N;
+}
+
+// Real operand whose rows are the symmetric pairs (i, k) of a row-major [N*N][K] tensor (the
+// Cholesky vectors L[i][k][m] with K = M contiguous).
+template <typename TIn>
+struct OpndSymPairs {
+  using in_t = TIn;
+  static constexpr bool is_cplx = false;
+  static constexpr bool kcontig = true;
+  static constexpr bool conj = false;
+  static constexpr bool mapped = true;
+  const TIn* p;
+  long ld;  // K stride of one (i, k) row
+  long bs;
+  int N, nb;
+  __device__ long row_offset(int r) const {
+    int i, k;
+    return sym_pair(r, nb, N, i, k) ? ((long)i * N + k) * ld : -1;
+  }
 };
 
 // ---- epilogues: called once per output element with the accumulated (re, im) ----------------
@@ -81,6 +121,25 @@ struct EpiStore {
       out = {(TOut)r, (TOut)i};
     }
     c[idx] = out;
+  }
+};
+
+// symmetric pair output: column n is the pair (i, k) of OpndSymPairs; writes
+// C[b][m][i][k] = C[b][m][k][i] = alpha * acc (C row m holds an N x N matrix)
+template <typename T, typename TOut>
+struct EpiStoreSym {
+  cplx<TOut>* c;
+  long ldc, bs;  // ldc = N * N
+  T ar, ai;
+  int N, nb;
+  __device__ inline void operator()(int b, int m, int n, T re, T im) const {
+    int i, k;
+    if (!sym_pair(n, nb, N, i, k)) return;
+    T r = ar * re - ai * im, q = ar * im + ai * re;
+    cplx<TOut> out = {(TOut)r, (TOut)q};
+    long base = (long)b * bs + (long)m * ldc;
+    c[base + (long)i * N + k] = out;
+    c[base + (long)k * N + i] = out;
   }
 };
 
@@ -133,6 +192,18 @@ struct Stager {
   using TIn = typename OP::in_t;
   TIn re[E];
   TIn im[OP::is_cplx ? E : 1];
+  long roff[OP::mapped ? E : 1];
+
+  // row offsets of a mapped operand (rows are fixed per thread for the whole tile)
+  __device__ inline void init_rows(const OP& op, int r0, int R, int tid) {
+    if constexpr (OP::mapped) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        int gr = r0 + (tid + e * NT) / BK;
+        roff[e] = gr < R ? op.row_offset(gr) : -1;
+      }
+    }
+  }
 
   __device__ inline void load(const OP& op, int b, int r0, int k0, int R, int K, int tid) {
 #pragma unroll
@@ -148,7 +219,13 @@ struct Stager {
       }
       int gr = r0 + r, gk = k0 + k;
       bool in = (gr < R) && (gk < K);
-      long a = (long)b * op.bs + (OP::kcontig ? (long)gr * op.ld + gk : (long)gk * op.ld + gr);
+      long a;
+      if constexpr (OP::mapped) {
+        in = (roff[e] >= 0) && (gk < K);
+        a = (long)b * op.bs + roff[e] + gk;
+      } else {
+        a = (long)b * op.bs + (OP::kcontig ? (long)gr * op.ld + gk : (long)gk * op.ld + gr);
+      }
       if (OP::is_cplx) {
         cplx<TIn> v = in ? reinterpret_cast<const cplx<TIn>*>(op.p)[a] : cplx<TIn>{0, 0};
         re[e] = v.re;
@@ -212,6 +289,8 @@ __global__ void __launch_bounds__(WAVES_M * WAVES_N * 64)
 
   Stager<T, OA, BM, BK, NT, LDA> ga;
   Stager<T, OB, BN, BK, NT, LDB> gb;
+  ga.init_rows(A, m0, M, tid);
+  gb.init_rows(B, n0, N, tid);
   ga.load(A, b, m0, 0, M, K, tid);
   gb.load(B, b, n0, 0, N, K, tid);
 

@@ -109,6 +109,7 @@ class CholeskyOps {
   const z64* haj() const { return haj_; }
   const double* vexx() const { return vexx_; }
   bool real_trial() const { return real_trial_; }
+  bool symmetric_L() const { return sym_L_; }
 
   // v[w][m] = sqrt(dt) c sum_{a,k} G[w][a][k] Lnak[m][a][k]   (vbias, :328-428)
   // prec == kMixed runs the contraction in FP32 and writes FP64 v.
@@ -131,6 +132,7 @@ class CholeskyOps {
   int walker_type_, N_, nup_, ndown_, M_, prec_;
   double E0_;
   bool real_trial_ = false;
+  bool sym_L_ = false;    // every L^m symmetric: vHS computes one triangle and mirrors it
   z64* PsiT_ = nullptr;   // [nel][N]
   double* Likn_ = nullptr;  // [N][N][M]
   float* Likn32_ = nullptr;

@@ -219,6 +219,28 @@ def test_mixed_contractions_close_to_fp64(gpu, system):
     assert np.max(np.abs(afq.host(E).sum(1) - Er.sum(1))) < 1e-4  # 0.1 mHa
 
 
+@pytest.mark.parametrize("sym", [True, False], ids=["symmetric-L", "general-L"])
+def test_vhs_paths_vs_oracle(gpu, sym):
+    """vHS with symmetric Cholesky vectors (triangle + mirror) and with a general tensor."""
+    afq, ctx, torch = gpu
+    N, n, M, W = 37, 6, 50, 9
+    say(f"case: vHS N={N} M={M} W={W} symmetric={sym}")
+    p = make_problem(N, n, M, seed=21)
+    Likn = p.Likn.copy()
+    if not sym:
+        Likn += 0.01 * np.random.default_rng(2).normal(size=Likn.shape)
+    ham = afq.HamiltonianOperations(ctx, p.h1, Likn, p.PsiT, n, precision=afq.AFQ_MIXED)
+    rng = np.random.default_rng(4)
+    X = rng.normal(size=(W, M)) + 1j * rng.normal(size=(W, M))
+    V = afq.empty((W, N, N), torch.complex128)
+    ham.vHS(afq.device_tensor(X), V, 0.01, precision=afq.AFQ_FP64)
+    Vr = orc.vhs(Likn, X, 0.01)
+    assert rel(afq.host(V), Vr) < TOL
+    V32 = afq.empty((W, N, N), torch.complex64)
+    ham.vHS(afq.device_tensor(X.astype(np.complex64)), V32, 0.01, precision=afq.AFQ_MIXED)
+    assert rel(afq.host(V32).astype(np.complex128), Vr) < 1e-5
+
+
 def test_mixed_vhs_and_propagate_close_to_fp64(gpu, system):
     afq, ctx, torch = gpu
     p, ham, osys = system
