@@ -284,6 +284,191 @@ __global__ void __launch_bounds__(64)
   }
 }
 
+// Workgroup variant (256 threads): the matrix in LDS, every elimination step spread over all
+// n^2 entries (pivot search by wavefront 0, row swap, pivot-row scaling, rank-1 update), one
+// barrier between phases.  Same pivot rule and log det as k_lu_inverse_logdet.
+constexpr int kLuThreads = 256;
+
+__global__ void __launch_bounds__(kLuThreads)
+    k_lu_blk(const z64* __restrict__ A, long a_bs, int n, z64* __restrict__ Ainv, long i_bs,
+             z64* __restrict__ ovlp, double ov_scale, double ov_add_re, double ov_add_im,
+             int want_inverse, int no_pivot) {
+  extern __shared__ double2 sm2[];
+  const int ld = n + 1;
+  double2* a = sm2;               // [n][ld]
+  double2* fcol = sm2 + n * ld;   // [n]
+  __shared__ int piv_s[64];
+  __shared__ int p_s;
+  __shared__ double ld_s[2];
+  __shared__ int sing_s;
+  const int tid = threadIdx.x;
+  const long b = blockIdx.x;
+  const z64* Ab = A + b * a_bs;
+  for (int e = tid; e < n * n; e += kLuThreads) {
+    z64 v = Ab[e];
+    a[(e / n) * ld + e % n] = make_double2(v.re, v.im);
+  }
+  if (tid == 0) {
+    ld_s[0] = 0.0;
+    ld_s[1] = 0.0;
+    sing_s = 0;
+  }
+  __syncthreads();
+  for (int j = 0; j < n; ++j) {
+    if (tid < 64) {
+      double best = -1.0;
+      int p = n;
+      if (no_pivot) {
+        p = j;
+      } else {
+        for (int r = j + tid; r < n; r += 64) {
+          double2 v = a[r * ld + j];
+          double c = fabs(v.x) + fabs(v.y);
+          if (c > best) { best = c; p = r; }
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+          double ob = __shfl_xor(best, off, 64);
+          int op = __shfl_xor(p, off, 64);
+          if (ob > best || (ob == best && op < p)) { best = ob; p = op; }
+        }
+      }
+      if (tid == 0) {
+        p_s = p;
+        piv_s[j] = p;
+      }
+    }
+    __syncthreads();
+    const int p = p_s;
+    if (p != j)
+      for (int c = tid; c < n; c += kLuThreads) {
+        double2 x = a[j * ld + c];
+        a[j * ld + c] = a[p * ld + c];
+        a[p * ld + c] = x;
+      }
+    __syncthreads();
+    const double2 d = a[j * ld + j];
+    if (tid == 0) {
+      double sr = (p == j) ? d.x : -d.x, si = (p == j) ? d.y : -d.y;
+      double lr, li;
+      clog(sr + 2.2250738585072014e-308, si, lr, li);
+      ld_s[0] += lr;
+      ld_s[1] += li;
+    }
+    if (d.x == 0.0 && d.y == 0.0) {
+      if (tid == 0) sing_s = 1;
+      continue;  // uniform: every thread read the same pivot
+    }
+    const double den = d.x * d.x + d.y * d.y;
+    const double ir = d.x / den, ii = -d.y / den;
+    for (int r = tid; r < n; r += kLuThreads)
+      if (r != j) fcol[r] = a[r * ld + j];
+    __syncthreads();
+    for (int c = tid; c < n; c += kLuThreads) {
+      if (c == j) {
+        a[j * ld + c] = make_double2(ir, ii);
+      } else {
+        double2 x = a[j * ld + c];
+        a[j * ld + c] = make_double2(x.x * ir - x.y * ii, x.x * ii + x.y * ir);
+      }
+    }
+    __syncthreads();
+    for (int e = tid; e < n * n; e += kLuThreads) {
+      const int r = e / n, c = e % n;
+      if (r == j) continue;
+      const double2 f = fcol[r];
+      if (c == j) {
+        a[r * ld + c] = make_double2(-(f.x * ir - f.y * ii), -(f.x * ii + f.y * ir));
+      } else {
+        const double2 pj = a[j * ld + c];
+        double2 x = a[r * ld + c];
+        x.x -= f.x * pj.x - f.y * pj.y;
+        x.y -= f.x * pj.y + f.y * pj.x;
+        a[r * ld + c] = x;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0 && ovlp != nullptr) {
+    z64 o = ovlp[b];
+    double re = o.re + ld_s[0], im = o.im + ld_s[1];
+    if (im > kPi) {
+      while (im > kPi) im -= 2 * kPi;
+    } else if (im < -kPi) {
+      while (im < -kPi) im += 2 * kPi;
+    }
+    ovlp[b] = {ov_scale * re + ov_add_re, ov_scale * im + ov_add_im};
+  }
+  if (!want_inverse) return;
+  __syncthreads();
+  const bool singular = sing_s != 0;
+  if (!singular) {
+    for (int j = n - 1; j >= 0; --j) {
+      const int p = piv_s[j];
+      if (p != j)
+        for (int r = tid; r < n; r += kLuThreads) {
+          double2 x = a[r * ld + j];
+          a[r * ld + j] = a[r * ld + p];
+          a[r * ld + p] = x;
+        }
+      __syncthreads();
+    }
+  }
+  z64* Ib = Ainv + b * i_bs;
+  for (int e = tid; e < n * n; e += kLuThreads) {
+    double2 v = a[(e / n) * ld + e % n];
+    Ib[e] = singular ? z64{0.0, 0.0} : z64{v.x, v.y};
+  }
+}
+
+// Workgroup Cholesky: S = R^H R with R upper triangular (positive real diagonal), written to R
+// with zeros below the diagonal.
+__global__ void __launch_bounds__(kLuThreads)
+    k_chol_blk(const z64* __restrict__ S, int n, z64* __restrict__ R) {
+  extern __shared__ double2 sm2[];
+  const int ld = n + 1;
+  double2* a = sm2;
+  const int tid = threadIdx.x;
+  const long b = blockIdx.x;
+  const z64* Sb = S + b * (long)n * n;
+  for (int e = tid; e < n * n; e += kLuThreads) {
+    z64 v = Sb[e];
+    a[(e / n) * ld + e % n] = make_double2(v.re, v.im);
+  }
+  __syncthreads();
+  for (int j = 0; j < n; ++j) {
+    const double d = sqrt(a[j * ld + j].x);
+    __syncthreads();
+    for (int c = tid; c < n; c += kLuThreads) {
+      if (c == j) {
+        a[j * ld + c] = make_double2(d, 0.0);
+      } else if (c > j) {
+        double2 x = a[j * ld + c];
+        a[j * ld + c] = make_double2(x.x / d, x.y / d);
+      }
+    }
+    __syncthreads();
+    // A[i][c] -= conj(R[j][i]) R[j][c] for j < i <= c
+    const int m = n - j - 1;
+    for (int e = tid; e < m * m; e += kLuThreads) {
+      const int i = j + 1 + e / m, c = j + 1 + e % m;
+      if (c < i) continue;
+      const double2 ri = a[j * ld + i], rc = a[j * ld + c];
+      double2 x = a[i * ld + c];
+      x.x -= ri.x * rc.x + ri.y * rc.y;
+      x.y -= ri.x * rc.y - ri.y * rc.x;
+      a[i * ld + c] = x;
+    }
+    __syncthreads();
+  }
+  z64* Rb = R + b * (long)n * n;
+  for (int e = tid; e < n * n; e += kLuThreads) {
+    const int r = e / n, c = e % n;
+    double2 v = a[r * ld + c];
+    Rb[e] = (c >= r) ? z64{v.x, v.y} : z64{0.0, 0.0};
+  }
+}
+
 // Cholesky of a Hermitian positive-definite S (upper R, S = R^H R) and the inverse of R.
 // Rinv is written as a full n x n matrix (zeros below the diagonal). ldet += sum_i log R_ii.
 __global__ void __launch_bounds__(64)
@@ -369,15 +554,11 @@ void lu_inverse_logdet(hipStream_t s, const z64* A, long a_bs, int nb, int n, z6
     AFQ_LU_REG(8);
   else if (n <= 16)
     AFQ_LU_REG(16);
-  else if (n <= 32)
-    AFQ_LU_REG(32);
-  else if (n <= 48)
-    AFQ_LU_REG(48);
   else {
-    size_t lds = sizeof(double) * (2 * (size_t)n * (n + 1) + 2 * n) + sizeof(int) * n;
+    size_t lds = sizeof(double2) * ((size_t)n * (n + 1) + n);
     AFQ_CHECK(lds <= 65536, kErrShape, "matrix too large for one workgroup's LDS");
-    hipLaunchKernelGGL(k_lu_inverse_logdet, dim3(nb), dim3(64), lds, s, A, a_bs, n, Ainv, i_bs,
-                       ovlp, ov_scale, ov_add.re, ov_add.im, wi);
+    hipLaunchKernelGGL(k_lu_blk, dim3(nb), dim3(kLuThreads), lds, s, A, a_bs, n, Ainv, i_bs,
+                       ovlp, ov_scale, ov_add.re, ov_add.im, wi, 0);
   }
 #undef AFQ_LU_REG
   AFQ_LAUNCH_CHECK();
@@ -386,9 +567,14 @@ void lu_inverse_logdet(hipStream_t s, const z64* A, long a_bs, int nb, int n, z6
 void chol_upper_inverse(hipStream_t s, const z64* S, int nb, int n, z64* Rinv, z64* ldet) {
   AFQ_CHECK(n >= 1 && n <= 64, kErrShape, "n must be in [1, 64]");
   if (nb <= 0) return;
-  size_t lds = sizeof(double) * 2 * (size_t)n * (n + 1);
+  size_t lds = sizeof(double2) * ((size_t)n * (n + 1) + n);
   AFQ_CHECK(lds <= 65536, kErrShape, "matrix too large for one workgroup's LDS");
-  hipLaunchKernelGGL(k_chol_upper_inverse, dim3(nb), dim3(64), lds, s, S, n, Rinv, ldet);
+  // R is written into Rinv, then inverted in place (upper triangular: no pivoting needed);
+  // the inversion's log det is sum_i log R_ii (R_ii real and positive).
+  hipLaunchKernelGGL(k_chol_blk, dim3(nb), dim3(kLuThreads), lds, s, S, n, Rinv);
+  AFQ_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_lu_blk, dim3(nb), dim3(kLuThreads), lds, s, Rinv, (long)n * n, n, Rinv,
+                     (long)n * n, ldet, 1.0, 0.0, 0.0, 1, 1);
   AFQ_LAUNCH_CHECK();
 }
 

@@ -100,6 +100,27 @@ def test_golden_inverse_logdet_on_gpu(gpu):
     assert rel(afq.host(inv), ref_inv) < 1e-13
 
 
+@pytest.mark.parametrize("n", [3, 12, 40])
+def test_inverse_logdet_singular_and_random(gpu, n):
+    """getri_or_zero: a singular matrix gives a zero inverse; regular ones match the oracle."""
+    afq, ctx, torch = gpu
+    say(f"case: inverse_logdet n={n} (one singular matrix in the batch)")
+    rng = np.random.default_rng(n)
+    A = rng.normal(size=(6, n, n)) + 1j * rng.normal(size=(6, n, n))
+    A[2, 1, :] = 0.0  # singular
+    dA = afq.device_tensor(A)
+    inv = afq.empty((6, n, n), torch.complex128)
+    ov = afq.zeros(6, torch.complex128)
+    afq.det_ops.inverse_logdet(ctx, dA, ov, inv)
+    ref_inv, ref_ov = orc.inverse_logdet(A)
+    gi, go = afq.host(inv), afq.host(ov)
+    assert np.all(gi[2] == 0) and np.all(ref_inv[2] == 0)
+    assert go[2].real < -600
+    keep = [0, 1, 3, 4, 5]
+    assert rel(gi[keep], ref_inv[keep]) < TOL
+    assert rel(go[keep], ref_ov[keep]) < TOL
+
+
 @pytest.mark.parametrize("N,n,W", [(24, 5, 17), (108, 24, 33), (200, 40, 9)])
 def test_det_ops_vs_oracle(gpu, N, n, W):
     afq, ctx, torch = gpu
