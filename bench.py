@@ -194,6 +194,7 @@ def main():
         if args.only == "step":
             break
 
+    log("autotuned tiles:\n" + afq.tuning_report())
     emax = emed = erel = emag = None
     if args.only is None:
         log("energy error mixed vs fp64 (one step, 64 walkers)")

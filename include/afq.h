@@ -65,6 +65,9 @@ const char* afq_last_error(void);
 int afq_ctx_create(int device, void* stream, afq_ctx** out);
 int afq_ctx_destroy(afq_ctx* ctx);
 int afq_ctx_synchronize(afq_ctx* ctx);
+/* Tile shapes picked by the autotuner so far, one "site:precision:shape -> candidate" line per
+ * tuned product (written into buf, truncated to len - 1 characters). */
+int afq_tuning_report(char* buf, int len);
 /* Bytes of scratch held by the context. */
 int afq_ctx_scratch_bytes(afq_ctx* ctx, uint64_t* bytes);
 

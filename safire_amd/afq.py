@@ -59,6 +59,7 @@ _SIGS = {
     "afq_ctx_destroy": [_P],
     "afq_ctx_synchronize": [_P],
     "afq_ctx_scratch_bytes": [_P, ctypes.POINTER(ctypes.c_uint64)],
+    "afq_tuning_report": [ctypes.c_char_p, _I],
     "afq_ham_create": [_P, _I, _I, _I, _I, _I, _P, _P, _P, _D, _I, ctypes.POINTER(_P)],
     "afq_ham_destroy": [_P],
     "afq_ham_tensors": [_P, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_P),
@@ -110,6 +111,12 @@ def lib():
 
 def exported_symbols():
     return list(_SIGS)
+
+
+def tuning_report() -> str:
+    buf = ctypes.create_string_buffer(1 << 16)
+    _ck(lib().afq_tuning_report(buf, len(buf)))
+    return buf.value.decode()
 
 
 def _ck(rc):

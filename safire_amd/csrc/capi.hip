@@ -4,6 +4,7 @@
 
 #include "../../include/afq.h"
 #include "ops.hpp"
+#include "tuner.hpp"
 
 using afq::z64;
 
@@ -86,6 +87,16 @@ int afq_ctx_synchronize(afq_ctx* ctx) {
   return guard([&] {
     REQUIRE(ctx, "ctx is NULL");
     AFQ_HIP(hipStreamSynchronize(ctx->c.stream()));
+  });
+}
+
+int afq_tuning_report(char* buf, int len) {
+  return guard([&] {
+    REQUIRE(buf && len > 0, "NULL argument");
+    std::string r = afq::Tuner::get().report();
+    size_t n = std::min<size_t>(r.size(), (size_t)len - 1);
+    r.copy(buf, n);
+    buf[n] = 0;
   });
 }
 

@@ -23,6 +23,7 @@
 
 #include "gemm.hpp"
 #include "ops.hpp"
+#include "tuner.hpp"
 
 namespace afq {
 
@@ -179,10 +180,10 @@ void CholeskyOps::vbias(const z64* G, int W, double dt, z64* v, Precision prec) 
       EpiStore<double, double> e{v, M_, 0, a, 0, acc};
       if (real_trial_) {
         Opnd<double, false, true> B{Lnak_r_ + loff, K, 0};
-        launch_large<double>(s, 1, W, M_, (int)K, A, B, e);
+        gemm_large<double>(ctx_, "vbias_r", 1, W, M_, (int)K, A, B, e, acc == 0);
       } else {
         Opnd<double, true, true> B{&(Lnak_ + loff)->re, K, 0};
-        launch_large<double>(s, 1, W, M_, (int)K, A, B, e);
+        gemm_large<double>(ctx_, "vbias_c", 1, W, M_, (int)K, A, B, e, acc == 0);
       }
     } else {
       AFQ_CHECK(prec_ == kMixed, kErrState, "FP32 tensors were not built (precision FP64)");
@@ -190,10 +191,10 @@ void CholeskyOps::vbias(const z64* G, int W, double dt, z64* v, Precision prec) 
       EpiStore<float, double> e{v, M_, 0, (float)a, 0.f, acc};
       if (real_trial_) {
         Opnd<float, false, true> B{Lnak_r32_ + loff, K, 0};
-        launch_large<float>(s, 1, W, M_, (int)K, A, B, e);
+        gemm_large<float>(ctx_, "vbias_r", 1, W, M_, (int)K, A, B, e, acc == 0);
       } else {
         Opnd<float, true, true> B{&(Lnak32_ + loff)->re, K, 0};
-        launch_large<float>(s, 1, W, M_, (int)K, A, B, e);
+        gemm_large<float>(ctx_, "vbias_c", 1, W, M_, (int)K, A, B, e, acc == 0);
       }
     }
   }
@@ -212,30 +213,30 @@ void CholeskyOps::vHS(const void* X, int W, double dt, void* V, Precision prec) 
     if (prec == kFP64) {
       Opnd<double, true, true> A{reinterpret_cast<const double*>(X), M_, 0};
       gemm::OpndSymPairs<double> B{Likn_, M_, 0, N_, nb};
-      launch_large<double>(s, 1, W, npairs, M_, A, B,
-                           gemm::EpiStoreSym<double, double>{reinterpret_cast<z64*>(V), NN, 0, a,
-                                                             0, N_, nb});
+      gemm_large<double>(ctx_, "vhs_sym", 1, W, npairs, M_, A, B,
+                         gemm::EpiStoreSym<double, double>{reinterpret_cast<z64*>(V), NN, 0, a,
+                                                           0, N_, nb});
     } else {
       AFQ_CHECK(prec_ == kMixed, kErrState, "FP32 tensors were not built (precision FP64)");
       Opnd<float, true, true> A{reinterpret_cast<const float*>(X), M_, 0};
       gemm::OpndSymPairs<float> B{Likn32_, M_, 0, N_, nb};
-      launch_large<float>(s, 1, W, npairs, M_, A, B,
-                          gemm::EpiStoreSym<float, float>{reinterpret_cast<c32*>(V), NN, 0,
-                                                          (float)a, 0.f, N_, nb});
+      gemm_large<float>(ctx_, "vhs_sym", 1, W, npairs, M_, A, B,
+                        gemm::EpiStoreSym<float, float>{reinterpret_cast<c32*>(V), NN, 0,
+                                                        (float)a, 0.f, N_, nb});
     }
     return;
   }
   if (prec == kFP64) {
     Opnd<double, true, true> A{reinterpret_cast<const double*>(X), M_, 0};
     Opnd<double, false, true> B{Likn_, M_, 0};
-    launch_large<double>(s, 1, W, (int)NN, M_, A, B,
-                         EpiStore<double, double>{reinterpret_cast<z64*>(V), NN, 0, a, 0, 0});
+    gemm_large<double>(ctx_, "vhs", 1, W, (int)NN, M_, A, B,
+                       EpiStore<double, double>{reinterpret_cast<z64*>(V), NN, 0, a, 0, 0});
   } else {
     AFQ_CHECK(prec_ == kMixed, kErrState, "FP32 tensors were not built (precision FP64)");
     Opnd<float, true, true> A{reinterpret_cast<const float*>(X), M_, 0};
     Opnd<float, false, true> B{Likn32_, M_, 0};
-    launch_large<float>(s, 1, W, (int)NN, M_, A, B,
-                        EpiStore<float, float>{reinterpret_cast<c32*>(V), NN, 0, (float)a, 0.f, 0});
+    gemm_large<float>(ctx_, "vhs", 1, W, (int)NN, M_, A, B,
+                      EpiStore<float, float>{reinterpret_cast<c32*>(V), NN, 0, (float)a, 0.f, 0});
   }
 }
 
@@ -289,7 +290,7 @@ void CholeskyOps::energy(const z64* G, int W, z64* E, bool addH1, bool addEJ, bo
       if (prec == kFP64) {
         Opnd<double, true, true> A{&Gblk->re, N_, 0};
         Opnd<double, true, true> B{&(Lsp + (size_t)m0 * ns * N_)->re, N_, 0};
-        launch_large<double>(s, 1, W * ns, mc * ns, N_, A, B, e);
+        gemm_large<double>(ctx_, "exx_chunk", 1, W * ns, mc * ns, N_, A, B, e);
       } else {
         AFQ_CHECK(prec_ == kMixed, kErrState, "FP32 tensors were not built (precision FP64)");
         if (!G32) G32 = scratch_as<c32>(ctx_, "en_G32", (size_t)W * ns * N_);
@@ -299,11 +300,11 @@ void CholeskyOps::energy(const z64* G, int W, z64* E, bool addH1, bool addEJ, bo
         if (real_trial_) {
           Opnd<float, false, true> B{Lnak_r32_ + (sp == 0 ? 0 : (size_t)M_ * nup_ * N_) +
                                          (size_t)m0 * ns * N_, N_, 0};
-          launch_large<float>(s, 1, W * ns, mc * ns, N_, A, B, ef);
+          gemm_large<float>(ctx_, "exx_chunk_r", 1, W * ns, mc * ns, N_, A, B, ef);
         } else {
           Opnd<float, true, true> B{&(Lnak32_ + (sp == 0 ? 0 : (size_t)M_ * nup_ * N_) +
                                       (size_t)m0 * ns * N_)->re, N_, 0};
-          launch_large<float>(s, 1, W * ns, mc * ns, N_, A, B, ef);
+          gemm_large<float>(ctx_, "exx_chunk_c", 1, W * ns, mc * ns, N_, A, B, ef);
         }
       }
       exchange_reduce(s, W, ns, MC, m0, M_, T, part + (size_t)sp * W * nchunk, nchunk, ch,

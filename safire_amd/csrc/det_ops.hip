@@ -13,6 +13,7 @@
 // GEMM-shaped Gram/solve steps keep the work on the matrix cores.
 #include "gemm.hpp"
 #include "ops.hpp"
+#include "tuner.hpp"
 
 namespace afq {
 
@@ -74,30 +75,6 @@ using gemm::EpiStore2;
 using gemm::EpiTaylor;
 using gemm::Opnd;
 
-namespace {
-
-// tile shapes
-template <typename T, class OA, class OB, class EPI>
-void launch_small(hipStream_t s, int batch, int M, int N, int K, const OA& A, const OB& B,
-                  const EPI& e) {
-  gemm::launch<T, 32, 32, 16, 1, 1>(s, batch, M, N, K, A, B, e);
-}
-
-// tall x narrow products (N rows of orbitals x n electrons): 64-row tiles, BN fitted to n
-template <typename T, class OA, class OB, class EPI>
-void launch_tall(hipStream_t s, int batch, int M, int N, int K, const OA& A, const OB& B,
-                 const EPI& e) {
-  if (N <= 16)
-    gemm::launch<T, 64, 16, 16, 4, 1>(s, batch, M, N, K, A, B, e);
-  else if (N <= 32)
-    gemm::launch<T, 64, 32, 16, 4, 1>(s, batch, M, N, K, A, B, e);
-  else if (N <= 48)
-    gemm::launch<T, 64, 48, 16, 4, 1>(s, batch, M, N, K, A, B, e);
-  else
-    gemm::launch<T, 64, 64, 16, 4, 1>(s, batch, M, N, K, A, B, e);
-}
-
-}  // namespace
 
 void log_overlap(Context& c, const z64* PsiT, int n, int N, const z64* Phi, long phi_bs, int W,
                  z64* ovlp, double scale, z64 add) {
@@ -106,7 +83,7 @@ void log_overlap(Context& c, const z64* PsiT, int n, int N, const z64* Phi, long
   z64* T = scratch_as<z64>(c, "lo_T", (size_t)W * n * n);
   Opnd<double, true, true> A{&PsiT->re, N, 0};
   Opnd<double, true, false> B{&Phi->re, n, phi_bs};
-  launch_small<double>(s, W, n, n, N, A, B, EpiStore<double, double>{T, n, (long)n * n, 1, 0, 0});
+  gemm_small<double>(c, "lo_T", W, n, n, N, A, B, EpiStore<double, double>{T, n, (long)n * n, 1, 0, 0});
   lu_inverse_logdet(s, T, (long)n * n, W, n, nullptr, 0, ovlp, scale, add, false);
 }
 
@@ -122,7 +99,7 @@ void mixed_density_matrix(Context& c, const z64* PsiT, int n, int N, const z64* 
   {
     Opnd<double, true, true> A{&PsiT->re, N, 0};
     Opnd<double, true, false> B{&Phi->re, n, phi_bs};
-    launch_small<double>(s, W, n, n, N, A, B, EpiStore<double, double>{T, n, (long)n * n, 1, 0, 0});
+    gemm_small<double>(c, "mdm_T", W, n, n, N, A, B, EpiStore<double, double>{T, n, (long)n * n, 1, 0, 0});
   }
   lu_inverse_logdet(s, T, (long)n * n, W, n, T, (long)n * n, ovlp, scale, {0, 0}, true);
   // compact G[a][k] = sum_b Tinv[b][a] Phi[k][b]    (product<'T','T'>(TNN, B, C))
@@ -131,13 +108,13 @@ void mixed_density_matrix(Context& c, const z64* PsiT, int n, int N, const z64* 
   {
     Opnd<double, true, false> A{&T->re, n, (long)n * n};
     Opnd<double, true, true> B{&Phi->re, n, phi_bs};
-    launch_small<double>(s, W, n, N, n, A, B, EpiStore<double, double>{Gc, N, gc_bs, 1, 0, 0});
+    gemm_small<double>(c, "mdm_G", W, n, N, n, A, B, EpiStore<double, double>{Gc, N, gc_bs, 1, 0, 0});
   }
   if (!compact) {
     // G[i][k] = sum_a PsiT[a][i] Gc[a][k]           (product<'T'>(A, TNM, C))
     Opnd<double, true, false> A{&PsiT->re, N, 0};
     Opnd<double, true, false> B{&Gc->re, N, (long)n * N};
-    launch_small<double>(s, W, N, N, n, A, B, EpiStore<double, double>{G, N, g_bs, 1, 0, 0});
+    gemm_small<double>(c, "mdm_full", W, N, N, n, A, B, EpiStore<double, double>{G, N, g_bs, 1, 0, 0});
   }
 }
 
@@ -174,24 +151,34 @@ void propagate_t(Context& c, const PropagateArgs& a) {
     EpiStore2<T> e{S, T1, n, tb};
     if (P1r) {
       Opnd<T, false, true> A{P1r, N, 0};
-      launch_tall<T>(s, W, N, n, N, A, B, e);
+      gemm_tall<T>(c, "prop_P1phi_r", W, N, n, N, A, B, e);
     } else {
       Opnd<T, true, true> A{&P1c->re, N, 0};
-      launch_tall<T>(s, W, N, n, N, A, B, e);
+      gemm_tall<T>(c, "prop_P1phi_c", W, N, n, N, A, B, e);
     }
   }
-  // Taylor terms: T2 = (+-i/k) op(V) T1 ; S += T2
+  // Taylor terms: T2 = (+-i/k) op(V) T1 ; S += T2.  The accumulating epilogue cannot be timed
+  // in place, so the terms use the tile tuned for the P1 * Phi product of the same shape.
   const cplx<T>* V = reinterpret_cast<const cplx<T>*>(a.V);
+  int tc = Tuner::get().lookup(tune_key(P1r ? "prop_P1phi_r" : "prop_P1phi_c", sizeof(T), W, N, n, N));
+  auto taylor = [&](auto const& A, auto const& B, auto const& e) {
+    if (tc == 1)
+      tall_bn<T, 208, 13>(s, W, N, n, N, A, B, e);
+    else if (tc == 2)
+      tall_bn<T, 128, 8>(s, W, N, n, N, A, B, e);
+    else
+      tall_bn<T, 64, 4>(s, W, N, n, N, A, B, e);
+  };
   for (int k = 1; k <= a.order; ++k) {
     T fi = (T)((a.TA == 'H' ? -1.0 : 1.0) / (double)k);
     Opnd<T, true, false> B{&T1->re, n, tb};
     EpiTaylor<T> e{T2, S, n, tb, (T)0, fi};
     if (a.TA == 'H') {
       Opnd<T, true, false, true> A{&V->re, N, a.v_bs};  // A(i, j) = conj(V[j][i])
-      launch_tall<T>(s, W, N, n, N, A, B, e);
+      taylor(A, B, e);
     } else {
       Opnd<T, true, true> A{&V->re, N, a.v_bs};
-      launch_tall<T>(s, W, N, n, N, A, B, e);
+      taylor(A, B, e);
     }
     std::swap(T1, T2);
   }
@@ -201,10 +188,10 @@ void propagate_t(Context& c, const PropagateArgs& a) {
     EpiStore<T, double> e{a.Phi, n, a.phi_bs, (T)1, (T)0, 0};
     if (P1r) {
       Opnd<T, false, true> A{P1r, N, 0};
-      launch_tall<T>(s, W, N, n, N, A, B, e);
+      gemm_tall<T>(c, "prop_P1S_r", W, N, n, N, A, B, e);
     } else {
       Opnd<T, true, true> A{&P1c->re, N, 0};
-      launch_tall<T>(s, W, N, n, N, A, B, e);
+      gemm_tall<T>(c, "prop_P1S_c", W, N, n, N, A, B, e);
     }
   }
 }
@@ -238,13 +225,13 @@ void orthogonalize(Context& c, z64* Phi, long phi_bs, int W, int N, int n, z64* 
     {  // S = X^H X
       Opnd<double, true, false, true> A{&X->re, n, xbs};
       Opnd<double, true, false> B{&X->re, n, xbs};
-      launch_small<double>(s, W, n, n, N, A, B, EpiStore<double, double>{S, n, (long)n * n, 1, 0, 0});
+      gemm_small<double>(c, "qr_gram", W, n, n, N, A, B, EpiStore<double, double>{S, n, (long)n * n, 1, 0, 0});
     }
     chol_upper_inverse(s, S, W, n, Ri, ldet);
     {  // Y = X R^{-1}
       Opnd<double, true, true> A{&X->re, n, xbs};
       Opnd<double, true, false> B{&Ri->re, n, (long)n * n};
-      launch_tall<double>(s, W, N, n, n, A, B, EpiStore<double, double>{Y, n, ybs, 1, 0, 0});
+      gemm_tall<double>(c, "qr_Q", W, N, n, n, A, B, EpiStore<double, double>{Y, n, ybs, 1, 0, 0});
     }
   }
 }

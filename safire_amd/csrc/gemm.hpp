@@ -187,8 +187,9 @@ struct EpiTaylor {
 
 template <typename T, class OP, int ROWS, int BK, int NT, int LDP>
 struct Stager {
-  static constexpr int E = ROWS * BK / NT;
-  static_assert((ROWS * BK) % NT == 0, "tile must divide the thread count");
+  static constexpr int TOT = ROWS * BK;
+  static constexpr int E = (TOT + NT - 1) / NT;
+  static constexpr bool EXACT = (TOT % NT) == 0;
   using TIn = typename OP::in_t;
   TIn re[E];
   TIn im[OP::is_cplx ? E : 1];
@@ -200,7 +201,7 @@ struct Stager {
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         int gr = r0 + (tid + e * NT) / BK;
-        roff[e] = gr < R ? op.row_offset(gr) : -1;
+        roff[e] = (gr < R && (EXACT || tid + e * NT < TOT)) ? op.row_offset(gr) : -1;
       }
     }
   }
@@ -218,10 +219,10 @@ struct Stager {
         k = idx / ROWS;
       }
       int gr = r0 + r, gk = k0 + k;
-      bool in = (gr < R) && (gk < K);
+      bool in = (gr < R) && (gk < K) && (EXACT || idx < TOT);
       long a;
       if constexpr (OP::mapped) {
-        in = (roff[e] >= 0) && (gk < K);
+        in = (roff[e] >= 0) && (gk < K) && (EXACT || idx < TOT);
         a = (long)b * op.bs + roff[e] + gk;
       } else {
         a = (long)b * op.bs + (OP::kcontig ? (long)gr * op.ld + gk : (long)gk * op.ld + gr);
@@ -247,6 +248,7 @@ struct Stager {
         r = idx % ROWS;
         k = idx / ROWS;
       }
+      if (!EXACT && idx >= TOT) continue;
       sre[k * LDP + r] = (T)re[e];
       if (OP::is_cplx) sim[k * LDP + r] = (T)im[e];
     }

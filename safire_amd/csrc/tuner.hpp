@@ -1,0 +1,158 @@
+// Tile-shape autotuner for the MFMA GEMM engine.
+//
+// Each call site offers a few tile configurations of the same product.  The first time a
+// (site, precision, shape) key is seen outside graph capture, every candidate is run once and
+// then timed twice with HIP events on the context's stream; the fastest is cached for the
+// process and used from then on (also inside captured graphs).  All candidates accumulate over
+// k in the same order, so the choice never changes the result bits.  Tuning needs an
+// idempotent product (output independent of its previous contents); accumulating call sites
+// use the first candidate.  AFQ_TUNE=0 in the environment disables tuning.
+#pragma once
+
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <string>
+
+#include "gemm.hpp"
+#include "ops.hpp"
+
+namespace afq {
+
+class Tuner {
+ public:
+  static Tuner& get() {
+    static Tuner t;
+    return t;
+  }
+  int lookup(const std::string& key) {
+    std::lock_guard<std::mutex> g(mu_);
+    auto it = best_.find(key);
+    return it == best_.end() ? -1 : it->second;
+  }
+  void store(const std::string& key, int idx, float ms) {
+    std::lock_guard<std::mutex> g(mu_);
+    best_[key] = idx;
+    ms_[key] = ms;
+  }
+  bool enabled() const { return enabled_; }
+  std::string report() {
+    std::lock_guard<std::mutex> g(mu_);
+    std::string r;
+    for (auto& kv : best_)
+      r += kv.first + " -> " + std::to_string(kv.second) + " (" + std::to_string(ms_[kv.first]) +
+           " ms)\n";
+    return r;
+  }
+
+ private:
+  Tuner() {
+    const char* e = std::getenv("AFQ_TUNE");
+    enabled_ = !(e && e[0] == '0');
+  }
+  std::mutex mu_;
+  std::map<std::string, int> best_;
+  std::map<std::string, float> ms_;
+  bool enabled_ = true;
+};
+
+template <class F>
+void tuned(Context& c, const std::string& key, int ncand, F&& launch, bool idempotent) {
+  Tuner& t = Tuner::get();
+  int choice = t.lookup(key);
+  if (choice >= 0) {
+    launch(choice);
+    return;
+  }
+  if (ncand <= 1 || !idempotent || !t.enabled() || c.capturing()) {
+    launch(0);
+    return;
+  }
+  hipStream_t s = c.stream();
+  hipEvent_t e0, e1;
+  AFQ_HIP(hipEventCreate(&e0));
+  AFQ_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  int bi = 0;
+  for (int i = 0; i < ncand; ++i) {
+    launch(i);
+    AFQ_HIP(hipEventRecord(e0, s));
+    launch(i);
+    launch(i);
+    AFQ_HIP(hipEventRecord(e1, s));
+    AFQ_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    AFQ_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) {
+      best = ms;
+      bi = i;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  t.store(key, bi, best / 2);
+  launch(bi);
+}
+
+inline std::string tune_key(const char* site, size_t tsize, int batch, int M, int N, int K) {
+  return std::string(site) + (tsize == 8 ? ":f64:" : ":f32:") + std::to_string(batch) + "x" +
+         std::to_string(M) + "x" + std::to_string(N) + "x" + std::to_string(K);
+}
+
+// Large products (vbias, vHS, energy chunks): square-ish 2x2-wave tiles of several sizes.
+template <typename T, class OA, class OB, class EPI>
+void gemm_large(Context& c, const char* site, int batch, int M, int N, int K, const OA& A,
+                const OB& B, const EPI& e, bool idempotent = true) {
+  hipStream_t s = c.stream();
+  tuned(c, tune_key(site, sizeof(T), batch, M, N, K), 4, [&](int i) {
+    switch (i) {
+      case 0: gemm::launch<T, 64, 64, 16, 2, 2>(s, batch, M, N, K, A, B, e); break;
+      case 1: gemm::launch<T, 128, 128, 16, 2, 2>(s, batch, M, N, K, A, B, e); break;
+      case 2: gemm::launch<T, 128, 64, 16, 2, 2>(s, batch, M, N, K, A, B, e); break;
+      default: gemm::launch<T, 64, 128, 16, 2, 2>(s, batch, M, N, K, A, B, e); break;
+    }
+  }, idempotent);
+}
+
+// Tall x narrow per-walker products (orbitals x electrons): BN fitted to the electron count,
+// BM = 64 rows or a full-height 208-row tile (one workgroup per walker when N <= 208).
+template <typename T, int BM, int WM, class OA, class OB, class EPI>
+void tall_bn(hipStream_t s, int batch, int M, int N, int K, const OA& A, const OB& B,
+             const EPI& e) {
+  if (N <= 16)
+    gemm::launch<T, BM, 16, 16, WM, 1>(s, batch, M, N, K, A, B, e);
+  else if (N <= 32)
+    gemm::launch<T, BM, 32, 16, WM, 1>(s, batch, M, N, K, A, B, e);
+  else if (N <= 48)
+    gemm::launch<T, BM, 48, 16, WM, 1>(s, batch, M, N, K, A, B, e);
+  else
+    gemm::launch<T, BM, 64, 16, WM, 1>(s, batch, M, N, K, A, B, e);
+}
+
+template <typename T, class OA, class OB, class EPI>
+void gemm_tall(Context& c, const char* site, int batch, int M, int N, int K, const OA& A,
+               const OB& B, const EPI& e, bool idempotent = true) {
+  hipStream_t s = c.stream();
+  tuned(c, tune_key(site, sizeof(T), batch, M, N, K), 3, [&](int i) {
+    switch (i) {
+      case 0: tall_bn<T, 64, 4>(s, batch, M, N, K, A, B, e); break;
+      case 1: tall_bn<T, 208, 13>(s, batch, M, N, K, A, B, e); break;
+      default: tall_bn<T, 128, 8>(s, batch, M, N, K, A, B, e); break;
+    }
+  }, idempotent);
+}
+
+// Small per-walker products (electrons x electrons, electrons x orbitals).
+template <typename T, class OA, class OB, class EPI>
+void gemm_small(Context& c, const char* site, int batch, int M, int N, int K, const OA& A,
+                const OB& B, const EPI& e, bool idempotent = true) {
+  hipStream_t s = c.stream();
+  tuned(c, tune_key(site, sizeof(T), batch, M, N, K), 2, [&](int i) {
+    switch (i) {
+      case 0: gemm::launch<T, 32, 32, 16, 1, 1>(s, batch, M, N, K, A, B, e); break;
+      default: gemm::launch<T, 48, 48, 16, 1, 1>(s, batch, M, N, K, A, B, e); break;
+    }
+  }, idempotent);
+}
+
+}  // namespace afq
