@@ -193,11 +193,19 @@ int afq_prop_orthogonalize(afq_prop* prop, const afq_walkers* w);
 int afq_prop_set_step(afq_prop* prop, uint32_t step);
 /* NOMSD::Energy (NOMSD.cpp:159-217): E [W][3], ovlp [W] (log overlap). */
 int afq_prop_energy(afq_prop* prop, const afq_walkers* w, int precision, double* E, double* ovlp);
-/* One step + optional orthogonalization + growth estimator (-> *energy_out, device) + Eshift
- * relaxation (relax > 0), captured in a HIP graph on first reuse and replayed afterwards
- * (run_afqmc.cpp:69-100 without population control).  Uses the in-kernel RNG. */
+/* One step + optional orthogonalization + optional comb population control + growth
+ * estimator (-> *energy_out, device) + Eshift relaxation (relax > 0), captured in a HIP graph
+ * on first reuse and replayed afterwards (the loop body of run_afqmc.cpp:69-100).  Uses the
+ * in-kernel RNG for the fields and the comb. */
 int afq_prop_step_graph(afq_prop* prop, const afq_walkers* w, double* Eshift, int ortho,
-                        double relax, double* energy_out);
+                        int popctl, double relax, double* energy_out);
+/* population_control (population_control.hpp:216-257) on this GPU's walkers with the comb's
+ * uniform r in (0, 1): offspring by |w|, copies into the slots of dropped walkers in walker
+ * order, then every weight set to unit magnitude (phase kept).  W <= 16384. */
+int afq_prop_population_control(afq_prop* prop, const afq_walkers* w, double r);
+/* *collapsed = 1 when the last population control found the total weight <= 1e-6 (the
+ * population is then left unchanged; the reference aborts).  Synchronizes the stream. */
+int afq_prop_status(afq_prop* prop, int* collapsed);
 
 #ifdef __cplusplus
 }

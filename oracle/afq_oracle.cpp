@@ -31,6 +31,7 @@
 #include <complex>
 #include <cstdint>
 #include <cstring>
+#include <numeric>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -821,6 +822,60 @@ void orc_step(const orc_problem* P, int W, const double* u, double Eshift, doubl
                     (const double*)mf.data(), (const double*)HW.data(), P->lower_cut, P->upper_cut,
                     0, 0, P->weight_floor, P->weight_frac, P->target_pop, weight_, phase_, eloc_,
                     ovlp_, theta_);
+}
+
+// comb (population_control.hpp:34-64): offspring counts for |w| with the uniform r, and the
+// single-rank copy plan of swap_walkers_async (:140-209): walkers with no offspring free their
+// slots, which are refilled in walker order by the extra copies of the others.
+// src[d] = walker whose copy lands in slot d (src[d] == d for kept walkers).
+// Returns 0, or -1 when the total weight collapsed (<= 1e-6).
+int orc_comb(int W, const double* weight_, double r, int* offspring, int* src) {
+  const cd* weight = (const cd*)weight_;
+  std::vector<double> w(W), cum(W);
+  for (int i = 0; i < W; ++i) w[i] = std::abs(weight[i]);
+  std::partial_sum(w.begin(), w.end(), cum.begin());
+  double total = cum[W - 1];
+  if (!(total > 1e-6)) return -1;
+  int last = W - 1;
+  while (w[last] <= 0.0) --last;
+  for (int i = 0; i < W; ++i) offspring[i] = 0;
+  int iw = 0, ic = 0;
+  while (ic < W) {
+    double comb = (ic + r) * total / W;
+    if (comb < cum[iw] || iw == last) {
+      offspring[iw] += 1;
+      ic++;
+    } else {
+      iw++;
+    }
+  }
+  std::vector<int> free_slots;
+  for (int i = 0; i < W; ++i) {
+    src[i] = i;
+    if (offspring[i] == 0) free_slots.push_back(i);
+  }
+  size_t next = 0;
+  for (int i = 0; i < W; ++i)
+    for (int c = 1; c < offspring[i]; ++c) src[free_slots[next++]] = i;
+  return 0;
+}
+
+// population_control (population_control.hpp:216-257) on one rank: comb, copy every walker row
+// (Slater matrices of both spin blocks and all properties), then weight <- weight / |weight|.
+// rows: W rows of `row_len` complex numbers each (the caller packs a walker into one row).
+int orc_population_control(int W, double r, int row_len, double* rows_, double* weight_) {
+  cd* rows = (cd*)rows_;
+  cd* weight = (cd*)weight_;
+  std::vector<int> off(W), src(W);
+  if (orc_comb(W, weight_, r, off.data(), src.data()) != 0) return -1;
+  for (int d = 0; d < W; ++d)
+    if (src[d] != d) {
+      std::copy(rows + (size_t)src[d] * row_len, rows + (size_t)(src[d] + 1) * row_len,
+                rows + (size_t)d * row_len);
+      weight[d] = weight[src[d]];
+    }
+  for (int i = 0; i < W; ++i) weight[i] = weight[i] / std::abs(weight[i]);
+  return 0;
 }
 
 int orc_num_threads() {

@@ -55,6 +55,8 @@ def lib():
             "orc_vmf": [_P, _P, _I, _I, _I, _D, _D, _P],
             "orc_step": [_P, _I, _P, _D, _P, _P, _P, _P, _P, _P],
             "orc_num_threads": [],
+            "orc_comb": [_I, _P, _D, _P, _P],
+            "orc_population_control": [_I, _D, _I, _P, _P],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -63,6 +65,8 @@ def lib():
         L.orc_probit.restype = _D
         L.orc_average_pseudo_energy.restype = _D
         L.orc_num_threads.restype = _I
+        L.orc_comb.restype = _I
+        L.orc_population_control.restype = _I
         _lib = L
     return _lib
 
@@ -209,6 +213,34 @@ def average_pseudo_energy(weight, eloc, dt):
     weight = _c(weight, np.complex128)
     eloc = _c(eloc, np.complex128)
     return lib().orc_average_pseudo_energy(weight.shape[0], _p(weight), _p(eloc), dt)
+
+
+def comb(weight, r):
+    weight = _c(weight, np.complex128)
+    W = weight.shape[0]
+    off = np.zeros(W, np.int32)
+    src = np.zeros(W, np.int32)
+    rc = lib().orc_comb(W, _p(weight), float(r), _p(off), _p(src))
+    if rc != 0:
+        raise ValueError("total walker weight collapsed")
+    return off, src
+
+
+def population_control(state, r):
+    """Comb resampling of a walker-state dict (Phi, weight, phase, eloc, ovlp, theta) in place."""
+    W = state["weight"].shape[0]
+    keys = ["Phi", "phase", "eloc", "ovlp", "theta"]
+    parts = [np.ascontiguousarray(state[k].reshape(W, -1)) for k in keys]
+    rows = np.ascontiguousarray(np.concatenate(parts, axis=1))
+    w = state["weight"]
+    rc = lib().orc_population_control(W, float(r), rows.shape[1], _p(rows), _p(w))
+    if rc != 0:
+        raise ValueError("total walker weight collapsed")
+    o = 0
+    for k, part in zip(keys, parts):
+        n = part.shape[1]
+        state[k][...] = rows[:, o:o + n].reshape(state[k].shape)
+        o += n
 
 
 # --- setup ---------------------------------------------------------------------------------

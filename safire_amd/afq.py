@@ -86,7 +86,9 @@ _SIGS = {
     "afq_prop_orthogonalize": [_P, ctypes.POINTER(Walkers)],
     "afq_prop_set_step": [_P, ctypes.c_uint32],
     "afq_prop_energy": [_P, ctypes.POINTER(Walkers), _I, _P, _P],
-    "afq_prop_step_graph": [_P, ctypes.POINTER(Walkers), _P, _I, _D, _P],
+    "afq_prop_step_graph": [_P, ctypes.POINTER(Walkers), _P, _I, _I, _D, _P],
+    "afq_prop_population_control": [_P, ctypes.POINTER(Walkers), _D],
+    "afq_prop_status": [_P, ctypes.POINTER(_I)],
 }
 
 _lib = None
@@ -433,9 +435,18 @@ class Propagator:
     def set_step(self, step):
         _ck(lib().afq_prop_set_step(self.h, step))
 
-    def step_graph(self, wset: WalkerSet, Eshift, ortho=False, relax=0.0, energy_out=None):
+    def step_graph(self, wset: WalkerSet, Eshift, ortho=False, relax=0.0, energy_out=None,
+                   popctl=False):
         _ck(lib().afq_prop_step_graph(self.h, ctypes.byref(wset.cstruct()), ptr(Eshift), int(ortho),
-                                      relax, ptr(energy_out)))
+                                      int(popctl), relax, ptr(energy_out)))
+
+    def population_control(self, wset: WalkerSet, r):
+        _ck(lib().afq_prop_population_control(self.h, ctypes.byref(wset.cstruct()), float(r)))
+
+    def collapsed(self) -> bool:
+        v = _I()
+        _ck(lib().afq_prop_status(self.h, ctypes.byref(v)))
+        return bool(v.value)
 
     def close(self):
         if getattr(self, "h", None):

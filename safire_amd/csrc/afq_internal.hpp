@@ -37,6 +37,13 @@ void walker_update(hipStream_t s, int W, const UpdateParams& p, const double* Es
                    const z64* new_ovlp, const z64* mf, const z64* HW, z64* weight, z64* phase,
                    z64* eloc, z64* ovlp, z64* theta);
 void increment_counter(hipStream_t s, uint32_t* c);
+// comb population control of one rank (population_control.hpp:216-257); r_dev = injected
+// uniform or nullptr for Philox(seed, stream, *step_ptr).  cum [W] doubles, src [W] ints and
+// status [1] int are device scratch.
+void population_control(hipStream_t s, int W, const double* r_dev, uint64_t seed, uint32_t stream,
+                        const uint32_t* step_ptr, long rowA, long rowB, z64* PhiA, z64* PhiB,
+                        z64* weight, z64* phase, z64* eloc, z64* ovlp, z64* theta, double* cum,
+                        int* src, int* status);
 // growth estimator log(sum w e^{dt E} / sum w)/dt -> out[0]; Eshift[0] += f (out - Eshift) when
 // relax > 0.
 void average_pseudo_energy(hipStream_t s, int W, const z64* weight, const z64* eloc, double dt,

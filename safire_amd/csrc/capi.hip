@@ -349,11 +349,30 @@ int afq_prop_energy(afq_prop* prop, const afq_walkers* w, int precision, double*
 }
 
 int afq_prop_step_graph(afq_prop* prop, const afq_walkers* w, double* Eshift, int ortho,
-                        double relax, double* energy_out) {
+                        int popctl, double relax, double* energy_out) {
   return guard([&] {
     REQUIRE(prop && Eshift, "NULL argument");
     check_walkers(prop, w);
-    prop->p->step_graph(to_state(w), Eshift, ortho != 0, relax, energy_out);
+    prop->p->step_graph(to_state(w), Eshift, ortho != 0, popctl != 0, relax, energy_out);
+  });
+}
+
+int afq_prop_population_control(afq_prop* prop, const afq_walkers* w, double r) {
+  return guard([&] {
+    REQUIRE(prop, "prop is NULL");
+    check_walkers(prop, w);
+    REQUIRE(r > 0.0 && r < 1.0, "r must be in (0, 1)");
+    double* rd = afq::scratch_as<double>(prop->ham->ctx->c, "capi_comb_r", 1);
+    AFQ_HIP(hipMemcpyAsync(rd, &r, sizeof(double), hipMemcpyHostToDevice, prop->ham->ctx->c.stream()));
+    AFQ_HIP(hipStreamSynchronize(prop->ham->ctx->c.stream()));
+    prop->p->population_control(to_state(w), rd);
+  });
+}
+
+int afq_prop_status(afq_prop* prop, int* collapsed) {
+  return guard([&] {
+    REQUIRE(prop && collapsed, "NULL argument");
+    *collapsed = prop->p->status();
   });
 }
 

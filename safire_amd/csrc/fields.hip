@@ -212,6 +212,103 @@ __global__ void __launch_bounds__(kAvgThreads)
   }
 }
 
+// comb (population_control.hpp:34-64) and the single-rank slot plan of swap_walkers_async
+// (:140-209).  One workgroup: the cumulative |w| is summed left to right by one thread (the
+// reference's std::partial_sum), every tooth (ic + r) * total / W finds its walker by binary
+// search (the first cumulative weight above it, or the last walker with weight), and one
+// thread lays out the copies: walkers without offspring free their slots, refilled in walker
+// order.  src[d] = walker copied into slot d (d itself when kept).  status[0] = 1 when the
+// total weight collapsed (<= 1e-6); the population is then left unchanged.
+constexpr int kCombThreads = 1024;
+
+__global__ void __launch_bounds__(kCombThreads)
+    k_comb_plan(int W, const z64* __restrict__ weight, const double* r_in, uint64_t seed,
+                uint32_t stream, const uint32_t* step_ptr, double* cum, int* src, int* status) {
+  extern __shared__ int off[];  // [W]
+  __shared__ double tot_s, r_s;
+  __shared__ int last_s, ok_s;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < W; i += kCombThreads) off[i] = 0;
+  if (tid == 0) {
+    double acc = 0.0;
+    int last = W - 1;
+    for (int i = 0; i < W; ++i) {
+      acc += hypot(weight[i].re, weight[i].im);
+      cum[i] = acc;
+    }
+    while (last > 0 && hypot(weight[last].re, weight[last].im) <= 0.0) --last;
+    tot_s = acc;
+    last_s = last;
+    ok_s = acc > 1e-6 ? 1 : 0;
+    double r;
+    if (r_in) {
+      r = *r_in;
+    } else {
+      uint32_t c[4] = {0u, 0u, step_ptr ? *step_ptr : 0u, stream};
+      philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+      uint64_t bits = ((uint64_t)c[0] << 32) | c[1];
+      r = ((double)(bits >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    }
+    r_s = r;
+    status[0] = ok_s ? 0 : 1;
+  }
+  __syncthreads();
+  if (!ok_s) {
+    for (int i = tid; i < W; i += kCombThreads) src[i] = i;
+    return;
+  }
+  const double total = tot_s, r = r_s;
+  const int last = last_s;
+  for (int ic = tid; ic < W; ic += kCombThreads) {
+    const double comb = (ic + r) * total / W;
+    int lo = 0, hi = W;  // first j with comb < cum[j]
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (comb < cum[mid]) hi = mid; else lo = mid + 1;
+    }
+    int j = lo < last ? lo : last;
+    atomicAdd(&off[j], 1);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int next = 0;  // scan position for free slots
+    for (int i = 0; i < W; ++i) src[i] = i;
+    for (int i = 0; i < W; ++i)
+      for (int c = 1; c < off[i]; ++c) {
+        while (off[next] != 0) ++next;
+        src[next++] = i;
+      }
+  }
+}
+
+// copy walker rows into the freed slots: Slater blocks and all properties
+__global__ void k_copy_walkers(int W, const int* __restrict__ src, long rowA, long rowB,
+                               z64* PhiA, z64* PhiB, z64* weight, z64* phase, z64* eloc,
+                               z64* ovlp, z64* theta) {
+  const int d = blockIdx.x;
+  const int s = src[d];
+  if (s == d) return;
+  for (long e = threadIdx.x; e < rowA; e += blockDim.x) PhiA[d * rowA + e] = PhiA[s * rowA + e];
+  if (PhiB)
+    for (long e = threadIdx.x; e < rowB; e += blockDim.x) PhiB[d * rowB + e] = PhiB[s * rowB + e];
+  if (threadIdx.x == 0) {
+    weight[d] = weight[s];
+    phase[d] = phase[s];
+    eloc[d] = eloc[s];
+    ovlp[d] = ovlp[s];
+    theta[d] = theta[s];
+  }
+}
+
+// every copy carries its parent's weight; set |w| = 1 keeping the phase
+__global__ void k_unit_weights(int W, const int* status, z64* weight) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W || status[0] != 0) return;
+  z64 w = weight[i];
+  double a = hypot(w.re, w.im);
+  weight[i] = {w.re / a, w.im / a};
+}
+
 __global__ void k_increment(uint32_t* c) {
   if (threadIdx.x == 0) *c += 1u;
 }
@@ -220,6 +317,22 @@ __global__ void k_increment(uint32_t* c) {
 
 void increment_counter(hipStream_t s, uint32_t* c) {
   hipLaunchKernelGGL(k_increment, dim3(1), dim3(64), 0, s, c);
+  AFQ_LAUNCH_CHECK();
+}
+
+void population_control(hipStream_t s, int W, const double* r_dev, uint64_t seed, uint32_t stream,
+                        const uint32_t* step_ptr, long rowA, long rowB, z64* PhiA, z64* PhiB,
+                        z64* weight, z64* phase, z64* eloc, z64* ovlp, z64* theta, double* cum,
+                        int* src, int* status) {
+  if (W <= 0) return;
+  AFQ_CHECK((size_t)W * sizeof(int) <= 65536, kErrShape, "population control supports W <= 16384");
+  hipLaunchKernelGGL(k_comb_plan, dim3(1), dim3(kCombThreads), sizeof(int) * W, s, W, weight, r_dev,
+                     seed, stream, step_ptr, cum, src, status);
+  AFQ_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_copy_walkers, dim3(W), dim3(256), 0, s, W, src, rowA, rowB, PhiA, PhiB,
+                     weight, phase, eloc, ovlp, theta);
+  AFQ_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_unit_weights, dim3(ceil_div(W, 256)), dim3(256), 0, s, W, status, weight);
   AFQ_LAUNCH_CHECK();
 }
 

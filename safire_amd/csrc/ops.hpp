@@ -191,9 +191,15 @@ class Propagator {
   // Device step counter used by in-kernel RNG
   uint32_t* step_counter() { return step_dev_; }
   void set_step(uint32_t step);
-  // capture one step (+ optional orthogonalization) in a HIP graph and replay it
-  void step_graph(const WalkerState& s, const double* Eshift, bool ortho, double relax,
-                  double* energy_out);
+  // comb population control (population_control.hpp:216-257); r_dev = injected uniform (device
+  // scalar) or nullptr to draw it from Philox keyed by the device step counter
+  void population_control(const WalkerState& s, const double* r_dev);
+  // 1 when the last population control found a collapsed total weight (synchronizes)
+  int status();
+  // capture one step (+ optional orthogonalization and population control) in a HIP graph
+  // and replay it
+  void step_graph(const WalkerState& s, const double* Eshift, bool ortho, bool popctl,
+                  double relax, double* energy_out);
   Precision precision() const { return prec_; }
 
  private:
@@ -209,6 +215,7 @@ class Propagator {
   c32* P1c32_ = nullptr;
   bool real_P1_ = false;
   uint32_t* step_dev_ = nullptr;
+  int* status_dev_ = nullptr;
   // graph cache, one executable graph per (walker buffers, ortho, relaxation) key
   struct GraphEntry {
     std::vector<const void*> key;

@@ -44,6 +44,8 @@ Propagator::Propagator(CholeskyOps& ham, int W, const PropagatorParams& p)
     P1_ = dalloc<z64>((size_t)N * N);
     step_dev_ = dalloc<uint32_t>(1);
     AFQ_HIP(hipMemsetAsync(step_dev_, 0, sizeof(uint32_t), s));
+    status_dev_ = dalloc<int>(1);
+    AFQ_HIP(hipMemsetAsync(status_dev_, 0, sizeof(int), s));
     // generateP1: vMF (subtractMF = true), then H1 and P1
     ham.vMF(p_.dt, vMF_);
     ham.one_body_propagator(vMF_, p_.dt, P1_);
@@ -67,7 +69,7 @@ Propagator::Propagator(CholeskyOps& ham, int W, const PropagatorParams& p)
     AFQ_HIP(hipStreamSynchronize(s));
   } catch (...) {
     for (void* q : {(void*)vMF_, (void*)P1_, (void*)P1r_, (void*)P1r32_, (void*)P1c32_,
-                    (void*)step_dev_})
+                    (void*)step_dev_, (void*)status_dev_})
       if (q) (void)hipFree(q);
     throw;
   }
@@ -165,6 +167,23 @@ void Propagator::orthogonalize(const WalkerState& st) {
   log_overlap(st, st.ovlp);
 }
 
+void Propagator::population_control(const WalkerState& st, const double* r_dev) {
+  const int N = ham_.N(), nup = ham_.nup(), ndown = ham_.ndown();
+  const bool coll = ham_.walker_type() == kCollinear && ndown > 0;
+  double* cum = scratch_as<double>(ctx_, "pc_cum", st.W);
+  int* src = scratch_as<int>(ctx_, "pc_src", st.W);
+  afq::population_control(ctx_.stream(), st.W, r_dev, p_.seed, p_.stream + 1000u, step_dev_,
+                          (long)N * nup, coll ? (long)N * ndown : 0, st.Phi, coll ? st.Phi_b : nullptr,
+                          st.weight, st.phase, st.eloc, st.ovlp, st.theta, cum, src, status_dev_);
+}
+
+int Propagator::status() {
+  int h = 0;
+  AFQ_HIP(hipMemcpyAsync(&h, status_dev_, sizeof(int), hipMemcpyDeviceToHost, ctx_.stream()));
+  AFQ_HIP(hipStreamSynchronize(ctx_.stream()));
+  return h;
+}
+
 void Propagator::energy(const WalkerState& st, z64* E, z64* ovlp, Precision prec) {
   hipStream_t s = ctx_.stream();
   const int W = st.W, N = ham_.N(), nup = ham_.nup(), ndown = ham_.ndown(), nel = ham_.nel();
@@ -180,12 +199,12 @@ void Propagator::energy(const WalkerState& st, z64* E, z64* ovlp, Precision prec
   ham_.energy(G, W, E, true, true, true, prec);
 }
 
-void Propagator::step_graph(const WalkerState& st, const double* Eshift, bool ortho, double relax,
-                            double* energy_out) {
+void Propagator::step_graph(const WalkerState& st, const double* Eshift, bool ortho, bool popctl,
+                            double relax, double* energy_out) {
   hipStream_t s = ctx_.stream();
   std::vector<const void*> key = {st.Phi, st.Phi_b, st.weight, st.phase, st.eloc, st.ovlp,
                                   st.theta, Eshift, energy_out, (const void*)(long)st.W,
-                                  (const void*)(long)(ortho ? 1 : 0)};
+                                  (const void*)(long)((ortho ? 1 : 0) | (popctl ? 2 : 0))};
   long rbits;
   static_assert(sizeof(rbits) == sizeof(relax), "");
   std::memcpy(&rbits, &relax, sizeof(rbits));
@@ -193,6 +212,7 @@ void Propagator::step_graph(const WalkerState& st, const double* Eshift, bool or
   auto body = [&]() {
     step_impl(st, nullptr, Eshift);
     if (ortho) orthogonalize(st);
+    if (popctl) population_control(st, nullptr);
     average_pseudo_energy(s, st.W, st.weight, st.eloc, p_.dt, relax, energy_out,
                           const_cast<double*>(Eshift));
     increment_counter(s, step_dev_);

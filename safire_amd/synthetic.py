@@ -70,10 +70,12 @@ class Problem:
         return np.ascontiguousarray(np.broadcast_to(self.trial, (W, self.N, self.nup)))
 
 
-def make_problem(N: int, nup: int, M: int, seed: int) -> Problem:
+def make_problem(N: int, nup: int, M: int, seed: int, lscale: float = 0.5) -> Problem:
+    """The workload recipe; `lscale` (0.5 in every benchmark workload) scales the Cholesky
+    vectors, so smaller values give weaker interactions (used by long-run estimator tests)."""
     rng = np.random.default_rng(seed)
     h1 = _sym(rng, (N, N), 0.1) + np.diag(np.linspace(-2.0, 1.0, N))
-    decay = 0.5 * np.exp(-np.arange(M) / 20.0)
+    decay = lscale * np.exp(-np.arange(M) / 20.0)
     # generated as [M][N][N] (one symmetric L^m per vector), stored as Likn [N][N][M]
     Lm = np.empty((M, N, N))
     chunk = max(1, (1 << 24) // (N * N))

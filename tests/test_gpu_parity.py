@@ -450,3 +450,76 @@ def test_graph_replay_matches_eager(gpu):
         out[mode] = (afq.host(ws.Phi), afq.host(ws.weight), afq.host(Esh))
     for a, b in zip(out["eager"], out["graph"]):
         assert np.array_equal(a, b)
+
+
+def test_population_control_vs_oracle(gpu):
+    afq, ctx, torch = gpu
+    say("case: comb population control N=24 n=5 W=97 (zero weights included)")
+    p = make_problem(24, 5, 12, seed=5)
+    W = 97
+    rng = np.random.default_rng(3)
+    c = afq.Context(0)
+    ham = afq.HamiltonianOperations(c, p.h1, p.Likn, p.PsiT, p.nup)
+    prop = afq.Propagator(ham, W, dt=0.005)
+    st = dict(Phi=random_walkers(p, W, seed=7),
+              weight=rng.exponential(size=W) * np.exp(1j * rng.uniform(0, 6.28, size=W)),
+              phase=np.exp(1j * rng.uniform(0, 6.28, size=W)), eloc=rng.normal(size=W) + 0j,
+              ovlp=rng.normal(size=W) + 1j * rng.normal(size=W), theta=rng.normal(size=W) + 0j)
+    st["weight"][[0, 5, 96]] = 0.0
+    ws = afq.WalkerSet(st["Phi"])
+    for k in ("weight", "phase", "eloc", "ovlp", "theta"):
+        getattr(ws, k).copy_(torch.from_numpy(st[k]))
+    prop.population_control(ws, 0.618)
+    orc.population_control(st, 0.618)
+    assert not prop.collapsed()
+    assert np.array_equal(afq.host(ws.Phi), st["Phi"])
+    for k in ("phase", "eloc", "ovlp", "theta"):
+        assert np.array_equal(afq.host(getattr(ws, k)), st[k]), k
+    assert rel(afq.host(ws.weight), st["weight"]) < 1e-15
+    # a collapsed population is reported and left unchanged
+    ws.weight.zero_()
+    before = afq.host(ws.Phi)
+    prop.population_control(ws, 0.5)
+    assert prop.collapsed()
+    assert np.array_equal(afq.host(ws.Phi), before)
+
+
+def test_graph_with_population_control_matches_eager(gpu):
+    """The captured step with QR and comb equals the eager sequence with the comb's uniform taken
+    from the same Philox counter (stream + 1000, step counter)."""
+    afq, ctx, torch = gpu
+    p = make_problem(24, 5, 96, seed=1001)
+    W = 40
+    say("case: graph step + QR + comb vs eager, cfg1 W=40, 12 steps")
+    out = {}
+    seed = 99
+    for mode in ("eager", "graph"):
+        c = afq.Context(0)
+        ham = afq.HamiltonianOperations(c, p.h1, p.Likn, p.PsiT, p.nup)
+        prop = afq.Propagator(ham, W, dt=0.005, seed=seed)
+        ws = afq.WalkerSet(p.walkers(W))
+        E = afq.empty((W, 3), torch.complex128)
+        ov = afq.empty(W, torch.complex128)
+        prop.Energy(ws, E, ov)
+        ws.eloc.copy_(E.sum(1))
+        ws.ovlp.copy_(ov)
+        Esh = afq.device_tensor(np.array([float(afq.host(E.sum(1)).real.mean())]))
+        e_out = afq.empty(1, torch.float64)
+        relax = 1 - np.exp(-0.1)
+        for step in range(12):
+            ortho, pc = step % 5 == 0, step % 4 == 0
+            if mode == "eager":
+                prop.Propagate(ws, Esh)
+                if ortho:
+                    prop.Orthogonalize(ws)
+                if pc:
+                    ctr = orc.philox4x32_10([0, 0, step, 1 + 1000], [seed & 0xffffffff, seed >> 32])
+                    bits = (int(ctr[0]) << 32) | int(ctr[1])
+                    prop.population_control(ws, ((bits >> 11) + 0.5) * 2.0 ** -53)
+                afq.average_pseudo_energy(c, ws.weight, ws.eloc, 0.005, e_out, relax, Esh)
+            else:
+                prop.step_graph(ws, Esh, ortho=ortho, popctl=pc, relax=relax, energy_out=e_out)
+            torch.cuda.synchronize()
+        out[mode] = (afq.host(ws.Phi), afq.host(ws.weight), afq.host(Esh))
+    for a, b in zip(out["eager"], out["graph"]):
+        assert np.array_equal(a, b)

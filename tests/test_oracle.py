@@ -313,3 +313,33 @@ def test_oracle_steps_stay_finite():
     assert np.all(np.isfinite(state["eloc"]))
     e = orc.average_pseudo_energy(state["weight"], state["eloc"], 0.005)
     assert np.isfinite(e)
+
+
+def test_comb_properties():
+    """Systematic (comb) resampling: the population is preserved, every walker gets floor or ceil
+    of W |w_i| / sum |w| copies, zero-weight walkers get none, and the copy plan fills exactly the
+    slots of the dropped walkers with the extra copies, in walker order."""
+    rng = np.random.default_rng(12)
+    W = 97
+    w = rng.exponential(size=W) * np.exp(1j * rng.uniform(0, 2 * np.pi, size=W))
+    w[[3, 10, 11, 96]] = 0.0
+    for r in (1e-9, 0.37, 1 - 1e-9):
+        off, src = orc.comb(w, r)
+        assert off.sum() == W
+        expect = W * np.abs(w) / np.abs(w).sum()
+        assert np.all(off >= np.floor(expect) - 1e-12) and np.all(off <= np.ceil(expect) + 1e-12)
+        assert np.all(off[np.abs(w) == 0] == 0)
+        free = np.flatnonzero(off == 0)
+        assert np.array_equal(src[off > 0], np.flatnonzero(off > 0))
+        extra = np.concatenate([[i] * (off[i] - 1) for i in range(W) if off[i] > 1] or [[]])
+        assert np.array_equal(src[free], extra.astype(src.dtype))
+    st = dict(Phi=rng.normal(size=(W, 4, 2)) + 0j, weight=w.copy(), phase=np.ones(W, complex),
+              eloc=np.arange(W) + 0j, ovlp=np.zeros(W, complex), theta=np.zeros(W, complex))
+    Phi0, e0 = st["Phi"].copy(), st["eloc"].copy()
+    orc.population_control(st, 0.37)
+    off, src = orc.comb(w, 0.37)
+    assert np.array_equal(st["Phi"], Phi0[src]) and np.array_equal(st["eloc"], e0[src])
+    assert np.allclose(np.abs(st["weight"]), 1.0)
+    assert np.allclose(st["weight"], w[src] / np.abs(w[src]))
+    with pytest.raises(ValueError):
+        orc.comb(np.zeros(5, complex), 0.5)
