@@ -148,6 +148,9 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--only", choices=["vhs", "vhs64", "copy", "step"], default=None,
                     help="profiling modes: launch only the named kernel(s)")
+    ap.add_argument("--serve-jobs", type=int, default=24,
+                    help="jobs of the serving run (Poisson arrivals at 4/s; 200 = the full "
+                         "serving workload; 0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -200,6 +203,19 @@ def main():
         log("energy error mixed vs fp64 (one step, 64 walkers)")
         emax, emed, erel, emag = energy_error(problem)
 
+    serve_summary = None
+    if args.only is None and args.serve_jobs > 0:
+        from safire_amd import serve
+        log(f"serving run: {args.serve_jobs} jobs, Poisson arrivals at 4 jobs/s, 400 mixed steps each")
+        specs = serve.poisson_jobs(args.serve_jobs, rate=4.0)
+        problems = {sp.id: make_problem(sp.N, sp.nup, sp.M, sp.seed, lscale=sp.lscale) for sp in specs}
+        res = serve.Server(max_active=16).run(specs, problems)
+        serve_summary = serve.summarize(res)
+        serve_summary.update({"arrival_rate_per_s": 4.0, "steps_per_job": 400,
+                              "workload": "N in {32..160 step 16}, n=N/4, M=5N, W in [64,512], "
+                                          "Cholesky amplitude 0.1, mixed precision"})
+        log(f"serve: {serve_summary}")
+
     cpu = None
     if not args.skip_cpu and args.only is None:
         log("cpu baseline (FP64 oracle)")
@@ -237,6 +253,7 @@ def main():
         "mixed_over_fp64": None if "fp64" not in out else out["fp64"]["sec"] / sec,
         "energy_error_mixed_vs_fp64_Ha": None if emax is None else
             {"max": emax, "median": emed, "max_rel": erel, "median_abs_E": emag},
+        "serve": serve_summary,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

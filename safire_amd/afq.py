@@ -183,14 +183,16 @@ def shared_stream(device: int = 0):
 
 
 class Context:
-    """afq_ctx on the process-wide stream of `device` (see shared_stream)."""
+    """afq_ctx on the process-wide stream of `device` (see shared_stream), or on an explicit
+    torch stream (the serving loop gives every job its own); torch work on that context's
+    buffers must then be issued on the same stream."""
 
-    def __init__(self, device: int = 0):
+    def __init__(self, device: int = 0, stream=None):
         torch = _torch()
         if not torch.cuda.is_available():
             raise AfqError("no GPU available: the AFQMC step runs only on the device")
         torch.cuda.set_device(device)
-        self.stream = shared_stream(device)
+        self.stream = shared_stream(device) if stream is None else stream
         h = _P()
         _ck(lib().afq_ctx_create(device, ctypes.c_void_p(self.stream.cuda_stream), ctypes.byref(h)))
         self.h = h

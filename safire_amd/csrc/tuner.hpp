@@ -94,9 +94,17 @@ void tuned(Context& c, const std::string& key, int ncand, F&& launch, bool idemp
   launch(bi);
 }
 
+// Shapes are bucketed (each extent rounded up to a power of two) so that products of similar
+// size, e.g. the walker counts of different jobs, share one tuning decision.
+inline int tune_bucket(int x) {
+  int b = 1;
+  while (b < x) b <<= 1;
+  return b;
+}
 inline std::string tune_key(const char* site, size_t tsize, int batch, int M, int N, int K) {
-  return std::string(site) + (tsize == 8 ? ":f64:" : ":f32:") + std::to_string(batch) + "x" +
-         std::to_string(M) + "x" + std::to_string(N) + "x" + std::to_string(K);
+  return std::string(site) + (tsize == 8 ? ":f64:" : ":f32:") + std::to_string(tune_bucket(batch)) +
+         "x" + std::to_string(tune_bucket(M)) + "x" + std::to_string(tune_bucket(N)) + "x" +
+         std::to_string(tune_bucket(K));
 }
 
 // Large products (vbias, vHS, energy chunks): square-ish 2x2-wave tiles of several sizes.
