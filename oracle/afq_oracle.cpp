@@ -37,9 +37,9 @@
 #include <omp.h>
 #endif
 
-typedef std::complex<double> cd;
+#include "orc_dense.hpp"
 
-namespace {
+namespace orc {
 
 const double kPi = 3.14159265358979323846;
 
@@ -259,7 +259,9 @@ void jacobi_eigh(double* A, int n, double* w, double* Vout) {
   std::copy(V.begin(), V.end(), Vout);
 }
 
-}  // namespace
+}  // namespace orc
+
+using namespace orc;
 
 extern "C" {
 

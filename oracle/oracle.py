@@ -57,6 +57,14 @@ def lib():
             "orc_num_threads": [],
             "orc_comb": [_I, _P, _D, _P, _P],
             "orc_population_control": [_I, _D, _I, _P, _P],
+            "orc_ph_excited_1body": [_P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _P],
+            "orc_ph_excited_2body": [_P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P, _P, _P],
+            "orc_phmsd_mdm": [_P, _P, _P, _I, _P, _P],
+            "orc_phmsd_log_overlap": [_P, _P, _P, _I, _P],
+            "orc_phmsd_energy": [_P, _P, _P, _I, _D, _P, _P, _I, _P, _P],
+            "orc_phmsd_gmf": [_P, _P],
+            "orc_msd_brute": [_I, _I, _I, _I, _I, _P, _P, _I, _P, _P, _P, _P, _P, _I, _D, _P, _P,
+                              _I, _P, _P, _P],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -155,7 +163,8 @@ def energy(haj, Lnak, G, nup, ndown=0, closed=True, E0=0.0):
     G = _c(G, np.complex128)
     W = G.shape[0]
     N = haj.shape[-1]
-    M = Lnak.shape[0]
+    # CLOSED: Lnak [M][nup][N]; COLLINEAR: [M][nup][N] followed by [M][ndown][N]
+    M = Lnak.size // ((nup if closed else nup + ndown) * N)
     E = np.empty((W, 3), np.complex128)
     lib().orc_energy(_p(haj), _p(Lnak), _p(G), W, N, nup, ndown, M, int(closed), E0, _p(E))
     return E
@@ -350,3 +359,120 @@ class OracleSystem:
         Phi, _ = orthogonalize(state["Phi"])
         state["Phi"][...] = Phi
         state["ovlp"][...] = 2.0 * log_overlap(self.PsiT, Phi)
+
+
+# --- particle-hole multi-determinant trial (phmsd_oracle.cpp) ------------------------------
+
+class _Ph(ctypes.Structure):
+    _fields_ = [("N", _I), ("nel", _I * 2), ("nact", _I * 2), ("orbs", _P * 2), ("nconf", _I * 2),
+                ("rank", _P * 2), ("exc_start", _P * 2), ("exc", _P * 2), ("refc", _P * 2),
+                ("ndet", _I), ("det_a", _P), ("det_b", _P), ("ci", _P)]
+
+
+class PhTrial:
+    """The oracle's view of a PHMSD trial: `ph` is a safire_amd.phmsd.PhStruct (the excitation
+    numbering), `orbs` the trial orbitals [nact][N] shared by both spins."""
+
+    def __init__(self, ph, orbs):
+        self.ph = ph
+        self.orbs = _c(orbs, np.complex128)
+        self._keep = [self.orbs]
+        k = self._keep
+        arr = lambda a, t=np.int32: (k.append(_c(a, t)), k[-1])[1]
+        s = _Ph()
+        s.N = self.orbs.shape[1]
+        for sp in range(2):
+            s.nel[sp] = ph.nel[sp]
+            s.nact[sp] = self.orbs.shape[0]
+            s.orbs[sp] = _p(self.orbs)
+            s.nconf[sp] = len(ph.rank[sp])
+            s.rank[sp] = _p(arr(ph.rank[sp]))
+            s.exc_start[sp] = _p(arr(ph.exc_start[sp]))
+            s.exc[sp] = _p(arr(ph.exc[sp]))
+            s.refc[sp] = _p(arr(ph.refc[sp]))
+        s.ndet = ph.ndet
+        s.det_a = _p(arr(ph.det_a))
+        s.det_b = _p(arr(ph.det_b))
+        s.ci = _p(arr(ph.ci, np.complex128))
+        self.s = s
+
+    @property
+    def nact(self):
+        return self.orbs.shape[0]
+
+    def mixed_density_matrix(self, Phi_a, Phi_b):
+        Pa, Pb = _c(Phi_a, np.complex128), _c(Phi_b, np.complex128)
+        W, N = Pa.shape[:2]
+        G = np.empty((W, 2 * self.nact, N), np.complex128)
+        ov = np.empty(W, np.complex128)
+        lib().orc_phmsd_mdm(ctypes.byref(self.s), _p(Pa), _p(Pb), W, _p(G), _p(ov))
+        return G, ov
+
+    def log_overlap(self, Phi_a, Phi_b):
+        Pa, Pb = _c(Phi_a, np.complex128), _c(Phi_b, np.complex128)
+        W = Pa.shape[0]
+        ov = np.empty(W, np.complex128)
+        lib().orc_phmsd_log_overlap(ctypes.byref(self.s), _p(Pa), _p(Pb), W, _p(ov))
+        return ov
+
+    def energy(self, haj, Lnak, Phi_a, Phi_b, E0=0.0):
+        """haj [2 nact][N], Lnak [M][nact][N] (alpha) followed by the beta block."""
+        haj = _c(haj, np.complex128)
+        Lnak = _c(Lnak, np.complex128)
+        M = Lnak.size // (2 * self.nact * self.orbs.shape[1])
+        Pa, Pb = _c(Phi_a, np.complex128), _c(Phi_b, np.complex128)
+        W = Pa.shape[0]
+        E = np.empty((W, 3), np.complex128)
+        ov = np.empty(W, np.complex128)
+        lib().orc_phmsd_energy(ctypes.byref(self.s), _p(haj), _p(Lnak), M, E0, _p(Pa), _p(Pb), W,
+                               _p(E), _p(ov))
+        return E, ov
+
+    def gmf(self):
+        G = np.empty((2 * self.nact, self.orbs.shape[1]), np.complex128)
+        lib().orc_phmsd_gmf(ctypes.byref(self.s), _p(G))
+        return G
+
+
+def ph_excited_1body(iexcit, refc, S, R, wgt):
+    iexcit, refc = _c(iexcit, np.int32), _c(refc, np.int32)
+    S, R, wgt = _c(S, np.complex128), _c(R, np.complex128), _c(wgt, np.complex128)
+    nwalk, ndet, nex, nact = R.shape
+    E = np.zeros(nwalk, np.complex128)
+    lib().orc_ph_excited_1body(_p(iexcit), _p(refc), _p(S), _p(R), _p(wgt), nwalk, ndet, nex, nact,
+                               S.shape[1], _p(E))
+    return E
+
+
+def ph_excited_2body(iexcit, refc, Twina, R, wgt):
+    """Twina [nwalk][nelec][nchol][nact], R [nwalk][ndet][nex][nact], wgt [ndet][nwalk] ->
+    (EX, EJ) [nwalk] and KE [ndet][nwalk][nchol]."""
+    iexcit, refc = _c(iexcit, np.int32), _c(refc, np.int32)
+    T, R, wgt = _c(Twina, np.complex128), _c(R, np.complex128), _c(wgt, np.complex128)
+    nwalk, ndet, nex, nact = R.shape
+    nelec, nchol = T.shape[1], T.shape[2]
+    EX = np.zeros(nwalk, np.complex128)
+    EJ = np.zeros(nwalk, np.complex128)
+    KE = np.zeros((ndet, nwalk, nchol), np.complex128)
+    lib().orc_ph_excited_2body(_p(iexcit), _p(refc), _p(T), _p(R), _p(wgt), nwalk, ndet, nex, nact,
+                               nelec, nchol, _p(EX), _p(EJ), _p(KE))
+    return EX, EJ, KE
+
+
+def msd_brute(orbs, occa, occb, ci, haj, Lnak, Phi_a, Phi_b, E0=0.0):
+    """Explicit sum over determinants: G [W][2 nact][N], E [W][3], log overlap [W]."""
+    orbs = _c(orbs, np.complex128)
+    occa, occb = _c(occa, np.int32), _c(occb, np.int32)
+    ci = _c(ci, np.complex128)
+    haj, Lnak = _c(haj, np.complex128), _c(Lnak, np.complex128)
+    Pa, Pb = _c(Phi_a, np.complex128), _c(Phi_b, np.complex128)
+    nact, N = orbs.shape
+    W, _, nup = Pa.shape
+    ndn = Pb.shape[2]
+    M = Lnak.size // (2 * nact * N)
+    G = np.empty((W, 2 * nact, N), np.complex128)
+    E = np.empty((W, 3), np.complex128)
+    ov = np.empty(W, np.complex128)
+    lib().orc_msd_brute(N, nup, ndn, nact, nact, _p(orbs), _p(orbs), len(ci), _p(occa), _p(occb),
+                        _p(ci), _p(haj), _p(Lnak), M, E0, _p(Pa), _p(Pb), W, _p(G), _p(E), _p(ov))
+    return G, E, ov

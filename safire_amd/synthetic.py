@@ -39,6 +39,9 @@ WORKLOADS = {
     "cfg1": Workload("cfg1: FP64 N=24 n=5+5 M=96 W=64", 24, 5, 96, 64, 100, seed=1001),
     "cfg2": Workload("cfg2: FP64 N=108 n=24+24 M=600 W=512", 108, 24, 600, 512, 200, seed=1002),
     "cfg3": Workload("cfg3: mixed N=200 n=40+40 M=1600 W=1024", 200, 40, 1600, 1024, 300, seed=1003),
+    # COLLINEAR, particle-hole multi-determinant trial of 2000 determinants (make_phmsd_problem)
+    "cfg4": Workload("cfg4: PHMSD N=64 n=10+10 M=320 W=256 ndet=2000", 64, 10, 320, 256, 100,
+                     seed=1004),
 }
 
 
@@ -88,6 +91,97 @@ def make_problem(N: int, nup: int, M: int, seed: int, lscale: float = 0.5) -> Pr
     psi = evecs[:, :nup]
     PsiT = np.ascontiguousarray(psi.T.astype(np.complex128))
     return Problem(N=N, nup=nup, M=M, h1=h1, Likn=Likn, PsiT=PsiT)
+
+
+@dataclasses.dataclass
+class PhmsdProblem:
+    """A COLLINEAR problem with a particle-hole multi-determinant trial.  The trial orbitals
+    `orbs` [nact][N] (conjugate-transposed, one set shared by both spins) are the nact lowest
+    eigenvectors of h1; determinant d occupies the sorted active orbitals occa[d] / occb[d]
+    with coefficient ci[d]; determinant 0 is the reference (orbitals 0..nel-1 of each spin)."""
+    N: int
+    nup: int
+    ndn: int
+    M: int
+    h1: np.ndarray
+    Likn: np.ndarray
+    orbs: np.ndarray
+    occa: np.ndarray
+    occb: np.ndarray
+    ci: np.ndarray
+    E0: float = 0.0
+
+    @property
+    def nact(self) -> int:
+        return self.orbs.shape[0]
+
+    @property
+    def PsiT(self) -> np.ndarray:
+        """Rows that half-rotate the hamiltonian: the active orbitals for each spin."""
+        return np.ascontiguousarray(np.concatenate([self.orbs, self.orbs]))
+
+    def walkers(self, W: int, seed: int | None = None, noise: float = 0.0):
+        """(Phi_a, Phi_b) [W][N][nel]: the reference determinant, optionally with a complex
+        perturbation (generic walkers for parity tests)."""
+        ref_a = self.orbs[: self.nup].conj().T
+        ref_b = self.orbs[: self.ndn].conj().T
+        Pa = np.broadcast_to(ref_a, (W, self.N, self.nup)).copy()
+        Pb = np.broadcast_to(ref_b, (W, self.N, self.ndn)).copy()
+        if noise:
+            rng = np.random.default_rng(seed)
+            for P in (Pa, Pb):
+                P += noise * (rng.normal(size=P.shape) + 1j * rng.normal(size=P.shape)) / np.sqrt(self.N)
+        return np.ascontiguousarray(Pa), np.ascontiguousarray(Pb)
+
+
+def make_phmsd_problem(N: int = 64, nel: int = 10, M: int = 320, ndet: int = 2000,
+                       ncore: int = 5, nactive: int = 10, max_rank: int = 4, seed: int = 1004,
+                       lscale: float = 0.5) -> PhmsdProblem:
+    """h1 and L^m by the same recipe as make_problem; the trial's determinants are drawn from
+    the (nel - ncore) electrons per spin in the `nactive` orbitals above the core: the
+    reference plus ndet-1 distinct (alpha, beta) excitations of total rank 1..max_rank
+    (rank drawn uniformly, split between the spins uniformly), coefficients
+    N(0,1) exp(-rank/2), normalised."""
+    base = make_problem(N, nel, M, seed, lscale=lscale)
+    rng = np.random.default_rng(seed + 7)
+    nact = ncore + nactive
+    evals, evecs = np.linalg.eigh(base.h1)
+    orbs = np.ascontiguousarray(evecs[:, :nact].T.astype(np.complex128))
+    occ_act = np.arange(ncore, nel)           # active occupied in the reference
+    vir_act = np.arange(nel, nact)            # active virtual
+    nao, nav = len(occ_act), len(vir_act)
+    ref = tuple(range(nel))
+
+    def excite(r):
+        h = rng.choice(occ_act, size=r, replace=False)
+        p = rng.choice(vir_act, size=r, replace=False)
+        return tuple(sorted((set(ref) - set(h.tolist())) | set(p.tolist())))
+
+    dets = [(ref, ref)]
+    ranks = [0]
+    seen = {(ref, ref)}
+    tries = 0
+    while len(dets) < ndet:
+        tries += 1
+        if tries > 200 * ndet:
+            raise ValueError("could not draw enough distinct determinants")
+        r = int(rng.integers(1, max_rank + 1))
+        lo, hi = max(0, r - min(nao, nav)), min(r, nao, nav)
+        if lo > hi:
+            continue
+        ra = int(rng.integers(lo, hi + 1))
+        key = (excite(ra), excite(r - ra))
+        if key in seen:
+            continue
+        seen.add(key)
+        dets.append(key)
+        ranks.append(r)
+    occa = np.array([d[0] for d in dets], np.int32)
+    occb = np.array([d[1] for d in dets], np.int32)
+    ci = rng.normal(size=ndet) * np.exp(-0.5 * np.asarray(ranks))
+    ci = (ci / np.linalg.norm(ci)).astype(np.complex128)
+    return PhmsdProblem(N=N, nup=nel, ndn=nel, M=M, h1=base.h1, Likn=base.Likn, orbs=orbs,
+                        occa=occa, occb=occb, ci=ci)
 
 
 def workload_problem(key: str) -> tuple[Workload, Problem]:

@@ -1,0 +1,181 @@
+"""Pins the oracle's particle-hole multi-determinant (PHMSD) restatement.
+
+(a) ph_excited_2body against the known answers of the reference's own component test
+    (tests/test_phmsd_components.cpp:35-110, transcribed into golden/phmsd_components.json);
+(b) overlap, mixed density matrix and local energy of the Woodbury/Wick algorithm against the
+    explicit sum over determinants (orc_msd_brute: sorted occupation strings, unsigned
+    coefficients, one single-determinant evaluation per determinant) -- the metamorphic check
+    of the reference's test_phmsd.cpp:274-397, which also pins the excitation numbering and
+    the permutation signs of build_ph_struct;
+(c) the trial density behind vMF against its closed forms: the single-determinant density,
+    and the exact one-body density matrix of a small expansion whose single excitations carry
+    no fermionic sign.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from safire_amd.phmsd import build_ph_struct, excitation
+from safire_amd.synthetic import make_phmsd_problem
+
+GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "phmsd_components.json")))
+
+
+def component_inputs(nex, g=GOLD):
+    """The deterministic inputs of test_phmsd_components.cpp:50-110."""
+    ndet, nelec, nact, nchol, nw = g["ndet"], g["nelec"], g["nact"], g["nchol"], g["nw"]
+    i = np.arange(nw * nelec * nchol * nact)
+    Tna = (np.sin(i) + 1j * np.cos(i * i)).reshape(nw, nelec, nchol, nact)
+    i = np.arange(nw * ndet * nex * nact)
+    R = (np.sin(i * 3) + 1j * np.cos(i + i * i)).reshape(nw, ndet, nex, nact)
+    i = np.arange(ndet * nw)
+    wgt = (np.sin(i * 2) + 1j * np.cos(i - 2 * i * i)).reshape(ndet, nw)
+    iex = np.zeros((ndet, 2 * nex), np.int32)
+    for nd in range(ndet):
+        occ = [0] * nelec
+        for k in range(nex):
+            v = (k * k) % nelec
+            while occ[v]:
+                v = (v + 1) % nelec
+            occ[v] = 1
+            iex[nd, k] = v
+        occ = [0] * (nact - nelec)
+        for k in range(nex):
+            v = (k + k * k) % (nact - nelec)
+            while occ[v]:
+                v = (v + 1) % (nact - nelec)
+            occ[v] = 1
+            iex[nd, k + nex] = v + nelec
+    return iex, np.arange(nelec, dtype=np.int32), Tna, R, wgt
+
+
+@pytest.mark.parametrize("nex", [1, 2, 3, 4, 5])
+def test_golden_ph_excited_2body(nex):
+    iex, refc, Tna, R, wgt = component_inputs(nex)
+    EX, EJ, KE = orc.ph_excited_2body(iex, refc, Tna, R, wgt)
+    ref = GOLD["EXJ_ref"][nex - 1]
+    # utils::Approx default (test_common.hpp): 1e-8 relative
+    assert abs(EX[0] - complex(*ref[0])) <= 1e-8 * abs(complex(*ref[0]))
+    assert abs(EJ[0] - complex(*ref[1])) <= 1e-8 * abs(complex(*ref[1]))
+
+
+def test_excitation_sign():
+    # reference 0 1 2 3, configuration 0 2 3 5: holes at positions 1 (orbital 1) ... replaced in
+    # place -> [0 5 2 3] -> two inversions -> +1
+    h, p, s = excitation([0, 1, 2, 3], [0, 2, 3, 5])
+    assert (h, p, s) == ([1], [5], 1.0)
+    h, p, s = excitation([0, 1, 2, 3], [1, 2, 3, 4])   # [4 1 2 3]: three inversions
+    assert (h, p, s) == ([0], [4], -1.0)
+    h, p, s = excitation([0, 1, 2], [0, 3, 4])          # [0 3 4]: sorted
+    assert (h, p, s) == ([1, 2], [3, 4], 1.0)
+
+
+def spin_blocked_tensors(P):
+    """Half-rotated tensors of a COLLINEAR hamiltonian whose two spins share the trial orbitals:
+    Lnak [M][nact][N] of alpha followed by the same block for beta, haj [2 nact][N]."""
+    L1, h1 = orc.half_rotate(P.orbs, P.Likn, P.h1)
+    return np.concatenate([L1, L1]), np.concatenate([h1, h1])
+
+
+def _problem(seed, ndet, max_rank, ncore=2, nactive=8, nel=6, N=24, M=40):
+    P = make_phmsd_problem(N=N, nel=nel, M=M, ndet=ndet, ncore=ncore, nactive=nactive,
+                           max_rank=max_rank, seed=seed)
+    ph = build_ph_struct(P.occa, P.occb, P.ci)
+    return P, ph, *spin_blocked_tensors(P)
+
+
+def _unsigned_ci(P):
+    return P.ci * (-1.0 if P.ci[0].real < 0 else 1.0)
+
+
+@pytest.mark.parametrize("seed,ndet,max_rank,nactive", [(5, 60, 4, 8), (6, 40, 6, 10), (7, 25, 2, 8)])
+def test_wick_matches_explicit_sum(seed, ndet, max_rank, nactive):
+    P, ph, Lnak, haj = _problem(seed, ndet, max_rank, nactive=nactive)
+    assert ph.max_rank(0) >= min(max_rank, 3) or ph.max_rank(1) >= min(max_rank, 3)
+    tr = orc.PhTrial(ph, P.orbs)
+    Pa, Pb = P.walkers(4, seed=seed, noise=0.3)
+    G, ov = tr.mixed_density_matrix(Pa, Pb)
+    ov2 = tr.log_overlap(Pa, Pb)
+    E, ov3 = tr.energy(haj, Lnak, Pa, Pb, E0=1.25)
+    Gb, Eb, ovb = orc.msd_brute(P.orbs, P.occa, P.occb, _unsigned_ci(P), haj, Lnak, Pa, Pb, E0=1.25)
+    assert np.abs(G - Gb).max() <= 1e-11 * np.abs(Gb).max()
+    assert np.abs(E - Eb).max() <= 1e-11 * np.abs(Eb).max()
+    for o in (ov, ov2, ov3):
+        assert np.abs(np.exp(o - ovb) - 1).max() < 1e-11   # equal modulo 2 pi i
+        assert np.abs(o - ov).max() == 0.0
+
+
+def test_single_determinant_reduces_to_nomsd():
+    P, ph, Lnak, haj = _problem(8, 1, 1)
+    tr = orc.PhTrial(ph, P.orbs)
+    Pa, Pb = P.walkers(3, seed=1, noise=0.4)
+    n = P.nup
+    G, ov = tr.mixed_density_matrix(Pa, Pb)
+    Ga, oa = orc.mixed_density_matrix(P.orbs[:n], Pa)
+    Gb_, ob = orc.mixed_density_matrix(P.orbs[:n], Pb)
+    assert np.abs(G[:, :n] - Ga).max() < 1e-12 and np.abs(G[:, P.nact:P.nact + n] - Gb_).max() < 1e-12
+    assert np.abs(G[:, n:P.nact]).max() == 0.0
+    assert np.abs(np.exp(ov - (oa + ob)) - 1).max() < 1e-12
+    # single-determinant COLLINEAR energy with the stacked half-rotated tensors
+    E, _ = tr.energy(haj, Lnak, Pa, Pb)
+    rows = np.r_[0:n, P.nact:P.nact + n]
+    Lsub = np.concatenate([Lnak[:P.M, :n], Lnak[P.M:, :n]], axis=0)
+    Gs = np.concatenate([Ga, Gb_], axis=1)
+    E1 = orc.energy(haj[rows], Lsub, Gs, n, n, closed=False)
+    assert np.abs(E - E1).max() < 1e-10 * np.abs(E1).max()
+
+
+def _fock_density(dets, norb):
+    """Exact alpha one-body density gamma[x][y] = <Psi| a+_x a_y |Psi> (orthonormal orbitals)
+    of sum_d c_d |alpha_d beta_d>, by occupation-number strings with fermionic signs."""
+    gam = np.zeros((norb, norb), complex)
+    for (a1, b1, c1) in dets:
+        for (a2, b2, c2) in dets:
+            if b1 != b2:
+                continue
+            for y in a2:
+                for x in range(norb):
+                    occ = sorted(a2)
+                    if x != y and x in occ:
+                        continue
+                    sgn = (-1) ** occ.index(y)
+                    occ.remove(y)
+                    pos = sum(1 for o in occ if o < x)
+                    sgn *= (-1) ** pos
+                    new = sorted(occ + [x])
+                    if tuple(new) == tuple(sorted(a1)):
+                        gam[x, y] += np.conj(c1) * c2 * sgn
+    return gam
+
+
+def test_trial_density_closed_forms():
+    P, ph, Lnak, haj = _problem(9, 1, 1)
+    tr = orc.PhTrial(ph, P.orbs)
+    G = tr.gmf()
+    n, na = P.nup, P.nact
+    # one determinant: the occupied orbitals (conjugated rows) for both spins
+    assert np.abs(G[:n] - P.orbs[:n].conj()).max() < 1e-12
+    assert np.abs(G[n:na]).max() == 0.0
+    assert np.abs(G[na:na + n] - P.orbs[:n].conj()).max() < 1e-12
+    # four determinants coupled by the single excitation n-1 -> n in each spin (no occupied
+    # orbital between hole and particle, so every fermionic sign is +1)
+    ref = tuple(range(n))
+    exc = tuple(range(n - 1)) + (n,)
+    c = np.array([0.8, -0.4, 0.3, 0.2]) + 0j
+    c /= np.linalg.norm(c)
+    occa = np.array([ref, exc, ref, exc], np.int32)
+    occb = np.array([ref, ref, exc, exc], np.int32)
+    ph4 = build_ph_struct(occa, occb, c)
+    tr4 = orc.PhTrial(ph4, P.orbs)
+    G4 = tr4.gmf()
+    dets = [(occa[d].tolist(), tuple(occb[d].tolist()), c[d]) for d in range(4)]
+    gam = _fock_density(dets, na)
+    expect = gam.T @ P.orbs.real      # orbitals are real here
+    assert np.abs(P.orbs.imag).max() == 0.0
+    assert np.abs(G4[:na] - expect).max() < 1e-12
+    detsb = [(occb[d].tolist(), tuple(occa[d].tolist()), c[d]) for d in range(4)]
+    gamb = _fock_density(detsb, na)
+    assert np.abs(G4[na:] - gamb.T @ P.orbs.real).max() < 1e-12
