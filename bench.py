@@ -139,6 +139,46 @@ def cpu_baseline(problem, budget_s=15.0):
                       f"oracle, {t_tot:.1f} s"}
 
 
+def phmsd_bench(warmup, steps, energy_reps=5):
+    """The multi-determinant workload: N=64, 10+10 electrons, M=320, 2000 particle-hole
+    determinants (ranks <= 4, 15 active orbitals per spin), 256 COLLINEAR walkers in FP64.
+    Walkers start from the reference determinant with a small seeded perturbation (exactly at
+    the reference every excitation overlap vanishes and the Wick energy terms cancel
+    catastrophically). Returns step and energy timings."""
+    import torch
+    from safire_amd import afq
+    from safire_amd.phmsd import PHMSD, build_ph_struct
+    from safire_amd.synthetic import make_phmsd_problem
+    Wp = 256
+    P = make_phmsd_problem()
+    ph = build_ph_struct(P.occa, P.occb, P.ci)
+    ctx = afq.Context(0)
+    ham = afq.HamiltonianOperations(ctx, P.h1, P.Likn, P.PsiT, P.nact, P.nact,
+                                    walker_type=afq.AFQ_COLLINEAR)
+    trial = PHMSD(ham, ph)
+    prop = afq.Propagator(ham, Wp, dt=DT, order=ORDER, seed=4343, trial=trial)
+    ws = afq.WalkerSet(*P.walkers(Wp, seed=11, noise=0.05))
+    E = afq.empty((Wp, 3), torch.complex128)
+    ov = afq.empty(Wp, torch.complex128)
+    prop.Energy(ws, E, ov)
+    ws.eloc.copy_(E.sum(1))
+    ws.ovlp.copy_(ov)
+    Esh = afq.device_tensor(np.array([float(afq.host(E.sum(1)).real.mean())]))
+    eout = afq.empty(1, torch.float64)
+    sec = time_steps(prop, ws, Esh, eout, warmup, steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(energy_reps):
+        prop.Energy(ws, E, ov)
+    torch.cuda.synchronize()
+    te = (time.perf_counter() - t0) / energy_reps
+    Emean = float(afq.host(E.sum(1)).real.mean())
+    return {"workload": "N=64, 10+10 electrons, M=320, 2000 determinants (ranks <= 4, "
+                        f"{ph.nconf[0]}+{ph.nconf[1]} unique configurations), 256 walkers, FP64",
+            "value": Wp / sec, "unit": "walker-steps/s", "ms_per_step": sec * 1e3,
+            "energy_ms": te * 1e3, "mean_local_energy": Emean}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,7 +186,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=6)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
-    ap.add_argument("--only", choices=["vhs", "vhs64", "copy", "step"], default=None,
+    ap.add_argument("--only", choices=["vhs", "vhs64", "copy", "step", "phmsd"], default=None,
                     help="profiling modes: launch only the named kernel(s)")
     ap.add_argument("--serve-jobs", type=int, default=24,
                     help="jobs of the serving run (Poisson arrivals at 4/s; 200 = the full "
@@ -170,6 +210,10 @@ def main():
                                         precision=prec)
         t = time_vhs(ham, prec, W, reps=args.steps)
         log(f"vHS launch {t * 1e3:.3f} ms")
+        return
+    if args.only == "phmsd":
+        log(f"PHMSD workload: {args.steps} steps after {args.warmup} warmup")
+        log(json.dumps(phmsd_bench(args.warmup, args.steps)))
         return
     if args.only == "copy":
         # calibration kernel of known traffic for the FETCH/WRITE counters: copy 1 GiB
@@ -202,6 +246,12 @@ def main():
     if args.only is None:
         log("energy error mixed vs fp64 (one step, 64 walkers)")
         emax, emed, erel, emag = energy_error(problem)
+
+    phmsd = None
+    if args.only is None:
+        log("PHMSD workload (2000 determinants, 256 walkers, FP64)")
+        phmsd = phmsd_bench(args.warmup, min(args.steps, 30))
+        log(f"phmsd: {phmsd}")
 
     serve_summary = None
     if args.only is None and args.serve_jobs > 0:
@@ -253,6 +303,7 @@ def main():
         "mixed_over_fp64": None if "fp64" not in out else out["fp64"]["sec"] / sec,
         "energy_error_mixed_vs_fp64_Ha": None if emax is None else
             {"max": emax, "median": emed, "max_rel": erel, "median_abs_E": emag},
+        "phmsd": phmsd,
         "serve": serve_summary,
         "cpu_baseline": cpu,
     }

@@ -53,6 +53,7 @@ enum { AFQ_FP64 = 0, AFQ_MIXED = 1 };                /* compute precision */
 typedef struct afq_ctx afq_ctx;
 typedef struct afq_ham afq_ham;
 typedef struct afq_prop afq_prop;
+typedef struct afq_phmsd afq_phmsd;
 
 /* ---- context ------------------------------------------------------------------------------ */
 
@@ -180,6 +181,9 @@ typedef struct {
 
 /* AFQMCBasePropagator constructor (AFQMCBasePropagator.h:45-171): builds vMF and P1. */
 int afq_prop_create(afq_ham* ham, int W, const afq_prop_params* p, afq_prop** out);
+/* The same propagator over a PHMSD trial (Wavefunction<PHMSD>, COLLINEAR): vMF from the trial
+ * density, G for vbias, overlaps and energies from the multi-determinant expansion. */
+int afq_prop_create_phmsd(afq_phmsd* trial, int W, const afq_prop_params* p, afq_prop** out);
 int afq_prop_destroy(afq_prop* prop);
 /* Device views: vMF complex [M], P1 complex [N][N]. */
 int afq_prop_views(afq_prop* prop, const double** vMF, const double** P1);
@@ -191,7 +195,8 @@ int afq_prop_step(afq_prop* prop, const afq_walkers* w, const double* u, const d
 int afq_prop_orthogonalize(afq_prop* prop, const afq_walkers* w);
 /* Sets the device step counter used by the in-kernel RNG. */
 int afq_prop_set_step(afq_prop* prop, uint32_t step);
-/* NOMSD::Energy (NOMSD.cpp:159-217): E [W][3], ovlp [W] (log overlap). */
+/* NOMSD::Energy (NOMSD.cpp:159-217) or, for a PHMSD propagator, PHMSD::Energy (energy_alg1,
+ * PHMSD.cpp:376-544, FP64): E [W][3], ovlp [W] (log overlap). */
 int afq_prop_energy(afq_prop* prop, const afq_walkers* w, int precision, double* E, double* ovlp);
 /* One step + optional orthogonalization + optional comb population control + growth
  * estimator (-> *energy_out, device) + Eshift relaxation (relax > 0), captured in a HIP graph
@@ -206,6 +211,48 @@ int afq_prop_population_control(afq_prop* prop, const afq_walkers* w, double r);
 /* *collapsed = 1 when the last population control found the total weight <= 1e-6 (the
  * population is then left unchanged; the reference aborts).  Synchronizes the stream. */
 int afq_prop_status(afq_prop* prop, int* collapsed);
+
+/* ---- particle-hole multi-determinant trial (PHMSD, COLLINEAR walkers) ----------------------- */
+
+/* The expansion in the ph_excitations numbering (Excitations.hpp:203-530, as build_ph_struct
+ * makes it, readWfn.cpp:238-340). HOST pointers, copied at creation. Per spin s:
+ *   nconf[s] unique configurations, configuration 0 = the reference, the rest ordered by rank;
+ *   rank[s][u]; configuration u >= 1 is 2 rank ints at exc[s][exc_start[s][u]]: the hole
+ *   positions in the reference, then the particle active orbitals;
+ *   refc[s] [nel[s]] the reference occupation, which must be 0 .. nel-1 (woodbury algorithm).
+ * Determinant d couples configurations det_a[d], det_b[d] with coefficient ci[d] (complex,
+ * permutation sign included). Ranks up to 6. */
+typedef struct {
+  int nel[2];
+  int nconf[2];
+  const int* rank[2];
+  const int* exc_start[2];
+  const int* exc[2];
+  const int* refc[2];
+  int ndet;
+  const int* det_a;
+  const int* det_b;
+  const double* ci;
+} afq_ph_excitations;
+
+/* PHMSD constructor (PHMSD.hpp:63-115). `ham` must be COLLINEAR with the trial's active
+ * orbitals as its PsiT rows (nup = nact_alpha, ndown = nact_beta), as Wavefunction.cpp:459-474
+ * builds it from PsiT_MO. */
+int afq_phmsd_create(afq_ham* ham, const afq_ph_excitations* ex, afq_phmsd** out);
+int afq_phmsd_destroy(afq_phmsd* trial);
+/* PHMSD::MixedDensityMatrix, compact (PHMSD.cpp:549-790): Phi_a [W][N][nel_a], Phi_b
+ * [W][N][nel_b] -> G [W][nact_a + nact_b][N], ovlp [W] = log overlap (PHMSD::vbias feeds this G
+ * to afq_vbias). */
+int afq_phmsd_mixed_density_matrix(afq_phmsd* trial, const double* Phi_a, const double* Phi_b,
+                                   int W, double* G, double* ovlp);
+/* PHMSD::Log_Overlap (PHMSD.cpp:804-900): ovlp [W]. */
+int afq_phmsd_log_overlap(afq_phmsd* trial, const double* Phi_a, const double* Phi_b, int W,
+                          double* ovlp);
+/* PHMSD::Energy, energy_alg1 (PHMSD.cpp:376-544): E [W][3] = (E1 + E0, EXX, EJ), ovlp [W]. */
+int afq_phmsd_energy(afq_phmsd* trial, const double* Phi_a, const double* Phi_b, int W, double* E,
+                     double* ovlp);
+/* PHMSD::vMF (PHMSD.cpp:1329-1573) + zero_imag (AFQMCBasePropagator.cpp:105-108): vMF [M]. */
+int afq_phmsd_vmf(afq_phmsd* trial, double dt, double* vMF);
 
 #ifdef __cplusplus
 }

@@ -89,6 +89,13 @@ _SIGS = {
     "afq_prop_step_graph": [_P, ctypes.POINTER(Walkers), _P, _I, _I, _D, _P],
     "afq_prop_population_control": [_P, ctypes.POINTER(Walkers), _D],
     "afq_prop_status": [_P, ctypes.POINTER(_I)],
+    "afq_prop_create_phmsd": [_P, _I, ctypes.POINTER(PropParams), ctypes.POINTER(_P)],
+    "afq_phmsd_create": [_P, _P, ctypes.POINTER(_P)],
+    "afq_phmsd_destroy": [_P],
+    "afq_phmsd_mixed_density_matrix": [_P, _P, _P, _I, _P, _P],
+    "afq_phmsd_log_overlap": [_P, _P, _P, _I, _P],
+    "afq_phmsd_energy": [_P, _P, _P, _I, _P, _P],
+    "afq_phmsd_vmf": [_P, _D, _P],
 }
 
 _lib = None
@@ -397,15 +404,24 @@ class WalkerSet:
 
 
 class Propagator:
+    """AFQMCBasePropagator over the hamiltonian's single-determinant trial, or over a PHMSD
+    trial (safire_amd.phmsd.PHMSD) built on the same hamiltonian."""
+
     def __init__(self, ham: HamiltonianOperations, W, dt=0.005, order=6, vbias_bound=50.0,
-                 seed=0, stream=1, target_pop=0.0, free_projection=False):
+                 seed=0, stream=1, target_pop=0.0, free_projection=False, trial=None):
         self.ham = ham
         self.ctx = ham.ctx
         self.dt = dt
+        self.trial = trial
         p = PropParams(dt, order, vbias_bound, 1.0, 10.0, 100.0, 0.1, target_pop,
                        int(free_projection), 0, 0, seed, stream)
         h = _P()
-        _ck(lib().afq_prop_create(ham.h, W, ctypes.byref(p), ctypes.byref(h)))
+        if trial is None:
+            _ck(lib().afq_prop_create(ham.h, W, ctypes.byref(p), ctypes.byref(h)))
+        else:
+            if trial.ham is not ham:
+                raise AfqError("the trial was built over another hamiltonian")
+            _ck(lib().afq_prop_create_phmsd(trial.h, W, ctypes.byref(p), ctypes.byref(h)))
         self.h = h
         v, P = _P(), _P()
         _ck(lib().afq_prop_views(h, ctypes.byref(v), ctypes.byref(P)))

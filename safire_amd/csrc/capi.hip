@@ -16,6 +16,10 @@ struct afq_ham {
   afq_ctx* ctx;
   afq::CholeskyOps* ops;
 };
+struct afq_phmsd {
+  afq_ham* ham;
+  afq::PhmsdTrial* t;
+};
 struct afq_prop {
   afq_ham* ham;
   afq::Propagator* p;
@@ -273,26 +277,44 @@ int afq_average_pseudo_energy(afq_ctx* ctx, int W, const double* weight, const d
   });
 }
 
+static afq::PropagatorParams to_params(const afq_prop_params* p) {
+  afq::PropagatorParams pp;
+  pp.dt = p->dt;
+  pp.order = p->order;
+  pp.vbias_bound = p->vbias_bound;
+  pp.lower_cut = p->lower_cut;
+  pp.upper_cut = p->upper_cut;
+  pp.weight_floor = p->weight_floor;
+  pp.weight_frac = p->weight_frac;
+  pp.target_pop = p->target_pop;
+  pp.free_projection = p->free_projection;
+  pp.use_cp = p->use_cp;
+  pp.project_force_bias = p->project_force_bias;
+  pp.seed = p->seed;
+  pp.stream = p->stream;
+  return pp;
+}
+
 int afq_prop_create(afq_ham* ham, int W, const afq_prop_params* p, afq_prop** out) {
   return guard([&] {
     REQUIRE(ham && p && out, "NULL argument");
-    afq::PropagatorParams pp;
-    pp.dt = p->dt;
-    pp.order = p->order;
-    pp.vbias_bound = p->vbias_bound;
-    pp.lower_cut = p->lower_cut;
-    pp.upper_cut = p->upper_cut;
-    pp.weight_floor = p->weight_floor;
-    pp.weight_frac = p->weight_frac;
-    pp.target_pop = p->target_pop;
-    pp.free_projection = p->free_projection;
-    pp.use_cp = p->use_cp;
-    pp.project_force_bias = p->project_force_bias;
-    pp.seed = p->seed;
-    pp.stream = p->stream;
     auto* pr = new afq_prop{ham, nullptr};
     try {
-      pr->p = new afq::Propagator(*ham->ops, W, pp);
+      pr->p = new afq::Propagator(*ham->ops, W, to_params(p));
+    } catch (...) {
+      delete pr;
+      throw;
+    }
+    *out = pr;
+  });
+}
+
+int afq_prop_create_phmsd(afq_phmsd* trial, int W, const afq_prop_params* p, afq_prop** out) {
+  return guard([&] {
+    REQUIRE(trial && p && out, "NULL argument");
+    auto* pr = new afq_prop{trial->ham, nullptr};
+    try {
+      pr->p = new afq::Propagator(*trial->ham->ops, W, to_params(p), trial->t);
     } catch (...) {
       delete pr;
       throw;
@@ -373,6 +395,77 @@ int afq_prop_status(afq_prop* prop, int* collapsed) {
   return guard([&] {
     REQUIRE(prop && collapsed, "NULL argument");
     *collapsed = prop->p->status();
+  });
+}
+
+/* ---- PHMSD ---- */
+
+int afq_phmsd_create(afq_ham* ham, const afq_ph_excitations* ex, afq_phmsd** out) {
+  return guard([&] {
+    REQUIRE(ham && ex && out, "NULL argument");
+    afq::PhExcitations e;
+    for (int s = 0; s < 2; ++s) {
+      e.nel[s] = ex->nel[s];
+      e.nconf[s] = ex->nconf[s];
+      e.rank[s] = ex->rank[s];
+      e.exc_start[s] = ex->exc_start[s];
+      e.exc[s] = ex->exc[s];
+      e.refc[s] = ex->refc[s];
+    }
+    e.ndet = ex->ndet;
+    e.det_a = ex->det_a;
+    e.det_b = ex->det_b;
+    e.ci = Z(ex->ci);
+    auto* t = new afq_phmsd{ham, nullptr};
+    try {
+      t->t = new afq::PhmsdTrial(*ham->ops, e);
+    } catch (...) {
+      delete t;
+      throw;
+    }
+    *out = t;
+  });
+}
+
+int afq_phmsd_destroy(afq_phmsd* trial) {
+  return guard([&] {
+    if (!trial) return;
+    delete trial->t;
+    delete trial;
+  });
+}
+
+int afq_phmsd_mixed_density_matrix(afq_phmsd* trial, const double* Phi_a, const double* Phi_b,
+                                   int W, double* G, double* ovlp) {
+  return guard([&] {
+    REQUIRE(trial && Phi_a && Phi_b && G && ovlp, "NULL argument");
+    REQUIRE(W > 0, "W must be positive");
+    trial->t->mixed_density_matrix(Z(Phi_a), Z(Phi_b), W, Z(G), Z(ovlp));
+  });
+}
+
+int afq_phmsd_log_overlap(afq_phmsd* trial, const double* Phi_a, const double* Phi_b, int W,
+                          double* ovlp) {
+  return guard([&] {
+    REQUIRE(trial && Phi_a && Phi_b && ovlp, "NULL argument");
+    REQUIRE(W > 0, "W must be positive");
+    trial->t->log_overlap(Z(Phi_a), Z(Phi_b), W, Z(ovlp));
+  });
+}
+
+int afq_phmsd_energy(afq_phmsd* trial, const double* Phi_a, const double* Phi_b, int W, double* E,
+                     double* ovlp) {
+  return guard([&] {
+    REQUIRE(trial && Phi_a && Phi_b && E && ovlp, "NULL argument");
+    REQUIRE(W > 0, "W must be positive");
+    trial->t->energy(Z(Phi_a), Z(Phi_b), W, Z(E), Z(ovlp));
+  });
+}
+
+int afq_phmsd_vmf(afq_phmsd* trial, double dt, double* vMF) {
+  return guard([&] {
+    REQUIRE(trial && vMF, "NULL argument");
+    trial->t->vMF(dt, Z(vMF));
   });
 }
 

@@ -147,6 +147,79 @@ class CholeskyOps {
 };
 
 // ---------------------------------------------------------------------------------------------
+// Particle-hole multi-determinant trial (PHMSD, COLLINEAR walkers)
+// ---------------------------------------------------------------------------------------------
+
+// Host description of the expansion in the reference's ph_excitations numbering
+// (Excitations.hpp:203-530, built by readWfn.cpp:238-340): per spin, unique configurations with
+// the reference first and the rest ordered by rank; configuration u of rank r >= 1 is 2r ints at
+// exc[s][exc_start[s][u]] (hole positions, then particle active orbitals). Determinant d couples
+// alpha configuration det_a[d] and beta configuration det_b[d] with coefficient ci[d].
+struct PhExcitations {
+  int nel[2];
+  int nconf[2];
+  const int* rank[2];
+  const int* exc_start[2];
+  const int* exc[2];
+  const int* refc[2];
+  int ndet;
+  const int* det_a;
+  const int* det_b;
+  const z64* ci;
+};
+
+// PHMSD (PHMSD.hpp/.cpp) over a COLLINEAR CholeskyOps whose trial rows are the active orbitals of
+// each spin (ham.nup() = nact_alpha, ham.ndown() = nact_beta), as the reference builds its
+// HamiltonianOperations from PsiT_MO (Wavefunction.cpp:459-474). The woodbury algorithm
+// (refc[i] == i) is the one implemented.
+class PhmsdTrial {
+ public:
+  static constexpr int kMaxRank = 6;
+  PhmsdTrial(CholeskyOps& ham, const PhExcitations& ex);
+  ~PhmsdTrial();
+  int nup() const { return sp_[0].nel; }
+  int ndn() const { return sp_[1].nel; }
+  CholeskyOps& ham() const { return ham_; }
+  // compact G [W][nact_a + nact_b][N] and log overlap Ov (PHMSD::MixedDensityMatrix, compact)
+  void mixed_density_matrix(const z64* Pa, const z64* Pb, int W, z64* G, z64* Ov);
+  // PHMSD::Log_Overlap
+  void log_overlap(const z64* Pa, const z64* Pb, int W, z64* Ov);
+  // PHMSD::Energy (energy_alg1): E [W][3], Ov [W]
+  void energy(const z64* Pa, const z64* Pb, int W, z64* E, z64* Ov);
+  // PHMSD::vMF followed by the charge-decomposition zero_imag
+  void vMF(double dt, z64* v);
+
+ private:
+  struct SpinDev {
+    int nel = 0, nact = 0, nvir = 0, nconf = 0, rmax = 0, P = 0, P2 = 0;
+    std::vector<int> rank_off;   // host: configurations of rank r are [rank_off[r], rank_off[r+1])
+    std::vector<int> rank_h, exc_h;  // host copies: rank, exc [nconf][2 kMaxRank]
+    int* exc = nullptr;          // device [nconf][2 kMaxRank]: holes then particles
+    int* rank = nullptr;         // device [nconf]
+    int* mstart = nullptr;       // device [nconf] offset of Minv of configuration u (r^2 each)
+    int* hptr = nullptr;         // device [nel + 1] CSR of (u, p) entries by hole position
+    int* hent = nullptr;         // device: u * 8 + p
+    int* cptr = nullptr;         // device [nconf + 1] couplings to the other spin
+    int* ccol = nullptr;
+    z64* cval = nullptr;         // conj(ci)
+    z64* Lext = nullptr;         // device [(M + 1)][nact][N]: half-rotated L then haj
+    const z64* orbs = nullptr;   // device [nact][N] (rows of the hamiltonian's trial)
+  };
+  CholeskyOps& ham_;
+  Context& ctx_;
+  SpinDev sp_[2];
+  int ndet_ = 0;
+  int* det_ab_ = nullptr;        // device [ndet][2]
+  z64* det_c_ = nullptr;         // device [ndet] conj(ci)
+  std::vector<int> det_ab_h_;
+  std::vector<z64> det_c_h_;
+  void woodbury(int s, const z64* Phi, int W, bool want_G, z64* log_ov);
+  void ratios(int s, int W, bool want_inverse);
+  void weights(int W, z64* Ov);
+  void free_all();
+};
+
+// ---------------------------------------------------------------------------------------------
 // Propagator: one hybrid step of the whole walker set on the device
 // ---------------------------------------------------------------------------------------------
 struct PropagatorParams {
@@ -175,7 +248,9 @@ struct WalkerState {
 
 class Propagator {
  public:
-  Propagator(CholeskyOps& ham, int W, const PropagatorParams& p);
+  // trial == nullptr: single determinant (the hamiltonian's PsiT rows, NOMSD); otherwise the
+  // PHMSD trial built over `ham` supplies G, overlaps, energies and vMF.
+  Propagator(CholeskyOps& ham, int W, const PropagatorParams& p, PhmsdTrial* trial = nullptr);
   ~Propagator();
   const z64* vMF() const { return vMF_; }
   const z64* P1() const { return P1_; }
@@ -205,7 +280,10 @@ class Propagator {
  private:
   CholeskyOps& ham_;
   Context& ctx_;
+  PhmsdTrial* trial_;
   int W_;
+  int wup_, wdn_;   // electrons per spin of the walkers (columns of Phi, Phi_b)
+  bool closed_;
   PropagatorParams p_;
   Precision prec_;
   z64* vMF_ = nullptr;

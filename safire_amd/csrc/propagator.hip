@@ -32,9 +32,13 @@ T* dalloc(size_t count) {
 
 }  // namespace
 
-Propagator::Propagator(CholeskyOps& ham, int W, const PropagatorParams& p)
-    : ham_(ham), ctx_(ham.ctx()), W_(W), p_(p), prec_((Precision)ham.precision()) {
+Propagator::Propagator(CholeskyOps& ham, int W, const PropagatorParams& p, PhmsdTrial* trial)
+    : ham_(ham), ctx_(ham.ctx()), trial_(trial), W_(W),
+      wup_(trial ? trial->nup() : ham.nup()),
+      wdn_(trial ? trial->ndn() : ham.ndown()),
+      closed_(ham.walker_type() == kClosed), p_(p), prec_((Precision)ham.precision()) {
   AFQ_CHECK(W > 0, kErrArg, "W must be positive");
+  AFQ_CHECK(!trial || &trial->ham() == &ham, kErrArg, "the trial belongs to another hamiltonian");
   AFQ_CHECK(p.order >= 0, kErrArg, "order");
   if (p_.target_pop <= 0) p_.target_pop = W;
   hipStream_t s = ctx_.stream();
@@ -47,7 +51,10 @@ Propagator::Propagator(CholeskyOps& ham, int W, const PropagatorParams& p)
     status_dev_ = dalloc<int>(1);
     AFQ_HIP(hipMemsetAsync(status_dev_, 0, sizeof(int), s));
     // generateP1: vMF (subtractMF = true), then H1 and P1
-    ham.vMF(p_.dt, vMF_);
+    if (trial_)
+      trial_->vMF(p_.dt, vMF_);
+    else
+      ham.vMF(p_.dt, vMF_);
     ham.one_body_propagator(vMF_, p_.dt, P1_);
     std::vector<z64> h((size_t)N * N);
     AFQ_HIP(hipMemcpyAsync(h.data(), P1_, sizeof(z64) * h.size(), hipMemcpyDeviceToHost, s));
@@ -82,8 +89,8 @@ Propagator::~Propagator() {
     (void)hipGraphDestroy(g.graph);
   }
   for (void* q : {(void*)vMF_, (void*)P1_, (void*)P1r_, (void*)P1r32_, (void*)P1c32_,
-                  (void*)step_dev_})
-    if (q) hipFree(q);
+                  (void*)step_dev_, (void*)status_dev_})
+    if (q) (void)hipFree(q);
 }
 
 void Propagator::set_step(uint32_t step) {
@@ -92,9 +99,13 @@ void Propagator::set_step(uint32_t step) {
 }
 
 void Propagator::log_overlap(const WalkerState& st, z64* ovlp) {
+  if (trial_) {
+    trial_->log_overlap(st.Phi, st.Phi_b, st.W, ovlp);
+    return;
+  }
   hipStream_t s = ctx_.stream();
   const int N = ham_.N(), nup = ham_.nup(), ndown = ham_.ndown();
-  const bool closed = ham_.walker_type() == kClosed;
+  const bool closed = closed_;
   fill_z64(s, st.W, {0, 0}, ovlp);
   det_ops::log_overlap(ctx_, ham_.PsiT(), nup, N, st.Phi, (long)N * nup, st.W, ovlp,
                        closed ? 2.0 : 1.0);
@@ -106,9 +117,9 @@ void Propagator::log_overlap(const WalkerState& st, z64* ovlp) {
 
 void Propagator::step_impl(const WalkerState& st, const double* u, const double* Eshift) {
   hipStream_t s = ctx_.stream();
-  const int W = st.W, N = ham_.N(), M = ham_.M(), nup = ham_.nup(), ndown = ham_.ndown();
-  const int nel = ham_.nel();
-  const bool closed = ham_.walker_type() == kClosed;
+  const int W = st.W, N = ham_.N(), M = ham_.M(), nup = wup_, ndown = wdn_;
+  const int nel = ham_.nel();  // rows of G: electrons (NOMSD) or active orbitals (PHMSD)
+  const bool closed = closed_;
   AFQ_CHECK(W <= W_, kErrShape, "walker count exceeds the propagator's capacity");
   z64* G = scratch_as<z64>(ctx_, "st_G", (size_t)W * nel * N);
   z64* ov = scratch_as<z64>(ctx_, "st_ov", W);
@@ -119,13 +130,18 @@ void Propagator::step_impl(const WalkerState& st, const double* u, const double*
   z64* nov = scratch_as<z64>(ctx_, "st_nov", W);
   void* V = ctx_.scratch("st_V", (size_t)W * N * N * (prec_ == kMixed ? sizeof(c32) : sizeof(z64)));
 
-  // 1. G and vbias (NOMSD::vbias)
-  fill_z64(s, W, {0, 0}, ov);
-  det_ops::mixed_density_matrix(ctx_, ham_.PsiT(), nup, N, st.Phi, (long)N * nup, W, G,
-                                (long)nel * N, ov, true, closed ? 2.0 : 1.0);
-  if (!closed && ndown > 0)
-    det_ops::mixed_density_matrix(ctx_, ham_.PsiT() + (size_t)nup * N, ndown, N, st.Phi_b,
-                                  (long)N * ndown, W, G + (size_t)nup * N, (long)nel * N, ov, true);
+  // 1. G and vbias (NOMSD::vbias, NOMSD.cpp:39-61; PHMSD::vbias, PHMSD.cpp:53-72)
+  if (trial_) {
+    trial_->mixed_density_matrix(st.Phi, st.Phi_b, W, G, ov);
+  } else {
+    fill_z64(s, W, {0, 0}, ov);
+    det_ops::mixed_density_matrix(ctx_, ham_.PsiT(), nup, N, st.Phi, (long)N * nup, W, G,
+                                  (long)nel * N, ov, true, closed ? 2.0 : 1.0);
+    if (!closed && ndown > 0)
+      det_ops::mixed_density_matrix(ctx_, ham_.PsiT() + (size_t)nup * N, ndown, N, st.Phi_b,
+                                    (long)N * ndown, W, G + (size_t)nup * N, (long)nel * N, ov,
+                                    true);
+  }
   ham_.vbias(G, W, p_.dt, X, prec_);
   // 2. fields, hybrid weight, mean-field factor
   FieldParams fp{p_.vbias_bound, (p_.use_cp || p_.project_force_bias) ? 1 : 0, p_.free_projection,
@@ -158,18 +174,18 @@ void Propagator::step(const WalkerState& st, const double* u, const double* Eshi
 
 void Propagator::orthogonalize(const WalkerState& st) {
   hipStream_t s = ctx_.stream();
-  const int N = ham_.N(), nup = ham_.nup(), ndown = ham_.ndown();
+  const int N = ham_.N(), nup = wup_, ndown = wdn_;
   z64* ldet = scratch_as<z64>(ctx_, "or_ldet", st.W);
   fill_z64(s, st.W, {0, 0}, ldet);
   det_ops::orthogonalize(ctx_, st.Phi, (long)N * nup, st.W, N, nup, ldet);
-  if (ham_.walker_type() == kCollinear && ndown > 0)
+  if (!closed_ && ndown > 0)
     det_ops::orthogonalize(ctx_, st.Phi_b, (long)N * ndown, st.W, N, ndown, ldet);
   log_overlap(st, st.ovlp);
 }
 
 void Propagator::population_control(const WalkerState& st, const double* r_dev) {
-  const int N = ham_.N(), nup = ham_.nup(), ndown = ham_.ndown();
-  const bool coll = ham_.walker_type() == kCollinear && ndown > 0;
+  const int N = ham_.N(), nup = wup_, ndown = wdn_;
+  const bool coll = !closed_ && ndown > 0;
   double* cum = scratch_as<double>(ctx_, "pc_cum", st.W);
   int* src = scratch_as<int>(ctx_, "pc_src", st.W);
   afq::population_control(ctx_.stream(), st.W, r_dev, p_.seed, p_.stream + 1000u, step_dev_,
@@ -185,9 +201,13 @@ int Propagator::status() {
 }
 
 void Propagator::energy(const WalkerState& st, z64* E, z64* ovlp, Precision prec) {
+  if (trial_) {  // PHMSD::Energy (energy_alg1), FP64
+    trial_->energy(st.Phi, st.Phi_b, st.W, E, ovlp);
+    return;
+  }
   hipStream_t s = ctx_.stream();
   const int W = st.W, N = ham_.N(), nup = ham_.nup(), ndown = ham_.ndown(), nel = ham_.nel();
-  const bool closed = ham_.walker_type() == kClosed;
+  const bool closed = closed_;
   z64* G = scratch_as<z64>(ctx_, "en_G", (size_t)W * nel * N);
   fill_z64(s, W, {0, 0}, ovlp);
   det_ops::mixed_density_matrix(ctx_, ham_.PsiT(), nup, N, st.Phi, (long)N * nup, W, G,

@@ -116,3 +116,97 @@ def build_ph_struct(occa, occb, ci) -> PhStruct:
     return PhStruct(nel=(occa.shape[1], occb.shape[1]), refc=refc, rank=tuple(rank),
                     exc_start=tuple(exc_start), exc=tuple(exc), det_a=det_idx[0].copy(),
                     det_b=det_idx[1].copy(), ci=ci)
+
+
+# ---------------------------------------------------------------------------------------------
+# device trial (C ABI: afq_phmsd_*)
+# ---------------------------------------------------------------------------------------------
+
+import ctypes  # noqa: E402
+
+from . import afq  # noqa: E402
+
+
+class PhExcitations(ctypes.Structure):
+    _fields_ = [("nel", ctypes.c_int * 2), ("nconf", ctypes.c_int * 2),
+                ("rank", ctypes.c_void_p * 2), ("exc_start", ctypes.c_void_p * 2),
+                ("exc", ctypes.c_void_p * 2), ("refc", ctypes.c_void_p * 2),
+                ("ndet", ctypes.c_int), ("det_a", ctypes.c_void_p), ("det_b", ctypes.c_void_p),
+                ("ci", ctypes.c_void_p)]
+
+
+class PHMSD:
+    """PHMSD<DEVICE_MEMORY> for COLLINEAR walkers (PHMSD.hpp/.cpp): `ham` must be a COLLINEAR
+    HamiltonianOperations whose trial rows are the active orbitals of both spins
+    (PhmsdProblem.PsiT, nup = ndown = nact), as Wavefunction.cpp:459-474 builds it."""
+
+    def __init__(self, ham, ph: PhStruct):
+        if ham.walker_type != afq.AFQ_COLLINEAR:
+            raise afq.AfqError("PHMSD requires COLLINEAR walkers")
+        self.ham = ham
+        self.ctx = ham.ctx
+        self.ph = ph
+        self.nup, self.ndn = ph.nel
+        keep = []
+
+        def arr(a, t=np.int32):
+            a = np.ascontiguousarray(a, dtype=t)
+            keep.append(a)
+            return a.ctypes.data_as(ctypes.c_void_p)
+
+        ex = PhExcitations()
+        for s in range(2):
+            ex.nel[s] = ph.nel[s]
+            ex.nconf[s] = len(ph.rank[s])
+            ex.rank[s] = arr(ph.rank[s])
+            ex.exc_start[s] = arr(ph.exc_start[s])
+            ex.exc[s] = arr(ph.exc[s])
+            ex.refc[s] = arr(ph.refc[s])
+        ex.ndet = ph.ndet
+        ex.det_a = arr(ph.det_a)
+        ex.det_b = arr(ph.det_b)
+        ex.ci = arr(ph.ci, np.complex128)
+        h = ctypes.c_void_p()
+        afq._ck(afq.lib().afq_phmsd_create(ham.h, ctypes.byref(ex), ctypes.byref(h)))
+        self.h = h
+
+    @property
+    def nact(self):
+        return (self.ham.nup, self.ham.ndown)
+
+    def MixedDensityMatrix(self, wset, G, Ov):
+        afq._ck(afq.lib().afq_phmsd_mixed_density_matrix(self.h, afq.ptr(wset.Phi),
+                                                          afq.ptr(wset.Phi_b), wset.W,
+                                                          afq.ptr(G), afq.ptr(Ov)))
+
+    def Log_Overlap(self, wset, Ov):
+        afq._ck(afq.lib().afq_phmsd_log_overlap(self.h, afq.ptr(wset.Phi), afq.ptr(wset.Phi_b),
+                                                 wset.W, afq.ptr(Ov)))
+
+    def Energy(self, wset, E, Ov):
+        afq._ck(afq.lib().afq_phmsd_energy(self.h, afq.ptr(wset.Phi), afq.ptr(wset.Phi_b), wset.W,
+                                            afq.ptr(E), afq.ptr(Ov)))
+
+    def vbias(self, wset, v, dt):
+        """PHMSD::vbias (PHMSD.cpp:53-72): compact multi-determinant G, then the
+        hamiltonian's vbias."""
+        torch = afq._torch()
+        N = self.ham.N
+        G = torch.empty((wset.W, sum(self.nact), N), dtype=torch.complex128, device="cuda")
+        Ov = torch.empty(wset.W, dtype=torch.complex128, device="cuda")
+        self.MixedDensityMatrix(wset, G, Ov)
+        self.ham.vbias(G, v, dt)
+
+    def vMF(self, v, dt):
+        afq._ck(afq.lib().afq_phmsd_vmf(self.h, dt, afq.ptr(v)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            afq.lib().afq_phmsd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -91,9 +91,11 @@ def _unsigned_ci(P):
     return P.ci * (-1.0 if P.ci[0].real < 0 else 1.0)
 
 
-@pytest.mark.parametrize("seed,ndet,max_rank,nactive", [(5, 60, 4, 8), (6, 40, 6, 10), (7, 25, 2, 8)])
-def test_wick_matches_explicit_sum(seed, ndet, max_rank, nactive):
-    P, ph, Lnak, haj = _problem(seed, ndet, max_rank, nactive=nactive)
+@pytest.mark.parametrize("seed,ndet,max_rank,nactive,ncore,nel",
+                         [(5, 60, 4, 8, 2, 6), (6, 40, 6, 10, 2, 6), (7, 25, 2, 8, 2, 6),
+                          (8, 50, 6, 14, 1, 8)])
+def test_wick_matches_explicit_sum(seed, ndet, max_rank, nactive, ncore, nel):
+    P, ph, Lnak, haj = _problem(seed, ndet, max_rank, nactive=nactive, ncore=ncore, nel=nel)
     assert ph.max_rank(0) >= min(max_rank, 3) or ph.max_rank(1) >= min(max_rank, 3)
     tr = orc.PhTrial(ph, P.orbs)
     Pa, Pb = P.walkers(4, seed=seed, noise=0.3)
