@@ -36,8 +36,14 @@ def flops_per_walker_step(N, n, M, K=ORDER, k_ortho=ORTHO):
     prop = 8 * (K + 2) * N * N * n
     ov = 8 * n * n * N + (8 / 3) * n ** 3
     qr = 16 * n * n * N / k_ortho
-    vhs = 4 * N * N * M
+    vhs = vhs_flops_per_walker(N, M)
     return g + vb + prop + ov + qr + vhs
+
+
+def vhs_flops_per_walker(N, M):
+    """vHS of one walker: every L^m is symmetric, so V is symmetric and its N(N+1)/2 distinct
+    entries each take M complex x real MACs (4 FLOP each)."""
+    return 2.0 * N * (N + 1) * M
 
 
 def log(msg):
@@ -273,7 +279,7 @@ def main():
 
     sec = out["mixed"]["sec"]
     fstep = flops_per_walker_step(N, NUP, M)
-    vhs_flops = 4.0 * N * N * M * W
+    vhs_flops = vhs_flops_per_walker(N, M) * W
     tv = out["mixed"]["vhs"]
     achieved = vhs_flops / tv
     line = {
