@@ -284,6 +284,139 @@ __global__ void __launch_bounds__(64)
   }
 }
 
+// Register variant for 16 < n <= 64: one wavefront per matrix, lane r holds row r in registers.
+// Rows are never moved: each lane tracks the position its row would occupy in getrf's swapped
+// order, the pivot of step j is the unused row of largest |re|+|im| in column j with ties going
+// to the lowest position (getrf's first maximum), and the row at position j trades positions
+// with it. Gauss-Jordan then eliminates column j from every other row with the pivot row
+// broadcast through LDS, so the only dynamically indexed register is column j. The log det
+// takes log(+-U_jj + DBL_MIN) with the sign flipped exactly on getrf's swapped steps; the
+// in-place result Y gives A^{-1}[k][c] = Y[p_k][s_c] (p_k = pivot row of step k, s_c = step at
+// which row c pivoted), applied while storing.
+__device__ inline double rdl(double v, int src) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __builtin_amdgcn_readlane(lo, src);
+  hi = __builtin_amdgcn_readlane(hi, src);
+  return __hiloint2double(hi, lo);
+}
+
+template <int NMAX>
+__global__ void __launch_bounds__(64)
+    k_lu_rows(const z64* __restrict__ A, long a_bs, int n, z64* __restrict__ Ainv, long i_bs,
+              z64* __restrict__ ovlp, double ov_scale, double ov_add_re, double ov_add_im,
+              int want_inverse, int no_pivot) {
+  __shared__ int pivrow[64];
+  __shared__ double2 prow[NMAX];
+  const int lane = threadIdx.x;
+  const long b = blockIdx.x;
+  const z64* Ab = A + b * a_bs;
+  double xr[NMAX], xi[NMAX];
+#pragma unroll
+  for (int c = 0; c < NMAX; ++c) {
+    z64 v = (lane < n && c < n) ? Ab[(long)lane * n + c] : z64{0.0, 0.0};
+    xr[c] = v.re;
+    xi[c] = v.im;
+  }
+  bool used = lane >= n;
+  int pos = lane;         // position of my row in getrf's order
+  int mystep = n;         // step at which my row pivoted
+  double ur = 1.0, ui = 0.0;  // my pivot value with getrf's sign (+-U_jj)
+  bool singular = false;
+  for (int j = 0; j < n; ++j) {
+    double fr = 0.0, fi = 0.0;  // my row's entry in column j
+#pragma unroll
+    for (int c = 0; c < NMAX; ++c)
+      if (c == j) { fr = xr[c]; fi = xi[c]; }
+    // pivot: max |re|+|im| over unused rows, ties to the lowest position
+    // (a NaN entry ranks below every number but above used rows, so the reduction stays
+    // well defined and uniform)
+    double mag = fabs(fr) + fabs(fi);
+    double best = used ? -1.0 : (mag == mag ? mag : -0.5);
+    int bpos = used ? 1 << 20 : pos;
+    if (no_pivot) {
+      best = (pos == j) ? 1.0 : -1.0;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      double ob = __shfl_xor(best, off, 64);
+      int op = __shfl_xor(bpos, off, 64);
+      if (ob > best || (ob == best && op < bpos)) { best = ob; bpos = op; }
+    }
+    // the pivot row is the lane at position bpos; the row at position j moves to bpos's slot
+    const unsigned long long owner = __ballot(!used && pos == bpos);
+    const int p = __builtin_amdgcn_readfirstlane(owner ? __ffsll(owner) - 1 : j);
+    const bool swapped = bpos != j;
+    if (lane == p) {
+      pos = j;
+    } else if (!used && pos == j) {
+      pos = bpos;
+    }
+    const double dr = rdl(fr, p), di = rdl(fi, p);
+    if (lane == p) {
+      used = true;
+      mystep = j;
+      ur = swapped ? -dr : dr;
+      ui = swapped ? -di : di;
+    }
+    if (lane == 0) pivrow[j] = p;
+    if (dr == 0.0 && di == 0.0) {
+      singular = true;
+      continue;  // uniform
+    }
+    const double den = dr * dr + di * di;
+    const double ir = dr / den, ii = -di / den;
+    // h = g / d with g = d - 1 on the pivot row (which becomes prow / d), f elsewhere
+    const double gr = (lane == p) ? dr - 1.0 : fr, gi = (lane == p) ? di : fi;
+    const double hr = gr * ir - gi * ii, hi = gr * ii + gi * ir;
+    // column j of the result: 1/d on the pivot row, -f/d elsewhere
+    const double njr = (lane == p) ? ir : -(fr * ir - fi * ii);
+    const double nji = (lane == p) ? ii : -(fr * ii + fi * ir);
+    // the pivot row goes through LDS (one writer lane, broadcast reads)
+    if (lane == p) {
+#pragma unroll
+      for (int c = 0; c < NMAX; ++c) prow[c] = make_double2(xr[c], xi[c]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < NMAX; ++c) {
+      const double2 pv = prow[c];
+      double nr = xr[c] - (hr * pv.x - hi * pv.y);
+      double ni = xi[c] - (hr * pv.y + hi * pv.x);
+      if (c == j) { nr = njr; ni = nji; }
+      xr[c] = nr;
+      xi[c] = ni;
+    }
+    __syncthreads();
+  }
+  if (ovlp != nullptr) {
+    double lr = 0.0, li = 0.0;
+    if (lane < n) clog(ur + 2.2250738585072014e-308, ui, lr, li);
+    lr = wave_sum(lr);
+    li = wave_sum(li);
+    if (lane == 0) {
+      z64 o = ovlp[b];
+      double re = o.re + lr, im = o.im + li;
+      if (im > kPi) {
+        while (im > kPi) im -= 2 * kPi;
+      } else if (im < -kPi) {
+        while (im < -kPi) im += 2 * kPi;
+      }
+      ovlp[b] = {ov_scale * re + ov_add_re, ov_scale * im + ov_add_im};
+    }
+  }
+  if (!want_inverse) return;
+  __syncthreads();
+  z64* Ib = Ainv + b * i_bs;
+  if (lane < n && mystep < n) {
+#pragma unroll
+    for (int m = 0; m < NMAX; ++m) {
+      const int col = m < n ? pivrow[m] : -1;
+      if (col >= 0 && col < n)
+        Ib[(long)mystep * n + col] = singular ? z64{0.0, 0.0} : z64{xr[m], xi[m]};
+    }
+  }
+}
+
 // Workgroup variant (256 threads): the matrix in LDS, every elimination step spread over all
 // n^2 entries (pivot search by wavefront 0, row swap, pivot-row scaling, rank-1 update), one
 // barrier between phases.  Same pivot rule and log det as k_lu_inverse_logdet.
@@ -550,16 +683,19 @@ void lu_inverse_logdet(hipStream_t s, const z64* A, long a_bs, int nb, int n, z6
 #define AFQ_LU_REG(NM)                                                                        \
   hipLaunchKernelGGL(k_lu_reg<NM>, dim3(nb), dim3(64), 0, s, A, a_bs, n, Ainv, i_bs, ovlp,     \
                      ov_scale, ov_add.re, ov_add.im, wi)
+#define AFQ_LU_ROWS(NM, NP)                                                                    \
+  hipLaunchKernelGGL(k_lu_rows<NM>, dim3(nb), dim3(64), 0, s, A, a_bs, n, Ainv, i_bs, ovlp,    \
+                     ov_scale, ov_add.re, ov_add.im, wi, NP)
   if (n <= 8)
     AFQ_LU_REG(8);
   else if (n <= 16)
     AFQ_LU_REG(16);
-  else {
-    size_t lds = sizeof(double2) * ((size_t)n * (n + 1) + n);
-    AFQ_CHECK(lds <= 65536, kErrShape, "matrix too large for one workgroup's LDS");
-    hipLaunchKernelGGL(k_lu_blk, dim3(nb), dim3(kLuThreads), lds, s, A, a_bs, n, Ainv, i_bs,
-                       ovlp, ov_scale, ov_add.re, ov_add.im, wi, 0);
-  }
+  else if (n <= 32)
+    AFQ_LU_ROWS(32, 0);
+  else if (n <= 48)
+    AFQ_LU_ROWS(48, 0);
+  else
+    AFQ_LU_ROWS(64, 0);
 #undef AFQ_LU_REG
   AFQ_LAUNCH_CHECK();
 }
@@ -573,8 +709,16 @@ void chol_upper_inverse(hipStream_t s, const z64* S, int nb, int n, z64* Rinv, z
   // the inversion's log det is sum_i log R_ii (R_ii real and positive).
   hipLaunchKernelGGL(k_chol_blk, dim3(nb), dim3(kLuThreads), lds, s, S, n, Rinv);
   AFQ_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_lu_blk, dim3(nb), dim3(kLuThreads), lds, s, Rinv, (long)n * n, n, Rinv,
-                     (long)n * n, ldet, 1.0, 0.0, 0.0, 1, 1);
+  const long nn = (long)n * n;
+  if (n <= 32)
+    hipLaunchKernelGGL(k_lu_rows<32>, dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet, 1.0,
+                       0.0, 0.0, 1, 1);
+  else if (n <= 48)
+    hipLaunchKernelGGL(k_lu_rows<48>, dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet, 1.0,
+                       0.0, 0.0, 1, 1);
+  else
+    hipLaunchKernelGGL(k_lu_rows<64>, dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet, 1.0,
+                       0.0, 0.0, 1, 1);
   AFQ_LAUNCH_CHECK();
 }
 
