@@ -180,10 +180,10 @@ void CholeskyOps::vbias(const z64* G, int W, double dt, z64* v, Precision prec) 
       EpiStore<double, double> e{v, M_, 0, a, 0, acc};
       if (real_trial_) {
         Opnd<double, false, true> B{Lnak_r_ + loff, K, 0};
-        gemm_large<double>(ctx_, "vbias_r", 1, W, M_, (int)K, A, B, e, acc == 0);
+        gemm_large_splitk<double>(ctx_, "vbias_r", W, M_, (int)K, A, B, e, acc == 0);
       } else {
         Opnd<double, true, true> B{&(Lnak_ + loff)->re, K, 0};
-        gemm_large<double>(ctx_, "vbias_c", 1, W, M_, (int)K, A, B, e, acc == 0);
+        gemm_large_splitk<double>(ctx_, "vbias_c", W, M_, (int)K, A, B, e, acc == 0);
       }
     } else {
       AFQ_CHECK(prec_ == kMixed, kErrState, "FP32 tensors were not built (precision FP64)");
@@ -191,10 +191,10 @@ void CholeskyOps::vbias(const z64* G, int W, double dt, z64* v, Precision prec) 
       EpiStore<float, double> e{v, M_, 0, (float)a, 0.f, acc};
       if (real_trial_) {
         Opnd<float, false, true> B{Lnak_r32_ + loff, K, 0};
-        gemm_large<float>(ctx_, "vbias_r", 1, W, M_, (int)K, A, B, e, acc == 0);
+        gemm_large_splitk<float>(ctx_, "vbias_r", W, M_, (int)K, A, B, e, acc == 0);
       } else {
         Opnd<float, true, true> B{&(Lnak32_ + loff)->re, K, 0};
-        gemm_large<float>(ctx_, "vbias_c", 1, W, M_, (int)K, A, B, e, acc == 0);
+        gemm_large_splitk<float>(ctx_, "vbias_c", W, M_, (int)K, A, B, e, acc == 0);
       }
     }
   }

@@ -189,7 +189,7 @@ __global__ void __launch_bounds__(64)
     xr[r] = v.re;
     xi[r] = v.im;
   }
-  double ldr = 0.0, ldi = 0.0;
+  double ur = 1.0, ui = 0.0;  // pivot of step `lane` with getrf's sign, logged after the loop
   bool singular = false;
   for (int j = 0; j < n; ++j) {
     double best = -1.0;
@@ -220,12 +220,9 @@ __global__ void __launch_bounds__(64)
     for (int r = 0; r < NMAX; ++r)
       if (r == j) { mjr = xr[r]; mji = xi[r]; }
     const double dr = bcast(mjr, j), di = bcast(mji, j);
-    {
-      double sr = (p == j) ? dr : -dr, si = (p == j) ? di : -di;
-      double lr, li;
-      clog(sr + 2.2250738585072014e-308, si, lr, li);
-      ldr += lr;
-      ldi += li;
+    if (lane == j) {
+      ur = (p == j) ? dr : -dr;
+      ui = (p == j) ? di : -di;
     }
     if (dr == 0.0 && di == 0.0) {
       singular = true;
@@ -252,15 +249,21 @@ __global__ void __launch_bounds__(64)
       }
     }
   }
-  if (lane == 0 && ovlp != nullptr) {
-    z64 o = ovlp[b];
-    double re = o.re + ldr, im = o.im + ldi;
-    if (im > kPi) {
-      while (im > kPi) im -= 2 * kPi;
-    } else if (im < -kPi) {
-      while (im < -kPi) im += 2 * kPi;
+  if (ovlp != nullptr) {
+    double ldr = 0.0, ldi = 0.0;
+    if (lane < n) clog(ur + 2.2250738585072014e-308, ui, ldr, ldi);
+    ldr = wave_sum(ldr);
+    ldi = wave_sum(ldi);
+    if (lane == 0) {
+      z64 o = ovlp[b];
+      double re = o.re + ldr, im = o.im + ldi;
+      if (im > kPi) {
+        while (im > kPi) im -= 2 * kPi;
+      } else if (im < -kPi) {
+        while (im < -kPi) im += 2 * kPi;
+      }
+      ovlp[b] = {ov_scale * re + ov_add_re, ov_scale * im + ov_add_im};
     }
-    ovlp[b] = {ov_scale * re + ov_add_re, ov_scale * im + ov_add_im};
   }
   if (!want_inverse) return;
   __syncthreads();

@@ -255,9 +255,22 @@ struct Stager {
   }
 };
 
+// Partial products of a split-K launch: slice ks of batch b goes to
+// ws[(b * ksplit + ks) * M * N + m * N + n] (the kernel hands the epilogue b * ksplit + ks).
+template <typename T>
+struct EpiPartial {
+  cplx<T>* ws;
+  long mn;   // M * N
+  int ldn;   // N
+  __device__ inline void operator()(int bk, int m, int n, T re, T im) const {
+    ws[(long)bk * mn + (long)m * ldn + n] = {re, im};
+  }
+};
+
 template <typename T, int BM, int BN, int BK, int WAVES_M, int WAVES_N, class OA, class OB, class EPI>
 __global__ void __launch_bounds__(WAVES_M * WAVES_N * 64)
-    kgemm(int M, int N, int K, int tiles_m, int tiles_n, OA A, OB B, EPI epi) {
+    kgemm(int M, int N, int K, int tiles_m, int tiles_n, int ksplit, int kchunk, OA A, OB B,
+          EPI epi) {
   constexpr int NT = WAVES_M * WAVES_N * 64;
   constexpr int WM = BM / WAVES_M, WN = BN / WAVES_N;
   constexpr int RB = WM / 16, CB = WN / 16;
@@ -277,8 +290,13 @@ __global__ void __launch_bounds__(WAVES_M * WAVES_N * 64)
   long bid = blockIdx.x;
   const int tn = bid % tiles_n;
   const int tm = (bid / tiles_n) % tiles_m;
-  const int b = (int)(bid / ((long)tiles_n * tiles_m));
+  const long rest = bid / ((long)tiles_n * tiles_m);
+  const int ks = (int)(rest % ksplit);
+  const int b = (int)(rest / ksplit);
   const int m0 = tm * BM, n0 = tn * BN;
+  // k range of this block: [kb, ke)
+  const int kb = ks * kchunk;
+  const int ke = min(K, kb + kchunk);
 
   V4 accr[RB][CB], acci[RB][CB];
 #pragma unroll
@@ -293,18 +311,18 @@ __global__ void __launch_bounds__(WAVES_M * WAVES_N * 64)
   Stager<T, OB, BN, BK, NT, LDB> gb;
   ga.init_rows(A, m0, M, tid);
   gb.init_rows(B, n0, N, tid);
-  ga.load(A, b, m0, 0, M, K, tid);
-  gb.load(B, b, n0, 0, N, K, tid);
+  ga.load(A, b, m0, kb, M, ke, tid);
+  gb.load(B, b, n0, kb, N, ke, tid);
 
   const int fr = lane & 15, fk = lane >> 4;
-  for (int k0 = 0; k0 < K; k0 += BK) {
+  for (int k0 = kb; k0 < ke; k0 += BK) {
     __syncthreads();
     ga.store(sAr, sAi, tid);
     gb.store(sBr, sBi, tid);
     __syncthreads();
-    if (k0 + BK < K) {
-      ga.load(A, b, m0, k0 + BK, M, K, tid);
-      gb.load(B, b, n0, k0 + BK, N, K, tid);
+    if (k0 + BK < ke) {
+      ga.load(A, b, m0, k0 + BK, M, ke, tid);
+      gb.load(B, b, n0, k0 + BK, N, ke, tid);
     }
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
@@ -346,8 +364,27 @@ __global__ void __launch_bounds__(WAVES_M * WAVES_N * 64)
       for (int r = 0; r < 4; ++r) {
         int m = m0 + wm0 + i * 16 + Acc<T>::row(fk, r);
         int n = n0 + wn0 + j * 16 + fr;
-        if (m < M && n < N) epi(b, m, n, accr[i][j][r], acci[i][j][r]);
+        if (m < M && n < N) epi(b * ksplit + ks, m, n, accr[i][j][r], acci[i][j][r]);
       }
+}
+
+// Sum of the split-K partial slices in slice order, then the final epilogue.
+template <typename T, class EPI>
+__global__ void __launch_bounds__(256)
+    kreduce_splitk(int batch, int M, int N, int ksplit, const cplx<T>* __restrict__ ws, EPI epi) {
+  const long mn = (long)M * N;
+  long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (long)batch * mn) return;
+  const int b = (int)(t / mn);
+  const long e = t % mn;
+  const cplx<T>* p = ws + (long)b * ksplit * mn + e;
+  T sr = 0, si = 0;
+  for (int k = 0; k < ksplit; ++k) {
+    cplx<T> v = p[(long)k * mn];
+    sr += v.re;
+    si += v.im;
+  }
+  epi(b, (int)(e / N), (int)(e % N), sr, si);
 }
 
 // Host launcher.  batch = number of independent products (grid = batch * tiles).
@@ -359,7 +396,34 @@ inline void launch(hipStream_t s, int batch, int M, int N, int K, const OA& A, c
   long blocks = (long)batch * tm * tn;
   AFQ_CHECK(blocks < (1L << 31), kErrShape, "grid too large");
   hipLaunchKernelGGL((kgemm<T, BM, BN, BK, WAVES_M, WAVES_N, OA, OB, EPI>), dim3((unsigned)blocks),
-                     dim3(WAVES_M * WAVES_N * 64), 0, s, M, N, K, tm, tn, A, B, epi);
+                     dim3(WAVES_M * WAVES_N * 64), 0, s, M, N, K, tm, tn, 1, K > 0 ? K : 1, A, B,
+                     epi);
+  AFQ_LAUNCH_CHECK();
+}
+
+// Split-K launch: K cut into ksplit chunks of kchunk (a multiple of BK) computed by separate
+// workgroups into ws (ksplit * batch * M * N complex), then summed in chunk order and passed to
+// the epilogue. The summation order depends only on kchunk, not on the tile.
+template <typename T, int BM, int BN, int BK, int WAVES_M, int WAVES_N, class OA, class OB, class EPI>
+inline void launch_splitk(hipStream_t s, int batch, int M, int N, int K, int ksplit, int kchunk,
+                          cplx<T>* ws, const OA& A, const OB& B, const EPI& epi) {
+  if (batch <= 0 || M <= 0 || N <= 0) return;
+  if (ksplit <= 1) {
+    launch<T, BM, BN, BK, WAVES_M, WAVES_N>(s, batch, M, N, K, A, B, epi);
+    return;
+  }
+  AFQ_CHECK(kchunk % BK == 0 && (long)kchunk * ksplit >= K, kErrArg, "bad split-K chunking");
+  int tm = ceil_div(M, BM), tn = ceil_div(N, BN);
+  long blocks = (long)batch * tm * tn * ksplit;
+  AFQ_CHECK(blocks < (1L << 31), kErrShape, "grid too large");
+  EpiPartial<T> ep{ws, (long)M * N, N};
+  hipLaunchKernelGGL((kgemm<T, BM, BN, BK, WAVES_M, WAVES_N, OA, OB, EpiPartial<T>>),
+                     dim3((unsigned)blocks), dim3(WAVES_M * WAVES_N * 64), 0, s, M, N, K, tm, tn,
+                     ksplit, kchunk, A, B, ep);
+  AFQ_LAUNCH_CHECK();
+  long tot = (long)batch * M * N;
+  hipLaunchKernelGGL((kreduce_splitk<T, EPI>), dim3((unsigned)ceil_div(tot, 256)), dim3(256), 0, s,
+                     batch, M, N, ksplit, ws, epi);
   AFQ_LAUNCH_CHECK();
 }
 

@@ -197,8 +197,9 @@ class PhmsdTrial {
     int* exc = nullptr;          // device [nconf][2 kMaxRank]: holes then particles
     int* rank = nullptr;         // device [nconf]
     int* mstart = nullptr;       // device [nconf] offset of Minv of configuration u (r^2 each)
-    int* hptr = nullptr;         // device [nel + 1] CSR of (u, p) entries by hole position
-    int* hent = nullptr;         // device: u * 8 + p
+    int* rstart = nullptr;       // device [nconf] first compact-R row of configuration u
+    int* hptr = nullptr;         // device [nel + 1] CSR of excitation rows by hole position
+    int* hent = nullptr;         // device int4 entries: (configuration, compact-R row, particle, 0)
     int* cptr = nullptr;         // device [nconf + 1] couplings to the other spin
     int* ccol = nullptr;
     z64* cval = nullptr;         // conj(ci)

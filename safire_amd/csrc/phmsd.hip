@@ -160,11 +160,16 @@ __device__ inline void small_lu(z64 (&a)[R][R], z64& det, bool want_inv, z64 (&i
 }
 
 // ratio[w][u] = det M_u, M_u[p][q] = QQ0[x_p][h_q]; Minv of every excitation (woodbury R).
+// With want_inv also the compact R rows of get_compact_ph_R_matrices (phmsd_helpers.hpp:312-390),
+// Rc[p][a] = sum_q Minv[p][q] ([a == x_q] - QQ0[x_q][pos(a)]) with pos(a) the electron holding
+// active orbital a in the excitation (none for an emptied hole or an unoccupied orbital),
+// written to Rc[w][rstart[u] + p][0..nact).
 template <int R>
 __global__ void __launch_bounds__(256)
     k_ph_ratios(int W, int nel, int nvir, int u0, int cnt, int nconf, const int* __restrict__ exc,
                 const int* __restrict__ mstart, int P2, const z64* __restrict__ Qv,
-                z64* __restrict__ ovr, z64* __restrict__ Minv, int want_inv) {
+                z64* __restrict__ ovr, z64* __restrict__ Minv, int want_inv,
+                const int* __restrict__ rstart, int P, z64* __restrict__ Rc) {
   long idx = (long)blockIdx.x * 256 + threadIdx.x;
   if (idx >= (long)W * cnt) return;
   const int w = (int)(idx / cnt), u = u0 + (int)(idx % cnt);
@@ -184,76 +189,86 @@ __global__ void __launch_bounds__(256)
     for (int p = 0; p < R; ++p)
 #pragma unroll
       for (int q = 0; q < R; ++q) out[p * R + q] = inv[p][q];
+    const int nact = nel + nvir;
+    z64* rc = Rc + ((long)w * P + rstart[u]) * nact;
+    for (int a = 0; a < nact; ++a) {
+      int pos = a < nel ? a : -1;
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        if (e[q] == a) pos = -1;
+        if (e[R + q] == a) pos = e[q];
+      }
+      z64 coef[R];
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        coef[q] = {e[R + q] == a ? 1.0 : 0.0, 0.0};
+        if (pos >= 0) coef[q] = zsub(coef[q], Q[(e[R + q] - nel) * nel + pos]);
+      }
+#pragma unroll
+      for (int p = 0; p < R; ++p) {
+        z64 v = {0.0, 0.0};
+#pragma unroll
+        for (int q = 0; q < R; ++q) v = zfma(inv[p][q], coef[q], v);
+        rc[(long)p * nact + a] = v;
+      }
+    }
   }
 }
 
 // wa[u] = ova[u] sum_v C0[u][v] ovb[v], Ov = sum_u wa[u]; wb[v] = ovb[v] sum_u C1[v][u] ova[u].
+// One wavefront per coupling row (rows differ widely in length: the reference configuration
+// couples to every determinant that leaves its spin unexcited); fixed summation order.
 __global__ void __launch_bounds__(256)
     k_ph_weights(int na, int nb, const z64* __restrict__ ova, const z64* __restrict__ ovb,
                  const int* __restrict__ cpa, const int* __restrict__ cca, const z64* __restrict__ cva,
                  const int* __restrict__ cpb, const int* __restrict__ ccb, const z64* __restrict__ cvb,
                  z64* __restrict__ wa, z64* __restrict__ wb, z64* __restrict__ Ov) {
-  __shared__ z64 red[256];
-  const int w = blockIdx.x;
+  __shared__ z64 part[4];
+  const int w = blockIdx.x, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const z64* oa = ova + (long)w * na;
   const z64* ob = ovb + (long)w * nb;
-  z64 part = {0.0, 0.0};
-  for (int u = threadIdx.x; u < na; u += 256) {
+  z64 acc = {0.0, 0.0};
+  for (int u = wave; u < na; u += 4) {
     z64 s = {0.0, 0.0};
-    for (int k = cpa[u]; k < cpa[u + 1]; ++k) s = zfma(cva[k], ob[cca[k]], s);
-    z64 x = zmul(oa[u], s);
-    part = zadd(part, x);
-    wa[(long)w * na + u] = x;
+    for (int k = cpa[u] + lane; k < cpa[u + 1]; k += 64) s = zfma(cva[k], ob[cca[k]], s);
+    s.re = wave_sum(s.re);
+    s.im = wave_sum(s.im);
+    const z64 x = zmul(oa[u], s);
+    acc = zadd(acc, x);
+    if (lane == 0) wa[(long)w * na + u] = x;
   }
-  for (int v = threadIdx.x; v < nb; v += 256) {
+  for (int v = wave; v < nb; v += 4) {
     z64 s = {0.0, 0.0};
-    for (int k = cpb[v]; k < cpb[v + 1]; ++k) s = zfma(cvb[k], oa[ccb[k]], s);
-    wb[(long)w * nb + v] = zmul(ob[v], s);
+    for (int k = cpb[v] + lane; k < cpb[v + 1]; k += 64) s = zfma(cvb[k], oa[ccb[k]], s);
+    s.re = wave_sum(s.re);
+    s.im = wave_sum(s.im);
+    if (lane == 0) wb[(long)w * nb + v] = zmul(ob[v], s);
   }
-  z64 tot = block_sum<256>(part, red);
-  if (threadIdx.x == 0) Ov[w] = tot;
+  if (lane == 0) part[wave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) Ov[w] = zadd(zadd(part[0], part[1]), zadd(part[2], part[3]));
 }
 
-// R[w][j][a] / Ov with R = sum_u w_u (P_u + sum_p e_{h_p} Rc_u[p]), Rc_u = Minv (E - Q P_u)
-// (calculate_R), walked by hole position j over the excitations that empty it.
+// R[w][j][a] / Ov with R = sum_u w_u (P_u + sum_p e_{h_p} Rc_u[p]) (calculate_R,
+// phmsd_helpers.hpp:201-290), walked by hole position j over the excitation rows that empty it:
+// hent = (configuration, row of Rc, particle orbital).
 __global__ void __launch_bounds__(256)
-    k_ph_R(int nel, int nact, int nvir, int nconf, const int* __restrict__ exc,
-           const int* __restrict__ rank, const int* __restrict__ mstart, int P2,
-           const int* __restrict__ hptr, const int* __restrict__ hent, const z64* __restrict__ Minv,
-           const z64* __restrict__ Qv, const z64* __restrict__ wgt, const z64* __restrict__ Ov,
-           z64* __restrict__ Rout) {
+    k_ph_R(int nel, int nact, int nconf, int P, const int* __restrict__ hptr,
+           const int4* __restrict__ hent, const z64* __restrict__ Rc,
+           const z64* __restrict__ wgt, const z64* __restrict__ Ov, z64* __restrict__ Rout) {
   const int w = blockIdx.x;
   const z64 ov = Ov[w];
   const z64 inv = zrecip(ov);
-  const z64* Q = Qv + (long)w * nvir * nel;
   const z64* wg = wgt + (long)w * nconf;
-  const z64* Mi = Minv + (long)w * P2;
+  const z64* rcw = Rc + (long)w * P * nact;
   for (int t = threadIdx.x; t < nel * nact; t += 256) {
     const int j = t / nact, a = t % nact;
     z64 val = a == j ? ov : z64{0.0, 0.0};
     for (int k = hptr[j]; k < hptr[j + 1]; ++k) {
-      const int u = hent[k] >> 3, p = hent[k] & 7;
-      const z64 wu = wg[u];
-      if (zzero(wu)) continue;
-      const int r = rank[u];
-      const int* e = exc + (long)u * kXS;
-      int pos = -1;
-      if (a < nel) {
-        pos = a;
-        for (int q = 0; q < r; ++q)
-          if (e[q] == a) pos = -1;
-      } else {
-        for (int q = 0; q < r; ++q)
-          if (e[r + q] == a) pos = e[q];
-      }
-      const z64* mi = Mi + mstart[u] + p * r;
-      z64 rc = {0.0, 0.0};
-      for (int q = 0; q < r; ++q) {
-        z64 coef = {e[r + q] == a ? 1.0 : 0.0, 0.0};
-        if (pos >= 0) coef = zsub(coef, Q[(e[r + q] - nel) * nel + pos]);
-        rc = zfma(mi[q], coef, rc);
-      }
-      if (e[r + p] == a) rc.re += 1.0;
+      const int4 e = hent[k];
+      const z64 wu = wg[e.x];
+      z64 rc = rcw[(long)e.y * nact + a];
+      if (e.z == a) rc.re += 1.0;
       if (a == j) rc.re -= 1.0;
       val = zfma(wu, rc, val);
     }
@@ -310,7 +325,39 @@ __global__ void k_ph_energy_prep(int W, int NS, int nel, int nact, int nvir,
 // only in the hole rows, Y[h_p] = Zref[h_p] + g_p, hence
 //   Kn = Jref + sum_p g_p[h_p],   eX = sum_pq g_p[h_q] g_q[h_p] + 2 sum_pl g_p[l] T[l][h_p],
 // which are tr(Y) and tr(Y^2) - tr(Zref^2) of the reference's per-determinant evaluation.
-// part [W][nch][nconf][3] <- (e1 on the haj slice, sum eX, sum Kn^2 - Jref^2); KE [W][M][nconf].
+// When there are fewer excitations than threads, G threads share an excitation and take every
+// G-th slice; their sums are combined in a fixed order.
+// part [W][nch][nconf][3] <- (e1 on the haj slice, sum eX, sum Kn^2 - Jref^2); KE [W][nconf][M].
+template <int R>
+__device__ inline void ph_slice_terms(const z64* __restrict__ Ts, int nel, const int (&h)[R],
+                                      const int (&x)[R], const z64 (&Mi)[R][R], z64& k, z64& ex) {
+  z64 gh[R][R];
+  z64 acc = {0.0, 0.0};
+  for (int l = 0; l < nel; ++l) {
+    z64 V[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) V[q] = Ts[x[q] * nel + l];
+#pragma unroll
+    for (int p = 0; p < R; ++p) {
+      z64 g = {0.0, 0.0};
+#pragma unroll
+      for (int q = 0; q < R; ++q) g = zfma(Mi[p][q], V[q], g);
+      acc = zfma(g, Ts[l * nel + h[p]], acc);
+#pragma unroll
+      for (int q = 0; q < R; ++q)
+        if (h[q] == l) gh[p][q] = g;
+    }
+  }
+  k = {0.0, 0.0};
+  ex = {2.0 * acc.re, 2.0 * acc.im};
+#pragma unroll
+  for (int p = 0; p < R; ++p) {
+    k = zadd(k, gh[p][p]);
+#pragma unroll
+    for (int q = 0; q < R; ++q) ex = zfma(gh[p][q], gh[q][p], ex);
+  }
+}
+
 template <int R>
 __global__ void __launch_bounds__(256)
     k_ph_energy_rank(int nel, int nact, int nconf, int M, int NS, int NC, int u0, int u1,
@@ -321,73 +368,81 @@ __global__ void __launch_bounds__(256)
   const int slice = nact * nel;
   const int chunk = blockIdx.x, w = blockIdx.y, nch = gridDim.x;
   const int n0 = chunk * NC, nc = min(NC, NS - n0);
+  z64* red = smem + (long)NC * slice;   // [256][3]
   const z64* src = T3 + ((long)w * NS + n0) * slice;
   for (int t = threadIdx.x; t < nc * slice; t += 256) smem[t] = src[t];
   __syncthreads();
   const z64* jw = jr + (long)w * NS + n0;
-  for (int u = u0 + threadIdx.x; u < u1; u += 256) {
-    const int* e = exc + (long)u * kXS;
-    const z64* mi = Minv + (long)w * P2 + mstart[u];
-    int h[R], x[R];
-    z64 Mi[R][R];
-#pragma unroll
-    for (int p = 0; p < R; ++p) {
-      h[p] = e[p];
-      x[p] = e[R + p];
-#pragma unroll
-      for (int q = 0; q < R; ++q) Mi[p][q] = mi[p * R + q];
-    }
+  const int cnt = u1 - u0;
+  const int G = cnt >= 256 ? 1 : max(1, min(nc, 256 / cnt));
+  const int nitem = cnt >= 256 ? cnt : cnt * G;
+  for (int it0 = 0; it0 < nitem; it0 += 256) {
+    const int it = it0 + threadIdx.x;
     z64 a0 = {0.0, 0.0}, a1 = {0.0, 0.0}, a2 = {0.0, 0.0};
-    for (int c = 0; c < nc; ++c) {
-      const z64* Ts = smem + (long)c * slice;
-      z64 gh[R][R];
-      z64 acc = {0.0, 0.0};
-      for (int l = 0; l < nel; ++l) {
-        z64 V[R];
-#pragma unroll
-        for (int q = 0; q < R; ++q) V[q] = Ts[x[q] * nel + l];
-#pragma unroll
-        for (int p = 0; p < R; ++p) {
-          z64 g = {0.0, 0.0};
-#pragma unroll
-          for (int q = 0; q < R; ++q) g = zfma(Mi[p][q], V[q], g);
-          acc = zfma(g, Ts[l * nel + h[p]], acc);
-#pragma unroll
-          for (int q = 0; q < R; ++q)
-            if (h[q] == l) gh[p][q] = g;
-        }
-      }
-      z64 k = {0.0, 0.0}, ex = {2.0 * acc.re, 2.0 * acc.im};
+    if (it < nitem) {
+      const int u = u0 + it / G, g = it % G;
+      const int* e = exc + (long)u * kXS;
+      const z64* mi = Minv + (long)w * P2 + mstart[u];
+      int h[R], x[R];
+      z64 Mi[R][R];
 #pragma unroll
       for (int p = 0; p < R; ++p) {
-        k = zadd(k, gh[p][p]);
+        h[p] = e[p];
+        x[p] = e[R + p];
 #pragma unroll
-        for (int q = 0; q < R; ++q) ex = zfma(gh[p][q], gh[q][p], ex);
+        for (int q = 0; q < R; ++q) Mi[p][q] = mi[p * R + q];
       }
-      const int n = n0 + c;
-      if (n < M) {
-        const z64 kn = zadd(jw[c], k);
-        KE[((long)w * M + n) * nconf + u] = kn;
-        a1 = zadd(a1, ex);
-        a2 = zadd(a2, zsub(zmul(kn, kn), zmul(jw[c], jw[c])));
-      } else {
-        a0 = zadd(a0, k);
+      z64* ke = KE + ((long)w * nconf + u) * M;
+      for (int c = g; c < nc; c += G) {
+        z64 k, ex;
+        ph_slice_terms<R>(smem + (long)c * slice, nel, h, x, Mi, k, ex);
+        const int n = n0 + c;
+        if (n < M) {
+          const z64 kn = zadd(jw[c], k);
+          ke[n] = kn;
+          a1 = zadd(a1, ex);
+          a2 = zadd(a2, zsub(zmul(kn, kn), zmul(jw[c], jw[c])));
+        } else {
+          a0 = zadd(a0, k);
+        }
       }
     }
-    z64* pw = part + (((long)w * nch + chunk) * nconf + u) * 3;
-    pw[0] = a0;
-    pw[1] = a1;
-    pw[2] = a2;
+    if (G == 1) {
+      if (it < nitem) {
+        z64* pw = part + (((long)w * nch + chunk) * nconf + u0 + it) * 3;
+        pw[0] = a0;
+        pw[1] = a1;
+        pw[2] = a2;
+      }
+    } else {
+      red[3 * threadIdx.x + 0] = a0;
+      red[3 * threadIdx.x + 1] = a1;
+      red[3 * threadIdx.x + 2] = a2;
+      __syncthreads();
+      if ((int)threadIdx.x < cnt) {
+        z64 s0 = {0.0, 0.0}, s1 = {0.0, 0.0}, s2 = {0.0, 0.0};
+        for (int g = 0; g < G; ++g) {
+          const z64* r = red + 3 * (threadIdx.x * G + g);
+          s0 = zadd(s0, r[0]);
+          s1 = zadd(s1, r[1]);
+          s2 = zadd(s2, r[2]);
+        }
+        z64* pw = part + (((long)w * nch + chunk) * nconf + u0 + threadIdx.x) * 3;
+        pw[0] = s0;
+        pw[1] = s1;
+        pw[2] = s2;
+      }
+    }
   }
 }
 
-// KE[w][n][0] = Jref for n < M (the reference configuration's Coulomb vector)
+// KE[w][0][n] = Jref for n < M (the reference configuration's Coulomb vector)
 __global__ void k_ph_ke_ref(int W, int M, int NS, int nconf, const z64* __restrict__ jr,
                             z64* __restrict__ KE) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)W * M) return;
   const int w = (int)(t / M), n = (int)(t % M);
-  KE[t * nconf] = jr[(long)w * NS + n];
+  KE[(long)w * nconf * M + n] = jr[(long)w * NS + n];
 }
 
 // energy_alg1 assembly per walker: reference terms times Ov, weighted excitation terms of both
@@ -400,9 +455,9 @@ __global__ void __launch_bounds__(256)
                         const z64* __restrict__ wa, const z64* __restrict__ wb,
                         const z64* __restrict__ Osum, const z64* __restrict__ log_ov,
                         const z64* __restrict__ KEa, const z64* __restrict__ KEb,
-                        const int* __restrict__ det_ab, const z64* __restrict__ det_c,
-                        const z64* __restrict__ ova, const z64* __restrict__ ovb,
-                        z64* __restrict__ E, z64* __restrict__ Ov) {
+                        const int* __restrict__ cpa, const int* __restrict__ cca,
+                        const z64* __restrict__ cva, const z64* __restrict__ ova,
+                        const z64* __restrict__ ovb, z64* __restrict__ E, z64* __restrict__ Ov) {
   __shared__ z64 red[256];
   const int w = blockIdx.x;
   const z64 ov = Osum[w];
@@ -435,14 +490,25 @@ __global__ void __launch_bounds__(256)
     ej = zfma(ov, {0.5 * r2.re, 0.5 * r2.im}, ej);
     if (threadIdx.x == 0) e1 = zfma(ov, jr[M], e1);
   }
-  const z64* Ka = KEa + (long)w * M * na;
-  const z64* Kb = KEb + (long)w * M * nb;
-  for (int d = threadIdx.x; d < ndet; d += 256) {
-    const int ua = det_ab[2 * d], vb = det_ab[2 * d + 1];
-    z64 s = {0.0, 0.0};
-    for (int n = 0; n < M; ++n) s = zfma(Ka[(long)n * na + ua], Kb[(long)n * nb + vb], s);
-    z64 c = zmul(det_c[d], zmul(ova[(long)w * na + ua], ovb[(long)w * nb + vb]));
-    ej = zfma(c, s, ej);
+  // opposite-spin Coulomb: sum_u ova[u] sum_{row u} conj(ci) ovb[v] Ka[u] . Kb[v], one wavefront
+  // per alpha configuration, lanes over the Cholesky index
+  {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const z64* Ka = KEa + (long)w * na * M;
+    const z64* Kb = KEb + (long)w * nb * M;
+    const z64* oa = ova + (long)w * na;
+    const z64* ob = ovb + (long)w * nb;
+    for (int u = wave; u < na; u += 4) {
+      z64 su = {0.0, 0.0};
+      for (int k = cpa[u]; k < cpa[u + 1]; ++k) {
+        const z64 cv = zmul(cva[k], ob[cca[k]]);
+        const z64* kb = Kb + (long)cca[k] * M;
+        z64 d = {0.0, 0.0};
+        for (int n = lane; n < M; n += 64) d = zfma(Ka[(long)u * M + n], kb[n], d);
+        su = zfma(cv, d, su);
+      }
+      ej = zfma(oa[u], su, ej);   // per-lane partial; the block sum below completes it
+    }
   }
   e1 = block_sum<256>(e1, red);
   ex = block_sum<256>(ex, red);
@@ -573,7 +639,7 @@ PhmsdTrial::PhmsdTrial(CholeskyOps& ham, const PhExcitations& ex) : ham_(ham), c
       AFQ_CHECK(ex.rank[sp][0] == 0, kErrArg, "configuration 0 must be the reference");
       d.rank_h.assign(ex.rank[sp], ex.rank[sp] + d.nconf);
       d.exc_h.assign((size_t)d.nconf * kXS, 0);
-      std::vector<int> mstart(d.nconf, 0);
+      std::vector<int> mstart(d.nconf, 0), rstart(d.nconf, 0);
       std::vector<std::vector<int>> byhole(d.nel);
       d.rank_off.assign(kMaxRank + 2, d.nconf);
       d.rank_off[0] = 0;
@@ -593,22 +659,25 @@ PhmsdTrial::PhmsdTrial(CholeskyOps& ham, const PhExcitations& ex) : ham_(ham), c
                     "particle must be an active orbital outside the reference");
           d.exc_h[(size_t)u * kXS + p] = e[p];
           d.exc_h[(size_t)u * kXS + r + p] = e[r + p];
-          byhole[e[p]].push_back(u * 8 + p);
+          byhole[e[p]].insert(byhole[e[p]].end(), {u, d.P + p, e[r + p], 0});
         }
         mstart[u] = d.P2;
+        rstart[u] = d.P;
         d.P += r;
         d.P2 += r * r;
       }
       for (int q = prev + 1; q <= kMaxRank + 1; ++q) d.rank_off[q] = d.nconf;
-      std::vector<int> hptr(d.nel + 1, 0), hent;
+      std::vector<int> hptr(d.nel + 1, 0), hent;   // hent: (configuration, Rc row, particle, 0)
       for (int j = 0; j < d.nel; ++j) {
-        hptr[j] = (int)hent.size();
+        hptr[j] = (int)hent.size() / 4;
         hent.insert(hent.end(), byhole[j].begin(), byhole[j].end());
       }
-      hptr[d.nel] = (int)hent.size();
+      hptr[d.nel] = (int)hent.size() / 4;
+      if (hent.empty()) hent.assign(4, 0);
       d.exc = dcopy(s, d.exc_h);
       d.rank = dcopy(s, d.rank_h);
       d.mstart = dcopy(s, mstart);
+      d.rstart = dcopy(s, rstart);
       d.hptr = dcopy(s, hptr);
       d.hent = dcopy(s, hent);
       d.orbs = ham.PsiT() + (sp == 0 ? 0 : (size_t)nact[0] * N);
@@ -669,7 +738,7 @@ PhmsdTrial::~PhmsdTrial() {
 
 void PhmsdTrial::free_all() {
   for (auto& d : sp_)
-    for (void* p : {(void*)d.exc, (void*)d.rank, (void*)d.mstart, (void*)d.hptr, (void*)d.hent,
+    for (void* p : {(void*)d.exc, (void*)d.rank, (void*)d.mstart, (void*)d.rstart, (void*)d.hptr, (void*)d.hent,
                     (void*)d.cptr, (void*)d.ccol, (void*)d.cval, (void*)d.Lext})
       if (p) (void)hipFree(p);
   for (void* p : {(void*)det_ab_, (void*)det_c_})
@@ -718,6 +787,8 @@ void PhmsdTrial::ratios(int s, int W, bool want_inverse) {
   z64* ovr = scratch_as<z64>(ctx_, tag("ph_ovr", s), (size_t)W * d.nconf);
   z64* Mi = scratch_as<z64>(ctx_, tag("ph_Minv", s), (size_t)W * std::max(d.P2, 1));
   const z64* Qv = scratch_as<z64>(ctx_, tag("ph_Qv", s), (size_t)W * std::max(d.nvir, 1) * d.nel);
+  z64* Rc = want_inverse ? scratch_as<z64>(ctx_, tag("ph_Rc", s), (size_t)W * std::max(d.P, 1) * d.nact)
+                         : nullptr;
   hipLaunchKernelGGL(k_ph_fill_one, dim3(ceil_div(W, 256)), dim3(256), 0, st, W, d.nconf, ovr);
   AFQ_LAUNCH_CHECK();
   for (int r = 1; r <= d.rmax; ++r) {
@@ -727,7 +798,7 @@ void PhmsdTrial::ratios(int s, int W, bool want_inverse) {
     const int wi = want_inverse ? 1 : 0;
 #define AFQ_RATIOS(R)                                                                          \
   hipLaunchKernelGGL(k_ph_ratios<R>, g, dim3(256), 0, st, W, d.nel, d.nvir, u0, cnt, d.nconf, \
-                     d.exc, d.mstart, d.P2, Qv, ovr, Mi, wi)
+                     d.exc, d.mstart, d.P2, Qv, ovr, Mi, wi, d.rstart, d.P, Rc)
     switch (r) {
       case 1: AFQ_RATIOS(1); break;
       case 2: AFQ_RATIOS(2); break;
@@ -769,10 +840,9 @@ void PhmsdTrial::mixed_density_matrix(const z64* Pa, const z64* Pb, int W, z64* 
   for (int sp = 0; sp < 2; ++sp) {
     const SpinDev& d = sp_[sp];
     z64* R = scratch_as<z64>(ctx_, "ph_R", (size_t)W * d.nel * d.nact);
-    hipLaunchKernelGGL(k_ph_R, dim3(W), dim3(256), 0, s, d.nel, d.nact, d.nvir, d.nconf, d.exc,
-                       d.rank, d.mstart, d.P2, d.hptr, d.hent,
-                       scratch_as<z64>(ctx_, tag("ph_Minv", sp), (size_t)W * std::max(d.P2, 1)),
-                       scratch_as<z64>(ctx_, tag("ph_Qv", sp), (size_t)W * std::max(d.nvir, 1) * d.nel),
+    hipLaunchKernelGGL(k_ph_R, dim3(W), dim3(256), 0, s, d.nel, d.nact, d.nconf, d.P, d.hptr,
+                       reinterpret_cast<const int4*>(d.hent),
+                       scratch_as<z64>(ctx_, tag("ph_Rc", sp), (size_t)W * std::max(d.P, 1) * d.nact),
                        scratch_as<z64>(ctx_, tag("ph_w", sp), (size_t)W * d.nconf), Osum, R);
     AFQ_LAUNCH_CHECK();
     // G rows of this spin: G[a][k] = sum_j R[j][a] GA[j][k]
@@ -824,7 +894,7 @@ void PhmsdTrial::energy(const z64* Pa, const z64* Pb, int W, z64* E, z64* Ov) {
   for (int sp = 0; sp < 2; ++sp) {
     const SpinDev& d = sp_[sp];
     const long slice = (long)d.nact * d.nel;
-    const int NC = (int)std::min<long>(32, (60L * 1024) / (slice * (long)sizeof(z64)));
+    const int NC = (int)std::min<long>(32, (48L * 1024) / (slice * (long)sizeof(z64)));
     AFQ_CHECK(NC >= 1, kErrShape, "active space too large for the PHMSD energy kernel");
     nch[sp] = ceil_div(NS, NC);
     z64* T3 = scratch_as<z64>(ctx_, "ph_T3", (size_t)W * NS * slice);
@@ -848,7 +918,7 @@ void PhmsdTrial::energy(const z64* Pa, const z64* Pb, int W, z64* E, z64* Ov) {
                        d.nconf, jr[sp], KE[sp]);
     AFQ_LAUNCH_CHECK();
     const z64* Mi = scratch_as<z64>(ctx_, tag("ph_Minv", sp), (size_t)W * std::max(d.P2, 1));
-    const size_t shm = sizeof(z64) * (size_t)NC * slice;
+    const size_t shm = sizeof(z64) * ((size_t)NC * slice + 3 * 256);
     for (int r = 1; r <= d.rmax; ++r) {
       const int u0 = d.rank_off[r], u1 = d.rank_off[r + 1];
       if (u1 <= u0) continue;
@@ -872,7 +942,7 @@ void PhmsdTrial::energy(const z64* Pa, const z64* Pb, int W, z64* E, z64* Ov) {
                      nch[0], nch[1], M, NS, ndet_, ham_.E0(), part[0], part[1], jr[0], tz[0],
                      jr[1], tz[1], scratch_as<z64>(ctx_, "ph_w0", (size_t)W * sp_[0].nconf),
                      scratch_as<z64>(ctx_, "ph_w1", (size_t)W * sp_[1].nconf), Osum, log_ov,
-                     KE[0], KE[1], det_ab_, det_c_,
+                     KE[0], KE[1], sp_[0].cptr, sp_[0].ccol, sp_[0].cval,
                      scratch_as<z64>(ctx_, "ph_ovr0", (size_t)W * sp_[0].nconf),
                      scratch_as<z64>(ctx_, "ph_ovr1", (size_t)W * sp_[1].nconf), E, Ov);
   AFQ_LAUNCH_CHECK();

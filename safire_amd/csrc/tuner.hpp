@@ -9,6 +9,7 @@
 // use the first candidate.  AFQ_TUNE=0 in the environment disables tuning.
 #pragma once
 
+#include <algorithm>
 #include <cstdlib>
 #include <map>
 #include <mutex>
@@ -118,6 +119,36 @@ void gemm_large(Context& c, const char* site, int batch, int M, int N, int K, co
       case 1: gemm::launch<T, 128, 128, 16, 2, 2>(s, batch, M, N, K, A, B, e); break;
       case 2: gemm::launch<T, 128, 64, 16, 2, 2>(s, batch, M, N, K, A, B, e); break;
       default: gemm::launch<T, 64, 128, 16, 2, 2>(s, batch, M, N, K, A, B, e); break;
+    }
+  }, idempotent);
+}
+
+// Large products whose output grid is small against the machine (vbias: W x M outputs over a
+// long K): K is cut into chunks by a rule that depends on the shape only, so every tile
+// candidate sums in the same order and the tuned choice still never changes the result.
+inline void splitk_plan(int M, int N, int K, int& ksplit, int& kchunk) {
+  const long tiles = (long)ceil_div(M, 64) * ceil_div(N, 64);
+  int ks = (int)std::max<long>(1, std::min<long>((1024 + tiles - 1) / tiles, K / 128));
+  kchunk = ((ceil_div(K, ks) + 63) / 64) * 64;
+  ksplit = ceil_div(K, kchunk);
+}
+
+template <typename T, class OA, class OB, class EPI>
+void gemm_large_splitk(Context& c, const char* site, int M, int N, int K, const OA& A,
+                       const OB& B, const EPI& e, bool idempotent = true) {
+  hipStream_t s = c.stream();
+  int ks, kc;
+  splitk_plan(M, N, K, ks, kc);
+  cplx<T>* ws = ks > 1 ? scratch_as<cplx<T>>(c, std::string("splitk_ws") + (sizeof(T) == 8 ? "64" : "32"),
+                                              (size_t)ks * M * N)
+                       : nullptr;
+  tuned(c, tune_key(site, sizeof(T), ks, M, N, K), 5, [&](int i) {
+    switch (i) {
+      case 0: gemm::launch_splitk<T, 64, 64, 16, 2, 2>(s, 1, M, N, K, ks, kc, ws, A, B, e); break;
+      case 1: gemm::launch_splitk<T, 128, 128, 16, 2, 2>(s, 1, M, N, K, ks, kc, ws, A, B, e); break;
+      case 2: gemm::launch_splitk<T, 128, 64, 16, 2, 2>(s, 1, M, N, K, ks, kc, ws, A, B, e); break;
+      case 3: gemm::launch_splitk<T, 64, 128, 16, 2, 2>(s, 1, M, N, K, ks, kc, ws, A, B, e); break;
+      default: gemm::launch_splitk<T, 32, 32, 16, 1, 1>(s, 1, M, N, K, ks, kc, ws, A, B, e); break;
     }
   }, idempotent);
 }
