@@ -124,18 +124,359 @@ void inverse_logdet(Context& c, const z64* A, int nb, int n, z64* ovlp, z64* Ain
 
 namespace {
 
+// ---- fused propagation ------------------------------------------------------------------------
+//
+// Phi <- P1 expM(V) P1 Phi in one launch (propagate_impl + apply_expM for the 'N' operator).
+// The Taylor recurrence T_k = (i/k) V T_{k-1} and both P1 products act on each column of the
+// walker matrix separately, so a workgroup owns one walker and one 16-column tile of it:
+//   * the current term lives in LDS as the B operand, stored [column][row] so the four k values
+//     a lane feeds to four consecutive MFMAs are one 16-byte word;
+//   * wavefront w owns the 16-row stripes w, w + NW, ... of the output: it streams the matching
+//     rows of V (or P1) from memory straight into MFMA A fragments (no LDS staging, no barrier
+//     inside a product), keeps its accumulators and its part of the running sum S in registers;
+//   * MFMA step j of the 16-wide k group g multiplies k = 16 g + 4 (lane / 16) + j, the same
+//     grouping in A and B.
+// Workgroups are numbered so that each XCD (8 of them, dispatched round-robin) processes a
+// contiguous run of (walker, tile) pairs: the tiles of one walker share its V in that XCD's L2.
+
+template <typename T, bool CPLX>
+struct AFrag {
+  T re[4];
+  T im[CPLX ? 4 : 1];
+};
+
+// A[row][k .. k + 3] (complex elements interleaved); rows >= rows or k >= K read as zero.
+// vec: the rows of A are 16-byte aligned, so in-range fragments use vector loads.
+template <typename T, bool CPLX>
+__device__ inline void load_afrag(AFrag<T, CPLX>& f, const T* __restrict__ A, long lda, int row,
+                                  int rows, int k, int K, bool vec) {
+  if (vec && row < rows && k + 3 < K) {
+    if constexpr (CPLX && sizeof(T) == 4) {
+      const float4* p = reinterpret_cast<const float4*>(A + 2 * ((long)row * lda + k));
+      const float4 x = p[0], y = p[1];
+      f.re[0] = x.x; f.im[0] = x.y; f.re[1] = x.z; f.im[1] = x.w;
+      f.re[2] = y.x; f.im[2] = y.y; f.re[3] = y.z; f.im[3] = y.w;
+    } else if constexpr (CPLX) {
+      const double2* p = reinterpret_cast<const double2*>(A + 2 * ((long)row * lda + k));
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const double2 v = p[j];
+        f.re[j] = v.x;
+        f.im[j] = v.y;
+      }
+    } else if constexpr (sizeof(T) == 4) {
+      const float4 x = *reinterpret_cast<const float4*>(A + (long)row * lda + k);
+      f.re[0] = x.x; f.re[1] = x.y; f.re[2] = x.z; f.re[3] = x.w;
+    } else {
+      const double2* p = reinterpret_cast<const double2*>(A + (long)row * lda + k);
+      const double2 x = p[0], y = p[1];
+      f.re[0] = x.x; f.re[1] = x.y; f.re[2] = y.x; f.re[3] = y.y;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const bool in = row < rows && k + j < K;
+      const long e = (long)row * lda + k + j;
+      if constexpr (CPLX) {
+        f.re[j] = in ? A[2 * e] : (T)0;
+        f.im[j] = in ? A[2 * e + 1] : (T)0;
+      } else {
+        f.re[j] = in ? A[e] : (T)0;
+      }
+    }
+  }
+}
+
+template <typename T>
+__device__ inline void read_bfrag(T (&br)[4], T (&bi)[4], const T* Br, const T* Bi, int off) {
+  if constexpr (sizeof(T) == 4) {
+    const float4 x = *reinterpret_cast<const float4*>(Br + off);
+    const float4 y = *reinterpret_cast<const float4*>(Bi + off);
+    br[0] = x.x; br[1] = x.y; br[2] = x.z; br[3] = x.w;
+    bi[0] = y.x; bi[1] = y.y; bi[2] = y.z; bi[3] = y.w;
+  } else {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const double2 x = *reinterpret_cast<const double2*>(Br + off + 2 * h);
+      const double2 y = *reinterpret_cast<const double2*>(Bi + off + 2 * h);
+      br[2 * h] = x.x; br[2 * h + 1] = x.y;
+      bi[2 * h] = y.x; bi[2 * h + 1] = y.y;
+    }
+  }
+}
+
+// One 16-wide k group of acc[s][t] += A[stripe s rows][k group] * B[k group][column tile t].
+template <typename T, bool CPLX, int SPW, int CB>
+__device__ inline void group_mfma(const AFrag<T, CPLX> (&f)[SPW], int g, int NS, int NW, int wave,
+                                  int fr, int fk, const T* Br, const T* Bi, int LDK,
+                                  typename gemm::Acc<T>::v4 (&ar)[SPW][CB],
+                                  typename gemm::Acc<T>::v4 (&ai)[SPW][CB]) {
+  using gemm::Acc;
+#pragma unroll
+  for (int t = 0; t < CB; ++t) {
+    T br[4], bi[4];
+    read_bfrag(br, bi, Br, Bi, (16 * t + fr) * LDK + 16 * g + 4 * fk);
+#pragma unroll
+    for (int s = 0; s < SPW; ++s) {
+      if (wave + s * NW >= NS) continue;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        ar[s][t] = Acc<T>::mfma(f[s].re[j], br[j], ar[s][t]);
+        ai[s][t] = Acc<T>::mfma(f[s].re[j], bi[j], ai[s][t]);
+        if constexpr (CPLX) {
+          ar[s][t] = Acc<T>::mfma(-f[s].im[j], bi[j], ar[s][t]);
+          ai[s][t] = Acc<T>::mfma(f[s].im[j], br[j], ai[s][t]);
+        }
+      }
+    }
+  }
+}
+
+template <typename T, bool CPLX, int SPW>
+__device__ inline void load_group(AFrag<T, CPLX> (&f)[SPW], const T* __restrict__ A, long lda,
+                                  bool vec, int N, int NS, int NW, int wave, int fr, int fk, int g) {
+  if (g >= NS) return;
+#pragma unroll
+  for (int s = 0; s < SPW; ++s)
+    if (wave + s * NW < NS) load_afrag(f[s], A, lda, (wave + s * NW) * 16 + fr, N, 16 * g + 4 * fk, N, vec);
+}
+
+// acc = A[stripe rows][0:N] * B[0:N][tiles] for the wave's stripes (complex B in LDS). The A
+// fragments of the next two k groups are in flight while a group is multiplied.
+template <typename T, bool CPLX, int SPW, int CB>
+__device__ inline void stripes_times_b(const T* __restrict__ A, long lda, bool vec, int N, int NS,
+                                       int NW, int wave, int fr, int fk, const T* Br,
+                                       const T* Bi, int LDK,
+                                       typename gemm::Acc<T>::v4 (&ar)[SPW][CB],
+                                       typename gemm::Acc<T>::v4 (&ai)[SPW][CB]) {
+  using V4 = typename gemm::Acc<T>::v4;
+#pragma unroll
+  for (int s = 0; s < SPW; ++s)
+#pragma unroll
+    for (int t = 0; t < CB; ++t) {
+      ar[s][t] = V4{0, 0, 0, 0};
+      ai[s][t] = V4{0, 0, 0, 0};
+    }
+  AFrag<T, CPLX> f0[SPW], f1[SPW], f2[SPW];
+  load_group(f0, A, lda, vec, N, NS, NW, wave, fr, fk, 0);
+  load_group(f1, A, lda, vec, N, NS, NW, wave, fr, fk, 1);
+  for (int g = 0; g < NS; g += 3) {
+    load_group(f2, A, lda, vec, N, NS, NW, wave, fr, fk, g + 2);
+    group_mfma<T, CPLX, SPW, CB>(f0, g, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai);
+    if (g + 1 >= NS) break;
+    load_group(f0, A, lda, vec, N, NS, NW, wave, fr, fk, g + 3);
+    group_mfma<T, CPLX, SPW, CB>(f1, g + 1, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai);
+    if (g + 2 >= NS) break;
+    load_group(f1, A, lda, vec, N, NS, NW, wave, fr, fk, g + 4);
+    group_mfma<T, CPLX, SPW, CB>(f2, g + 2, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai);
+  }
+}
+
+// B[column][row] <- the wave's stripes of (xr, xi)
+template <typename T, int SPW, int CB>
+__device__ inline void write_b(T* Br, T* Bi, int LDK, int NS, int NW, int wave, int fr, int fk,
+                               const typename gemm::Acc<T>::v4 (&xr)[SPW][CB],
+                               const typename gemm::Acc<T>::v4 (&xi)[SPW][CB]) {
+#pragma unroll
+  for (int s = 0; s < SPW; ++s) {
+    if (wave + s * NW >= NS) continue;
+    const int r0 = (wave + s * NW) * 16;
+#pragma unroll
+    for (int t = 0; t < CB; ++t) {
+      const int off = (16 * t + fr) * LDK + r0;
+      if constexpr (sizeof(T) == 4) {
+        // FP32 accumulator register r holds row 4 (lane / 16) + r: four consecutive rows
+        *reinterpret_cast<float4*>(Br + off + 4 * fk) =
+            float4{xr[s][t][0], xr[s][t][1], xr[s][t][2], xr[s][t][3]};
+        *reinterpret_cast<float4*>(Bi + off + 4 * fk) =
+            float4{xi[s][t][0], xi[s][t][1], xi[s][t][2], xi[s][t][3]};
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = gemm::Acc<T>::row(fk, r);
+          Br[off + row] = xr[s][t][r];
+          Bi[off + row] = xi[s][t][r];
+        }
+      }
+    }
+  }
+}
+
+// One workgroup per (walker, CB 16-column tiles); NW wavefronts.
+template <typename T, bool P1C, int SPW, int CB>
+__global__ void __launch_bounds__(1024)
+    k_prop_taylor(int W, int N, int n, int order, double sgn, const T* __restrict__ P1, int vecP,
+                  const T* __restrict__ V, long v_bs, int vecV, z64* __restrict__ Phi, long phi_bs,
+                  int NW, int LDK) {
+  using V4 = typename gemm::Acc<T>::v4;
+  constexpr int BC = 16 * CB;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* Br = reinterpret_cast<T*>(smem_raw);
+  T* Bi = Br + BC * LDK;
+  const int nct = (n + BC - 1) / BC;
+  const int NS = (N + 15) / 16;
+  const long total = (long)W * nct;
+  const long bid = blockIdx.x, xcd = bid % 8, slot = bid / 8;
+  const long q = xcd * (total / 8) + min(xcd, total % 8) + slot;
+  const int w = (int)(q / nct), c0 = (int)(q % nct) * BC;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int fr = lane & 15, fk = lane >> 4;
+  z64* Pw = Phi + (long)w * phi_bs;
+  const T* Vw = V + 2 * (long)w * v_bs;
+
+  for (int t = tid; t < BC * NS * 16; t += blockDim.x) {
+    const int cc = t % BC, k = t / BC;
+    const z64 v = (k < N && c0 + cc < n) ? Pw[(long)k * n + c0 + cc] : z64{0.0, 0.0};
+    Br[cc * LDK + k] = (T)v.re;
+    Bi[cc * LDK + k] = (T)v.im;
+  }
+  __syncthreads();
+
+  V4 ar[SPW][CB], ai[SPW][CB], sr[SPW][CB], si[SPW][CB];
+  // T_0 = P1 Phi, S = T_0
+  stripes_times_b<T, P1C, SPW, CB>(P1, N, vecP, N, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai);
+#pragma unroll
+  for (int s = 0; s < SPW; ++s)
+#pragma unroll
+    for (int t = 0; t < CB; ++t) {
+      sr[s][t] = ar[s][t];
+      si[s][t] = ai[s][t];
+    }
+  __syncthreads();
+  write_b<T, SPW, CB>(Br, Bi, LDK, NS, NW, wave, fr, fk, ar, ai);
+  __syncthreads();
+  for (int k = 1; k <= order; ++k) {
+    stripes_times_b<T, true, SPW, CB>(Vw, N, vecV, N, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai);
+    // T_k = (i sgn / k) V T_{k-1}; S += T_k
+    const T fi = (T)(sgn / (double)k);
+#pragma unroll
+    for (int s = 0; s < SPW; ++s)
+#pragma unroll
+      for (int t = 0; t < CB; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const T tr = -(fi * ai[s][t][r]), ti = fi * ar[s][t][r];
+          ar[s][t][r] = tr;
+          ai[s][t][r] = ti;
+          sr[s][t][r] = sr[s][t][r] + tr;
+          si[s][t][r] = si[s][t][r] + ti;
+        }
+    __syncthreads();
+    write_b<T, SPW, CB>(Br, Bi, LDK, NS, NW, wave, fr, fk, ar, ai);
+    __syncthreads();
+  }
+  write_b<T, SPW, CB>(Br, Bi, LDK, NS, NW, wave, fr, fk, sr, si);
+  __syncthreads();
+  // Phi = P1 S
+  stripes_times_b<T, P1C, SPW, CB>(P1, N, vecP, N, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai);
+#pragma unroll
+  for (int s = 0; s < SPW; ++s) {
+    if (wave + s * NW >= NS) continue;
+#pragma unroll
+    for (int t = 0; t < CB; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = (wave + s * NW) * 16 + gemm::Acc<T>::row(fk, r), col = c0 + 16 * t + fr;
+        if (row < N && col < n) Pw[(long)row * n + col] = z64{(double)ar[s][t][r], (double)ai[s][t][r]};
+      }
+  }
+}
+
+struct FusedShape {
+  int NW, spw, LDK;
+};
+
+inline int max_lds_per_block() {
+  static int v = [] {
+    int dev = 0, x = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&x, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      x = 64 * 1024;
+    return x;
+  }();
+  return v;
+}
+
+template <typename T, bool P1C, int SPW, int CB>
+void launch_fused(hipStream_t s, const PropagateArgs& a, const FusedShape& f, const T* P1, int vecP,
+                  const T* V, int vecV, z64* Phi) {
+  const size_t lds = 2 * 16 * CB * (size_t)f.LDK * sizeof(T);
+  auto kern = k_prop_taylor<T, P1C, SPW, CB>;
+  if (lds > 64 * 1024)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  const long blocks = (long)a.W * ceil_div(a.n, 16 * CB);
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(64 * f.NW), lds, s, a.W, a.N, a.n, a.order,
+                     1.0, P1, vecP, V, a.v_bs, vecV, Phi, a.phi_bs, f.NW, f.LDK);
+  AFQ_LAUNCH_CHECK();
+}
+
+// Launches the fused kernel when the shape fits it (returns false otherwise). Candidates differ
+// in the number of 16-column tiles per workgroup (CB); they multiply in the same order, so the
+// tuner (timed on a scratch copy of the walkers) never changes the result.
+template <typename T>
+bool propagate_fused(Context& c, const PropagateArgs& a, const T* P1, bool p1c, const void* V) {
+  if (a.TA != 'N') return false;
+  const int N = a.N, n = a.n, W = a.W;
+  FusedShape f;
+  const int NS = ceil_div(N, 16);
+  f.NW = std::min(NS, 16);
+  f.spw = ceil_div(NS, f.NW);
+  f.LDK = 16 * NS + 4;
+  // more stripes per wave spill registers (FP32 beyond 2, FP64 beyond 1)
+  if (f.spw > (sizeof(T) == 4 ? 2 : 1)) return false;
+  const size_t lds1 = 2 * 16 * (size_t)f.LDK * sizeof(T);
+  // wider column tiles spill in FP64 and with two stripes per wave
+  const int maxcb = sizeof(T) == 4 && f.spw == 1 ? 3 : 1;
+  int ncand = 0;
+  int cbs[3];
+  for (int cb = 1; cb <= maxcb; ++cb)
+    if (lds1 * cb <= (size_t)max_lds_per_block() && (cb == 1 || 16 * (cb - 1) < n)) cbs[ncand++] = cb;
+  if (ncand == 0) return false;
+  AFQ_CHECK((long)W * ceil_div(n, 16) < (1L << 31), kErrShape, "grid too large");
+  const size_t esz = sizeof(T) * (p1c ? 2 : 1);
+  const int vecP = ((size_t)N * esz) % 16 == 0 && (reinterpret_cast<uintptr_t>(P1) % 16) == 0;
+  const int vecV = ((size_t)N * 2 * sizeof(T)) % 16 == 0 &&
+                   ((size_t)a.v_bs * 2 * sizeof(T)) % 16 == 0 &&
+                   (reinterpret_cast<uintptr_t>(V) % 16) == 0;
+  hipStream_t s = c.stream();
+  const T* Vt = reinterpret_cast<const T*>(V);
+  auto run = [&](int i, z64* Phi) {
+    const int cb = cbs[i];
+    constexpr int SPW2 = sizeof(T) == 4 ? 2 : 1;
+#define AFQ_PT(P1C, SPW, CB) launch_fused<T, P1C, SPW, CB>(s, a, f, P1, vecP, Vt, vecV, Phi)
+    if (f.spw == 1) {
+      if constexpr (sizeof(T) == 4) {
+        if (p1c) {
+          if (cb == 1) AFQ_PT(true, 1, 1);
+          else if (cb == 2) AFQ_PT(true, 1, 2);
+          else AFQ_PT(true, 1, 3);
+        } else {
+          if (cb == 1) AFQ_PT(false, 1, 1);
+          else if (cb == 2) AFQ_PT(false, 1, 2);
+          else AFQ_PT(false, 1, 3);
+        }
+      } else {
+        if (p1c) AFQ_PT(true, 1, 1);
+        else AFQ_PT(false, 1, 1);
+      }
+    } else {
+      if (p1c) AFQ_PT(true, SPW2, 1);
+      else AFQ_PT(false, SPW2, 1);
+    }
+#undef AFQ_PT
+  };
+  tuned_on_copy(c, tune_key("prop_fused", sizeof(T), W, N, n, N), ncand,
+                a.Phi, (size_t)W * a.phi_bs, run);
+  return true;
+}
+
 // Phi <- P1 expM(op(V)) P1 Phi in compute type T.  TIn of the walker matrix is always double.
 template <typename T>
 void propagate_t(Context& c, const PropagateArgs& a) {
   hipStream_t s = c.stream();
   const int W = a.W, N = a.N, n = a.n;
   const long tb = (long)N * n;
-  cplx<T>* S = scratch_as<cplx<T>>(c, std::string("prop_S") + (sizeof(T) == 8 ? "64" : "32"),
-                                    (size_t)W * tb);
-  cplx<T>* T1 = scratch_as<cplx<T>>(c, std::string("prop_T1") + (sizeof(T) == 8 ? "64" : "32"),
-                                     (size_t)W * tb);
-  cplx<T>* T2 = scratch_as<cplx<T>>(c, std::string("prop_T2") + (sizeof(T) == 8 ? "64" : "32"),
-                                     (size_t)W * tb);
   const T* P1r = nullptr;
   const cplx<T>* P1c = nullptr;
   if constexpr (sizeof(T) == 8) {
@@ -145,6 +486,13 @@ void propagate_t(Context& c, const PropagateArgs& a) {
     P1r = a.P1r32;
     P1c = a.P1c32;
   }
+  if (a.fused && propagate_fused<T>(c, a, P1r ? P1r : &P1c->re, P1r == nullptr, a.V)) return;
+  cplx<T>* S = scratch_as<cplx<T>>(c, std::string("prop_S") + (sizeof(T) == 8 ? "64" : "32"),
+                                    (size_t)W * tb);
+  cplx<T>* T1 = scratch_as<cplx<T>>(c, std::string("prop_T1") + (sizeof(T) == 8 ? "64" : "32"),
+                                     (size_t)W * tb);
+  cplx<T>* T2 = scratch_as<cplx<T>>(c, std::string("prop_T2") + (sizeof(T) == 8 ? "64" : "32"),
+                                     (size_t)W * tb);
   // TMN = P1 * Phi  (written to S and T1)
   {
     Opnd<double, true, false> B{&a.Phi->re, n, a.phi_bs};

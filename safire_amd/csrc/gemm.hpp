@@ -287,10 +287,14 @@ __global__ void __launch_bounds__(WAVES_M * WAVES_N * 64)
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wm0 = (wave / WAVES_N) * WM, wn0 = (wave % WAVES_N) * WN;
-  long bid = blockIdx.x;
-  const int tn = bid % tiles_n;
-  const int tm = (bid / tiles_n) % tiles_m;
-  const long rest = bid / ((long)tiles_n * tiles_m);
+  // Workgroups are dispatched round-robin over the 8 XCDs; renumber them so that each XCD takes a
+  // contiguous run of tiles, row tiles fastest: the workgroups resident on one XCD at a time
+  // then share a few column tiles of B (the Cholesky operand of vbias/vHS) through its L2.
+  const long total = gridDim.x, xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
+  const long q = xcd * (total / 8) + min(xcd, total % 8) + slot;
+  const int tm = (int)(q % tiles_m);
+  const int tn = (int)((q / tiles_m) % tiles_n);
+  const long rest = q / ((long)tiles_n * tiles_m);
   const int ks = (int)(rest % ksplit);
   const int b = (int)(rest / ksplit);
   const int m0 = tm * BM, n0 = tn * BN;

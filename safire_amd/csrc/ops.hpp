@@ -76,6 +76,7 @@ struct PropagateArgs {
   z64* Phi;                // [W][N][n] (FP64 master copy in both precisions)
   long phi_bs;
   Precision prec;
+  bool fused = true;       // one-launch kernel when the shape allows; false: per-product GEMMs
 };
 void propagate(Context& c, const PropagateArgs& a);
 // orthogonalize (orthogonalize.hpp:131-169): Phi <- Q with Phi = Q R, R upper triangular with a

@@ -95,6 +95,48 @@ void tuned(Context& c, const std::string& key, int ncand, F&& launch, bool idemp
   launch(bi);
 }
 
+// In-place kernels (the fused propagation): candidates are timed on a scratch copy of the buffer
+// they update, which they all turn into the same bits, then the chosen one runs on the buffer.
+template <class F>
+void tuned_on_copy(Context& c, const std::string& key, int ncand, z64* buf, size_t count, F&& run) {
+  Tuner& t = Tuner::get();
+  int choice = t.lookup(key);
+  if (choice >= 0 && choice < ncand) {
+    run(choice, buf);
+    return;
+  }
+  if (ncand <= 1 || !t.enabled() || c.capturing()) {
+    run(0, buf);
+    return;
+  }
+  hipStream_t s = c.stream();
+  z64* tmp = scratch_as<z64>(c, "tune_copy", count);
+  hipEvent_t e0, e1;
+  AFQ_HIP(hipEventCreate(&e0));
+  AFQ_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  int bi = 0;
+  for (int i = 0; i < ncand; ++i) {
+    AFQ_HIP(hipMemcpyAsync(tmp, buf, count * sizeof(z64), hipMemcpyDeviceToDevice, s));
+    run(i, tmp);
+    AFQ_HIP(hipMemcpyAsync(tmp, buf, count * sizeof(z64), hipMemcpyDeviceToDevice, s));
+    AFQ_HIP(hipEventRecord(e0, s));
+    run(i, tmp);
+    AFQ_HIP(hipEventRecord(e1, s));
+    AFQ_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    AFQ_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) {
+      best = ms;
+      bi = i;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  t.store(key, bi, best);
+  run(bi, buf);
+}
+
 // Shapes are bucketed (each extent rounded up to a power of two) so that products of similar
 // size, e.g. the walker counts of different jobs, share one tuning decision.
 inline int tune_bucket(int x) {
