@@ -192,7 +192,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=6)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
-    ap.add_argument("--only", choices=["vhs", "vhs64", "copy", "step", "phmsd"], default=None,
+    ap.add_argument("--only", choices=["vhs", "vhs64", "copy", "step", "step64", "phmsd"], default=None,
                     help="profiling modes: launch only the named kernel(s)")
     ap.add_argument("--serve-jobs", type=int, default=24,
                     help="jobs of the serving run (Poisson arrivals at 4/s; 200 = the full "
@@ -233,7 +233,7 @@ def main():
 
     out = {}
     for prec, name in ((afq.AFQ_MIXED, "mixed"), (afq.AFQ_FP64, "fp64")):
-        if name == "fp64" and args.skip_fp64:
+        if (name == "fp64" and args.skip_fp64) or (name == "mixed" and args.only == "step64"):
             continue
         log(f"[{name}] setup")
         ctx, ham, prop, ws, Esh, eout = gpu_setup(problem, prec, W)
@@ -248,6 +248,8 @@ def main():
             break
 
     log("autotuned tiles:\n" + afq.tuning_report())
+    if args.only == "step64":
+        return
     emax = emed = erel = emag = None
     if args.only is None:
         log("energy error mixed vs fp64 (one step, 64 walkers)")

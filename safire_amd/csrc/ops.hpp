@@ -204,6 +204,9 @@ class PhmsdTrial {
     int* cptr = nullptr;         // device [nconf + 1] couplings to the other spin
     int* ccol = nullptr;
     z64* cval = nullptr;         // conj(ci)
+    int* kptr = nullptr;         // device [nconf + 1] coupling-chunk range of each row
+    long nchunk = 0;
+    int hlen_max = 0;            // longest hole list
     z64* Lext = nullptr;         // device [(M + 1)][nact][N]: half-rotated L then haj
     const z64* orbs = nullptr;   // device [nact][N] (rows of the hamiltonian's trial)
   };
@@ -213,6 +216,7 @@ class PhmsdTrial {
   int ndet_ = 0;
   int* det_ab_ = nullptr;        // device [ndet][2]
   z64* det_c_ = nullptr;         // device [ndet] conj(ci)
+  int4* chunks_ = nullptr;       // device coupling chunks (row, begin, end, 0), alpha then beta
   std::vector<int> det_ab_h_;
   std::vector<z64> det_c_h_;
   void woodbury(int s, const z64* Phi, int W, bool want_G, z64* log_ov);

@@ -249,22 +249,86 @@ __global__ void __launch_bounds__(256)
   if (threadIdx.x == 0) Ov[w] = zadd(zadd(part[0], part[1]), zadd(part[2], part[3]));
 }
 
+// The same weights from coupling chunks (at most kChunk entries of one row each): one thread
+// per chunk with both spins' overlaps in LDS, then one thread per row adds its chunks in order,
+// and Ov adds the alpha weights in a fixed tree.  chunks [nka + nkb] = (row, begin, end); chunk
+// ranges per row in kpa / kpb.
+constexpr int kChunk = 16;
+__global__ void __launch_bounds__(256)
+    k_ph_weights_chunked(int na, int nb, const z64* __restrict__ ova, const z64* __restrict__ ovb,
+                         const int* __restrict__ cca, const z64* __restrict__ cva,
+                         const int* __restrict__ ccb, const z64* __restrict__ cvb,
+                         const int4* __restrict__ chunks, int nka, int nkb,
+                         const int* __restrict__ kpa, const int* __restrict__ kpb,
+                         z64* __restrict__ wa, z64* __restrict__ wb, z64* __restrict__ Ov) {
+  extern __shared__ z64 sm[];
+  z64* soa = sm;                  // [na]
+  z64* sob = soa + na;            // [nb]
+  z64* part = sob + nb;           // [nka + nkb]
+  __shared__ z64 red[256];
+  const int w = blockIdx.x, tid = threadIdx.x;
+  for (int u = tid; u < na; u += 256) soa[u] = ova[(long)w * na + u];
+  for (int v = tid; v < nb; v += 256) sob[v] = ovb[(long)w * nb + v];
+  __syncthreads();
+  for (int t = tid; t < nka + nkb; t += 256) {
+    const bool isa = t < nka;
+    const int4 c = chunks[t];
+    const int* cc = isa ? cca : ccb;
+    const z64* cv = isa ? cva : cvb;
+    const z64* other = isa ? sob : soa;
+    z64 acc = {0.0, 0.0};
+    for (int k = c.y; k < c.z; ++k) acc = zfma(cv[k], other[cc[k]], acc);
+    part[t] = acc;
+  }
+  __syncthreads();
+  z64 osum = {0.0, 0.0};
+  for (int r = tid; r < na + nb; r += 256) {
+    const bool isa = r < na;
+    const int u = isa ? r : r - na;
+    const int* kp = isa ? kpa : kpb;
+    const z64* pp = isa ? part : part + nka;
+    z64 acc = {0.0, 0.0};
+    for (int i = kp[u]; i < kp[u + 1]; ++i) acc = zadd(acc, pp[i]);
+    if (isa) {
+      const z64 x = zmul(soa[u], acc);
+      wa[(long)w * na + u] = x;
+      osum = zadd(osum, x);
+    } else {
+      wb[(long)w * nb + u] = zmul(sob[u], acc);
+    }
+  }
+  osum = block_sum<256>(osum, red);
+  if (tid == 0) Ov[w] = osum;
+}
+
 // R[w][j][a] / Ov with R = sum_u w_u (P_u + sum_p e_{h_p} Rc_u[p]) (calculate_R,
 // phmsd_helpers.hpp:201-290), walked by hole position j over the excitation rows that empty it:
-// hent = (configuration, row of Rc, particle orbital).
-__global__ void __launch_bounds__(256)
+// hent = (configuration, row of Rc, particle orbital).  S = blockDim / (nel * nact) threads
+// share one (j, a): thread `sub` takes every S-th entry, and the S partial sums are added in
+// order.  The weights are staged in LDS when they fit (wlds).
+__global__ void __launch_bounds__(1024)
     k_ph_R(int nel, int nact, int nconf, int P, const int* __restrict__ hptr,
            const int4* __restrict__ hent, const z64* __restrict__ Rc,
-           const z64* __restrict__ wgt, const z64* __restrict__ Ov, z64* __restrict__ Rout) {
-  const int w = blockIdx.x;
-  const z64 ov = Ov[w];
-  const z64 inv = zrecip(ov);
+           const z64* __restrict__ wgt, const z64* __restrict__ Ov, z64* __restrict__ Rout,
+           int wlds) {
+  extern __shared__ z64 sm[];
+  const int w = blockIdx.x, tid = threadIdx.x;
+  const int NQ = nel * nact;
+  const int S = max(1, (int)blockDim.x / NQ);
+  z64* part = sm;                    // [S][NQ]
+  z64* swg = sm + (long)S * NQ;      // [nconf] when wlds
   const z64* wg = wgt + (long)w * nconf;
+  if (wlds) {
+    for (int u = tid; u < nconf; u += blockDim.x) swg[u] = wg[u];
+    __syncthreads();
+    wg = swg;
+  }
   const z64* rcw = Rc + (long)w * P * nact;
-  for (int t = threadIdx.x; t < nel * nact; t += 256) {
-    const int j = t / nact, a = t % nact;
-    z64 val = a == j ? ov : z64{0.0, 0.0};
-    for (int k = hptr[j]; k < hptr[j + 1]; ++k) {
+  for (int t = tid; t < S * NQ; t += blockDim.x) {
+    const int q = t % NQ, sub = t / NQ;
+    const int j = q / nact, a = q % nact;
+    z64 val = {0.0, 0.0};
+    for (int k = hptr[j] + sub; k < hptr[j + 1]; k += S) {
       const int4 e = hent[k];
       const z64 wu = wg[e.x];
       z64 rc = rcw[(long)e.y * nact + a];
@@ -272,6 +336,15 @@ __global__ void __launch_bounds__(256)
       if (a == j) rc.re -= 1.0;
       val = zfma(wu, rc, val);
     }
+    part[(long)sub * NQ + q] = val;
+  }
+  __syncthreads();
+  const z64 ov = Ov[w];
+  const z64 inv = zrecip(ov);
+  for (int q = tid; q < NQ; q += blockDim.x) {
+    const int j = q / nact, a = q % nact;
+    z64 val = a == j ? ov : z64{0.0, 0.0};
+    for (int sub = 0; sub < S; ++sub) val = zadd(val, part[(long)sub * NQ + q]);
     Rout[((long)w * nel + j) * nact + a] = zmul(val, inv);
   }
 }
@@ -673,6 +746,7 @@ PhmsdTrial::PhmsdTrial(CholeskyOps& ham, const PhExcitations& ex) : ham_(ham), c
         hent.insert(hent.end(), byhole[j].begin(), byhole[j].end());
       }
       hptr[d.nel] = (int)hent.size() / 4;
+      for (int j = 0; j < d.nel; ++j) d.hlen_max = std::max(d.hlen_max, hptr[j + 1] - hptr[j]);
       if (hent.empty()) hent.assign(4, 0);
       d.exc = dcopy(s, d.exc_h);
       d.rank = dcopy(s, d.rank_h);
@@ -693,6 +767,7 @@ PhmsdTrial::PhmsdTrial(CholeskyOps& ham, const PhExcitations& ex) : ham_(ham), c
     // configurations with conj(ci), in determinant order
     ndet_ = ex.ndet;
     std::vector<std::vector<std::pair<int, z64>>> rows[2];
+    std::vector<int> chunk_h;   // (row, begin, end, 0) per chunk, alpha rows then beta rows
     rows[0].resize(sp_[0].nconf);
     rows[1].resize(sp_[1].nconf);
     det_ab_h_.resize(2 * (size_t)ndet_);
@@ -721,7 +796,18 @@ PhmsdTrial::PhmsdTrial(CholeskyOps& ham, const PhExcitations& ex) : ham_(ham), c
       sp_[sp].cptr = dcopy(s, ptr);
       sp_[sp].ccol = dcopy(s, col);
       sp_[sp].cval = dcopy(s, val);
+      // chunks of at most kChunk couplings per row, rows in order
+      std::vector<int> kp(1, 0);
+      for (int u = 0; u < sp_[sp].nconf; ++u) {
+        for (int k0 = ptr[u]; k0 < ptr[u + 1]; k0 += kChunk)
+          chunk_h.insert(chunk_h.end(), {u, k0, std::min(k0 + kChunk, ptr[u + 1]), 0});
+        kp.push_back((int)chunk_h.size() / 4 - (sp == 0 ? 0 : (int)sp_[0].nchunk));
+      }
+      sp_[sp].nchunk = kp.back();
+      sp_[sp].kptr = dcopy(s, kp);
     }
+    if (chunk_h.empty()) chunk_h.assign(4, 0);
+    chunks_ = reinterpret_cast<int4*>(dcopy(s, chunk_h));
     det_ab_ = dcopy(s, det_ab_h_);
     det_c_ = dcopy(s, det_c_h_);
     AFQ_HIP(hipStreamSynchronize(s));
@@ -739,13 +825,14 @@ PhmsdTrial::~PhmsdTrial() {
 void PhmsdTrial::free_all() {
   for (auto& d : sp_)
     for (void* p : {(void*)d.exc, (void*)d.rank, (void*)d.mstart, (void*)d.rstart, (void*)d.hptr, (void*)d.hent,
-                    (void*)d.cptr, (void*)d.ccol, (void*)d.cval, (void*)d.Lext})
+                    (void*)d.cptr, (void*)d.ccol, (void*)d.cval, (void*)d.kptr, (void*)d.Lext})
       if (p) (void)hipFree(p);
-  for (void* p : {(void*)det_ab_, (void*)det_c_})
+  for (void* p : {(void*)det_ab_, (void*)det_c_, (void*)chunks_})
     if (p) (void)hipFree(p);
   for (auto& d : sp_) d = SpinDev{};
   det_ab_ = nullptr;
   det_c_ = nullptr;
+  chunks_ = nullptr;
 }
 
 static std::string tag(const char* base, int s) { return std::string(base) + (s ? "1" : "0"); }
@@ -818,8 +905,16 @@ void PhmsdTrial::weights(int W, z64* Ov) {
   z64* ovb = scratch_as<z64>(ctx_, "ph_ovr1", (size_t)W * b.nconf);
   z64* wa = scratch_as<z64>(ctx_, "ph_w0", (size_t)W * a.nconf);
   z64* wb = scratch_as<z64>(ctx_, "ph_w1", (size_t)W * b.nconf);
-  hipLaunchKernelGGL(k_ph_weights, dim3(W), dim3(256), 0, ctx_.stream(), a.nconf, b.nconf, ova, ovb,
-                     a.cptr, a.ccol, a.cval, b.cptr, b.ccol, b.cval, wa, wb, Ov);
+  const int nka = (int)a.nchunk, nkb = (int)b.nchunk;
+  const size_t lds = sizeof(z64) * ((size_t)a.nconf + b.nconf + nka + nkb);
+  if (lds <= 48 * 1024) {
+    hipLaunchKernelGGL(k_ph_weights_chunked, dim3(W), dim3(256), lds, ctx_.stream(), a.nconf,
+                       b.nconf, ova, ovb, a.ccol, a.cval, b.ccol, b.cval, chunks_, nka, nkb,
+                       a.kptr, b.kptr, wa, wb, Ov);
+  } else {
+    hipLaunchKernelGGL(k_ph_weights, dim3(W), dim3(256), 0, ctx_.stream(), a.nconf, b.nconf, ova,
+                       ovb, a.cptr, a.ccol, a.cval, b.cptr, b.ccol, b.cval, wa, wb, Ov);
+  }
   AFQ_LAUNCH_CHECK();
 }
 
@@ -840,11 +935,24 @@ void PhmsdTrial::mixed_density_matrix(const z64* Pa, const z64* Pb, int W, z64* 
   for (int sp = 0; sp < 2; ++sp) {
     const SpinDev& d = sp_[sp];
     z64* R = scratch_as<z64>(ctx_, "ph_R", (size_t)W * d.nel * d.nact);
-    hipLaunchKernelGGL(k_ph_R, dim3(W), dim3(256), 0, s, d.nel, d.nact, d.nconf, d.P, d.hptr,
-                       reinterpret_cast<const int4*>(d.hent),
-                       scratch_as<z64>(ctx_, tag("ph_Rc", sp), (size_t)W * std::max(d.P, 1) * d.nact),
-                       scratch_as<z64>(ctx_, tag("ph_w", sp), (size_t)W * d.nconf), Osum, R);
-    AFQ_LAUNCH_CHECK();
+    {
+      const int NQ = d.nel * d.nact;
+      // enough threads per (j, a) to cover the longest hole list in about 8 passes; the kernel
+      // derives the same S = threads / NQ
+      const int S0 = std::max(1, std::min({1024 / NQ, ceil_div(d.hlen_max, 8), 32}));
+      const int threads = NQ >= 1024 ? 1024 : std::min(1024, ((S0 * NQ + 63) / 64) * 64);
+      const int S = std::max(1, threads / NQ);
+      const size_t base = sizeof(z64) * (size_t)S * NQ;
+      const int wlds = base + sizeof(z64) * (size_t)d.nconf <= 48 * 1024 ? 1 : 0;
+      AFQ_CHECK(base <= 48 * 1024, kErrShape, "active space too large for the R assembly kernel");
+      const size_t lds = base + (wlds ? sizeof(z64) * (size_t)d.nconf : 0);
+      hipLaunchKernelGGL(k_ph_R, dim3(W), dim3(threads), lds, s, d.nel, d.nact, d.nconf, d.P,
+                         d.hptr, reinterpret_cast<const int4*>(d.hent),
+                         scratch_as<z64>(ctx_, tag("ph_Rc", sp), (size_t)W * std::max(d.P, 1) * d.nact),
+                         scratch_as<z64>(ctx_, tag("ph_w", sp), (size_t)W * d.nconf), Osum, R,
+                         wlds);
+      AFQ_LAUNCH_CHECK();
+    }
     // G rows of this spin: G[a][k] = sum_j R[j][a] GA[j][k]
     const z64* GA = scratch_as<z64>(ctx_, tag("ph_GA", sp), (size_t)W * d.nel * N);
     Opnd<double, true, false> A{&R->re, d.nact, (long)d.nel * d.nact};
