@@ -66,7 +66,9 @@ def initialize(prop: "afq.Propagator", ws: "afq.WalkerSet", precision=afq.AFQ_FP
 
 def run_afqmc(prop: "afq.Propagator", ws: "afq.WalkerSet", steps: int, equilibration: int = 100,
               ortho_interval: int = 5, popctl_interval: int = 10, measure_interval: int = 10,
-              relax: float | None = None, energy_precision=afq.AFQ_FP64, Eshift=None) -> Trace:
+              relax: float | None = None, energy_precision=afq.AFQ_FP64, Eshift=None,
+              progress=None, progress_interval: int = 1000) -> Trace:
+    """progress(step, trace), when given, is called every progress_interval steps."""
     import torch
     W = ws.W
     if relax is None:  # parameter_defaults.cpp:310-318
@@ -94,6 +96,8 @@ def run_afqmc(prop: "afq.Propagator", ws: "afq.WalkerSet", steps: int, equilibra
             tr.energy.append(complex(vals[:3].sum()))
             tr.total_weight.append(complex(vals[3]))
             tr.eff_walkers.append(float(vals[4].real))
+        if progress is not None and (step + 1) % progress_interval == 0:
+            progress(step + 1, tr)
     torch.cuda.synchronize()
     tr.growth = float(afq.host(e_out)[0])
     tr.eshift = float(afq.host(Eshift)[0])

@@ -1,0 +1,80 @@
+"""Copies the BH (boron hydride, 19 orbitals, 78 Cholesky vectors) input files of the reference's
+functional tests into tests/golden/bh/ and extracts its statistical reference results
+(tests/functional/statistical_references/BH/dense_rhf_basis_*/<trial>/<walkers>/results.h5: the
+run parameters from `input_file`, and mean / error of Energy and its components) with h5dump.
+Run in the container that holds the reference; the GPU box only reads the copies.
+
+    python tests/golden/make_bh_fixtures.py
+"""
+import json
+import os
+import re
+import shutil
+import subprocess
+
+REF = "/root/reference/tests/functional"
+H5DUMP = "/opt/conda/bin/h5dump"
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUT = os.path.join(HERE, "bh")
+INPUTS = ["afqmc_H_rhf_closed.h5", "afqmc_H_rhf_collinear.h5", "afqmc_rhf_nomsd.h5",
+          "afqmc_uhf_nomsd.h5", "afqmc_casci_rhf_phmsd.h5", "afqmc_casci_uhf_phmsd.h5"]
+CASES = [("dense_rhf_basis_closed", "rhf_nomsd", "closed"),
+         ("dense_rhf_basis_closed", "rhf_nomsd", "collinear"),
+         ("dense_rhf_basis_closed", "uhf_nomsd", "collinear"),
+         ("dense_rhf_basis_closed", "rcasci_rhf_phmsd", "collinear"),
+         ("dense_rhf_basis_closed", "rcasci_uhf_phmsd", "collinear")]
+
+
+def dump_value(path, dataset):
+    out = subprocess.run([H5DUMP, "-m", "%.17g", "-d", dataset, path], check=True,
+                         capture_output=True, text=True).stdout
+    body = out[out.index("DATA {"):]
+    nums = re.findall(r"[-+]?\d+\.?\d*(?:[eE][-+]?\d+)?", body.split("}")[0] + "}" + body.split("}")[1])
+    return [float(x) for x in nums[1:]]   # drop the "(0)" index
+
+
+def dump_string(path, dataset):
+    out = subprocess.run([H5DUMP, "-d", dataset, path], check=True, capture_output=True,
+                         text=True).stdout
+    m = re.search(r'\(0\): "(.*)"\s*\}', out, re.S)
+    return m.group(1).replace('\\"', '"')
+
+
+def main():
+    os.makedirs(OUT, exist_ok=True)
+    for f in INPUTS:
+        shutil.copyfile(os.path.join(REF, "afqmc_inputs", "BH", f), os.path.join(OUT, f))
+    refs = {}
+    for basis, trial, walkers in CASES:
+        path = os.path.join(REF, "statistical_references", "BH", basis, trial, walkers, "results.h5")
+        if not os.path.exists(path):
+            continue
+        inp = json.loads(dump_string(path, "/input_file"))
+        ex = inp["execute"]
+        nranks = int(dump_value(path, "/num_ranks")[0])
+        obs = {}
+        for name in ("Energy", "OnebodyEnergy", "CoulombEnergy", "ExchangeEnergy"):
+            mean = dump_value(path, f"/measurements/Stage0/{name}/mean")
+            err = dump_value(path, f"/measurements/Stage0/{name}/error")
+            obs[name] = {"mean": mean[:2], "error": err[:2]}
+        refs[f"{trial}/{walkers}"] = {
+            "hamiltonian": os.path.basename(ex["hamiltonian"]["filename"]),
+            "wavefunction": os.path.basename(ex["wavefunction"]["filename"]),
+            "walker_type": ex["walker_set"]["walker_type"],
+            "timestep": ex["timestep"], "steps": ex["steps"],
+            "walkers": ex.get("n_walkers_per_mpi_task", ex.get("num_walkers", 0)) * nranks,
+            "num_ranks": nranks,
+            "equilibration_steps": ex.get("equilibration_steps"),
+            "measure_interval": ex.get("measure_interval"),
+            "population_control_interval": ex.get("population_control_interval"),
+            "walker_ortho_interval": ex.get("walker_ortho_interval"),
+            "seed": inp.get("seed"),
+            "observables": obs,
+        }
+    with open(os.path.join(OUT, "statistical_refs.json"), "w") as f:
+        json.dump(refs, f, indent=1, sort_keys=True)
+    print(json.dumps(refs, indent=1)[:2000])
+
+
+if __name__ == "__main__":
+    main()
