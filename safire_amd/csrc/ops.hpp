@@ -217,6 +217,7 @@ class PhmsdTrial {
   int* det_ab_ = nullptr;        // device [ndet][2]
   z64* det_c_ = nullptr;         // device [ndet] conj(ci)
   int4* chunks_ = nullptr;       // device coupling chunks (row, begin, end, 0), alpha then beta
+  int* crow_ = nullptr;          // device alpha row of each coupling [ndet]
   std::vector<int> det_ab_h_;
   std::vector<z64> det_c_h_;
   void woodbury(int s, const z64* Phi, int W, bool want_G, z64* log_ov);

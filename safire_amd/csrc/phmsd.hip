@@ -363,34 +363,32 @@ __global__ void k_ph_fill_one(int W, int nconf, z64* __restrict__ ovr) {
 
 // ---- energy -----------------------------------------------------------------------------------
 
-// In place on T3 [W][NS][nact][nel]: particle rows x >= nel become V[x][l] = T[x][l] -
-// sum_i QQ0[x][i] T[i][l]; jr[w][n] = sum_i T[i][i] (Jref), tz[w][n] = sum_ik T[k][i] T[i][k].
-__global__ void k_ph_energy_prep(int W, int NS, int nel, int nact, int nvir,
-                                 const z64* __restrict__ Qv, z64* __restrict__ T3,
+// T3 [W][NS][nact][nel] <- T3[w][n][a][l] = sum_k Lext[n][a][k] GA[w][l][k]: one GEMM over all
+// walkers, rows (n, a), columns (w, l), stored walker-major.
+struct EpiStoreWL {
+  z64* t3;
+  int nel;
+  long wstride;  // NS * nact * nel
+  __device__ inline void operator()(int, int m, int c, double re, double im) const {
+    const int w = c / nel, l = c % nel;
+    t3[(long)w * wstride + (long)m * nel + l] = {re, im};
+  }
+};
+
+// jr[w][n] = sum_i T[i][i] (Jref), tz[w][n] = sum_ik T[k][i] T[i][k] over the reference rows.
+__global__ void k_ph_energy_prep(int W, int NS, int nel, int nact, const z64* __restrict__ T3,
                                  z64* __restrict__ jr, z64* __restrict__ tz) {
   const long slice = (long)nact * nel;
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long nv = (long)W * NS * nvir * nel;
-  if (t < nv) {
-    const int l = (int)(t % nel), x = (int)((t / nel) % nvir);
-    const long wn = t / ((long)nvir * nel);
-    const int w = (int)(wn / NS);
-    z64* T = T3 + wn * slice;
-    const z64* Q = Qv + ((long)w * nvir + x) * nel;
-    z64 b = {0.0, 0.0};
-    for (int i = 0; i < nel; ++i) b = zfma(Q[i], T[i * nel + l], b);
-    T[(nel + x) * nel + l] = zsub(T[(nel + x) * nel + l], b);
-  } else if (t < nv + (long)W * NS) {
-    const long wn = t - nv;
-    const z64* T = T3 + wn * slice;
-    z64 j = {0.0, 0.0}, z = {0.0, 0.0};
-    for (int i = 0; i < nel; ++i) {
-      j = zadd(j, T[i * nel + i]);
-      for (int k = 0; k < nel; ++k) z = zfma(T[k * nel + i], T[i * nel + k], z);
-    }
-    jr[wn] = j;
-    tz[wn] = z;
+  const long wn = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (wn >= (long)W * NS) return;
+  const z64* T = T3 + wn * slice;
+  z64 j = {0.0, 0.0}, z = {0.0, 0.0};
+  for (int i = 0; i < nel; ++i) {
+    j = zadd(j, T[i * nel + i]);
+    for (int k = 0; k < nel; ++k) z = zfma(T[k * nel + i], T[i * nel + k], z);
   }
+  jr[wn] = j;
+  tz[wn] = z;
 }
 
 // Excitations of rank R, configurations [u0, u1), slices [n0, n0 + nc) of walker w. With
@@ -435,15 +433,28 @@ template <int R>
 __global__ void __launch_bounds__(256)
     k_ph_energy_rank(int nel, int nact, int nconf, int M, int NS, int NC, int u0, int u1,
                      const int* __restrict__ exc, const int* __restrict__ mstart, int P2,
-                     const z64* __restrict__ T3, const z64* __restrict__ jr,
-                     const z64* __restrict__ Minv, z64* __restrict__ part, z64* __restrict__ KE) {
+                     const z64* __restrict__ T3, const z64* __restrict__ Qv,
+                     const z64* __restrict__ jr, const z64* __restrict__ Minv,
+                     z64* __restrict__ part, z64* __restrict__ KE) {
   extern __shared__ z64 smem[];
-  const int slice = nact * nel;
+  const int slice = nact * nel, nvir = nact - nel;
   const int chunk = blockIdx.x, w = blockIdx.y, nch = gridDim.x;
   const int n0 = chunk * NC, nc = min(NC, NS - n0);
   z64* red = smem + (long)NC * slice;   // [256][3]
+  z64* sq = red + 3 * 256;              // [nvir][nel] QQ0 rows of the particles
   const z64* src = T3 + ((long)w * NS + n0) * slice;
   for (int t = threadIdx.x; t < nc * slice; t += 256) smem[t] = src[t];
+  for (int t = threadIdx.x; t < nvir * nel; t += 256) sq[t] = Qv[(long)w * nvir * nel + t];
+  __syncthreads();
+  // particle rows x >= nel become V[x][l] = T[x][l] - sum_i QQ0[x][i] T[i][l] (the reference
+  // rows are only read)
+  for (int t = threadIdx.x; t < nc * nvir * nel; t += 256) {
+    const int l = t % nel, x = (t / nel) % nvir, c = t / (nvir * nel);
+    z64* T = smem + (long)c * slice;
+    z64 b = {0.0, 0.0};
+    for (int i = 0; i < nel; ++i) b = zfma(sq[x * nel + i], T[i * nel + l], b);
+    T[(nel + x) * nel + l] = zsub(T[(nel + x) * nel + l], b);
+  }
   __syncthreads();
   const z64* jw = jr + (long)w * NS + n0;
   const int cnt = u1 - u0;
@@ -518,19 +529,66 @@ __global__ void k_ph_ke_ref(int W, int M, int NS, int nconf, const z64* __restri
   KE[(long)w * nconf * M + n] = jr[(long)w * NS + n];
 }
 
-// energy_alg1 assembly per walker: reference terms times Ov, weighted excitation terms of both
-// spins, the opposite-spin Coulomb coupling sum_d conj(ci) ova ovb Ka . Kb, then / Ov.
+// Opposite-spin Coulomb coupling over Cholesky slices [n0, n0 + CH) of walker w:
+//   out[w][chunk] = sum_d conj(ci_d) ova[a_d] ovb[b_d] sum_n Ka[a_d][n] Kb[b_d][n]
+// with both spins' slices staged in LDS; determinants in alpha-row order (crow, cca, cva);
+// per-thread sums over a fixed set of determinants, then a fixed tree.
 __global__ void __launch_bounds__(256)
-    k_ph_energy_combine(int na, int nb, int ncha, int nchb, int M, int NS, int ndet, double E0,
+    k_ph_cross(int na, int nb, int M, int CH, int lds, int ndet, const int* __restrict__ crow,
+               const int* __restrict__ ccol, const z64* __restrict__ cval,
+               const z64* __restrict__ ova, const z64* __restrict__ ovb,
+               const z64* __restrict__ KEa, const z64* __restrict__ KEb, z64* __restrict__ out) {
+  extern __shared__ z64 sk[];
+  __shared__ z64 red[256];
+  const int chunk = blockIdx.x, w = blockIdx.y, nch = gridDim.x, tid = threadIdx.x;
+  const int n0 = chunk * CH, cn = min(CH, M - n0);
+  const z64* Ka = KEa + (long)w * na * M;
+  const z64* Kb = KEb + (long)w * nb * M;
+  // slices in LDS ([na][CH] then [nb][CH]) when they fit, else read in place
+  const z64* sa = Ka + n0;
+  const z64* sb = Kb + n0;
+  long ld = M;
+  if (lds) {
+    z64* da = sk;
+    z64* db = sk + (long)na * CH;
+    for (int t = tid; t < na * CH; t += 256) {
+      const int u = t / CH, j = t % CH;
+      da[t] = j < cn ? Ka[(long)u * M + n0 + j] : z64{0.0, 0.0};
+    }
+    for (int t = tid; t < nb * CH; t += 256) {
+      const int v = t / CH, j = t % CH;
+      db[t] = j < cn ? Kb[(long)v * M + n0 + j] : z64{0.0, 0.0};
+    }
+    __syncthreads();
+    sa = da;
+    sb = db;
+    ld = CH;
+  }
+  const z64* oa = ova + (long)w * na;
+  const z64* ob = ovb + (long)w * nb;
+  z64 acc = {0.0, 0.0};
+  for (int d = tid; d < ndet; d += 256) {
+    const int u = crow[d], v = ccol[d];
+    z64 dot = {0.0, 0.0};
+    for (int j = 0; j < cn; ++j) dot = zfma(sa[u * ld + j], sb[v * ld + j], dot);
+    acc = zfma(zmul(cval[d], zmul(oa[u], ob[v])), dot, acc);
+  }
+  acc = block_sum<256>(acc, red);
+  if (tid == 0) out[(long)w * nch + chunk] = acc;
+}
+
+// energy_alg1 assembly per walker: reference terms times Ov, weighted excitation terms of both
+// spins, the opposite-spin Coulomb coupling sum_d conj(ci) ova ovb Ka . Kb (partial sums over
+// Cholesky chunks in cross[w][0..ncross)), then / Ov.
+__global__ void __launch_bounds__(256)
+    k_ph_energy_combine(int na, int nb, int ncha, int nchb, int M, int NS, double E0,
                         const z64* __restrict__ pa, const z64* __restrict__ pb,
                         const z64* __restrict__ jra, const z64* __restrict__ tza,
                         const z64* __restrict__ jrb, const z64* __restrict__ tzb,
                         const z64* __restrict__ wa, const z64* __restrict__ wb,
                         const z64* __restrict__ Osum, const z64* __restrict__ log_ov,
-                        const z64* __restrict__ KEa, const z64* __restrict__ KEb,
-                        const int* __restrict__ cpa, const int* __restrict__ cca,
-                        const z64* __restrict__ cva, const z64* __restrict__ ova,
-                        const z64* __restrict__ ovb, z64* __restrict__ E, z64* __restrict__ Ov) {
+                        const z64* __restrict__ cross, int ncross, z64* __restrict__ E,
+                        z64* __restrict__ Ov) {
   __shared__ z64 red[256];
   const int w = blockIdx.x;
   const z64 ov = Osum[w];
@@ -563,30 +621,11 @@ __global__ void __launch_bounds__(256)
     ej = zfma(ov, {0.5 * r2.re, 0.5 * r2.im}, ej);
     if (threadIdx.x == 0) e1 = zfma(ov, jr[M], e1);
   }
-  // opposite-spin Coulomb: sum_u ova[u] sum_{row u} conj(ci) ovb[v] Ka[u] . Kb[v], one wavefront
-  // per alpha configuration, lanes over the Cholesky index
-  {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const z64* Ka = KEa + (long)w * na * M;
-    const z64* Kb = KEb + (long)w * nb * M;
-    const z64* oa = ova + (long)w * na;
-    const z64* ob = ovb + (long)w * nb;
-    for (int u = wave; u < na; u += 4) {
-      z64 su = {0.0, 0.0};
-      for (int k = cpa[u]; k < cpa[u + 1]; ++k) {
-        const z64 cv = zmul(cva[k], ob[cca[k]]);
-        const z64* kb = Kb + (long)cca[k] * M;
-        z64 d = {0.0, 0.0};
-        for (int n = lane; n < M; n += 64) d = zfma(Ka[(long)u * M + n], kb[n], d);
-        su = zfma(cv, d, su);
-      }
-      ej = zfma(oa[u], su, ej);   // per-lane partial; the block sum below completes it
-    }
-  }
   e1 = block_sum<256>(e1, red);
   ex = block_sum<256>(ex, red);
   ej = block_sum<256>(ej, red);
   if (threadIdx.x == 0) {
+    for (int c = 0; c < ncross; ++c) ej = zadd(ej, cross[(long)w * ncross + c]);
     const z64 inv = zrecip(ov);
     z64 a = zmul(e1, inv);
     E[(long)w * 3 + 0] = {a.re + E0, a.im};
@@ -796,6 +835,12 @@ PhmsdTrial::PhmsdTrial(CholeskyOps& ham, const PhExcitations& ex) : ham_(ham), c
       sp_[sp].cptr = dcopy(s, ptr);
       sp_[sp].ccol = dcopy(s, col);
       sp_[sp].cval = dcopy(s, val);
+      if (sp == 0) {  // alpha row of every coupling (determinants in alpha-row order)
+        std::vector<int> crow(col.size() ? col.size() : 1, 0);
+        for (int u = 0; u < sp_[0].nconf; ++u)
+          for (int k = ptr[u]; k < ptr[u + 1]; ++k) crow[k] = u;
+        crow_ = dcopy(s, crow);
+      }
       // chunks of at most kChunk couplings per row, rows in order
       std::vector<int> kp(1, 0);
       for (int u = 0; u < sp_[sp].nconf; ++u) {
@@ -827,12 +872,13 @@ void PhmsdTrial::free_all() {
     for (void* p : {(void*)d.exc, (void*)d.rank, (void*)d.mstart, (void*)d.rstart, (void*)d.hptr, (void*)d.hent,
                     (void*)d.cptr, (void*)d.ccol, (void*)d.cval, (void*)d.kptr, (void*)d.Lext})
       if (p) (void)hipFree(p);
-  for (void* p : {(void*)det_ab_, (void*)det_c_, (void*)chunks_})
+  for (void* p : {(void*)det_ab_, (void*)det_c_, (void*)chunks_, (void*)crow_})
     if (p) (void)hipFree(p);
   for (auto& d : sp_) d = SpinDev{};
   det_ab_ = nullptr;
   det_c_ = nullptr;
   chunks_ = nullptr;
+  crow_ = nullptr;
 }
 
 static std::string tag(const char* base, int s) { return std::string(base) + (s ? "1" : "0"); }
@@ -1002,23 +1048,24 @@ void PhmsdTrial::energy(const z64* Pa, const z64* Pb, int W, z64* E, z64* Ov) {
   for (int sp = 0; sp < 2; ++sp) {
     const SpinDev& d = sp_[sp];
     const long slice = (long)d.nact * d.nel;
-    const int NC = (int)std::min<long>(32, (48L * 1024) / (slice * (long)sizeof(z64)));
+    // LDS: NC slices, the [256][3] reduction buffer and the particle rows of QQ0 within 64 KB
+    const long budget = 64L * 1024 - (long)sizeof(z64) * (3 * 256 + (long)d.nvir * d.nel);
+    const int NC = (int)std::min<long>(32, budget / (slice * (long)sizeof(z64)));
     AFQ_CHECK(NC >= 1, kErrShape, "active space too large for the PHMSD energy kernel");
     nch[sp] = ceil_div(NS, NC);
     z64* T3 = scratch_as<z64>(ctx_, "ph_T3", (size_t)W * NS * slice);
-    {  // T3[w][n][a][l] = sum_k Lext[n][a][k] GA[w][l][k]
+    {  // T3[w][n][a][l] = sum_k Lext[n][a][k] GA[w][l][k]: rows (n, a), columns (w, l)
       const z64* GA = scratch_as<z64>(ctx_, tag("ph_GA", sp), (size_t)W * d.nel * N);
       Opnd<double, true, true> A{&d.Lext->re, N, 0};
-      Opnd<double, true, true> B{&GA->re, N, (long)d.nel * N};
-      gemm_tall<double>(ctx_, "ph_T3", W, NS * d.nact, d.nel, N, A, B,
-                        EpiStore<double, double>{T3, d.nel, NS * slice, 1, 0, 0});
+      Opnd<double, true, true> B{&GA->re, N, 0};
+      gemm_large<double>(ctx_, "ph_T3", 1, NS * d.nact, W * d.nel, N, A, B,
+                         EpiStoreWL{T3, d.nel, (long)NS * slice});
     }
     jr[sp] = scratch_as<z64>(ctx_, tag("ph_jr", sp), (size_t)W * NS);
     tz[sp] = scratch_as<z64>(ctx_, tag("ph_tz", sp), (size_t)W * NS);
     const z64* Qv = scratch_as<z64>(ctx_, tag("ph_Qv", sp), (size_t)W * std::max(d.nvir, 1) * d.nel);
-    const long nprep = (long)W * NS * d.nvir * d.nel + (long)W * NS;
-    hipLaunchKernelGGL(k_ph_energy_prep, dim3(ceil_div(nprep, 256)), dim3(256), 0, s, W, NS, d.nel,
-                       d.nact, d.nvir, Qv, T3, jr[sp], tz[sp]);
+    hipLaunchKernelGGL(k_ph_energy_prep, dim3(ceil_div((long)W * NS, 256)), dim3(256), 0, s, W, NS,
+                       d.nel, d.nact, T3, jr[sp], tz[sp]);
     AFQ_LAUNCH_CHECK();
     part[sp] = scratch_as<z64>(ctx_, tag("ph_part", sp), (size_t)W * nch[sp] * d.nconf * 3);
     KE[sp] = scratch_as<z64>(ctx_, tag("ph_KE", sp), (size_t)W * M * d.nconf);
@@ -1026,14 +1073,14 @@ void PhmsdTrial::energy(const z64* Pa, const z64* Pb, int W, z64* E, z64* Ov) {
                        d.nconf, jr[sp], KE[sp]);
     AFQ_LAUNCH_CHECK();
     const z64* Mi = scratch_as<z64>(ctx_, tag("ph_Minv", sp), (size_t)W * std::max(d.P2, 1));
-    const size_t shm = sizeof(z64) * ((size_t)NC * slice + 3 * 256);
+    const size_t shm = sizeof(z64) * ((size_t)NC * slice + 3 * 256 + (size_t)d.nvir * d.nel);
     for (int r = 1; r <= d.rmax; ++r) {
       const int u0 = d.rank_off[r], u1 = d.rank_off[r + 1];
       if (u1 <= u0) continue;
       dim3 g(nch[sp], W);
 #define AFQ_ERANK(R)                                                                           \
   hipLaunchKernelGGL(k_ph_energy_rank<R>, g, dim3(256), shm, s, d.nel, d.nact, d.nconf, M, NS, \
-                     NC, u0, u1, d.exc, d.mstart, d.P2, T3, jr[sp], Mi, part[sp], KE[sp])
+                     NC, u0, u1, d.exc, d.mstart, d.P2, T3, Qv, jr[sp], Mi, part[sp], KE[sp])
       switch (r) {
         case 1: AFQ_ERANK(1); break;
         case 2: AFQ_ERANK(2); break;
@@ -1046,13 +1093,25 @@ void PhmsdTrial::energy(const z64* Pa, const z64* Pb, int W, z64* E, z64* Ov) {
       AFQ_LAUNCH_CHECK();
     }
   }
+  // opposite-spin Coulomb term over Cholesky chunks of CH slices
+  const int na = sp_[0].nconf, nb = sp_[1].nconf;
+  // (both spins' slices of a chunk in 64 KB of LDS; read in place for larger expansions)
+  const int CHl = (int)std::min<long>(16, (64L * 1024) / ((long)sizeof(z64) * (na + nb)));
+  const int use_lds = CHl >= 1 ? 1 : 0;
+  const int CH = use_lds ? CHl : 16;
+  const int ncross = ceil_div(M, CH);
+  z64* cross = scratch_as<z64>(ctx_, "ph_cross", (size_t)W * ncross);
+  hipLaunchKernelGGL(k_ph_cross, dim3(ncross, W), dim3(256),
+                     use_lds ? sizeof(z64) * (size_t)(na + nb) * CH : 0, s, na, nb, M, CH,
+                     use_lds, ndet_, crow_, sp_[0].ccol, sp_[0].cval,
+                     scratch_as<z64>(ctx_, "ph_ovr0", (size_t)W * na),
+                     scratch_as<z64>(ctx_, "ph_ovr1", (size_t)W * nb), KE[0], KE[1], cross);
+  AFQ_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_ph_energy_combine, dim3(W), dim3(256), 0, s, sp_[0].nconf, sp_[1].nconf,
-                     nch[0], nch[1], M, NS, ndet_, ham_.E0(), part[0], part[1], jr[0], tz[0],
+                     nch[0], nch[1], M, NS, ham_.E0(), part[0], part[1], jr[0], tz[0],
                      jr[1], tz[1], scratch_as<z64>(ctx_, "ph_w0", (size_t)W * sp_[0].nconf),
                      scratch_as<z64>(ctx_, "ph_w1", (size_t)W * sp_[1].nconf), Osum, log_ov,
-                     KE[0], KE[1], sp_[0].cptr, sp_[0].ccol, sp_[0].cval,
-                     scratch_as<z64>(ctx_, "ph_ovr0", (size_t)W * sp_[0].nconf),
-                     scratch_as<z64>(ctx_, "ph_ovr1", (size_t)W * sp_[1].nconf), E, Ov);
+                     cross, ncross, E, Ov);
   AFQ_LAUNCH_CHECK();
 }
 
