@@ -27,6 +27,11 @@ DT, ORDER, ORTHO = 0.005, 6, 5
 SEED = 1003
 PEAK = {"fp32": 157.3e12, "fp64": 78.6e12}
 HBM_PEAK = 8.0e12
+# HBM bytes of one FP32 vHS launch at this workload: FETCH_SIZE x 2 + WRITE_SIZE x 1 (the
+# factors that make a 1 GiB copy read and write 1 GiB), rocprofv3 --pmc of `bench.py --only vhs`
+# (profiles/vhs_mixed_cfg3_pmc_traffic.md).  Algorithmic: X 13.1 MB + L 128.6 MB + V 327.7 MB.
+VHS_TRAFFIC_BYTES = (615418 * 2 + 516147) * 1024
+VHS_ALGORITHMIC_BYTES = W * M * 8 + (N * (N + 1) // 2) * M * 4 + W * N * N * 8
 
 
 def flops_per_walker_step(N, n, M, K=ORDER, k_ortho=ORTHO):
@@ -99,6 +104,40 @@ def time_vhs(ham, precision, W, reps=10):
     ev1.record()
     ev1.synchronize()
     return ev0.elapsed_time(ev1) / 1e3 / reps
+
+
+def bh_mixed_error(W_e=256, steps=10):
+    """Mixed vs FP64 on BH read from the reference's inputs (19 orbitals, 78 Cholesky vectors,
+    RHF trial): the same walkers and field draws propagated `steps` steps in each precision, then
+    the per-walker local energies (evaluated in FP64) compared, in Hartree."""
+    import torch
+    from safire_amd import afq, h5inputs
+    g = os.path.join(ROOT, "tests", "golden", "bh")
+    s = h5inputs.load_system(os.path.join(g, "afqmc_H_rhf_closed.h5"),
+                             os.path.join(g, "afqmc_rhf_nomsd.h5"))
+    res = {}
+    for prec in (afq.AFQ_FP64, afq.AFQ_MIXED):
+        ctx = afq.Context(0)
+        ham = afq.HamiltonianOperations(ctx, s.h1, s.Likn, s.PsiT, s.nup, E0=s.E0, precision=prec)
+        prop = afq.Propagator(ham, W_e, dt=0.01, seed=42)
+        ws = afq.WalkerSet(s.walkers(W_e)[0])
+        E = afq.empty((W_e, 3), torch.complex128)
+        ov = afq.empty(W_e, torch.complex128)
+        prop.Energy(ws, E, ov)
+        ws.eloc.copy_(E.sum(1))
+        ws.ovlp.copy_(ov)
+        Esh = afq.device_tensor(np.array([float(afq.host(E.sum(1)).real.mean())]))
+        for k in range(steps):
+            prop.set_step(k)
+            prop.Propagate(ws, Esh)
+        prop.Energy(ws, E, ov)
+        res[prec] = afq.host(E.sum(1))
+        prop.close()
+    d = np.abs(res[afq.AFQ_MIXED] - res[afq.AFQ_FP64])
+    return {"system": "BH (19 orbitals, 78 Cholesky vectors, 3+3 electrons, RHF trial) from the "
+                      "reference's test inputs", "walkers": W_e, "steps": steps,
+            "max_Ha": float(d.max()), "median_Ha": float(np.median(d)),
+            "mean_energy_diff_Ha": float(abs(res[afq.AFQ_MIXED].mean() - res[afq.AFQ_FP64].mean()))}
 
 
 def energy_error(problem, W_e=64):
@@ -192,7 +231,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=6)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
-    ap.add_argument("--only", choices=["vhs", "vhs64", "copy", "step", "step64", "phmsd"], default=None,
+    ap.add_argument("--only", choices=["vhs", "vhs64", "copy", "step", "step64", "phmsd", "vendor"],
+                    default=None,
                     help="profiling modes: launch only the named kernel(s)")
     ap.add_argument("--serve-jobs", type=int, default=24,
                     help="jobs of the serving run (Poisson arrivals at 4/s; 200 = the full "
@@ -220,6 +260,23 @@ def main():
     if args.only == "phmsd":
         log(f"PHMSD workload: {args.steps} steps after {args.warmup} warmup")
         log(json.dumps(phmsd_bench(args.warmup, args.steps)))
+        return
+    if args.only == "vendor":
+        # practical ceiling: the vendor GEMM (torch.matmul -> hipBLASLt / rocBLAS) on real
+        # operands of the vHS and vbias shapes and on a large cube
+        for dt in (torch.float32, torch.float64):
+            for (m, k, n) in ((2048, M, N * (N + 1) // 2), (W, NUP * N, 2 * M), (8192, 8192, 8192)):
+                a = torch.randn(m, k, dtype=dt, device="cuda")
+                b = torch.randn(k, n, dtype=dt, device="cuda")
+                a @ b
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(5):
+                    a @ b
+                torch.cuda.synchronize()
+                t = (time.perf_counter() - t0) / 5
+                log(f"{str(dt)[6:]} {m}x{k}x{n}: {t * 1e3:.3f} ms {2 * m * k * n / t / 1e12:.1f} TFLOP/s")
+                del a, b
         return
     if args.only == "copy":
         # calibration kernel of known traffic for the FETCH/WRITE counters: copy 1 GiB
@@ -251,9 +308,13 @@ def main():
     if args.only == "step64":
         return
     emax = emed = erel = emag = None
+    bh_err = None
     if args.only is None:
         log("energy error mixed vs fp64 (one step, 64 walkers)")
         emax, emed, erel, emag = energy_error(problem)
+        log("energy error mixed vs fp64 on BH (10 steps, 256 walkers)")
+        bh_err = bh_mixed_error()
+        log(f"bh: {bh_err}")
 
     phmsd = None
     if args.only is None:
@@ -302,7 +363,9 @@ def main():
                    "precision": "mixed", "graph": "one HIP graph per step"},
         "roofline": {"bound": "mfma", "kernel": "vHS GEMM (FP32 MFMA, [W x M] . [M x N^2])",
                      "achieved": achieved / 1e12, "peak": PEAK["fp32"] / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK["fp32"], "traffic": None},
+                     "unit": "TFLOP/s", "frac": achieved / PEAK["fp32"],
+                     "traffic": VHS_TRAFFIC_BYTES,
+                     "traffic_over_algorithmic": VHS_TRAFFIC_BYTES / VHS_ALGORITHMIC_BYTES},
         "step_roofline_frac": (fstep * W / sec) / PEAK["fp32"],
         "fp64": None if "fp64" not in out else {
             "value": W / out["fp64"]["sec"], "ms_per_step": out["fp64"]["sec"] * 1e3,
@@ -311,6 +374,7 @@ def main():
         "mixed_over_fp64": None if "fp64" not in out else out["fp64"]["sec"] / sec,
         "energy_error_mixed_vs_fp64_Ha": None if emax is None else
             {"max": emax, "median": emed, "max_rel": erel, "median_abs_E": emag},
+        "energy_error_mixed_vs_fp64_BH": bh_err,
         "phmsd": phmsd,
         "serve": serve_summary,
         "cpu_baseline": cpu,
