@@ -212,6 +212,28 @@ int afq_prop_population_control(afq_prop* prop, const afq_walkers* w, double r);
  * population is then left unchanged; the reference aborts).  Synchronizes the stream. */
 int afq_prop_status(afq_prop* prop, int* collapsed);
 
+/* ---- back-propagation (single-determinant trial) --------------------------------------------
+ * Replaces the walker set's back-propagation buffers (WalkerSetBase::resize_bp, storeFields,
+ * getWeightFactors, setSlaterMatrixN) and AFQMCBasePropagator::BackPropagate
+ * (AFQMCBasePropagator.cpp:404-560) as BackPropEstimator (BackPropEstimator.hpp:150-260) uses
+ * them. */
+/* From now on every step stores its fields in a ring of nback slots and its path-restoration
+ * factor (w / |w|) / scale (WalkerSetUpdate.hpp:136-146) in a ring of 3 nback slots per walker,
+ * both indexed by the device step counter (steps must draw their own fields, u = NULL);
+ * population control moves the rings and the anchor with their walker. */
+int afq_prop_enable_backprop(afq_prop* prop, int nback);
+/* Anchor = the walkers' current Slater matrices (BackPropEstimator::setAnchor). */
+int afq_prop_bp_anchor(afq_prop* prop, const afq_walkers* w);
+/* Back-propagates the trial through the last nbp stored fields (newest first, the adjoint
+ * propagator, QR every `ortho` steps and at the end) and writes G [W][nspin][N][N] (complex,
+ * device; nspin = 1 CLOSED, 2 COLLINEAR): the full mixed density matrix between the
+ * back-propagated trial and each walker's anchor, and weights [W] (complex, device): WEIGHT
+ * times the path-restoration factors of the last nbp steps (restore != 0) or WEIGHT. */
+int afq_prop_backpropagate(afq_prop* prop, const afq_walkers* w, int nbp, int ortho, int restore,
+                           double* G, double* weights);
+/* Device views of the rings: fields [W][nback][M] complex, factors [W][3 nback] complex. */
+int afq_prop_bp_views(afq_prop* prop, const double** fields, const double** factors, int* nback);
+
 /* ---- particle-hole multi-determinant trial (PHMSD, COLLINEAR walkers) ----------------------- */
 
 /* The expansion in the ph_excitations numbering (Excitations.hpp:203-530, as build_ph_struct

@@ -89,6 +89,10 @@ _SIGS = {
     "afq_prop_step_graph": [_P, ctypes.POINTER(Walkers), _P, _I, _I, _D, _P],
     "afq_prop_population_control": [_P, ctypes.POINTER(Walkers), _D],
     "afq_prop_status": [_P, ctypes.POINTER(_I)],
+    "afq_prop_enable_backprop": [_P, _I],
+    "afq_prop_bp_anchor": [_P, ctypes.POINTER(Walkers)],
+    "afq_prop_backpropagate": [_P, ctypes.POINTER(Walkers), _I, _I, _I, _P, _P],
+    "afq_prop_bp_views": [_P, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_I)],
     "afq_prop_create_phmsd": [_P, _I, ctypes.POINTER(PropParams), ctypes.POINTER(_P)],
     "afq_phmsd_create": [_P, _P, ctypes.POINTER(_P)],
     "afq_phmsd_destroy": [_P],
@@ -465,6 +469,41 @@ class Propagator:
         v = _I()
         _ck(lib().afq_prop_status(self.h, ctypes.byref(v)))
         return bool(v.value)
+
+    # ---- back-propagation (BackPropEstimator, single-determinant trial) ----
+    def enable_backprop(self, nback: int):
+        """Every later step stores its fields and path-restoration factor (rings of nback and
+        3 nback slots per walker)."""
+        _ck(lib().afq_prop_enable_backprop(self.h, int(nback)))
+        self.nback = int(nback)
+
+    def bp_anchor(self, wset: WalkerSet):
+        _ck(lib().afq_prop_bp_anchor(self.h, ctypes.byref(wset.cstruct())))
+
+    def backpropagate(self, wset: WalkerSet, nbp: int, ortho: int = 10, restore: bool = True):
+        """(G [W][nspin][N][N], weights [W]) of the trial back-propagated over the last nbp
+        steps against the anchors."""
+        torch = _torch()
+        nspin = 2 if self.ham.walker_type == AFQ_COLLINEAR else 1
+        N = self.ham.N
+        G = torch.empty((wset.W, nspin, N, N), dtype=torch.complex128, device="cuda")
+        w = torch.empty(wset.W, dtype=torch.complex128, device="cuda")
+        _ck(lib().afq_prop_backpropagate(self.h, ctypes.byref(wset.cstruct()), int(nbp), int(ortho),
+                                         int(bool(restore)), ptr(G), ptr(w)))
+        return G, w
+
+    def bp_rings(self, W: int):
+        """Copies of the field ring [W][nback][M] and the factor ring [W][3 nback]."""
+        torch = _torch()
+        f, q, n = _P(), _P(), _I()
+        _ck(lib().afq_prop_bp_views(self.h, ctypes.byref(f), ctypes.byref(q), ctypes.byref(n)))
+        self.ctx.synchronize()
+        nb, M = n.value, self.ham.M
+        fields = torch.empty((W, nb, M), dtype=torch.complex128, device="cuda")
+        facs = torch.empty((W, 3 * nb), dtype=torch.complex128, device="cuda")
+        _copy_d2d(fields.data_ptr(), f.value, fields.numel() * 16)
+        _copy_d2d(facs.data_ptr(), q.value, facs.numel() * 16)
+        return host(fields), host(facs)
 
     def close(self):
         if getattr(self, "h", None):

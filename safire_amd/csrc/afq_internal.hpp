@@ -32,10 +32,28 @@ struct UpdateParams {
   double dt, E0, lower_cut, upper_cut, weight_floor, weight_frac, target_pop;
   int free_projection, use_cp;
 };
-// Eshift is read from device memory (so a captured step needs no host value).
+// Eshift is read from device memory (so a captured step needs no host value).  When wfac is
+// given, the step's path-restoration factor (w / |w|) / scale of every walker (0 for a dropped
+// walker; WalkerSetUpdate.hpp:136-146, 184-187) is stored in the ring wfac [W][hl] at slot
+// *step_ptr % hl.
 void walker_update(hipStream_t s, int W, const UpdateParams& p, const double* Eshift,
                    const z64* new_ovlp, const z64* mf, const z64* HW, z64* weight, z64* phase,
-                   z64* eloc, z64* ovlp, z64* theta);
+                   z64* eloc, z64* ovlp, z64* theta, z64* wfac = nullptr, int hl = 0,
+                   const uint32_t* step_ptr = nullptr);
+// ring[w][*step_ptr % nring][m] = X[w][m] (WalkerSetBase::storeFields)
+void store_fields(hipStream_t s, int W, int M, const z64* X, z64* ring, int nring,
+                  const uint32_t* step_ptr);
+// X[w][m] = ring[w][slot][m]
+void gather_fields(hipStream_t s, int W, int M, const z64* ring, int nring, int slot, z64* X);
+// weights[w] = weight[w] * prod_{k < n} wfac[w][(pos - 1 - k) mod hl]  (BackPropEstimator.hpp
+// path restoration); n = 0 copies the weights
+void restore_weights(hipStream_t s, int W, const z64* weight, const z64* wfac, int hl, uint32_t pos,
+                     int n, z64* out);
+// per-walker rows that move with their walker in population control (back-propagation state)
+struct WalkerRows {
+  z64* p[4] = {nullptr, nullptr, nullptr, nullptr};
+  long n[4] = {0, 0, 0, 0};
+};
 void increment_counter(hipStream_t s, uint32_t* c);
 // comb population control of one rank (population_control.hpp:216-257); r_dev = injected
 // uniform or nullptr for Philox(seed, stream, *step_ptr).  cum [W] doubles, src [W] ints and
@@ -43,7 +61,7 @@ void increment_counter(hipStream_t s, uint32_t* c);
 void population_control(hipStream_t s, int W, const double* r_dev, uint64_t seed, uint32_t stream,
                         const uint32_t* step_ptr, long rowA, long rowB, z64* PhiA, z64* PhiB,
                         z64* weight, z64* phase, z64* eloc, z64* ovlp, z64* theta, double* cum,
-                        int* src, int* status);
+                        int* src, int* status, const WalkerRows& extra = WalkerRows{});
 // growth estimator log(sum w e^{dt E} / sum w)/dt -> out[0]; Eshift[0] += f (out - Eshift) when
 // relax > 0.
 void average_pseudo_energy(hipStream_t s, int W, const z64* weight, const z64* eloc, double dt,
@@ -64,6 +82,7 @@ void identity(hipStream_t s, int N, z64* A);
 void axpy_z64(hipStream_t s, long n, double ar, double ai, const z64* x, z64* y);  // y += a x
 void permute_akm_mak(hipStream_t s, int A, int K, int M, const z64* in, z64* out);
 void conj_transpose(hipStream_t s, int R, int C, const z64* in, z64* out);
+void broadcast_z64(hipStream_t s, long W, long n, const z64* src, z64* dst);  // dst[w][:] = src
 // true when every Cholesky vector L[.][.][m] of an [N][N][M] tensor is exactly symmetric
 bool symmetric_cholesky(hipStream_t s, int N, int M, const double* L);
 void real_part(hipStream_t s, long n, const z64* x, double* y);

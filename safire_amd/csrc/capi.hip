@@ -398,6 +398,39 @@ int afq_prop_status(afq_prop* prop, int* collapsed) {
   });
 }
 
+int afq_prop_enable_backprop(afq_prop* prop, int nback) {
+  return guard([&] {
+    REQUIRE(prop, "prop is NULL");
+    prop->p->enable_backprop(nback);
+  });
+}
+
+int afq_prop_bp_anchor(afq_prop* prop, const afq_walkers* w) {
+  return guard([&] {
+    REQUIRE(prop, "prop is NULL");
+    check_walkers(prop, w);
+    prop->p->bp_anchor(to_state(w));
+  });
+}
+
+int afq_prop_backpropagate(afq_prop* prop, const afq_walkers* w, int nbp, int ortho, int restore,
+                           double* G, double* weights) {
+  return guard([&] {
+    REQUIRE(prop && G && weights, "NULL argument");
+    check_walkers(prop, w);
+    prop->p->backpropagate(to_state(w), nbp, ortho, restore != 0, Z(G), Z(weights));
+  });
+}
+
+int afq_prop_bp_views(afq_prop* prop, const double** fields, const double** factors, int* nback) {
+  return guard([&] {
+    REQUIRE(prop && fields && factors && nback, "NULL argument");
+    *fields = reinterpret_cast<const double*>(prop->p->bp_fields());
+    *factors = reinterpret_cast<const double*>(prop->p->bp_factors());
+    *nback = prop->p->backprop_length();
+  });
+}
+
 /* ---- PHMSD ---- */
 
 int afq_phmsd_create(afq_ham* ham, const afq_ph_excitations* ex, afq_phmsd** out) {

@@ -118,6 +118,31 @@ void mixed_density_matrix(Context& c, const z64* PsiT, int n, int N, const z64* 
   }
 }
 
+void mixed_density_matrix_left(Context& c, const z64* L, long l_bs, int n, int N, const z64* Phi,
+                               long phi_bs, int W, z64* G, long g_bs, z64* ovlp) {
+  if (W == 0) return;
+  AFQ_CHECK(n > 0 && n <= N && g_bs >= (long)N * N, kErrShape, "left density matrix shape");
+  hipStream_t s = c.stream();
+  z64* T = scratch_as<z64>(c, "mdl_T", (size_t)W * n * n);
+  {  // T[a][b] = sum_k conj(L[k][a]) Phi[k][b]
+    Opnd<double, true, false, true> A{&L->re, n, l_bs};
+    Opnd<double, true, false> B{&Phi->re, n, phi_bs};
+    gemm_small<double>(c, "mdl_T", W, n, n, N, A, B, EpiStore<double, double>{T, n, (long)n * n, 1, 0, 0});
+  }
+  lu_inverse_logdet(s, T, (long)n * n, W, n, T, (long)n * n, ovlp, 1.0, {0, 0}, true);
+  z64* Gc = scratch_as<z64>(c, "mdl_Gc", (size_t)W * n * N);
+  {  // Gc[a][k] = sum_b Tinv[b][a] Phi[k][b]
+    Opnd<double, true, false> A{&T->re, n, (long)n * n};
+    Opnd<double, true, true> B{&Phi->re, n, phi_bs};
+    gemm_small<double>(c, "mdm_G", W, n, N, n, A, B, EpiStore<double, double>{Gc, N, (long)n * N, 1, 0, 0});
+  }
+  {  // G[i][k] = sum_a conj(L[i][a]) Gc[a][k]
+    Opnd<double, true, true, true> A{&L->re, n, l_bs};
+    Opnd<double, true, false> B{&Gc->re, N, (long)n * N};
+    gemm_small<double>(c, "mdl_full", W, N, N, n, A, B, EpiStore<double, double>{G, N, g_bs, 1, 0, 0});
+  }
+}
+
 void inverse_logdet(Context& c, const z64* A, int nb, int n, z64* ovlp, z64* Ainv) {
   lu_inverse_logdet(c.stream(), A, (long)n * n, nb, n, Ainv, (long)n * n, ovlp, 1.0, {0, 0}, true);
 }

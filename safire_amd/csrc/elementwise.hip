@@ -102,6 +102,11 @@ __global__ void k_conj_transpose(int R, int C, const z64* in, z64* out) {
   z64 v = in[t];
   out[(long)c * R + r] = {v.re, -v.im};
 }
+__global__ void k_broadcast(long W, long n, const z64* src, z64* dst) {
+  long t = (long)blockIdx.x * kT + threadIdx.x;
+  if (t >= W * n) return;
+  dst[t] = src[t % n];
+}
 __global__ void k_real_part(long n, const z64* x, double* y) {
   long i = (long)blockIdx.x * kT + threadIdx.x;
   if (i < n) y[i] = x[i].re;
@@ -291,6 +296,11 @@ void conj_transpose(hipStream_t s, int R, int C, const z64* in, z64* out) {
   long n = (long)R * C;
   if (n <= 0) return;
   hipLaunchKernelGGL(k_conj_transpose, grid1(n), dim3(kT), 0, s, R, C, in, out);
+  AFQ_LAUNCH_CHECK();
+}
+void broadcast_z64(hipStream_t s, long W, long n, const z64* src, z64* dst) {
+  if (W * n <= 0) return;
+  hipLaunchKernelGGL(k_broadcast, grid1(W * n), dim3(kT), 0, s, W, n, src, dst);
   AFQ_LAUNCH_CHECK();
 }
 void real_part(hipStream_t s, long n, const z64* x, double* y) {

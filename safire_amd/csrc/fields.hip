@@ -144,7 +144,8 @@ __global__ void k_walker_update(int W, UpdateParams p, const double* __restrict_
                                 const z64* __restrict__ nov, const z64* __restrict__ mf,
                                 const z64* __restrict__ HW, z64* __restrict__ weight,
                                 z64* __restrict__ phase, z64* __restrict__ eloc_p,
-                                z64* __restrict__ ovlp, z64* __restrict__ theta) {
+                                z64* __restrict__ ovlp, z64* __restrict__ theta,
+                                z64* __restrict__ wfac, int hl, const uint32_t* __restrict__ step_ptr) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= W) return;
   const double Eshift = *Eshift_p;
@@ -161,6 +162,7 @@ __global__ void k_walker_update(int W, UpdateParams p, const double* __restrict_
   z64 e = {(m.re - hw.re - (nv.re - old_ov.re)) / p.dt + p.E0,
            (m.im - hw.im - (nv.im - old_ov.im)) / p.dt};
   z64 w = weight[i], ph = phase[i];
+  z64 fac = {0.0, 0.0};  // path-restoration factor of the step
   if (!isfinite(e.re)) {
     w = {0.0, 0.0};
     ph = {0.0, 0.0};
@@ -169,6 +171,7 @@ __global__ void k_walker_update(int W, UpdateParams p, const double* __restrict_
     if (!p.free_projection) e.re = fmin(fmax(e.re, lo), hi);
     z64 wf = cexp({-p.dt * (e.re - Eshift), -p.dt * e.im});
     double aw = hypot(wf.re, wf.im);
+    if (scale > 2.2250738585072014e-308) fac = {wf.re / aw / scale, wf.im / aw / scale};
     if (p.free_projection) {
       w = cmul(w, z64{scale * wf.re, scale * wf.im});
     } else {
@@ -184,6 +187,36 @@ __global__ void k_walker_update(int W, UpdateParams p, const double* __restrict_
   phase[i] = ph;
   eloc_p[i] = e;
   ovlp[i] = nv;
+  if (wfac) wfac[(long)i * hl + (*step_ptr) % (uint32_t)hl] = fac;
+}
+
+__global__ void k_store_fields(int W, int M, const z64* __restrict__ X, z64* __restrict__ ring,
+                               int nring, const uint32_t* __restrict__ step_ptr) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)W * M) return;
+  const int w = (int)(t / M), m = (int)(t % M);
+  const int slot = (int)((*step_ptr) % (uint32_t)nring);
+  ring[((long)w * nring + slot) * M + m] = X[t];
+}
+
+__global__ void k_gather_fields(int W, int M, const z64* __restrict__ ring, int nring, int slot,
+                                z64* __restrict__ X) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)W * M) return;
+  const int w = (int)(t / M), m = (int)(t % M);
+  X[t] = ring[((long)w * nring + slot) * M + m];
+}
+
+__global__ void k_restore_weights(int W, const z64* __restrict__ weight, const z64* __restrict__ wfac,
+                                  int hl, uint32_t pos, int n, z64* __restrict__ out) {
+  int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  z64 x = weight[w];
+  for (int k = 0; k < n; ++k) {
+    const int slot = (int)(((long)pos - 1 - k) % hl + hl) % hl;
+    x = cmul(x, wfac[(long)w * hl + slot]);
+  }
+  out[w] = x;
 }
 
 constexpr int kAvgThreads = 1024;
@@ -284,13 +317,20 @@ __global__ void __launch_bounds__(kCombThreads)
 // copy walker rows into the freed slots: Slater blocks and all properties
 __global__ void k_copy_walkers(int W, const int* __restrict__ src, long rowA, long rowB,
                                z64* PhiA, z64* PhiB, z64* weight, z64* phase, z64* eloc,
-                               z64* ovlp, z64* theta) {
+                               z64* ovlp, z64* theta, WalkerRows extra) {
   const int d = blockIdx.x;
   const int s = src[d];
   if (s == d) return;
   for (long e = threadIdx.x; e < rowA; e += blockDim.x) PhiA[d * rowA + e] = PhiA[s * rowA + e];
   if (PhiB)
     for (long e = threadIdx.x; e < rowB; e += blockDim.x) PhiB[d * rowB + e] = PhiB[s * rowB + e];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    z64* q = extra.p[r];
+    const long n = extra.n[r];
+    if (q)
+      for (long e = threadIdx.x; e < n; e += blockDim.x) q[d * n + e] = q[s * n + e];
+  }
   if (threadIdx.x == 0) {
     weight[d] = weight[s];
     phase[d] = phase[s];
@@ -323,14 +363,14 @@ void increment_counter(hipStream_t s, uint32_t* c) {
 void population_control(hipStream_t s, int W, const double* r_dev, uint64_t seed, uint32_t stream,
                         const uint32_t* step_ptr, long rowA, long rowB, z64* PhiA, z64* PhiB,
                         z64* weight, z64* phase, z64* eloc, z64* ovlp, z64* theta, double* cum,
-                        int* src, int* status) {
+                        int* src, int* status, const WalkerRows& extra) {
   if (W <= 0) return;
   AFQ_CHECK((size_t)W * sizeof(int) <= 65536, kErrShape, "population control supports W <= 16384");
   hipLaunchKernelGGL(k_comb_plan, dim3(1), dim3(kCombThreads), sizeof(int) * W, s, W, weight, r_dev,
                      seed, stream, step_ptr, cum, src, status);
   AFQ_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_copy_walkers, dim3(W), dim3(256), 0, s, W, src, rowA, rowB, PhiA, PhiB,
-                     weight, phase, eloc, ovlp, theta);
+                     weight, phase, eloc, ovlp, theta, extra);
   AFQ_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_unit_weights, dim3(ceil_div(W, 256)), dim3(256), 0, s, W, status, weight);
   AFQ_LAUNCH_CHECK();
@@ -354,10 +394,39 @@ void field_uniforms(hipStream_t s, uint64_t seed, uint32_t stream, uint32_t step
 
 void walker_update(hipStream_t s, int W, const UpdateParams& p, const double* Eshift,
                    const z64* new_ovlp, const z64* mf, const z64* HW, z64* weight, z64* phase,
-                   z64* eloc, z64* ovlp, z64* theta) {
+                   z64* eloc, z64* ovlp, z64* theta, z64* wfac, int hl, const uint32_t* step_ptr) {
   if (W <= 0) return;
+  AFQ_CHECK(!wfac || (hl > 0 && step_ptr), kErrArg, "weight-factor ring needs a length and a step");
   hipLaunchKernelGGL(k_walker_update, dim3(ceil_div(W, 256)), dim3(256), 0, s, W, p, Eshift,
-                     new_ovlp, mf, HW, weight, phase, eloc, ovlp, theta);
+                     new_ovlp, mf, HW, weight, phase, eloc, ovlp, theta, wfac, hl, step_ptr);
+  AFQ_LAUNCH_CHECK();
+}
+
+void store_fields(hipStream_t s, int W, int M, const z64* X, z64* ring, int nring,
+                  const uint32_t* step_ptr) {
+  const long n = (long)W * M;
+  if (n <= 0) return;
+  AFQ_CHECK(nring > 0 && step_ptr, kErrArg, "field ring needs a length and a step");
+  hipLaunchKernelGGL(k_store_fields, dim3(ceil_div(n, 256)), dim3(256), 0, s, W, M, X, ring, nring,
+                     step_ptr);
+  AFQ_LAUNCH_CHECK();
+}
+
+void gather_fields(hipStream_t s, int W, int M, const z64* ring, int nring, int slot, z64* X) {
+  const long n = (long)W * M;
+  if (n <= 0) return;
+  AFQ_CHECK(slot >= 0 && slot < nring, kErrArg, "field slot out of range");
+  hipLaunchKernelGGL(k_gather_fields, dim3(ceil_div(n, 256)), dim3(256), 0, s, W, M, ring, nring,
+                     slot, X);
+  AFQ_LAUNCH_CHECK();
+}
+
+void restore_weights(hipStream_t s, int W, const z64* weight, const z64* wfac, int hl, uint32_t pos,
+                     int n, z64* out) {
+  if (W <= 0) return;
+  AFQ_CHECK(n == 0 || (wfac && hl > 0 && n <= hl), kErrArg, "restoration longer than the history");
+  hipLaunchKernelGGL(k_restore_weights, dim3(ceil_div(W, 256)), dim3(256), 0, s, W, weight, wfac,
+                     hl > 0 ? hl : 1, pos, n, out);
   AFQ_LAUNCH_CHECK();
 }
 

@@ -62,6 +62,11 @@ void mixed_density_matrix(Context& c, const z64* PsiT, int n, int N, const z64* 
                           int W, z64* G, long g_bs, z64* ovlp, bool compact, double scale = 1.0);
 // inverse_logdet (density_matrix.hpp:82-111)
 void inverse_logdet(Context& c, const z64* A, int nb, int n, z64* ovlp, z64* Ainv);
+// Full mixed density matrices with a left state per walker (the 3-d MixedDensityMatrix of
+// density_matrix.hpp:604-650, compact=false): T = L^H Phi, G[w] = (Phi T^-1 L^H)^T [N][N]
+// at G + w * g_bs; L [W][N][n] (l_bs apart), ovlp += log det T.
+void mixed_density_matrix_left(Context& c, const z64* L, long l_bs, int n, int N, const z64* Phi,
+                               long phi_bs, int W, z64* G, long g_bs, z64* ovlp);
 // Propagate (propagate.hpp:241-298): Phi <- P1 expM(V) P1 Phi, order-K Taylor series.
 // P1 [N][N] (real when P1r != nullptr), V [W][N][N] in the compute precision.
 struct PropagateArgs {
@@ -284,6 +289,24 @@ class Propagator {
                   double relax, double* energy_out);
   Precision precision() const { return prec_; }
 
+  // ---- back-propagation (AFQMCBasePropagator::BackPropagate, AFQMCBasePropagator.cpp:404-560;
+  // BackPropEstimator.hpp).  enable_backprop makes every later step store its fields in a ring
+  // of nback slots and its path-restoration factor in a ring of 3 nback slots per walker (both
+  // indexed by the device step counter, so steps must draw their own fields); population
+  // control moves these rows and the anchor Slater matrices with their walkers.
+  void enable_backprop(int nback);
+  int backprop_length() const { return bp_n_; }
+  // anchor = the current Slater matrices (BackPropEstimator::setAnchor)
+  void bp_anchor(const WalkerState& s);
+  // Back-propagates the trial through the last nbp stored fields (newest first, the adjoint
+  // propagator, QR every ortho steps and at the end) and returns, per walker, the full mixed
+  // density matrices between it and the anchor, G [W][nspin][N][N], and the weights times the
+  // path-restoration factors of those steps (when restore), w [W].
+  void backpropagate(const WalkerState& s, int nbp, int ortho, bool restore, z64* G, z64* w);
+  // device views of the rings (tests): fields [W][nback][M], factors [W][3 nback]
+  const z64* bp_fields() const { return bp_fields_; }
+  const z64* bp_factors() const { return bp_wfac_; }
+
  private:
   CholeskyOps& ham_;
   Context& ctx_;
@@ -309,6 +332,12 @@ class Propagator {
   };
   std::vector<GraphEntry> graphs_;
   bool warmed_ = false;
+  int bp_n_ = 0, bp_hl_ = 0;
+  z64* bp_fields_ = nullptr;
+  z64* bp_wfac_ = nullptr;
+  z64* bp_anch_a_ = nullptr;
+  z64* bp_anch_b_ = nullptr;
+  void clear_graphs();
   void step_impl(const WalkerState& s, const double* u, const double* Eshift);
 };
 

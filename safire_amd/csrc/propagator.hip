@@ -89,7 +89,8 @@ Propagator::~Propagator() {
     (void)hipGraphDestroy(g.graph);
   }
   for (void* q : {(void*)vMF_, (void*)P1_, (void*)P1r_, (void*)P1r32_, (void*)P1c32_,
-                  (void*)step_dev_, (void*)status_dev_})
+                  (void*)step_dev_, (void*)status_dev_, (void*)bp_fields_, (void*)bp_wfac_,
+                  (void*)bp_anch_a_, (void*)bp_anch_b_})
     if (q) (void)hipFree(q);
 }
 
@@ -147,6 +148,7 @@ void Propagator::step_impl(const WalkerState& st, const double* u, const double*
   FieldParams fp{p_.vbias_bound, (p_.use_cp || p_.project_force_bias) ? 1 : 0, p_.free_projection,
                  p_.seed, p_.stream, 0, u ? nullptr : step_dev_};
   construct_x(s, u, vMF_, W, M, fp, X, HW, mf, X32);
+  if (bp_n_ > 0) store_fields(s, W, M, X, bp_fields_, bp_n_, step_dev_);
   // 3. vHS
   ham_.vHS(prec_ == kMixed ? (const void*)X32 : (const void*)X, W, p_.dt, V, prec_);
   // 4. propagate both spin blocks with the same V
@@ -164,7 +166,8 @@ void Propagator::step_impl(const WalkerState& st, const double* u, const double*
   // 6. weights
   UpdateParams up{p_.dt, ham_.E0(), p_.lower_cut, p_.upper_cut, p_.weight_floor, p_.weight_frac,
                   p_.target_pop, p_.free_projection, p_.use_cp};
-  walker_update(s, W, up, Eshift, nov, mf, HW, st.weight, st.phase, st.eloc, st.ovlp, st.theta);
+  walker_update(s, W, up, Eshift, nov, mf, HW, st.weight, st.phase, st.eloc, st.ovlp, st.theta,
+                bp_n_ > 0 ? bp_wfac_ : nullptr, bp_hl_, step_dev_);
 }
 
 void Propagator::step(const WalkerState& st, const double* u, const double* Eshift) {
@@ -188,9 +191,122 @@ void Propagator::population_control(const WalkerState& st, const double* r_dev) 
   const bool coll = !closed_ && ndown > 0;
   double* cum = scratch_as<double>(ctx_, "pc_cum", st.W);
   int* src = scratch_as<int>(ctx_, "pc_src", st.W);
+  WalkerRows extra;
+  if (bp_n_ > 0) {
+    extra.p[0] = bp_fields_;
+    extra.n[0] = (long)bp_n_ * ham_.M();
+    extra.p[1] = bp_wfac_;
+    extra.n[1] = bp_hl_;
+    extra.p[2] = bp_anch_a_;
+    extra.n[2] = (long)N * nup;
+    if (coll) {
+      extra.p[3] = bp_anch_b_;
+      extra.n[3] = (long)N * ndown;
+    }
+  }
   afq::population_control(ctx_.stream(), st.W, r_dev, p_.seed, p_.stream + 1000u, step_dev_,
                           (long)N * nup, coll ? (long)N * ndown : 0, st.Phi, coll ? st.Phi_b : nullptr,
-                          st.weight, st.phase, st.eloc, st.ovlp, st.theta, cum, src, status_dev_);
+                          st.weight, st.phase, st.eloc, st.ovlp, st.theta, cum, src, status_dev_,
+                          extra);
+}
+
+void Propagator::clear_graphs() {
+  for (auto& g : graphs_) {
+    (void)hipGraphExecDestroy(g.exec);
+    (void)hipGraphDestroy(g.graph);
+  }
+  graphs_.clear();
+  warmed_ = false;
+}
+
+void Propagator::enable_backprop(int nback) {
+  AFQ_CHECK(nback > 0, kErrArg, "back-propagation length must be positive");
+  AFQ_CHECK(!trial_, kErrArg, "back-propagation is built for single-determinant trials");
+  AFQ_CHECK(bp_n_ == 0, kErrState, "back-propagation is already enabled");
+  hipStream_t s = ctx_.stream();
+  const int N = ham_.N(), M = ham_.M();
+  z64* buf[4] = {nullptr, nullptr, nullptr, nullptr};
+  const size_t cnt[4] = {(size_t)W_ * nback * M, (size_t)W_ * 3 * nback,
+                         (size_t)W_ * N * std::max(wup_, 1), (size_t)W_ * N * std::max(wdn_, 1)};
+  try {
+    for (int i = 0; i < 4; ++i) buf[i] = dalloc<z64>(cnt[i]);
+  } catch (...) {
+    for (z64* q : buf)
+      if (q) (void)hipFree(q);
+    throw;
+  }
+  bp_fields_ = buf[0];
+  bp_wfac_ = buf[1];
+  bp_anch_a_ = buf[2];
+  bp_anch_b_ = buf[3];
+  bp_n_ = nback;
+  bp_hl_ = 3 * nback;
+  // an unwritten history entry restores nothing (WalkerSetBase::resize_bp initializes it to 1)
+  fill_z64(s, (long)cnt[1], {1.0, 0.0}, bp_wfac_);
+  fill_z64(s, (long)cnt[0], {0.0, 0.0}, bp_fields_);
+  AFQ_HIP(hipStreamSynchronize(s));
+  clear_graphs();  // captured steps do not store fields yet
+}
+
+void Propagator::bp_anchor(const WalkerState& st) {
+  AFQ_CHECK(bp_n_ > 0, kErrState, "back-propagation is not enabled");
+  hipStream_t s = ctx_.stream();
+  const int N = ham_.N();
+  AFQ_HIP(hipMemcpyAsync(bp_anch_a_, st.Phi, sizeof(z64) * st.W * N * wup_, hipMemcpyDeviceToDevice, s));
+  if (!closed_ && wdn_ > 0)
+    AFQ_HIP(hipMemcpyAsync(bp_anch_b_, st.Phi_b, sizeof(z64) * st.W * N * wdn_,
+                           hipMemcpyDeviceToDevice, s));
+}
+
+void Propagator::backpropagate(const WalkerState& st, int nbp, int ortho, bool restore, z64* G,
+                               z64* wout) {
+  AFQ_CHECK(bp_n_ > 0, kErrState, "back-propagation is not enabled");
+  AFQ_CHECK(nbp > 0 && nbp <= bp_n_, kErrArg, "back-propagation length exceeds the field ring");
+  AFQ_CHECK(ortho > 0, kErrArg, "orthogonalization interval must be positive");
+  AFQ_CHECK(st.W <= W_, kErrShape, "walker count exceeds the propagator's capacity");
+  hipStream_t s = ctx_.stream();
+  const int W = st.W, N = ham_.N(), M = ham_.M();
+  const bool coll = !closed_ && wdn_ > 0;
+  uint32_t C = 0;  // completed steps: the newest stored field sits at slot (C - 1) % nback
+  AFQ_HIP(hipMemcpyAsync(&C, step_dev_, sizeof(C), hipMemcpyDeviceToHost, s));
+  AFQ_HIP(hipStreamSynchronize(s));
+  // left states = the trial orbitals (Phi-shaped: PsiT^H of each spin), one copy per walker
+  const int nsp = coll ? 2 : 1;
+  z64* L[2] = {scratch_as<z64>(ctx_, "bp_L0", (size_t)W * N * wup_),
+               coll ? scratch_as<z64>(ctx_, "bp_L1", (size_t)W * N * wdn_) : nullptr};
+  const int nsl[2] = {wup_, wdn_};
+  z64* ref = scratch_as<z64>(ctx_, "bp_ref", (size_t)N * std::max(wup_, wdn_));
+  for (int sp = 0; sp < nsp; ++sp) {
+    conj_transpose(s, nsl[sp], N, ham_.PsiT() + (sp == 0 ? 0 : (size_t)wup_ * N), ref);
+    broadcast_z64(s, W, (long)N * nsl[sp], ref, L[sp]);
+  }
+  z64* X = scratch_as<z64>(ctx_, "bp_X", (size_t)W * M);
+  z64* V = scratch_as<z64>(ctx_, "bp_V", (size_t)W * N * N);
+  z64* ldet = scratch_as<z64>(ctx_, "bp_ldet", W);
+  for (int ni = nbp - 1; ni >= 0; --ni) {
+    const long back = (long)nbp - ni;  // 1 = the newest field
+    const int slot = (int)((((long)C - back) % bp_n_ + bp_n_) % bp_n_);
+    gather_fields(s, W, M, bp_fields_, bp_n_, slot, X);
+    ham_.vHS(X, W, p_.dt, V, kFP64);
+    for (int sp = 0; sp < nsp; ++sp) {
+      det_ops::PropagateArgs pa{W, N, nsl[sp], p_.order, 'H', P1_, real_P1_ ? P1r_ : nullptr,
+                                nullptr, nullptr, V, (long)N * N, L[sp], (long)N * nsl[sp], kFP64};
+      det_ops::propagate(ctx_, pa);
+      if (ni == 0 || ni % ortho == 0) {
+        fill_z64(s, W, {0, 0}, ldet);
+        det_ops::orthogonalize(ctx_, L[sp], (long)N * nsl[sp], W, N, nsl[sp], ldet);
+      }
+    }
+  }
+  z64* ov = scratch_as<z64>(ctx_, "bp_ov", W);
+  z64* anch[2] = {bp_anch_a_, bp_anch_b_};
+  for (int sp = 0; sp < nsp; ++sp) {
+    fill_z64(s, W, {0, 0}, ov);
+    det_ops::mixed_density_matrix_left(ctx_, L[sp], (long)N * nsl[sp], nsl[sp], N, anch[sp],
+                                       (long)N * nsl[sp], W, G + (size_t)sp * N * N,
+                                       (long)nsp * N * N, ov);
+  }
+  restore_weights(s, W, st.weight, bp_wfac_, bp_hl_, C, restore ? nbp : 0, wout);
 }
 
 int Propagator::status() {

@@ -28,6 +28,8 @@ class Trace:
     eff_walkers: list
     growth: float = float("nan")   # growth-estimator energy after the last step
     eshift: float = float("nan")   # Eshift after the last step
+    onerdm_mixed: list = dataclasses.field(default_factory=list)   # [nspin][N][N] per measurement
+    onerdm_bp: list = dataclasses.field(default_factory=list)      # [nspin][N][N] per window
 
     def mean_error(self, skip=0):
         e = np.real(np.asarray(self.energy[skip:]))
@@ -64,11 +66,39 @@ def initialize(prop: "afq.Propagator", ws: "afq.WalkerSet", precision=afq.AFQ_FP
     return esh
 
 
+def mixed_onerdm(prop, ws, psit_dev, nup, ndown):
+    """MixedEstimator's OneRDM (MixedEstimator.hpp:122-140, onerdm.hpp:40-60) of one
+    measurement: sum_w w G_w / sum_w w with G_w the full mixed density matrix of each spin."""
+    import torch
+    W, N = ws.W, prop.ham.N
+    coll = ws.Phi_b is not None
+    G = torch.empty((W, 2 if coll else 1, N, N), dtype=torch.complex128, device="cuda")
+    ov = torch.zeros(W, dtype=torch.complex128, device="cuda")
+    Gs = afq.empty((W, N, N), torch.complex128)
+    afq.det_ops.MixedDensityMatrix(prop.ctx, psit_dev[:nup], ws.Phi, Gs, ov, compact=False)
+    G[:, 0] = Gs
+    if coll:
+        afq.det_ops.MixedDensityMatrix(prop.ctx, psit_dev[nup:nup + ndown], ws.Phi_b, Gs, ov,
+                                       compact=False)
+        G[:, 1] = Gs
+    w = ws.weight
+    return (w[:, None, None, None] * G).sum(0) / w.sum()
+
+
 def run_afqmc(prop: "afq.Propagator", ws: "afq.WalkerSet", steps: int, equilibration: int = 100,
               ortho_interval: int = 5, popctl_interval: int = 10, measure_interval: int = 10,
               relax: float | None = None, energy_precision=afq.AFQ_FP64, Eshift=None,
-              progress=None, progress_interval: int = 1000) -> Trace:
-    """progress(step, trace), when given, is called every progress_interval steps."""
+              progress=None, progress_interval: int = 1000, onerdm_interval: int = 0,
+              psit=None, backprop: dict | None = None) -> Trace:
+    """progress(step, trace), when given, is called every progress_interval steps.
+
+    onerdm_interval > 0 (with psit, the trial rows [nup (+ ndown)][N]) records the mixed
+    estimator's OneRDM every onerdm_interval steps after equilibration.  backprop =
+    {"length": L, "ortho": k, "restore": True} records BackPropEstimator's OneRDM
+    (BackPropEstimator.hpp:150-260): the anchor is taken after the last equilibration step (or
+    before the first step without equilibration); L steps later the trial is back-propagated
+    over them and the weighted average of the back-propagated density matrices recorded, and the
+    anchor moves to that step."""
     import torch
     W = ws.W
     if relax is None:  # parameter_defaults.cpp:310-318
@@ -79,6 +109,20 @@ def run_afqmc(prop: "afq.Propagator", ws: "afq.WalkerSet", steps: int, equilibra
     E = afq.empty((W, 3), torch.complex128)
     ov = afq.empty(W, torch.complex128)
     tr = Trace([], [], [], [], [])
+    psit_dev = None
+    if onerdm_interval > 0:
+        if psit is None:
+            raise ValueError("the mixed OneRDM needs the trial rows (psit)")
+        psit_dev = afq.device_tensor(np.asarray(psit), np.complex128)
+    nup = ws.Phi.shape[2]
+    ndown = 0 if ws.Phi_b is None else ws.Phi_b.shape[2]
+    bp_len = 0
+    bp_pos = -1
+    if backprop:
+        bp_len = int(backprop["length"])
+        if getattr(prop, "nback", 0) < bp_len:
+            prop.enable_backprop(bp_len)
+        prop.bp_anchor(ws)   # the anchor before the first step
     for step in range(steps):
         prop.step_graph(ws, Eshift, ortho=(step % ortho_interval == 0),
                         popctl=(popctl_interval > 0 and step % popctl_interval == 0),
@@ -96,6 +140,18 @@ def run_afqmc(prop: "afq.Propagator", ws: "afq.WalkerSet", steps: int, equilibra
             tr.energy.append(complex(vals[:3].sum()))
             tr.total_weight.append(complex(vals[3]))
             tr.eff_walkers.append(float(vals[4].real))
+        if psit_dev is not None and step >= equilibration and step % onerdm_interval == 0:
+            tr.onerdm_mixed.append(afq.host(mixed_onerdm(prop, ws, psit_dev, nup, ndown)))
+        if bp_len:
+            if step == equilibration - 1:
+                prop.bp_anchor(ws)
+                bp_pos = step
+            elif step >= equilibration and step - bp_pos >= bp_len:
+                G, wr = prop.backpropagate(ws, bp_len, backprop.get("ortho", ortho_interval),
+                                           backprop.get("restore", True))
+                tr.onerdm_bp.append(afq.host((wr[:, None, None, None] * G).sum(0) / wr.sum()))
+                prop.bp_anchor(ws)
+                bp_pos = step
         if progress is not None and (step + 1) % progress_interval == 0:
             progress(step + 1, tr)
     torch.cuda.synchronize()

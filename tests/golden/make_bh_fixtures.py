@@ -40,6 +40,20 @@ def dump_string(path, dataset):
     return m.group(1).replace('\\"', '"')
 
 
+def dump_array(path, dataset):
+    """A complex dataset ({r, i} compound) as a nested list [[re, im], ...] with its shape."""
+    out = subprocess.run([H5DUMP, "-y", "-w", "0", "-m", "%.17g", "-d", dataset, path], check=True,
+                         capture_output=True, text=True).stdout
+    shape = [int(x) for x in re.search(r"DATASPACE\s+SIMPLE\s+\{\s*\(([^)]*)\)", out).group(1).split(",")]
+    body = out[out.index("DATA {") + len("DATA {"):]
+    nums = [float(x) for x in re.findall(r"[-+]?(?:\d+\.?\d*|\.\d+)(?:[eE][-+]?\d+)?", body)]
+    n = 1
+    for d in shape:
+        n *= d
+    assert len(nums) == 2 * n, (dataset, len(nums), shape)
+    return {"shape": shape, "re": nums[0::2], "im": nums[1::2]}
+
+
 def main():
     os.makedirs(OUT, exist_ok=True)
     for f in INPUTS:
@@ -57,7 +71,24 @@ def main():
             mean = dump_value(path, f"/measurements/Stage0/{name}/mean")
             err = dump_value(path, f"/measurements/Stage0/{name}/error")
             obs[name] = {"mean": mean[:2], "error": err[:2]}
+        rdm = {}
+        # one-body density matrices of the single-determinant cases (the device back-propagation
+        # covers single-determinant trials)
+        for name, ds in ((("mixed", "MixedEstimator/OneRDM"), ("backprop", None))
+                         if trial.endswith("nomsd") else ()):
+            if ds is None:
+                bp = inp["execute"].get("estimators", {}).get("backprop")
+                if not bp:
+                    continue
+                ds = f"BackPropEstimator/Steps={bp['propagation_steps'][-1]}/OneRDM"
+            try:
+                rdm[name] = {"mean": dump_array(path, f"/measurements/Stage0/{ds}/mean"),
+                             "error": dump_array(path, f"/measurements/Stage0/{ds}/error")}
+            except subprocess.CalledProcessError:
+                pass
         refs[f"{trial}/{walkers}"] = {
+            "onerdm": rdm,
+            "estimators": ex.get("estimators"),
             "hamiltonian": os.path.basename(ex["hamiltonian"]["filename"]),
             "wavefunction": os.path.basename(ex["wavefunction"]["filename"]),
             "walker_type": ex["walker_set"]["walker_type"],

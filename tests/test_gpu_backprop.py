@@ -1,0 +1,277 @@
+"""Back-propagation on the device (afq_prop_enable_backprop / bp_anchor / backpropagate) and the
+one-body density-matrix estimators.
+
+- The field ring holds, after every step, exactly the fields the oracle constructs for that step
+  from the same state and the same Philox draws (1e-10), and the factor ring the
+  path-restoration factor (w/|w|)/scale of WalkerSetUpdate.hpp:136-146 recomputed from the
+  step's bounded local energy and phase angle.
+- backpropagate: the back-propagated trial (adjoint propagator through the stored fields,
+  newest first, QR every k steps and at the end; AFQMCBasePropagator.cpp:404-560) and its full
+  mixed density matrix with the anchors (density_matrix.hpp:604-650) equal an oracle
+  back-propagation built from the oracle's vHS, propagate('H') and QR (1e-10); the restored
+  weights equal WEIGHT times the product of the stored factors.
+- Population control moves the rings and the anchors with their walkers.
+- BH (the reference's inputs): the mixed and back-propagated OneRDM of a seeded run agree with the
+  reference's statistical results element by element under its runner's criterion
+  (run_functional.py:385-410, Bonferroni over the 361 elements).
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import oracle as orc  # noqa: E402
+from safire_amd.synthetic import make_problem  # noqa: E402
+
+TOL = 1e-10
+G_BH = os.path.join(os.path.dirname(__file__), "golden", "bh")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape, (a.shape, b.shape)
+    d = np.max(np.abs(b))
+    return float(np.max(np.abs(a - b)) / (d if d > 0 else 1.0))
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from safire_amd import afq
+    return afq, afq.Context(0), torch
+
+
+def _setup(afq, torch, p, W, seed, nback):
+    ctx = afq.Context(0)
+    ham = afq.HamiltonianOperations(ctx, p.h1, p.Likn, p.PsiT, p.nup)
+    prop = afq.Propagator(ham, W, dt=0.005, seed=seed)
+    prop.enable_backprop(nback)
+    osys = orc.OracleSystem(p, dt=0.005)
+    st, Eshift = osys.init_walkers(p.walkers(W))
+    ws = afq.WalkerSet(st["Phi"])
+    for k in ("weight", "phase", "eloc", "ovlp", "theta"):
+        getattr(ws, k).copy_(torch.from_numpy(st[k]))
+    prop.set_step(0)
+    return ctx, ham, prop, osys, st, ws, Eshift
+
+
+def test_rings_and_backpropagation_vs_oracle(gpu):
+    afq, ctx0, torch = gpu
+    p = make_problem(24, 5, 96, seed=1001)
+    W, seed, nback, nbp, ortho = 16, 29, 6, 5, 2
+    say(f"case: back-propagation N=24 n=5 M=96 W={W}, ring {nback}, window {nbp}, QR every {ortho}")
+    ctx, ham, prop, osys, st, ws, Eshift = _setup(afq, torch, p, W, seed, nback)
+    Esh = afq.device_tensor(np.array([Eshift]))
+    dt = 0.005
+    step = 0
+    for _ in range(2):   # before the anchor
+        prop.Propagate(ws, Esh)
+        osys.step(st, orc.field_uniforms(seed, 1, step, W, p.M), Eshift)
+        step += 1
+    prop.bp_anchor(ws)
+    Phi_anchor = afq.host(ws.Phi)
+    for _ in range(nbp):
+        u = orc.field_uniforms(seed, 1, step, W, p.M)
+        Gm, _ = orc.mixed_density_matrix(osys.PsiT, afq.host(ws.Phi))
+        X, _, _ = orc.construct_x(u, orc.vbias(osys.Lnak, Gm, dt, c=2.0), osys.vMF, 50.0)
+        prop.Propagate(ws, Esh)
+        osys.step(st, u, Eshift)
+        fields, facs = prop.bp_rings(W)
+        eX = rel(fields[:, step % nback], X)
+        e = afq.host(ws.eloc)
+        th = afq.host(ws.theta).real
+        scale = np.maximum(0.0, np.cos(th))
+        # a walker the phaseless projection drops (scale 0) stores 0
+        fac = np.where(scale > 0, np.exp(-1j * dt * e.imag) / np.where(scale > 0, scale, 1), 0)
+        say(f"  step {step}: {int(np.sum(scale == 0))} walkers dropped")
+        eF = rel(facs[:, step % (3 * nback)], fac)
+        say(f"  step {step}: fields {eX:.2e}  factors {eF:.2e}")
+        assert eX < TOL and eF < 1e-12
+        step += 1
+    # device back-propagation
+    G, wr = prop.backpropagate(ws, nbp, ortho=ortho, restore=True)
+    fields, facs = prop.bp_rings(W)
+    # oracle back-propagation through the same stored fields
+    P1 = afq.host(prop.P1())
+    L = np.broadcast_to(osys.PsiT.conj().T, (W, p.N, p.nup)).copy()
+    for ni in range(nbp - 1, -1, -1):
+        slot = (step - (nbp - ni)) % nback
+        V = orc.vhs(osys.Likn, fields[:, slot], dt)
+        L = orc.propagate(P1, V, L, 6, "H")
+        if ni == 0 or ni % ortho == 0:
+            L, _ = orc.orthogonalize(L)
+    Gr = np.empty((W, 1, p.N, p.N), np.complex128)
+    for w in range(W):
+        A, B = L[w], Phi_anchor[w]
+        T = A.conj().T @ B
+        Gr[w, 0] = (B @ np.linalg.solve(T, A.conj().T)).T
+    eG = rel(afq.host(G), Gr)
+    wref = afq.host(ws.weight) * np.prod([facs[:, (step - 1 - k) % (3 * nback)] for k in range(nbp)],
+                                         axis=0)
+    eW = rel(afq.host(wr), wref)
+    say(f"  back-propagated G {eG:.2e}  restored weights {eW:.2e}")
+    assert eG < TOL and eW < 1e-12
+    # restore=False returns the weights unchanged
+    _, w0 = prop.backpropagate(ws, nbp, ortho=ortho, restore=False)
+    assert np.array_equal(afq.host(w0), afq.host(ws.weight))
+    prop.close()
+
+
+def test_collinear_backpropagation_vs_oracle(gpu):
+    """COLLINEAR walkers: each spin's trial block is back-propagated through the same fields
+    and gives its own density-matrix block against that spin's anchor."""
+    afq, ctx0, torch = gpu
+    p = make_problem(24, 5, 96, seed=1003)
+    nup, ndn, W, seed, nback, nbp, ortho = 5, 4, 8, 37, 5, 4, 3
+    say(f"case: COLLINEAR back-propagation N=24 nup={nup} ndown={ndn} M=96 W={W}, window {nbp}")
+    rng = np.random.default_rng(2)
+    evecs = np.linalg.eigh(p.h1)[1]
+    Q = np.linalg.qr(rng.normal(size=(6, 6)))[0]
+    dn = evecs[:, :6] @ Q[:, :ndn]           # a down-spin trial different from the up one
+    PsiT = np.ascontiguousarray(np.vstack([p.PsiT, dn.T]).astype(np.complex128))
+    ctx = afq.Context(0)
+    ham = afq.HamiltonianOperations(ctx, p.h1, p.Likn, PsiT, nup, ndn, walker_type=afq.AFQ_COLLINEAR)
+    prop = afq.Propagator(ham, W, dt=0.005, seed=seed)
+    prop.enable_backprop(nback)
+    Pa = np.broadcast_to(PsiT[:nup].conj().T, (W, p.N, nup)) + 0.05 * (
+        rng.normal(size=(W, p.N, nup)) + 1j * rng.normal(size=(W, p.N, nup)))
+    Pb = np.broadcast_to(PsiT[nup:].conj().T, (W, p.N, ndn)) + 0.05 * (
+        rng.normal(size=(W, p.N, ndn)) + 1j * rng.normal(size=(W, p.N, ndn)))
+    ws = afq.WalkerSet(Pa, Pb)
+    E = afq.empty((W, 3), torch.complex128)
+    ov = afq.empty(W, torch.complex128)
+    prop.Energy(ws, E, ov)
+    ws.eloc.copy_(E.sum(1))
+    ws.ovlp.copy_(ov)
+    Esh = afq.device_tensor(np.array([float(afq.host(E.sum(1)).real.mean())]))
+    prop.set_step(0)
+    prop.Propagate(ws, Esh)
+    prop.bp_anchor(ws)
+    anchors = (afq.host(ws.Phi), afq.host(ws.Phi_b))
+    for _ in range(nbp):
+        prop.Propagate(ws, Esh)
+    step = 1 + nbp
+    G, _ = prop.backpropagate(ws, nbp, ortho=ortho, restore=False)
+    fields, _ = prop.bp_rings(W)
+    P1 = afq.host(prop.P1())
+    Ls = [np.broadcast_to(PsiT[:nup].conj().T, (W, p.N, nup)).copy(),
+          np.broadcast_to(PsiT[nup:].conj().T, (W, p.N, ndn)).copy()]
+    for ni in range(nbp - 1, -1, -1):
+        V = orc.vhs(p.Likn, fields[:, (step - (nbp - ni)) % nback], 0.005)
+        for sp in range(2):
+            Ls[sp] = orc.propagate(P1, V, Ls[sp], 6, "H")
+            if ni == 0 or ni % ortho == 0:
+                Ls[sp], _ = orc.orthogonalize(Ls[sp])
+    Gr = np.empty((W, 2, p.N, p.N), np.complex128)
+    for sp in range(2):
+        for w in range(W):
+            A, B = Ls[sp][w], anchors[sp][w]
+            Gr[w, sp] = (B @ np.linalg.solve(A.conj().T @ B, A.conj().T)).T
+    eG = rel(afq.host(G), Gr)
+    say(f"  back-propagated G (both spins) {eG:.2e}")
+    assert eG < TOL
+    prop.close()
+
+
+def test_population_control_moves_backprop_rows(gpu):
+    afq, ctx0, torch = gpu
+    p = make_problem(24, 5, 96, seed=1001)
+    W, seed, nback = 32, 31, 4
+    say("case: population control carries the back-propagation rings")
+    ctx, ham, prop, osys, st, ws, Eshift = _setup(afq, torch, p, W, seed, nback)
+    Esh = afq.device_tensor(np.array([Eshift]))
+    for _ in range(6):
+        prop.Propagate(ws, Esh)
+    prop.bp_anchor(ws)
+    for _ in range(2):
+        prop.Propagate(ws, Esh)
+    # spread weights, so that the comb drops some walkers and copies others
+    rng = np.random.default_rng(5)
+    ws.weight.copy_(torch.from_numpy(rng.uniform(0.1, 2.0, W).astype(np.complex128)))
+    Phi0 = afq.host(ws.Phi)
+    f0, q0 = prop.bp_rings(W)
+    prop.population_control(ws, 0.37)
+    Phi1 = afq.host(ws.Phi)
+    f1, q1 = prop.bp_rings(W)
+    moved = 0
+    for d in range(W):
+        src = [s for s in range(W) if np.array_equal(Phi1[d], Phi0[s])]
+        assert src, d
+        s = src[0]
+        moved += s != d
+        assert np.array_equal(f1[d], f0[s]) and np.array_equal(q1[d], q0[s])
+    say(f"  {moved} walkers replaced by copies")
+    assert moved > 0
+    prop.close()
+
+
+def _cplx(d):
+    return (np.asarray(d["re"]) + 1j * np.asarray(d["im"])).reshape(d["shape"])
+
+
+def _compare_rdm(name, A, Aerr, B, Berr):
+    """run_functional.py:385-410 on one observable tensor."""
+    from scipy.stats import norm
+    z = norm.ppf(1 - 0.001 / (2 * A.size))
+    tol = np.maximum(z * np.sqrt(np.abs(Aerr) ** 2 + np.abs(Berr) ** 2), 1e-9 * (1 + np.abs(B)))
+    d = np.abs(A - B)
+    worst = np.unravel_index(np.argmax(d / tol), d.shape)
+    say(f"  {name}: worst |d| = {d[worst]:.3e}, tolerance {tol[worst]:.3e} at {worst} "
+        f"({A[worst]:.6f} +- {abs(Aerr[worst]):.6f} vs {B[worst]:.6f} +- {abs(Berr[worst]):.6f})")
+    return bool(np.all(d <= tol))
+
+
+@pytest.mark.parametrize("case", ["rhf_nomsd/closed", "uhf_nomsd/collinear"])
+def test_bh_onerdm_statistical_reference(gpu, case):
+    afq, ctx0, torch = gpu
+    from safire_amd import driver, h5inputs
+    refs = json.load(open(os.path.join(G_BH, "statistical_refs.json")))
+    r = refs[case]
+    say(f"case: BH {case} mixed and back-propagated OneRDM vs the reference's statistical results")
+    wt = afq.AFQ_CLOSED if r["walker_type"] == "CLOSED" else afq.AFQ_COLLINEAR
+    s = h5inputs.load_system(os.path.join(G_BH, r["hamiltonian"]), os.path.join(G_BH, r["wavefunction"]),
+                             walker_type=wt)
+    W = r["walkers"]
+    ctx = afq.Context(0)
+    ham = afq.HamiltonianOperations(ctx, s.h1, s.Likn, s.PsiT, s.nup, s.ndown, walker_type=wt,
+                                    E0=s.E0)
+    prop = afq.Propagator(ham, W, dt=r["timestep"], seed=42)
+    ws = afq.WalkerSet(*s.walkers(W))
+    est = r["estimators"]
+    eq = r["equilibration_steps"]
+    bp_len = est["backprop"]["propagation_steps"][-1]
+    tr = driver.run_afqmc(prop, ws, steps=eq + 9000, equilibration=eq,
+                          ortho_interval=r["walker_ortho_interval"],
+                          popctl_interval=r["population_control_interval"],
+                          measure_interval=r["measure_interval"],
+                          onerdm_interval=est["mixed"]["measure_interval"], psit=s.PsiT,
+                          backprop={"length": bp_len, "ortho": r["walker_ortho_interval"],
+                                    "restore": est["backprop"]["path_restoration"]},
+                          progress=lambda st, t: say(f"  step {st}"), progress_interval=3000)
+    ok = True
+    for name, series in (("mixed", tr.onerdm_mixed), ("backprop", tr.onerdm_bp)):
+        x = np.asarray(series)
+        mean = x.mean(0)
+        err = np.empty(mean.shape, np.complex128)
+        flat = x.reshape(len(x), -1)
+        errf = err.reshape(-1)
+        for i in range(flat.shape[1]):
+            errf[i] = complex(driver.blocking_error(flat[:, i].real),
+                              driver.blocking_error(flat[:, i].imag))
+        ref = r["onerdm"][name]
+        tr_ = [round(float(np.trace(mean[k]).real), 6) for k in range(mean.shape[0])]
+        say(f"  {name}: {len(x)} measurements, trace per spin {tr_}")
+        ok &= _compare_rdm(name, mean, err, _cplx(ref["mean"]), _cplx(ref["error"]))
+    prop.close()
+    assert ok
