@@ -228,11 +228,27 @@ int afq_prop_bp_anchor(afq_prop* prop, const afq_walkers* w);
  * propagator, QR every `ortho` steps and at the end) and writes G [W][nspin][N][N] (complex,
  * device; nspin = 1 CLOSED, 2 COLLINEAR): the full mixed density matrix between the
  * back-propagated trial and each walker's anchor, and weights [W] (complex, device): WEIGHT
- * times the path-restoration factors of the last nbp steps (restore != 0) or WEIGHT. */
+ * times the path-restoration factors of the last restore * nbp steps (restore = 0 none,
+ * 1 path_restoration, 2 extra_path_restoration; BackPropEstimator.hpp:212-228). */
 int afq_prop_backpropagate(afq_prop* prop, const afq_walkers* w, int nbp, int ortho, int restore,
                            double* G, double* weights);
 /* Device views of the rings: fields [W][nback][M] complex, factors [W][3 nback] complex. */
 int afq_prop_bp_views(afq_prop* prop, const double** fields, const double** factors, int* nback);
+
+/* ---- density-matrix observables --------------------------------------------------------------
+ * One measurement of OneRDM (onerdm.hpp:43-70) and DiagonalTwoRDM (diagonal_twordm.hpp:40-81)
+ * from per-walker density matrices G [W][nspin][N][N] (complex, device; the full mixed density
+ * matrices of the walkers, or afq_prop_backpropagate's output), divided by the sum of the
+ * weights as MeasurementOutput does (Measurements.hpp:98-121).  c_w = weights[w] * coeff[w]
+ * (coeff may be NULL: 1).  onerdm [nspin][N][N]; diag2rdm [nblk][N][N] with the blocks
+ * (aaaa), (aabb) for CLOSED and (aaaa), (aabb), (bbbb) for COLLINEAR, before the reference's
+ * fold over i <-> j (DiagonalTwoRDM::compress, a host-side layout step).  Either output may be
+ * NULL.  accumulate != 0 adds to the outputs (the sum over references of a multi-reference
+ * trial: one call per reference with its coefficients, the referenceLoop of
+ * MixedEstimator.hpp:77-103).  Device pointers, asynchronous on the context's stream. */
+int afq_rdm_measure(afq_ctx* ctx, int walker_type, int N, int W, const double* G,
+                    const double* weights, const double* coeff, int accumulate, double* onerdm,
+                    double* diag2rdm);
 
 /* ---- particle-hole multi-determinant trial (PHMSD, COLLINEAR walkers) ----------------------- */
 

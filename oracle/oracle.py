@@ -361,6 +361,47 @@ class OracleSystem:
         state["ovlp"][...] = 2.0 * log_overlap(self.PsiT, Phi)
 
 
+# --- density-matrix observables (numpy) -------------------------------------------------------
+
+def rdm_measure(G, weights, closed, coeff=None):
+    """One reference's contribution to OneRDM::measure (onerdm.hpp:53-58) and
+    DiagonalTwoRDM::measure (diagonal_twordm.hpp:55-78), divided by sum_w weights
+    (Measurements.hpp:98-121).  G [W][nspin][N][N]; returns (onerdm [nspin][N][N], the unfolded
+    diagonal blocks [2 CLOSED / 3 COLLINEAR][N][N] = (aaaa), (aabb)[, (bbbb)]), walker by walker."""
+    G = np.asarray(G, np.complex128)
+    w = np.asarray(weights, np.complex128)
+    c = w.copy() if coeff is None else w * np.asarray(coeff, np.complex128)
+    W, nsp, N, _ = G.shape
+    one = np.zeros((nsp, N, N), np.complex128)
+    two = np.zeros((2 if closed else 3, N, N), np.complex128)
+    for k in range(W):
+        for s in range(nsp):
+            g = G[k, s]
+            one[s] += c[k] * g
+            d = np.diag(g)
+            two[2 * s] += c[k] * (np.outer(d, d) - g * g.T)
+        dn = G[k, 0 if closed else 1]
+        two[1] += c[k] * np.outer(np.diag(G[k, 0]), np.diag(dn))
+    den = w.sum()
+    return one / den, two / den
+
+
+def fold_diag2rdm(full, closed):
+    """DiagonalTwoRDM::compress (diagonal_twordm.hpp:107-140), element by element."""
+    N = full.shape[1]
+    out = []
+    for i in range(N):
+        for j in range(i + 1, N):
+            out.append((full[0, i, j] + full[0, j, i]) / 2)
+        for j in range(N):
+            out.append(full[1, i, j])
+    if not closed:
+        for i in range(N):
+            for j in range(i + 1, N):
+                out.append((full[2, i, j] + full[2, j, i]) / 2)
+    return np.array(out, np.complex128)
+
+
 # --- particle-hole multi-determinant trial (phmsd_oracle.cpp) ------------------------------
 
 class _Ph(ctypes.Structure):

@@ -93,6 +93,7 @@ _SIGS = {
     "afq_prop_bp_anchor": [_P, ctypes.POINTER(Walkers)],
     "afq_prop_backpropagate": [_P, ctypes.POINTER(Walkers), _I, _I, _I, _P, _P],
     "afq_prop_bp_views": [_P, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_I)],
+    "afq_rdm_measure": [_P, _I, _I, _I, _P, _P, _P, _I, _P, _P],
     "afq_prop_create_phmsd": [_P, _I, ctypes.POINTER(PropParams), ctypes.POINTER(_P)],
     "afq_phmsd_create": [_P, _P, ctypes.POINTER(_P)],
     "afq_phmsd_destroy": [_P],
@@ -380,6 +381,44 @@ def _copy_d2d(dst, src, nbytes):
         raise AfqError(f"hipMemcpy failed ({rc})")
 
 
+def rdm_measure(ctx, G, weights, walker_type, coeff=None, onerdm=True, diag2rdm=True,
+                out=None):
+    """One OneRDM / DiagonalTwoRDM measurement from per-walker density matrices G
+    [W][nspin][N][N]: (onerdm [nspin][N][N] or None, diag2rdm [2 or 3][N][N] or None), each
+    sum_w weight_w coeff_w (...) / sum_w weight_w.  The diagonal two-body blocks are (aaaa),
+    (aabb)[, (bbbb)], unfolded (fold_diag2rdm gives the reference's layout).  With out =
+    (onerdm, diag2rdm) from an earlier call the measurement is added to those tensors."""
+    torch = _torch()
+    W, nsp, N = G.shape[0], G.shape[1], G.shape[2]
+    if nsp != (2 if walker_type == AFQ_COLLINEAR else 1):
+        raise AfqError("G's spin dimension does not match the walker type")
+    if out is not None:
+        one, two = out
+    else:
+        one = torch.empty((nsp, N, N), dtype=torch.complex128, device="cuda") if onerdm else None
+        two = (torch.empty((nsp + 1, N, N), dtype=torch.complex128, device="cuda") if diag2rdm
+               else None)
+    _ck(lib().afq_rdm_measure(ctx.h, int(walker_type), N, W, ptr(G), ptr(weights), ptr(coeff),
+                              int(out is not None), ptr(one), ptr(two)))
+    return one, two
+
+
+def fold_diag2rdm(full, walker_type):
+    """DiagonalTwoRDM::compress (diagonal_twordm.hpp:103-140): for each row i the symmetrized
+    (aaaa) entries j > i, then the whole (aabb) row; for COLLINEAR the symmetrized (bbbb) upper
+    triangle follows.  `full` is a host array [2 or 3][N][N]."""
+    full = np.asarray(full)
+    N = full.shape[1]
+    iu = np.triu_indices(N, 1)
+    out = []
+    for i in range(N):
+        out.append((full[0, i, i + 1:] + full[0, i + 1:, i]) / 2)
+        out.append(full[1, i, :])
+    if walker_type == AFQ_COLLINEAR:
+        out.append(((full[2] + full[2].T) / 2)[iu])
+    return np.concatenate(out)
+
+
 # ---------------------------------------------------------------------------------------------
 # walker set + propagator
 # ---------------------------------------------------------------------------------------------
@@ -480,16 +519,17 @@ class Propagator:
     def bp_anchor(self, wset: WalkerSet):
         _ck(lib().afq_prop_bp_anchor(self.h, ctypes.byref(wset.cstruct())))
 
-    def backpropagate(self, wset: WalkerSet, nbp: int, ortho: int = 10, restore: bool = True):
+    def backpropagate(self, wset: WalkerSet, nbp: int, ortho: int = 10, restore: int = 1):
         """(G [W][nspin][N][N], weights [W]) of the trial back-propagated over the last nbp
-        steps against the anchors."""
+        steps against the anchors; the weights carry the path-restoration factors of the last
+        restore * nbp steps (0 none, 1 path_restoration, 2 extra_path_restoration)."""
         torch = _torch()
         nspin = 2 if self.ham.walker_type == AFQ_COLLINEAR else 1
         N = self.ham.N
         G = torch.empty((wset.W, nspin, N, N), dtype=torch.complex128, device="cuda")
         w = torch.empty(wset.W, dtype=torch.complex128, device="cuda")
         _ck(lib().afq_prop_backpropagate(self.h, ctypes.byref(wset.cstruct()), int(nbp), int(ortho),
-                                         int(bool(restore)), ptr(G), ptr(w)))
+                                         int(restore), ptr(G), ptr(w)))
         return G, w
 
     def bp_rings(self, W: int):

@@ -418,7 +418,8 @@ int afq_prop_backpropagate(afq_prop* prop, const afq_walkers* w, int nbp, int or
   return guard([&] {
     REQUIRE(prop && G && weights, "NULL argument");
     check_walkers(prop, w);
-    prop->p->backpropagate(to_state(w), nbp, ortho, restore != 0, Z(G), Z(weights));
+    REQUIRE(restore >= 0 && restore <= 2, "restore must be 0, 1 or 2");
+    prop->p->backpropagate(to_state(w), nbp, ortho, restore, Z(G), Z(weights));
   });
 }
 
@@ -428,6 +429,20 @@ int afq_prop_bp_views(afq_prop* prop, const double** fields, const double** fact
     *fields = reinterpret_cast<const double*>(prop->p->bp_fields());
     *factors = reinterpret_cast<const double*>(prop->p->bp_factors());
     *nback = prop->p->backprop_length();
+  });
+}
+
+int afq_rdm_measure(afq_ctx* ctx, int walker_type, int N, int W, const double* G,
+                    const double* weights, const double* coeff, int accumulate, double* onerdm,
+                    double* diag2rdm) {
+  return guard([&] {
+    REQUIRE(ctx && G && weights, "NULL argument");
+    REQUIRE(walker_type == AFQ_CLOSED || walker_type == AFQ_COLLINEAR, "walker type");
+    REQUIRE(N > 0 && W > 0, "invalid sizes");
+    const int nsp = walker_type == AFQ_COLLINEAR ? 2 : 1;
+    afq::rdm_measure(ctx->c.stream(), W, nsp, N, nsp == 2, Z(G), (long)nsp * N * N, Z(weights),
+                     coeff ? Z(coeff) : nullptr, accumulate != 0, onerdm ? Z(onerdm) : nullptr,
+                     diag2rdm ? Z(diag2rdm) : nullptr);
   });
 }
 

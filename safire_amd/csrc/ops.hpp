@@ -301,8 +301,8 @@ class Propagator {
   // Back-propagates the trial through the last nbp stored fields (newest first, the adjoint
   // propagator, QR every ortho steps and at the end) and returns, per walker, the full mixed
   // density matrices between it and the anchor, G [W][nspin][N][N], and the weights times the
-  // path-restoration factors of those steps (when restore), w [W].
-  void backpropagate(const WalkerState& s, int nbp, int ortho, bool restore, z64* G, z64* w);
+  // path-restoration factors of the last restore * nbp steps (restore 0, 1 or 2), w [W].
+  void backpropagate(const WalkerState& s, int nbp, int ortho, int restore, z64* G, z64* w);
   // device views of the rings (tests): fields [W][nback][M], factors [W][3 nback]
   const z64* bp_fields() const { return bp_fields_; }
   const z64* bp_factors() const { return bp_wfac_; }

@@ -258,11 +258,12 @@ void Propagator::bp_anchor(const WalkerState& st) {
                            hipMemcpyDeviceToDevice, s));
 }
 
-void Propagator::backpropagate(const WalkerState& st, int nbp, int ortho, bool restore, z64* G,
+void Propagator::backpropagate(const WalkerState& st, int nbp, int ortho, int restore, z64* G,
                                z64* wout) {
   AFQ_CHECK(bp_n_ > 0, kErrState, "back-propagation is not enabled");
   AFQ_CHECK(nbp > 0 && nbp <= bp_n_, kErrArg, "back-propagation length exceeds the field ring");
   AFQ_CHECK(ortho > 0, kErrArg, "orthogonalization interval must be positive");
+  AFQ_CHECK(restore >= 0 && restore <= 2, kErrArg, "restore must be 0, 1 or 2");
   AFQ_CHECK(st.W <= W_, kErrShape, "walker count exceeds the propagator's capacity");
   hipStream_t s = ctx_.stream();
   const int W = st.W, N = ham_.N(), M = ham_.M();
@@ -306,7 +307,8 @@ void Propagator::backpropagate(const WalkerState& st, int nbp, int ortho, bool r
                                        (long)N * nsl[sp], W, G + (size_t)sp * N * N,
                                        (long)nsp * N * N, ov);
   }
-  restore_weights(s, W, st.weight, bp_wfac_, bp_hl_, C, restore ? nbp : 0, wout);
+  // the factor ring holds 3 nback steps, so extra path restoration (2 nbp) always fits
+  restore_weights(s, W, st.weight, bp_wfac_, bp_hl_, C, restore * nbp, wout);
 }
 
 int Propagator::status() {

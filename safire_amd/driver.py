@@ -28,8 +28,10 @@ class Trace:
     eff_walkers: list
     growth: float = float("nan")   # growth-estimator energy after the last step
     eshift: float = float("nan")   # Eshift after the last step
-    onerdm_mixed: list = dataclasses.field(default_factory=list)   # [nspin][N][N] per measurement
-    onerdm_bp: list = dataclasses.field(default_factory=list)      # [nspin][N][N] per window
+    # MixedEstimator: observable name -> one entry per measurement
+    mixed: dict = dataclasses.field(default_factory=dict)
+    # BackPropEstimator: propagation steps -> observable name -> one entry per window
+    backprop: dict = dataclasses.field(default_factory=dict)
 
     def mean_error(self, skip=0):
         e = np.real(np.asarray(self.energy[skip:]))
@@ -66,9 +68,9 @@ def initialize(prop: "afq.Propagator", ws: "afq.WalkerSet", precision=afq.AFQ_FP
     return esh
 
 
-def mixed_onerdm(prop, ws, psit_dev, nup, ndown):
-    """MixedEstimator's OneRDM (MixedEstimator.hpp:122-140, onerdm.hpp:40-60) of one
-    measurement: sum_w w G_w / sum_w w with G_w the full mixed density matrix of each spin."""
+def mixed_density_matrices(prop, ws, psit_dev, nup, ndown):
+    """The full mixed density matrix of every walker with the trial, G [W][nspin][N][N] (the
+    per-walker input of MixedEstimator's observables, MixedEstimator.hpp:122-140)."""
     import torch
     W, N = ws.W, prop.ham.N
     coll = ws.Phi_b is not None
@@ -81,24 +83,56 @@ def mixed_onerdm(prop, ws, psit_dev, nup, ndown):
         afq.det_ops.MixedDensityMatrix(prop.ctx, psit_dev[nup:nup + ndown], ws.Phi_b, Gs, ov,
                                        compact=False)
         G[:, 1] = Gs
-    w = ws.weight
-    return (w[:, None, None, None] * G).sum(0) / w.sum()
+    return G
+
+
+def density_observables(ctx, G, weights, walker_type, which) -> dict:
+    """{"OneRDM": [nspin][N][N], "DiagonalTwoRDM": folded vector} (host copies) of one
+    measurement, for the observables named in `which` ("onerdm", "diag_twordm")."""
+    one, two = afq.rdm_measure(ctx, G, weights, walker_type, onerdm="onerdm" in which,
+                               diag2rdm="diag_twordm" in which)
+    out = {}
+    if one is not None:
+        out["OneRDM"] = afq.host(one)
+    if two is not None:
+        out["DiagonalTwoRDM"] = afq.fold_diag2rdm(afq.host(two), walker_type)
+    return out
+
+
+def _record(store: dict, obs: dict):
+    for k, v in obs.items():
+        store.setdefault(k, []).append(v)
+
+
+def mean_error(series) -> tuple:
+    """(mean, blocking error) of a series of complex observable measurements, element by
+    element; the error of a complex element is error(re) + i error(im)."""
+    x = np.asarray(series)
+    flat = x.reshape(len(x), -1)
+    err = np.array([complex(blocking_error(flat[:, i].real), blocking_error(flat[:, i].imag))
+                    for i in range(flat.shape[1])])
+    return x.mean(0), err.reshape(x.shape[1:])
 
 
 def run_afqmc(prop: "afq.Propagator", ws: "afq.WalkerSet", steps: int, equilibration: int = 100,
               ortho_interval: int = 5, popctl_interval: int = 10, measure_interval: int = 10,
               relax: float | None = None, energy_precision=afq.AFQ_FP64, Eshift=None,
-              progress=None, progress_interval: int = 1000, onerdm_interval: int = 0,
-              psit=None, backprop: dict | None = None) -> Trace:
+              progress=None, progress_interval: int = 1000, estimators: dict | None = None,
+              psit=None) -> Trace:
     """progress(step, trace), when given, is called every progress_interval steps.
 
-    onerdm_interval > 0 (with psit, the trial rows [nup (+ ndown)][N]) records the mixed
-    estimator's OneRDM every onerdm_interval steps after equilibration.  backprop =
-    {"length": L, "ortho": k, "restore": True} records BackPropEstimator's OneRDM
-    (BackPropEstimator.hpp:150-260): the anchor is taken after the last equilibration step (or
-    before the first step without equilibration); L steps later the trial is back-propagated
-    over them and the weighted average of the back-propagated density matrices recorded, and the
-    anchor moves to that step."""
+    `estimators` takes the reference's input block (parameters.hpp estimator parameters):
+      {"mixed": {"measure_interval": k, "onerdm": {}, "diag_twordm": {}},
+       "backprop": {"propagation_steps": [L1, L2, ...], "path_restoration": True,
+                    "extra_path_restoration": False, "walker_ortho_interval": k,
+                    "onerdm": {}, "diag_twordm": {}}}
+    The mixed estimator (MixedEstimator.hpp:122-140; needs psit, the trial rows
+    [nup (+ ndown)][N]) measures its observables on the current walkers every measure_interval
+    steps after equilibration.  Back-propagation (BackPropEstimator.hpp:150-260) anchors after
+    the last equilibration step (before the first step without equilibration); when the steps
+    since the anchor equal one of the propagation lengths the trial is back-propagated over
+    them and the observables measured with the (path-restored) weights, and at the longest
+    length the anchor moves to the current step."""
     import torch
     W = ws.W
     if relax is None:  # parameter_defaults.cpp:310-318
@@ -109,20 +143,32 @@ def run_afqmc(prop: "afq.Propagator", ws: "afq.WalkerSet", steps: int, equilibra
     E = afq.empty((W, 3), torch.complex128)
     ov = afq.empty(W, torch.complex128)
     tr = Trace([], [], [], [], [])
+    est = estimators or {}
+    mixed = est.get("mixed")
+    bp = est.get("backprop")
+    wt = prop.ham.walker_type
     psit_dev = None
-    if onerdm_interval > 0:
+    if mixed is not None:
         if psit is None:
-            raise ValueError("the mixed OneRDM needs the trial rows (psit)")
+            raise ValueError("the mixed estimator needs the trial rows (psit)")
         psit_dev = afq.device_tensor(np.asarray(psit), np.complex128)
+        mixed_interval = int(mixed.get("measure_interval", measure_interval))
     nup = ws.Phi.shape[2]
     ndown = 0 if ws.Phi_b is None else ws.Phi_b.shape[2]
-    bp_len = 0
+    bp_lens = []
     bp_pos = -1
-    if backprop:
-        bp_len = int(backprop["length"])
-        if getattr(prop, "nback", 0) < bp_len:
-            prop.enable_backprop(bp_len)
+    if bp is not None:
+        bp_lens = sorted(int(x) for x in bp["propagation_steps"])
+        if len(set(bp_lens)) != len(bp_lens) or bp_lens[0] <= 0:
+            raise ValueError("propagation_steps must be distinct and positive")
+        restore = ((2 if bp.get("extra_path_restoration", False) else 1)
+                   if bp.get("path_restoration", True) else 0)
+        bp_ortho = int(bp.get("walker_ortho_interval", ortho_interval))
+        if getattr(prop, "nback", 0) < bp_lens[-1]:
+            prop.enable_backprop(bp_lens[-1])
         prop.bp_anchor(ws)   # the anchor before the first step
+        for L in bp_lens:
+            tr.backprop[L] = {}
     for step in range(steps):
         prop.step_graph(ws, Eshift, ortho=(step % ortho_interval == 0),
                         popctl=(popctl_interval > 0 and step % popctl_interval == 0),
@@ -140,18 +186,21 @@ def run_afqmc(prop: "afq.Propagator", ws: "afq.WalkerSet", steps: int, equilibra
             tr.energy.append(complex(vals[:3].sum()))
             tr.total_weight.append(complex(vals[3]))
             tr.eff_walkers.append(float(vals[4].real))
-        if psit_dev is not None and step >= equilibration and step % onerdm_interval == 0:
-            tr.onerdm_mixed.append(afq.host(mixed_onerdm(prop, ws, psit_dev, nup, ndown)))
-        if bp_len:
+        if mixed is not None and step >= equilibration and step % mixed_interval == 0:
+            G = mixed_density_matrices(prop, ws, psit_dev, nup, ndown)
+            _record(tr.mixed, density_observables(prop.ctx, G, ws.weight, wt, mixed))
+        if bp_lens:
             if step == equilibration - 1:
                 prop.bp_anchor(ws)
                 bp_pos = step
-            elif step >= equilibration and step - bp_pos >= bp_len:
-                G, wr = prop.backpropagate(ws, bp_len, backprop.get("ortho", ortho_interval),
-                                           backprop.get("restore", True))
-                tr.onerdm_bp.append(afq.host((wr[:, None, None, None] * G).sum(0) / wr.sum()))
-                prop.bp_anchor(ws)
-                bp_pos = step
+            elif step >= equilibration:
+                n = step - bp_pos
+                if n in tr.backprop:
+                    G, wr = prop.backpropagate(ws, n, bp_ortho, restore)
+                    _record(tr.backprop[n], density_observables(prop.ctx, G, wr, wt, bp))
+                if n >= bp_lens[-1]:
+                    prop.bp_anchor(ws)
+                    bp_pos = step
         if progress is not None and (step + 1) % progress_interval == 0:
             progress(step + 1, tr)
     torch.cuda.synchronize()

@@ -1,7 +1,8 @@
 """Copies the BH (boron hydride, 19 orbitals, 78 Cholesky vectors) input files of the reference's
 functional tests into tests/golden/bh/ and extracts its statistical reference results
 (tests/functional/statistical_references/BH/dense_rhf_basis_*/<trial>/<walkers>/results.h5: the
-run parameters from `input_file`, and mean / error of Energy and its components) with h5dump.
+run parameters from `input_file`, mean / error of Energy and its components, and of the mixed and
+back-propagated OneRDM and DiagonalTwoRDM) with h5dump.
 Run in the container that holds the reference; the GPU box only reads the copies.
 
     python tests/golden/make_bh_fixtures.py
@@ -71,23 +72,25 @@ def main():
             mean = dump_value(path, f"/measurements/Stage0/{name}/mean")
             err = dump_value(path, f"/measurements/Stage0/{name}/error")
             obs[name] = {"mean": mean[:2], "error": err[:2]}
+        # density-matrix observables of the mixed and back-propagation estimators
         rdm = {}
-        # one-body density matrices of the single-determinant cases (the device back-propagation
-        # covers single-determinant trials)
-        for name, ds in ((("mixed", "MixedEstimator/OneRDM"), ("backprop", None))
-                         if trial.endswith("nomsd") else ()):
-            if ds is None:
-                bp = inp["execute"].get("estimators", {}).get("backprop")
-                if not bp:
-                    continue
-                ds = f"BackPropEstimator/Steps={bp['propagation_steps'][-1]}/OneRDM"
-            try:
-                rdm[name] = {"mean": dump_array(path, f"/measurements/Stage0/{ds}/mean"),
-                             "error": dump_array(path, f"/measurements/Stage0/{ds}/error")}
-            except subprocess.CalledProcessError:
-                pass
+        est = ex.get("estimators") or {}
+        groups = []
+        if "mixed" in est:
+            groups.append(("mixed", "MixedEstimator"))
+        if "backprop" in est:
+            for L in est["backprop"]["propagation_steps"]:
+                groups.append((f"backprop/{L}", f"BackPropEstimator/Steps={L}"))
+        for key, grp in groups:
+            for obs in ("OneRDM", "DiagonalTwoRDM"):
+                try:
+                    rdm.setdefault(key, {})[obs] = {
+                        "mean": dump_array(path, f"/measurements/Stage0/{grp}/{obs}/mean"),
+                        "error": dump_array(path, f"/measurements/Stage0/{grp}/{obs}/error")}
+                except subprocess.CalledProcessError:
+                    pass
         refs[f"{trial}/{walkers}"] = {
-            "onerdm": rdm,
+            "rdm": rdm,
             "estimators": ex.get("estimators"),
             "hamiltonian": os.path.basename(ex["hamiltonian"]["filename"]),
             "wavefunction": os.path.basename(ex["wavefunction"]["filename"]),
@@ -103,7 +106,7 @@ def main():
             "observables": obs,
         }
     with open(os.path.join(OUT, "statistical_refs.json"), "w") as f:
-        json.dump(refs, f, indent=1, sort_keys=True)
+        json.dump(refs, f, sort_keys=True)
     print(json.dumps(refs, indent=1)[:2000])
 
 

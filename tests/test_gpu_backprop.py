@@ -11,9 +11,11 @@ one-body density-matrix estimators.
   back-propagation built from the oracle's vHS, propagate('H') and QR (1e-10); the restored
   weights equal WEIGHT times the product of the stored factors.
 - Population control moves the rings and the anchors with their walkers.
-- BH (the reference's inputs): the mixed and back-propagated OneRDM of a seeded run agree with the
-  reference's statistical results element by element under its runner's criterion
-  (run_functional.py:385-410, Bonferroni over the 361 elements).
+- afq_rdm_measure (OneRDM and DiagonalTwoRDM of one measurement) equals the oracle's
+  walker-by-walker sums (1e-12), with reference coefficients and accumulation.
+- BH (the reference's inputs): the mixed and back-propagated OneRDM and DiagonalTwoRDM of a
+  seeded run agree with the reference's statistical results element by element under its
+  runner's criterion (run_functional.py:385-410, Bonferroni over the elements).
 """
 import json
 import os
@@ -125,6 +127,14 @@ def test_rings_and_backpropagation_vs_oracle(gpu):
     # restore=False returns the weights unchanged
     _, w0 = prop.backpropagate(ws, nbp, ortho=ortho, restore=False)
     assert np.array_equal(afq.host(w0), afq.host(ws.weight))
+    # extra path restoration: 2 nbp = 10 factors, of which 3 were never written (they hold 1)
+    _, w2 = prop.backpropagate(ws, nbp, ortho=ortho, restore=2)
+    wref2 = afq.host(ws.weight) * np.prod(
+        [facs[:, (step - 1 - k) % (3 * nback)] for k in range(2 * nbp)], axis=0)
+    assert np.all(facs[:, step:] == 1.0)
+    eW2 = rel(afq.host(w2), wref2)
+    say(f"  extra path restoration {eW2:.2e}")
+    assert eW2 < 1e-12
     prop.close()
 
 
@@ -220,7 +230,7 @@ def _cplx(d):
     return (np.asarray(d["re"]) + 1j * np.asarray(d["im"])).reshape(d["shape"])
 
 
-def _compare_rdm(name, A, Aerr, B, Berr):
+def _compare(name, A, Aerr, B, Berr):
     """run_functional.py:385-410 on one observable tensor."""
     from scipy.stats import norm
     z = norm.ppf(1 - 0.001 / (2 * A.size))
@@ -232,13 +242,44 @@ def _compare_rdm(name, A, Aerr, B, Berr):
     return bool(np.all(d <= tol))
 
 
+@pytest.mark.parametrize("closed", [True, False])
+def test_rdm_measure_vs_oracle(gpu, closed):
+    """afq_rdm_measure against the oracle's walker-by-walker sums, with coefficients and with
+    accumulation over two references."""
+    afq, ctx0, torch = gpu
+    rng = np.random.default_rng(21)
+    W, N = 300, 19
+    nsp = 1 if closed else 2
+    say(f"case: rdm_measure W={W} N={N} {'CLOSED' if closed else 'COLLINEAR'}")
+    G1, G2 = (rng.normal(size=(W, nsp, N, N)) + 1j * rng.normal(size=(W, nsp, N, N)) for _ in range(2))
+    w = rng.uniform(0.5, 1.5, W) * np.exp(0.2j * rng.normal(size=W))
+    c1, c2 = (rng.normal(size=W) + 1j * rng.normal(size=W) for _ in range(2))
+    wt = afq.AFQ_CLOSED if closed else afq.AFQ_COLLINEAR
+    ctx = afq.Context(0)
+    dev = afq.device_tensor
+    one, two = afq.rdm_measure(ctx, dev(G1), dev(w), wt)
+    r1, r2 = orc.rdm_measure(G1, w, closed)
+    e1, e2 = rel(afq.host(one), r1), rel(afq.host(two), r2)
+    one, two = afq.rdm_measure(ctx, dev(G1), dev(w), wt, coeff=dev(c1))
+    one, two = afq.rdm_measure(ctx, dev(G2), dev(w), wt, coeff=dev(c2), out=(one, two))
+    q1, q2 = orc.rdm_measure(G1, w, closed, c1)
+    s1, s2 = orc.rdm_measure(G2, w, closed, c2)
+    e3, e4 = rel(afq.host(one), q1 + s1), rel(afq.host(two), q2 + s2)
+    say(f"  onerdm {e1:.2e} diag2rdm {e2:.2e}; two references {e3:.2e} {e4:.2e}")
+    assert max(e1, e2, e3, e4) < 1e-12
+
+
 @pytest.mark.parametrize("case", ["rhf_nomsd/closed", "uhf_nomsd/collinear"])
-def test_bh_onerdm_statistical_reference(gpu, case):
+def test_bh_rdm_statistical_reference(gpu, case):
+    """Seeded BH runs at the reference's parameters (1536 walkers, dt 0.01, 3000 equilibration
+    steps, 9000 measured steps against the reference's 45000): the mixed and back-propagated
+    (100 steps, path restoration) OneRDM and DiagonalTwoRDM agree with the reference's
+    statistical results element by element."""
     afq, ctx0, torch = gpu
     from safire_amd import driver, h5inputs
     refs = json.load(open(os.path.join(G_BH, "statistical_refs.json")))
     r = refs[case]
-    say(f"case: BH {case} mixed and back-propagated OneRDM vs the reference's statistical results")
+    say(f"case: BH {case} mixed and back-propagated RDMs vs the reference's statistical results")
     wt = afq.AFQ_CLOSED if r["walker_type"] == "CLOSED" else afq.AFQ_COLLINEAR
     s = h5inputs.load_system(os.path.join(G_BH, r["hamiltonian"]), os.path.join(G_BH, r["wavefunction"]),
                              walker_type=wt)
@@ -248,30 +289,22 @@ def test_bh_onerdm_statistical_reference(gpu, case):
                                     E0=s.E0)
     prop = afq.Propagator(ham, W, dt=r["timestep"], seed=42)
     ws = afq.WalkerSet(*s.walkers(W))
-    est = r["estimators"]
     eq = r["equilibration_steps"]
-    bp_len = est["backprop"]["propagation_steps"][-1]
     tr = driver.run_afqmc(prop, ws, steps=eq + 9000, equilibration=eq,
                           ortho_interval=r["walker_ortho_interval"],
                           popctl_interval=r["population_control_interval"],
                           measure_interval=r["measure_interval"],
-                          onerdm_interval=est["mixed"]["measure_interval"], psit=s.PsiT,
-                          backprop={"length": bp_len, "ortho": r["walker_ortho_interval"],
-                                    "restore": est["backprop"]["path_restoration"]},
+                          estimators=r["estimators"], psit=s.PsiT,
                           progress=lambda st, t: say(f"  step {st}"), progress_interval=3000)
     ok = True
-    for name, series in (("mixed", tr.onerdm_mixed), ("backprop", tr.onerdm_bp)):
-        x = np.asarray(series)
-        mean = x.mean(0)
-        err = np.empty(mean.shape, np.complex128)
-        flat = x.reshape(len(x), -1)
-        errf = err.reshape(-1)
-        for i in range(flat.shape[1]):
-            errf[i] = complex(driver.blocking_error(flat[:, i].real),
-                              driver.blocking_error(flat[:, i].imag))
-        ref = r["onerdm"][name]
-        tr_ = [round(float(np.trace(mean[k]).real), 6) for k in range(mean.shape[0])]
-        say(f"  {name}: {len(x)} measurements, trace per spin {tr_}")
-        ok &= _compare_rdm(name, mean, err, _cplx(ref["mean"]), _cplx(ref["error"]))
+    L = r["estimators"]["backprop"]["propagation_steps"][-1]
+    for group, series in (("mixed", tr.mixed), (f"backprop/{L}", tr.backprop[L])):
+        for obs in ("OneRDM", "DiagonalTwoRDM"):
+            mean, err = driver.mean_error(series[obs])
+            ref = r["rdm"][group][obs]
+            if obs == "OneRDM":
+                tr_ = [round(float(np.trace(mean[k]).real), 6) for k in range(mean.shape[0])]
+                say(f"  {group} {obs}: {len(series[obs])} measurements, trace per spin {tr_}")
+            ok &= _compare(f"{group} {obs}", mean, err, _cplx(ref["mean"]), _cplx(ref["error"]))
     prop.close()
     assert ok
