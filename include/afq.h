@@ -212,7 +212,7 @@ int afq_prop_population_control(afq_prop* prop, const afq_walkers* w, double r);
  * population is then left unchanged; the reference aborts).  Synchronizes the stream. */
 int afq_prop_status(afq_prop* prop, int* collapsed);
 
-/* ---- back-propagation (single-determinant trial) --------------------------------------------
+/* ---- back-propagation -------------------------------------------------------------------------
  * Replaces the walker set's back-propagation buffers (WalkerSetBase::resize_bp, storeFields,
  * getWeightFactors, setSlaterMatrixN) and AFQMCBasePropagator::BackPropagate
  * (AFQMCBasePropagator.cpp:404-560) as BackPropEstimator (BackPropEstimator.hpp:150-260) uses
@@ -229,11 +229,30 @@ int afq_prop_bp_anchor(afq_prop* prop, const afq_walkers* w);
  * device; nspin = 1 CLOSED, 2 COLLINEAR): the full mixed density matrix between the
  * back-propagated trial and each walker's anchor, and weights [W] (complex, device): WEIGHT
  * times the path-restoration factors of the last restore * nbp steps (restore = 0 none,
- * 1 path_restoration, 2 extra_path_restoration; BackPropEstimator.hpp:212-228). */
+ * 1 path_restoration, 2 extra_path_restoration; BackPropEstimator.hpp:212-228).  Single-
+ * determinant trials only (AFQ_ERR_ARG for PHMSD: use afq_prop_bp_measure). */
 int afq_prop_backpropagate(afq_prop* prop, const afq_walkers* w, int nbp, int ortho, int restore,
                            double* G, double* weights);
 /* Device views of the rings: fields [W][nback][M] complex, factors [W][3 nback] complex. */
 int afq_prop_bp_views(afq_prop* prop, const double** fields, const double** factors, int* nback);
+
+/* BackPropEstimator::backPropagate (BackPropEstimator.hpp:186-241) with the OneRDM and
+ * DiagonalTwoRDM observables: every reference determinant of the trial (one for NOMSD, the
+ * expansion for PHMSD, PHMSD::getReferences) is back-propagated through the last nbp stored
+ * fields as in afq_prop_backpropagate, weighted per walker by conj(ci) <L_d|anchor>
+ * e^{conj(log det R_d)} normalized over the references (constructBPMeasurementInputs), and the
+ * observables are measured with the path-restored weights (restore as above).  onerdm
+ * [nspin][N][N], diag2rdm [nspin + 1][N][N] as afq_rdm_measure writes them; either may be NULL.
+ * Device pointers; the call synchronizes the stream once. */
+int afq_prop_bp_measure(afq_prop* prop, const afq_walkers* w, int nbp, int ortho, int restore,
+                        double* onerdm, double* diag2rdm);
+/* MixedEstimator::measure (MixedEstimator.hpp:122-140) with the OneRDM and DiagonalTwoRDM
+ * observables: the walkers' full mixed density matrices with every reference determinant,
+ * weighted by conj(ci) <D_d|Phi> normalized over the references (constructMixedMeasurementInputs,
+ * :40-106), and WEIGHT.  Outputs as for afq_prop_bp_measure. */
+int afq_prop_mixed_measure(afq_prop* prop, const afq_walkers* w, double* onerdm, double* diag2rdm);
+/* Number of reference determinants of the trial (Wavefunction::total_number_of_references). */
+int afq_prop_number_of_references(afq_prop* prop, int* nref);
 
 /* ---- density-matrix observables --------------------------------------------------------------
  * One measurement of OneRDM (onerdm.hpp:43-70) and DiagonalTwoRDM (diagonal_twordm.hpp:40-81)

@@ -94,6 +94,9 @@ _SIGS = {
     "afq_prop_backpropagate": [_P, ctypes.POINTER(Walkers), _I, _I, _I, _P, _P],
     "afq_prop_bp_views": [_P, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_I)],
     "afq_rdm_measure": [_P, _I, _I, _I, _P, _P, _P, _I, _P, _P],
+    "afq_prop_bp_measure": [_P, ctypes.POINTER(Walkers), _I, _I, _I, _P, _P],
+    "afq_prop_mixed_measure": [_P, ctypes.POINTER(Walkers), _P, _P],
+    "afq_prop_number_of_references": [_P, ctypes.POINTER(_I)],
     "afq_prop_create_phmsd": [_P, _I, ctypes.POINTER(PropParams), ctypes.POINTER(_P)],
     "afq_phmsd_create": [_P, _P, ctypes.POINTER(_P)],
     "afq_phmsd_destroy": [_P],
@@ -531,6 +534,36 @@ class Propagator:
         _ck(lib().afq_prop_backpropagate(self.h, ctypes.byref(wset.cstruct()), int(nbp), int(ortho),
                                          int(restore), ptr(G), ptr(w)))
         return G, w
+
+    def _rdm_out(self, onerdm, diag2rdm):
+        torch = _torch()
+        nsp = 2 if self.ham.walker_type == AFQ_COLLINEAR else 1
+        N = self.ham.N
+        one = torch.empty((nsp, N, N), dtype=torch.complex128, device="cuda") if onerdm else None
+        two = (torch.empty((nsp + 1, N, N), dtype=torch.complex128, device="cuda") if diag2rdm
+               else None)
+        return one, two
+
+    def bp_measure(self, wset: WalkerSet, nbp: int, ortho: int = 10, restore: int = 1,
+                   onerdm=True, diag2rdm=True):
+        """BackPropEstimator's OneRDM [nspin][N][N] and unfolded DiagonalTwoRDM [nspin+1][N][N]
+        of the trial's references back-propagated over the last nbp steps (None where not
+        asked for)."""
+        one, two = self._rdm_out(onerdm, diag2rdm)
+        _ck(lib().afq_prop_bp_measure(self.h, ctypes.byref(wset.cstruct()), int(nbp), int(ortho),
+                                      int(restore), ptr(one), ptr(two)))
+        return one, two
+
+    def mixed_measure(self, wset: WalkerSet, onerdm=True, diag2rdm=True):
+        """MixedEstimator's OneRDM and unfolded DiagonalTwoRDM of the current walkers."""
+        one, two = self._rdm_out(onerdm, diag2rdm)
+        _ck(lib().afq_prop_mixed_measure(self.h, ctypes.byref(wset.cstruct()), ptr(one), ptr(two)))
+        return one, two
+
+    def number_of_references(self) -> int:
+        v = _I()
+        _ck(lib().afq_prop_number_of_references(self.h, ctypes.byref(v)))
+        return v.value
 
     def bp_rings(self, W: int):
         """Copies of the field ring [W][nback][M] and the factor ring [W][3 nback]."""

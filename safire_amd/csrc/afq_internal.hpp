@@ -74,6 +74,11 @@ void average_pseudo_energy(hipStream_t s, int W, const z64* weight, const z64* e
 // each [nsp][N][N].
 void rdm_measure(hipStream_t s, int W, int nsp, int N, bool collinear, const z64* G, long g_bs,
                  const z64* weight, const z64* coeff, bool accumulate, z64* one, z64* two);
+// Reference coefficients of a multi-reference measurement (MixedEstimator.hpp:61-103,
+// BackPropEstimator.hpp:62-120): x[d][w] = cc[d] exp(lo[d][w] + conj(ad[w][d])) (ad may be
+// null), coeff[d][w] = x[d][w] / sum_d' x[d'][w].  lo [R][W] log overlaps, cc [R] conj(ci).
+void reference_coefficients(hipStream_t s, int W, int R, const z64* lo, const z64* cc,
+                            const z64* ad, z64* coeff);
 
 // ---- elementwise.hip -------------------------------------------------------------------------
 void scale_add_cplx(hipStream_t s, long n, const z64* x, double ar, double ai, z64* y);  // y = a x
@@ -91,6 +96,10 @@ void axpy_z64(hipStream_t s, long n, double ar, double ai, const z64* x, z64* y)
 void permute_akm_mak(hipStream_t s, int A, int K, int M, const z64* in, z64* out);
 void conj_transpose(hipStream_t s, int R, int C, const z64* in, z64* out);
 void broadcast_z64(hipStream_t s, long W, long n, const z64* src, z64* dst);  // dst[w][:] = src
+// dst[r][:] = src[idx[r]][:] for R rows of C elements (idx on the device)
+void gather_rows(hipStream_t s, long R, int C, const int* idx, const z64* src, z64* dst);
+// out[b][j][i][:] = in[b][i][j][:] with in [B][D0][D1][K] and out [B][D1][D0][K]
+void swap_mid(hipStream_t s, long B, int D0, int D1, int K, const z64* in, z64* out);
 // true when every Cholesky vector L[.][.][m] of an [N][N][M] tensor is exactly symmetric
 bool symmetric_cholesky(hipStream_t s, int N, int M, const double* L);
 void real_part(hipStream_t s, long n, const z64* x, double* y);

@@ -432,6 +432,32 @@ int afq_prop_bp_views(afq_prop* prop, const double** fields, const double** fact
   });
 }
 
+int afq_prop_bp_measure(afq_prop* prop, const afq_walkers* w, int nbp, int ortho, int restore,
+                        double* onerdm, double* diag2rdm) {
+  return guard([&] {
+    REQUIRE(prop, "NULL argument");
+    check_walkers(prop, w);
+    prop->p->bp_measure(to_state(w), nbp, ortho, restore, onerdm ? Z(onerdm) : nullptr,
+                        diag2rdm ? Z(diag2rdm) : nullptr);
+  });
+}
+
+int afq_prop_mixed_measure(afq_prop* prop, const afq_walkers* w, double* onerdm, double* diag2rdm) {
+  return guard([&] {
+    REQUIRE(prop, "NULL argument");
+    check_walkers(prop, w);
+    prop->p->mixed_measure(to_state(w), onerdm ? Z(onerdm) : nullptr,
+                           diag2rdm ? Z(diag2rdm) : nullptr);
+  });
+}
+
+int afq_prop_number_of_references(afq_prop* prop, int* nref) {
+  return guard([&] {
+    REQUIRE(prop && nref, "NULL argument");
+    *nref = prop->p->number_of_references();
+  });
+}
+
 int afq_rdm_measure(afq_ctx* ctx, int walker_type, int N, int W, const double* G,
                     const double* weights, const double* coeff, int accumulate, double* onerdm,
                     double* diag2rdm) {

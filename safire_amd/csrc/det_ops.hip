@@ -92,6 +92,7 @@ void mixed_density_matrix(Context& c, const z64* PsiT, int n, int N, const z64* 
   if (W == 0) return;
   hipStream_t s = c.stream();
   if (n == 0) {
+    AFQ_CHECK(compact || g_bs == (long)N * N, kErrShape, "empty density matrix in a strided batch");
     if (!compact) fill_z64(s, (long)W * g_bs, {0, 0}, G);
     return;
   }
@@ -141,6 +142,17 @@ void mixed_density_matrix_left(Context& c, const z64* L, long l_bs, int n, int N
     Opnd<double, true, false> B{&Gc->re, N, (long)n * N};
     gemm_small<double>(c, "mdl_full", W, N, N, n, A, B, EpiStore<double, double>{G, N, g_bs, 1, 0, 0});
   }
+}
+
+void log_overlap_left(Context& c, const z64* L, long l_bs, int n, int N, const z64* Phi,
+                      long phi_bs, int W, z64* ovlp) {
+  if (n == 0 || W == 0) return;
+  AFQ_CHECK(n <= N, kErrShape, "left overlap shape");
+  z64* T = scratch_as<z64>(c, "lol_T", (size_t)W * n * n);
+  Opnd<double, true, false, true> A{&L->re, n, l_bs};
+  Opnd<double, true, false> B{&Phi->re, n, phi_bs};
+  gemm_small<double>(c, "mdl_T", W, n, n, N, A, B, EpiStore<double, double>{T, n, (long)n * n, 1, 0, 0});
+  lu_inverse_logdet(c.stream(), T, (long)n * n, W, n, nullptr, 0, ovlp, 1.0, {0, 0}, false);
 }
 
 void inverse_logdet(Context& c, const z64* A, int nb, int n, z64* ovlp, z64* Ainv) {

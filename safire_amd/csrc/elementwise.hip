@@ -107,6 +107,25 @@ __global__ void k_broadcast(long W, long n, const z64* src, z64* dst) {
   if (t >= W * n) return;
   dst[t] = src[t % n];
 }
+// dst[r][:] = src[idx[r]][:] (rows of length C)
+__global__ void k_gather_rows(long R, int C, const int* idx, const z64* src, z64* dst) {
+  long t = (long)blockIdx.x * kT + threadIdx.x;
+  if (t >= R * C) return;
+  const long r = t / C;
+  dst[t] = src[(long)idx[r] * C + t % C];
+}
+// out[b][j][i][:] = in[b][i][j][:]   (in [B][D0][D1][K], out [B][D1][D0][K])
+__global__ void k_swap_mid(long B, int D0, int D1, int K, const z64* in, z64* out) {
+  long t = (long)blockIdx.x * kT + threadIdx.x;
+  if (t >= B * D0 * D1 * K) return;
+  const int k = (int)(t % K);
+  long r = t / K;
+  const int j = (int)(r % D1);
+  r /= D1;
+  const int i = (int)(r % D0);
+  const long b = r / D0;
+  out[((b * D1 + j) * D0 + i) * K + k] = in[t];
+}
 __global__ void k_real_part(long n, const z64* x, double* y) {
   long i = (long)blockIdx.x * kT + threadIdx.x;
   if (i < n) y[i] = x[i].re;
@@ -301,6 +320,17 @@ void conj_transpose(hipStream_t s, int R, int C, const z64* in, z64* out) {
 void broadcast_z64(hipStream_t s, long W, long n, const z64* src, z64* dst) {
   if (W * n <= 0) return;
   hipLaunchKernelGGL(k_broadcast, grid1(W * n), dim3(kT), 0, s, W, n, src, dst);
+  AFQ_LAUNCH_CHECK();
+}
+void gather_rows(hipStream_t s, long R, int C, const int* idx, const z64* src, z64* dst) {
+  if (R * C <= 0) return;
+  hipLaunchKernelGGL(k_gather_rows, grid1(R * C), dim3(kT), 0, s, R, C, idx, src, dst);
+  AFQ_LAUNCH_CHECK();
+}
+void swap_mid(hipStream_t s, long B, int D0, int D1, int K, const z64* in, z64* out) {
+  const long n = B * D0 * D1 * K;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_swap_mid, grid1(n), dim3(kT), 0, s, B, D0, D1, K, in, out);
   AFQ_LAUNCH_CHECK();
 }
 void real_part(hipStream_t s, long n, const z64* x, double* y) {

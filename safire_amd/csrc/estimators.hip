@@ -9,6 +9,8 @@
 //    observable, its four waves take every fourth walker and the partial sums are combined in a
 //    fixed order, so a measurement is deterministic.  Each walker's row i, column i and diagonal
 //    are read once per workgroup (a few KB per walker, resident in L2 across the row blocks).
+//  * reference_coefficients: the per-walker share of each reference determinant in a
+//    multi-reference measurement (MixedEstimator.hpp:61-103, BackPropEstimator.hpp:62-120).
 #include "afq_internal.hpp"
 
 namespace afq {
@@ -102,7 +104,41 @@ __global__ void __launch_bounds__(kRdmT)
   }
 }
 
+// one thread per walker; the largest real part is taken out before exponentiating, which
+// cancels in the ratio and keeps every term finite
+__global__ void k_ref_coeff(int W, int R, const z64* __restrict__ lo, const z64* __restrict__ cc,
+                            const z64* __restrict__ ad, z64* __restrict__ coeff) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  auto expo = [&](int d) {
+    z64 e = lo[(long)d * W + w];
+    if (ad) {
+      const z64 a = ad[(long)w * R + d];
+      e = {e.re + a.re, e.im - a.im};
+    }
+    return e;
+  };
+  double m = -HUGE_VAL;
+  for (int d = 0; d < R; ++d) m = fmax(m, expo(d).re);
+  z64 tot = {0.0, 0.0};
+  for (int d = 0; d < R; ++d) {
+    const z64 e = expo(d);
+    const double r = exp(e.re - m);
+    const z64 x = cmul(cc[d], z64{r * cos(e.im), r * sin(e.im)});
+    coeff[(long)d * W + w] = x;
+    tot = cadd(tot, x);
+  }
+  for (int d = 0; d < R; ++d) coeff[(long)d * W + w] = cdiv(coeff[(long)d * W + w], tot);
+}
+
 }  // namespace
+
+void reference_coefficients(hipStream_t s, int W, int R, const z64* lo, const z64* cc,
+                            const z64* ad, z64* coeff) {
+  AFQ_CHECK(W > 0 && R > 0, kErrShape, "reference coefficients shape");
+  hipLaunchKernelGGL(k_ref_coeff, dim3(ceil_div(W, 256)), dim3(256), 0, s, W, R, lo, cc, ad, coeff);
+  AFQ_LAUNCH_CHECK();
+}
 
 void rdm_measure(hipStream_t s, int W, int nsp, int N, bool collinear, const z64* G, long g_bs,
                  const z64* weight, const z64* coeff, bool accumulate, z64* one, z64* two) {

@@ -67,6 +67,9 @@ void inverse_logdet(Context& c, const z64* A, int nb, int n, z64* ovlp, z64* Ain
 // at G + w * g_bs; L [W][N][n] (l_bs apart), ovlp += log det T.
 void mixed_density_matrix_left(Context& c, const z64* L, long l_bs, int n, int N, const z64* Phi,
                                long phi_bs, int W, z64* G, long g_bs, z64* ovlp);
+// ovlp[w] += log det(L_w^H Phi_w) for left states L [W][N][n] (batch stride l_bs)
+void log_overlap_left(Context& c, const z64* L, long l_bs, int n, int N, const z64* Phi,
+                      long phi_bs, int W, z64* ovlp);
 // Propagate (propagate.hpp:241-298): Phi <- P1 expM(V) P1 Phi, order-K Taylor series.
 // P1 [N][N] (real when P1r != nullptr), V [W][N][N] in the compute precision.
 struct PropagateArgs {
@@ -194,6 +197,12 @@ class PhmsdTrial {
   void energy(const z64* Pa, const z64* Pb, int W, z64* E, z64* Ov);
   // PHMSD::vMF followed by the charge-decomposition zero_imag
   void vMF(double dt, z64* v);
+  int ndet() const { return ndet_; }
+  // The determinants as references (PHMSD::getReferences, PHMSD.hpp:242-279), in PsiT form:
+  // rows_s [ndet][nel_s][N] are the trial rows each determinant occupies, in the order of
+  // ph_excitations::get_configuration (the reference string with holes replaced by particles);
+  // cc [ndet] = conj(ci).  Device buffers of those sizes.
+  void references(z64* rows0, z64* rows1, z64* cc);
 
  private:
   struct SpinDev {
@@ -225,6 +234,8 @@ class PhmsdTrial {
   int* crow_ = nullptr;          // device alpha row of each coupling [ndet]
   std::vector<int> det_ab_h_;
   std::vector<z64> det_c_h_;
+  // occupied active orbitals of every configuration of spin sp, [nconf][nel]
+  std::vector<int> occupations(int sp) const;
   void woodbury(int s, const z64* Phi, int W, bool want_G, z64* log_ov);
   void ratios(int s, int W, bool want_inverse);
   void weights(int W, z64* Ov);
@@ -307,6 +318,17 @@ class Propagator {
   const z64* bp_fields() const { return bp_fields_; }
   const z64* bp_factors() const { return bp_wfac_; }
 
+  // ---- density-matrix observables of the estimators, over every reference determinant of
+  // the trial (one for NOMSD, the expansion for PHMSD).  onerdm [nspin][N][N] and diag2rdm
+  // [nspin + 1][N][N] (unfolded blocks), either may be null; see rdm_measure.
+  // MixedEstimator::measure (MixedEstimator.hpp:122-140): the walkers' mixed density matrices.
+  void mixed_measure(const WalkerState& s, z64* onerdm, z64* diag2rdm);
+  // BackPropEstimator::backPropagate (BackPropEstimator.hpp:186-241): the references
+  // back-propagated over the last nbp steps against the anchors, with restored weights.
+  void bp_measure(const WalkerState& s, int nbp, int ortho, int restore, z64* onerdm,
+                  z64* diag2rdm);
+  int number_of_references();
+
  private:
   CholeskyOps& ham_;
   Context& ctx_;
@@ -337,6 +359,14 @@ class Propagator {
   z64* bp_wfac_ = nullptr;
   z64* bp_anch_a_ = nullptr;
   z64* bp_anch_b_ = nullptr;
+  // reference determinants in PsiT form: ref_rows_[s] [nref][nel_s][N], ref_cc_ [nref] conj(ci)
+  int nref_ = 0;
+  z64* ref_rows_[2] = {nullptr, nullptr};
+  z64* ref_cc_ = nullptr;
+  void ensure_references();
+  // back-propagates every reference through the last nbp stored fields; on return Lp[s] holds
+  // [W][nref][N][nel_s] and ldR [W][nref] the accumulated log det R of their QR steps
+  void bp_propagate_refs(const WalkerState& s, int nbp, int ortho, z64* Lp[2], z64* ldR);
   void clear_graphs();
   void step_impl(const WalkerState& s, const double* u, const double* Eshift);
 };

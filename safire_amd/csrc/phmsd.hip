@@ -1115,6 +1115,40 @@ void PhmsdTrial::energy(const z64* Pa, const z64* Pb, int W, z64* E, z64* Ov) {
   AFQ_LAUNCH_CHECK();
 }
 
+std::vector<int> PhmsdTrial::occupations(int sp) const {
+  // configurations in place (ph_excitations::get_configuration, Excitations.hpp:442-483)
+  const SpinDev& d = sp_[sp];
+  std::vector<int> occ((size_t)d.nconf * d.nel);
+  for (int u = 0; u < d.nconf; ++u) {
+    for (int i = 0; i < d.nel; ++i) occ[(size_t)u * d.nel + i] = i;
+    const int r = d.rank_h[u];
+    for (int p = 0; p < r; ++p)
+      occ[(size_t)u * d.nel + d.exc_h[(size_t)u * kXS + p]] = d.exc_h[(size_t)u * kXS + r + p];
+  }
+  return occ;
+}
+
+void PhmsdTrial::references(z64* rows0, z64* rows1, z64* cc) {
+  hipStream_t s = ctx_.stream();
+  const int N = ham_.N();
+  z64* rows[2] = {rows0, rows1};
+  for (int sp = 0; sp < 2; ++sp) {
+    const SpinDev& d = sp_[sp];
+    const std::vector<int> occ = occupations(sp);
+    std::vector<int> idx((size_t)ndet_ * d.nel);
+    for (int dd = 0; dd < ndet_; ++dd) {
+      const int u = det_ab_h_[2 * dd + sp];
+      std::copy_n(occ.begin() + (size_t)u * d.nel, d.nel, idx.begin() + (size_t)dd * d.nel);
+    }
+    int* idx_d = dcopy(s, idx);
+    gather_rows(s, (long)ndet_ * d.nel, N, idx_d, d.orbs, rows[sp]);
+    AFQ_HIP(hipStreamSynchronize(s));
+    (void)hipFree(idx_d);
+  }
+  AFQ_HIP(hipMemcpyAsync(cc, det_c_h_.data(), sizeof(z64) * ndet_, hipMemcpyHostToDevice, s));
+  AFQ_HIP(hipStreamSynchronize(s));
+}
+
 void PhmsdTrial::vMF(double dt, z64* v) {
   using gemm::EpiStore;
   using gemm::Opnd;
@@ -1123,18 +1157,7 @@ void PhmsdTrial::vMF(double dt, z64* v) {
   z64* Gfull = scratch_as<z64>(ctx_, "phmf_G", (size_t)ntot * N);
   z64* rat[2];
   z64* wd[2];
-  std::vector<int> occ_h[2];
-  for (int sp = 0; sp < 2; ++sp) {
-    const SpinDev& d = sp_[sp];
-    // configurations in place (ph_excitations::get_configuration)
-    occ_h[sp].resize((size_t)d.nconf * d.nel);
-    for (int u = 0; u < d.nconf; ++u) {
-      for (int i = 0; i < d.nel; ++i) occ_h[sp][(size_t)u * d.nel + i] = i;
-      int r = d.rank_h[u];
-      for (int p = 0; p < r; ++p)
-        occ_h[sp][(size_t)u * d.nel + d.exc_h[(size_t)u * kXS + p]] = d.exc_h[(size_t)u * kXS + r + p];
-    }
-  }
+  std::vector<int> occ_h[2] = {occupations(0), occupations(1)};
   std::vector<void*> tmp;
   auto talloc = [&](size_t bytes) {
     void* p = nullptr;

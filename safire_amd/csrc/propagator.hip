@@ -90,7 +90,8 @@ Propagator::~Propagator() {
   }
   for (void* q : {(void*)vMF_, (void*)P1_, (void*)P1r_, (void*)P1r32_, (void*)P1c32_,
                   (void*)step_dev_, (void*)status_dev_, (void*)bp_fields_, (void*)bp_wfac_,
-                  (void*)bp_anch_a_, (void*)bp_anch_b_})
+                  (void*)bp_anch_a_, (void*)bp_anch_b_, (void*)ref_rows_[0], (void*)ref_rows_[1],
+                  (void*)ref_cc_})
     if (q) (void)hipFree(q);
 }
 
@@ -221,7 +222,6 @@ void Propagator::clear_graphs() {
 
 void Propagator::enable_backprop(int nback) {
   AFQ_CHECK(nback > 0, kErrArg, "back-propagation length must be positive");
-  AFQ_CHECK(!trial_, kErrArg, "back-propagation is built for single-determinant trials");
   AFQ_CHECK(bp_n_ == 0, kErrState, "back-propagation is already enabled");
   hipStream_t s = ctx_.stream();
   const int N = ham_.N(), M = ham_.M();
@@ -258,57 +258,190 @@ void Propagator::bp_anchor(const WalkerState& st) {
                            hipMemcpyDeviceToDevice, s));
 }
 
-void Propagator::backpropagate(const WalkerState& st, int nbp, int ortho, int restore, z64* G,
-                               z64* wout) {
+void Propagator::ensure_references() {
+  if (nref_ > 0) return;
+  hipStream_t s = ctx_.stream();
+  const int N = ham_.N();
+  const int R = trial_ ? trial_->ndet() : 1;
+  const int nsl[2] = {wup_, wdn_};
+  z64* buf[3] = {nullptr, nullptr, nullptr};
+  try {
+    for (int sp = 0; sp < 2; ++sp) buf[sp] = dalloc<z64>((size_t)R * std::max(nsl[sp], 1) * N);
+    buf[2] = dalloc<z64>(R);
+    if (trial_) {
+      trial_->references(buf[0], buf[1], buf[2]);
+    } else {
+      // NOMSD: the single determinant, coefficient 1
+      for (int sp = 0; sp < 2; ++sp)
+        if (nsl[sp] > 0)
+          AFQ_HIP(hipMemcpyAsync(buf[sp], ham_.PsiT() + (sp == 0 ? 0 : (size_t)wup_ * N),
+                                 sizeof(z64) * nsl[sp] * N, hipMemcpyDeviceToDevice, s));
+      fill_z64(s, 1, {1.0, 0.0}, buf[2]);
+      AFQ_HIP(hipStreamSynchronize(s));
+    }
+  } catch (...) {
+    for (z64* q : buf)
+      if (q) (void)hipFree(q);
+    throw;
+  }
+  ref_rows_[0] = buf[0];
+  ref_rows_[1] = buf[1];
+  ref_cc_ = buf[2];
+  nref_ = R;
+}
+
+int Propagator::number_of_references() {
+  ensure_references();
+  return nref_;
+}
+
+void Propagator::bp_propagate_refs(const WalkerState& st, int nbp, int ortho, z64* Lp[2], z64* ldR) {
   AFQ_CHECK(bp_n_ > 0, kErrState, "back-propagation is not enabled");
   AFQ_CHECK(nbp > 0 && nbp <= bp_n_, kErrArg, "back-propagation length exceeds the field ring");
   AFQ_CHECK(ortho > 0, kErrArg, "orthogonalization interval must be positive");
-  AFQ_CHECK(restore >= 0 && restore <= 2, kErrArg, "restore must be 0, 1 or 2");
   AFQ_CHECK(st.W <= W_, kErrShape, "walker count exceeds the propagator's capacity");
+  ensure_references();
   hipStream_t s = ctx_.stream();
-  const int W = st.W, N = ham_.N(), M = ham_.M();
+  const int W = st.W, N = ham_.N(), M = ham_.M(), R = nref_;
   const bool coll = !closed_ && wdn_ > 0;
+  const int nsp = coll ? 2 : 1;
+  const int nsl[2] = {wup_, wdn_};
   uint32_t C = 0;  // completed steps: the newest stored field sits at slot (C - 1) % nback
   AFQ_HIP(hipMemcpyAsync(&C, step_dev_, sizeof(C), hipMemcpyDeviceToHost, s));
   AFQ_HIP(hipStreamSynchronize(s));
-  // left states = the trial orbitals (Phi-shaped: PsiT^H of each spin), one copy per walker
-  const int nsp = coll ? 2 : 1;
-  z64* L[2] = {scratch_as<z64>(ctx_, "bp_L0", (size_t)W * N * wup_),
-               coll ? scratch_as<z64>(ctx_, "bp_L1", (size_t)W * N * wdn_) : nullptr};
-  const int nsl[2] = {wup_, wdn_};
-  z64* ref = scratch_as<z64>(ctx_, "bp_ref", (size_t)N * std::max(wup_, wdn_));
+  // L_s [W][N][R n_s]: every walker starts from the references' orbitals, reference d in
+  // columns d n_s .. (d + 1) n_s - 1; the propagators act on all of them at once.  A QR step
+  // needs each reference contiguous, [W][R][N][n_s] (Lp), and back.
+  z64* L[2] = {nullptr, nullptr};
   for (int sp = 0; sp < nsp; ++sp) {
-    conj_transpose(s, nsl[sp], N, ham_.PsiT() + (sp == 0 ? 0 : (size_t)wup_ * N), ref);
-    broadcast_z64(s, W, (long)N * nsl[sp], ref, L[sp]);
+    const long cols = (long)R * nsl[sp];
+    z64* ref = scratch_as<z64>(ctx_, sp == 0 ? "bp_ref0" : "bp_ref1", (size_t)N * cols);
+    conj_transpose(s, (int)cols, N, ref_rows_[sp], ref);
+    L[sp] = scratch_as<z64>(ctx_, sp == 0 ? "bp_L0" : "bp_L1", (size_t)W * N * cols);
+    broadcast_z64(s, W, (long)N * cols, ref, L[sp]);
+    Lp[sp] = R > 1 ? scratch_as<z64>(ctx_, sp == 0 ? "bp_Lp0" : "bp_Lp1", (size_t)W * N * cols)
+                   : L[sp];
   }
+  fill_z64(s, (long)W * R, {0, 0}, ldR);
   z64* X = scratch_as<z64>(ctx_, "bp_X", (size_t)W * M);
   z64* V = scratch_as<z64>(ctx_, "bp_V", (size_t)W * N * N);
-  z64* ldet = scratch_as<z64>(ctx_, "bp_ldet", W);
   for (int ni = nbp - 1; ni >= 0; --ni) {
     const long back = (long)nbp - ni;  // 1 = the newest field
     const int slot = (int)((((long)C - back) % bp_n_ + bp_n_) % bp_n_);
     gather_fields(s, W, M, bp_fields_, bp_n_, slot, X);
     ham_.vHS(X, W, p_.dt, V, kFP64);
+    const bool qr = ni == 0 || ni % ortho == 0;
     for (int sp = 0; sp < nsp; ++sp) {
-      det_ops::PropagateArgs pa{W, N, nsl[sp], p_.order, 'H', P1_, real_P1_ ? P1r_ : nullptr,
-                                nullptr, nullptr, V, (long)N * N, L[sp], (long)N * nsl[sp], kFP64};
+      const int n = nsl[sp];
+      det_ops::PropagateArgs pa{W, N, R * n, p_.order, 'H', P1_, real_P1_ ? P1r_ : nullptr,
+                                nullptr, nullptr, V, (long)N * N, L[sp], (long)N * R * n, kFP64};
       det_ops::propagate(ctx_, pa);
-      if (ni == 0 || ni % ortho == 0) {
-        fill_z64(s, W, {0, 0}, ldet);
-        det_ops::orthogonalize(ctx_, L[sp], (long)N * nsl[sp], W, N, nsl[sp], ldet);
+      if (qr) {
+        if (R > 1) swap_mid(s, W, N, R, n, L[sp], Lp[sp]);
+        // logdetR[w][d] accumulates log det R of both spins (AFQMCBasePropagator.cpp:473-478)
+        det_ops::orthogonalize(ctx_, Lp[sp], (long)N * n, W * R, N, n, ldR);
+        if (R > 1 && ni != 0) swap_mid(s, W, R, N, n, Lp[sp], L[sp]);
       }
     }
   }
-  z64* ov = scratch_as<z64>(ctx_, "bp_ov", W);
+}
+
+void Propagator::backpropagate(const WalkerState& st, int nbp, int ortho, int restore, z64* G,
+                               z64* wout) {
+  AFQ_CHECK(restore >= 0 && restore <= 2, kErrArg, "restore must be 0, 1 or 2");
+  ensure_references();
+  AFQ_CHECK(nref_ == 1, kErrArg,
+            "per-walker density matrices are defined for a single reference; use bp_measure");
+  hipStream_t s = ctx_.stream();
+  const int W = st.W, N = ham_.N();
+  const bool coll = !closed_ && wdn_ > 0;
+  const int nsp = coll ? 2 : 1;
+  const int nsl[2] = {wup_, wdn_};
+  z64* Lp[2] = {nullptr, nullptr};
+  z64* ldR = scratch_as<z64>(ctx_, "bp_ldR", (size_t)W);
+  bp_propagate_refs(st, nbp, ortho, Lp, ldR);
   z64* anch[2] = {bp_anch_a_, bp_anch_b_};
-  for (int sp = 0; sp < nsp; ++sp) {
-    fill_z64(s, W, {0, 0}, ov);
-    det_ops::mixed_density_matrix_left(ctx_, L[sp], (long)N * nsl[sp], nsl[sp], N, anch[sp],
+  for (int sp = 0; sp < nsp; ++sp)
+    det_ops::mixed_density_matrix_left(ctx_, Lp[sp], (long)N * nsl[sp], nsl[sp], N, anch[sp],
                                        (long)N * nsl[sp], W, G + (size_t)sp * N * N,
-                                       (long)nsp * N * N, ov);
-  }
+                                       (long)nsp * N * N, nullptr);
   // the factor ring holds 3 nback steps, so extra path restoration (2 nbp) always fits
+  uint32_t C = 0;
+  AFQ_HIP(hipMemcpyAsync(&C, step_dev_, sizeof(C), hipMemcpyDeviceToHost, s));
+  AFQ_HIP(hipStreamSynchronize(s));
   restore_weights(s, W, st.weight, bp_wfac_, bp_hl_, C, restore * nbp, wout);
+}
+
+void Propagator::bp_measure(const WalkerState& st, int nbp, int ortho, int restore, z64* onerdm,
+                            z64* diag2rdm) {
+  AFQ_CHECK(restore >= 0 && restore <= 2, kErrArg, "restore must be 0, 1 or 2");
+  ensure_references();
+  hipStream_t s = ctx_.stream();
+  const int W = st.W, N = ham_.N(), R = nref_;
+  const bool coll = !closed_ && wdn_ > 0;
+  const int nsp = coll ? 2 : 1;
+  const int nsl[2] = {wup_, wdn_};
+  z64* Lp[2] = {nullptr, nullptr};
+  z64* ldR = scratch_as<z64>(ctx_, "bp_ldR", (size_t)W * R);
+  bp_propagate_refs(st, nbp, ortho, Lp, ldR);
+  z64* anch[2] = {bp_anch_a_, bp_anch_b_};
+  // reference coefficients (constructBPMeasurementInputs, BackPropEstimator.hpp:62-104):
+  // conj(ci) <L_d|anchor> e^{conj(logdetR)}, normalized over the references of each walker
+  z64* lo = scratch_as<z64>(ctx_, "bpm_lo", (size_t)R * W);
+  z64* cf = scratch_as<z64>(ctx_, "bpm_cf", (size_t)R * W);
+  fill_z64(s, (long)R * W, {0, 0}, lo);
+  for (int d = 0; d < R; ++d)
+    for (int sp = 0; sp < nsp; ++sp)
+      det_ops::log_overlap_left(ctx_, Lp[sp] + (size_t)d * N * nsl[sp], (long)R * N * nsl[sp],
+                                nsl[sp], N, anch[sp], (long)N * nsl[sp], W, lo + (size_t)d * W);
+  reference_coefficients(s, W, R, lo, ref_cc_, ldR, cf);
+  // path-restored weights (BackPropEstimator.hpp:212-228)
+  uint32_t C = 0;
+  AFQ_HIP(hipMemcpyAsync(&C, step_dev_, sizeof(C), hipMemcpyDeviceToHost, s));
+  AFQ_HIP(hipStreamSynchronize(s));
+  z64* wr = scratch_as<z64>(ctx_, "bpm_w", W);
+  restore_weights(s, W, st.weight, bp_wfac_, bp_hl_, C, restore * nbp, wr);
+  z64* G = scratch_as<z64>(ctx_, "bpm_G", (size_t)W * nsp * N * N);
+  for (int d = 0; d < R; ++d) {
+    for (int sp = 0; sp < nsp; ++sp)
+      det_ops::mixed_density_matrix_left(ctx_, Lp[sp] + (size_t)d * N * nsl[sp],
+                                         (long)R * N * nsl[sp], nsl[sp], N, anch[sp],
+                                         (long)N * nsl[sp], W, G + (size_t)sp * N * N,
+                                         (long)nsp * N * N, nullptr);
+    rdm_measure(s, W, nsp, N, coll, G, (long)nsp * N * N, wr, cf + (size_t)d * W, d > 0, onerdm,
+                diag2rdm);
+  }
+}
+
+void Propagator::mixed_measure(const WalkerState& st, z64* onerdm, z64* diag2rdm) {
+  AFQ_CHECK(st.W <= W_, kErrShape, "walker count exceeds the propagator's capacity");
+  ensure_references();
+  hipStream_t s = ctx_.stream();
+  const int W = st.W, N = ham_.N(), R = nref_;
+  const bool coll = !closed_ && wdn_ > 0;
+  const int nsp = coll ? 2 : 1;
+  const int nsl[2] = {wup_, wdn_};
+  const z64* Phi[2] = {st.Phi, st.Phi_b};
+  // reference coefficients (constructMixedMeasurementInputs, MixedEstimator.hpp:40-106):
+  // conj(ci) <D_d|Phi>, normalized over the references of each walker (the reference's shift by
+  // OVLP cancels in the ratio)
+  z64* lo = scratch_as<z64>(ctx_, "mm_lo", (size_t)R * W);
+  z64* cf = scratch_as<z64>(ctx_, "mm_cf", (size_t)R * W);
+  fill_z64(s, (long)R * W, {0, 0}, lo);
+  for (int d = 0; d < R; ++d)
+    for (int sp = 0; sp < nsp; ++sp)
+      det_ops::log_overlap(ctx_, ref_rows_[sp] + (size_t)d * nsl[sp] * N, nsl[sp], N, Phi[sp],
+                           (long)N * nsl[sp], W, lo + (size_t)d * W);
+  reference_coefficients(s, W, R, lo, ref_cc_, nullptr, cf);
+  z64* G = scratch_as<z64>(ctx_, "mm_G", (size_t)W * nsp * N * N);
+  for (int d = 0; d < R; ++d) {
+    for (int sp = 0; sp < nsp; ++sp)
+      det_ops::mixed_density_matrix(ctx_, ref_rows_[sp] + (size_t)d * nsl[sp] * N, nsl[sp], N,
+                                    Phi[sp], (long)N * nsl[sp], W, G + (size_t)sp * N * N,
+                                    (long)nsp * N * N, nullptr, false);
+    rdm_measure(s, W, nsp, N, coll, G, (long)nsp * N * N, st.weight, cf + (size_t)d * W, d > 0,
+                onerdm, diag2rdm);
+  }
 }
 
 int Propagator::status() {

@@ -68,29 +68,9 @@ def initialize(prop: "afq.Propagator", ws: "afq.WalkerSet", precision=afq.AFQ_FP
     return esh
 
 
-def mixed_density_matrices(prop, ws, psit_dev, nup, ndown):
-    """The full mixed density matrix of every walker with the trial, G [W][nspin][N][N] (the
-    per-walker input of MixedEstimator's observables, MixedEstimator.hpp:122-140)."""
-    import torch
-    W, N = ws.W, prop.ham.N
-    coll = ws.Phi_b is not None
-    G = torch.empty((W, 2 if coll else 1, N, N), dtype=torch.complex128, device="cuda")
-    ov = torch.zeros(W, dtype=torch.complex128, device="cuda")
-    Gs = afq.empty((W, N, N), torch.complex128)
-    afq.det_ops.MixedDensityMatrix(prop.ctx, psit_dev[:nup], ws.Phi, Gs, ov, compact=False)
-    G[:, 0] = Gs
-    if coll:
-        afq.det_ops.MixedDensityMatrix(prop.ctx, psit_dev[nup:nup + ndown], ws.Phi_b, Gs, ov,
-                                       compact=False)
-        G[:, 1] = Gs
-    return G
-
-
-def density_observables(ctx, G, weights, walker_type, which) -> dict:
-    """{"OneRDM": [nspin][N][N], "DiagonalTwoRDM": folded vector} (host copies) of one
-    measurement, for the observables named in `which` ("onerdm", "diag_twordm")."""
-    one, two = afq.rdm_measure(ctx, G, weights, walker_type, onerdm="onerdm" in which,
-                               diag2rdm="diag_twordm" in which)
+def _observables(one, two, walker_type) -> dict:
+    """{"OneRDM": [nspin][N][N], "DiagonalTwoRDM": the reference's folded vector} (host
+    copies) of one measurement."""
     out = {}
     if one is not None:
         out["OneRDM"] = afq.host(one)
@@ -117,8 +97,8 @@ def mean_error(series) -> tuple:
 def run_afqmc(prop: "afq.Propagator", ws: "afq.WalkerSet", steps: int, equilibration: int = 100,
               ortho_interval: int = 5, popctl_interval: int = 10, measure_interval: int = 10,
               relax: float | None = None, energy_precision=afq.AFQ_FP64, Eshift=None,
-              progress=None, progress_interval: int = 1000, estimators: dict | None = None,
-              psit=None) -> Trace:
+              progress=None, progress_interval: int = 1000,
+              estimators: dict | None = None) -> Trace:
     """progress(step, trace), when given, is called every progress_interval steps.
 
     `estimators` takes the reference's input block (parameters.hpp estimator parameters):
@@ -126,9 +106,9 @@ def run_afqmc(prop: "afq.Propagator", ws: "afq.WalkerSet", steps: int, equilibra
        "backprop": {"propagation_steps": [L1, L2, ...], "path_restoration": True,
                     "extra_path_restoration": False, "walker_ortho_interval": k,
                     "onerdm": {}, "diag_twordm": {}}}
-    The mixed estimator (MixedEstimator.hpp:122-140; needs psit, the trial rows
-    [nup (+ ndown)][N]) measures its observables on the current walkers every measure_interval
-    steps after equilibration.  Back-propagation (BackPropEstimator.hpp:150-260) anchors after
+    The mixed estimator (MixedEstimator.hpp:122-140) measures its observables on the current
+    walkers every measure_interval steps after equilibration.  Both estimators sum over the
+    trial's reference determinants (one for NOMSD, the expansion for PHMSD).  Back-propagation (BackPropEstimator.hpp:150-260) anchors after
     the last equilibration step (before the first step without equilibration); when the steps
     since the anchor equal one of the propagation lengths the trial is back-propagated over
     them and the observables measured with the (path-restored) weights, and at the longest
@@ -147,14 +127,8 @@ def run_afqmc(prop: "afq.Propagator", ws: "afq.WalkerSet", steps: int, equilibra
     mixed = est.get("mixed")
     bp = est.get("backprop")
     wt = prop.ham.walker_type
-    psit_dev = None
     if mixed is not None:
-        if psit is None:
-            raise ValueError("the mixed estimator needs the trial rows (psit)")
-        psit_dev = afq.device_tensor(np.asarray(psit), np.complex128)
         mixed_interval = int(mixed.get("measure_interval", measure_interval))
-    nup = ws.Phi.shape[2]
-    ndown = 0 if ws.Phi_b is None else ws.Phi_b.shape[2]
     bp_lens = []
     bp_pos = -1
     if bp is not None:
@@ -187,8 +161,8 @@ def run_afqmc(prop: "afq.Propagator", ws: "afq.WalkerSet", steps: int, equilibra
             tr.total_weight.append(complex(vals[3]))
             tr.eff_walkers.append(float(vals[4].real))
         if mixed is not None and step >= equilibration and step % mixed_interval == 0:
-            G = mixed_density_matrices(prop, ws, psit_dev, nup, ndown)
-            _record(tr.mixed, density_observables(prop.ctx, G, ws.weight, wt, mixed))
+            one, two = prop.mixed_measure(ws, "onerdm" in mixed, "diag_twordm" in mixed)
+            _record(tr.mixed, _observables(one, two, wt))
         if bp_lens:
             if step == equilibration - 1:
                 prop.bp_anchor(ws)
@@ -196,8 +170,9 @@ def run_afqmc(prop: "afq.Propagator", ws: "afq.WalkerSet", steps: int, equilibra
             elif step >= equilibration:
                 n = step - bp_pos
                 if n in tr.backprop:
-                    G, wr = prop.backpropagate(ws, n, bp_ortho, restore)
-                    _record(tr.backprop[n], density_observables(prop.ctx, G, wr, wt, bp))
+                    one, two = prop.bp_measure(ws, n, bp_ortho, restore, "onerdm" in bp,
+                                               "diag_twordm" in bp)
+                    _record(tr.backprop[n], _observables(one, two, wt))
                 if n >= bp_lens[-1]:
                     prop.bp_anchor(ws)
                     bp_pos = step

@@ -124,6 +124,16 @@ def test_rings_and_backpropagation_vs_oracle(gpu):
     eW = rel(afq.host(wr), wref)
     say(f"  back-propagated G {eG:.2e}  restored weights {eW:.2e}")
     assert eG < TOL and eW < 1e-12
+    # the estimator entry point on the single reference
+    o1, o2 = orc.rdm_measure(Gr, wref, True)
+    d1, d2 = prop.bp_measure(ws, nbp, ortho=ortho, restore=1)
+    eM = max(rel(afq.host(d1), o1), rel(afq.host(d2), o2))
+    d1, d2 = prop.mixed_measure(ws)
+    Gm, _ = orc.mixed_density_matrix(osys.PsiT, afq.host(ws.Phi), compact=False)
+    o1, o2 = orc.rdm_measure(Gm[:, None], afq.host(ws.weight), True)
+    eX = max(rel(afq.host(d1), o1), rel(afq.host(d2), o2))
+    say(f"  bp_measure {eM:.2e}  mixed_measure {eX:.2e}")
+    assert eM < TOL and eX < TOL
     # restore=False returns the weights unchanged
     _, w0 = prop.backpropagate(ws, nbp, ortho=ortho, restore=False)
     assert np.array_equal(afq.host(w0), afq.host(ws.weight))
@@ -192,6 +202,97 @@ def test_collinear_backpropagation_vs_oracle(gpu):
     say(f"  back-propagated G (both spins) {eG:.2e}")
     assert eG < TOL
     prop.close()
+
+
+def test_phmsd_measurements_vs_oracle(gpu):
+    """Multi-reference measurements over a 60-determinant PHMSD trial: the mixed and
+    back-propagated OneRDM / DiagonalTwoRDM against the oracle's per-determinant sums
+    (density matrices, overlaps, QR log-determinants and coefficients of every reference)."""
+    afq, ctx0, torch = gpu
+    from safire_amd.phmsd import PHMSD, build_ph_struct
+    from safire_amd.synthetic import make_phmsd_problem
+    P = make_phmsd_problem(N=24, nel=6, M=40, ndet=60, ncore=2, nactive=8, max_rank=4, seed=21)
+    W, seed, nback, nbp, ortho, dt = 8, 41, 5, 4, 2, 0.005
+    say(f"case: PHMSD measurements N=24 nel=6 ndet=60 W={W}, window {nbp}, QR every {ortho}")
+    ctx = afq.Context(0)
+    ham = afq.HamiltonianOperations(ctx, P.h1, P.Likn, P.PsiT, P.nact, P.nact,
+                                    walker_type=afq.AFQ_COLLINEAR)
+    trial = PHMSD(ham, build_ph_struct(P.occa, P.occb, P.ci))
+    prop = afq.Propagator(ham, W, dt=dt, seed=seed, trial=trial)
+    assert prop.number_of_references() == len(P.ci)
+    prop.enable_backprop(nback)
+    ws = afq.WalkerSet(*P.walkers(W, seed=3, noise=0.3))
+    E = afq.empty((W, 3), torch.complex128)
+    ov = afq.empty(W, torch.complex128)
+    prop.Energy(ws, E, ov)
+    ws.eloc.copy_(E.sum(1))
+    ws.ovlp.copy_(ov)
+    Esh = afq.device_tensor(np.array([float(afq.host(E.sum(1)).real.mean())]))
+    prop.set_step(0)
+    prop.Propagate(ws, Esh)
+    prop.bp_anchor(ws)
+    anchors = (afq.host(ws.Phi), afq.host(ws.Phi_b))
+    for _ in range(nbp):
+        prop.Propagate(ws, Esh)
+    step = 1 + nbp
+    Pa, Pb = afq.host(ws.Phi), afq.host(ws.Phi_b)
+    w = afq.host(ws.weight)
+    refs = [(P.orbs[list(a)], P.orbs[list(b)], c) for a, b, c in zip(P.occa, P.occb, P.ci)]
+
+    def measure(Gs, los, weights):
+        lo = np.array(los)
+        x = np.conj(np.array([c for _, _, c in refs]))[:, None] * np.exp(lo - lo.real.max(0))
+        cf = x / x.sum(0)
+        one = two = 0
+        for d, G in enumerate(Gs):
+            o, t = orc.rdm_measure(G, weights, False, cf[d])
+            one, two = one + o, two + t
+        return one, two
+
+    # mixed estimator
+    Gs, los = [], []
+    for ra, rb, _ in refs:
+        Ga, oa = orc.mixed_density_matrix(ra, Pa, compact=False)
+        Gb, ob = orc.mixed_density_matrix(rb, Pb, compact=False)
+        Gs.append(np.stack([Ga, Gb], axis=1))
+        los.append(oa + ob)
+    m1, m2 = measure(Gs, los, w)
+    d1, d2 = prop.mixed_measure(ws)
+    e1, e2 = rel(afq.host(d1), m1), rel(afq.host(d2), m2)
+    say(f"  mixed: OneRDM {e1:.2e}  DiagonalTwoRDM {e2:.2e}")
+    # back-propagation estimator
+    fields, facs = prop.bp_rings(W)
+    P1 = afq.host(prop.P1())
+    Vs = [orc.vhs(P.Likn, fields[:, (step - (nbp - ni)) % nback], dt) for ni in range(nbp)]
+    Gs, los = [], []
+    for ra, rb, _ in refs:
+        Ls = [np.broadcast_to(r.conj().T, (W,) + r.conj().T.shape).copy() for r in (ra, rb)]
+        ldR = np.zeros(W, np.complex128)
+        for ni in range(nbp - 1, -1, -1):
+            for sp in range(2):
+                Ls[sp] = orc.propagate(P1, Vs[ni], Ls[sp], 6, "H")
+                if ni == 0 or ni % ortho == 0:
+                    Ls[sp], ld = orc.orthogonalize(Ls[sp])
+                    ldR += ld
+        G = np.empty((W, 2, P.N, P.N), np.complex128)
+        lo = np.conj(ldR).copy()
+        for sp in range(2):
+            for k in range(W):
+                A, B = Ls[sp][k], anchors[sp][k]
+                T = A.conj().T @ B
+                G[k, sp] = (B @ np.linalg.solve(T, A.conj().T)).T
+                sg, la = np.linalg.slogdet(T)
+                lo[k] += np.log(sg) + la
+        Gs.append(G)
+        los.append(lo)
+    wr = w * np.prod([facs[:, (step - 1 - k) % (3 * nback)] for k in range(nbp)], axis=0)
+    b1, b2 = measure(Gs, los, wr)
+    d1, d2 = prop.bp_measure(ws, nbp, ortho=ortho, restore=1)
+    e3, e4 = rel(afq.host(d1), b1), rel(afq.host(d2), b2)
+    say(f"  back-propagated: OneRDM {e3:.2e}  DiagonalTwoRDM {e4:.2e}")
+    prop.close()
+    trial.close()
+    assert max(e1, e2, e3, e4) < TOL
 
 
 def test_population_control_moves_backprop_rows(gpu):
@@ -269,14 +370,17 @@ def test_rdm_measure_vs_oracle(gpu, closed):
     assert max(e1, e2, e3, e4) < 1e-12
 
 
-@pytest.mark.parametrize("case", ["rhf_nomsd/closed", "uhf_nomsd/collinear"])
+@pytest.mark.parametrize("case", ["rhf_nomsd/closed", "uhf_nomsd/collinear",
+                                  "rcasci_rhf_phmsd/collinear", "rcasci_uhf_phmsd/collinear"])
 def test_bh_rdm_statistical_reference(gpu, case):
     """Seeded BH runs at the reference's parameters (1536 walkers, dt 0.01, 3000 equilibration
     steps, 9000 measured steps against the reference's 45000): the mixed and back-propagated
     (100 steps, path restoration) OneRDM and DiagonalTwoRDM agree with the reference's
-    statistical results element by element."""
+    statistical results element by element; the PHMSD trials sum over their 44 (RHF) and
+    UHF-CASCI determinants as references."""
     afq, ctx0, torch = gpu
     from safire_amd import driver, h5inputs
+    from safire_amd.phmsd import PHMSD
     refs = json.load(open(os.path.join(G_BH, "statistical_refs.json")))
     r = refs[case]
     say(f"case: BH {case} mixed and back-propagated RDMs vs the reference's statistical results")
@@ -285,16 +389,27 @@ def test_bh_rdm_statistical_reference(gpu, case):
                              walker_type=wt)
     W = r["walkers"]
     ctx = afq.Context(0)
-    ham = afq.HamiltonianOperations(ctx, s.h1, s.Likn, s.PsiT, s.nup, s.ndown, walker_type=wt,
-                                    E0=s.E0)
-    prop = afq.Propagator(ham, W, dt=r["timestep"], seed=42)
-    ws = afq.WalkerSet(*s.walkers(W))
+    trial = None
+    if s.ph is None:
+        ham = afq.HamiltonianOperations(ctx, s.h1, s.Likn, s.PsiT, s.nup, s.ndown, walker_type=wt,
+                                        E0=s.E0)
+        Pa, Pb = s.walkers(W)
+    else:
+        ham = afq.HamiltonianOperations(ctx, s.h1, s.Likn, s.PsiT, s.nact, s.nact,
+                                        walker_type=afq.AFQ_COLLINEAR, E0=s.E0)
+        trial = PHMSD(ham, s.ph)
+        up = np.ascontiguousarray(s.PsiT[:s.nup].conj().T)
+        Pa = np.broadcast_to(up, (W,) + up.shape).copy()
+        Pb = Pa.copy()
+    prop = afq.Propagator(ham, W, dt=r["timestep"], seed=42, trial=trial)
+    say(f"  {prop.number_of_references()} reference determinants")
+    ws = afq.WalkerSet(Pa, Pb)
     eq = r["equilibration_steps"]
     tr = driver.run_afqmc(prop, ws, steps=eq + 9000, equilibration=eq,
                           ortho_interval=r["walker_ortho_interval"],
                           popctl_interval=r["population_control_interval"],
                           measure_interval=r["measure_interval"],
-                          estimators=r["estimators"], psit=s.PsiT,
+                          estimators=r["estimators"],
                           progress=lambda st, t: say(f"  step {st}"), progress_interval=3000)
     ok = True
     L = r["estimators"]["backprop"]["propagation_steps"][-1]
@@ -307,4 +422,6 @@ def test_bh_rdm_statistical_reference(gpu, case):
                 say(f"  {group} {obs}: {len(series[obs])} measurements, trace per spin {tr_}")
             ok &= _compare(f"{group} {obs}", mean, err, _cplx(ref["mean"]), _cplx(ref["error"]))
     prop.close()
+    if trial is not None:
+        trial.close()
     assert ok
