@@ -303,9 +303,8 @@ void Propagator::bp_propagate_refs(const WalkerState& st, int nbp, int ortho, z6
   ensure_references();
   hipStream_t s = ctx_.stream();
   const int W = st.W, N = ham_.N(), M = ham_.M(), R = nref_;
-  const bool coll = !closed_ && wdn_ > 0;
-  const int nsp = coll ? 2 : 1;
-  const int nsl[2] = {wup_, wdn_};
+  const int nsp = closed_ ? 1 : 2;
+  const int nsl[2] = {wup_, closed_ ? 0 : wdn_};
   uint32_t C = 0;  // completed steps: the newest stored field sits at slot (C - 1) % nback
   AFQ_HIP(hipMemcpyAsync(&C, step_dev_, sizeof(C), hipMemcpyDeviceToHost, s));
   AFQ_HIP(hipStreamSynchronize(s));
@@ -313,7 +312,9 @@ void Propagator::bp_propagate_refs(const WalkerState& st, int nbp, int ortho, z6
   // columns d n_s .. (d + 1) n_s - 1; the propagators act on all of them at once.  A QR step
   // needs each reference contiguous, [W][R][N][n_s] (Lp), and back.
   z64* L[2] = {nullptr, nullptr};
+  Lp[0] = Lp[1] = nullptr;
   for (int sp = 0; sp < nsp; ++sp) {
+    if (nsl[sp] == 0) continue;  // a spin without electrons (a polarized COLLINEAR set)
     const long cols = (long)R * nsl[sp];
     z64* ref = scratch_as<z64>(ctx_, sp == 0 ? "bp_ref0" : "bp_ref1", (size_t)N * cols);
     conj_transpose(s, (int)cols, N, ref_rows_[sp], ref);
@@ -333,6 +334,7 @@ void Propagator::bp_propagate_refs(const WalkerState& st, int nbp, int ortho, z6
     const bool qr = ni == 0 || ni % ortho == 0;
     for (int sp = 0; sp < nsp; ++sp) {
       const int n = nsl[sp];
+      if (n == 0) continue;
       det_ops::PropagateArgs pa{W, N, R * n, p_.order, 'H', P1_, real_P1_ ? P1r_ : nullptr,
                                 nullptr, nullptr, V, (long)N * N, L[sp], (long)N * R * n, kFP64};
       det_ops::propagate(ctx_, pa);
@@ -354,15 +356,16 @@ void Propagator::backpropagate(const WalkerState& st, int nbp, int ortho, int re
             "per-walker density matrices are defined for a single reference; use bp_measure");
   hipStream_t s = ctx_.stream();
   const int W = st.W, N = ham_.N();
-  const bool coll = !closed_ && wdn_ > 0;
-  const int nsp = coll ? 2 : 1;
-  const int nsl[2] = {wup_, wdn_};
+  const int nsp = closed_ ? 1 : 2;
+  const int nsl[2] = {wup_, closed_ ? 0 : wdn_};
   z64* Lp[2] = {nullptr, nullptr};
   z64* ldR = scratch_as<z64>(ctx_, "bp_ldR", (size_t)W);
   bp_propagate_refs(st, nbp, ortho, Lp, ldR);
   z64* anch[2] = {bp_anch_a_, bp_anch_b_};
+  if (nsl[0] == 0 || (nsp == 2 && nsl[1] == 0)) fill_z64(s, (long)W * nsp * N * N, {0, 0}, G);
   for (int sp = 0; sp < nsp; ++sp)
-    det_ops::mixed_density_matrix_left(ctx_, Lp[sp], (long)N * nsl[sp], nsl[sp], N, anch[sp],
+    if (nsl[sp] > 0)
+      det_ops::mixed_density_matrix_left(ctx_, Lp[sp], (long)N * nsl[sp], nsl[sp], N, anch[sp],
                                        (long)N * nsl[sp], W, G + (size_t)sp * N * N,
                                        (long)nsp * N * N, nullptr);
   // the factor ring holds 3 nback steps, so extra path restoration (2 nbp) always fits
@@ -378,9 +381,9 @@ void Propagator::bp_measure(const WalkerState& st, int nbp, int ortho, int resto
   ensure_references();
   hipStream_t s = ctx_.stream();
   const int W = st.W, N = ham_.N(), R = nref_;
-  const bool coll = !closed_ && wdn_ > 0;
+  const bool coll = !closed_;
   const int nsp = coll ? 2 : 1;
-  const int nsl[2] = {wup_, wdn_};
+  const int nsl[2] = {wup_, coll ? wdn_ : 0};
   z64* Lp[2] = {nullptr, nullptr};
   z64* ldR = scratch_as<z64>(ctx_, "bp_ldR", (size_t)W * R);
   bp_propagate_refs(st, nbp, ortho, Lp, ldR);
@@ -392,7 +395,8 @@ void Propagator::bp_measure(const WalkerState& st, int nbp, int ortho, int resto
   fill_z64(s, (long)R * W, {0, 0}, lo);
   for (int d = 0; d < R; ++d)
     for (int sp = 0; sp < nsp; ++sp)
-      det_ops::log_overlap_left(ctx_, Lp[sp] + (size_t)d * N * nsl[sp], (long)R * N * nsl[sp],
+      if (nsl[sp] > 0)
+        det_ops::log_overlap_left(ctx_, Lp[sp] + (size_t)d * N * nsl[sp], (long)R * N * nsl[sp],
                                 nsl[sp], N, anch[sp], (long)N * nsl[sp], W, lo + (size_t)d * W);
   reference_coefficients(s, W, R, lo, ref_cc_, ldR, cf);
   // path-restored weights (BackPropEstimator.hpp:212-228)
@@ -402,9 +406,12 @@ void Propagator::bp_measure(const WalkerState& st, int nbp, int ortho, int resto
   z64* wr = scratch_as<z64>(ctx_, "bpm_w", W);
   restore_weights(s, W, st.weight, bp_wfac_, bp_hl_, C, restore * nbp, wr);
   z64* G = scratch_as<z64>(ctx_, "bpm_G", (size_t)W * nsp * N * N);
+  // a spin without electrons keeps a zero density matrix
+  if (nsl[0] == 0 || (nsp == 2 && nsl[1] == 0)) fill_z64(s, (long)W * nsp * N * N, {0, 0}, G);
   for (int d = 0; d < R; ++d) {
     for (int sp = 0; sp < nsp; ++sp)
-      det_ops::mixed_density_matrix_left(ctx_, Lp[sp] + (size_t)d * N * nsl[sp],
+      if (nsl[sp] > 0)
+        det_ops::mixed_density_matrix_left(ctx_, Lp[sp] + (size_t)d * N * nsl[sp],
                                          (long)R * N * nsl[sp], nsl[sp], N, anch[sp],
                                          (long)N * nsl[sp], W, G + (size_t)sp * N * N,
                                          (long)nsp * N * N, nullptr);
@@ -418,9 +425,9 @@ void Propagator::mixed_measure(const WalkerState& st, z64* onerdm, z64* diag2rdm
   ensure_references();
   hipStream_t s = ctx_.stream();
   const int W = st.W, N = ham_.N(), R = nref_;
-  const bool coll = !closed_ && wdn_ > 0;
+  const bool coll = !closed_;
   const int nsp = coll ? 2 : 1;
-  const int nsl[2] = {wup_, wdn_};
+  const int nsl[2] = {wup_, coll ? wdn_ : 0};
   const z64* Phi[2] = {st.Phi, st.Phi_b};
   // reference coefficients (constructMixedMeasurementInputs, MixedEstimator.hpp:40-106):
   // conj(ci) <D_d|Phi>, normalized over the references of each walker (the reference's shift by
@@ -430,13 +437,16 @@ void Propagator::mixed_measure(const WalkerState& st, z64* onerdm, z64* diag2rdm
   fill_z64(s, (long)R * W, {0, 0}, lo);
   for (int d = 0; d < R; ++d)
     for (int sp = 0; sp < nsp; ++sp)
-      det_ops::log_overlap(ctx_, ref_rows_[sp] + (size_t)d * nsl[sp] * N, nsl[sp], N, Phi[sp],
+      if (nsl[sp] > 0)
+        det_ops::log_overlap(ctx_, ref_rows_[sp] + (size_t)d * nsl[sp] * N, nsl[sp], N, Phi[sp],
                            (long)N * nsl[sp], W, lo + (size_t)d * W);
   reference_coefficients(s, W, R, lo, ref_cc_, nullptr, cf);
   z64* G = scratch_as<z64>(ctx_, "mm_G", (size_t)W * nsp * N * N);
+  if (nsl[0] == 0 || (nsp == 2 && nsl[1] == 0)) fill_z64(s, (long)W * nsp * N * N, {0, 0}, G);
   for (int d = 0; d < R; ++d) {
     for (int sp = 0; sp < nsp; ++sp)
-      det_ops::mixed_density_matrix(ctx_, ref_rows_[sp] + (size_t)d * nsl[sp] * N, nsl[sp], N,
+      if (nsl[sp] > 0)
+        det_ops::mixed_density_matrix(ctx_, ref_rows_[sp] + (size_t)d * nsl[sp] * N, nsl[sp], N,
                                     Phi[sp], (long)N * nsl[sp], W, G + (size_t)sp * N * N,
                                     (long)nsp * N * N, nullptr, false);
     rdm_measure(s, W, nsp, N, coll, G, (long)nsp * N * N, st.weight, cf + (size_t)d * W, d > 0,

@@ -26,6 +26,7 @@ class Trace:
     components: list      # (E1, EXX, EJ) weighted averages
     total_weight: list
     eff_walkers: list
+    overlap: list = dataclasses.field(default_factory=list)  # sum_w w e^{log ovlp_w} / sum_w w
     growth: float = float("nan")   # growth-estimator energy after the last step
     eshift: float = float("nan")   # Eshift after the last step
     # MixedEstimator: observable name -> one entry per measurement
@@ -154,12 +155,16 @@ def run_afqmc(prop: "afq.Propagator", ws: "afq.WalkerSet", steps: int, equilibra
             comps = (w[:, None] * E).sum(0) / sw
             aw = w.abs()
             neff = (aw.sum() ** 2 / (aw ** 2).sum())
-            vals = afq.host(torch.cat([comps, sw.reshape(1), neff.reshape(1).to(torch.complex128)]))
+            # EnergyEstimator.h:86-87: the weighted average of the walkers' overlaps
+            ovl = (w * torch.exp(ov)).sum() / sw
+            vals = afq.host(torch.cat([comps, sw.reshape(1), neff.reshape(1).to(torch.complex128),
+                                       ovl.reshape(1)]))
             tr.steps.append(step)
             tr.components.append(vals[:3])
             tr.energy.append(complex(vals[:3].sum()))
             tr.total_weight.append(complex(vals[3]))
             tr.eff_walkers.append(float(vals[4].real))
+            tr.overlap.append(complex(vals[5]))
         if mixed is not None and step >= equilibration and step % mixed_interval == 0:
             one, two = prop.mixed_measure(ws, "onerdm" in mixed, "diag_twordm" in mixed)
             _record(tr.mixed, _observables(one, two, wt))

@@ -1,11 +1,14 @@
-"""Copies the BH (boron hydride, 19 orbitals, 78 Cholesky vectors) input files of the reference's
-functional tests into tests/golden/bh/ and extracts its statistical reference results
-(tests/functional/statistical_references/BH/dense_rhf_basis_*/<trial>/<walkers>/results.h5: the
-run parameters from `input_file`, mean / error of Energy and its components, and of the mixed and
-back-propagated OneRDM and DiagonalTwoRDM) with h5dump.
+"""Copies the input files of the reference's functional tests for three molecules into
+tests/golden/<molecule>/ and extracts their statistical reference results
+(tests/functional/statistical_references/<molecule>/<hamiltonian>/<trial>/<walkers>/results.h5:
+the run parameters from `input_file`, mean / error of Energy, its components and Overlap, and of
+the mixed and back-propagated OneRDM and DiagonalTwoRDM) with h5dump:
+  BH  19 orbitals, 78 Cholesky vectors, RHF / UHF / CASCI-PHMSD trials
+  N2  28 orbitals, 134 Cholesky vectors, a 50-determinant CAS trial (PHMSD)
+  Li  14 orbitals, 70 Cholesky vectors, a fully polarized ROHF trial (3 + 0 electrons)
 Run in the container that holds the reference; the GPU box only reads the copies.
 
-    python tests/golden/make_bh_fixtures.py
+    python tests/golden/make_molecule_fixtures.py
 """
 import json
 import os
@@ -16,14 +19,25 @@ import subprocess
 REF = "/root/reference/tests/functional"
 H5DUMP = "/opt/conda/bin/h5dump"
 HERE = os.path.dirname(os.path.abspath(__file__))
-OUT = os.path.join(HERE, "bh")
-INPUTS = ["afqmc_H_rhf_closed.h5", "afqmc_H_rhf_collinear.h5", "afqmc_rhf_nomsd.h5",
-          "afqmc_uhf_nomsd.h5", "afqmc_casci_rhf_phmsd.h5", "afqmc_casci_uhf_phmsd.h5"]
-CASES = [("dense_rhf_basis_closed", "rhf_nomsd", "closed"),
-         ("dense_rhf_basis_closed", "rhf_nomsd", "collinear"),
-         ("dense_rhf_basis_closed", "uhf_nomsd", "collinear"),
-         ("dense_rhf_basis_closed", "rcasci_rhf_phmsd", "collinear"),
-         ("dense_rhf_basis_closed", "rcasci_uhf_phmsd", "collinear")]
+MOLECULES = {
+    "BH": {
+        "inputs": ["afqmc_H_rhf_closed.h5", "afqmc_H_rhf_collinear.h5", "afqmc_rhf_nomsd.h5",
+                   "afqmc_uhf_nomsd.h5", "afqmc_casci_rhf_phmsd.h5", "afqmc_casci_uhf_phmsd.h5"],
+        "cases": [("dense_rhf_basis_closed", "rhf_nomsd", "closed"),
+                  ("dense_rhf_basis_closed", "rhf_nomsd", "collinear"),
+                  ("dense_rhf_basis_closed", "uhf_nomsd", "collinear"),
+                  ("dense_rhf_basis_closed", "rcasci_rhf_phmsd", "collinear"),
+                  ("dense_rhf_basis_closed", "rcasci_uhf_phmsd", "collinear")],
+    },
+    "N2": {
+        "inputs": ["cas_basis_hamil.h5", "cas_wfn.h5"],
+        "cases": [("cas_basis_hamil_closed", "cas_wfn_collinear", "collinear")],
+    },
+    "Li": {
+        "inputs": ["hamil_closed.h5", "rohf_nomsd_polarized.h5"],
+        "cases": [("hamil_closed", "rohf_wfn_polarized", "collinear")],
+    },
+}
 
 
 def dump_value(path, dataset):
@@ -55,20 +69,22 @@ def dump_array(path, dataset):
     return {"shape": shape, "re": nums[0::2], "im": nums[1::2]}
 
 
-def main():
-    os.makedirs(OUT, exist_ok=True)
-    for f in INPUTS:
-        shutil.copyfile(os.path.join(REF, "afqmc_inputs", "BH", f), os.path.join(OUT, f))
+def extract(molecule, spec):
+    out = os.path.join(HERE, molecule.lower())
+    os.makedirs(out, exist_ok=True)
+    for f in spec["inputs"]:
+        shutil.copyfile(os.path.join(REF, "afqmc_inputs", molecule, f), os.path.join(out, f))
     refs = {}
-    for basis, trial, walkers in CASES:
-        path = os.path.join(REF, "statistical_references", "BH", basis, trial, walkers, "results.h5")
+    for basis, trial, walkers in spec["cases"]:
+        path = os.path.join(REF, "statistical_references", molecule, basis, trial, walkers,
+                            "results.h5")
         if not os.path.exists(path):
             continue
         inp = json.loads(dump_string(path, "/input_file"))
         ex = inp["execute"]
         nranks = int(dump_value(path, "/num_ranks")[0])
         obs = {}
-        for name in ("Energy", "OnebodyEnergy", "CoulombEnergy", "ExchangeEnergy"):
+        for name in ("Energy", "OnebodyEnergy", "CoulombEnergy", "ExchangeEnergy", "Overlap"):
             mean = dump_value(path, f"/measurements/Stage0/{name}/mean")
             err = dump_value(path, f"/measurements/Stage0/{name}/error")
             obs[name] = {"mean": mean[:2], "error": err[:2]}
@@ -82,11 +98,11 @@ def main():
             for L in est["backprop"]["propagation_steps"]:
                 groups.append((f"backprop/{L}", f"BackPropEstimator/Steps={L}"))
         for key, grp in groups:
-            for obs in ("OneRDM", "DiagonalTwoRDM"):
+            for o in ("OneRDM", "DiagonalTwoRDM"):
                 try:
-                    rdm.setdefault(key, {})[obs] = {
-                        "mean": dump_array(path, f"/measurements/Stage0/{grp}/{obs}/mean"),
-                        "error": dump_array(path, f"/measurements/Stage0/{grp}/{obs}/error")}
+                    rdm.setdefault(key, {})[o] = {
+                        "mean": dump_array(path, f"/measurements/Stage0/{grp}/{o}/mean"),
+                        "error": dump_array(path, f"/measurements/Stage0/{grp}/{o}/error")}
                 except subprocess.CalledProcessError:
                     pass
         refs[f"{trial}/{walkers}"] = {
@@ -105,9 +121,16 @@ def main():
             "seed": inp.get("seed"),
             "observables": obs,
         }
-    with open(os.path.join(OUT, "statistical_refs.json"), "w") as f:
+    with open(os.path.join(out, "statistical_refs.json"), "w") as f:
         json.dump(refs, f, sort_keys=True)
-    print(json.dumps(refs, indent=1)[:2000])
+    return refs
+
+
+def main():
+    for molecule, spec in MOLECULES.items():
+        refs = extract(molecule, spec)
+        for k, v in refs.items():
+            print(molecule, k, v["walkers"], v["steps"], {o: d["mean"] for o, d in v["observables"].items()})
 
 
 if __name__ == "__main__":

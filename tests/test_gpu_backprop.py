@@ -13,11 +13,11 @@ one-body density-matrix estimators.
 - Population control moves the rings and the anchors with their walkers.
 - afq_rdm_measure (OneRDM and DiagonalTwoRDM of one measurement) equals the oracle's
   walker-by-walker sums (1e-12), with reference coefficients and accumulation.
-- BH (the reference's inputs): the mixed and back-propagated OneRDM and DiagonalTwoRDM of a
-  seeded run agree with the reference's statistical results element by element under its
-  runner's criterion (run_functional.py:385-410, Bonferroni over the elements).
+- Multi-reference (PHMSD) mixed and back-propagated measurements equal the oracle's sums over
+  determinants.
+The statistical comparisons of the estimators with the reference's runs are in
+test_gpu_molecules.py.
 """
-import json
 import os
 import sys
 
@@ -30,7 +30,6 @@ import oracle as orc  # noqa: E402
 from safire_amd.synthetic import make_problem  # noqa: E402
 
 TOL = 1e-10
-G_BH = os.path.join(os.path.dirname(__file__), "golden", "bh")
 
 
 def say(*a):
@@ -327,22 +326,6 @@ def test_population_control_moves_backprop_rows(gpu):
     prop.close()
 
 
-def _cplx(d):
-    return (np.asarray(d["re"]) + 1j * np.asarray(d["im"])).reshape(d["shape"])
-
-
-def _compare(name, A, Aerr, B, Berr):
-    """run_functional.py:385-410 on one observable tensor."""
-    from scipy.stats import norm
-    z = norm.ppf(1 - 0.001 / (2 * A.size))
-    tol = np.maximum(z * np.sqrt(np.abs(Aerr) ** 2 + np.abs(Berr) ** 2), 1e-9 * (1 + np.abs(B)))
-    d = np.abs(A - B)
-    worst = np.unravel_index(np.argmax(d / tol), d.shape)
-    say(f"  {name}: worst |d| = {d[worst]:.3e}, tolerance {tol[worst]:.3e} at {worst} "
-        f"({A[worst]:.6f} +- {abs(Aerr[worst]):.6f} vs {B[worst]:.6f} +- {abs(Berr[worst]):.6f})")
-    return bool(np.all(d <= tol))
-
-
 @pytest.mark.parametrize("closed", [True, False])
 def test_rdm_measure_vs_oracle(gpu, closed):
     """afq_rdm_measure against the oracle's walker-by-walker sums, with coefficients and with
@@ -368,60 +351,3 @@ def test_rdm_measure_vs_oracle(gpu, closed):
     e3, e4 = rel(afq.host(one), q1 + s1), rel(afq.host(two), q2 + s2)
     say(f"  onerdm {e1:.2e} diag2rdm {e2:.2e}; two references {e3:.2e} {e4:.2e}")
     assert max(e1, e2, e3, e4) < 1e-12
-
-
-@pytest.mark.parametrize("case", ["rhf_nomsd/closed", "uhf_nomsd/collinear",
-                                  "rcasci_rhf_phmsd/collinear", "rcasci_uhf_phmsd/collinear"])
-def test_bh_rdm_statistical_reference(gpu, case):
-    """Seeded BH runs at the reference's parameters (1536 walkers, dt 0.01, 3000 equilibration
-    steps, 9000 measured steps against the reference's 45000): the mixed and back-propagated
-    (100 steps, path restoration) OneRDM and DiagonalTwoRDM agree with the reference's
-    statistical results element by element; the PHMSD trials sum over their 44 (RHF) and
-    UHF-CASCI determinants as references."""
-    afq, ctx0, torch = gpu
-    from safire_amd import driver, h5inputs
-    from safire_amd.phmsd import PHMSD
-    refs = json.load(open(os.path.join(G_BH, "statistical_refs.json")))
-    r = refs[case]
-    say(f"case: BH {case} mixed and back-propagated RDMs vs the reference's statistical results")
-    wt = afq.AFQ_CLOSED if r["walker_type"] == "CLOSED" else afq.AFQ_COLLINEAR
-    s = h5inputs.load_system(os.path.join(G_BH, r["hamiltonian"]), os.path.join(G_BH, r["wavefunction"]),
-                             walker_type=wt)
-    W = r["walkers"]
-    ctx = afq.Context(0)
-    trial = None
-    if s.ph is None:
-        ham = afq.HamiltonianOperations(ctx, s.h1, s.Likn, s.PsiT, s.nup, s.ndown, walker_type=wt,
-                                        E0=s.E0)
-        Pa, Pb = s.walkers(W)
-    else:
-        ham = afq.HamiltonianOperations(ctx, s.h1, s.Likn, s.PsiT, s.nact, s.nact,
-                                        walker_type=afq.AFQ_COLLINEAR, E0=s.E0)
-        trial = PHMSD(ham, s.ph)
-        up = np.ascontiguousarray(s.PsiT[:s.nup].conj().T)
-        Pa = np.broadcast_to(up, (W,) + up.shape).copy()
-        Pb = Pa.copy()
-    prop = afq.Propagator(ham, W, dt=r["timestep"], seed=42, trial=trial)
-    say(f"  {prop.number_of_references()} reference determinants")
-    ws = afq.WalkerSet(Pa, Pb)
-    eq = r["equilibration_steps"]
-    tr = driver.run_afqmc(prop, ws, steps=eq + 9000, equilibration=eq,
-                          ortho_interval=r["walker_ortho_interval"],
-                          popctl_interval=r["population_control_interval"],
-                          measure_interval=r["measure_interval"],
-                          estimators=r["estimators"],
-                          progress=lambda st, t: say(f"  step {st}"), progress_interval=3000)
-    ok = True
-    L = r["estimators"]["backprop"]["propagation_steps"][-1]
-    for group, series in (("mixed", tr.mixed), (f"backprop/{L}", tr.backprop[L])):
-        for obs in ("OneRDM", "DiagonalTwoRDM"):
-            mean, err = driver.mean_error(series[obs])
-            ref = r["rdm"][group][obs]
-            if obs == "OneRDM":
-                tr_ = [round(float(np.trace(mean[k]).real), 6) for k in range(mean.shape[0])]
-                say(f"  {group} {obs}: {len(series[obs])} measurements, trace per spin {tr_}")
-            ok &= _compare(f"{group} {obs}", mean, err, _cplx(ref["mean"]), _cplx(ref["error"]))
-    prop.close()
-    if trial is not None:
-        trial.close()
-    assert ok

@@ -1,17 +1,23 @@
-"""The device path on a real molecule read from the reference's own inputs: BH (19 orbitals,
-78 Cholesky vectors, 3+3 electrons; tests/golden/bh, copied from the reference's functional
-test inputs).
+"""The device path on real molecules read from the reference's own inputs (tests/golden/<molecule>,
+copied from the reference's functional test inputs by tests/golden/make_molecule_fixtures.py):
+BH (19 orbitals, 78 Cholesky vectors, 3+3 electrons), N2 (28 orbitals, 134 vectors, 7+7
+electrons, a 50-determinant CAS trial) and Li (14 orbitals, 70 vectors, a fully polarized ROHF
+trial with 3+0 electrons).
 
-- Deterministic: the local energy at the RHF trial equals the closed-form Hartree-Fock energy
+- Deterministic: the local energy at the BH RHF trial equals the closed-form Hartree-Fock energy
   of the integrals (1e-10), and the PHMSD (CASCI, 44 determinants) overlap, density matrix and
   local energy of perturbed walkers equal the explicit sum over determinants of the oracle.
 - Statistical: seeded hybrid AFQMC runs at the reference's parameters (dt = 0.01, 1536 walkers,
   3000 equilibration steps, measurements, population control and orthogonalization every 10
-  steps) agree with the reference's statistical results
-  (tests/functional/statistical_references/BH/..., extracted into statistical_refs.json) by the
-  reference runner's criterion (run_functional.py:385-410): |E - E_ref| <= z sqrt(s^2 + s_ref^2)
-  with z the two-sided 0.001 quantile (3.29), s the blocking error.  The runs are shorter than
-  the reference's (9000 measured steps instead of 45000), so s is the larger error.
+  steps, the reference's estimator blocks) agree with the reference's statistical results
+  (tests/functional/statistical_references/<molecule>/..., extracted into
+  statistical_refs.json) observable by observable (Energy, its three components, Overlap, and the
+  mixed and back-propagated OneRDM and DiagonalTwoRDM where the reference measured them) by
+  the reference runner's criterion (run_functional.py:385-410): |A - B| <= z sqrt(s^2 + s_ref^2)
+  element-wise, z the two-sided 0.001 quantile Bonferroni-adjusted over the elements (3.29 for
+  a scalar), s the blocking error, with the 1e-9 (1 + |B|) floor for elements without
+  stochastic error.  The runs are shorter than the reference's (9000 measured steps instead of
+  9000-45000), so s is usually the larger error.
 """
 import json
 import os
@@ -25,8 +31,8 @@ pytestmark = pytest.mark.gpu
 import oracle as orc  # noqa: E402
 from safire_amd import h5inputs  # noqa: E402
 
-G = os.path.join(os.path.dirname(__file__), "golden", "bh")
-Z_CRIT = 3.2905267314919255   # scipy.stats.norm.ppf(1 - 0.001 / 2)
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+G = os.path.join(GOLD, "bh")
 
 
 def say(*a):
@@ -100,19 +106,22 @@ def test_bh_phmsd_vs_explicit_sum(gpu):
     assert np.abs(afq.host(ov2) - afq.host(ov)).max() < 1e-12
 
 
-CASES = [("rhf_nomsd/closed", "fp64"), ("rhf_nomsd/closed", "mixed"),
-         ("uhf_nomsd/collinear", "fp64"), ("rcasci_rhf_phmsd/collinear", "fp64"),
-         ("rcasci_uhf_phmsd/collinear", "fp64")]
+CASES = [("BH", "rhf_nomsd/closed", "fp64"), ("BH", "rhf_nomsd/closed", "mixed"),
+         ("BH", "uhf_nomsd/collinear", "fp64"), ("BH", "rcasci_rhf_phmsd/collinear", "fp64"),
+         ("BH", "rcasci_uhf_phmsd/collinear", "fp64"), ("N2", "cas_wfn_collinear/collinear", "fp64"),
+         ("Li", "rohf_wfn_polarized/collinear", "fp64")]
+SCALARS = ("Energy", "OnebodyEnergy", "ExchangeEnergy", "CoulombEnergy", "Overlap")
 
 
-def run_case(afq, ctx, case, prec, measured_steps):
+def run_case(afq, ctx, molecule, case, prec, measured_steps):
     from safire_amd import driver
     from safire_amd.phmsd import PHMSD
-    refs = json.load(open(os.path.join(G, "statistical_refs.json")))
+    gdir = os.path.join(GOLD, molecule.lower())
+    refs = json.load(open(os.path.join(gdir, "statistical_refs.json")))
     r = refs[case]
     wt = afq.AFQ_CLOSED if r["walker_type"] == "CLOSED" else afq.AFQ_COLLINEAR
-    s = h5inputs.load_system(os.path.join(G, r["hamiltonian"]), os.path.join(G, r["wavefunction"]),
-                             walker_type=wt)
+    s = h5inputs.load_system(os.path.join(gdir, r["hamiltonian"]),
+                             os.path.join(gdir, r["wavefunction"]), walker_type=wt)
     W = r["walkers"]
     p = afq.AFQ_MIXED if prec == "mixed" else afq.AFQ_FP64
     trial = None
@@ -128,35 +137,62 @@ def run_case(afq, ctx, case, prec, measured_steps):
         Pa = np.broadcast_to(up, (W,) + up.shape).copy()
         Pb = Pa.copy()
     prop = afq.Propagator(ham, W, dt=r["timestep"], seed=42, trial=trial)
+    say(f"  N={s.h1.shape[0]} M={s.Likn.shape[2]} electrons {s.nup}+{s.ndown}, "
+        f"{prop.number_of_references()} reference determinant(s), {W} walkers")
     ws = afq.WalkerSet(Pa, Pb)
     eq = r["equilibration_steps"]
     tr = driver.run_afqmc(prop, ws, steps=eq + measured_steps, equilibration=eq,
                           ortho_interval=r["walker_ortho_interval"],
                           popctl_interval=r["population_control_interval"],
-                          measure_interval=r["measure_interval"],
+                          measure_interval=r["measure_interval"], estimators=r["estimators"],
                           progress=lambda st, t: say(f"  step {st}"), progress_interval=3000)
-    e = np.asarray(tr.energy)
-    mean = complex(e.mean())
-    err = complex(driver.blocking_error(e.real), driver.blocking_error(e.imag))
     prop.close()
     if trial is not None:
         trial.close()
-    return r, mean, err, np.real(np.mean(np.asarray(tr.components), axis=0))
+    return r, tr
 
 
-@pytest.mark.parametrize("case,prec", CASES)
-def test_bh_statistical_reference(gpu, case, prec):
+def _cplx(d):
+    return (np.asarray(d["re"]) + 1j * np.asarray(d["im"])).reshape(d["shape"])
+
+
+def compare(name, A, Aerr, B, Berr):
+    """run_functional.py:385-410 on one observable (any rank, a scalar being the one-element
+    case)."""
+    from scipy.stats import norm
+    A, Aerr = np.atleast_1d(A), np.abs(np.atleast_1d(Aerr))
+    B, Berr = np.atleast_1d(B), np.abs(np.atleast_1d(Berr))
+    z = norm.ppf(1 - 0.001 / (2 * A.size))
+    tol = np.maximum(z * np.sqrt(Aerr ** 2 + Berr ** 2), 1e-9 * (1 + np.abs(B)))
+    d = np.abs(A - B)
+    worst = np.unravel_index(np.argmax(d / tol), d.shape)
+    ok = bool(d[worst] <= tol[worst])
+    say(f"  {'ok      ' if ok else 'MISMATCH'} {name}: |d| = {d[worst]:.3e}, tolerance {tol[worst]:.3e}"
+        f" (z {z:.2f}): {A[worst]:.6f} +- {Aerr[worst]:.6f} vs {B[worst]:.6f} +- {Berr[worst]:.6f}"
+        f" at {tuple(int(x) for x in worst)}")
+    return ok
+
+
+@pytest.mark.parametrize("molecule,case,prec", CASES)
+def test_statistical_reference(gpu, molecule, case, prec):
     afq, ctx, torch = gpu
-    say(f"case: BH statistical {case} {prec}")
-    r, mean, err, comps = run_case(afq, ctx, case, prec, measured_steps=9000)
-    ref = r["observables"]["Energy"]
-    rmean = complex(*ref["mean"])
-    rerr = complex(*ref["error"])
-    d = abs(mean - rmean)
-    tol = Z_CRIT * np.hypot(abs(err), abs(rerr))
-    say(f"  E = {mean.real:.6f} +- {err.real:.6f}   reference {rmean.real:.6f} +- {rerr.real:.6f}"
-        f"   |dE| = {d:.2e}, tolerance {tol:.2e}")
-    say(f"  (E1, EX, EJ) = {comps}   reference "
-        f"{[r['observables'][k]['mean'][0] for k in ('OnebodyEnergy', 'ExchangeEnergy', 'CoulombEnergy')]}")
-    assert np.isfinite(mean) and err.real > 0
-    assert d <= tol
+    from safire_amd import driver
+    say(f"case: {molecule} statistical {case} {prec}")
+    r, tr = run_case(afq, ctx, molecule, case, prec, measured_steps=9000)
+    series = {"Energy": tr.energy, "OnebodyEnergy": [c[0] for c in tr.components],
+              "ExchangeEnergy": [c[1] for c in tr.components],
+              "CoulombEnergy": [c[2] for c in tr.components], "Overlap": tr.overlap}
+    ok = True
+    for name in SCALARS:
+        mean, err = driver.mean_error(np.asarray(series[name], np.complex128)[:, None])
+        ref = r["observables"][name]
+        ok &= compare(name, mean[0], err[0], complex(*ref["mean"]), complex(*ref["error"]))
+    e = np.asarray(tr.energy)
+    assert np.all(np.isfinite(e)) and driver.blocking_error(e.real) > 0
+    for group, obs in r["rdm"].items():
+        store = tr.mixed if group == "mixed" else tr.backprop[int(group.split("/")[1])]
+        for name, ref in obs.items():
+            mean, err = driver.mean_error(store[name])
+            ok &= compare(f"{group} {name} ({len(store[name])} measurements)", mean, err,
+                          _cplx(ref["mean"]), _cplx(ref["error"]))
+    assert ok

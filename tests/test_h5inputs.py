@@ -1,5 +1,5 @@
 """The HDF5 input readers (include/afq_h5.h, safire_amd/h5inputs.py) on the reference's BH
-fixtures (tests/golden/bh, copied by tests/golden/make_bh_fixtures.py): extents, the energy
+fixtures (tests/golden/bh, copied by tests/golden/make_molecule_fixtures.py): extents, the energy
 offset, orthonormal trials, symmetric integrals, the PHMSD active space and excitation
 structure, and the oracle's local energy at the RHF trial against the closed-form Hartree-Fock
 energy of the same integrals."""
