@@ -7,7 +7,7 @@ hamiltonian (safire_amd.synthetic, seed 1003).  A step is one hybrid propagation
 relaxation, with the QR every fifth step, replayed from a HIP graph.  `value` is the mixed
 precision path (FP32 MFMA contractions, FP64 G/LU/QR/weights); the FP64 path is timed beside it.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu] [--only vhs|copy]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu] [--only MODE]
 """
 from __future__ import annotations
 
@@ -140,6 +140,70 @@ def bh_mixed_error(W_e=256, steps=10):
             "mean_energy_diff_Ha": float(abs(res[afq.AFQ_MIXED].mean() - res[afq.AFQ_FP64].mean()))}
 
 
+def estimator_bench(reps=3):
+    """Cost of one back-propagated and one mixed measurement (OneRDM + DiagonalTwoRDM) on BH read
+    from the reference's inputs at its run parameters (1536 walkers, dt 0.01, 100-step
+    back-propagation with QR every 10 steps): the RHF trial (one reference) and the CASCI
+    PHMSD trial (44 references)."""
+    import torch
+    from safire_amd import afq, h5inputs
+    from safire_amd.phmsd import PHMSD
+    g = os.path.join(ROOT, "tests", "golden", "bh")
+    W, L = 1536, 100
+    res = {}
+    for name, wfn in (("rhf", "afqmc_rhf_nomsd.h5"), ("casci_phmsd", "afqmc_casci_rhf_phmsd.h5")):
+        s = h5inputs.load_system(os.path.join(g, "afqmc_H_rhf_closed.h5"), os.path.join(g, wfn),
+                                 walker_type=None)
+        ctx = afq.Context(0)
+        trial = None
+        if s.ph is None:
+            ham = afq.HamiltonianOperations(ctx, s.h1, s.Likn, s.PsiT, s.nup, s.ndown,
+                                            walker_type=s.walker_type, E0=s.E0)
+            ws = afq.WalkerSet(*s.walkers(W))
+        else:
+            ham = afq.HamiltonianOperations(ctx, s.h1, s.Likn, s.PsiT, s.nact, s.nact,
+                                            walker_type=afq.AFQ_COLLINEAR, E0=s.E0)
+            trial = PHMSD(ham, s.ph)
+            up = np.ascontiguousarray(s.PsiT[:s.nup].conj().T)
+            Pa = np.broadcast_to(up, (W,) + up.shape).copy()
+            ws = afq.WalkerSet(Pa, Pa.copy())
+        prop = afq.Propagator(ham, W, dt=0.01, seed=42, trial=trial)
+        prop.enable_backprop(L)
+        E = afq.empty((W, 3), torch.complex128)
+        ov = afq.empty(W, torch.complex128)
+        prop.Energy(ws, E, ov)
+        ws.eloc.copy_(E.sum(1))
+        ws.ovlp.copy_(ov)
+        Esh = afq.device_tensor(np.array([float(afq.host(E.sum(1)).real.mean())]))
+        prop.bp_anchor(ws)
+        t0 = time.perf_counter()
+        for k in range(L):
+            prop.step_graph(ws, Esh, ortho=(k % 10 == 0), popctl=(k % 10 == 0))
+        torch.cuda.synchronize()
+        t_step = (time.perf_counter() - t0) / L
+        prop.bp_measure(ws, L, 10, 1)
+        prop.mixed_measure(ws)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            prop.bp_measure(ws, L, 10, 1)
+        torch.cuda.synchronize()
+        t_bp = (time.perf_counter() - t0) / reps
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            prop.mixed_measure(ws)
+        torch.cuda.synchronize()
+        t_mx = (time.perf_counter() - t0) / reps
+        res[name] = {"references": prop.number_of_references(), "walkers": W,
+                     "step_ms": t_step * 1e3, "backprop_100_ms": t_bp * 1e3,
+                     "mixed_ms": t_mx * 1e3,
+                     "backprop_share_at_one_per_100_steps": t_bp / (t_bp + 100 * t_step)}
+        prop.close()
+        if trial is not None:
+            trial.close()
+    return res
+
+
 def energy_error(problem, W_e=64):
     """Per-walker local energy of the mixed path vs the FP64 path after one step from the same
     state and the same field draws (energies evaluated in FP64)."""
@@ -231,7 +295,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=6)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
-    ap.add_argument("--only", choices=["vhs", "vhs64", "copy", "step", "step64", "phmsd", "vendor"],
+    ap.add_argument("--only", choices=["vhs", "vhs64", "copy", "step", "step64", "phmsd", "vendor",
+                                       "estimators"],
                     default=None,
                     help="profiling modes: launch only the named kernel(s)")
     ap.add_argument("--serve-jobs", type=int, default=24,
@@ -256,6 +321,10 @@ def main():
                                         precision=prec)
         t = time_vhs(ham, prec, W, reps=args.steps)
         log(f"vHS launch {t * 1e3:.3f} ms")
+        return
+    if args.only == "estimators":
+        log("back-propagation and mixed estimators on BH")
+        log(json.dumps(estimator_bench()))
         return
     if args.only == "phmsd":
         log(f"PHMSD workload: {args.steps} steps after {args.warmup} warmup")

@@ -67,13 +67,7 @@ void population_control(hipStream_t s, int W, const double* r_dev, uint64_t seed
 void average_pseudo_energy(hipStream_t s, int W, const z64* weight, const z64* eloc, double dt,
                            double relax, double* out, double* Eshift);
 
-// ---- estimators.hip --------------------------------------------------------------------------
-// one [nsp][N][N] = sum_w c_w G_w / sum_w weight_w and two [2 or 3][N][N] = the diagonal two-body
-// blocks (aaaa), (aabb)[, (bbbb)] with c_w = weight_w * coeff_w (coeff may be null: 1); either
-// output may be null; accumulate adds to the outputs instead.  G [W] blocks of g_bs elements,
-// each [nsp][N][N].
-void rdm_measure(hipStream_t s, int W, int nsp, int N, bool collinear, const z64* G, long g_bs,
-                 const z64* weight, const z64* coeff, bool accumulate, z64* one, z64* two);
+// ---- estimators.hip (rdm_measure, which needs the context's scratch, is declared in ops.hpp) ---
 // Reference coefficients of a multi-reference measurement (MixedEstimator.hpp:61-103,
 // BackPropEstimator.hpp:62-120): x[d][w] = cc[d] exp(lo[d][w] + conj(ad[w][d])) (ad may be
 // null), coeff[d][w] = x[d][w] / sum_d' x[d'][w].  lo [R][W] log overlaps, cc [R] conj(ci).

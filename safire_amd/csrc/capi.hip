@@ -466,7 +466,7 @@ int afq_rdm_measure(afq_ctx* ctx, int walker_type, int N, int W, const double* G
     REQUIRE(walker_type == AFQ_CLOSED || walker_type == AFQ_COLLINEAR, "walker type");
     REQUIRE(N > 0 && W > 0, "invalid sizes");
     const int nsp = walker_type == AFQ_COLLINEAR ? 2 : 1;
-    afq::rdm_measure(ctx->c.stream(), W, nsp, N, nsp == 2, Z(G), (long)nsp * N * N, Z(weights),
+    afq::rdm_measure(ctx->c, W, nsp, N, nsp == 2, Z(G), (long)nsp * N * N, Z(weights),
                      coeff ? Z(coeff) : nullptr, accumulate != 0, onerdm ? Z(onerdm) : nullptr,
                      diag2rdm ? Z(diag2rdm) : nullptr);
   });

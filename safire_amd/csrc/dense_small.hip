@@ -325,7 +325,13 @@ void chol_upper_inverse(hipStream_t s, const z64* S, int nb, int n, z64* Rinv, z
   hipLaunchKernelGGL(k_chol_blk, dim3(nb), dim3(kLuThreads), lds, s, S, n, Rinv);
   AFQ_LAUNCH_CHECK();
   const long nn = (long)n * n;
-  if (n <= 32)
+  if (n <= 8)
+    hipLaunchKernelGGL(k_lu_rows<8>, dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet, 1.0,
+                       0.0, 0.0, 1, 1);
+  else if (n <= 16)
+    hipLaunchKernelGGL(k_lu_rows<16>, dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet, 1.0,
+                       0.0, 0.0, 1, 1);
+  else if (n <= 32)
     hipLaunchKernelGGL(k_lu_rows<32>, dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet, 1.0,
                        0.0, 0.0, 1, 1);
   else if (n <= 48)

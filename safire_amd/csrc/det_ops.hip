@@ -144,6 +144,37 @@ void mixed_density_matrix_left(Context& c, const z64* L, long l_bs, int n, int N
   }
 }
 
+void mixed_density_matrix_batched(Context& c, const z64* L, long l_bs, int ldiv, int n, int N,
+                                  const z64* Phi, long phi_bs, int W, int B, z64* G, long g_bs,
+                                  z64* ovlp) {
+  using gemm::OpndB;
+  if (B == 0) return;
+  AFQ_CHECK(n > 0 && n <= N && W > 0 && ldiv > 0 && (!G || g_bs >= (long)N * N), kErrShape,
+            "batched density matrix shape");
+  hipStream_t s = c.stream();
+  const int big = 1 << 30;
+  z64* T = scratch_as<z64>(c, "mdb_T", (size_t)B * n * n);
+  {  // T[a][b] = sum_k conj(L[k][a]) Phi[k][b]
+    OpndB<double, true, false, true> A{{&L->re, n, l_bs}, ldiv, big};
+    OpndB<double, true, false> Bp{{&Phi->re, n, phi_bs}, 1, W};
+    gemm_small<double>(c, "mdb_T", B, n, n, N, A, Bp, EpiStore<double, double>{T, n, (long)n * n, 1, 0, 0});
+  }
+  lu_inverse_logdet(s, T, (long)n * n, B, n, G ? T : nullptr, (long)n * n, ovlp, 1.0, {0, 0},
+                    G != nullptr);
+  if (!G) return;
+  z64* Gc = scratch_as<z64>(c, "mdb_Gc", (size_t)B * n * N);
+  {  // Gc[a][k] = sum_b Tinv[b][a] Phi[k][b]
+    Opnd<double, true, false> A{&T->re, n, (long)n * n};
+    OpndB<double, true, true> Bp{{&Phi->re, n, phi_bs}, 1, W};
+    gemm_small<double>(c, "mdb_G", B, n, N, n, A, Bp, EpiStore<double, double>{Gc, N, (long)n * N, 1, 0, 0});
+  }
+  {  // G[i][k] = sum_a conj(L[i][a]) Gc[a][k]
+    OpndB<double, true, true, true> A{{&L->re, n, l_bs}, ldiv, big};
+    Opnd<double, true, false> Bp{&Gc->re, N, (long)n * N};
+    gemm_small<double>(c, "mdb_full", B, N, N, n, A, Bp, EpiStore<double, double>{G, N, g_bs, 1, 0, 0});
+  }
+}
+
 void log_overlap_left(Context& c, const z64* L, long l_bs, int n, int N, const z64* Phi,
                       long phi_bs, int W, z64* ovlp) {
   if (n == 0 || W == 0) return;
@@ -242,8 +273,9 @@ __device__ inline void read_bfrag(T (&br)[4], T (&bi)[4], const T* Br, const T* 
   }
 }
 
-// One 16-wide k group of acc[s][t] += A[stripe s rows][k group] * B[k group][column tile t].
-template <typename T, bool CPLX, int SPW, int CB>
+// One 16-wide k group of acc[s][t] += op(A)[stripe s rows][k group] * B[k group][column tile t],
+// op(A) = conj(A) when CONJ.
+template <typename T, bool CPLX, int SPW, int CB, bool CONJ = false>
 __device__ inline void group_mfma(const AFrag<T, CPLX> (&f)[SPW], int g, int NS, int NW, int wave,
                                   int fr, int fk, const T* Br, const T* Bi, int LDK,
                                   typename gemm::Acc<T>::v4 (&ar)[SPW][CB],
@@ -260,9 +292,12 @@ __device__ inline void group_mfma(const AFrag<T, CPLX> (&f)[SPW], int g, int NS,
       for (int j = 0; j < 4; ++j) {
         ar[s][t] = Acc<T>::mfma(f[s].re[j], br[j], ar[s][t]);
         ai[s][t] = Acc<T>::mfma(f[s].re[j], bi[j], ai[s][t]);
-        if constexpr (CPLX) {
+        if constexpr (CPLX && !CONJ) {
           ar[s][t] = Acc<T>::mfma(-f[s].im[j], bi[j], ar[s][t]);
           ai[s][t] = Acc<T>::mfma(f[s].im[j], br[j], ai[s][t]);
+        } else if constexpr (CPLX) {
+          ar[s][t] = Acc<T>::mfma(f[s].im[j], bi[j], ar[s][t]);
+          ai[s][t] = Acc<T>::mfma(-f[s].im[j], br[j], ai[s][t]);
         }
       }
     }
@@ -280,7 +315,7 @@ __device__ inline void load_group(AFrag<T, CPLX> (&f)[SPW], const T* __restrict_
 
 // acc = A[stripe rows][0:N] * B[0:N][tiles] for the wave's stripes (complex B in LDS). The A
 // fragments of the next two k groups are in flight while a group is multiplied.
-template <typename T, bool CPLX, int SPW, int CB>
+template <typename T, bool CPLX, int SPW, int CB, bool CONJ = false>
 __device__ inline void stripes_times_b(const T* __restrict__ A, long lda, bool vec, int N, int NS,
                                        int NW, int wave, int fr, int fk, const T* Br,
                                        const T* Bi, int LDK,
@@ -299,13 +334,13 @@ __device__ inline void stripes_times_b(const T* __restrict__ A, long lda, bool v
   load_group(f1, A, lda, vec, N, NS, NW, wave, fr, fk, 1);
   for (int g = 0; g < NS; g += 3) {
     load_group(f2, A, lda, vec, N, NS, NW, wave, fr, fk, g + 2);
-    group_mfma<T, CPLX, SPW, CB>(f0, g, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai);
+    group_mfma<T, CPLX, SPW, CB, CONJ>(f0, g, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai);
     if (g + 1 >= NS) break;
     load_group(f0, A, lda, vec, N, NS, NW, wave, fr, fk, g + 3);
-    group_mfma<T, CPLX, SPW, CB>(f1, g + 1, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai);
+    group_mfma<T, CPLX, SPW, CB, CONJ>(f1, g + 1, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai);
     if (g + 2 >= NS) break;
     load_group(f1, A, lda, vec, N, NS, NW, wave, fr, fk, g + 4);
-    group_mfma<T, CPLX, SPW, CB>(f2, g + 2, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai);
+    group_mfma<T, CPLX, SPW, CB, CONJ>(f2, g + 2, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai);
   }
 }
 
@@ -339,8 +374,9 @@ __device__ inline void write_b(T* Br, T* Bi, int LDK, int NS, int NW, int wave, 
   }
 }
 
-// One workgroup per (walker, CB 16-column tiles); NW wavefronts.
-template <typename T, bool P1C, int SPW, int CB>
+// One workgroup per (walker, CB 16-column tiles); NW wavefronts.  CONJV: the V products use
+// conj(V), which is V^H for the complex-symmetric V of vHS (the adjoint propagator, sgn = -1).
+template <typename T, bool P1C, int SPW, int CB, bool CONJV = false>
 __global__ void __launch_bounds__(1024)
     k_prop_taylor(int W, int N, int n, int order, double sgn, const T* __restrict__ P1, int vecP,
                   const T* __restrict__ V, long v_bs, int vecV, z64* __restrict__ Phi, long phi_bs,
@@ -383,7 +419,8 @@ __global__ void __launch_bounds__(1024)
   write_b<T, SPW, CB>(Br, Bi, LDK, NS, NW, wave, fr, fk, ar, ai);
   __syncthreads();
   for (int k = 1; k <= order; ++k) {
-    stripes_times_b<T, true, SPW, CB>(Vw, N, vecV, N, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai);
+    stripes_times_b<T, true, SPW, CB, CONJV>(Vw, N, vecV, N, NS, NW, wave, fr, fk, Br, Bi, LDK, ar,
+                                             ai);
     // T_k = (i sgn / k) V T_{k-1}; S += T_k
     const T fi = (T)(sgn / (double)k);
 #pragma unroll
@@ -434,17 +471,17 @@ inline int max_lds_per_block() {
   return v;
 }
 
-template <typename T, bool P1C, int SPW, int CB>
+template <typename T, bool P1C, int SPW, int CB, bool CONJV = false>
 void launch_fused(hipStream_t s, const PropagateArgs& a, const FusedShape& f, const T* P1, int vecP,
                   const T* V, int vecV, z64* Phi) {
   const size_t lds = 2 * 16 * CB * (size_t)f.LDK * sizeof(T);
-  auto kern = k_prop_taylor<T, P1C, SPW, CB>;
+  auto kern = k_prop_taylor<T, P1C, SPW, CB, CONJV>;
   if (lds > 64 * 1024)
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   const long blocks = (long)a.W * ceil_div(a.n, 16 * CB);
   hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(64 * f.NW), lds, s, a.W, a.N, a.n, a.order,
-                     1.0, P1, vecP, V, a.v_bs, vecV, Phi, a.phi_bs, f.NW, f.LDK);
+                     CONJV ? -1.0 : 1.0, P1, vecP, V, a.v_bs, vecV, Phi, a.phi_bs, f.NW, f.LDK);
   AFQ_LAUNCH_CHECK();
 }
 
@@ -453,7 +490,10 @@ void launch_fused(hipStream_t s, const PropagateArgs& a, const FusedShape& f, co
 // tuner (timed on a scratch copy of the walkers) never changes the result.
 template <typename T>
 bool propagate_fused(Context& c, const PropagateArgs& a, const T* P1, bool p1c, const void* V) {
-  if (a.TA != 'N') return false;
+  // the adjoint needs V^H; the kernel reads V by rows, so it takes V^H = conj(V) for a
+  // complex-symmetric V only (FP64, the back-propagation)
+  const bool adj = a.TA == 'H';
+  if (adj && !(a.v_symmetric && sizeof(T) == 8)) return false;
   const int N = a.N, n = a.n, W = a.W;
   FusedShape f;
   const int NS = ceil_div(N, 16);
@@ -481,6 +521,13 @@ bool propagate_fused(Context& c, const PropagateArgs& a, const T* P1, bool p1c, 
   auto run = [&](int i, z64* Phi) {
     const int cb = cbs[i];
     constexpr int SPW2 = sizeof(T) == 4 ? 2 : 1;
+    if constexpr (sizeof(T) == 8) {
+      if (adj) {  // FP64 takes one stripe per wave and one column tile
+        if (p1c) launch_fused<T, true, 1, 1, true>(s, a, f, P1, vecP, Vt, vecV, Phi);
+        else launch_fused<T, false, 1, 1, true>(s, a, f, P1, vecP, Vt, vecV, Phi);
+        return;
+      }
+    }
 #define AFQ_PT(P1C, SPW, CB) launch_fused<T, P1C, SPW, CB>(s, a, f, P1, vecP, Vt, vecV, Phi)
     if (f.spw == 1) {
       if constexpr (sizeof(T) == 4) {
@@ -503,7 +550,7 @@ bool propagate_fused(Context& c, const PropagateArgs& a, const T* P1, bool p1c, 
     }
 #undef AFQ_PT
   };
-  tuned_on_copy(c, tune_key("prop_fused", sizeof(T), W, N, n, N), ncand,
+  tuned_on_copy(c, tune_key(adj ? "prop_fused_adj" : "prop_fused", sizeof(T), W, N, n, N), ncand,
                 a.Phi, (size_t)W * a.phi_bs, run);
   return true;
 }

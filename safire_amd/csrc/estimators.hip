@@ -5,13 +5,14 @@
 //    mixed estimator (full mixed density matrix of the current walkers) or back-propagation
 //    (back-propagated trial against the anchors) produce, each divided by the sum of the
 //    weights as MeasurementOutput does (Measurements.hpp:98-121).  The reference contracts with
-//    gemv and tensor contractions; here one workgroup owns one output row (i, spin) of every
-//    observable, its four waves take every fourth walker and the partial sums are combined in a
-//    fixed order, so a measurement is deterministic.  Each walker's row i, column i and diagonal
-//    are read once per workgroup (a few KB per walker, resident in L2 across the row blocks).
+//    gemv and tensor contractions; here a workgroup owns one output row (i, spin) of every
+//    observable for one block of walkers (about 1024 workgroups in all), its four waves take
+//    every fourth walker, and the walker blocks' partial sums are added in a fixed order by a
+//    second kernel, so a measurement is deterministic.  Each walker's row i, column i and
+//    diagonal are read once per workgroup (a few KB per walker, resident in L2 across rows).
 //  * reference_coefficients: the per-walker share of each reference determinant in a
 //    multi-reference measurement (MixedEstimator.hpp:61-103, BackPropEstimator.hpp:62-120).
-#include "afq_internal.hpp"
+#include "ops.hpp"
 
 namespace afq {
 
@@ -26,41 +27,47 @@ __device__ inline z64 cdiv(z64 a, z64 b) {
   return {(a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d};
 }
 
-// grid (N, nsp).  one [nsp][N][N]; two [nblk][N][N] with the blocks (aaaa), (aabb) and, for
-// COLLINEAR, (bbbb); the (aabb) block pairs spin 0 with spin `dn` (0 CLOSED, 1 COLLINEAR).
-__global__ void __launch_bounds__(kRdmT)
-    k_rdm_measure(int W, int nsp, int N, int dn, const z64* __restrict__ G, long g_bs,
-                  const z64* __restrict__ weight, const z64* __restrict__ coeff, int accumulate,
-                  z64* __restrict__ one, z64* __restrict__ two) {
-  const int i = blockIdx.x, s = blockIdx.y;
-  const int lane = threadIdx.x % kRdmLanes, grp = threadIdx.x / kRdmLanes;
-  __shared__ z64 red[kRdmGroups][kRdmLanes][3];
-  __shared__ z64 wsum[kRdmT];
-  // denominator: sum of the weights (not of weight x coefficient), fixed order
+// den[0] = sum_w weight[w], one workgroup, fixed order
+__global__ void __launch_bounds__(kRdmT) k_weight_sum(int W, const z64* __restrict__ weight,
+                                                      z64* __restrict__ den) {
+  __shared__ z64 part[kRdmT];
   z64 d = {0.0, 0.0};
   for (int w = threadIdx.x; w < W; w += kRdmT) d = cadd(d, weight[w]);
-  wsum[threadIdx.x] = d;
+  part[threadIdx.x] = d;
   __syncthreads();
   if (threadIdx.x == 0) {
     z64 t = {0.0, 0.0};
-    for (int k = 0; k < kRdmT; ++k) t = cadd(t, wsum[k]);
-    wsum[0] = t;
+    for (int k = 0; k < kRdmT; ++k) t = cadd(t, part[k]);
+    den[0] = t;
   }
-  __syncthreads();
-  const z64 den = wsum[0];
+}
+
+// grid (N, nsp, WB).  Workgroup (i, s, b) sums walkers [b wbs, min(W, (b + 1) wbs)) into row i
+// of the partial block b: [one: nsp][N][N] then [two: nsp + 1][N][N], unnormalized.  The
+// (aabb) block pairs spin 0 with spin `dn` (0 CLOSED, 1 COLLINEAR).
+__global__ void __launch_bounds__(kRdmT)
+    k_rdm_partial(int W, int wbs, int wmod, int nsp, int N, int dn, const z64* __restrict__ G,
+                  long g_bs, const z64* __restrict__ weight, const z64* __restrict__ coeff,
+                  int want_two, z64* __restrict__ part) {
+  const int i = blockIdx.x, s = blockIdx.y, b = blockIdx.z;
+  const int lane = threadIdx.x % kRdmLanes, grp = threadIdx.x / kRdmLanes;
+  __shared__ z64 red[kRdmGroups][kRdmLanes][3];
   const long NN = (long)N * N;
-  const bool pair = (s == 0) && two;  // this row also owns the (aabb) row
+  const long P = (long)(2 * nsp + 1) * NN;
+  z64* out = part + (long)b * P;
+  const int w0 = b * wbs, w1 = min(W, w0 + wbs);
+  const bool pair = (s == 0) && want_two;  // this row also owns the (aabb) row
   for (int j0 = 0; j0 < N; j0 += kRdmLanes) {
     const int j = j0 + lane;
     z64 a1 = {0, 0}, a2 = {0, 0}, a3 = {0, 0};
     if (j < N) {
-      for (int w = grp; w < W; w += kRdmGroups) {
-        z64 c = weight[w];
+      for (int w = w0 + grp; w < w1; w += kRdmGroups) {
+        z64 c = weight[w % wmod];
         if (coeff) c = cmul(c, coeff[w]);
         const z64* Gs = G + (long)w * g_bs + s * NN;
         const z64 gij = Gs[(long)i * N + j];
         a1 = cadd(a1, cmul(c, gij));
-        if (two) {
+        if (want_two) {
           const z64 gii = Gs[(long)i * N + i], gjj = Gs[(long)j * N + j], gji = Gs[(long)j * N + i];
           const z64 ci = cmul(c, gii);
           const z64 cij = cmul(c, gij);
@@ -86,22 +93,29 @@ __global__ void __launch_bounds__(kRdmT)
         t3 = cadd(t3, red[g][lane][2]);
       }
       const long o = (long)i * N + j;
-      // accumulate: add to the outputs (a sum over references, one launch per reference)
-      if (one) {
-        const z64 v = cdiv(t1, den);
-        one[s * NN + o] = accumulate ? cadd(one[s * NN + o], v) : v;
-      }
-      if (two) {
-        const z64 v = cdiv(t2, den);
-        two[(2 * s) * NN + o] = accumulate ? cadd(two[(2 * s) * NN + o], v) : v;
-        if (pair) {
-          const z64 u = cdiv(t3, den);
-          two[NN + o] = accumulate ? cadd(two[NN + o], u) : u;
-        }
+      out[s * NN + o] = t1;
+      if (want_two) {
+        out[(nsp + 2 * s) * NN + o] = t2;
+        if (pair) out[(nsp + 1) * NN + o] = t3;
       }
     }
     __syncthreads();
   }
+}
+
+// element t of [one][two]: the walker blocks' partial sums added in order, divided by den;
+// accumulate adds to the output (a sum over references, one measurement per reference)
+__global__ void k_rdm_finish(long n_one, long n_all, int WB, const z64* __restrict__ part,
+                             const z64* __restrict__ den, int accumulate, z64* __restrict__ one,
+                             z64* __restrict__ two) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_all) return;
+  z64* dst = t < n_one ? (one ? one + t : nullptr) : (two ? two + (t - n_one) : nullptr);
+  if (!dst) return;
+  z64 v = {0.0, 0.0};
+  for (int b = 0; b < WB; ++b) v = cadd(v, part[(long)b * n_all + t]);
+  v = cdiv(v, den[0]);
+  *dst = accumulate ? cadd(*dst, v) : v;
 }
 
 // one thread per walker; the largest real part is taken out before exponentiating, which
@@ -140,14 +154,30 @@ void reference_coefficients(hipStream_t s, int W, int R, const z64* lo, const z6
   AFQ_LAUNCH_CHECK();
 }
 
-void rdm_measure(hipStream_t s, int W, int nsp, int N, bool collinear, const z64* G, long g_bs,
-                 const z64* weight, const z64* coeff, bool accumulate, z64* one, z64* two) {
+void rdm_measure(Context& c, int W, int nsp, int N, bool collinear, const z64* G, long g_bs,
+                 const z64* weight, const z64* coeff, bool accumulate, z64* one, z64* two,
+                 int wmod) {
+  if (wmod <= 0) wmod = W;
+  AFQ_CHECK(wmod <= W, kErrShape, "weights of more walkers than density matrices");
   AFQ_CHECK(W > 0 && N > 0 && (nsp == 1 || nsp == 2) && (nsp == 2) == collinear, kErrShape,
             "density-matrix measurement shape");
   AFQ_CHECK(g_bs >= (long)nsp * N * N, kErrShape, "density-matrix batch stride");
   if (!one && !two) return;
-  hipLaunchKernelGGL(k_rdm_measure, dim3(N, nsp), dim3(kRdmT), 0, s, W, nsp, N, collinear ? 1 : 0,
-                     G, g_bs, weight, coeff, accumulate ? 1 : 0, one, two);
+  hipStream_t s = c.stream();
+  // walker blocks: about 1024 workgroups in the partial pass, at least 8 walkers per block
+  int WB = std::max(1, std::min(ceil_div(1024, (long)N * nsp), ceil_div(W, 8)));
+  const int wbs = ceil_div(W, WB);
+  WB = ceil_div(W, wbs);
+  const long NN = (long)N * N, n_one = nsp * NN, n_all = (long)(2 * nsp + 1) * NN;
+  z64* part = scratch_as<z64>(c, "rdm_part", (size_t)WB * n_all);
+  z64* den = scratch_as<z64>(c, "rdm_den", 1);
+  hipLaunchKernelGGL(k_weight_sum, dim3(1), dim3(kRdmT), 0, s, wmod, weight, den);
+  AFQ_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_rdm_partial, dim3(N, nsp, WB), dim3(kRdmT), 0, s, W, wbs, wmod, nsp, N,
+                     collinear ? 1 : 0, G, g_bs, weight, coeff, two ? 1 : 0, part);
+  AFQ_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_rdm_finish, dim3(ceil_div(n_all, 256)), dim3(256), 0, s, n_one, n_all, WB,
+                     part, den, accumulate ? 1 : 0, one, two);
   AFQ_LAUNCH_CHECK();
 }
 

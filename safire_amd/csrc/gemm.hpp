@@ -61,6 +61,15 @@ struct Opnd {
   long ld;
   long bs;
   __device__ long row_offset(int) const { return 0; }
+  __device__ long batch_offset(int b) const { return (long)b * bs; }
+};
+
+// Operand shared between batches: batch b reads block (b / div) % mod, so consecutive batches
+// can share one block (div) or the blocks can repeat every mod batches.
+template <typename TIn, bool CPLX, bool KCONTIG, bool CONJ = false>
+struct OpndB : Opnd<TIn, CPLX, KCONTIG, CONJ> {
+  int div, mod;
+  __device__ long batch_offset(int b) const { return (long)((b / div) % mod) * this->bs; }
 };
 
 // Upper-triangle 8x8 blocks of an N x N pair space (i, k): block (bi, bk), bi <= bk, enumerated
@@ -99,6 +108,7 @@ struct OpndSymPairs {
     int i, k;
     return sym_pair(r, nb, N, i, k) ? ((long)i * N + k) * ld : -1;
   }
+  __device__ long batch_offset(int b) const { return (long)b * bs; }
 };
 
 // ---- epilogues: called once per output element with the accumulated (re, im) ----------------
@@ -223,9 +233,9 @@ struct Stager {
       long a;
       if constexpr (OP::mapped) {
         in = (roff[e] >= 0) && (gk < K) && (EXACT || idx < TOT);
-        a = (long)b * op.bs + roff[e] + gk;
+        a = op.batch_offset(b) + roff[e] + gk;
       } else {
-        a = (long)b * op.bs + (OP::kcontig ? (long)gr * op.ld + gk : (long)gk * op.ld + gr);
+        a = op.batch_offset(b) + (OP::kcontig ? (long)gr * op.ld + gk : (long)gk * op.ld + gr);
       }
       if (OP::is_cplx) {
         cplx<TIn> v = in ? reinterpret_cast<const cplx<TIn>*>(op.p)[a] : cplx<TIn>{0, 0};

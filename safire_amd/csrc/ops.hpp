@@ -67,6 +67,13 @@ void inverse_logdet(Context& c, const z64* A, int nb, int n, z64* ovlp, z64* Ain
 // at G + w * g_bs; L [W][N][n] (l_bs apart), ovlp += log det T.
 void mixed_density_matrix_left(Context& c, const z64* L, long l_bs, int n, int N, const z64* Phi,
                                long phi_bs, int W, z64* G, long g_bs, z64* ovlp);
+// Reference-major batches b = d W + w (B of them): left state L_b = L + (b / ldiv) l_bs
+// (ldiv = W: one left state per reference shared by the walkers; 1: one per batch), walker
+// Phi_(b mod W) = Phi + (b mod W) phi_bs.  ovlp[b] += log det(L_b^H Phi); G, when given, [B]
+// blocks of g_bs: the full (Phi (L^H Phi)^-1 L^H)^T.
+void mixed_density_matrix_batched(Context& c, const z64* L, long l_bs, int ldiv, int n, int N,
+                                  const z64* Phi, long phi_bs, int W, int B, z64* G, long g_bs,
+                                  z64* ovlp);
 // ovlp[w] += log det(L_w^H Phi_w) for left states L [W][N][n] (batch stride l_bs)
 void log_overlap_left(Context& c, const z64* L, long l_bs, int n, int N, const z64* Phi,
                       long phi_bs, int W, z64* ovlp);
@@ -85,6 +92,7 @@ struct PropagateArgs {
   long phi_bs;
   Precision prec;
   bool fused = true;       // one-launch kernel when the shape allows; false: per-product GEMMs
+  bool v_symmetric = false;  // V[i][j] == V[j][i] exactly (vHS output): lets 'H' use the fused kernel
 };
 void propagate(Context& c, const PropagateArgs& a);
 // orthogonalize (orthogonalize.hpp:131-169): Phi <- Q with Phi = Q R, R upper triangular with a
@@ -198,6 +206,10 @@ class PhmsdTrial {
   // PHMSD::vMF followed by the charge-decomposition zero_imag
   void vMF(double dt, z64* v);
   int ndet() const { return ndet_; }
+  int nact(int sp) const { return sp_[sp].nact; }
+  const z64* orbitals(int sp) const { return sp_[sp].orbs; }  // device [nact][N]
+  // active orbital of column k of determinant d's spin-sp block, [ndet][nel_s]
+  std::vector<int> reference_orbitals(int sp) const;
   // The determinants as references (PHMSD::getReferences, PHMSD.hpp:242-279), in PsiT form:
   // rows_s [ndet][nel_s][N] are the trial rows each determinant occupies, in the order of
   // ph_excitations::get_configuration (the reference string with holes replaced by particles);
@@ -241,6 +253,16 @@ class PhmsdTrial {
   void weights(int W, z64* Ov);
   void free_all();
 };
+
+// One OneRDM / DiagonalTwoRDM measurement (estimators.hip): one [nsp][N][N] = sum_w c_w G_w /
+// sum_w weight_w and two [nsp + 1][N][N] = the diagonal two-body blocks (aaaa), (aabb)[, (bbbb)]
+// with c_w = weight_w * coeff_w (coeff may be null: 1); either output may be null; accumulate
+// adds to the outputs instead.  G [W] blocks of g_bs elements, each [nsp][N][N].  wmod > 0:
+// the W blocks are several references' blocks of the same wmod walkers, block b has the weight
+// of walker b mod wmod and the denominator sums those wmod weights.
+void rdm_measure(Context& c, int W, int nsp, int N, bool collinear, const z64* G, long g_bs,
+                 const z64* weight, const z64* coeff, bool accumulate, z64* one, z64* two,
+                 int wmod = 0);
 
 // ---------------------------------------------------------------------------------------------
 // Propagator: one hybrid step of the whole walker set on the device
@@ -363,10 +385,19 @@ class Propagator {
   int nref_ = 0;
   z64* ref_rows_[2] = {nullptr, nullptr};
   z64* ref_cc_ = nullptr;
+  // PHMSD: selection matrices [nact_s][nref n_s], column d n_s + k = unit vector of the active
+  // orbital in column k of reference d (the references as combinations of the active orbitals)
+  z64* ref_sel_[2] = {nullptr, nullptr};
+  // the references as Slater matrices, [nref][N][nel_s] (the conjugate transposes of ref_rows_)
+  z64* ref_cols_[2] = {nullptr, nullptr};
   void ensure_references();
   // back-propagates every reference through the last nbp stored fields; on return Lp[s] holds
   // [W][nref][N][nel_s] and ldR [W][nref] the accumulated log det R of their QR steps
   void bp_propagate_refs(const WalkerState& s, int nbp, int ortho, z64* Lp[2], z64* ldR);
+  void bp_propagate_orbitals(const WalkerState& s, uint32_t C, int nbp, int ortho, z64* Lp[2],
+                             z64* ldR);
+  void measure_refs(int W, const z64* const Lp[2], bool per_walker, const z64* const Phi[2],
+                    const z64* weight, const z64* ldR, z64* onerdm, z64* diag2rdm);
   void clear_graphs();
   void step_impl(const WalkerState& s, const double* u, const double* Eshift);
 };

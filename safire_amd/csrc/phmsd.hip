@@ -1128,18 +1128,24 @@ std::vector<int> PhmsdTrial::occupations(int sp) const {
   return occ;
 }
 
+std::vector<int> PhmsdTrial::reference_orbitals(int sp) const {
+  const SpinDev& d = sp_[sp];
+  const std::vector<int> occ = occupations(sp);
+  std::vector<int> idx((size_t)ndet_ * d.nel);
+  for (int dd = 0; dd < ndet_; ++dd) {
+    const int u = det_ab_h_[2 * dd + sp];
+    std::copy_n(occ.begin() + (size_t)u * d.nel, d.nel, idx.begin() + (size_t)dd * d.nel);
+  }
+  return idx;
+}
+
 void PhmsdTrial::references(z64* rows0, z64* rows1, z64* cc) {
   hipStream_t s = ctx_.stream();
   const int N = ham_.N();
   z64* rows[2] = {rows0, rows1};
   for (int sp = 0; sp < 2; ++sp) {
     const SpinDev& d = sp_[sp];
-    const std::vector<int> occ = occupations(sp);
-    std::vector<int> idx((size_t)ndet_ * d.nel);
-    for (int dd = 0; dd < ndet_; ++dd) {
-      const int u = det_ab_h_[2 * dd + sp];
-      std::copy_n(occ.begin() + (size_t)u * d.nel, d.nel, idx.begin() + (size_t)dd * d.nel);
-    }
+    const std::vector<int> idx = reference_orbitals(sp);
     int* idx_d = dcopy(s, idx);
     gather_rows(s, (long)ndet_ * d.nel, N, idx_d, d.orbs, rows[sp]);
     AFQ_HIP(hipStreamSynchronize(s));

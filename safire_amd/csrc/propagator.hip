@@ -16,7 +16,9 @@
 // after every propagation.
 #include <vector>
 
+#include "gemm.hpp"
 #include "ops.hpp"
+#include "tuner.hpp"
 
 namespace afq {
 
@@ -91,7 +93,8 @@ Propagator::~Propagator() {
   for (void* q : {(void*)vMF_, (void*)P1_, (void*)P1r_, (void*)P1r32_, (void*)P1c32_,
                   (void*)step_dev_, (void*)status_dev_, (void*)bp_fields_, (void*)bp_wfac_,
                   (void*)bp_anch_a_, (void*)bp_anch_b_, (void*)ref_rows_[0], (void*)ref_rows_[1],
-                  (void*)ref_cc_})
+                  (void*)ref_cc_, (void*)ref_sel_[0], (void*)ref_sel_[1], (void*)ref_cols_[0],
+                  (void*)ref_cols_[1]})
     if (q) (void)hipFree(q);
 }
 
@@ -264,12 +267,26 @@ void Propagator::ensure_references() {
   const int N = ham_.N();
   const int R = trial_ ? trial_->ndet() : 1;
   const int nsl[2] = {wup_, wdn_};
-  z64* buf[3] = {nullptr, nullptr, nullptr};
+  z64* buf[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   try {
-    for (int sp = 0; sp < 2; ++sp) buf[sp] = dalloc<z64>((size_t)R * std::max(nsl[sp], 1) * N);
+    for (int sp = 0; sp < 2; ++sp) {
+      buf[sp] = dalloc<z64>((size_t)R * std::max(nsl[sp], 1) * N);
+      buf[5 + sp] = dalloc<z64>((size_t)R * std::max(nsl[sp], 1) * N);
+    }
     buf[2] = dalloc<z64>(R);
     if (trial_) {
       trial_->references(buf[0], buf[1], buf[2]);
+      for (int sp = 0; sp < 2; ++sp) {
+        const int na = trial_->nact(sp), n = nsl[sp];
+        const std::vector<int> occ = trial_->reference_orbitals(sp);
+        std::vector<z64> sel((size_t)na * R * n, z64{0.0, 0.0});
+        for (int d = 0; d < R; ++d)
+          for (int k = 0; k < n; ++k) sel[(size_t)occ[(size_t)d * n + k] * R * n + (size_t)d * n + k] = {1.0, 0.0};
+        buf[3 + sp] = dalloc<z64>(sel.size());
+        AFQ_HIP(hipMemcpyAsync(buf[3 + sp], sel.data(), sizeof(z64) * sel.size(),
+                               hipMemcpyHostToDevice, s));
+        AFQ_HIP(hipStreamSynchronize(s));
+      }
     } else {
       // NOMSD: the single determinant, coefficient 1
       for (int sp = 0; sp < 2; ++sp)
@@ -284,9 +301,21 @@ void Propagator::ensure_references() {
       if (q) (void)hipFree(q);
     throw;
   }
+  // Slater-matrix form [R][N][n]: conj-transpose all rows ([N][R n]), then each reference contiguous
+  for (int sp = 0; sp < 2; ++sp) {
+    if (nsl[sp] == 0) continue;
+    z64* tmp = scratch_as<z64>(ctx_, "ref_tmp", (size_t)R * nsl[sp] * N);
+    conj_transpose(s, R * nsl[sp], N, buf[sp], tmp);
+    swap_mid(s, 1, N, R, nsl[sp], tmp, buf[5 + sp]);
+  }
+  AFQ_HIP(hipStreamSynchronize(s));
   ref_rows_[0] = buf[0];
   ref_rows_[1] = buf[1];
   ref_cc_ = buf[2];
+  ref_sel_[0] = buf[3];
+  ref_sel_[1] = buf[4];
+  ref_cols_[0] = buf[5];
+  ref_cols_[1] = buf[6];
   nref_ = R;
 }
 
@@ -308,22 +337,19 @@ void Propagator::bp_propagate_refs(const WalkerState& st, int nbp, int ortho, z6
   uint32_t C = 0;  // completed steps: the newest stored field sits at slot (C - 1) % nback
   AFQ_HIP(hipMemcpyAsync(&C, step_dev_, sizeof(C), hipMemcpyDeviceToHost, s));
   AFQ_HIP(hipStreamSynchronize(s));
-  // L_s [W][N][R n_s]: every walker starts from the references' orbitals, reference d in
-  // columns d n_s .. (d + 1) n_s - 1; the propagators act on all of them at once.  A QR step
-  // needs each reference contiguous, [W][R][N][n_s] (Lp), and back.
-  z64* L[2] = {nullptr, nullptr};
-  Lp[0] = Lp[1] = nullptr;
-  for (int sp = 0; sp < nsp; ++sp) {
-    if (nsl[sp] == 0) continue;  // a spin without electrons (a polarized COLLINEAR set)
-    const long cols = (long)R * nsl[sp];
-    z64* ref = scratch_as<z64>(ctx_, sp == 0 ? "bp_ref0" : "bp_ref1", (size_t)N * cols);
-    conj_transpose(s, (int)cols, N, ref_rows_[sp], ref);
-    L[sp] = scratch_as<z64>(ctx_, sp == 0 ? "bp_L0" : "bp_L1", (size_t)W * N * cols);
-    broadcast_z64(s, W, (long)N * cols, ref, L[sp]);
-    Lp[sp] = R > 1 ? scratch_as<z64>(ctx_, sp == 0 ? "bp_Lp0" : "bp_Lp1", (size_t)W * N * cols)
-                   : L[sp];
+  if (ref_sel_[0]) {
+    bp_propagate_orbitals(st, C, nbp, ortho, Lp, ldR);
+    return;
   }
-  fill_z64(s, (long)W * R, {0, 0}, ldR);
+  // a single reference: L_s [W][N][n_s] starts from the trial's orbitals in every walker
+  AFQ_CHECK(R == 1, kErrState, "several references are back-propagated through their orbitals");
+  for (int sp = 0; sp < nsp; ++sp) {
+    Lp[sp] = nullptr;
+    if (nsl[sp] == 0) continue;  // a spin without electrons (a polarized COLLINEAR set)
+    Lp[sp] = scratch_as<z64>(ctx_, sp == 0 ? "bp_L0" : "bp_L1", (size_t)W * N * nsl[sp]);
+    broadcast_z64(s, W, (long)N * nsl[sp], ref_cols_[sp], Lp[sp]);
+  }
+  fill_z64(s, W, {0, 0}, ldR);
   z64* X = scratch_as<z64>(ctx_, "bp_X", (size_t)W * M);
   z64* V = scratch_as<z64>(ctx_, "bp_V", (size_t)W * N * N);
   for (int ni = nbp - 1; ni >= 0; --ni) {
@@ -335,16 +361,91 @@ void Propagator::bp_propagate_refs(const WalkerState& st, int nbp, int ortho, z6
     for (int sp = 0; sp < nsp; ++sp) {
       const int n = nsl[sp];
       if (n == 0) continue;
-      det_ops::PropagateArgs pa{W, N, R * n, p_.order, 'H', P1_, real_P1_ ? P1r_ : nullptr,
-                                nullptr, nullptr, V, (long)N * N, L[sp], (long)N * R * n, kFP64};
+      det_ops::PropagateArgs pa{W, N, n, p_.order, 'H', P1_, real_P1_ ? P1r_ : nullptr,
+                                nullptr, nullptr, V, (long)N * N, Lp[sp], (long)N * n, kFP64};
+      pa.v_symmetric = true;  // vHS writes both triangles from the same sums
       det_ops::propagate(ctx_, pa);
-      if (qr) {
-        if (R > 1) swap_mid(s, W, N, R, n, L[sp], Lp[sp]);
-        // logdetR[w][d] accumulates log det R of both spins (AFQMCBasePropagator.cpp:473-478)
-        det_ops::orthogonalize(ctx_, Lp[sp], (long)N * n, W * R, N, n, ldR);
-        if (R > 1 && ni != 0) swap_mid(s, W, R, N, n, Lp[sp], L[sp]);
-      }
+      // logdetR accumulates log det R of both spins (AFQMCBasePropagator.cpp:473-478)
+      if (qr) det_ops::orthogonalize(ctx_, Lp[sp], (long)N * n, W, N, n, ldR);
     }
+  }
+}
+
+void Propagator::bp_propagate_orbitals(const WalkerState& st, uint32_t C, int nbp, int ortho,
+                                       z64* Lp[2], z64* ldR) {
+  using gemm::EpiStore;
+  using gemm::Opnd;
+  hipStream_t s = ctx_.stream();
+  const int W = st.W, N = ham_.N(), M = ham_.M(), R = nref_;
+  const int nsl[2] = {wup_, wdn_};
+  z64 *O[2], *Oold[2], *Mall[2], *Mtmp[2], *Mp[2], *Tm[2];
+  const char* nm[2][6] = {{"bpo_O0", "bpo_Oo0", "bpo_M0", "bpo_Mt0", "bpo_Mp0", "bpo_T0"},
+                          {"bpo_O1", "bpo_Oo1", "bpo_M1", "bpo_Mt1", "bpo_Mp1", "bpo_T1"}};
+  for (int sp = 0; sp < 2; ++sp) {
+    const int na = trial_->nact(sp), n = nsl[sp];
+    AFQ_CHECK(n <= na && na <= N && na <= 64, kErrShape, "active space for back-propagation");
+    const size_t Rn = (size_t)R * n;
+    z64* cols = scratch_as<z64>(ctx_, sp == 0 ? "bpo_c0" : "bpo_c1", (size_t)N * na);
+    conj_transpose(s, na, N, trial_->orbitals(sp), cols);
+    O[sp] = scratch_as<z64>(ctx_, nm[sp][0], (size_t)W * N * na);
+    broadcast_z64(s, W, (long)N * na, cols, O[sp]);
+    Oold[sp] = scratch_as<z64>(ctx_, nm[sp][1], (size_t)W * N * na);
+    Mall[sp] = scratch_as<z64>(ctx_, nm[sp][2], (size_t)W * na * Rn);
+    broadcast_z64(s, W, (long)(na * Rn), ref_sel_[sp], Mall[sp]);
+    Mtmp[sp] = scratch_as<z64>(ctx_, nm[sp][3], (size_t)W * na * Rn);
+    Mp[sp] = scratch_as<z64>(ctx_, nm[sp][4], (size_t)W * na * Rn);
+    Tm[sp] = scratch_as<z64>(ctx_, nm[sp][5], (size_t)W * na * na);
+  }
+  fill_z64(s, (long)W * R, {0, 0}, ldR);
+  z64* ld0 = scratch_as<z64>(ctx_, "bpo_ld", W);
+  z64* X = scratch_as<z64>(ctx_, "bp_X", (size_t)W * M);
+  z64* V = scratch_as<z64>(ctx_, "bp_V", (size_t)W * N * N);
+  for (int ni = nbp - 1; ni >= 0; --ni) {
+    const long back = (long)nbp - ni;
+    const int slot = (int)((((long)C - back) % bp_n_ + bp_n_) % bp_n_);
+    gather_fields(s, W, M, bp_fields_, bp_n_, slot, X);
+    ham_.vHS(X, W, p_.dt, V, kFP64);
+    const bool qr = ni == 0 || ni % ortho == 0;
+    for (int sp = 0; sp < 2; ++sp) {
+      const int na = trial_->nact(sp), n = nsl[sp];
+      const int Rn = R * n;
+      det_ops::PropagateArgs pa{W, N, na, p_.order, 'H', P1_, real_P1_ ? P1r_ : nullptr,
+                                nullptr, nullptr, V, (long)N * N, O[sp], (long)N * na, kFP64};
+      pa.v_symmetric = true;
+      det_ops::propagate(ctx_, pa);
+      if (!qr) continue;
+      AFQ_HIP(hipMemcpyAsync(Oold[sp], O[sp], sizeof(z64) * W * N * na, hipMemcpyDeviceToDevice, s));
+      fill_z64(s, W, {0, 0}, ld0);
+      det_ops::orthogonalize(ctx_, O[sp], (long)N * na, W, N, na, ld0);
+      {  // T = Q^H O_old
+        Opnd<double, true, false, true> A{&O[sp]->re, na, (long)N * na};
+        Opnd<double, true, false> B{&Oold[sp]->re, na, (long)N * na};
+        gemm_small<double>(ctx_, "bpo_T", W, na, na, N, A, B,
+                           EpiStore<double, double>{Tm[sp], na, (long)na * na, 1, 0, 0});
+      }
+      {  // T M_d for every reference
+        Opnd<double, true, true> A{&Tm[sp]->re, na, (long)na * na};
+        Opnd<double, true, false> B{&Mall[sp]->re, Rn, (long)na * Rn};
+        gemm_small<double>(ctx_, "bpo_TM", W, na, Rn, na, A, B,
+                           EpiStore<double, double>{Mtmp[sp], Rn, (long)na * Rn, 1, 0, 0});
+      }
+      swap_mid(s, W, na, R, n, Mtmp[sp], Mp[sp]);
+      det_ops::orthogonalize(ctx_, Mp[sp], (long)na * n, W * R, na, n, ldR);
+      swap_mid(s, W, R, na, n, Mp[sp], Mall[sp]);
+    }
+  }
+  // L_d = O M_d, then each reference contiguous: Lp [W][R][N][n]
+  for (int sp = 0; sp < 2; ++sp) {
+    const int na = trial_->nact(sp), n = nsl[sp];
+    const int Rn = R * n;
+    z64* L = scratch_as<z64>(ctx_, sp == 0 ? "bp_L0" : "bp_L1", (size_t)W * N * Rn);
+    Opnd<double, true, true> A{&O[sp]->re, na, (long)N * na};
+    Opnd<double, true, false> B{&Mall[sp]->re, Rn, (long)na * Rn};
+    gemm_small<double>(ctx_, "bpo_L", W, N, Rn, na, A, B,
+                       EpiStore<double, double>{L, Rn, (long)N * Rn, 1, 0, 0});
+    // [W][N][R][n] -> [R][W N][n]: reference-major, each reference's walkers contiguous
+    Lp[sp] = scratch_as<z64>(ctx_, sp == 0 ? "bp_Lp0" : "bp_Lp1", (size_t)W * N * Rn);
+    swap_mid(s, 1, W * N, R, n, L, Lp[sp]);
   }
 }
 
@@ -375,83 +476,95 @@ void Propagator::backpropagate(const WalkerState& st, int nbp, int ortho, int re
   restore_weights(s, W, st.weight, bp_wfac_, bp_hl_, C, restore * nbp, wout);
 }
 
+namespace {
+
+// references per chunk of a measurement: the density matrices, the compact intermediate and the
+// overlap matrices of chunk * W (reference, walker) pairs within about 512 MB
+int reference_chunk(int R, int W, int nsp, int N, int n) {
+  const double per = 16.0 * W * ((double)nsp * N * N + (double)n * N + (double)n * n);
+  return std::max(1, std::min(R, (int)(512.0 * (1 << 20) / per)));
+}
+
+}  // namespace
+
+// The observables of one measurement from the left states L (Slater-matrix form) of R
+// references against the walker matrices Phi: reference coefficients from the log overlaps (+
+// conj(ldR) when given), then per chunk of references the density matrices of every
+// (reference, walker) pair and their weighted sums.  Lp[s] holds [R][W][N][n_s] left states
+// (per_walker) or [R][N][n_s] shared by the walkers.
+void Propagator::measure_refs(int W, const z64* const Lp[2], bool per_walker,
+                              const z64* const Phi[2], const z64* weight, const z64* ldR,
+                              z64* onerdm, z64* diag2rdm) {
+  hipStream_t s = ctx_.stream();
+  const int N = ham_.N(), R = nref_;
+  const bool coll = !closed_;
+  const int nsp = coll ? 2 : 1;
+  const int nsl[2] = {wup_, coll ? wdn_ : 0};
+  const int nmax = std::max(nsl[0], nsl[1]);
+  const int RC = reference_chunk(R, W, nsp, N, nmax);
+  const int ldiv = per_walker ? 1 : W;
+  // elements between consecutive references of spin sp
+  auto rstride = [&](int sp) { return (long)(per_walker ? W : 1) * N * nsl[sp]; };
+  z64* lo = scratch_as<z64>(ctx_, "ms_lo", (size_t)R * W);
+  z64* cf = scratch_as<z64>(ctx_, "ms_cf", (size_t)R * W);
+  fill_z64(s, (long)R * W, {0, 0}, lo);
+  for (int d0 = 0; d0 < R; d0 += RC) {
+    const int B = std::min(RC, R - d0) * W;
+    for (int sp = 0; sp < nsp; ++sp)
+      if (nsl[sp] > 0)
+        det_ops::mixed_density_matrix_batched(ctx_, Lp[sp] + (size_t)d0 * rstride(sp),
+                                              (long)N * nsl[sp], ldiv, nsl[sp], N, Phi[sp],
+                                              (long)N * nsl[sp], W, B, nullptr, 0,
+                                              lo + (size_t)d0 * W);
+  }
+  reference_coefficients(s, W, R, lo, ref_cc_, ldR, cf);
+  z64* G = scratch_as<z64>(ctx_, "ms_G", (size_t)RC * W * nsp * N * N);
+  // a spin without electrons keeps a zero density matrix
+  if (nsl[0] == 0 || (nsp == 2 && nsl[1] == 0))
+    fill_z64(s, (long)RC * W * nsp * N * N, {0, 0}, G);
+  for (int d0 = 0; d0 < R; d0 += RC) {
+    const int B = std::min(RC, R - d0) * W;
+    for (int sp = 0; sp < nsp; ++sp)
+      if (nsl[sp] > 0)
+        det_ops::mixed_density_matrix_batched(ctx_, Lp[sp] + (size_t)d0 * rstride(sp),
+                                              (long)N * nsl[sp], ldiv, nsl[sp], N, Phi[sp],
+                                              (long)N * nsl[sp], W, B, G + (size_t)sp * N * N,
+                                              (long)nsp * N * N, nullptr);
+    rdm_measure(ctx_, B, nsp, N, coll, G, (long)nsp * N * N, weight, cf + (size_t)d0 * W, d0 > 0,
+                onerdm, diag2rdm, W);
+  }
+}
+
 void Propagator::bp_measure(const WalkerState& st, int nbp, int ortho, int restore, z64* onerdm,
                             z64* diag2rdm) {
   AFQ_CHECK(restore >= 0 && restore <= 2, kErrArg, "restore must be 0, 1 or 2");
   ensure_references();
   hipStream_t s = ctx_.stream();
-  const int W = st.W, N = ham_.N(), R = nref_;
-  const bool coll = !closed_;
-  const int nsp = coll ? 2 : 1;
-  const int nsl[2] = {wup_, coll ? wdn_ : 0};
+  const int W = st.W, R = nref_;
   z64* Lp[2] = {nullptr, nullptr};
   z64* ldR = scratch_as<z64>(ctx_, "bp_ldR", (size_t)W * R);
   bp_propagate_refs(st, nbp, ortho, Lp, ldR);
-  z64* anch[2] = {bp_anch_a_, bp_anch_b_};
-  // reference coefficients (constructBPMeasurementInputs, BackPropEstimator.hpp:62-104):
-  // conj(ci) <L_d|anchor> e^{conj(logdetR)}, normalized over the references of each walker
-  z64* lo = scratch_as<z64>(ctx_, "bpm_lo", (size_t)R * W);
-  z64* cf = scratch_as<z64>(ctx_, "bpm_cf", (size_t)R * W);
-  fill_z64(s, (long)R * W, {0, 0}, lo);
-  for (int d = 0; d < R; ++d)
-    for (int sp = 0; sp < nsp; ++sp)
-      if (nsl[sp] > 0)
-        det_ops::log_overlap_left(ctx_, Lp[sp] + (size_t)d * N * nsl[sp], (long)R * N * nsl[sp],
-                                nsl[sp], N, anch[sp], (long)N * nsl[sp], W, lo + (size_t)d * W);
-  reference_coefficients(s, W, R, lo, ref_cc_, ldR, cf);
   // path-restored weights (BackPropEstimator.hpp:212-228)
   uint32_t C = 0;
   AFQ_HIP(hipMemcpyAsync(&C, step_dev_, sizeof(C), hipMemcpyDeviceToHost, s));
   AFQ_HIP(hipStreamSynchronize(s));
   z64* wr = scratch_as<z64>(ctx_, "bpm_w", W);
   restore_weights(s, W, st.weight, bp_wfac_, bp_hl_, C, restore * nbp, wr);
-  z64* G = scratch_as<z64>(ctx_, "bpm_G", (size_t)W * nsp * N * N);
-  // a spin without electrons keeps a zero density matrix
-  if (nsl[0] == 0 || (nsp == 2 && nsl[1] == 0)) fill_z64(s, (long)W * nsp * N * N, {0, 0}, G);
-  for (int d = 0; d < R; ++d) {
-    for (int sp = 0; sp < nsp; ++sp)
-      if (nsl[sp] > 0)
-        det_ops::mixed_density_matrix_left(ctx_, Lp[sp] + (size_t)d * N * nsl[sp],
-                                         (long)R * N * nsl[sp], nsl[sp], N, anch[sp],
-                                         (long)N * nsl[sp], W, G + (size_t)sp * N * N,
-                                         (long)nsp * N * N, nullptr);
-    rdm_measure(s, W, nsp, N, coll, G, (long)nsp * N * N, wr, cf + (size_t)d * W, d > 0, onerdm,
-                diag2rdm);
-  }
+  // reference coefficients conj(ci) <L_d|anchor> e^{conj(logdetR)} (constructBPMeasurementInputs,
+  // BackPropEstimator.hpp:62-104); the back-propagated references are [R][W][N][n]
+  const z64* L[2] = {Lp[0], Lp[1]};
+  const z64* anch[2] = {bp_anch_a_, bp_anch_b_};
+  measure_refs(W, L, true, anch, wr, ldR, onerdm, diag2rdm);
 }
 
 void Propagator::mixed_measure(const WalkerState& st, z64* onerdm, z64* diag2rdm) {
   AFQ_CHECK(st.W <= W_, kErrShape, "walker count exceeds the propagator's capacity");
   ensure_references();
-  hipStream_t s = ctx_.stream();
-  const int W = st.W, N = ham_.N(), R = nref_;
-  const bool coll = !closed_;
-  const int nsp = coll ? 2 : 1;
-  const int nsl[2] = {wup_, coll ? wdn_ : 0};
+  // constructMixedMeasurementInputs (MixedEstimator.hpp:40-106): conj(ci) <D_d|Phi> normalized
+  // over the references of each walker (the reference's shift by OVLP cancels in the ratio)
+  const z64* L[2] = {ref_cols_[0], ref_cols_[1]};
   const z64* Phi[2] = {st.Phi, st.Phi_b};
-  // reference coefficients (constructMixedMeasurementInputs, MixedEstimator.hpp:40-106):
-  // conj(ci) <D_d|Phi>, normalized over the references of each walker (the reference's shift by
-  // OVLP cancels in the ratio)
-  z64* lo = scratch_as<z64>(ctx_, "mm_lo", (size_t)R * W);
-  z64* cf = scratch_as<z64>(ctx_, "mm_cf", (size_t)R * W);
-  fill_z64(s, (long)R * W, {0, 0}, lo);
-  for (int d = 0; d < R; ++d)
-    for (int sp = 0; sp < nsp; ++sp)
-      if (nsl[sp] > 0)
-        det_ops::log_overlap(ctx_, ref_rows_[sp] + (size_t)d * nsl[sp] * N, nsl[sp], N, Phi[sp],
-                           (long)N * nsl[sp], W, lo + (size_t)d * W);
-  reference_coefficients(s, W, R, lo, ref_cc_, nullptr, cf);
-  z64* G = scratch_as<z64>(ctx_, "mm_G", (size_t)W * nsp * N * N);
-  if (nsl[0] == 0 || (nsp == 2 && nsl[1] == 0)) fill_z64(s, (long)W * nsp * N * N, {0, 0}, G);
-  for (int d = 0; d < R; ++d) {
-    for (int sp = 0; sp < nsp; ++sp)
-      if (nsl[sp] > 0)
-        det_ops::mixed_density_matrix(ctx_, ref_rows_[sp] + (size_t)d * nsl[sp] * N, nsl[sp], N,
-                                    Phi[sp], (long)N * nsl[sp], W, G + (size_t)sp * N * N,
-                                    (long)nsp * N * N, nullptr, false);
-    rdm_measure(s, W, nsp, N, coll, G, (long)nsp * N * N, st.weight, cf + (size_t)d * W, d > 0,
-                onerdm, diag2rdm);
-  }
+  measure_refs(st.W, L, false, Phi, st.weight, nullptr, onerdm, diag2rdm);
 }
 
 int Propagator::status() {
