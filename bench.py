@@ -106,6 +106,23 @@ def time_vhs(ham, precision, W, reps=10):
     return ev0.elapsed_time(ev1) / 1e3 / reps
 
 
+def time_energy(prop, ws, precision, reps=3):
+    """Seconds per local-energy evaluation of the whole walker set (Energy: G, E1, exchange and
+    Coulomb terms), timed around synchronized calls after one warm-up."""
+    import torch
+    from safire_amd import afq
+    Wn = ws.W
+    E = afq.empty((Wn, 3), torch.complex128)
+    ov = afq.empty(Wn, torch.complex128)
+    prop.Energy(ws, E, ov, precision)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        prop.Energy(ws, E, ov, precision)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps
+
+
 def bh_mixed_error(W_e=256, steps=10):
     """Mixed vs FP64 on BH read from the reference's inputs (19 orbitals, 78 Cholesky vectors,
     RHF trial): the same walkers and field draws propagated `steps` steps in each precision, then
@@ -296,7 +313,7 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--only", choices=["vhs", "vhs64", "copy", "step", "step64", "phmsd", "vendor",
-                                       "estimators"],
+                                       "estimators", "energy"],
                     default=None,
                     help="profiling modes: launch only the named kernel(s)")
     ap.add_argument("--serve-jobs", type=int, default=24,
@@ -321,6 +338,15 @@ def main():
                                         precision=prec)
         t = time_vhs(ham, prec, W, reps=args.steps)
         log(f"vHS launch {t * 1e3:.3f} ms")
+        return
+    if args.only == "energy":
+        for prec, name in ((afq.AFQ_MIXED, "mixed"), (afq.AFQ_FP64, "fp64")):
+            ctx, ham, prop, ws, Esh, eout = gpu_setup(problem, prec, W)
+            te = time_energy(prop, ws, prec, reps=args.steps)
+            log(f"[{name}] energy {te * 1e3:.2f} ms, exchange "
+                f"{4.0 * NUP * NUP * N * M * W / te / 1e12:.1f} TFLOP/s")
+            del ctx, ham, prop, ws
+            torch.cuda.empty_cache()
         return
     if args.only == "estimators":
         log("back-propagation and mixed estimators on BH")
@@ -366,8 +392,10 @@ def main():
         log(f"[{name}] timing {args.steps} steps after {args.warmup} warmup")
         sec = time_steps(prop, ws, Esh, eout, args.warmup, args.steps)
         tv = time_vhs(ham, prec, W)
-        out[name] = dict(sec=sec, vhs=tv)
-        log(f"[{name}] {sec * 1e3:.2f} ms/step, {W / sec:.4g} walker-steps/s, vHS {tv * 1e3:.3f} ms")
+        te = time_energy(prop, ws, prec) if args.only is None else None
+        out[name] = dict(sec=sec, vhs=tv, energy=te)
+        log(f"[{name}] {sec * 1e3:.2f} ms/step, {W / sec:.4g} walker-steps/s, vHS {tv * 1e3:.3f} ms"
+            + (f", energy {te * 1e3:.2f} ms" if te else ""))
         del ctx, ham, prop, ws
         torch.cuda.empty_cache()
         if args.only == "step":
@@ -441,6 +469,12 @@ def main():
             "vhs_ms": out["fp64"]["vhs"] * 1e3,
             "step_roofline_frac": (fstep * W / out["fp64"]["sec"]) / PEAK["fp64"]},
         "mixed_over_fp64": None if "fp64" not in out else out["fp64"]["sec"] / sec,
+        # one local-energy evaluation of all walkers (measured every measure_interval steps);
+        # the exchange contraction dominates: 4 n^2 N M FLOP per walker with the real trial
+        "energy": None if out["mixed"].get("energy") is None else {
+            name: {"ms": out[name]["energy"] * 1e3,
+                   "exchange_tflops": 4.0 * NUP * NUP * N * M * W / out[name]["energy"] / 1e12}
+            for name in out if out[name].get("energy")},
         "energy_error_mixed_vs_fp64_Ha": None if emax is None else
             {"max": emax, "median": emed, "max_rel": erel, "median_abs_E": emag},
         "energy_error_mixed_vs_fp64_BH": bh_err,

@@ -103,6 +103,14 @@ void dot_rows(hipStream_t s, int W, long K, const z64* G, const z64* H, double c
               double add, z64* out, long out_stride);
 void exchange_reduce(hipStream_t s, int W, int n, int MC, int m0, int M, const z64* T,
                      z64* exx_part, int nchunk, int chunk_id, z64* Kl);
+// Cholesky vectors per workgroup of exchange_blocks for n electrons
+int exchange_group(int n);
+// exchange / Coulomb partial sums of a chunk of mcount Cholesky vectors starting at m0, from
+// T [W][n][MC][n] (complex TT): exx_part[w][g0 + g] for the chunk's groups g of exchange_group(n)
+// vectors (ng_total partials per walker), Kl[w][m0 + m] += tr T^m (Kl may be null)
+template <typename TT>
+void exchange_blocks(hipStream_t s, int W, int n, int MC, int m0, int mcount, int M,
+                     const cplx<TT>* T, z64* exx_part, int ng_total, int g0, z64* Kl);
 void energy_finish(hipStream_t s, int W, int M, int nchunk, const z64* exx_part, const z64* Kl,
                    double cx, double cj, int addEXX, int addEJ, z64* E);
 

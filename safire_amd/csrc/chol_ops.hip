@@ -256,24 +256,30 @@ void CholeskyOps::energy(const z64* G, int W, z64* E, bool addH1, bool addEJ, bo
   if (!addEXX) return;
   z64* Kl = scratch_as<z64>(ctx_, "en_Kl", (size_t)W * M_);
   fill_z64(s, (long)W * M_, {0, 0}, Kl);
-  // chunk of Cholesky vectors: T chunk of W*ns*MC*ns complex within a 1 GiB budget
+  // chunks of Cholesky vectors: T chunk of W*ns*MC*ns complex (FP32 in mixed) within 1 GiB, MC a
+  // multiple of the exchange group so that the groups (one partial sum per walker each) tile M
   const int nspin = walker_type_ == kCollinear ? 2 : 1;
-  int nmax = std::max(nup_, ndown_);
-  long per_m = (long)W * nmax * nmax * (long)sizeof(z64);
+  const int nmax = std::max(nup_, ndown_);
+  const bool t32 = prec != kFP64;
+  const size_t tsz = t32 ? sizeof(c32) : sizeof(z64);
+  const int mg = exchange_group(std::max(nmax, 1));
+  const long per_m = (long)W * nmax * nmax * (long)tsz;
   int MC = (int)std::max<long>(1, std::min<long>(M_, (1L << 30) / std::max<long>(per_m, 1)));
+  MC = std::max(mg, MC / mg * mg);
   const int nchunk = ceil_div(M_, MC);
-  z64* T = scratch_as<z64>(ctx_, "en_T", (size_t)W * nmax * nmax * MC);
-  z64* part = scratch_as<z64>(ctx_, "en_part", (size_t)W * nchunk * nspin);
+  const int ng_total = ceil_div(M_, mg);
+  void* T = ctx_.scratch("en_T", (size_t)W * nmax * nmax * MC * tsz);
+  z64* part = scratch_as<z64>(ctx_, "en_part", (size_t)W * ng_total * nspin);
   z64* Gs = nullptr;
   c32* G32 = nullptr;
   for (int sp = 0; sp < nspin; ++sp) {
     int ns = sp == 0 ? nup_ : ndown_;
     if (ns == 0) {
-      fill_z64(s, (long)W * nchunk, {0, 0}, part + (size_t)sp * W * nchunk);
+      fill_z64(s, (long)W * ng_total, {0, 0}, part + (size_t)sp * W * ng_total);
       continue;
     }
     size_t goff = sp == 0 ? 0 : (size_t)nup_ * N_;
-    const z64* Lsp = Lnak_ + (sp == 0 ? 0 : (size_t)M_ * nup_ * N_);
+    const size_t loff = sp == 0 ? 0 : (size_t)M_ * nup_ * N_;
     // contiguous spin block of G: [W][ns][N]
     const z64* Gblk = G;
     if (nspin == 2) {
@@ -285,36 +291,44 @@ void CholeskyOps::energy(const z64* G, int W, z64* E, bool addH1, bool addEJ, bo
     for (int ch = 0; ch < nchunk; ++ch) {
       int m0 = ch * MC;
       int mc = std::min(MC, M_ - m0);
+      const size_t lc = loff + (size_t)m0 * ns * N_;
       // T[(w,b)][(m,a)] = sum_k G[w][b][k] Lnak[m0+m][a][k]
-      EpiStore<double, double> e{T, (long)MC * ns, 0, 1, 0, 0};
-      if (prec == kFP64) {
+      if (!t32) {
+        EpiStore<double, double> e{static_cast<z64*>(T), (long)MC * ns, 0, 1, 0, 0};
         Opnd<double, true, true> A{&Gblk->re, N_, 0};
-        Opnd<double, true, true> B{&(Lsp + (size_t)m0 * ns * N_)->re, N_, 0};
-        gemm_large<double>(ctx_, "exx_chunk", 1, W * ns, mc * ns, N_, A, B, e);
+        if (real_trial_) {
+          Opnd<double, false, true> B{Lnak_r_ + lc, N_, 0};
+          gemm_large<double>(ctx_, "exx_chunk_r", 1, W * ns, mc * ns, N_, A, B, e);
+        } else {
+          Opnd<double, true, true> B{&(Lnak_ + lc)->re, N_, 0};
+          gemm_large<double>(ctx_, "exx_chunk", 1, W * ns, mc * ns, N_, A, B, e);
+        }
+        exchange_blocks<double>(s, W, ns, MC, m0, mc, M_, static_cast<const z64*>(T),
+                                part + (size_t)sp * W * ng_total, ng_total, m0 / mg,
+                                addEJ ? Kl : nullptr);
       } else {
         AFQ_CHECK(prec_ == kMixed, kErrState, "FP32 tensors were not built (precision FP64)");
         if (!G32) G32 = scratch_as<c32>(ctx_, "en_G32", (size_t)W * ns * N_);
         if (ch == 0) cast_z64_c32(s, (long)W * ns * N_, Gblk, G32);
-        EpiStore<float, double> ef{T, (long)MC * ns, 0, 1.f, 0.f, 0};
+        EpiStore<float, float> ef{static_cast<c32*>(T), (long)MC * ns, 0, 1.f, 0.f, 0};
         Opnd<float, true, true> A{&G32->re, N_, 0};
         if (real_trial_) {
-          Opnd<float, false, true> B{Lnak_r32_ + (sp == 0 ? 0 : (size_t)M_ * nup_ * N_) +
-                                         (size_t)m0 * ns * N_, N_, 0};
+          Opnd<float, false, true> B{Lnak_r32_ + lc, N_, 0};
           gemm_large<float>(ctx_, "exx_chunk_r", 1, W * ns, mc * ns, N_, A, B, ef);
         } else {
-          Opnd<float, true, true> B{&(Lnak32_ + (sp == 0 ? 0 : (size_t)M_ * nup_ * N_) +
-                                      (size_t)m0 * ns * N_)->re, N_, 0};
+          Opnd<float, true, true> B{&(Lnak32_ + lc)->re, N_, 0};
           gemm_large<float>(ctx_, "exx_chunk_c", 1, W * ns, mc * ns, N_, A, B, ef);
         }
+        exchange_blocks<float>(s, W, ns, MC, m0, mc, M_, static_cast<const c32*>(T),
+                               part + (size_t)sp * W * ng_total, ng_total, m0 / mg,
+                               addEJ ? Kl : nullptr);
       }
-      exchange_reduce(s, W, ns, MC, m0, M_, T, part + (size_t)sp * W * nchunk, nchunk, ch,
-                      addEJ ? Kl : nullptr);
     }
     G32 = nullptr;
   }
   // E[w][1] = -1/2 c sum exx;  E[w][2] = 1/2 c^2 sum_m Kl^2  (energy :169-173)
   for (int sp = 0; sp < nspin; ++sp)
-    energy_finish(s, W, M_, nchunk, part + (size_t)sp * W * nchunk, Kl, -0.5 * c, 0.5 * c * c,
+    energy_finish(s, W, M_, ng_total, part + (size_t)sp * W * ng_total, Kl, -0.5 * c, 0.5 * c * c,
                   1, (addEJ && sp == nspin - 1) ? 1 : 0, E);
 }
 

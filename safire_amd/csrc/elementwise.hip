@@ -2,6 +2,8 @@
 // The reference's equivalents are the copy/accumulate/add_scalar/zero_imag device kernels
 // (numerics/device_kernels/cuda/*.cu) and the host loops of energy_impl
 // (Real3IndexFactorization.hpp:529-587).
+#include <algorithm>
+
 #include "afq_internal.hpp"
 
 namespace afq {
@@ -207,6 +209,57 @@ __global__ void __launch_bounds__(kT)
   }
 }
 
+// Exchange and Coulomb vectors of a group of mg Cholesky vectors per workgroup.  T holds, per
+// walker, rows b (electrons) of ldt = MC n elements: T[w][b][m][a] = sum_k G[w][b][k] L[m][a][k]
+// for the chunk's m (local index).  Workgroup (w, g) copies rows b of its mg blocks into LDS
+// (mg n contiguous elements per row, one coalesced segment each), then
+//   exx_part[w][g0 + g] = sum_{m, a, b} T^m[b][a] T^m[a][b]   (fixed order, FP64)
+//   Kl[w][m0 + m] += sum_a T^m[a][a]
+template <typename TT>
+__global__ void __launch_bounds__(kT)
+    k_exchange_blocks(int n, int MC, int mcount, int mg, int m0, int M, const cplx<TT>* T,
+                      z64* exx_part, int ng_total, int g0, z64* Kl) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  TT* tr = reinterpret_cast<TT*>(smem_raw);
+  const int w = blockIdx.x, g = blockIdx.y;
+  const int mb = g * mg, me = min(mcount, mb + mg), cnt = me - mb;
+  const int rl = cnt * n, ld = rl + 1;  // row length in LDS (+1 against bank conflicts)
+  TT* ti = tr + (long)n * ld;
+  const long ldt = (long)MC * n;
+  const cplx<TT>* Tw = T + (long)w * n * ldt + (long)mb * n;
+  for (int t = threadIdx.x; t < n * rl; t += kT) {
+    const int b = t / rl, c = t % rl;
+    const cplx<TT> v = Tw[(long)b * ldt + c];
+    tr[b * ld + c] = v.re;
+    ti[b * ld + c] = v.im;
+  }
+  __syncthreads();
+  __shared__ double red[kT / 64];
+  double er = 0, ei = 0;
+  const int nn = n * n;
+  for (int t = threadIdx.x; t < cnt * nn; t += kT) {
+    const int m = t / nn, r = t % nn, b = r / n, a = r % n;
+    const int i1 = b * ld + m * n + a, i2 = a * ld + m * n + b;  // T^m[b][a], T^m[a][b]
+    const double xr = tr[i1], xi = ti[i1], yr = tr[i2], yi = ti[i2];
+    er += xr * yr - xi * yi;
+    ei += xr * yi + xi * yr;
+  }
+  er = block_sum<kT>(er, red);
+  ei = block_sum<kT>(ei, red);
+  if (threadIdx.x == 0) exx_part[(long)w * ng_total + g0 + g] = {er, ei};
+  if (Kl) {
+    for (int m = threadIdx.x; m < cnt; m += kT) {
+      double kr = 0, ki = 0;
+      for (int a = 0; a < n; ++a) {
+        kr += tr[a * ld + m * n + a];
+        ki += ti[a * ld + m * n + a];
+      }
+      z64 o = Kl[(long)w * M + m0 + mb + m];
+      Kl[(long)w * M + m0 + mb + m] = {o.re + kr, o.im + ki};
+    }
+  }
+}
+
 // E[w][1] += cx * sum_chunks exx;  E[w][2] += cj * sum_m Kl^2
 __global__ void __launch_bounds__(kT)
     k_energy_finish(int M, int nchunk, const z64* exx_part, const z64* Kl, double cx, double cj,
@@ -357,6 +410,32 @@ void exchange_reduce(hipStream_t s, int W, int n, int MC, int m0, int M, const z
                      nchunk, chunk_id, Kl);
   AFQ_LAUNCH_CHECK();
 }
+int exchange_group(int n) {
+  // Cholesky vectors per workgroup: the blocks' FP64 re/im planes within 64 KB of LDS
+  const long per = 2L * sizeof(double) * n * (n + 1);
+  return (int)std::max<long>(1, std::min<long>(16, 65536 / per));
+}
+
+template <typename TT>
+void exchange_blocks(hipStream_t s, int W, int n, int MC, int m0, int mcount, int M,
+                     const cplx<TT>* T, z64* exx_part, int ng_total, int g0, z64* Kl) {
+  if (W <= 0 || mcount <= 0) return;
+  const int mg = exchange_group(n);
+  const int ng = ceil_div(mcount, mg);
+  const size_t lds = 2 * sizeof(TT) * (size_t)n * (mg * n + 1);
+  AFQ_CHECK(lds <= 160 * 1024 && g0 + ng <= ng_total, kErrShape, "exchange block layout");
+  if (lds > 65536)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_exchange_blocks<TT>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(k_exchange_blocks<TT>, dim3(W, ng), dim3(kT), lds, s, n, MC, mcount, mg, m0,
+                     M, T, exx_part, ng_total, g0, Kl);
+  AFQ_LAUNCH_CHECK();
+}
+template void exchange_blocks<double>(hipStream_t, int, int, int, int, int, int, const z64*, z64*,
+                                      int, int, z64*);
+template void exchange_blocks<float>(hipStream_t, int, int, int, int, int, int, const c32*, z64*,
+                                     int, int, z64*);
+
 void energy_finish(hipStream_t s, int W, int M, int nchunk, const z64* exx_part, const z64* Kl,
                    double cx, double cj, int addEXX, int addEJ, z64* E) {
   if (W <= 0) return;
