@@ -319,7 +319,10 @@ void chol_upper_inverse(hipStream_t s, const z64* S, int nb, int n, z64* Rinv, z
   AFQ_CHECK(n >= 1 && n <= 64, kErrShape, "n must be in [1, 64]");
   if (nb <= 0) return;
   size_t lds = sizeof(double2) * ((size_t)n * (n + 1) + n);
-  AFQ_CHECK(lds <= 65536, kErrShape, "matrix too large for one workgroup's LDS");
+  AFQ_CHECK(lds <= 160 * 1024, kErrShape, "matrix too large for one workgroup's LDS");
+  if (lds > 65536)  // n = 64
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_chol_blk),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   // R is written into Rinv, then inverted in place (upper triangular: no pivoting needed);
   // the inversion's log det is sum_i log R_ii (R_ii real and positive).
   hipLaunchKernelGGL(k_chol_blk, dim3(nb), dim3(kLuThreads), lds, s, S, n, Rinv);

@@ -523,3 +523,51 @@ def test_graph_with_population_control_matches_eager(gpu):
         out[mode] = (afq.host(ws.Phi), afq.host(ws.weight), afq.host(Esh))
     for a, b in zip(out["eager"], out["graph"]):
         assert np.array_equal(a, b)
+
+
+# ---- shapes at the edges of the tilings -------------------------------------------------------
+
+ODD = [(7, 1, 3, 1), (17, 3, 5, 5), (33, 16, 40, 3), (65, 64, 10, 2), (16, 16, 16, 4)]
+
+
+@pytest.mark.parametrize("N,n,M,W", ODD, ids=[f"N{a}-n{b}-M{c}-W{d}" for a, b, c, d in ODD])
+def test_edge_shapes_vs_oracle(gpu, N, n, M, W):
+    """Orbitals, electrons, Cholesky vectors and walkers that fill no 16-wide tile (a single
+    electron, a single walker, n = N, n = 64, N one past a tile): the energy and vbias of
+    perturbed walkers and three full steps with a QR in the first, against the oracle."""
+    afq, ctx0, torch = gpu
+    say(f"case: edge shape N={N} n={n} M={M} W={W}")
+    p = make_problem(N, n, M, seed=2000 + N + n)
+    ctx = afq.Context(0)
+    ham = afq.HamiltonianOperations(ctx, p.h1, p.Likn, p.PsiT, n)
+    osys = orc.OracleSystem(p, dt=0.005)
+    Phi = random_walkers(p, W, seed=9)
+    Gr, _ = orc.mixed_density_matrix(p.PsiT, Phi)
+    dG = afq.device_tensor(Gr)
+    v = afq.empty((W, M), torch.complex128)
+    ham.vbias(dG, v, 0.005, precision=afq.AFQ_FP64)
+    E = afq.empty((W, 3), torch.complex128)
+    ham.energy(E, dG, precision=afq.AFQ_FP64)
+    Er = orc.energy(osys.haj, osys.Lnak, Gr, n)
+    ev = rel(afq.host(v), orc.vbias(osys.Lnak, Gr, 0.005))
+    ee = max(rel(afq.host(E)[:, c], Er[:, c]) for c in range(3))
+    say(f"  vbias {ev:.2e}  energy {ee:.2e}")
+    assert ev < TOL and ee < TOL
+    prop = afq.Propagator(ham, W, dt=0.005)
+    st, Eshift = osys.init_walkers(Phi)
+    ws = afq.WalkerSet(st["Phi"])
+    for k in ("weight", "phase", "eloc", "ovlp", "theta"):
+        getattr(ws, k).copy_(torch.from_numpy(st[k]))
+    Esh = afq.device_tensor(np.array([Eshift]))
+    for step in range(3):
+        u = orc.field_uniforms(5, 1, step, W, M)
+        prop.Propagate(ws, Esh, u=afq.device_tensor(u))
+        osys.step(st, u, Eshift)
+        if step == 0:
+            prop.Orthogonalize(ws)
+            osys.orthogonalize(st)
+        e = max([rel(afq.host(ws.Phi), st["Phi"])] +
+                [rel(afq.host(getattr(ws, k)), st[k]) for k in ("ovlp", "eloc", "weight")])
+        say(f"  step {step}: {e:.2e}")
+        assert e < TOL, step
+    prop.close()
