@@ -313,7 +313,7 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--only", choices=["vhs", "vhs64", "copy", "step", "step64", "phmsd", "vendor",
-                                       "estimators", "energy"],
+                                       "estimators", "energy", "serve"],
                     default=None,
                     help="profiling modes: launch only the named kernel(s)")
     ap.add_argument("--serve-jobs", type=int, default=24,
@@ -385,6 +385,8 @@ def main():
 
     out = {}
     for prec, name in ((afq.AFQ_MIXED, "mixed"), (afq.AFQ_FP64, "fp64")):
+        if args.only == "serve":
+            break
         if (name == "fp64" and args.skip_fp64) or (name == "mixed" and args.only == "step64"):
             continue
         log(f"[{name}] setup")
@@ -401,7 +403,8 @@ def main():
         if args.only == "step":
             break
 
-    log("autotuned tiles:\n" + afq.tuning_report())
+    if args.only != "serve":
+        log("autotuned tiles:\n" + afq.tuning_report())
     if args.only == "step64":
         return
     emax = emed = erel = emag = None
@@ -420,17 +423,25 @@ def main():
         log(f"phmsd: {phmsd}")
 
     serve_summary = None
-    if args.only is None and args.serve_jobs > 0:
+    if args.only in (None, "serve") and args.serve_jobs > 0:
         from safire_amd import serve
         log(f"serving run: {args.serve_jobs} jobs, Poisson arrivals at 4 jobs/s, 400 mixed steps each")
         specs = serve.poisson_jobs(args.serve_jobs, rate=4.0)
-        problems = {sp.id: make_problem(sp.N, sp.nup, sp.M, sp.seed, lscale=sp.lscale) for sp in specs}
-        res = serve.Server(max_active=16).run(specs, problems)
+        problems = {}
+        for i, sp in enumerate(specs):
+            problems[sp.id] = make_problem(sp.N, sp.nup, sp.M, sp.seed, lscale=sp.lscale)
+            if (i + 1) % 50 == 0:
+                log(f"  generated {i + 1} job inputs")
+        res = serve.Server(max_active=16).run(
+            specs, problems, log=log if args.serve_jobs > 50 else None)
         serve_summary = serve.summarize(res)
         serve_summary.update({"arrival_rate_per_s": 4.0, "steps_per_job": 400,
                               "workload": "N in {32..160 step 16}, n=N/4, M=5N, W in [64,512], "
                                           "Cholesky amplitude 0.1, mixed precision"})
         log(f"serve: {serve_summary}")
+        if args.only == "serve":
+            print(json.dumps(serve_summary))
+            return
 
     cpu = None
     if not args.skip_cpu and args.only is None:
