@@ -240,6 +240,34 @@ void CholeskyOps::vHS(const void* X, int W, double dt, void* V, Precision prec) 
   }
 }
 
+void CholeskyOps::vbias32(const c32* G, int W, double dt, z64* v) const {
+  if (W == 0) return;
+  AFQ_CHECK(prec_ == kMixed, kErrState, "FP32 tensors were not built (precision FP64)");
+  hipStream_t s = ctx_.stream();
+  const float a = (float)(std::sqrt(dt) * (walker_type_ == kClosed ? 2.0 : 1.0));
+  const int nel_ = nel();
+  for (int sp = 0; sp < (walker_type_ == kCollinear ? 2 : 1); ++sp) {
+    int ns = sp == 0 ? nup_ : ndown_;
+    int acc = (sp == 1 && nup_ > 0) ? 1 : 0;
+    if (ns == 0) {
+      if (sp == 0) fill_z64(s, (long)W * M_, {0, 0}, v);
+      continue;
+    }
+    size_t goff = sp == 0 ? 0 : (size_t)nup_ * N_;
+    size_t loff = sp == 0 ? 0 : (size_t)M_ * nup_ * N_;
+    const long K = (long)ns * N_;
+    Opnd<float, true, true> A{&(G + goff)->re, (long)nel_ * N_, 0};
+    EpiStore<float, double> e{v, M_, 0, a, 0.f, acc};
+    if (real_trial_) {
+      Opnd<float, false, true> B{Lnak_r32_ + loff, K, 0};
+      gemm_large_splitk<float>(ctx_, "vbias32_r", W, M_, (int)K, A, B, e, acc == 0);
+    } else {
+      Opnd<float, true, true> B{&(Lnak32_ + loff)->re, K, 0};
+      gemm_large_splitk<float>(ctx_, "vbias32_c", W, M_, (int)K, A, B, e, acc == 0);
+    }
+  }
+}
+
 void CholeskyOps::energy(const z64* G, int W, z64* E, bool addH1, bool addEJ, bool addEXX,
                          Precision prec) const {
   if (W == 0) return;

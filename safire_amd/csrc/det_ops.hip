@@ -88,7 +88,9 @@ void log_overlap(Context& c, const z64* PsiT, int n, int N, const z64* Phi, long
 }
 
 void mixed_density_matrix(Context& c, const z64* PsiT, int n, int N, const z64* Phi, long phi_bs,
-                          int W, z64* G, long g_bs, z64* ovlp, bool compact, double scale) {
+                          int W, z64* G, long g_bs, z64* ovlp, bool compact, double scale,
+                          c32* G32) {
+  AFQ_CHECK(!G32 || compact, kErrArg, "the complex64 copy is for the compact density matrix");
   if (W == 0) return;
   hipStream_t s = c.stream();
   if (n == 0) {
@@ -109,7 +111,10 @@ void mixed_density_matrix(Context& c, const z64* PsiT, int n, int N, const z64* 
   {
     Opnd<double, true, false> A{&T->re, n, (long)n * n};
     Opnd<double, true, true> B{&Phi->re, n, phi_bs};
-    gemm_small<double>(c, "mdm_G", W, n, N, n, A, B, EpiStore<double, double>{Gc, N, gc_bs, 1, 0, 0});
+    if (G32)
+      gemm_small<double>(c, "mdm_G", W, n, N, n, A, B, EpiStore<double, float>{G32, N, g_bs, 1, 0, 0});
+    else
+      gemm_small<double>(c, "mdm_G", W, n, N, n, A, B, EpiStore<double, double>{Gc, N, gc_bs, 1, 0, 0});
   }
   if (!compact) {
     // G[i][k] = sum_a PsiT[a][i] Gc[a][k]           (product<'T'>(A, TNM, C))

@@ -57,9 +57,11 @@ namespace det_ops {
 void log_overlap(Context& c, const z64* PsiT, int n, int N, const z64* Phi, long phi_bs, int W,
                  z64* ovlp, double scale = 1.0, z64 add = {0, 0});
 // MixedDensityMatrix: compact G [W][n][N] or full G [W][N][N]; ovlp accumulated as in
-// log_overlap.
+// log_overlap.  G32 (compact only): G is computed in FP64 and stored rounded to complex64 at
+// G32 (same layout) instead of G.
 void mixed_density_matrix(Context& c, const z64* PsiT, int n, int N, const z64* Phi, long phi_bs,
-                          int W, z64* G, long g_bs, z64* ovlp, bool compact, double scale = 1.0);
+                          int W, z64* G, long g_bs, z64* ovlp, bool compact, double scale = 1.0,
+                          c32* G32 = nullptr);
 // inverse_logdet (density_matrix.hpp:82-111)
 void inverse_logdet(Context& c, const z64* A, int nb, int n, z64* ovlp, z64* Ainv);
 // Full mixed density matrices with a left state per walker (the 3-d MixedDensityMatrix of
@@ -131,6 +133,9 @@ class CholeskyOps {
   // v[w][m] = sqrt(dt) c sum_{a,k} G[w][a][k] Lnak[m][a][k]   (vbias, :328-428)
   // prec == kMixed runs the contraction in FP32 and writes FP64 v.
   void vbias(const z64* G, int W, double dt, z64* v, Precision prec) const;
+  // vbias of a complex64 copy of G (FP32 MFMA; the same values as vbias(G, ..., kMixed), which
+  // rounds G to FP32 while staging it)
+  void vbias32(const c32* G, int W, double dt, z64* v) const;
   // V[w][i][k] = sqrt(dt) sum_m L[i][k][m] X[w][m]   (vHS, :282-326); V is z64 (FP64) or c32
   void vHS(const void* X, int W, double dt, void* V, Precision prec) const;
   // E[w][0..2] = E1, EXX, EJ  (energy :138-174, energy_impl :476-590); G compact [W][nel][N]

@@ -135,19 +135,26 @@ void Propagator::step_impl(const WalkerState& st, const double* u, const double*
   z64* nov = scratch_as<z64>(ctx_, "st_nov", W);
   void* V = ctx_.scratch("st_V", (size_t)W * N * N * (prec_ == kMixed ? sizeof(c32) : sizeof(z64)));
 
-  // 1. G and vbias (NOMSD::vbias, NOMSD.cpp:39-61; PHMSD::vbias, PHMSD.cpp:53-72)
+  // 1. G and vbias (NOMSD::vbias, NOMSD.cpp:39-61; PHMSD::vbias, PHMSD.cpp:53-72).  In mixed
+  // precision G is computed in FP64 and stored as complex64 for the FP32 vbias contraction (the
+  // values the contraction would round it to anyway; half the bytes it re-reads).
   if (trial_) {
     trial_->mixed_density_matrix(st.Phi, st.Phi_b, W, G, ov);
+    ham_.vbias(G, W, p_.dt, X, prec_);
   } else {
+    c32* G32 = prec_ == kMixed ? scratch_as<c32>(ctx_, "st_G32", (size_t)W * nel * N) : nullptr;
     fill_z64(s, W, {0, 0}, ov);
     det_ops::mixed_density_matrix(ctx_, ham_.PsiT(), nup, N, st.Phi, (long)N * nup, W, G,
-                                  (long)nel * N, ov, true, closed ? 2.0 : 1.0);
+                                  (long)nel * N, ov, true, closed ? 2.0 : 1.0, G32);
     if (!closed && ndown > 0)
       det_ops::mixed_density_matrix(ctx_, ham_.PsiT() + (size_t)nup * N, ndown, N, st.Phi_b,
                                     (long)N * ndown, W, G + (size_t)nup * N, (long)nel * N, ov,
-                                    true);
+                                    true, 1.0, G32 ? G32 + (size_t)nup * N : nullptr);
+    if (G32)
+      ham_.vbias32(G32, W, p_.dt, X);
+    else
+      ham_.vbias(G, W, p_.dt, X, prec_);
   }
-  ham_.vbias(G, W, p_.dt, X, prec_);
   // 2. fields, hybrid weight, mean-field factor
   FieldParams fp{p_.vbias_bound, (p_.use_cp || p_.project_force_bias) ? 1 : 0, p_.free_projection,
                  p_.seed, p_.stream, 0, u ? nullptr : step_dev_};
