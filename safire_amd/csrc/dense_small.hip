@@ -31,8 +31,9 @@ __device__ inline void clog(double re, double im, double& lr, double& li) {
 // Rows are never moved: each lane tracks the position its row would occupy in getrf's swapped
 // order, the pivot of step j is the unused row of largest |re|+|im| in column j with ties going
 // to the lowest position (getrf's first maximum), and the row at position j trades positions
-// with it. Gauss-Jordan then eliminates column j from every other row with the pivot row
-// broadcast through LDS, so the only dynamically indexed register is column j. The log det
+// with it. Gauss-Jordan then eliminates column j from every other row with the pivot row read
+// from the pivot lane's registers (readlane), so the only dynamically indexed register is
+// column j. The log det
 // takes log(+-U_jj + DBL_MIN) with the sign flipped exactly on getrf's swapped steps; the
 // in-place result Y gives A^{-1}[k][c] = Y[p_k][s_c] (p_k = pivot row of step k, s_c = step at
 // which row c pivoted), applied while storing.
@@ -49,7 +50,6 @@ __global__ void __launch_bounds__(64)
               z64* __restrict__ ovlp, double ov_scale, double ov_add_re, double ov_add_im,
               int want_inverse, int no_pivot) {
   __shared__ int pivrow[64];
-  __shared__ double2 prow[NMAX];
   const int lane = threadIdx.x;
   const long b = blockIdx.x;
   const z64* Ab = A + b * a_bs;
@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(64)
   // Steps j = CH * J + jj: the chunk J is a loop variable, jj is unrolled, so column j of the
   // register arrays is selected among NMAX / CH candidates and the column update needs a
   // select only on the NMAX / CH columns congruent to jj.
-  constexpr int CH = NMAX < 16 ? NMAX : 16;
+  constexpr int CH = NMAX < 16 ? NMAX : (NMAX % 16 == 0 ? 16 : 8);
   constexpr int NCH = NMAX / CH;
   static_assert(NMAX % CH == 0, "NMAX must be a multiple of the chunk width");
   for (int J = 0; J * CH < n; ++J) {
@@ -118,29 +118,24 @@ __global__ void __launch_bounds__(64)
       }
       const double den = dr * dr + di * di;
       const double ir = dr / den, ii = -di / den;
-      // h = g / d with g = d - 1 on the pivot row (which becomes prow / d), f elsewhere
+      // h = g / d with g = d - 1 on the pivot row (which becomes the pivot row / d), f elsewhere
       const double gr = (lane == p) ? dr - 1.0 : fr, gi = (lane == p) ? di : fi;
       const double hr = gr * ir - gi * ii, hi = gr * ii + gi * ir;
       // column j of the result: 1/d on the pivot row, -f/d elsewhere
       const double njr = (lane == p) ? ir : -(fr * ir - fi * ii);
       const double nji = (lane == p) ? ii : -(fr * ii + fi * ir);
-      // the pivot row goes through LDS (one writer lane, broadcast reads)
-      if (lane == p) {
-#pragma unroll
-        for (int c = 0; c < NMAX; ++c) prow[c] = make_double2(xr[c], xi[c]);
-      }
-      __syncthreads();
+      // the pivot row is read from the pivot lane's registers (scalar reads of one lane, no
+      // LDS round trip on the step's critical path)
 #pragma unroll
       for (int c = 0; c < NMAX; ++c) {
-        const double2 pv = prow[c];
+        const double pr = rdl(xr[c], p), pim = rdl(xi[c], p);
         // x - h p as four fused multiply-adds
-        double nr = fma(hi, pv.y, fma(-hr, pv.x, xr[c]));
-        double ni = fma(-hi, pv.x, fma(-hr, pv.y, xi[c]));
+        double nr = fma(hi, pim, fma(-hr, pr, xr[c]));
+        double ni = fma(-hi, pr, fma(-hr, pim, xi[c]));
         if (c % CH == jj && c / CH == J) { nr = njr; ni = nji; }
         xr[c] = nr;
         xi[c] = ni;
       }
-      __syncthreads();
     }
   }
   if (ovlp != nullptr) {
@@ -308,6 +303,8 @@ void lu_inverse_logdet(hipStream_t s, const z64* A, long a_bs, int nb, int n, z6
     AFQ_LU_ROWS(16, 0);
   else if (n <= 32)
     AFQ_LU_ROWS(32, 0);
+  else if (n <= 40)
+    AFQ_LU_ROWS(40, 0);
   else if (n <= 48)
     AFQ_LU_ROWS(48, 0);
   else
@@ -336,6 +333,9 @@ void chol_upper_inverse(hipStream_t s, const z64* S, int nb, int n, z64* Rinv, z
                        0.0, 0.0, 1, 1);
   else if (n <= 32)
     hipLaunchKernelGGL(k_lu_rows<32>, dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet, 1.0,
+                       0.0, 0.0, 1, 1);
+  else if (n <= 40)
+    hipLaunchKernelGGL(k_lu_rows<40>, dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet, 1.0,
                        0.0, 0.0, 1, 1);
   else if (n <= 48)
     hipLaunchKernelGGL(k_lu_rows<48>, dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet, 1.0,
