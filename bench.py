@@ -403,6 +403,19 @@ def main():
         if args.only == "step":
             break
 
+    shard = None
+    if args.only is None:
+        # one GPU's share of an 8-GPU node: the same workload with W / 8 walkers per GPU
+        Ws = W // 8
+        log(f"[mixed] {Ws} walkers (one GPU's share of {W} over 8 GPUs)")
+        ctx, ham, prop, ws, Esh, eout = gpu_setup(problem, afq.AFQ_MIXED, Ws)
+        ss = time_steps(prop, ws, Esh, eout, args.warmup, args.steps)
+        shard = {"walkers": Ws, "ms_per_step": ss * 1e3, "value": Ws / ss,
+                 "unit": "walker-steps/s", "node_of_8_if_linear": 8 * Ws / ss}
+        log(f"[mixed] W={Ws}: {ss * 1e3:.3f} ms/step, {Ws / ss:.4g} walker-steps/s")
+        del ctx, ham, prop, ws
+        torch.cuda.empty_cache()
+
     if args.only != "serve":
         log("autotuned tiles:\n" + afq.tuning_report())
     if args.only == "step64":
@@ -482,10 +495,16 @@ def main():
         "mixed_over_fp64": None if "fp64" not in out else out["fp64"]["sec"] / sec,
         # one local-energy evaluation of all walkers (measured every measure_interval steps);
         # the exchange contraction dominates: 4 n^2 N M FLOP per walker with the real trial
+        # walker energy evaluations/s, and the headline with one evaluation every 10 steps
+        # (the mixed estimator's default measure interval) charged to the steps
         "energy": None if out["mixed"].get("energy") is None else {
             name: {"ms": out[name]["energy"] * 1e3,
-                   "exchange_tflops": 4.0 * NUP * NUP * N * M * W / out[name]["energy"] / 1e12}
+                   "exchange_tflops": 4.0 * NUP * NUP * N * M * W / out[name]["energy"] / 1e12,
+                   "walker_evaluations_per_s": W / out[name]["energy"],
+                   "walker_steps_per_s_with_energy_every_10":
+                       W / (out[name]["sec"] + out[name]["energy"] / 10)}
             for name in out if out[name].get("energy")},
+        "shard_one_of_8": shard,
         "energy_error_mixed_vs_fp64_Ha": None if emax is None else
             {"max": emax, "median": emed, "max_rel": erel, "median_abs_E": emag},
         "energy_error_mixed_vs_fp64_BH": bh_err,

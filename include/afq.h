@@ -254,6 +254,35 @@ int afq_prop_mixed_measure(afq_prop* prop, const afq_walkers* w, double* onerdm,
 /* Number of reference determinants of the trial (Wavefunction::total_number_of_references). */
 int afq_prop_number_of_references(afq_prop* prop, int* nref);
 
+/* ---- back-propagation by operators evolved forward in time (TimeEvolvedBPEstimator.hpp) ------
+ * The operator states X, Y, M [W][nspin][N][N] (complex) of the propagator's W walker slots.
+ * They stay with the slots when population control copies walkers, as the reference's
+ * estimator-held arrays do.  All three calls need afq_prop_enable_backprop (the field and
+ * factor rings) and the full walker count W.
+ * afq_prop_te_anchor: TimeEvolvedBPEstimator::setAnchor (:271-276), X = Y = I, M = 0. */
+int afq_prop_te_anchor(afq_prop* prop);
+/* AFQMCBasePropagator::PropagateOperators (AFQMCBasePropagator.cpp:538-684): the nsteps most
+ * recent stored fields, oldest first, each X <- P1 expM(V) P1 X and
+ * Y <- P1inv expM(-V)^H P1inv Y (FP64, P1inv = exp(+1/2 H1) of generateP1(true)); then, with
+ * the walkers' Slater matrices S (orthonormal when the step was orthogonalized),
+ * Y <- S S^H Y, M += Y^H X, X <- X - S S^H X.  1 <= nsteps <= nback.  Synchronizes once. */
+int afq_prop_te_propagate(afq_prop* prop, const afq_walkers* w, int nsteps);
+/* TimeEvolvedBPEstimator::evolveAndMeasure steps 2-3 (:228-262) with
+ * constructTimeEvolvedMeasurementInputs (:49-146): per reference d of the trial,
+ * G_d = M + (Ref_d^H X)^T (Gc_d conj(Y)), Gc_d the compact mixed density matrix of Ref_d and
+ * the walker, coefficient conj(c_d) <Ref_d|Phi> normalized over the references, and WEIGHT
+ * times the path-restoration factors of the last restore_steps steps (bp_step, or 2 bp_step
+ * for extra_path_restoration, or 0).  transpose_m != 0 adds M^T in place of M: the reference
+ * accumulates M = sum Y^H X, whose transpose is the term that makes G_d equal to the
+ * back-propagated estimator of afq_prop_bp_measure; 0 reproduces the reference.  Outputs as
+ * afq_prop_bp_measure. */
+int afq_prop_te_measure(afq_prop* prop, const afq_walkers* w, int restore_steps, int transpose_m,
+                        double* onerdm, double* diag2rdm);
+/* Device views of X, Y, M (NULL before the first afq_prop_te_anchor) and of P1inv [N][N]
+ * (built by this call when needed). */
+int afq_prop_te_views(afq_prop* prop, const double** X, const double** Y, const double** M,
+                      const double** P1inv);
+
 /* ---- density-matrix observables --------------------------------------------------------------
  * One measurement of OneRDM (onerdm.hpp:43-70) and DiagonalTwoRDM (diagonal_twordm.hpp:40-81)
  * from per-walker density matrices G [W][nspin][N][N] (complex, device; the full mixed density

@@ -737,8 +737,9 @@ void orc_exp_hermitian(const double* A_, int N, double* out_) {
 // decomposition: H1 = dt*(h + vexx) + vHS(vMF) (one_body.hpp add_one_body_shifts), hermitize
 // (tensor.hpp:75-94), scale by -1/2, P1 = exp_hermitian(H1), then the CSR truncation
 // |P1_ij| <= 1e-8 -> 0 (csr_utils.hpp:43-66) kept as a dense matrix.
+// inverse != 0: exp(+1/2 H1), the P1x_inv of generateP1(true) (:118-123).
 void orc_generate_p1(const double* h_, const double* vexx, const double* Likn, const double* vMF_,
-                     int N, int M, double dt, double* P1_) {
+                     int N, int M, double dt, int inverse, double* P1_) {
   const cd* h = (const cd*)h_;
   std::vector<cd> H1((size_t)N * N), shift((size_t)N * N);
   orc_vhs(Likn, vMF_, 1, N, M, dt, (double*)shift.data());
@@ -752,7 +753,7 @@ void orc_generate_p1(const double* h_, const double* vexx, const double* Likn, c
         H1[(size_t)j * N + i] = std::conj(H1[(size_t)i * N + j]);
       }
     }
-  for (auto& x : H1) x *= -0.5;
+  for (auto& x : H1) x *= inverse ? 0.5 : -0.5;
   orc_exp_hermitian((const double*)H1.data(), N, P1_);
   cd* P1 = (cd*)P1_;
   for (size_t e = 0; e < (size_t)N * N; ++e)

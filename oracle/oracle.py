@@ -51,7 +51,7 @@ def lib():
             "orc_half_rotate": [_P, _P, _P, _I, _I, _I, _P, _P],
             "orc_vexx": [_P, _I, _I, _P],
             "orc_exp_hermitian": [_P, _I, _P],
-            "orc_generate_p1": [_P, _P, _P, _P, _I, _I, _D, _P],
+            "orc_generate_p1": [_P, _P, _P, _P, _I, _I, _D, _I, _P],
             "orc_vmf": [_P, _P, _I, _I, _I, _D, _D, _P],
             "orc_step": [_P, _I, _P, _D, _P, _P, _P, _P, _P, _P],
             "orc_num_threads": [],
@@ -290,7 +290,7 @@ def vmf(PsiT, Lnak, dt, c=2.0):
     return out
 
 
-def generate_p1(h1, vexx_, Likn, vMF, dt):
+def generate_p1(h1, vexx_, Likn, vMF, dt, inverse=False):
     h = _c(h1, np.complex128)
     vx = _c(vexx_, np.float64)
     Likn = _c(Likn, np.float64)
@@ -298,7 +298,7 @@ def generate_p1(h1, vexx_, Likn, vMF, dt):
     N = h.shape[0]
     M = Likn.shape[2]
     P1 = np.empty((N, N), np.complex128)
-    lib().orc_generate_p1(_p(h), _p(vx), _p(Likn), _p(vMF), N, M, dt, _p(P1))
+    lib().orc_generate_p1(_p(h), _p(vx), _p(Likn), _p(vMF), N, M, dt, int(inverse), _p(P1))
     return P1
 
 
@@ -400,6 +400,41 @@ def fold_diag2rdm(full, closed):
             for j in range(i + 1, N):
                 out.append((full[2, i, j] + full[2, j, i]) / 2)
     return np.array(out, np.complex128)
+
+
+# --- back-propagation by time-evolved operators (numpy over the oracle's kernels) ---------------
+
+def propagate_operators(P1, P1inv, Vs, X, Y, M, S, order=6):
+    """AFQMCBasePropagator::PropagateOperators (AFQMCBasePropagator.cpp:538-684) for one spin
+    block: for each V of Vs (oldest first) X <- P1 expM(V) P1 X ('N') and, with vHS scaled by
+    -1, Y <- P1inv expM(-V)^H P1inv Y ('H' with P1x_inv, :585-588); then the regularization
+    with the walker Slater matrices S [W][N][n] (:607-618): SY = S^H Y, Y <- S SY,
+    M += Y^H X, SY = S^H X, X <- X - S SY.  X, Y, M [W][N][N]; returns new (X, Y, M)."""
+    X = np.array(X, np.complex128)
+    Y = np.array(Y, np.complex128)
+    M = np.array(M, np.complex128)
+    for V in Vs:
+        X = propagate(P1, V, X, order, "N")
+        Y = propagate(P1inv, -np.asarray(V), Y, order, "H")
+    S = np.asarray(S, np.complex128)
+    for w in range(X.shape[0]):
+        SH = S[w].conj().T
+        Y[w] = S[w] @ (SH @ Y[w])
+        M[w] += Y[w].conj().T @ X[w]
+        X[w] = X[w] - S[w] @ (SH @ X[w])
+    return X, Y, M
+
+
+def time_evolved_density_matrix(RefT, X, Y, M, Phi, transpose_m=False):
+    """One spin block of constructTimeEvolvedMeasurementInputs (TimeEvolvedBPEstimator.hpp:
+    104-131): G = M + (Ref^H X)^T (Gc conj(Y)), with Gc the compact mixed density matrix of the
+    reference (RefT = Ref^H [n][N]) and the walker Phi [W][N][n].  transpose_m adds M^T."""
+    Gc, _ = mixed_density_matrix(RefT, Phi)
+    RefT = np.asarray(RefT, np.complex128)
+    out = np.empty_like(np.asarray(M, np.complex128))
+    for w in range(out.shape[0]):
+        out[w] = (M[w].T if transpose_m else M[w]) + (RefT @ X[w]).T @ (Gc[w] @ Y[w].conj())
+    return out
 
 
 # --- particle-hole multi-determinant trial (phmsd_oracle.cpp) ------------------------------

@@ -95,6 +95,11 @@ _SIGS = {
     "afq_prop_bp_views": [_P, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_I)],
     "afq_rdm_measure": [_P, _I, _I, _I, _P, _P, _P, _I, _P, _P],
     "afq_prop_bp_measure": [_P, ctypes.POINTER(Walkers), _I, _I, _I, _P, _P],
+    "afq_prop_te_anchor": [_P],
+    "afq_prop_te_propagate": [_P, ctypes.POINTER(Walkers), _I],
+    "afq_prop_te_measure": [_P, ctypes.POINTER(Walkers), _I, _I, _P, _P],
+    "afq_prop_te_views": [_P, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_P),
+                          ctypes.POINTER(_P)],
     "afq_prop_mixed_measure": [_P, ctypes.POINTER(Walkers), _P, _P],
     "afq_prop_number_of_references": [_P, ctypes.POINTER(_I)],
     "afq_prop_create_phmsd": [_P, _I, ctypes.POINTER(PropParams), ctypes.POINTER(_P)],
@@ -458,6 +463,7 @@ class Propagator:
         self.ham = ham
         self.ctx = ham.ctx
         self.dt = dt
+        self.W = int(W)
         self.trial = trial
         p = PropParams(dt, order, vbias_bound, 1.0, 10.0, 100.0, 0.1, target_pop,
                        int(free_projection), 0, 0, seed, stream)
@@ -564,6 +570,48 @@ class Propagator:
         v = _I()
         _ck(lib().afq_prop_number_of_references(self.h, ctypes.byref(v)))
         return v.value
+
+    # ---- back-propagation by time-evolved operators (TimeEvolvedBPEstimator) ----
+    def te_anchor(self):
+        """X = Y = I and M = 0 for every walker slot (setAnchor)."""
+        _ck(lib().afq_prop_te_anchor(self.h))
+
+    def te_propagate(self, wset: WalkerSet, nsteps: int):
+        """PropagateOperators over the nsteps most recent stored fields, then the
+        regularization with the current walker matrices."""
+        _ck(lib().afq_prop_te_propagate(self.h, ctypes.byref(wset.cstruct()), int(nsteps)))
+
+    def te_measure(self, wset: WalkerSet, restore_steps: int, transpose_m: bool = False,
+                   onerdm=True, diag2rdm=True):
+        """OneRDM [nspin][N][N] and unfolded DiagonalTwoRDM [nspin+1][N][N] of the Green
+        functions dressed by the time-evolved operators, with WEIGHT times the factors of the
+        last restore_steps steps.  transpose_m=False is the reference's G_d = M + ...; True
+        adds M^T, which equals the back-propagated estimator."""
+        one, two = self._rdm_out(onerdm, diag2rdm)
+        _ck(lib().afq_prop_te_measure(self.h, ctypes.byref(wset.cstruct()), int(restore_steps),
+                                      int(bool(transpose_m)), ptr(one), ptr(two)))
+        return one, two
+
+    def te_operators(self):
+        """Host copies (X, Y, M) [W][nspin][N][N] and P1inv [N][N]."""
+        torch = _torch()
+        x, y, m, q = _P(), _P(), _P(), _P()
+        _ck(lib().afq_prop_te_views(self.h, ctypes.byref(x), ctypes.byref(y), ctypes.byref(m),
+                                    ctypes.byref(q)))
+        self.ctx.synchronize()
+        N = self.ham.N
+        nsp = 2 if self.ham.walker_type == AFQ_COLLINEAR else 1
+        out = []
+        for v in (x, y, m):
+            if not v.value:
+                out.append(None)
+                continue
+            t = torch.empty((self.W, nsp, N, N), dtype=torch.complex128, device="cuda")
+            _copy_d2d(t.data_ptr(), v.value, t.numel() * 16)
+            out.append(host(t))
+        p1 = torch.empty((N, N), dtype=torch.complex128, device="cuda")
+        _copy_d2d(p1.data_ptr(), q.value, p1.numel() * 16)
+        return out[0], out[1], out[2], host(p1)
 
     def bp_rings(self, W: int):
         """Copies of the field ring [W][nback][M] and the factor ring [W][3 nback]."""

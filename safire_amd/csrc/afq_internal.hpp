@@ -80,6 +80,9 @@ void cast_z64_c32(hipStream_t s, long n, const z64* x, c32* y);
 void cast_c32_z64(hipStream_t s, long n, const c32* x, z64* y);
 void cast_f64_f32(hipStream_t s, long n, const double* x, float* y);
 void fill_z64(hipStream_t s, long n, z64 v, z64* y);
+void fill_blocks_z64(hipStream_t s, long B, long n, long stride, z64 v, z64* y);
+// out[b][i][j] = in[b][j][i] for B square N x N blocks (out != in)
+void transpose_blocks(hipStream_t s, long B, int N, const z64* in, z64* out);
 void zero_imag(hipStream_t s, long n, z64* y);
 void truncate_small(hipStream_t s, long n, double cut, z64* y);
 void hermitize_scale(hipStream_t s, int N, double scale, z64* A);

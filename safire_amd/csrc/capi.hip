@@ -458,6 +458,42 @@ int afq_prop_number_of_references(afq_prop* prop, int* nref) {
   });
 }
 
+int afq_prop_te_anchor(afq_prop* prop) {
+  return guard([&] {
+    REQUIRE(prop, "NULL argument");
+    prop->p->te_anchor();
+  });
+}
+
+int afq_prop_te_propagate(afq_prop* prop, const afq_walkers* w, int nsteps) {
+  return guard([&] {
+    REQUIRE(prop, "NULL argument");
+    check_walkers(prop, w);
+    prop->p->te_propagate(to_state(w), nsteps);
+  });
+}
+
+int afq_prop_te_measure(afq_prop* prop, const afq_walkers* w, int restore_steps, int transpose_m,
+                        double* onerdm, double* diag2rdm) {
+  return guard([&] {
+    REQUIRE(prop, "NULL argument");
+    check_walkers(prop, w);
+    prop->p->te_measure(to_state(w), restore_steps, transpose_m != 0,
+                        onerdm ? Z(onerdm) : nullptr, diag2rdm ? Z(diag2rdm) : nullptr);
+  });
+}
+
+int afq_prop_te_views(afq_prop* prop, const double** X, const double** Y, const double** M,
+                      const double** P1inv) {
+  return guard([&] {
+    REQUIRE(prop && X && Y && M && P1inv, "NULL argument");
+    *X = reinterpret_cast<const double*>(prop->p->te_operator(0));
+    *Y = reinterpret_cast<const double*>(prop->p->te_operator(1));
+    *M = reinterpret_cast<const double*>(prop->p->te_operator(2));
+    *P1inv = reinterpret_cast<const double*>(prop->p->P1_inverse());
+  });
+}
+
 int afq_rdm_measure(afq_ctx* ctx, int walker_type, int N, int W, const double* G,
                     const double* weights, const double* coeff, int accumulate, double* onerdm,
                     double* diag2rdm) {

@@ -379,7 +379,7 @@ void CholeskyOps::vMF(double dt, z64* v) const {
   zero_imag(s, M_, v);
 }
 
-void CholeskyOps::one_body_propagator(const z64* vMF, double dt, z64* P1) const {
+void CholeskyOps::one_body_propagator(const z64* vMF, double dt, z64* P1, bool inverse) const {
   hipStream_t s = ctx_.stream();
   const int N = N_;
   const long NN = (long)N * N;
@@ -390,7 +390,7 @@ void CholeskyOps::one_body_propagator(const z64* vMF, double dt, z64* P1) const 
   // meanfield shift = vHS(vMF) (includes sqrt(dt)); H1 = dt h + dt vexx + shift
   vHS(vMF, 1, dt, shift, kFP64);
   add_one_body(s, N, dt, h1_, vexx_, shift, H);
-  hermitize_scale(s, N, -0.5, H);  // hermitize(H1); H1 *= -1/2
+  hermitize_scale(s, N, inverse ? 0.5 : -0.5, H);  // hermitize(H1); H1 *= -1/2 (+1/2: inverse)
   // scaling and squaring: ||H||_1 on the host (setup path)
   std::vector<z64> hH(NN);
   AFQ_HIP(hipMemcpyAsync(hH.data(), H, sizeof(z64) * NN, hipMemcpyDeviceToHost, s));

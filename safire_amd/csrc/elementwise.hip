@@ -42,6 +42,19 @@ __global__ void k_fill(long n, z64 v, z64* y) {
   long i = (long)blockIdx.x * kT + threadIdx.x;
   if (i < n) y[i] = v;
 }
+// y[b * stride + i] = v for i < n (one block of n elements in every stride)
+__global__ void k_fill_blocks(long B, long n, long stride, z64 v, z64* y) {
+  long t = (long)blockIdx.x * kT + threadIdx.x;
+  if (t < B * n) y[(t / n) * stride + t % n] = v;
+}
+// out[b][i][j] = in[b][j][i] for B square N x N blocks
+__global__ void k_transpose_blocks(long B, int N, const z64* __restrict__ in, z64* __restrict__ out) {
+  long t = (long)blockIdx.x * kT + threadIdx.x;
+  const long NN = (long)N * N;
+  if (t >= B * NN) return;
+  const long b = t / NN, e = t % NN;
+  out[t] = in[b * NN + (e % N) * N + e / N];
+}
 __global__ void k_zero_imag(long n, z64* y) {
   long i = (long)blockIdx.x * kT + threadIdx.x;
   if (i < n) y[i].im = 0.0;
@@ -320,6 +333,16 @@ void cast_f64_f32(hipStream_t s, long n, const double* x, float* y) {
 void fill_z64(hipStream_t s, long n, z64 v, z64* y) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_fill, grid1(n), dim3(kT), 0, s, n, v, y);
+  AFQ_LAUNCH_CHECK();
+}
+void fill_blocks_z64(hipStream_t s, long B, long n, long stride, z64 v, z64* y) {
+  if (B * n <= 0) return;
+  hipLaunchKernelGGL(k_fill_blocks, grid1(B * n), dim3(kT), 0, s, B, n, stride, v, y);
+  AFQ_LAUNCH_CHECK();
+}
+void transpose_blocks(hipStream_t s, long B, int N, const z64* in, z64* out) {
+  if (B * N <= 0) return;
+  hipLaunchKernelGGL(k_transpose_blocks, grid1(B * N * N), dim3(kT), 0, s, B, N, in, out);
   AFQ_LAUNCH_CHECK();
 }
 void zero_imag(hipStream_t s, long n, z64* y) {

@@ -145,7 +145,8 @@ class CholeskyOps {
   void vMF(double dt, z64* v) const;
   // P1 = exp(-1/2 hermitize(dt (h + vexx) + vHS(vMF))) with |x| <= 1e-8 dropped
   // (generateP1, AFQMCBasePropagator.cpp:43-214). P1 [N][N] complex.
-  void one_body_propagator(const z64* vMF, double dt, z64* P1) const;
+  // inverse: exp(+1/2 H1) instead, the P1x_inv of generateP1(true) (:118-123)
+  void one_body_propagator(const z64* vMF, double dt, z64* P1, bool inverse = false) const;
 
   Context& ctx() const { return ctx_; }
 
@@ -356,6 +357,29 @@ class Propagator {
                   z64* diag2rdm);
   int number_of_references();
 
+  // ---- back-propagation by operators evolved forward in time (TimeEvolvedBPEstimator.hpp;
+  // AFQMCBasePropagator::PropagateOperators, AFQMCBasePropagator.cpp:538-684).  Uses the field
+  // and factor rings of enable_backprop.  X, Y, M [W][nspin][N][N] (nspin 1 CLOSED, 2
+  // COLLINEAR) belong to walker slots, not to walkers: population control does not move them
+  // (the reference keeps them in the estimator).
+  // setAnchor: X = Y = I, M = 0
+  void te_anchor();
+  // PropagateOperators: replays the nsteps most recent stored fields, oldest first,
+  // X <- P1 expM(V) P1 X and Y <- P1inv expM(-V)^H P1inv Y, then regularizes with the current
+  // walker matrices S (orthonormal after a QR): Y <- S S^H Y, M += Y^H X, X <- X - S S^H X.
+  void te_propagate(const WalkerState& s, int nsteps);
+  // constructTimeEvolvedMeasurementInputs + the observables: per reference d,
+  // G_d = M + (Ref_d^H X)^T (Gc_d conj(Y)) with Gc_d the compact mixed density matrix of the
+  // reference and the walker, coefficients conj(c_d) <Ref_d|Phi> normalized over the
+  // references, weights times the factors of the last restore_steps steps.  transpose_m adds
+  // M^T instead of M (the orientation under which G_d equals the back-propagated estimator).
+  void te_measure(const WalkerState& s, int restore_steps, bool transpose_m, z64* onerdm,
+                  z64* diag2rdm);
+  // device views (tests): 0 X, 1 Y, 2 M; nullptr before the first te_anchor
+  const z64* te_operator(int which) const;
+  // P1x_inv = exp(+1/2 H1) [N][N], built on first use
+  const z64* P1_inverse();
+
  private:
   CholeskyOps& ham_;
   Context& ctx_;
@@ -395,6 +419,10 @@ class Propagator {
   z64* ref_sel_[2] = {nullptr, nullptr};
   // the references as Slater matrices, [nref][N][nel_s] (the conjugate transposes of ref_rows_)
   z64* ref_cols_[2] = {nullptr, nullptr};
+  // time-evolved operators and the inverse one-body propagator
+  z64* te_op_[3] = {nullptr, nullptr, nullptr};
+  z64* P1inv_ = nullptr;
+  double* P1invr_ = nullptr;
   void ensure_references();
   // back-propagates every reference through the last nbp stored fields; on return Lp[s] holds
   // [W][nref][N][nel_s] and ldR [W][nref] the accumulated log det R of their QR steps

@@ -94,7 +94,8 @@ Propagator::~Propagator() {
                   (void*)step_dev_, (void*)status_dev_, (void*)bp_fields_, (void*)bp_wfac_,
                   (void*)bp_anch_a_, (void*)bp_anch_b_, (void*)ref_rows_[0], (void*)ref_rows_[1],
                   (void*)ref_cc_, (void*)ref_sel_[0], (void*)ref_sel_[1], (void*)ref_cols_[0],
-                  (void*)ref_cols_[1]})
+                  (void*)ref_cols_[1], (void*)te_op_[0], (void*)te_op_[1], (void*)te_op_[2],
+                  (void*)P1inv_, (void*)P1invr_})
     if (q) (void)hipFree(q);
 }
 
@@ -572,6 +573,214 @@ void Propagator::mixed_measure(const WalkerState& st, z64* onerdm, z64* diag2rdm
   const z64* L[2] = {ref_cols_[0], ref_cols_[1]};
   const z64* Phi[2] = {st.Phi, st.Phi_b};
   measure_refs(st.W, L, false, Phi, st.weight, nullptr, onerdm, diag2rdm);
+}
+
+// ---- time-evolved operators ------------------------------------------------------------------
+
+const z64* Propagator::P1_inverse() {
+  if (P1inv_) return P1inv_;
+  hipStream_t s = ctx_.stream();
+  const int N = ham_.N();
+  z64* p = dalloc<z64>((size_t)N * N);
+  double* pr = nullptr;
+  try {
+    ham_.one_body_propagator(vMF_, p_.dt, p, true);
+    // H1 real <=> P1 real <=> its inverse real
+    if (real_P1_) {
+      pr = dalloc<double>((size_t)N * N);
+      real_part(s, (long)N * N, p, pr);
+    }
+    AFQ_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    (void)hipFree(p);
+    if (pr) (void)hipFree(pr);
+    throw;
+  }
+  P1inv_ = p;
+  P1invr_ = pr;
+  return P1inv_;
+}
+
+const z64* Propagator::te_operator(int which) const {
+  AFQ_CHECK(which >= 0 && which < 3, kErrArg, "operator index must be 0 (X), 1 (Y) or 2 (M)");
+  return te_op_[which];
+}
+
+void Propagator::te_anchor() {
+  hipStream_t s = ctx_.stream();
+  const int N = ham_.N();
+  const int nsp = closed_ ? 1 : 2;
+  const size_t cnt = (size_t)W_ * nsp * N * N;
+  if (!te_op_[0]) {
+    z64* buf[3] = {nullptr, nullptr, nullptr};
+    try {
+      for (int i = 0; i < 3; ++i) buf[i] = dalloc<z64>(cnt);
+    } catch (...) {
+      for (z64* q : buf)
+        if (q) (void)hipFree(q);
+      throw;
+    }
+    for (int i = 0; i < 3; ++i) te_op_[i] = buf[i];
+  }
+  z64* I = scratch_as<z64>(ctx_, "te_I", (size_t)N * N);
+  identity(s, N, I);
+  broadcast_z64(s, (long)W_ * nsp, (long)N * N, I, te_op_[0]);
+  broadcast_z64(s, (long)W_ * nsp, (long)N * N, I, te_op_[1]);
+  fill_z64(s, (long)cnt, {0.0, 0.0}, te_op_[2]);
+}
+
+void Propagator::te_propagate(const WalkerState& st, int nsteps) {
+  using gemm::EpiStore;
+  using gemm::Opnd;
+  AFQ_CHECK(bp_n_ > 0, kErrState, "back-propagation is not enabled");
+  AFQ_CHECK(te_op_[0], kErrState, "the time-evolved operators have no anchor");
+  AFQ_CHECK(nsteps > 0 && nsteps <= bp_n_, kErrArg,
+            "operator propagation length must be positive and fit the field ring");
+  AFQ_CHECK(st.W == W_, kErrShape, "the operators belong to the propagator's walker slots");
+  hipStream_t s = ctx_.stream();
+  const int W = st.W, N = ham_.N(), M = ham_.M();
+  const int nsp = closed_ ? 1 : 2;
+  const int nsl[2] = {wup_, closed_ ? 0 : wdn_};
+  const long NN = (long)N * N, obs = (long)nsp * NN;  // batch stride of X, Y, M
+  P1_inverse();
+  uint32_t C = 0;  // completed steps: the newest stored field sits at slot (C - 1) % nback
+  AFQ_HIP(hipMemcpyAsync(&C, step_dev_, sizeof(C), hipMemcpyDeviceToHost, s));
+  AFQ_HIP(hipStreamSynchronize(s));
+  z64* Xf = scratch_as<z64>(ctx_, "bp_X", (size_t)W * M);
+  z64* V = scratch_as<z64>(ctx_, "bp_V", (size_t)W * NN);
+  z64 *Xo = te_op_[0], *Yo = te_op_[1], *Mo = te_op_[2];
+  for (int i = 0; i < nsteps; ++i) {  // oldest first
+    const long back = (long)nsteps - i;
+    const int slot = (int)((((long)C - back) % bp_n_ + bp_n_) % bp_n_);
+    gather_fields(s, W, M, bp_fields_, bp_n_, slot, Xf);
+    ham_.vHS(Xf, W, p_.dt, V, kFP64);
+    for (int sp = 0; sp < nsp; ++sp) {
+      det_ops::PropagateArgs pa{W, N, N, p_.order, 'N', P1_, real_P1_ ? P1r_ : nullptr,
+                                nullptr, nullptr, V, NN, Xo + sp * NN, obs, kFP64};
+      det_ops::propagate(ctx_, pa);
+    }
+    // B^{-H} = P1inv expM(-V)^H P1inv: the adjoint propagator of -V with the inverse P1
+    scale_add_cplx(s, (long)W * NN, V, -1.0, 0.0, V);
+    for (int sp = 0; sp < nsp; ++sp) {
+      det_ops::PropagateArgs pa{W, N, N, p_.order, 'H', P1inv_, real_P1_ ? P1invr_ : nullptr,
+                                nullptr, nullptr, V, NN, Yo + sp * NN, obs, kFP64};
+      pa.v_symmetric = true;
+      det_ops::propagate(ctx_, pa);
+    }
+  }
+  // regularization with the walker matrices S [W][N][n]
+  const z64* Sp[2] = {st.Phi, st.Phi_b};
+  for (int sp = 0; sp < nsp; ++sp) {
+    const int n = nsl[sp];
+    z64 *X = Xo + sp * NN, *Y = Yo + sp * NN, *Mm = Mo + sp * NN;
+    if (n == 0) {  // no occupied space: Y <- 0, M and X unchanged
+      fill_blocks_z64(s, W, NN, obs, {0.0, 0.0}, Y);
+      continue;
+    }
+    const z64* S = Sp[sp];
+    z64* SY = scratch_as<z64>(ctx_, "te_SY", (size_t)W * n * N);
+    const Opnd<double, true, false, true> SH{&S->re, n, (long)N * n};  // S^H: (a, k) = conj S[k][a]
+    const Opnd<double, true, true> Sn{&S->re, n, (long)N * n};          // S: (i, a) = S[i][a]
+    const Opnd<double, true, false> SYt{&SY->re, N, (long)n * N};       // (j, a) = SY[a][j]
+    {  // SY = S^H Y; Y <- S SY
+      Opnd<double, true, false> Bt{&Y->re, N, obs};
+      gemm_small<double>(ctx_, "te_SY", W, n, N, N, SH, Bt,
+                         EpiStore<double, double>{SY, N, (long)n * N, 1, 0, 0});
+      gemm_small<double>(ctx_, "te_Y", W, N, N, n, Sn, SYt,
+                         EpiStore<double, double>{Y, N, obs, 1, 0, 0});
+    }
+    {  // M += Y^H X
+      Opnd<double, true, false, true> YH{&Y->re, N, obs};
+      Opnd<double, true, false> Bt{&X->re, N, obs};
+      gemm_small<double>(ctx_, "te_M", W, N, N, N, YH, Bt,
+                         EpiStore<double, double>{Mm, N, obs, 1, 0, 1}, false);
+    }
+    {  // SY = S^H X; X <- X - S SY
+      Opnd<double, true, false> Bt{&X->re, N, obs};
+      gemm_small<double>(ctx_, "te_SY", W, n, N, N, SH, Bt,
+                         EpiStore<double, double>{SY, N, (long)n * N, 1, 0, 0});
+      gemm_small<double>(ctx_, "te_X", W, N, N, n, Sn, SYt,
+                         EpiStore<double, double>{X, N, obs, -1, 0, 1}, false);
+    }
+  }
+}
+
+void Propagator::te_measure(const WalkerState& st, int restore_steps, bool transpose_m,
+                            z64* onerdm, z64* diag2rdm) {
+  using gemm::EpiStore;
+  using gemm::Opnd;
+  AFQ_CHECK(bp_n_ > 0, kErrState, "back-propagation is not enabled");
+  AFQ_CHECK(te_op_[0], kErrState, "the time-evolved operators have no anchor");
+  AFQ_CHECK(st.W == W_, kErrShape, "the operators belong to the propagator's walker slots");
+  AFQ_CHECK(restore_steps >= 0 && restore_steps <= bp_hl_, kErrArg,
+            "path restoration over more steps than the factor ring holds");
+  ensure_references();
+  hipStream_t s = ctx_.stream();
+  const int W = st.W, N = ham_.N(), R = nref_;
+  const bool coll = !closed_;
+  const int nsp = coll ? 2 : 1;
+  const int nsl[2] = {wup_, coll ? wdn_ : 0};
+  const long NN = (long)N * N, obs = (long)nsp * NN;
+  const z64* Phi[2] = {st.Phi, st.Phi_b};
+  // weights times the path-restoration factors
+  uint32_t C = 0;
+  AFQ_HIP(hipMemcpyAsync(&C, step_dev_, sizeof(C), hipMemcpyDeviceToHost, s));
+  AFQ_HIP(hipStreamSynchronize(s));
+  z64* wr = scratch_as<z64>(ctx_, "te_w", W);
+  restore_weights(s, W, st.weight, bp_wfac_, bp_hl_, C, restore_steps, wr);
+  // reference coefficients from <Ref_d|Phi> (the reference's OVLP shift cancels in the ratio)
+  z64* lo = scratch_as<z64>(ctx_, "ms_lo", (size_t)R * W);
+  z64* cf = scratch_as<z64>(ctx_, "ms_cf", (size_t)R * W);
+  const int nmax = std::max(nsl[0], nsl[1]);
+  const int RC = reference_chunk(R, W, nsp, N, nmax);
+  fill_z64(s, (long)R * W, {0, 0}, lo);
+  for (int d0 = 0; d0 < R; d0 += RC)
+    for (int sp = 0; sp < nsp; ++sp)
+      if (nsl[sp] > 0)
+        det_ops::mixed_density_matrix_batched(ctx_, ref_cols_[sp] + (size_t)d0 * N * nsl[sp],
+                                              (long)N * nsl[sp], W, nsl[sp], N, Phi[sp],
+                                              (long)N * nsl[sp], W, std::min(RC, R - d0) * W,
+                                              nullptr, 0, lo + (size_t)d0 * W);
+  reference_coefficients(s, W, R, lo, ref_cc_, nullptr, cf);
+  z64* Gc = scratch_as<z64>(ctx_, "te_Gc", (size_t)W * nmax * N);
+  z64* GY = scratch_as<z64>(ctx_, "te_GY", (size_t)W * nmax * N);
+  z64* XR = scratch_as<z64>(ctx_, "te_XR", (size_t)W * nmax * N);
+  z64* G = scratch_as<z64>(ctx_, "te_G", (size_t)W * obs);
+  z64* dum = scratch_as<z64>(ctx_, "te_ov", (size_t)W);
+  for (int d = 0; d < R; ++d) {
+    if (transpose_m)
+      transpose_blocks(s, (long)W * nsp, N, te_op_[2], G);
+    else
+      AFQ_HIP(hipMemcpyAsync(G, te_op_[2], sizeof(z64) * W * obs, hipMemcpyDeviceToDevice, s));
+    for (int sp = 0; sp < nsp; ++sp) {
+      const int n = nsl[sp];
+      if (n == 0) continue;
+      const z64* ref = ref_rows_[sp] + (size_t)d * n * N;  // Ref_d^H [n][N]
+      const z64 *X = te_op_[0] + sp * NN, *Y = te_op_[1] + sp * NN;
+      // Gc = T^{-T} Phi^T with T = Ref^H Phi
+      det_ops::mixed_density_matrix(ctx_, ref, n, N, Phi[sp], (long)N * n, W, Gc, (long)n * N,
+                                    dum, true);
+      {  // GY = Gc conj(Y)
+        Opnd<double, true, true> A{&Gc->re, N, (long)n * N};
+        Opnd<double, true, false, true> Bt{&Y->re, N, obs};
+        gemm_small<double>(ctx_, "te_GY", W, n, N, N, A, Bt,
+                           EpiStore<double, double>{GY, N, (long)n * N, 1, 0, 0});
+      }
+      {  // XR = Ref^H X
+        Opnd<double, true, true> A{&ref->re, N, 0};
+        Opnd<double, true, false> Bt{&X->re, N, obs};
+        gemm_small<double>(ctx_, "te_XR", W, n, N, N, A, Bt,
+                           EpiStore<double, double>{XR, N, (long)n * N, 1, 0, 0});
+      }
+      {  // G += XR^T GY
+        Opnd<double, true, false> A{&XR->re, N, (long)n * N};
+        Opnd<double, true, false> Bt{&GY->re, N, (long)n * N};
+        gemm_small<double>(ctx_, "te_G", W, N, N, n, A, Bt,
+                           EpiStore<double, double>{G + sp * NN, N, obs, 1, 0, 1}, false);
+      }
+    }
+    rdm_measure(ctx_, W, nsp, N, coll, G, obs, wr, cf + (size_t)d * W, d > 0, onerdm, diag2rdm, W);
+  }
 }
 
 int Propagator::status() {

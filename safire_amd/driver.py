@@ -33,6 +33,8 @@ class Trace:
     mixed: dict = dataclasses.field(default_factory=dict)
     # BackPropEstimator: propagation steps -> observable name -> one entry per window
     backprop: dict = dataclasses.field(default_factory=dict)
+    # TimeEvolvedBPEstimator: the same layout
+    time_evolved_bp: dict = dataclasses.field(default_factory=dict)
 
     def mean_error(self, skip=0):
         e = np.real(np.asarray(self.energy[skip:]))
@@ -106,14 +108,22 @@ def run_afqmc(prop: "afq.Propagator", ws: "afq.WalkerSet", steps: int, equilibra
       {"mixed": {"measure_interval": k, "onerdm": {}, "diag_twordm": {}},
        "backprop": {"propagation_steps": [L1, L2, ...], "path_restoration": True,
                     "extra_path_restoration": False, "walker_ortho_interval": k,
-                    "onerdm": {}, "diag_twordm": {}}}
+                    "onerdm": {}, "diag_twordm": {}},
+       "time_evolved_bp": {"propagation_steps": [L1, L2, ...], "path_restoration": True,
+                           "extra_path_restoration": False, "transpose_m": False,
+                           "onerdm": {}, "diag_twordm": {}}}
     The mixed estimator (MixedEstimator.hpp:122-140) measures its observables on the current
     walkers every measure_interval steps after equilibration.  Both estimators sum over the
     trial's reference determinants (one for NOMSD, the expansion for PHMSD).  Back-propagation (BackPropEstimator.hpp:150-260) anchors after
     the last equilibration step (before the first step without equilibration); when the steps
     since the anchor equal one of the propagation lengths the trial is back-propagated over
     them and the observables measured with the (path-restored) weights, and at the longest
-    length the anchor moves to the current step."""
+    length the anchor moves to the current step.  The time-evolved estimator
+    (TimeEvolvedBPEstimator.hpp:199-262) keeps the same anchors and windows, but evolves the
+    operators forward over each segment between consecutive lengths and dresses the current
+    walkers' density matrices with them; its regularization assumes orthonormal walkers, so it
+    is exact when the walkers are orthogonalized every step (ortho_interval = 1).  Both
+    back-propagation estimators share one field ring."""
     import torch
     W = ws.W
     if relax is None:  # parameter_defaults.cpp:310-318
@@ -130,20 +140,40 @@ def run_afqmc(prop: "afq.Propagator", ws: "afq.WalkerSet", steps: int, equilibra
     wt = prop.ham.walker_type
     if mixed is not None:
         mixed_interval = int(mixed.get("measure_interval", measure_interval))
-    bp_lens = []
-    bp_pos = -1
-    if bp is not None:
-        bp_lens = sorted(int(x) for x in bp["propagation_steps"])
-        if len(set(bp_lens)) != len(bp_lens) or bp_lens[0] <= 0:
+    te = est.get("time_evolved_bp")
+
+    def lengths(block):
+        lens = sorted(int(x) for x in block["propagation_steps"])
+        if not lens or len(set(lens)) != len(lens) or lens[0] <= 0:
             raise ValueError("propagation_steps must be distinct and positive")
-        restore = ((2 if bp.get("extra_path_restoration", False) else 1)
-                   if bp.get("path_restoration", True) else 0)
+        return lens
+
+    def restoration(block):
+        return ((2 if block.get("extra_path_restoration", False) else 1)
+                if block.get("path_restoration", True) else 0)
+
+    bp_lens, te_lens = [], []
+    bp_pos = te_pos = -1
+    if bp is not None:
+        bp_lens = lengths(bp)
+        restore = restoration(bp)
         bp_ortho = int(bp.get("walker_ortho_interval", ortho_interval))
-        if getattr(prop, "nback", 0) < bp_lens[-1]:
-            prop.enable_backprop(bp_lens[-1])
-        prop.bp_anchor(ws)   # the anchor before the first step
         for L in bp_lens:
             tr.backprop[L] = {}
+    if te is not None:
+        te_lens = lengths(te)
+        te_restore = restoration(te)
+        te_transpose = bool(te.get("transpose_m", False))
+        for L in te_lens:
+            tr.time_evolved_bp[L] = {}
+    nback = max(bp_lens[-1:] + te_lens[-1:] + [0])
+    if nback and getattr(prop, "nback", 0) < nback:
+        prop.enable_backprop(nback)
+    # the anchors before the first step
+    if bp_lens:
+        prop.bp_anchor(ws)
+    if te_lens:
+        prop.te_anchor()
     for step in range(steps):
         prop.step_graph(ws, Eshift, ortho=(step % ortho_interval == 0),
                         popctl=(popctl_interval > 0 and step % popctl_interval == 0),
@@ -181,6 +211,22 @@ def run_afqmc(prop: "afq.Propagator", ws: "afq.WalkerSet", steps: int, equilibra
                 if n >= bp_lens[-1]:
                     prop.bp_anchor(ws)
                     bp_pos = step
+        if te_lens:
+            if step == equilibration - 1:
+                prop.te_anchor()
+                te_pos = step
+            elif step >= equilibration:
+                n = step - te_pos
+                if n in tr.time_evolved_bp:
+                    # the operators already cover the previous (shorter) length
+                    i = te_lens.index(n)
+                    prop.te_propagate(ws, n - (te_lens[i - 1] if i > 0 else 0))
+                    one, two = prop.te_measure(ws, te_restore * n, te_transpose,
+                                               "onerdm" in te, "diag_twordm" in te)
+                    _record(tr.time_evolved_bp[n], _observables(one, two, wt))
+                if n >= te_lens[-1]:
+                    prop.te_anchor()
+                    te_pos = step
         if progress is not None and (step + 1) % progress_interval == 0:
             progress(step + 1, tr)
     torch.cuda.synchronize()

@@ -101,3 +101,65 @@ def test_fold_matches_reference_layout(closed):
     assert ref.shape == (size,)
     got = afq.fold_diag2rdm(full, afq.AFQ_CLOSED if closed else afq.AFQ_COLLINEAR)
     assert np.array_equal(got, ref)
+
+
+def test_p1_inverse_is_the_inverse():
+    """generateP1(true) (AFQMCBasePropagator.cpp:118-123): exp(+1/2 H1) times exp(-1/2 H1) is
+    the identity up to the 1e-8 truncation of both."""
+    from safire_amd.synthetic import make_problem
+    p = make_problem(12, 3, 40, seed=7)
+    osys = orc.OracleSystem(p, dt=0.01)
+    vx = orc.vexx(p.Likn)
+    P1 = orc.generate_p1(p.h1, vx, p.Likn, osys.vMF, 0.01)
+    P1i = orc.generate_p1(p.h1, vx, p.Likn, osys.vMF, 0.01, inverse=True)
+    assert np.abs(P1 @ P1i - np.eye(p.N)).max() < 1e-7
+    assert np.abs(P1 - P1i).max() > 1e-3
+
+
+def _bp_density_matrix(P1, Vs, RefT, anchor, order):
+    """The back-propagated estimator's G (BackPropEstimator): Ref^H propagated back through
+    the fields (adjoint propagator, newest first) against the anchor walker."""
+    W, N, n = anchor.shape
+    L = np.broadcast_to(RefT.conj().T, (W, N, n)).copy()
+    for V in Vs[::-1]:
+        L = orc.propagate(P1, V, L, order, "H")
+    return np.array([(anchor[w] @ np.linalg.solve(L[w].conj().T @ anchor[w], L[w].conj().T)).T
+                     for w in range(W)])
+
+
+def test_time_evolved_operators_equal_backpropagation():
+    """The time-evolved operators (PropagateOperators + constructTimeEvolvedMeasurementInputs)
+    evaluate the back-propagated density matrix <Psi|B_n..B_1 c+_i c_j|Phi_0> as
+    X^T G(n) conj(Y) with X = B_n..B_1, Y = X^{-H}, the regularization moving the part of it
+    that the orthonormal walker's occupied space fixes into M.  With M entering transposed, the
+    oracle's G equals the standard back-propagation over the same fields to rounding, in two
+    segments (the estimator's lengths {2, 4}) with a QR of the walkers after every step; a wrong
+    sign of V, a missing P1 inverse or a dropped projection fails it.  The reference's own
+    formula adds M untransposed (AFQMCBasePropagator.cpp:640-645, 666-669), which differs once
+    the walkers are complex."""
+    from safire_amd.synthetic import make_problem
+    p = make_problem(10, 3, 30, seed=5)
+    dt, order, W = 0.005, 12, 3
+    osys = orc.OracleSystem(p, dt=dt)
+    vx = orc.vexx(p.Likn)
+    P1 = orc.generate_p1(p.h1, vx, p.Likn, osys.vMF, dt)
+    P1i = orc.generate_p1(p.h1, vx, p.Likn, osys.vMF, dt, inverse=True)
+    rng = np.random.default_rng(1)
+    N, n = p.N, p.nup
+    noise = rng.normal(size=(W, N, n)) + 1j * rng.normal(size=(W, N, n))
+    Phi0, _ = orc.orthogonalize(np.broadcast_to(osys.PsiT.conj().T, (W, N, n)) + 0.1 * noise)
+    Vs = [orc.vhs(p.Likn, rng.normal(size=(W, p.M)) + 0.3j * rng.normal(size=(W, p.M)), dt)
+          for _ in range(4)]
+    X = np.broadcast_to(np.eye(N, dtype=complex), (W, N, N)).copy()
+    Y, M = X.copy(), np.zeros((W, N, N), complex)
+    Phi = Phi0.copy()
+    for seg in (Vs[:2], Vs[2:]):
+        for V in seg:
+            Phi, _ = orc.orthogonalize(orc.propagate(P1, V, Phi, order, "N"))
+        X, Y, M = orc.propagate_operators(P1, P1i, seg, X, Y, M, Phi, order)
+    Gbp = _bp_density_matrix(P1, Vs, osys.PsiT, Phi0, order)
+    Gt = orc.time_evolved_density_matrix(osys.PsiT, X, Y, M, Phi, transpose_m=True)
+    Gr = orc.time_evolved_density_matrix(osys.PsiT, X, Y, M, Phi)
+    scale = np.abs(Gbp).max()
+    assert np.abs(Gt - Gbp).max() < 1e-11 * scale
+    assert np.abs(Gr - Gbp).max() > 1e-3 * scale
