@@ -161,7 +161,8 @@ def estimator_bench(reps=3):
     """Cost of one back-propagated and one mixed measurement (OneRDM + DiagonalTwoRDM) on BH read
     from the reference's inputs at its run parameters (1536 walkers, dt 0.01, 100-step
     back-propagation with QR every 10 steps): the RHF trial (one reference) and the CASCI
-    PHMSD trial (44 references)."""
+    PHMSD trial (44 references); and the time-evolved estimator over the same 100 steps (the
+    operator propagation once, the dressed measurement)."""
     import torch
     from safire_amd import afq, h5inputs
     from safire_amd.phmsd import PHMSD
@@ -193,11 +194,28 @@ def estimator_bench(reps=3):
         ws.ovlp.copy_(ov)
         Esh = afq.device_tensor(np.array([float(afq.host(E.sum(1)).real.mean())]))
         prop.bp_anchor(ws)
+        prop.te_anchor()
         t0 = time.perf_counter()
         for k in range(L):
             prop.step_graph(ws, Esh, ortho=(k % 10 == 0), popctl=(k % 10 == 0))
         torch.cuda.synchronize()
         t_step = (time.perf_counter() - t0) / L
+        # time-evolved operators over the same window: one PropagateOperators call over the
+        # 100 stored fields (timed on a second call, after the first has tuned its products),
+        # then one dressed measurement
+        prop.te_propagate(ws, L)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        prop.te_propagate(ws, L)
+        torch.cuda.synchronize()
+        t_tep = time.perf_counter() - t0
+        prop.te_measure(ws, L)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            prop.te_measure(ws, L)
+        torch.cuda.synchronize()
+        t_tem = (time.perf_counter() - t0) / reps
         prop.bp_measure(ws, L, 10, 1)
         prop.mixed_measure(ws)
         torch.cuda.synchronize()
@@ -214,6 +232,8 @@ def estimator_bench(reps=3):
         res[name] = {"references": prop.number_of_references(), "walkers": W,
                      "step_ms": t_step * 1e3, "backprop_100_ms": t_bp * 1e3,
                      "mixed_ms": t_mx * 1e3,
+                     "time_evolved_propagate_100_ms": t_tep * 1e3,
+                     "time_evolved_measure_ms": t_tem * 1e3,
                      "backprop_share_at_one_per_100_steps": t_bp / (t_bp + 100 * t_step)}
         prop.close()
         if trial is not None:
