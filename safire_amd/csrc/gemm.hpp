@@ -277,7 +277,11 @@ struct EpiPartial {
   }
 };
 
-template <typename T, int BM, int BN, int BK, int WAVES_M, int WAVES_N, class OA, class OB, class EPI>
+// PP: two LDS buffers used in turn (the next slab is stored into the other buffer after the
+// current one is multiplied), one barrier per slab instead of two.  The MFMA sequence, and so
+// the result, is the same either way.
+template <typename T, int BM, int BN, int BK, int WAVES_M, int WAVES_N, class OA, class OB, class EPI,
+          bool PP = false>
 __global__ void __launch_bounds__(WAVES_M * WAVES_N * 64)
     kgemm(int M, int N, int K, int tiles_m, int tiles_n, int ksplit, int kchunk, OA A, OB B,
           EPI epi) {
@@ -287,12 +291,13 @@ __global__ void __launch_bounds__(WAVES_M * WAVES_N * 64)
   static_assert(WM % 16 == 0 && WN % 16 == 0 && BK % 4 == 0, "bad tile");
   constexpr int LDA = BM + 4, LDB = BN + 4;
   constexpr bool CA = OA::is_cplx, CBX = OB::is_cplx;
+  constexpr int NBUF = PP ? 2 : 1;
   using V4 = typename Acc<T>::v4;
 
-  __shared__ T sAr[BK * LDA];
-  __shared__ T sAi[CA ? BK * LDA : 1];
-  __shared__ T sBr[BK * LDB];
-  __shared__ T sBi[CBX ? BK * LDB : 1];
+  __shared__ T sAr[NBUF * BK * LDA];
+  __shared__ T sAi[CA ? NBUF * BK * LDA : 1];
+  __shared__ T sBr[NBUF * BK * LDB];
+  __shared__ T sBi[CBX ? NBUF * BK * LDB : 1];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -329,28 +334,21 @@ __global__ void __launch_bounds__(WAVES_M * WAVES_N * 64)
   gb.load(B, b, n0, kb, N, ke, tid);
 
   const int fr = lane & 15, fk = lane >> 4;
-  for (int k0 = kb; k0 < ke; k0 += BK) {
-    __syncthreads();
-    ga.store(sAr, sAi, tid);
-    gb.store(sBr, sBi, tid);
-    __syncthreads();
-    if (k0 + BK < ke) {
-      ga.load(A, b, m0, k0 + BK, M, ke, tid);
-      gb.load(B, b, n0, k0 + BK, N, ke, tid);
-    }
+  // one BK slab from LDS buffer (pa, pai, pb, pbi)
+  auto slab = [&](const T* pa, const T* pai, const T* pb, const T* pbi) {
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       const int kl = kk + fk;
       T ar[RB], ai[RB], br[CB], bi[CB];
 #pragma unroll
       for (int i = 0; i < RB; ++i) {
-        ar[i] = sAr[kl * LDA + wm0 + i * 16 + fr];
-        if (CA) ai[i] = sAi[kl * LDA + wm0 + i * 16 + fr];
+        ar[i] = pa[kl * LDA + wm0 + i * 16 + fr];
+        if (CA) ai[i] = pai[kl * LDA + wm0 + i * 16 + fr];
       }
 #pragma unroll
       for (int j = 0; j < CB; ++j) {
-        br[j] = sBr[kl * LDB + wn0 + j * 16 + fr];
-        if (CBX) bi[j] = sBi[kl * LDB + wn0 + j * 16 + fr];
+        br[j] = pb[kl * LDB + wn0 + j * 16 + fr];
+        if (CBX) bi[j] = pbi[kl * LDB + wn0 + j * 16 + fr];
       }
 #pragma unroll
       for (int i = 0; i < RB; ++i)
@@ -367,6 +365,42 @@ __global__ void __launch_bounds__(WAVES_M * WAVES_N * 64)
             acci[i][j] = Acc<T>::mfma(ar[i], bi[j], acci[i][j]);
           }
         }
+    }
+  };
+  if constexpr (PP) {
+    // buffer (it & 1) holds slab it; slab it + 1 goes into the other buffer, whose last readers
+    // (slab it - 1) all passed the barrier that ended the previous iteration
+    ga.store(sAr, sAi, tid);
+    gb.store(sBr, sBi, tid);
+    __syncthreads();
+    int cur = 0;
+    for (int k0 = kb; k0 < ke; k0 += BK) {
+      const bool more = k0 + BK < ke;
+      if (more) {
+        ga.load(A, b, m0, k0 + BK, M, ke, tid);
+        gb.load(B, b, n0, k0 + BK, N, ke, tid);
+      }
+      slab(sAr + cur * BK * LDA, sAi + (CA ? cur * BK * LDA : 0), sBr + cur * BK * LDB,
+           sBi + (CBX ? cur * BK * LDB : 0));
+      if (more) {
+        const int nxt = cur ^ 1;
+        ga.store(sAr + nxt * BK * LDA, sAi + (CA ? nxt * BK * LDA : 0), tid);
+        gb.store(sBr + nxt * BK * LDB, sBi + (CBX ? nxt * BK * LDB : 0), tid);
+      }
+      __syncthreads();
+      cur ^= 1;
+    }
+  } else {
+    for (int k0 = kb; k0 < ke; k0 += BK) {
+      __syncthreads();
+      ga.store(sAr, sAi, tid);
+      gb.store(sBr, sBi, tid);
+      __syncthreads();
+      if (k0 + BK < ke) {
+        ga.load(A, b, m0, k0 + BK, M, ke, tid);
+        gb.load(B, b, n0, k0 + BK, N, ke, tid);
+      }
+      slab(sAr, sAi, sBr, sBi);
     }
   }
 
@@ -402,28 +436,30 @@ __global__ void __launch_bounds__(256)
 }
 
 // Host launcher.  batch = number of independent products (grid = batch * tiles).
-template <typename T, int BM, int BN, int BK, int WAVES_M, int WAVES_N, class OA, class OB, class EPI>
+template <typename T, int BM, int BN, int BK, int WAVES_M, int WAVES_N, bool PP = false, class OA,
+          class OB, class EPI>
 inline void launch(hipStream_t s, int batch, int M, int N, int K, const OA& A, const OB& B,
                    const EPI& epi) {
   if (batch <= 0 || M <= 0 || N <= 0) return;
   int tm = ceil_div(M, BM), tn = ceil_div(N, BN);
   long blocks = (long)batch * tm * tn;
   AFQ_CHECK(blocks < (1L << 31), kErrShape, "grid too large");
-  hipLaunchKernelGGL((kgemm<T, BM, BN, BK, WAVES_M, WAVES_N, OA, OB, EPI>), dim3((unsigned)blocks),
-                     dim3(WAVES_M * WAVES_N * 64), 0, s, M, N, K, tm, tn, 1, K > 0 ? K : 1, A, B,
-                     epi);
+  hipLaunchKernelGGL((kgemm<T, BM, BN, BK, WAVES_M, WAVES_N, OA, OB, EPI, PP>),
+                     dim3((unsigned)blocks), dim3(WAVES_M * WAVES_N * 64), 0, s, M, N, K, tm, tn,
+                     1, K > 0 ? K : 1, A, B, epi);
   AFQ_LAUNCH_CHECK();
 }
 
 // Split-K launch: K cut into ksplit chunks of kchunk (a multiple of BK) computed by separate
 // workgroups into ws (ksplit * batch * M * N complex), then summed in chunk order and passed to
 // the epilogue. The summation order depends only on kchunk, not on the tile.
-template <typename T, int BM, int BN, int BK, int WAVES_M, int WAVES_N, class OA, class OB, class EPI>
+template <typename T, int BM, int BN, int BK, int WAVES_M, int WAVES_N, bool PP = false, class OA,
+          class OB, class EPI>
 inline void launch_splitk(hipStream_t s, int batch, int M, int N, int K, int ksplit, int kchunk,
                           cplx<T>* ws, const OA& A, const OB& B, const EPI& epi) {
   if (batch <= 0 || M <= 0 || N <= 0) return;
   if (ksplit <= 1) {
-    launch<T, BM, BN, BK, WAVES_M, WAVES_N>(s, batch, M, N, K, A, B, epi);
+    launch<T, BM, BN, BK, WAVES_M, WAVES_N, PP>(s, batch, M, N, K, A, B, epi);
     return;
   }
   AFQ_CHECK(kchunk % BK == 0 && (long)kchunk * ksplit >= K, kErrArg, "bad split-K chunking");
@@ -431,7 +467,7 @@ inline void launch_splitk(hipStream_t s, int batch, int M, int N, int K, int ksp
   long blocks = (long)batch * tm * tn * ksplit;
   AFQ_CHECK(blocks < (1L << 31), kErrShape, "grid too large");
   EpiPartial<T> ep{ws, (long)M * N, N};
-  hipLaunchKernelGGL((kgemm<T, BM, BN, BK, WAVES_M, WAVES_N, OA, OB, EpiPartial<T>>),
+  hipLaunchKernelGGL((kgemm<T, BM, BN, BK, WAVES_M, WAVES_N, OA, OB, EpiPartial<T>, PP>),
                      dim3((unsigned)blocks), dim3(WAVES_M * WAVES_N * 64), 0, s, M, N, K, tm, tn,
                      ksplit, kchunk, A, B, ep);
   AFQ_LAUNCH_CHECK();

@@ -155,20 +155,29 @@ template <typename T, class OA, class OB, class EPI>
 void gemm_large(Context& c, const char* site, int batch, int M, int N, int K, const OA& A,
                 const OB& B, const EPI& e, bool idempotent = true) {
   hipStream_t s = c.stream();
-  // FP32 also offers 32-deep k slabs (twice the MFMA work per staged slab); the k order of the
-  // accumulation is the same for every slab depth
-  constexpr int ncand = sizeof(T) == 4 ? 7 : 4;
+  // two LDS buffers in turn (PP) for the two smaller tiles; FP32 also offers 32-deep k slabs
+  // (twice the MFMA work per staged slab).  The k order of the accumulation is the same for
+  // every candidate.
+  constexpr int ncand = sizeof(T) == 4 ? 11 : 8;
   tuned(c, tune_key(site, sizeof(T), batch, M, N, K), ncand, [&](int i) {
     switch (i) {
+      // 128 x 128 tiles over 16 and 8 wavefronts (32 x 32 and 32 x 64 per wavefront)
+      case 9: gemm::launch<T, 128, 128, 16, 4, 4, true>(s, batch, M, N, K, A, B, e); break;
+      case 10: gemm::launch<T, 128, 128, 16, 4, 2, true>(s, batch, M, N, K, A, B, e); break;
       case 0: gemm::launch<T, 64, 64, 16, 2, 2>(s, batch, M, N, K, A, B, e); break;
       case 1: gemm::launch<T, 128, 128, 16, 2, 2>(s, batch, M, N, K, A, B, e); break;
       case 2: gemm::launch<T, 128, 64, 16, 2, 2>(s, batch, M, N, K, A, B, e); break;
       case 3: gemm::launch<T, 64, 128, 16, 2, 2>(s, batch, M, N, K, A, B, e); break;
+      case 4: gemm::launch<T, 64, 128, 16, 2, 2, true>(s, batch, M, N, K, A, B, e); break;
+      case 5: gemm::launch<T, 64, 64, 16, 2, 2, true>(s, batch, M, N, K, A, B, e); break;
       default:
         if constexpr (sizeof(T) == 4) {
-          if (i == 4) gemm::launch<T, 64, 128, 32, 2, 2>(s, batch, M, N, K, A, B, e);
-          else if (i == 5) gemm::launch<T, 128, 128, 32, 2, 2>(s, batch, M, N, K, A, B, e);
+          if (i == 6) gemm::launch<T, 64, 128, 32, 2, 2>(s, batch, M, N, K, A, B, e);
+          else if (i == 7) gemm::launch<T, 128, 128, 32, 2, 2>(s, batch, M, N, K, A, B, e);
           else gemm::launch<T, 128, 64, 32, 2, 2>(s, batch, M, N, K, A, B, e);
+        } else {
+          if (i == 6) gemm::launch<T, 128, 128, 16, 4, 4, true>(s, batch, M, N, K, A, B, e);
+          else gemm::launch<T, 128, 128, 16, 4, 2, true>(s, batch, M, N, K, A, B, e);
         }
         break;
     }
@@ -194,19 +203,24 @@ void gemm_large_splitk(Context& c, const char* site, int M, int N, int K, const 
   cplx<T>* ws = ks > 1 ? scratch_as<cplx<T>>(c, std::string("splitk_ws") + (sizeof(T) == 8 ? "64" : "32"),
                                               (size_t)ks * M * N)
                        : nullptr;
-  constexpr int ncand = sizeof(T) == 4 ? 8 : 5;
+  constexpr int ncand = sizeof(T) == 4 ? 11 : 8;
   tuned(c, tune_key(site, sizeof(T), ks, M, N, K), ncand, [&](int i) {
     switch (i) {
+      case 10: gemm::launch_splitk<T, 128, 128, 16, 4, 4, true>(s, 1, M, N, K, ks, kc, ws, A, B, e); break;
       case 0: gemm::launch_splitk<T, 64, 64, 16, 2, 2>(s, 1, M, N, K, ks, kc, ws, A, B, e); break;
       case 1: gemm::launch_splitk<T, 128, 128, 16, 2, 2>(s, 1, M, N, K, ks, kc, ws, A, B, e); break;
       case 2: gemm::launch_splitk<T, 128, 64, 16, 2, 2>(s, 1, M, N, K, ks, kc, ws, A, B, e); break;
       case 3: gemm::launch_splitk<T, 64, 128, 16, 2, 2>(s, 1, M, N, K, ks, kc, ws, A, B, e); break;
       case 4: gemm::launch_splitk<T, 32, 32, 16, 1, 1>(s, 1, M, N, K, ks, kc, ws, A, B, e); break;
+      case 5: gemm::launch_splitk<T, 64, 64, 16, 2, 2, true>(s, 1, M, N, K, ks, kc, ws, A, B, e); break;
+      case 6: gemm::launch_splitk<T, 64, 128, 16, 2, 2, true>(s, 1, M, N, K, ks, kc, ws, A, B, e); break;
       default:
         if constexpr (sizeof(T) == 4) {
-          if (i == 5) gemm::launch_splitk<T, 64, 64, 32, 2, 2>(s, 1, M, N, K, ks, kc, ws, A, B, e);
-          else if (i == 6) gemm::launch_splitk<T, 128, 128, 32, 2, 2>(s, 1, M, N, K, ks, kc, ws, A, B, e);
+          if (i == 7) gemm::launch_splitk<T, 64, 64, 32, 2, 2>(s, 1, M, N, K, ks, kc, ws, A, B, e);
+          else if (i == 8) gemm::launch_splitk<T, 128, 128, 32, 2, 2>(s, 1, M, N, K, ks, kc, ws, A, B, e);
           else gemm::launch_splitk<T, 128, 64, 32, 2, 2>(s, 1, M, N, K, ks, kc, ws, A, B, e);
+        } else {
+          gemm::launch_splitk<T, 128, 128, 16, 4, 4, true>(s, 1, M, N, K, ks, kc, ws, A, B, e);
         }
         break;
     }
