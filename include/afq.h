@@ -254,6 +254,13 @@ int afq_prop_mixed_measure(afq_prop* prop, const afq_walkers* w, double* onerdm,
 /* Number of reference determinants of the trial (Wavefunction::total_number_of_references). */
 int afq_prop_number_of_references(afq_prop* prop, int* nref);
 
+/* Kernel timing of the step (no counterpart in the reference; for benchmarks): on != 0 makes
+ * every later step record two HIP events around its vHS launch, also inside captured step
+ * graphs (switching re-captures them); afq_prop_vhs_ms returns the last step's vHS duration in
+ * milliseconds, waiting for the end event. */
+int afq_prop_vhs_timing(afq_prop* prop, int on);
+int afq_prop_vhs_ms(afq_prop* prop, float* ms);
+
 /* ---- back-propagation by operators evolved forward in time (TimeEvolvedBPEstimator.hpp) ------
  * The operator states X, Y, M [W][nspin][N][N] (complex) of the propagator's W walker slots.
  * They stay with the slots when population control copies walkers, as the reference's

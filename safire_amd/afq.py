@@ -95,6 +95,8 @@ _SIGS = {
     "afq_prop_bp_views": [_P, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_I)],
     "afq_rdm_measure": [_P, _I, _I, _I, _P, _P, _P, _I, _P, _P],
     "afq_prop_bp_measure": [_P, ctypes.POINTER(Walkers), _I, _I, _I, _P, _P],
+    "afq_prop_vhs_timing": [_P, _I],
+    "afq_prop_vhs_ms": [_P, ctypes.POINTER(ctypes.c_float)],
     "afq_prop_te_anchor": [_P],
     "afq_prop_te_propagate": [_P, ctypes.POINTER(Walkers), _I],
     "afq_prop_te_measure": [_P, ctypes.POINTER(Walkers), _I, _I, _P, _P],
@@ -570,6 +572,16 @@ class Propagator:
         v = _I()
         _ck(lib().afq_prop_number_of_references(self.h, ctypes.byref(v)))
         return v.value
+
+    def vhs_timing(self, on: bool):
+        """Record HIP events around the vHS launch of every later step (inside the step graph)."""
+        _ck(lib().afq_prop_vhs_timing(self.h, int(bool(on))))
+
+    def vhs_ms(self) -> float:
+        """Duration of the last step's vHS launch in ms."""
+        v = ctypes.c_float()
+        _ck(lib().afq_prop_vhs_ms(self.h, ctypes.byref(v)))
+        return float(v.value)
 
     # ---- back-propagation by time-evolved operators (TimeEvolvedBPEstimator) ----
     def te_anchor(self):

@@ -458,6 +458,20 @@ int afq_prop_number_of_references(afq_prop* prop, int* nref) {
   });
 }
 
+int afq_prop_vhs_timing(afq_prop* prop, int on) {
+  return guard([&] {
+    REQUIRE(prop, "NULL argument");
+    prop->p->set_vhs_timing(on != 0);
+  });
+}
+
+int afq_prop_vhs_ms(afq_prop* prop, float* ms) {
+  return guard([&] {
+    REQUIRE(prop && ms, "NULL argument");
+    *ms = prop->p->vhs_ms();
+  });
+}
+
 int afq_prop_te_anchor(afq_prop* prop) {
   return guard([&] {
     REQUIRE(prop, "NULL argument");

@@ -327,6 +327,11 @@ class Propagator {
   void step_graph(const WalkerState& s, const double* Eshift, bool ortho, bool popctl,
                   double relax, double* energy_out);
   Precision precision() const { return prec_; }
+  // HIP events around the step's vHS launch, recorded inside captured steps as well, so that
+  // the duration is the kernel's as the step runs it.  Switching re-captures the step graphs.
+  void set_vhs_timing(bool on);
+  // duration of the last step's vHS in ms (waits for the end event)
+  float vhs_ms();
 
   // ---- back-propagation (AFQMCBasePropagator::BackPropagate, AFQMCBasePropagator.cpp:404-560;
   // BackPropEstimator.hpp).  enable_backprop makes every later step store its fields in a ring
@@ -405,6 +410,8 @@ class Propagator {
   };
   std::vector<GraphEntry> graphs_;
   bool warmed_ = false;
+  bool vhs_timing_ = false;
+  hipEvent_t vhs_ev_[2] = {nullptr, nullptr};
   int bp_n_ = 0, bp_hl_ = 0;
   z64* bp_fields_ = nullptr;
   z64* bp_wfac_ = nullptr;

@@ -86,6 +86,8 @@ Propagator::Propagator(CholeskyOps& ham, int W, const PropagatorParams& p, Phmsd
 
 Propagator::~Propagator() {
   (void)hipStreamSynchronize(ctx_.stream());
+  for (hipEvent_t e : vhs_ev_)
+    if (e) (void)hipEventDestroy(e);
   for (auto& g : graphs_) {
     (void)hipGraphExecDestroy(g.exec);
     (void)hipGraphDestroy(g.graph);
@@ -162,7 +164,9 @@ void Propagator::step_impl(const WalkerState& st, const double* u, const double*
   construct_x(s, u, vMF_, W, M, fp, X, HW, mf, X32);
   if (bp_n_ > 0) store_fields(s, W, M, X, bp_fields_, bp_n_, step_dev_);
   // 3. vHS
+  if (vhs_timing_) AFQ_HIP(hipEventRecord(vhs_ev_[0], s));
   ham_.vHS(prec_ == kMixed ? (const void*)X32 : (const void*)X, W, p_.dt, V, prec_);
+  if (vhs_timing_) AFQ_HIP(hipEventRecord(vhs_ev_[1], s));
   // 4. propagate both spin blocks with the same V
   det_ops::PropagateArgs pa{W, N, nup, p_.order, 'N', P1_, real_P1_ ? P1r_ : nullptr, P1r32_,
                             P1c32_, V, (long)N * N, st.Phi, (long)N * nup, prec_};
@@ -180,6 +184,26 @@ void Propagator::step_impl(const WalkerState& st, const double* u, const double*
                   p_.target_pop, p_.free_projection, p_.use_cp};
   walker_update(s, W, up, Eshift, nov, mf, HW, st.weight, st.phase, st.eloc, st.ovlp, st.theta,
                 bp_n_ > 0 ? bp_wfac_ : nullptr, bp_hl_, step_dev_);
+}
+
+void Propagator::set_vhs_timing(bool on) {
+  if (on && !vhs_ev_[0]) {
+    AFQ_HIP(hipEventCreate(&vhs_ev_[0]));
+    AFQ_HIP(hipEventCreate(&vhs_ev_[1]));
+  }
+  if (on != vhs_timing_) {
+    AFQ_HIP(hipStreamSynchronize(ctx_.stream()));
+    clear_graphs();
+    vhs_timing_ = on;
+  }
+}
+
+float Propagator::vhs_ms() {
+  AFQ_CHECK(vhs_ev_[0], kErrState, "vHS timing was never switched on");
+  AFQ_HIP(hipEventSynchronize(vhs_ev_[1]));
+  float ms = 0.0f;
+  AFQ_HIP(hipEventElapsedTime(&ms, vhs_ev_[0], vhs_ev_[1]));
+  return ms;
 }
 
 void Propagator::step(const WalkerState& st, const double* u, const double* Eshift) {
