@@ -50,6 +50,30 @@ int afq_h5_read_ci(const char* path, int kind, int ndets, double* ci);
 /* PHMSD occupations: occa [ndets][nup], occb [ndets][ndown] (the first ndets rows). */
 int afq_h5_read_occupations(const char* path, int ndets, int nup, int ndown, int* occa, int* occb);
 
+/* ---- results file: the bins of the estimators' observables --------------------------------
+ * The layout Estimators::write (Estimators.hpp:115-137), Measurements::write
+ * (Measurements.hpp:69-80) and Accumulator::write (Accumulator.hpp:34-143) produce: the root
+ * carries the integer attribute format_version = 1; observable `name` (a '/'-separated path
+ * such as "Energy" or "BackPropEstimator/Steps=100/OneRDM") of stage `stage` ("Stage0", ...)
+ * is the dataset Measurements/<stage>/<name>/bins of shape [bins][dims...], with a trailing
+ * axis of 2 and the string attribute __complex__ = "1" for complex values, extendable along
+ * the bin axis (chunked by one append, deflate level 1).
+ * afq_h5_results_append creates the file when it does not exist and appends nbins bins
+ * (data: nbins * prod(dims) values, complex interleaved) through a copy renamed over the file
+ * at the end, as the reference does; the dims and the complex flag must match an existing
+ * dataset. */
+int afq_h5_results_append(const char* path, const char* stage, const char* name, int rank,
+                          const long* dims, int is_complex, long nbins, const double* data);
+/* Bin count, rank and dims (at most 8, without the complex axis) and complex flag of an
+ * observable's bins. */
+int afq_h5_results_shape(const char* path, const char* stage, const char* name, long* nbins,
+                         int* rank, long* dims, int* is_complex);
+/* All bins of an observable (nbins * prod(dims) values, complex interleaved). */
+int afq_h5_results_read(const char* path, const char* stage, const char* name, double* out);
+/* Observable names of a stage, '\n'-separated, into buf of size cap (the length needed,
+ * including the terminating zero, in *len). */
+int afq_h5_results_names(const char* path, const char* stage, char* buf, long cap, long* len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cctype>
 #include <cstring>
+#include <filesystem>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -260,9 +261,191 @@ int guarded(F&& f) {
   }
 }
 
+// ---- results file ----------------------------------------------------------------------------
+
+// group `name` below loc, created when missing
+Id open_or_create_group(hid_t loc, const std::string& name) {
+  if (name.empty()) throw Fail(kErrArg, "empty group name in an observable path");
+  if (H5Lexists(loc, name.c_str(), H5P_DEFAULT) > 0) {
+    hid_t g = H5Gopen2(loc, name.c_str(), H5P_DEFAULT);
+    if (g < 0) throw Fail(kErrArg, "cannot open group " + name);
+    return Id(g, H5Gclose);
+  }
+  hid_t g = H5Gcreate2(loc, name.c_str(), H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT);
+  if (g < 0) throw Fail(kErrArg, "cannot create group " + name);
+  return Id(g, H5Gclose);
+}
+
+std::vector<std::string> split_path(const std::string& path) {
+  std::vector<std::string> out;
+  size_t p = 0;
+  while (p <= path.size()) {
+    size_t q = path.find('/', p);
+    if (q == std::string::npos) q = path.size();
+    out.push_back(path.substr(p, q - p));
+    p = q + 1;
+  }
+  return out;
+}
+
+std::string bins_path(const char* stage, const char* name) {
+  if (!stage || !name || !*stage || !*name) throw Fail(kErrArg, "stage and observable name needed");
+  return std::string("Measurements/") + stage + "/" + name + "/bins";
+}
+
+void write_string_attr(hid_t obj, const char* key, const char* value) {
+  Id t(H5Tcopy(H5T_C_S1), H5Tclose);
+  H5Tset_size(t, std::strlen(value) + 1);
+  H5Tset_strpad(t, H5T_STR_NULLTERM);
+  Id sp(H5Screate(H5S_SCALAR), H5Sclose);
+  Id a(H5Acreate2(obj, key, t, sp, H5P_DEFAULT, H5P_DEFAULT), H5Aclose);
+  if (a < 0 || H5Awrite(a, t, value) < 0) throw Fail(kErrArg, std::string("cannot write attribute ") + key);
+}
+
+struct NameCollector {
+  std::vector<std::string> names;
+};
+
+herr_t collect_bins(hid_t, const char* name, const H5O_info_t* info, void* op) {
+  if (info->type != H5O_TYPE_DATASET) return 0;
+  std::string n(name);
+  const std::string tail = "/bins";
+  if (n.size() > tail.size() && n.compare(n.size() - tail.size(), tail.size(), tail) == 0)
+    static_cast<NameCollector*>(op)->names.push_back(n.substr(0, n.size() - tail.size()));
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
+
+int afq_h5_results_append(const char* path, const char* stage, const char* name, int rank,
+                          const long* dims, int is_complex, long nbins, const double* data) {
+  return guarded([&] {
+    if (!path || rank < 0 || rank > 8 || (rank > 0 && !dims) || nbins < 0 || (nbins > 0 && !data))
+      throw Fail(kErrArg, "invalid results arguments");
+    const std::string bp = bins_path(stage, name);
+    quiet();
+    namespace fs = std::filesystem;
+    const fs::path target(path), tmp = fs::path(std::string(path) + ".tmp");
+    const bool existed = fs::exists(target);
+    if (existed) fs::copy_file(target, tmp, fs::copy_options::overwrite_existing);
+    {
+      Id f(existed ? H5Fopen(tmp.c_str(), H5F_ACC_RDWR, H5P_DEFAULT)
+                   : H5Fcreate(tmp.c_str(), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT),
+           H5Fclose);
+      if (f < 0) throw Fail(kErrArg, std::string("cannot open ") + tmp.string() + " for writing");
+      Id root(H5Gopen2(f, "/", H5P_DEFAULT), H5Gclose);
+      if (!attr_exists(root, "format_version")) {
+        const int v = 1;
+        Id sp(H5Screate(H5S_SCALAR), H5Sclose);
+        Id a(H5Acreate2(root, "format_version", H5T_STD_I32LE, sp, H5P_DEFAULT, H5P_DEFAULT),
+             H5Aclose);
+        if (a < 0 || H5Awrite(a, H5T_NATIVE_INT, &v) < 0)
+          throw Fail(kErrArg, "cannot write format_version");
+      }
+      // Measurements/<stage>/<name components>
+      std::vector<std::string> parts = split_path(std::string("Measurements/") + stage + "/" + name);
+      std::vector<Id> groups;
+      groups.push_back(open_or_create_group(root, parts[0]));
+      for (size_t i = 1; i < parts.size(); ++i) groups.push_back(open_or_create_group(groups.back(), parts[i]));
+      hid_t g = groups.back();
+      const int frank = rank + 1 + (is_complex ? 1 : 0);
+      std::vector<hsize_t> want(frank);
+      want[0] = (hsize_t)nbins;
+      for (int i = 0; i < rank; ++i) {
+        if (dims[i] < 0) throw Fail(kErrArg, "negative extent");
+        want[1 + i] = (hsize_t)dims[i];
+      }
+      if (is_complex) want[frank - 1] = 2;
+      if (H5Lexists(g, "bins", H5P_DEFAULT) <= 0) {
+        std::vector<hsize_t> d0 = want, maxd = want, chunk = want;
+        d0[0] = 0;
+        maxd[0] = H5S_UNLIMITED;
+        for (auto& c : chunk) c = std::max<hsize_t>(c, 1);
+        Id cp(H5Pcreate(H5P_DATASET_CREATE), H5Pclose);
+        if (H5Pset_chunk(cp, frank, chunk.data()) < 0) throw Fail(kErrArg, "H5Pset_chunk failed");
+        H5Pset_deflate(cp, 1);
+        Id sp(H5Screate_simple(frank, d0.data(), maxd.data()), H5Sclose);
+        Id ds(H5Dcreate2(g, "bins", H5T_IEEE_F64LE, sp, H5P_DEFAULT, cp, H5P_DEFAULT), H5Dclose);
+        if (ds < 0) throw Fail(kErrArg, "cannot create " + bp);
+        if (is_complex) write_string_attr(ds, "__complex__", "1");
+      }
+      Id ds(H5Dopen2(g, "bins", H5P_DEFAULT), H5Dclose);
+      if (ds < 0) throw Fail(kErrArg, "cannot open " + bp);
+      std::vector<hsize_t> cur = dims_of(ds);
+      const bool cplx = attr_exists(ds, "__complex__");
+      if ((int)cur.size() != frank || cplx != (is_complex != 0))
+        throw Fail(kErrShape, bp + " holds bins of another rank or type");
+      for (int i = 1; i < frank; ++i)
+        if (cur[i] != want[i]) throw Fail(kErrShape, bp + " holds bins of another shape");
+      if (nbins > 0) {
+        std::vector<hsize_t> grown = cur;
+        grown[0] += (hsize_t)nbins;
+        if (H5Dset_extent(ds, grown.data()) < 0) throw Fail(kErrArg, "H5Dset_extent failed");
+        Id fs_(H5Dget_space(ds), H5Sclose);
+        std::vector<hsize_t> start(frank, 0);
+        start[0] = cur[0];
+        if (H5Sselect_hyperslab(fs_, H5S_SELECT_SET, start.data(), nullptr, want.data(), nullptr) < 0)
+          throw Fail(kErrArg, "cannot select the appended bins");
+        Id ms(H5Screate_simple(frank, want.data(), nullptr), H5Sclose);
+        if (H5Dwrite(ds, H5T_NATIVE_DOUBLE, ms, fs_, H5P_DEFAULT, data) < 0)
+          throw Fail(kErrArg, "cannot write " + bp);
+      }
+    }
+    fs::rename(tmp, target);
+  });
+}
+
+int afq_h5_results_shape(const char* path, const char* stage, const char* name, long* nbins,
+                         int* rank, long* dims, int* is_complex) {
+  return guarded([&] {
+    if (!nbins || !rank || !dims || !is_complex) throw Fail(kErrArg, "null output");
+    Id f = open_file(path);
+    const std::string bp = bins_path(stage, name);
+    if (!exists(f, bp)) throw Fail(kErrArg, "no " + bp + " in " + path);
+    Id ds(H5Dopen2(f, bp.c_str(), H5P_DEFAULT), H5Dclose);
+    std::vector<hsize_t> d = dims_of(ds);
+    const bool cplx = attr_exists(ds, "__complex__");
+    const int r = (int)d.size() - 1 - (cplx ? 1 : 0);
+    if (r < 0 || r > 8) throw Fail(kErrShape, bp + " has an unexpected rank");
+    *nbins = (long)d[0];
+    *rank = r;
+    for (int i = 0; i < r; ++i) dims[i] = (long)d[1 + i];
+    *is_complex = cplx ? 1 : 0;
+  });
+}
+
+int afq_h5_results_read(const char* path, const char* stage, const char* name, double* out) {
+  return guarded([&] {
+    if (!out) throw Fail(kErrArg, "null output");
+    Id f = open_file(path);
+    const std::string bp = bins_path(stage, name);
+    if (!exists(f, bp)) throw Fail(kErrArg, "no " + bp + " in " + path);
+    Id ds(H5Dopen2(f, bp.c_str(), H5P_DEFAULT), H5Dclose);
+    if (count_of(dims_of(ds)) == 0) return;
+    if (H5Dread(ds, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, out) < 0)
+      throw Fail(kErrArg, "cannot read " + bp);
+  });
+}
+
+int afq_h5_results_names(const char* path, const char* stage, char* buf, long cap, long* len) {
+  return guarded([&] {
+    if (!len || !stage) throw Fail(kErrArg, "null argument");
+    Id f = open_file(path);
+    const std::string sp = std::string("Measurements/") + stage;
+    if (!exists(f, sp)) throw Fail(kErrArg, "no " + sp + " in " + path);
+    Id g(H5Gopen2(f, sp.c_str(), H5P_DEFAULT), H5Gclose);
+    NameCollector nc;
+    if (H5Ovisit(g, H5_INDEX_NAME, H5_ITER_INC, collect_bins, &nc) < 0)
+      throw Fail(kErrArg, "cannot list " + sp);
+    std::sort(nc.names.begin(), nc.names.end());
+    std::string all;
+    for (size_t i = 0; i < nc.names.size(); ++i) all += (i ? "\n" : "") + nc.names[i];
+    *len = (long)all.size() + 1;
+    if (buf && cap >= *len) std::memcpy(buf, all.c_str(), all.size() + 1);
+  });
+}
 
 const char* afq_h5_last_error(void) { return g_err.c_str(); }
 

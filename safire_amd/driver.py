@@ -7,8 +7,10 @@ After equilibration the energy estimator (EnergyEstimator.h:51-90) evaluates the
 every `measure_interval` steps and records the weighted average sum_w w E_w / sum_w w
 (MeasurementOutput, Measurements.hpp:92-121) together with the weight statistics.
 
-blocking_error is the Flyvbjerg-Petersen blocking analysis of a correlated series (Flyvbjerg &
-Petersen, J. Chem. Phys. 91, 461 (1989)), the method behind safiretools' rebinning analysis.
+Error bars (Trace.mean_error, mean_error) follow the reference's result tools: the series is
+rebinned with the bin size optimal_rebinsize chooses and the mean's error is the standard error
+of the bins (safire_amd.stats).  blocking_error, the Flyvbjerg-Petersen blocking analysis
+(J. Chem. Phys. 91, 461 (1989)), is kept as an independent estimate.
 """
 from __future__ import annotations
 
@@ -16,7 +18,7 @@ import dataclasses
 
 import numpy as np
 
-from . import afq
+from . import afq, stats
 
 
 @dataclasses.dataclass
@@ -27,6 +29,7 @@ class Trace:
     total_weight: list
     eff_walkers: list
     overlap: list = dataclasses.field(default_factory=list)  # sum_w w e^{log ovlp_w} / sum_w w
+    phase: list = dataclasses.field(default_factory=list)    # sum_w w / sum_w |w|
     growth: float = float("nan")   # growth-estimator energy after the last step
     eshift: float = float("nan")   # Eshift after the last step
     # MixedEstimator: observable name -> one entry per measurement
@@ -37,8 +40,9 @@ class Trace:
     time_evolved_bp: dict = dataclasses.field(default_factory=dict)
 
     def mean_error(self, skip=0):
-        e = np.real(np.asarray(self.energy[skip:]))
-        return float(e.mean()), blocking_error(e)
+        """(mean, error) of the real energy after `skip` measurements, rebinned."""
+        m, e = stats.average(np.real(np.asarray(self.energy)), skip=skip)
+        return float(m), float(e)
 
 
 def blocking_error(x) -> float:
@@ -87,14 +91,12 @@ def _record(store: dict, obs: dict):
         store.setdefault(k, []).append(v)
 
 
-def mean_error(series) -> tuple:
-    """(mean, blocking error) of a series of complex observable measurements, element by
-    element; the error of a complex element is error(re) + i error(im)."""
-    x = np.asarray(series)
-    flat = x.reshape(len(x), -1)
-    err = np.array([complex(blocking_error(flat[:, i].real), blocking_error(flat[:, i].imag))
-                    for i in range(flat.shape[1])])
-    return x.mean(0), err.reshape(x.shape[1:])
+def mean_error(series, skip=0) -> tuple:
+    """(mean, error) of a series of complex observable measurements, element by element, as
+    the reference's Results.average gives them: one bin size for the whole observable (the
+    one its slowest component needs), the mean of the bins and their standard error, whose
+    complex form is error(re) + i error(im)."""
+    return stats.average(np.asarray(series, np.complex128), skip=skip)
 
 
 def run_afqmc(prop: "afq.Propagator", ws: "afq.WalkerSet", steps: int, equilibration: int = 100,
@@ -187,14 +189,17 @@ def run_afqmc(prop: "afq.Propagator", ws: "afq.WalkerSet", steps: int, equilibra
             neff = (aw.sum() ** 2 / (aw ** 2).sum())
             # EnergyEstimator.h:86-87: the weighted average of the walkers' overlaps
             ovl = (w * torch.exp(ov)).sum() / sw
+            # weight_statistics (Measurements.hpp:130-146): phase = sum w / sum |w|
+            phase = sw / aw.sum()
             vals = afq.host(torch.cat([comps, sw.reshape(1), neff.reshape(1).to(torch.complex128),
-                                       ovl.reshape(1)]))
+                                       ovl.reshape(1), phase.reshape(1)]))
             tr.steps.append(step)
             tr.components.append(vals[:3])
             tr.energy.append(complex(vals[:3].sum()))
             tr.total_weight.append(complex(vals[3]))
             tr.eff_walkers.append(float(vals[4].real))
             tr.overlap.append(complex(vals[5]))
+            tr.phase.append(complex(vals[6]))
         if mixed is not None and step >= equilibration and step % mixed_interval == 0:
             one, two = prop.mixed_measure(ws, "onerdm" in mixed, "diag_twordm" in mixed)
             _record(tr.mixed, _observables(one, two, wt))

@@ -43,6 +43,10 @@ def lib():
             "afq_h5_read_psit": ([S, I, I, I, I, P], I),
             "afq_h5_read_ci": ([S, I, I, P], I),
             "afq_h5_read_occupations": ([S, I, I, I, P, P], I),
+            "afq_h5_results_append": ([S, S, S, I, P, I, ctypes.c_long, P], I),
+            "afq_h5_results_shape": ([S, S, S, ctypes.POINTER(ctypes.c_long), IP, P, IP], I),
+            "afq_h5_results_read": ([S, S, S, P], I),
+            "afq_h5_results_names": ([S, S, P, ctypes.c_long, ctypes.POINTER(ctypes.c_long)], I),
         }
         for name, (args, res) in sigs.items():
             f = getattr(L, name)
