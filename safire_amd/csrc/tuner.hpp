@@ -6,10 +6,12 @@
 // process and used from then on (also inside captured graphs).  All candidates accumulate over
 // k in the same order, so the choice never changes the result bits.  Tuning needs an
 // idempotent product (output independent of its previous contents); accumulating call sites
-// use the first candidate.  AFQ_TUNE=0 in the environment disables tuning.
+// use the first candidate.  AFQ_TUNE=0 in the environment disables tuning; AFQ_TUNE_VERBOSE=1
+// prints every candidate's time to stderr.
 #pragma once
 
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
 #include <map>
 #include <mutex>
@@ -37,6 +39,7 @@ class Tuner {
     ms_[key] = ms;
   }
   bool enabled() const { return enabled_; }
+  bool verbose() const { return verbose_; }
   std::string report() {
     std::lock_guard<std::mutex> g(mu_);
     std::string r;
@@ -50,11 +53,14 @@ class Tuner {
   Tuner() {
     const char* e = std::getenv("AFQ_TUNE");
     enabled_ = !(e && e[0] == '0');
+    const char* v = std::getenv("AFQ_TUNE_VERBOSE");
+    verbose_ = v && v[0] == '1';
   }
   std::mutex mu_;
   std::map<std::string, int> best_;
   std::map<std::string, float> ms_;
   bool enabled_ = true;
+  bool verbose_ = false;
 };
 
 template <class F>
@@ -84,6 +90,7 @@ void tuned(Context& c, const std::string& key, int ncand, F&& launch, bool idemp
     AFQ_HIP(hipEventSynchronize(e1));
     float ms = 0;
     AFQ_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (t.verbose()) std::fprintf(stderr, "tune %s candidate %d: %.4f ms\n", key.c_str(), i, ms / 2);
     if (ms < best) {
       best = ms;
       bi = i;
