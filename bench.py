@@ -277,9 +277,9 @@ def energy_error(problem, W_e=64):
         float(np.median(np.abs(res[afq.AFQ_FP64])))
 
 
-def cpu_baseline(problem, budget_s=15.0):
+def cpu_baseline(problem, budget_s=12.0):
     """The FP64 oracle (CPU restatement of the reference step, OpenMP over walkers) on a
-    bounded sample: a few walkers of the same workload for one step, repeated to ~budget."""
+    bounded sample: a few walkers of the same workload stepped for about budget_s seconds."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
     lib = orc.lib()
@@ -288,7 +288,7 @@ def cpu_baseline(problem, budget_s=15.0):
     Ws = max(threads, 8)
     st, Eshift = osys.init_walkers(problem.walkers(Ws))
     done, t_tot, step = 0, 0.0, 0
-    while t_tot < budget_s and step < 50:
+    while t_tot < budget_s and step < 1000:
         u = orc.field_uniforms(7, 1, step, Ws, problem.M)
         t0 = time.perf_counter()
         osys.step(st, u, Eshift)
