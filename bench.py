@@ -358,7 +358,7 @@ def main():
                          "serving workload; 0 skips it)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(walkers shard across GPUs; see the design notes)", file=sys.stderr)
         sys.exit(2)
 

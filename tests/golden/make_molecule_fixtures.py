@@ -6,7 +6,7 @@ the mixed and back-propagated OneRDM and DiagonalTwoRDM) with h5dump:
   BH  19 orbitals, 78 Cholesky vectors, RHF / UHF / CASCI-PHMSD trials
   N2  28 orbitals, 134 Cholesky vectors, a 50-determinant CAS trial (PHMSD)
   Li  14 orbitals, 70 Cholesky vectors, a fully polarized ROHF trial (3 + 0 electrons)
-Run in the container that holds the reference; the GPU box only reads the copies.
+Run where the reference's sources are; GPU runs only read the copies.
 
     python tests/golden/make_molecule_fixtures.py
 """
