@@ -211,3 +211,65 @@ def test_driver_schedule_matches_backpropagation(gpu):
     say(f"  3 + 3 windows, time-evolved vs back-propagated {worst:.2e}")
     prop.close()
     assert worst < 1e-9
+
+
+def test_time_evolved_collinear_vs_backpropagation(gpu):
+    """COLLINEAR walkers with different up and down trials (5 + 4 electrons): both spins'
+    operators against the oracle, and the transposed orientation against bp_measure over the
+    same window (Taylor order 16)."""
+    afq, torch = gpu
+    p = make_problem(24, 5, 96, seed=1003)
+    nup, ndn, W, seed, nback, nbp, order = 5, 4, 8, 37, 4, 3, 16
+    say(f"case: time-evolved operators, COLLINEAR N=24 nup={nup} ndown={ndn} W={W}, window {nbp}")
+    rng = np.random.default_rng(2)
+    evecs = np.linalg.eigh(p.h1)[1]
+    Q = np.linalg.qr(rng.normal(size=(6, 6)))[0]
+    dn = evecs[:, :6] @ Q[:, :ndn]
+    PsiT = np.ascontiguousarray(np.vstack([p.PsiT, dn.T]).astype(np.complex128))
+    ctx = afq.Context(0)
+    ham = afq.HamiltonianOperations(ctx, p.h1, p.Likn, PsiT, nup, ndn, walker_type=afq.AFQ_COLLINEAR)
+    prop = afq.Propagator(ham, W, dt=0.005, seed=seed, order=order)
+    prop.enable_backprop(nback)
+    Pa = np.broadcast_to(PsiT[:nup].conj().T, (W, p.N, nup)) + 0.05 * (
+        rng.normal(size=(W, p.N, nup)) + 1j * rng.normal(size=(W, p.N, nup)))
+    Pb = np.broadcast_to(PsiT[nup:].conj().T, (W, p.N, ndn)) + 0.05 * (
+        rng.normal(size=(W, p.N, ndn)) + 1j * rng.normal(size=(W, p.N, ndn)))
+    ws = afq.WalkerSet(Pa, Pb)
+    E = afq.empty((W, 3), torch.complex128)
+    ov = afq.empty(W, torch.complex128)
+    prop.Energy(ws, E, ov)
+    ws.eloc.copy_(E.sum(1))
+    ws.ovlp.copy_(ov)
+    Esh = afq.device_tensor(np.array([float(afq.host(E.sum(1)).real.mean())]))
+    prop.set_step(0)
+    prop.Propagate(ws, Esh)
+    prop.Orthogonalize(ws)
+    prop.bp_anchor(ws)
+    prop.te_anchor()
+    for _ in range(nbp):
+        prop.Propagate(ws, Esh)
+        prop.Orthogonalize(ws)
+    step = 1 + nbp
+    prop.te_propagate(ws, nbp)
+    fields, facs = prop.bp_rings(W)
+    P1 = afq.host(prop.P1())
+    Xd, Yd, Md, P1i = prop.te_operators()
+    Vs = [orc.vhs(p.Likn, fields[:, (step - nbp + i) % nback], 0.005) for i in range(nbp)]
+    I = np.broadcast_to(np.eye(p.N, dtype=complex), (W, p.N, p.N))
+    Phis = (afq.host(ws.Phi), afq.host(ws.Phi_b))
+    ops = [orc.propagate_operators(P1, P1i, Vs, I, I, np.zeros_like(I), Phis[sp], order)
+           for sp in range(2)]
+    eO = max(max(rel(Xd[:, sp], ops[sp][0]), rel(Yd[:, sp], ops[sp][1]), rel(Md[:, sp], ops[sp][2]))
+             for sp in range(2))
+    wr = _restored(afq.host(ws.weight), facs, step, nbp, nback)
+    G = np.stack([orc.time_evolved_density_matrix(r, *ops[sp], Phis[sp])
+                  for sp, r in enumerate((PsiT[:nup], PsiT[nup:]))], axis=1)
+    o1, o2 = orc.rdm_measure(G, wr, False)
+    d1, d2 = prop.te_measure(ws, nbp)
+    eM = max(rel(afq.host(d1), o1), rel(afq.host(d2), o2))
+    t1, t2 = prop.te_measure(ws, nbp, transpose_m=True)
+    b1, b2 = prop.bp_measure(ws, nbp, ortho=1, restore=1)
+    eB = max(rel(afq.host(t1), afq.host(b1)), rel(afq.host(t2), afq.host(b2)))
+    say(f"  X, Y, M {eO:.2e}; te_measure {eM:.2e}; transposed vs bp_measure {eB:.2e}")
+    prop.close()
+    assert eO < TOL and eM < TOL and eB < 1e-9
