@@ -263,11 +263,13 @@ int afq_prop_vhs_ms(afq_prop* prop, float* ms);
 
 /* ---- back-propagation by operators evolved forward in time (TimeEvolvedBPEstimator.hpp) ------
  * The operator states X, Y, M [W][nspin][N][N] (complex) of the propagator's W walker slots.
- * They stay with the slots when population control copies walkers, as the reference's
- * estimator-held arrays do.  All three calls need afq_prop_enable_backprop (the field and
- * factor rings) and the full walker count W.
- * afq_prop_te_anchor: TimeEvolvedBPEstimator::setAnchor (:271-276), X = Y = I, M = 0. */
-int afq_prop_te_anchor(afq_prop* prop);
+ * All three calls need afq_prop_enable_backprop (the field and factor rings) and the full
+ * walker count W.
+ * afq_prop_te_anchor: TimeEvolvedBPEstimator::setAnchor (:271-276), X = Y = I, M = 0.
+ * follow_walkers = 0 leaves the operators with their slots when population control copies
+ * walkers, as the reference's estimator-held arrays do (a copy then carries another path's
+ * operators); 1 makes population control copy them with their walkers. */
+int afq_prop_te_anchor(afq_prop* prop, int follow_walkers);
 /* AFQMCBasePropagator::PropagateOperators (AFQMCBasePropagator.cpp:538-684): the nsteps most
  * recent stored fields, oldest first, each X <- P1 expM(V) P1 X and
  * Y <- P1inv expM(-V)^H P1inv Y (FP64, P1inv = exp(+1/2 H1) of generateP1(true)); then, with

@@ -97,7 +97,7 @@ _SIGS = {
     "afq_prop_bp_measure": [_P, ctypes.POINTER(Walkers), _I, _I, _I, _P, _P],
     "afq_prop_vhs_timing": [_P, _I],
     "afq_prop_vhs_ms": [_P, ctypes.POINTER(ctypes.c_float)],
-    "afq_prop_te_anchor": [_P],
+    "afq_prop_te_anchor": [_P, _I],
     "afq_prop_te_propagate": [_P, ctypes.POINTER(Walkers), _I],
     "afq_prop_te_measure": [_P, ctypes.POINTER(Walkers), _I, _I, _P, _P],
     "afq_prop_te_views": [_P, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_P),
@@ -584,9 +584,11 @@ class Propagator:
         return float(v.value)
 
     # ---- back-propagation by time-evolved operators (TimeEvolvedBPEstimator) ----
-    def te_anchor(self):
-        """X = Y = I and M = 0 for every walker slot (setAnchor)."""
-        _ck(lib().afq_prop_te_anchor(self.h))
+    def te_anchor(self, follow_walkers: bool = False):
+        """X = Y = I and M = 0 for every walker slot (setAnchor); follow_walkers makes
+        population control copy the operators with their walkers (the reference leaves them
+        with the slots)."""
+        _ck(lib().afq_prop_te_anchor(self.h, int(bool(follow_walkers))))
 
     def te_propagate(self, wset: WalkerSet, nsteps: int):
         """PropagateOperators over the nsteps most recent stored fields, then the

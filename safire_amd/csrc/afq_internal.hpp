@@ -51,8 +51,9 @@ void restore_weights(hipStream_t s, int W, const z64* weight, const z64* wfac, i
                      int n, z64* out);
 // per-walker rows that move with their walker in population control (back-propagation state)
 struct WalkerRows {
-  z64* p[4] = {nullptr, nullptr, nullptr, nullptr};
-  long n[4] = {0, 0, 0, 0};
+  static constexpr int kMax = 7;
+  z64* p[kMax] = {};
+  long n[kMax] = {};
 };
 void increment_counter(hipStream_t s, uint32_t* c);
 // comb population control of one rank (population_control.hpp:216-257); r_dev = injected

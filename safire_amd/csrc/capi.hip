@@ -472,10 +472,10 @@ int afq_prop_vhs_ms(afq_prop* prop, float* ms) {
   });
 }
 
-int afq_prop_te_anchor(afq_prop* prop) {
+int afq_prop_te_anchor(afq_prop* prop, int follow_walkers) {
   return guard([&] {
     REQUIRE(prop, "NULL argument");
-    prop->p->te_anchor();
+    prop->p->te_anchor(follow_walkers != 0);
   });
 }
 

@@ -325,7 +325,7 @@ __global__ void k_copy_walkers(int W, const int* __restrict__ src, long rowA, lo
   if (PhiB)
     for (long e = threadIdx.x; e < rowB; e += blockDim.x) PhiB[d * rowB + e] = PhiB[s * rowB + e];
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
+  for (int r = 0; r < WalkerRows::kMax; ++r) {
     z64* q = extra.p[r];
     const long n = extra.n[r];
     if (q)

@@ -365,10 +365,11 @@ class Propagator {
   // ---- back-propagation by operators evolved forward in time (TimeEvolvedBPEstimator.hpp;
   // AFQMCBasePropagator::PropagateOperators, AFQMCBasePropagator.cpp:538-684).  Uses the field
   // and factor rings of enable_backprop.  X, Y, M [W][nspin][N][N] (nspin 1 CLOSED, 2
-  // COLLINEAR) belong to walker slots, not to walkers: population control does not move them
-  // (the reference keeps them in the estimator).
+  // COLLINEAR).  As in the reference (the estimator holds them) they belong to walker slots
+  // and population control leaves them where they are; with follow_walkers it copies them
+  // with their walkers, so that a copied walker keeps the operators of its own path.
   // setAnchor: X = Y = I, M = 0
-  void te_anchor();
+  void te_anchor(bool follow_walkers = false);
   // PropagateOperators: replays the nsteps most recent stored fields, oldest first,
   // X <- P1 expM(V) P1 X and Y <- P1inv expM(-V)^H P1inv Y, then regularizes with the current
   // walker matrices S (orthonormal after a QR): Y <- S S^H Y, M += Y^H X, X <- X - S S^H X.
@@ -428,6 +429,7 @@ class Propagator {
   z64* ref_cols_[2] = {nullptr, nullptr};
   // time-evolved operators and the inverse one-body propagator
   z64* te_op_[3] = {nullptr, nullptr, nullptr};
+  bool te_follow_ = false;
   z64* P1inv_ = nullptr;
   double* P1invr_ = nullptr;
   void ensure_references();

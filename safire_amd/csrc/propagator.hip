@@ -240,6 +240,13 @@ void Propagator::population_control(const WalkerState& st, const double* r_dev) 
       extra.n[3] = (long)N * ndown;
     }
   }
+  if (te_follow_ && te_op_[0]) {  // the time-evolved operators move with their walkers
+    const long rowte = (long)(closed_ ? 1 : 2) * N * N;
+    for (int i = 0; i < 3; ++i) {
+      extra.p[4 + i] = te_op_[i];
+      extra.n[4 + i] = rowte;
+    }
+  }
   afq::population_control(ctx_.stream(), st.W, r_dev, p_.seed, p_.stream + 1000u, step_dev_,
                           (long)N * nup, coll ? (long)N * ndown : 0, st.Phi, coll ? st.Phi_b : nullptr,
                           st.weight, st.phase, st.eloc, st.ovlp, st.theta, cum, src, status_dev_,
@@ -630,12 +637,21 @@ const z64* Propagator::te_operator(int which) const {
   return te_op_[which];
 }
 
-void Propagator::te_anchor() {
+void Propagator::te_anchor(bool follow_walkers) {
   hipStream_t s = ctx_.stream();
   const int N = ham_.N();
   const int nsp = closed_ ? 1 : 2;
   const size_t cnt = (size_t)W_ * nsp * N * N;
+  if (follow_walkers != te_follow_) {
+    AFQ_HIP(hipStreamSynchronize(s));
+    clear_graphs();  // captured population control moves different rows
+    te_follow_ = follow_walkers;
+  }
   if (!te_op_[0]) {
+    if (te_follow_) {
+      AFQ_HIP(hipStreamSynchronize(s));
+      clear_graphs();
+    }
     z64* buf[3] = {nullptr, nullptr, nullptr};
     try {
       for (int i = 0; i < 3; ++i) buf[i] = dalloc<z64>(cnt);

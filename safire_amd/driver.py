@@ -113,7 +113,7 @@ def run_afqmc(prop: "afq.Propagator", ws: "afq.WalkerSet", steps: int, equilibra
                     "onerdm": {}, "diag_twordm": {}},
        "time_evolved_bp": {"propagation_steps": [L1, L2, ...], "path_restoration": True,
                            "extra_path_restoration": False, "transpose_m": False,
-                           "onerdm": {}, "diag_twordm": {}}}
+                           "follow_walkers": False, "onerdm": {}, "diag_twordm": {}}}
     The mixed estimator (MixedEstimator.hpp:122-140) measures its observables on the current
     walkers every measure_interval steps after equilibration.  Both estimators sum over the
     trial's reference determinants (one for NOMSD, the expansion for PHMSD).  Back-propagation (BackPropEstimator.hpp:150-260) anchors after
@@ -166,6 +166,7 @@ def run_afqmc(prop: "afq.Propagator", ws: "afq.WalkerSet", steps: int, equilibra
         te_lens = lengths(te)
         te_restore = restoration(te)
         te_transpose = bool(te.get("transpose_m", False))
+        te_follow = bool(te.get("follow_walkers", False))
         for L in te_lens:
             tr.time_evolved_bp[L] = {}
     nback = max(bp_lens[-1:] + te_lens[-1:] + [0])
@@ -175,7 +176,7 @@ def run_afqmc(prop: "afq.Propagator", ws: "afq.WalkerSet", steps: int, equilibra
     if bp_lens:
         prop.bp_anchor(ws)
     if te_lens:
-        prop.te_anchor()
+        prop.te_anchor(te_follow)
     for step in range(steps):
         prop.step_graph(ws, Eshift, ortho=(step % ortho_interval == 0),
                         popctl=(popctl_interval > 0 and step % popctl_interval == 0),
@@ -218,7 +219,7 @@ def run_afqmc(prop: "afq.Propagator", ws: "afq.WalkerSet", steps: int, equilibra
                     bp_pos = step
         if te_lens:
             if step == equilibration - 1:
-                prop.te_anchor()
+                prop.te_anchor(te_follow)
                 te_pos = step
             elif step >= equilibration:
                 n = step - te_pos
@@ -230,7 +231,7 @@ def run_afqmc(prop: "afq.Propagator", ws: "afq.WalkerSet", steps: int, equilibra
                                                "onerdm" in te, "diag_twordm" in te)
                     _record(tr.time_evolved_bp[n], _observables(one, two, wt))
                 if n >= te_lens[-1]:
-                    prop.te_anchor()
+                    prop.te_anchor(te_follow)
                     te_pos = step
         if progress is not None and (step + 1) % progress_interval == 0:
             progress(step + 1, tr)

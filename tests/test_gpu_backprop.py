@@ -304,16 +304,20 @@ def test_population_control_moves_backprop_rows(gpu):
     for _ in range(6):
         prop.Propagate(ws, Esh)
     prop.bp_anchor(ws)
+    prop.te_anchor(follow_walkers=True)
     for _ in range(2):
         prop.Propagate(ws, Esh)
+    prop.te_propagate(ws, 2)   # per-walker operators
     # spread weights, so that the comb drops some walkers and copies others
     rng = np.random.default_rng(5)
     ws.weight.copy_(torch.from_numpy(rng.uniform(0.1, 2.0, W).astype(np.complex128)))
     Phi0 = afq.host(ws.Phi)
     f0, q0 = prop.bp_rings(W)
+    t0 = prop.te_operators()[:3]
     prop.population_control(ws, 0.37)
     Phi1 = afq.host(ws.Phi)
     f1, q1 = prop.bp_rings(W)
+    t1 = prop.te_operators()[:3]
     moved = 0
     for d in range(W):
         src = [s for s in range(W) if np.array_equal(Phi1[d], Phi0[s])]
@@ -321,7 +325,9 @@ def test_population_control_moves_backprop_rows(gpu):
         s = src[0]
         moved += s != d
         assert np.array_equal(f1[d], f0[s]) and np.array_equal(q1[d], q0[s])
-    say(f"  {moved} walkers replaced by copies")
+        # the time-evolved operators were anchored with follow_walkers
+        assert all(np.array_equal(b[d], a[s]) for a, b in zip(t0, t1))
+    say(f"  {moved} walkers replaced by copies (rings, anchors and X, Y, M with them)")
     assert moved > 0
     prop.close()
 

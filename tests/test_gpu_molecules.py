@@ -196,3 +196,42 @@ def test_statistical_reference(gpu, molecule, case, prec):
             ok &= compare(f"{group} {name} ({len(store[name])} measurements)", mean, err,
                           _cplx(ref["mean"]), _cplx(ref["error"]))
     assert ok
+
+
+def test_time_evolved_matches_backprop_reference(gpu):
+    """BH with the RHF trial (FP64): the time-evolved estimator with M transposed, over the
+    reference's back-propagation window (100 steps, path restoration), agrees with the
+    reference's back-propagated OneRDM and DiagonalTwoRDM under the runner's criterion.  Its
+    regularization is exact for orthonormal walkers, so the walkers are orthogonalized every
+    step, the Taylor series runs at order 12 (the estimator needs Y = X^{-H}), and population
+    control copies the operators with their walkers."""
+    afq, ctx, torch = gpu
+    from safire_amd import driver
+    gdir = os.path.join(GOLD, "bh")
+    r = json.load(open(os.path.join(gdir, "statistical_refs.json")))["rhf_nomsd/closed"]
+    say("case: BH time-evolved back-propagation rhf_nomsd/closed fp64")
+    s = h5inputs.load_system(os.path.join(gdir, r["hamiltonian"]),
+                             os.path.join(gdir, r["wavefunction"]), walker_type=afq.AFQ_CLOSED)
+    W = r["walkers"]
+    ham = afq.HamiltonianOperations(ctx, s.h1, s.Likn, s.PsiT, s.nup, s.ndown,
+                                    walker_type=afq.AFQ_CLOSED, E0=s.E0)
+    prop = afq.Propagator(ham, W, dt=r["timestep"], seed=42, order=12)
+    ws = afq.WalkerSet(*s.walkers(W))
+    bp = r["estimators"]["backprop"]
+    te = {"propagation_steps": bp["propagation_steps"], "path_restoration": bp["path_restoration"],
+          "transpose_m": True, "follow_walkers": True, "onerdm": {}, "diag_twordm": {}}
+    eq = r["equilibration_steps"]
+    tr = driver.run_afqmc(prop, ws, steps=eq + 9000, equilibration=eq, ortho_interval=1,
+                          popctl_interval=r["population_control_interval"],
+                          measure_interval=r["measure_interval"],
+                          estimators={"time_evolved_bp": te},
+                          progress=lambda st, t: say(f"  step {st}"), progress_interval=3000)
+    prop.close()
+    L = bp["propagation_steps"][0]
+    ok = True
+    for name, ref in r["rdm"][f"backprop/{L}"].items():
+        series = tr.time_evolved_bp[L][name]
+        mean, err = driver.mean_error(series)
+        ok &= compare(f"time-evolved/{L} {name} ({len(series)} measurements)", mean, err,
+                      _cplx(ref["mean"]), _cplx(ref["error"]))
+    assert ok
