@@ -797,7 +797,8 @@ void Propagator::te_measure(const WalkerState& st, int restore_steps, bool trans
       if (n == 0) continue;
       const z64* ref = ref_rows_[sp] + (size_t)d * n * N;  // Ref_d^H [n][N]
       const z64 *X = te_op_[0] + sp * NN, *Y = te_op_[1] + sp * NN;
-      // Gc = T^{-T} Phi^T with T = Ref^H Phi
+      // Gc = T^{-T} Phi^T with T = Ref^H Phi (the log overlap it also returns is not used)
+      fill_z64(s, W, {0.0, 0.0}, dum);
       det_ops::mixed_density_matrix(ctx_, ref, n, N, Phi[sp], (long)N * n, W, Gc, (long)n * N,
                                     dum, true);
       {  // GY = Gc conj(Y)
