@@ -166,6 +166,10 @@ typedef struct {
   int free_projection, use_cp, project_force_bias;
   uint64_t seed;             /* field RNG key */
   uint32_t stream;           /* field RNG stream index */
+  int local_energy;          /* 0: hybrid step (default); 1: the reference's hybrid = false --
+                                the local energy of the propagated walkers (Energy instead of
+                                Log_Overlap, AFQMCBasePropagator.cpp:368-373) drives the weights
+                                with the midpoint rule (WalkerSetUpdate.hpp:120-146) */
 } afq_prop_params;
 
 typedef struct {

@@ -602,13 +602,15 @@ void orc_construct_x(const double* u, const double* vMF_, double bound, int proj
   }
 }
 
-// walker_update (WalkerSetUpdate.hpp:70-191), hybrid branch, phaseless (no cp constraint
-// unless use_cp), followed by bound_walker_weights (:199-218).
+// walker_update (WalkerSetUpdate.hpp:70-191), phaseless (no cp constraint unless use_cp),
+// followed by bound_walker_weights (:199-218).  E_ == nullptr: the hybrid branch; otherwise
+// hybrid = false with the local energies E [W][3] (E0 included) and the midpoint rule (:120-146).
 void orc_walker_update(int W, double dt, double Eshift, double E0, const double* new_ovlp_,
                        const double* mf_, const double* HW_, double lower_cut, double upper_cut,
                        int free_projection, int use_cp, double weight_floor, double weight_frac,
                        double target_pop, double* weight_, double* phase_, double* eloc_,
-                       double* ovlp_, double* theta_) {
+                       double* ovlp_, double* theta_, const double* E_) {
+  const cd* Eloc = (const cd*)E_;
   const cd* nov = (const cd*)new_ovlp_;
   const cd* mf = (const cd*)mf_;
   const cd* HW = (const cd*)HW_;
@@ -627,14 +629,16 @@ void orc_walker_update(int W, double dt, double Eshift, double E0, const double*
       scale = use_cp ? (std::cos(dth) > 0.0 ? 1.0 : 0.0) : std::max(0.0, std::cos(dth));
     }
     theta[i] = dth;
-    cd eloc = (mf[i] - HW[i] - (nov[i] - old_ovlp)) / dt + E0;
+    cd eloc = Eloc ? Eloc[3 * i] + Eloc[3 * i + 1] + Eloc[3 * i + 2]
+                   : (mf[i] - HW[i] - (nov[i] - old_ovlp)) / dt + E0;
     if (!std::isfinite(eloc.real())) {
       weight[i] = 0.0;
       phase[i] = 0.0;
       eloc = old_eloc;
     } else {
       if (!free_projection) eloc = cd(std::min(std::max(eloc.real(), lo), hi), eloc.imag());
-      cd wf = std::exp(-dt * (eloc - Eshift));
+      const cd eff = Eloc ? 0.5 * (eloc + old_eloc) : eloc;
+      cd wf = std::exp(-dt * (eff - Eshift));
       weight[i] *= free_projection ? scale * wf : scale * std::abs(wf);
       phase[i] *= wf / std::abs(wf);
     }
@@ -790,6 +794,8 @@ struct orc_problem {
   const double* P1;     // [N][N] complex (truncated)
   const double* vMF;    // [M] complex
   double dt, E0, vbias_bound, lower_cut, upper_cut, weight_floor, weight_frac, target_pop;
+  const double* haj;    // [nel][N] complex (local-energy steps)
+  int local_energy;     // 1: hybrid = false (AFQMCBasePropagator.cpp:368-373)
 };
 
 // Advances every walker by one step. u: uniforms [W][M] (the field draws of this step).
@@ -798,7 +804,7 @@ void orc_step(const orc_problem* P, int W, const double* u, double Eshift, doubl
               double* weight_, double* phase_, double* eloc_, double* ovlp_, double* theta_) {
   const int N = P->N, n = P->nel, M = P->M;
   const cd* PsiT = (const cd*)P->PsiT;
-  std::vector<cd> new_ovlp(W), mf(W), HW(W);
+  std::vector<cd> new_ovlp(W), mf(W), HW(W), E(P->local_energy ? (size_t)W * 3 : 0);
 #pragma omp parallel
   {
     std::vector<cd> G((size_t)n * N), X(M), V((size_t)N * N), T((size_t)n * n);
@@ -816,15 +822,24 @@ void orc_step(const orc_problem* P, int W, const double* u, double Eshift, doubl
       mf[w] = mfw;
       orc_vhs(P->Likn, (const double*)X.data(), 1, N, M, P->dt, (double*)V.data());
       orc_propagate(P->P1, (const double*)V.data(), (double*)Phi, 1, N, n, P->order, 'N');
-      cd lo = 0.0;
-      log_overlap_one(PsiT, Phi, N, n, T.data(), lo, false);
-      new_ovlp[w] = 2.0 * lo;  // CLOSED (NOMSD.cpp:293-295); log(conj(ci0)) = 0 for ci0 = 1
+      if (P->local_energy) {  // Energy of the propagated walker (NOMSD::Energy)
+        cd ovn = 0.0;
+        orc_mixed_density_matrix(P->PsiT, (const double*)Phi, 1, N, n, (double*)G.data(),
+                                 (double*)&ovn, 1);
+        orc_energy(P->haj, P->Lnak, (const double*)G.data(), 1, N, n, 0, M, 1, P->E0,
+                   (double*)(E.data() + (size_t)w * 3));
+        new_ovlp[w] = 2.0 * ovn;
+      } else {
+        cd lo = 0.0;
+        log_overlap_one(PsiT, Phi, N, n, T.data(), lo, false);
+        new_ovlp[w] = 2.0 * lo;  // CLOSED (NOMSD.cpp:293-295); log(conj(ci0)) = 0 for ci0 = 1
+      }
     }
   }
   orc_walker_update(W, P->dt, Eshift, P->E0, (const double*)new_ovlp.data(),
                     (const double*)mf.data(), (const double*)HW.data(), P->lower_cut, P->upper_cut,
                     0, 0, P->weight_floor, P->weight_frac, P->target_pop, weight_, phase_, eloc_,
-                    ovlp_, theta_);
+                    ovlp_, theta_, P->local_energy ? (const double*)E.data() : nullptr);
 }
 
 // comb (population_control.hpp:34-64): offspring counts for |w| with the uniform r, and the

@@ -46,7 +46,7 @@ def lib():
             "orc_field_uniforms": [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _I, _I, _P],
             "orc_construct_x": [_P, _P, _D, _I, _I, _I, _I, _P, _P, _P],
             "orc_walker_update": [_I, _D, _D, _D, _P, _P, _P, _D, _D, _I, _I, _D, _D, _D,
-                                  _P, _P, _P, _P, _P],
+                                  _P, _P, _P, _P, _P, _P],
             "orc_average_pseudo_energy": [_I, _P, _P, _D],
             "orc_half_rotate": [_P, _P, _P, _I, _I, _I, _P, _P],
             "orc_vexx": [_P, _I, _I, _P],
@@ -215,7 +215,7 @@ def walker_update(state, dt, Eshift, E0, new_ovlp, mf, HW, lower_cut=1.0, upper_
     lib().orc_walker_update(W, dt, Eshift, E0, _p(new_ovlp), _p(mf), _p(HW), lower_cut, upper_cut,
                             int(free_projection), int(use_cp), weight_floor, weight_frac,
                             float(target_pop), _p(state["weight"]), _p(state["phase"]),
-                            _p(state["eloc"]), _p(state["ovlp"]), _p(state["theta"]))
+                            _p(state["eloc"]), _p(state["ovlp"]), _p(state["theta"]), None)
 
 
 def average_pseudo_energy(weight, eloc, dt):
@@ -306,7 +306,8 @@ class _Problem(ctypes.Structure):
     _fields_ = [("N", _I), ("nel", _I), ("M", _I), ("order", _I),
                 ("Likn", _P), ("Lnak", _P), ("PsiT", _P), ("P1", _P), ("vMF", _P),
                 ("dt", _D), ("E0", _D), ("vbias_bound", _D), ("lower_cut", _D), ("upper_cut", _D),
-                ("weight_floor", _D), ("weight_frac", _D), ("target_pop", _D)]
+                ("weight_floor", _D), ("weight_frac", _D), ("target_pop", _D),
+                ("haj", _P), ("local_energy", _I)]
 
 
 class OracleSystem:
@@ -327,11 +328,12 @@ class OracleSystem:
         self.target_pop = target_pop
         self._keep = []
 
-    def struct(self, W):
+    def struct(self, W, local_energy=False):
         s = _Problem(self.problem.N, self.problem.nup, self.problem.M, self.order,
                      _p(self.Likn), _p(self.Lnak), _p(self.PsiT), _p(self.P1), _p(self.vMF),
                      self.dt, self.E0, self.vbias_bound, 1.0, 10.0, 100.0, 0.1,
-                     float(self.target_pop if self.target_pop is not None else W))
+                     float(self.target_pop if self.target_pop is not None else W),
+                     _p(self.haj), int(local_energy))
         return s
 
     def init_walkers(self, Phi):
@@ -348,9 +350,9 @@ class OracleSystem:
         Eshift = float((np.sum(state["weight"] * eloc) / np.sum(state["weight"])).real)
         return state, Eshift
 
-    def step(self, state, u, Eshift):
+    def step(self, state, u, Eshift, local_energy=False):
         W = state["Phi"].shape[0]
-        s = self.struct(W)
+        s = self.struct(W, local_energy)
         u = _c(u, np.float64)
         lib().orc_step(ctypes.byref(s), W, _p(u), Eshift, _p(state["Phi"]), _p(state["weight"]),
                        _p(state["phase"]), _p(state["eloc"]), _p(state["ovlp"]), _p(state["theta"]))

@@ -43,7 +43,7 @@ class PropParams(ctypes.Structure):
     _fields_ = [("dt", _D), ("order", _I), ("vbias_bound", _D), ("lower_cut", _D),
                 ("upper_cut", _D), ("weight_floor", _D), ("weight_frac", _D), ("target_pop", _D),
                 ("free_projection", _I), ("use_cp", _I), ("project_force_bias", _I),
-                ("seed", ctypes.c_uint64), ("stream", ctypes.c_uint32)]
+                ("seed", ctypes.c_uint64), ("stream", ctypes.c_uint32), ("local_energy", _I)]
 
 
 class Walkers(ctypes.Structure):
@@ -461,14 +461,15 @@ class Propagator:
     trial (safire_amd.phmsd.PHMSD) built on the same hamiltonian."""
 
     def __init__(self, ham: HamiltonianOperations, W, dt=0.005, order=6, vbias_bound=50.0,
-                 seed=0, stream=1, target_pop=0.0, free_projection=False, trial=None):
+                 seed=0, stream=1, target_pop=0.0, free_projection=False, trial=None,
+                 local_energy=False):
         self.ham = ham
         self.ctx = ham.ctx
         self.dt = dt
         self.W = int(W)
         self.trial = trial
         p = PropParams(dt, order, vbias_bound, 1.0, 10.0, 100.0, 0.1, target_pop,
-                       int(free_projection), 0, 0, seed, stream)
+                       int(free_projection), 0, 0, seed, stream, int(bool(local_energy)))
         h = _P()
         if trial is None:
             _ck(lib().afq_prop_create(ham.h, W, ctypes.byref(p), ctypes.byref(h)))

@@ -31,6 +31,7 @@ void field_uniforms(hipStream_t s, uint64_t seed, uint32_t stream, uint32_t step
 struct UpdateParams {
   double dt, E0, lower_cut, upper_cut, weight_floor, weight_frac, target_pop;
   int free_projection, use_cp;
+  int local_energy;  // 1: E_loc = sum of the energies E [W][3] (E0 included), midpoint rule
 };
 // Eshift is read from device memory (so a captured step needs no host value).  When wfac is
 // given, the step's path-restoration factor (w / |w|) / scale of every walker (0 for a dropped
@@ -39,7 +40,7 @@ struct UpdateParams {
 void walker_update(hipStream_t s, int W, const UpdateParams& p, const double* Eshift,
                    const z64* new_ovlp, const z64* mf, const z64* HW, z64* weight, z64* phase,
                    z64* eloc, z64* ovlp, z64* theta, z64* wfac = nullptr, int hl = 0,
-                   const uint32_t* step_ptr = nullptr);
+                   const uint32_t* step_ptr = nullptr, const z64* E = nullptr);
 // ring[w][*step_ptr % nring][m] = X[w][m] (WalkerSetBase::storeFields)
 void store_fields(hipStream_t s, int W, int M, const z64* X, z64* ring, int nring,
                   const uint32_t* step_ptr);

@@ -263,7 +263,7 @@ int afq_walker_update(afq_ctx* ctx, int W, const afq_update_params* p, const dou
     REQUIRE(ctx && p && Eshift && new_ovlp && mf && HW && weight && phase && eloc && ovlp && theta,
             "NULL argument");
     afq::UpdateParams up{p->dt, p->E0, p->lower_cut, p->upper_cut, p->weight_floor, p->weight_frac,
-                         p->target_pop, p->free_projection, p->use_cp};
+                         p->target_pop, p->free_projection, p->use_cp, 0};
     afq::walker_update(ctx->c.stream(), W, up, Eshift, Z(new_ovlp), Z(mf), Z(HW), Z(weight),
                        Z(phase), Z(eloc), Z(ovlp), Z(theta));
   });
@@ -290,6 +290,7 @@ static afq::PropagatorParams to_params(const afq_prop_params* p) {
   pp.free_projection = p->free_projection;
   pp.use_cp = p->use_cp;
   pp.project_force_bias = p->project_force_bias;
+  pp.local_energy = p->local_energy;
   pp.seed = p->seed;
   pp.stream = p->stream;
   return pp;

@@ -145,7 +145,8 @@ __global__ void k_walker_update(int W, UpdateParams p, const double* __restrict_
                                 const z64* __restrict__ HW, z64* __restrict__ weight,
                                 z64* __restrict__ phase, z64* __restrict__ eloc_p,
                                 z64* __restrict__ ovlp, z64* __restrict__ theta,
-                                z64* __restrict__ wfac, int hl, const uint32_t* __restrict__ step_ptr) {
+                                z64* __restrict__ wfac, int hl, const uint32_t* __restrict__ step_ptr,
+                                const z64* __restrict__ E) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= W) return;
   const double Eshift = *Eshift_p;
@@ -158,9 +159,13 @@ __global__ void k_walker_update(int W, UpdateParams p, const double* __restrict_
     scale = p.use_cp ? (cos(dth) > 0.0 ? 1.0 : 0.0) : fmax(0.0, cos(dth));
   }
   theta[i] = {dth, 0.0};
-  // hybrid local energy: (mf - HW - (ov_new - ov_old)) / dt + E0
-  z64 e = {(m.re - hw.re - (nv.re - old_ov.re)) / p.dt + p.E0,
-           (m.im - hw.im - (nv.im - old_ov.im)) / p.dt};
+  // hybrid local energy: (mf - HW - (ov_new - ov_old)) / dt + E0; or the local energy of the
+  // propagated walker (E1 + EXX + EJ, E0 inside E1)
+  z64 e = p.local_energy
+              ? z64{E[3 * i].re + E[3 * i + 1].re + E[3 * i + 2].re,
+                    E[3 * i].im + E[3 * i + 1].im + E[3 * i + 2].im}
+              : z64{(m.re - hw.re - (nv.re - old_ov.re)) / p.dt + p.E0,
+                    (m.im - hw.im - (nv.im - old_ov.im)) / p.dt};
   z64 w = weight[i], ph = phase[i];
   z64 fac = {0.0, 0.0};  // path-restoration factor of the step
   if (!isfinite(e.re)) {
@@ -169,7 +174,10 @@ __global__ void k_walker_update(int W, UpdateParams p, const double* __restrict_
     e = old_e;
   } else {
     if (!p.free_projection) e.re = fmin(fmax(e.re, lo), hi);
-    z64 wf = cexp({-p.dt * (e.re - Eshift), -p.dt * e.im});
+    // the hybrid expression is the log of the step factor itself; a local energy enters
+    // through the midpoint of the old and the new time slice
+    const z64 ee = p.local_energy ? z64{0.5 * (e.re + old_e.re), 0.5 * (e.im + old_e.im)} : e;
+    z64 wf = cexp({-p.dt * (ee.re - Eshift), -p.dt * ee.im});
     double aw = hypot(wf.re, wf.im);
     if (scale > 2.2250738585072014e-308) fac = {wf.re / aw / scale, wf.im / aw / scale};
     if (p.free_projection) {
@@ -394,11 +402,13 @@ void field_uniforms(hipStream_t s, uint64_t seed, uint32_t stream, uint32_t step
 
 void walker_update(hipStream_t s, int W, const UpdateParams& p, const double* Eshift,
                    const z64* new_ovlp, const z64* mf, const z64* HW, z64* weight, z64* phase,
-                   z64* eloc, z64* ovlp, z64* theta, z64* wfac, int hl, const uint32_t* step_ptr) {
+                   z64* eloc, z64* ovlp, z64* theta, z64* wfac, int hl, const uint32_t* step_ptr,
+                   const z64* E) {
   if (W <= 0) return;
   AFQ_CHECK(!wfac || (hl > 0 && step_ptr), kErrArg, "weight-factor ring needs a length and a step");
+  AFQ_CHECK(!p.local_energy || E, kErrArg, "local-energy update needs the energies");
   hipLaunchKernelGGL(k_walker_update, dim3(ceil_div(W, 256)), dim3(256), 0, s, W, p, Eshift,
-                     new_ovlp, mf, HW, weight, phase, eloc, ovlp, theta, wfac, hl, step_ptr);
+                     new_ovlp, mf, HW, weight, phase, eloc, ovlp, theta, wfac, hl, step_ptr, E);
   AFQ_LAUNCH_CHECK();
 }
 

@@ -283,6 +283,7 @@ struct PropagatorParams {
   int free_projection = 0, use_cp = 0, project_force_bias = 0;
   uint64_t seed = 0;
   uint32_t stream = 1;
+  int local_energy = 0;  // 1: hybrid = false (the local energy drives the weights)
 };
 
 struct WalkerState {

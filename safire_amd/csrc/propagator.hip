@@ -177,13 +177,19 @@ void Propagator::step_impl(const WalkerState& st, const double* u, const double*
     pa.phi_bs = (long)N * ndown;
     det_ops::propagate(ctx_, pa);
   }
-  // 5. new overlaps (hybrid)
-  log_overlap(st, nov);
+  // 5. new overlaps (hybrid), or the local energies and overlaps (hybrid = false)
+  z64* En = nullptr;
+  if (p_.local_energy) {
+    En = scratch_as<z64>(ctx_, "st_E", (size_t)W * 3);
+    energy(st, En, nov, prec_);
+  } else {
+    log_overlap(st, nov);
+  }
   // 6. weights
   UpdateParams up{p_.dt, ham_.E0(), p_.lower_cut, p_.upper_cut, p_.weight_floor, p_.weight_frac,
-                  p_.target_pop, p_.free_projection, p_.use_cp};
+                  p_.target_pop, p_.free_projection, p_.use_cp, p_.local_energy};
   walker_update(s, W, up, Eshift, nov, mf, HW, st.weight, st.phase, st.eloc, st.ovlp, st.theta,
-                bp_n_ > 0 ? bp_wfac_ : nullptr, bp_hl_, step_dev_);
+                bp_n_ > 0 ? bp_wfac_ : nullptr, bp_hl_, step_dev_, En);
 }
 
 void Propagator::set_vhs_timing(bool on) {

@@ -337,12 +337,12 @@ def test_construct_x_and_update_vs_oracle(gpu):
 
 # ---- full step -------------------------------------------------------------------------------
 
-def _run_pair(afq, torch, p, W, steps, seed=11, ortho_every=5, precision=0):
+def _run_pair(afq, torch, p, W, steps, seed=11, ortho_every=5, precision=0, local_energy=False):
     """Advance W walkers `steps` steps on the GPU and in the oracle from the same state with the
     same injected uniforms; yields (step, gpu_state, oracle_state) after every step."""
     ctx = afq.Context(0)
     ham = afq.HamiltonianOperations(ctx, p.h1, p.Likn, p.PsiT, p.nup, precision=precision)
-    prop = afq.Propagator(ham, W, dt=0.005)
+    prop = afq.Propagator(ham, W, dt=0.005, local_energy=local_energy)
     osys = orc.OracleSystem(p, dt=0.005)
     st, Eshift = osys.init_walkers(p.walkers(W))
     ws = afq.WalkerSet(st["Phi"])
@@ -352,7 +352,7 @@ def _run_pair(afq, torch, p, W, steps, seed=11, ortho_every=5, precision=0):
     for step in range(steps):
         u = orc.field_uniforms(seed, 1, step, W, p.M)
         prop.Propagate(ws, Esh, u=afq.device_tensor(u))
-        osys.step(st, u, Eshift)
+        osys.step(st, u, Eshift, local_energy=local_energy)
         if step % ortho_every == 0:
             prop.Orthogonalize(ws)
             osys.orthogonalize(st)
@@ -364,6 +364,18 @@ def test_step_vs_oracle_cfg1(gpu):
     p = make_problem(24, 5, 96, seed=1001)
     say("case: full step cfg1 N=24 n=5 M=96 W=64, 10 steps, QR at 0 and 5")
     for step, ws, st, c in _run_pair(afq, torch, p, 64, 10):
+        assert rel(afq.host(ws.Phi), st["Phi"]) < TOL, step
+        for k in ("ovlp", "eloc", "weight", "phase", "theta"):
+            assert rel(afq.host(getattr(ws, k)), st[k]) < TOL, (step, k)
+
+
+def test_local_energy_step_vs_oracle(gpu):
+    """hybrid = false: the local energy of the propagated walkers (Energy instead of
+    Log_Overlap) drives the weights with the midpoint rule."""
+    afq, ctx, torch = gpu
+    p = make_problem(24, 5, 96, seed=1001)
+    say("case: local-energy step cfg1 N=24 n=5 M=96 W=32, 6 steps, QR at 0 and 5")
+    for step, ws, st, c in _run_pair(afq, torch, p, 32, 6, local_energy=True):
         assert rel(afq.host(ws.Phi), st["Phi"]) < TOL, step
         for k in ("ovlp", "eloc", "weight", "phase", "theta"):
             assert rel(afq.host(getattr(ws, k)), st[k]) < TOL, (step, k)

@@ -343,3 +343,31 @@ def test_comb_properties():
     assert np.allclose(st["weight"], w[src] / np.abs(w[src]))
     with pytest.raises(ValueError):
         orc.comb(np.zeros(5, complex), 0.5)
+
+
+def test_local_energy_step_uses_the_energy_of_the_propagated_walkers():
+    """hybrid = false (AFQMCBasePropagator.cpp:368-373, WalkerSetUpdate.hpp:120-146): the walkers
+    move exactly as in a hybrid step with the same fields; PSEUDO_ELOC_ becomes the bounded
+    local energy of the propagated walkers (Energy, pinned separately) and the weight takes
+    |exp(-dt ((E_new + E_old) / 2 - Eshift))| times the phaseless scale."""
+    from safire_amd.synthetic import make_problem
+    p = make_problem(12, 3, 40, seed=17)
+    osys = orc.OracleSystem(p, dt=0.01)
+    W = 6
+    s_h, Esh = osys.init_walkers(p.walkers(W))
+    s_l = {k: v.copy() for k, v in s_h.items()}
+    u = orc.field_uniforms(3, 1, 0, W, p.M)
+    old_e, old_w = s_l["eloc"].copy(), s_l["weight"].copy()
+    osys.step(s_h, u, Esh)
+    osys.step(s_l, u, Esh, local_energy=True)
+    assert np.array_equal(s_l["Phi"], s_h["Phi"])
+    G, ov = orc.mixed_density_matrix(osys.PsiT, s_l["Phi"])
+    E = orc.energy(osys.haj, osys.Lnak, G, p.nup, closed=True, E0=osys.E0).sum(1)
+    lo, hi = Esh - np.sqrt(2 / 0.01), Esh + 10 * np.sqrt(2 / 0.01)
+    e = np.clip(E.real, lo, hi) + 1j * E.imag
+    scale = np.maximum(0.0, np.cos(s_l["theta"].real))
+    w = old_w * scale * np.abs(np.exp(-0.01 * (0.5 * (e + old_e) - Esh)))
+    np.testing.assert_allclose(s_l["eloc"], e, rtol=1e-12)
+    np.testing.assert_allclose(s_l["weight"], w, rtol=1e-12)
+    np.testing.assert_allclose(s_l["ovlp"], 2 * ov, rtol=1e-12)
+    assert np.abs(s_l["eloc"] - s_h["eloc"]).max() > 1e-8   # the two modes differ
