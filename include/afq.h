@@ -57,7 +57,8 @@ typedef struct afq_phmsd afq_phmsd;
 
 /* ---- context ------------------------------------------------------------------------------ */
 
-/* Version of this ABI (major * 100 + minor). */
+/* Version of this ABI (major * 100 + minor).  2.00: afq_prop_params gained local_energy and
+ * afq_prop_te_anchor its follow_walkers argument. */
 int afq_abi_version(void);
 /* Text of the last error on this thread ("" when none). */
 const char* afq_last_error(void);

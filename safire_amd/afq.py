@@ -116,6 +116,9 @@ _SIGS = {
 _lib = None
 
 
+ABI_VERSION = 200   # afq_abi_version() of the library these bindings describe
+
+
 def lib():
     """The loaded library; raises when it has not been built (no fallback path exists)."""
     global _lib
@@ -129,6 +132,9 @@ def lib():
             f.argtypes = args
             f.restype = _I
         L.afq_last_error.restype = ctypes.c_char_p
+        if L.afq_abi_version() != ABI_VERSION:
+            raise AfqError(f"{LIB_PATH} has ABI {L.afq_abi_version()}, this mirror binds "
+                           f"{ABI_VERSION}: rebuild it")
         _lib = L
     return _lib
 

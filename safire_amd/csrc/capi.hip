@@ -69,7 +69,7 @@ void check_walkers(const afq_prop* p, const afq_walkers* w) {
 
 extern "C" {
 
-int afq_abi_version(void) { return 100; }
+int afq_abi_version(void) { return 200; }
 
 const char* afq_last_error(void) { return g_err.c_str(); }
 
