@@ -277,6 +277,76 @@ def energy_error(problem, W_e=64):
         float(np.median(np.abs(res[afq.AFQ_FP64])))
 
 
+def stage_errors(problem, W_e=128, seed=29, steps=1):
+    """Per-stage error budget of the mixed-precision step against the FP64 step from the same
+    walkers and the same field uniforms.  Each variant runs `steps` steps with one stage in FP32
+    (the force-bias contraction vbias, the vHS contraction, or the Taylor propagation) and the
+    rest in FP64; 'mixed' runs all three in FP32 (the step's kernels); 'floor' is the FP64 step
+    with the propagated walker matrices rounded to complex64 once, the smallest perturbation any
+    FP32 computation of them can make.  Local energies are evaluated in FP64 from FP64 G.
+    Returns per-variant max / median |dE| (Ha) and max |dE|/|E| over the walkers."""
+    import torch
+    from safire_amd import afq
+    ctx = afq.Context(0)
+    ham = afq.HamiltonianOperations(ctx, problem.h1, problem.Likn, problem.PsiT, problem.nup,
+                                    precision=afq.AFQ_MIXED)
+    prop = afq.Propagator(ham, W_e, dt=DT, order=ORDER)
+    P1 = prop.P1()
+    vmf = prop.vMF()
+    Nn, n, Mm = problem.N, problem.nup, problem.M
+    dP = afq.device_tensor(problem.PsiT)
+    Phi0 = afq.device_tensor(problem.walkers(W_e))
+    c64, c128 = torch.complex64, torch.complex128
+
+    def energy(Phi):
+        G = afq.empty((W_e, n, Nn), c128)
+        ov = afq.zeros(W_e, c128)
+        afq.det_ops.MixedDensityMatrix(ctx, dP, Phi, G, ov, compact=True)
+        E = afq.empty((W_e, 3), c128)
+        ham.energy(E, G, precision=afq.AFQ_FP64)
+        return afq.host(E.sum(1))
+
+    def run(fp32_vbias, fp32_vhs, fp32_taylor, round_out=False):
+        Phi = Phi0.clone()
+        for k in range(steps):
+            G = afq.empty((W_e, n, Nn), c128)
+            ov = afq.zeros(W_e, c128)
+            afq.det_ops.MixedDensityMatrix(ctx, dP, Phi, G, ov, compact=True)
+            X = afq.empty((W_e, Mm), c128)
+            ham.vbias(G, X, DT, precision=afq.AFQ_MIXED if fp32_vbias else afq.AFQ_FP64)
+            u = afq.empty((W_e, Mm), torch.float64)
+            afq.field_uniforms(ctx, seed, 1, k, u)
+            HW, mf = afq.empty(W_e, c128), afq.empty(W_e, c128)
+            afq.construct_x(ctx, u, vmf, X, HW, mf)
+            if fp32_vhs:
+                V32 = afq.empty((W_e, Nn, Nn), c64)
+                ham.vHS(X.to(c64), V32, DT, precision=afq.AFQ_MIXED)
+                V = V32.to(c128)
+            else:
+                V = afq.empty((W_e, Nn, Nn), c128)
+                ham.vHS(X, V, DT, precision=afq.AFQ_FP64)
+            if fp32_taylor:
+                afq.det_ops.Propagate(ctx, P1, V.to(c64).contiguous(), Phi, order=ORDER,
+                                      precision=afq.AFQ_MIXED)
+            else:
+                afq.det_ops.Propagate(ctx, P1, V, Phi, order=ORDER, precision=afq.AFQ_FP64)
+        if round_out:
+            Phi = Phi.to(c64).to(c128)
+        return energy(Phi)
+
+    ref = run(False, False, False)
+    out = {"walkers": W_e, "steps": steps, "median_abs_E_Ha": float(np.median(np.abs(ref)))}
+    for name, flags in (("floor", (False, False, False, True)), ("vbias", (True, False, False)),
+                        ("vhs", (False, True, False)), ("taylor", (False, False, True)),
+                        ("mixed", (True, True, True))):
+        d = np.abs(run(*flags) - ref)
+        out[name] = {"max_Ha": float(d.max()), "median_Ha": float(np.median(d)),
+                     "max_rel": float((d / np.abs(ref)).max())}
+    prop.close()
+    ham.close()
+    return out
+
+
 def cpu_baseline(problem, budget_s=12.0):
     """The FP64 oracle (CPU restatement of the reference step, OpenMP over walkers) on a
     bounded sample: a few walkers of the same workload stepped for about budget_s seconds."""

@@ -23,7 +23,12 @@
  *   Likn  double [N][N][M]        DenseFactorized/L (RealDenseHamiltonian.cpp:121-127)
  *   h1    double [N][N]           Hamiltonian/hcore
  *   PsiT  complex [nel][N]        trial, conjugate-transposed (NOMSD.hpp:292-344); alpha rows first
- *   Phi   complex [W][N][n]       walker Slater matrices of one spin block (WalkerSetBase.h:320-328)
+ *   Phi   complex [W][N][n]       walker Slater matrices of one spin block (WalkerSetBase.h:320-328);
+ *                                 each walker's [N][n] block is contiguous and consecutive walkers
+ *                                 are phi_stride complex elements apart (0: N * n, contiguous), so
+ *                                 SlaterMatrices(s) of SAFIRE's walker buffer, a view with the
+ *                                 walker row as its stride (WalkerSetBase.h:256-276), is passed
+ *                                 in place
  *   G     complex [W][nel][N]     compact mixed density matrix (density_matrix.hpp:576)
  *   X, v  complex [W][M]          fields / force bias
  *   V     complex [W][N][N]       vHS (complex128 in FP64, complex64 in mixed precision)
@@ -58,7 +63,9 @@ typedef struct afq_phmsd afq_phmsd;
 /* ---- context ------------------------------------------------------------------------------ */
 
 /* Version of this ABI (major * 100 + minor).  2.00: afq_prop_params gained local_energy and
- * afq_prop_te_anchor its follow_walkers argument. */
+ * afq_prop_te_anchor its follow_walkers argument.  3.00: walker strides (afq_walkers.phi_stride
+ * and prop_stride, the phi_stride argument of the Slater-determinant and PHMSD entry points) and
+ * afq_tune_force. */
 int afq_abi_version(void);
 /* Text of the last error on this thread ("" when none). */
 const char* afq_last_error(void);
@@ -72,6 +79,12 @@ int afq_ctx_synchronize(afq_ctx* ctx);
 int afq_tuning_report(char* buf, int len);
 /* Bytes of scratch held by the context. */
 int afq_ctx_scratch_bytes(afq_ctx* ctx, uint64_t* bytes);
+/* Pins the autotuner's choice for a product site (no counterpart in the reference; for tests
+ * that run every tile candidate): site "name" or "name:f32" / "name:f64" (the first field or two
+ * of an afq_tuning_report key), candidate >= 0 pins, < 0 unpins; site NULL unpins every site.
+ * Process-wide.  A pinned index beyond the site's candidates fails the product's call with
+ * AFQ_ERR_ARG.  The environment variable AFQ_TUNE_FORCE="site:i,..." sets pins at load time. */
+int afq_tune_force(const char* site, int candidate);
 
 /* ---- Cholesky hamiltonian operations (Real3IndexFactorization) ------------------------------ */
 
@@ -100,6 +113,14 @@ int afq_vhs(afq_ham* ham, const void* X, int W, double dt, int precision, void* 
  * AFQ_MIXED runs the exchange contraction in FP32 with FP64 accumulation. */
 int afq_energy(afq_ham* ham, const double* G, int W, int addH1, int addEJ, int addEXX,
                int precision, double* E);
+/* The per-spin Real3IndexFactorization::energy(spin, E, G, idet, EJn, addH1, addEJ, addEXX)
+ * (Real3IndexFactorization.hpp:176-210): G [W][n_s][N] is the compact density matrix of spin
+ * block `spin` (0 alpha, 1 beta: COLLINEAR only).  E [W][3] = (E0 (alpha only) + c sum G haj_s,
+ * -c/2 sum_{m,a,b} T_ab T_ba, 1/2 sum_m EJn^2) with c = 2 CLOSED / 1 COLLINEAR, and EJn [W][M] =
+ * sum_a T^m_aa, the Coulomb vectors a caller combines across spins (NULL when addEJ = 0).
+ * precision as afq_energy. */
+int afq_energy_spin(afq_ham* ham, int spin, const double* G, int W, int addH1, int addEJ,
+                    int addEXX, int precision, double* E, double* EJn);
 /* NOMSD::vMF (NOMSD.cpp:437-464) + zero_imag (AFQMCBasePropagator.cpp:105-108): vMF [M]. */
 int afq_vmf(afq_ham* ham, double dt, double* vMF);
 /* generateP1 + getOneBodyPropagatorMatrix (AFQMCBasePropagator.cpp:43-214,
@@ -113,21 +134,22 @@ int afq_one_body_propagator(afq_ham* ham, const double* vMF, double dt, double* 
  * accumulated: ovlp += log det(PsiT Phi), imaginary part wrapped to [-pi, pi]
  * (determinants_impl.hpp:33-75).  compact != 0: G [W][n][N]; else full G [W][N][N]. */
 int afq_mixed_density_matrix(afq_ctx* ctx, const double* PsiT, int n, int N, const double* Phi,
-                             int W, double* G, double* ovlp, int compact);
+                             int64_t phi_stride, int W, double* G, double* ovlp, int compact);
 /* det_ops::Log_Overlap (density_matrix.hpp:431-447): ovlp += log det(PsiT Phi). */
-int afq_log_overlap(afq_ctx* ctx, const double* PsiT, int n, int N, const double* Phi, int W,
-                    double* ovlp);
+int afq_log_overlap(afq_ctx* ctx, const double* PsiT, int n, int N, const double* Phi,
+                    int64_t phi_stride, int W, double* ovlp);
 /* det_ops::detail::inverse_logdet (density_matrix.hpp:82-111): Ainv = A^{-1} (zero when
  * singular), ovlp += log det A.  A, Ainv complex [nb][n][n]; A == Ainv is allowed. */
 int afq_inverse_logdet(afq_ctx* ctx, const double* A, int nb, int n, double* ovlp, double* Ainv);
 /* det_ops::detail::propagate_impl + apply_expM (propagate.hpp:41-70, :241-298):
  * Phi <- P1 * (sum_{k<=order} (i op(V))^k / k!) * P1 * Phi, TA = 'N' or 'H' (op = V or V^H, -i).
  * P1 complex128 [N][N]; V complex128 (AFQ_FP64) or complex64 (AFQ_MIXED) [W][N][N]. */
-int afq_propagate(afq_ctx* ctx, const double* P1, const void* V, double* Phi, int W, int N, int n,
-                  int order, char TA, int precision);
+int afq_propagate(afq_ctx* ctx, const double* P1, const void* V, double* Phi, int64_t phi_stride,
+                  int W, int N, int n, int order, char TA, int precision);
 /* det_ops::orthogonalize (orthogonalize.hpp:131-169): Phi = Q R with R upper triangular,
  * positive real diagonal; Phi <- Q; ldet += log det R. */
-int afq_orthogonalize(afq_ctx* ctx, double* Phi, int W, int N, int n, double* ldet);
+int afq_orthogonalize(afq_ctx* ctx, double* Phi, int64_t phi_stride, int W, int N, int n,
+                      double* ldet);
 
 /* ---- fields and weights ------------------------------------------------------------------- */
 
@@ -182,6 +204,23 @@ typedef struct {
   double* ovlp;    /* complex [W] OVLP (log overlap) */
   double* theta;   /* complex [W] THETA */
   int W;
+  /* Complex elements between consecutive walkers.  phi_stride: of Phi and of Phi_b (0: each
+   * block contiguous, N * nup and N * ndown); prop_stride: of the five property arrays (0 or 1:
+   * contiguous).  SAFIRE's walker buffer holds one row per walker, [Phi_a | Phi_b | WEIGHT,
+   * PHASE, PSEUDO_ELOC_, E1_, EXX_, EJ_, OVLP, ..., THETA] (WalkerSetBase.cpp:75-121): there
+   * Phi = row 0 + data_displ[SM], Phi_b = Phi + N * nup, each property = row 0 + its
+   * data_displ, and both strides are the row length walker_buffer.strides()[0]
+   * (SlaterMatrices, getProperty: WalkerSetBase.h:256-276, 463-481).  The step reads and writes
+   * the Slater blocks in place and stages the properties through 80 bytes per walker of scratch. */
+  int64_t phi_stride;
+  int64_t prop_stride;
+  /* E1_, EXX_, EJ_ columns (complex [W], prop_stride apart) or NULL.  When given, the
+   * local-energy step (afq_prop_params.local_energy = 1, the reference's hybrid = false) stores
+   * the components of each propagated walker's local energy there (WalkerSetUpdate.hpp:180-184)
+   * and population control copies them with their walkers; the hybrid step leaves them alone. */
+  double* E1;
+  double* EXX;
+  double* EJ;
 } afq_walkers;
 
 /* AFQMCBasePropagator constructor (AFQMCBasePropagator.h:45-171): builds vMF and P1. */
@@ -213,6 +252,16 @@ int afq_prop_step_graph(afq_prop* prop, const afq_walkers* w, double* Eshift, in
  * uniform r in (0, 1): offspring by |w|, copies into the slots of dropped walkers in walker
  * order, then every weight set to unit magnitude (phase kept).  W <= 16384. */
 int afq_prop_population_control(afq_prop* prop, const afq_walkers* w, double r);
+/* Bound-trigger statistics since the propagator's creation or the last reset (reset != 0
+ * zeroes them after reading), counts[9] in the order of AFQMCBasePropagator::printBoundStatistics
+ * (AFQMCBasePropagator.h:195-235): force-bias clamp |vbias| > vbias_bound -- operations (walker,
+ * field pairs), upper hits, lower hits (always 0); local-energy clamp outside
+ * Eshift - lower_cut sqrt(2/dt), Eshift + upper_cut sqrt(2/dt) (bound_local_energy,
+ * WalkerSetUpdate.hpp:46-58) -- operations (finite, constrained walkers), upper, lower; weight
+ * clamp |w| > max(weight_floor, weight_frac target_pop) (bound_walker_weights, :199-218) --
+ * operations (walkers), upper, lower (0).  Counted on the device by every step, captured steps
+ * included (the reference counts the force-bias clamp in its host path only).  Synchronizes. */
+int afq_prop_bound_stats(afq_prop* prop, int64_t* counts, int reset);
 /* *collapsed = 1 when the last population control found the total weight <= 1e-6 (the
  * population is then left unchanged; the reference aborts).  Synchronizes the stream. */
 int afq_prop_status(afq_prop* prop, int* collapsed);
@@ -344,13 +393,14 @@ int afq_phmsd_destroy(afq_phmsd* trial);
  * [W][N][nel_b] -> G [W][nact_a + nact_b][N], ovlp [W] = log overlap (PHMSD::vbias feeds this G
  * to afq_vbias). */
 int afq_phmsd_mixed_density_matrix(afq_phmsd* trial, const double* Phi_a, const double* Phi_b,
-                                   int W, double* G, double* ovlp);
+                                   int64_t phi_stride, int W, double* G, double* ovlp);
 /* PHMSD::Log_Overlap (PHMSD.cpp:804-900): ovlp [W]. */
-int afq_phmsd_log_overlap(afq_phmsd* trial, const double* Phi_a, const double* Phi_b, int W,
-                          double* ovlp);
-/* PHMSD::Energy, energy_alg1 (PHMSD.cpp:376-544): E [W][3] = (E1 + E0, EXX, EJ), ovlp [W]. */
-int afq_phmsd_energy(afq_phmsd* trial, const double* Phi_a, const double* Phi_b, int W, double* E,
-                     double* ovlp);
+int afq_phmsd_log_overlap(afq_phmsd* trial, const double* Phi_a, const double* Phi_b,
+                          int64_t phi_stride, int W, double* ovlp);
+/* PHMSD::Energy, energy_alg1 (PHMSD.cpp:376-544): E [W][3] = (E1 + E0, EXX, EJ), ovlp [W].
+ * phi_stride: as for afq_walkers (0: each block contiguous). */
+int afq_phmsd_energy(afq_phmsd* trial, const double* Phi_a, const double* Phi_b,
+                     int64_t phi_stride, int W, double* E, double* ovlp);
 /* PHMSD::vMF (PHMSD.cpp:1329-1573) + zero_imag (AFQMCBasePropagator.cpp:105-108): vMF [M]. */
 int afq_phmsd_vmf(afq_phmsd* trial, double dt, double* vMF);
 

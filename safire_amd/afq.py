@@ -48,7 +48,8 @@ class PropParams(ctypes.Structure):
 
 class Walkers(ctypes.Structure):
     _fields_ = [("Phi", _P), ("Phi_b", _P), ("weight", _P), ("phase", _P), ("eloc", _P),
-                ("ovlp", _P), ("theta", _P), ("W", _I)]
+                ("ovlp", _P), ("theta", _P), ("W", _I), ("phi_stride", ctypes.c_int64),
+                ("prop_stride", ctypes.c_int64), ("E1", _P), ("EXX", _P), ("EJ", _P)]
 
 
 # name -> argtypes (restype int unless noted)
@@ -60,6 +61,7 @@ _SIGS = {
     "afq_ctx_synchronize": [_P],
     "afq_ctx_scratch_bytes": [_P, ctypes.POINTER(ctypes.c_uint64)],
     "afq_tuning_report": [ctypes.c_char_p, _I],
+    "afq_tune_force": [ctypes.c_char_p, _I],
     "afq_ham_create": [_P, _I, _I, _I, _I, _I, _P, _P, _P, _D, _I, ctypes.POINTER(_P)],
     "afq_ham_destroy": [_P],
     "afq_ham_tensors": [_P, ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(_P),
@@ -67,13 +69,14 @@ _SIGS = {
     "afq_vbias": [_P, _P, _I, _D, _I, _P],
     "afq_vhs": [_P, _P, _I, _D, _I, _P],
     "afq_energy": [_P, _P, _I, _I, _I, _I, _I, _P],
+    "afq_energy_spin": [_P, _I, _P, _I, _I, _I, _I, _I, _P, _P],
     "afq_vmf": [_P, _D, _P],
     "afq_one_body_propagator": [_P, _P, _D, _P],
-    "afq_mixed_density_matrix": [_P, _P, _I, _I, _P, _I, _P, _P, _I],
-    "afq_log_overlap": [_P, _P, _I, _I, _P, _I, _P],
+    "afq_mixed_density_matrix": [_P, _P, _I, _I, _P, ctypes.c_int64, _I, _P, _P, _I],
+    "afq_log_overlap": [_P, _P, _I, _I, _P, ctypes.c_int64, _I, _P],
     "afq_inverse_logdet": [_P, _P, _I, _I, _P, _P],
-    "afq_propagate": [_P, _P, _P, _P, _I, _I, _I, _I, ctypes.c_char, _I],
-    "afq_orthogonalize": [_P, _P, _I, _I, _I, _P],
+    "afq_propagate": [_P, _P, _P, _P, ctypes.c_int64, _I, _I, _I, _I, ctypes.c_char, _I],
+    "afq_orthogonalize": [_P, _P, ctypes.c_int64, _I, _I, _I, _P],
     "afq_field_uniforms": [_P, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _I, _I, _P],
     "afq_construct_x": [_P, _P, _P, _I, _I, _D, _I, _I, _P, _P, _P],
     "afq_walker_update": [_P, _I, ctypes.POINTER(UpdateParams), _P, _P, _P, _P, _P, _P, _P, _P,
@@ -89,6 +92,7 @@ _SIGS = {
     "afq_prop_step_graph": [_P, ctypes.POINTER(Walkers), _P, _I, _I, _D, _P],
     "afq_prop_population_control": [_P, ctypes.POINTER(Walkers), _D],
     "afq_prop_status": [_P, ctypes.POINTER(_I)],
+    "afq_prop_bound_stats": [_P, ctypes.POINTER(ctypes.c_int64), _I],
     "afq_prop_enable_backprop": [_P, _I],
     "afq_prop_bp_anchor": [_P, ctypes.POINTER(Walkers)],
     "afq_prop_backpropagate": [_P, ctypes.POINTER(Walkers), _I, _I, _I, _P, _P],
@@ -107,16 +111,16 @@ _SIGS = {
     "afq_prop_create_phmsd": [_P, _I, ctypes.POINTER(PropParams), ctypes.POINTER(_P)],
     "afq_phmsd_create": [_P, _P, ctypes.POINTER(_P)],
     "afq_phmsd_destroy": [_P],
-    "afq_phmsd_mixed_density_matrix": [_P, _P, _P, _I, _P, _P],
-    "afq_phmsd_log_overlap": [_P, _P, _P, _I, _P],
-    "afq_phmsd_energy": [_P, _P, _P, _I, _P, _P],
+    "afq_phmsd_mixed_density_matrix": [_P, _P, _P, ctypes.c_int64, _I, _P, _P],
+    "afq_phmsd_log_overlap": [_P, _P, _P, ctypes.c_int64, _I, _P],
+    "afq_phmsd_energy": [_P, _P, _P, ctypes.c_int64, _I, _P, _P],
     "afq_phmsd_vmf": [_P, _D, _P],
 }
 
 _lib = None
 
 
-ABI_VERSION = 200   # afq_abi_version() of the library these bindings describe
+ABI_VERSION = 300   # afq_abi_version() of the library these bindings describe
 
 
 def lib():
@@ -147,6 +151,12 @@ def tuning_report() -> str:
     buf = ctypes.create_string_buffer(1 << 16)
     _ck(lib().afq_tuning_report(buf, len(buf)))
     return buf.value.decode()
+
+
+def tune_force(site, candidate: int = -1):
+    """Pin the autotuner's choice at a product site ("name" or "name:f32"/"name:f64"); a
+    negative candidate unpins the site, site None unpins every site."""
+    _ck(lib().afq_tune_force(None if site is None else site.encode(), int(candidate)))
 
 
 def _ck(rc):
@@ -249,18 +259,30 @@ class Context:
 # det_ops
 # ---------------------------------------------------------------------------------------------
 
+def _blocks(Phi):
+    """(pointer, walker stride) of a [W][N][n] walker array whose [N][n] blocks are contiguous
+    and whose walkers may be further apart (a SlaterMatrices view of a walker buffer)."""
+    W, N, n = Phi.shape
+    st = Phi.stride()
+    if W > 1:
+        assert st[1] == n and st[2] == 1, "each walker's [N][n] block must be contiguous"
+    return ctypes.c_void_p(Phi.data_ptr()), (st[0] if W > 1 else N * n)
+
+
 class det_ops:  # noqa: N801  (namespace mirroring sfqmc::afqmc::det_ops)
     @staticmethod
     def MixedDensityMatrix(ctx, PsiT, Phi, G, ovlp, compact=True):
         n, N = PsiT.shape
         W = Phi.shape[0]
-        _ck(lib().afq_mixed_density_matrix(ctx.h, ptr(PsiT), n, N, ptr(Phi), W, ptr(G), ptr(ovlp),
+        pp, bs = _blocks(Phi)
+        _ck(lib().afq_mixed_density_matrix(ctx.h, ptr(PsiT), n, N, pp, bs, W, ptr(G), ptr(ovlp),
                                            int(compact)))
 
     @staticmethod
     def Log_Overlap(ctx, PsiT, Phi, ovlp):
         n, N = PsiT.shape
-        _ck(lib().afq_log_overlap(ctx.h, ptr(PsiT), n, N, ptr(Phi), Phi.shape[0], ptr(ovlp)))
+        pp, bs = _blocks(Phi)
+        _ck(lib().afq_log_overlap(ctx.h, ptr(PsiT), n, N, pp, bs, Phi.shape[0], ptr(ovlp)))
 
     @staticmethod
     def inverse_logdet(ctx, A, ovlp, Ainv):
@@ -270,13 +292,15 @@ class det_ops:  # noqa: N801  (namespace mirroring sfqmc::afqmc::det_ops)
     @staticmethod
     def Propagate(ctx, P1, V, Phi, order=6, TA="N", precision=AFQ_FP64):
         W, N, n = Phi.shape
-        _ck(lib().afq_propagate(ctx.h, ptr(P1), ptr(V), ptr(Phi), W, N, n, order, TA.encode(),
+        pp, bs = _blocks(Phi)
+        _ck(lib().afq_propagate(ctx.h, ptr(P1), ptr(V), pp, bs, W, N, n, order, TA.encode(),
                                 precision))
 
     @staticmethod
     def orthogonalize(ctx, Phi, ldet):
         W, N, n = Phi.shape
-        _ck(lib().afq_orthogonalize(ctx.h, ptr(Phi), W, N, n, ptr(ldet)))
+        pp, bs = _blocks(Phi)
+        _ck(lib().afq_orthogonalize(ctx.h, pp, bs, W, N, n, ptr(ldet)))
 
 
 def field_uniforms(ctx, seed, stream, step, u):
@@ -369,6 +393,15 @@ class HamiltonianOperations:
         _ck(lib().afq_energy(self.h, ptr(G), W, int(addH1), int(addEJ), int(addEXX),
                              self.precision if precision is None else precision, ptr(E)))
 
+    def energy_spin(self, spin, E, G, EJn=None, addH1=True, addEJ=True, addEXX=True,
+                    precision=None):
+        """Per-spin energy of one block's compact G [W][n_spin][N]; EJn [W][M] receives the
+        Coulomb vectors when addEJ."""
+        W = G.shape[0]
+        _ck(lib().afq_energy_spin(self.h, int(spin), ptr(G), W, int(addH1), int(addEJ),
+                                  int(addEXX), self.precision if precision is None else precision,
+                                  ptr(E), ptr(EJn)))
+
     def vMF(self, dt, v):
         _ck(lib().afq_vmf(self.h, dt, ptr(v)))
 
@@ -458,7 +491,68 @@ class WalkerSet:
 
     def cstruct(self):
         self._c = Walkers(ptr(self.Phi), ptr(self.Phi_b), ptr(self.weight), ptr(self.phase),
-                          ptr(self.eloc), ptr(self.ovlp), ptr(self.theta), self.W)
+                          ptr(self.eloc), ptr(self.ovlp), ptr(self.theta), self.W, 0, 0,
+                          ptr(self.E1), ptr(self.EXX), ptr(self.EJ))
+        return self._c
+
+    # the energy-component columns are optional (the local-energy step fills them when given)
+    E1 = EXX = EJ = None
+
+    def keep_energy_components(self):
+        """Allocate the E1_, EXX_, EJ_ columns that a local-energy step fills."""
+        torch = _torch()
+        for k in ("E1", "EXX", "EJ"):
+            setattr(self, k, torch.zeros(self.W, dtype=torch.complex128, device="cuda"))
+
+    def phi_stride(self):
+        return 0
+
+
+# column offsets of SAFIRE's walker row at zero temperature (WalkerSetBase.cpp:75-121): the
+# Slater matrices, then WEIGHT, PHASE, PSEUDO_ELOC_, E1_, EXX_, EJ_, OVLP, LOGSCL_UP,
+# LOGSCL_DN, IS_UNITARY, THETA
+_ROW_PROPS = ("weight", "phase", "eloc", "E1", "EXX", "EJ", "ovlp", "logscl_up", "logscl_dn",
+              "is_unitary", "theta")
+
+
+class WalkerBuffer:
+    """Walkers in SAFIRE's own buffer layout: one complex128 row per walker,
+    [Phi_a (N x nup) | Phi_b (N x ndown) | WEIGHT, PHASE, PSEUDO_ELOC_, E1_, EXX_, EJ_, OVLP,
+    LOGSCL_UP, LOGSCL_DN, IS_UNITARY, THETA] (WalkerSetBase.cpp:75-121).  Phi / Phi_b are the
+    strided SlaterMatrices views and the properties the strided getProperty columns
+    (WalkerSetBase.h:256-276, 463-481); the library reads and writes them in place through the
+    walker strides of afq_walkers.  `pad` extra complex elements per row test that nothing
+    assumes the row length."""
+
+    def __init__(self, Phi, Phi_b=None, pad=0):
+        torch = _torch()
+        W, N, nup = Phi.shape
+        ndn = 0 if Phi_b is None else Phi_b.shape[2]
+        self.W, self.N, self.nup, self.ndn = W, N, nup, ndn
+        self.row = N * (nup + ndn) + len(_ROW_PROPS) + pad
+        self.buf = torch.zeros((W, self.row), dtype=torch.complex128, device="cuda")
+        self.Phi = self.buf[:, :N * nup].view(W, N, nup)
+        self.Phi.copy_(torch.from_numpy(np.ascontiguousarray(Phi, dtype=np.complex128)))
+        self.Phi_b = None
+        if ndn:
+            self.Phi_b = self.buf[:, N * nup:N * (nup + ndn)].view(W, N, ndn)
+            self.Phi_b.copy_(torch.from_numpy(np.ascontiguousarray(Phi_b, dtype=np.complex128)))
+        base = N * (nup + ndn)
+        for i, name in enumerate(_ROW_PROPS):
+            setattr(self, name, self.buf[:, base + i])
+        self.weight.fill_(1.0)
+        self.phase.fill_(1.0)
+        self._c = None
+
+    def phi_stride(self):
+        return self.row
+
+    def cstruct(self):
+        def a(t):
+            return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+        self._c = Walkers(a(self.Phi), a(self.Phi_b), a(self.weight), a(self.phase), a(self.eloc),
+                          a(self.ovlp), a(self.theta), self.W, self.row, self.row, a(self.E1),
+                          a(self.EXX), a(self.EJ))
         return self._c
 
 
@@ -521,6 +615,12 @@ class Propagator:
 
     def population_control(self, wset: WalkerSet, r):
         _ck(lib().afq_prop_population_control(self.h, ctypes.byref(wset.cstruct()), float(r)))
+
+    def bound_stats(self, reset=False):
+        """Bound-trigger counters: {'vbias'|'eloc'|'weight': (total, upper, lower)}."""
+        c = (ctypes.c_int64 * 9)()
+        _ck(lib().afq_prop_bound_stats(self.h, c, int(bool(reset))))
+        return {"vbias": tuple(c[0:3]), "eloc": tuple(c[3:6]), "weight": tuple(c[6:9])}
 
     def collapsed(self) -> bool:
         v = _I()

@@ -19,6 +19,9 @@ struct FieldParams {
   uint32_t stream;
   uint32_t step;
   const uint32_t* step_ptr;  // when set, the step index is read from device memory
+  // bound-trigger counters (may be null): [0] += W M (walker, field) pairs, [1] += pairs with
+  // |vbias| > bound (AFQMCBasePropagator.cpp:697-706)
+  unsigned long long* stats = nullptr;
 };
 // X holds vbias on entry (complex [W][M]), fields on exit; HW, mf [W]. u == nullptr draws the
 // uniforms in-kernel from Philox4x32-10 keyed by (seed, stream, step, walker, field). X32, when
@@ -32,6 +35,10 @@ struct UpdateParams {
   double dt, E0, lower_cut, upper_cut, weight_floor, weight_frac, target_pop;
   int free_projection, use_cp;
   int local_energy;  // 1: E_loc = sum of the energies E [W][3] (E0 included), midpoint rule
+  // bound-trigger counters (may be null): [3..5] local-energy clamp total / upper / lower
+  // (bound_local_energy, WalkerSetUpdate.hpp:46-58), [6..8] weight clamp total / upper / lower
+  // (bound_walker_weights, :199-218)
+  unsigned long long* stats = nullptr;
 };
 // Eshift is read from device memory (so a captured step needs no host value).  When wfac is
 // given, the step's path-restoration factor (w / |w|) / scale of every walker (0 for a dropped
@@ -50,20 +57,28 @@ void gather_fields(hipStream_t s, int W, int M, const z64* ring, int nring, int 
 // path restoration); n = 0 copies the weights
 void restore_weights(hipStream_t s, int W, const z64* weight, const z64* wfac, int hl, uint32_t pos,
                      int n, z64* out);
-// per-walker rows that move with their walker in population control (back-propagation state)
+// per-walker rows that move with their walker in population control (back-propagation state,
+// the energy components): n[r] elements at p[r] + w * bs[r] (bs 0: n[r])
 struct WalkerRows {
-  static constexpr int kMax = 7;
+  static constexpr int kMax = 10;
   z64* p[kMax] = {};
   long n[kMax] = {};
+  long bs[kMax] = {};
 };
+// e1[w bs], exx[w bs], ej[w bs] = E[w][0..2] (the walker's E1_, EXX_, EJ_ columns,
+// WalkerSetUpdate.hpp:180-184)
+void store_energy_components(hipStream_t s, int W, const z64* E, z64* e1, z64* exx, z64* ej,
+                             long bs);
 void increment_counter(hipStream_t s, uint32_t* c);
 // comb population control of one rank (population_control.hpp:216-257); r_dev = injected
 // uniform or nullptr for Philox(seed, stream, *step_ptr).  cum [W] doubles, src [W] ints and
-// status [1] int are device scratch.
+// status [1] int are device scratch.  The Slater blocks of walker w are rowA (rowB) elements at
+// PhiA + w bsA (PhiB + w bsB).
 void population_control(hipStream_t s, int W, const double* r_dev, uint64_t seed, uint32_t stream,
-                        const uint32_t* step_ptr, long rowA, long rowB, z64* PhiA, z64* PhiB,
-                        z64* weight, z64* phase, z64* eloc, z64* ovlp, z64* theta, double* cum,
-                        int* src, int* status, const WalkerRows& extra = WalkerRows{});
+                        const uint32_t* step_ptr, long rowA, long rowB, long bsA, long bsB,
+                        z64* PhiA, z64* PhiB, z64* weight, z64* phase, z64* eloc, z64* ovlp,
+                        z64* theta, double* cum, int* src, int* status,
+                        const WalkerRows& extra = WalkerRows{});
 // growth estimator log(sum w e^{dt E} / sum w)/dt -> out[0]; Eshift[0] += f (out - Eshift) when
 // relax > 0.
 void average_pseudo_energy(hipStream_t s, int W, const z64* weight, const z64* eloc, double dt,
@@ -97,6 +112,14 @@ void conj_transpose(hipStream_t s, int R, int C, const z64* in, z64* out);
 void broadcast_z64(hipStream_t s, long W, long n, const z64* src, z64* dst);  // dst[w][:] = src
 // dst[r][:] = src[idx[r]][:] for R rows of C elements (idx on the device)
 void gather_rows(hipStream_t s, long R, int C, const int* idx, const z64* src, z64* dst);
+// K strided columns of W walkers (element w of column k at p[k] + w * stride) copied to or from
+// packed [K][W]
+struct ColumnSet {
+  static constexpr int kMax = 8;
+  z64* p[kMax] = {};
+};
+void copy_columns(hipStream_t s, int K, int W, long stride, const ColumnSet& cols, z64* packed,
+                  bool to_packed);
 // out[b][j][i][:] = in[b][i][j][:] with in [B][D0][D1][K] and out [B][D1][D0][K]
 void swap_mid(hipStream_t s, long B, int D0, int D1, int K, const z64* in, z64* out);
 // true when every Cholesky vector L[.][.][m] of an [N][N][M] tensor is exactly symmetric

@@ -53,8 +53,22 @@ inline const z64* Z(const double* p) { return reinterpret_cast<const z64*>(p); }
 inline z64* Z(double* p) { return reinterpret_cast<z64*>(p); }
 
 afq::WalkerState to_state(const afq_walkers* w) {
-  return afq::WalkerState{Z(w->Phi), Z(w->Phi_b), Z(w->weight), Z(w->phase), Z(w->eloc),
-                          Z(w->ovlp), Z(w->theta), w->W};
+  REQUIRE(w->phi_stride >= 0 && w->prop_stride >= 0, "negative walker stride");
+  afq::WalkerState st{Z(w->Phi), Z(w->Phi_b), Z(w->weight), Z(w->phase), Z(w->eloc),
+                      Z(w->ovlp), Z(w->theta), w->W};
+  st.phi_bs = st.phi_b_bs = (long)w->phi_stride;
+  st.prop_bs = w->prop_stride > 0 ? (long)w->prop_stride : 1;
+  st.e1 = Z(w->E1);
+  st.exx = Z(w->EXX);
+  st.ej = Z(w->EJ);
+  st.e_bs = st.prop_bs;
+  return st;
+}
+
+// batch stride of a Slater block: the caller's, or N n when 0
+long block_stride(int64_t stride, int N, int n) {
+  REQUIRE(stride == 0 || stride >= (int64_t)N * n, "walker stride below N * n");
+  return stride ? (long)stride : (long)N * n;
 }
 
 void check_walkers(const afq_prop* p, const afq_walkers* w) {
@@ -69,7 +83,7 @@ void check_walkers(const afq_prop* p, const afq_walkers* w) {
 
 extern "C" {
 
-int afq_abi_version(void) { return 200; }
+int afq_abi_version(void) { return 300; }
 
 const char* afq_last_error(void) { return g_err.c_str(); }
 
@@ -102,6 +116,10 @@ int afq_tuning_report(char* buf, int len) {
     r.copy(buf, n);
     buf[n] = 0;
   });
+}
+
+int afq_tune_force(const char* site, int candidate) {
+  return guard([&] { afq::Tuner::get().force(site ? std::string(site) : std::string(), candidate); });
 }
 
 int afq_ctx_scratch_bytes(afq_ctx* ctx, uint64_t* bytes) {
@@ -171,6 +189,16 @@ int afq_energy(afq_ham* ham, const double* G, int W, int addH1, int addEJ, int a
   });
 }
 
+int afq_energy_spin(afq_ham* ham, int spin, const double* G, int W, int addH1, int addEJ,
+                    int addEXX, int precision, double* E, double* EJn) {
+  return guard([&] {
+    REQUIRE(ham && G && E, "NULL argument");
+    REQUIRE(W >= 0, "W");
+    ham->ops->energy_spin(spin, Z(G), W, Z(E), Z(EJn), addH1 != 0, addEJ != 0, addEXX != 0,
+                          (afq::Precision)precision);
+  });
+}
+
 int afq_vmf(afq_ham* ham, double dt, double* vMF) {
   return guard([&] {
     REQUIRE(ham && vMF, "NULL argument");
@@ -186,22 +214,23 @@ int afq_one_body_propagator(afq_ham* ham, const double* vMF, double dt, double* 
 }
 
 int afq_mixed_density_matrix(afq_ctx* ctx, const double* PsiT, int n, int N, const double* Phi,
-                             int W, double* G, double* ovlp, int compact) {
+                             int64_t phi_stride, int W, double* G, double* ovlp, int compact) {
   return guard([&] {
     REQUIRE(ctx && PsiT && Phi && G && ovlp, "NULL argument");
     REQUIRE(n >= 0 && n <= 64 && N >= n && W >= 0, "invalid sizes");
     long gbs = compact ? (long)n * N : (long)N * N;
-    afq::det_ops::mixed_density_matrix(ctx->c, Z(PsiT), n, N, Z(Phi), (long)N * n, W, Z(G), gbs,
-                                       Z(ovlp), compact != 0);
+    afq::det_ops::mixed_density_matrix(ctx->c, Z(PsiT), n, N, Z(Phi), block_stride(phi_stride, N, n),
+                                       W, Z(G), gbs, Z(ovlp), compact != 0);
   });
 }
 
-int afq_log_overlap(afq_ctx* ctx, const double* PsiT, int n, int N, const double* Phi, int W,
-                    double* ovlp) {
+int afq_log_overlap(afq_ctx* ctx, const double* PsiT, int n, int N, const double* Phi,
+                    int64_t phi_stride, int W, double* ovlp) {
   return guard([&] {
     REQUIRE(ctx && PsiT && Phi && ovlp, "NULL argument");
     REQUIRE(n >= 0 && n <= 64 && N >= n && W >= 0, "invalid sizes");
-    afq::det_ops::log_overlap(ctx->c, Z(PsiT), n, N, Z(Phi), (long)N * n, W, Z(ovlp));
+    afq::det_ops::log_overlap(ctx->c, Z(PsiT), n, N, Z(Phi), block_stride(phi_stride, N, n), W,
+                              Z(ovlp));
   });
 }
 
@@ -213,15 +242,16 @@ int afq_inverse_logdet(afq_ctx* ctx, const double* A, int nb, int n, double* ovl
   });
 }
 
-int afq_propagate(afq_ctx* ctx, const double* P1, const void* V, double* Phi, int W, int N, int n,
-                  int order, char TA, int precision) {
+int afq_propagate(afq_ctx* ctx, const double* P1, const void* V, double* Phi, int64_t phi_stride,
+                  int W, int N, int n, int order, char TA, int precision) {
   return guard([&] {
     REQUIRE(ctx && P1 && V && Phi, "NULL argument");
     REQUIRE(W >= 0 && N > 0 && n >= 0 && order >= 0, "invalid sizes");
     REQUIRE(TA == 'N' || TA == 'H', "TA must be 'N' or 'H'");
     REQUIRE(precision == AFQ_FP64 || precision == AFQ_MIXED, "precision");
     afq::det_ops::PropagateArgs a{W, N, n, order, TA, Z(P1), nullptr, nullptr, nullptr, V,
-                                  (long)N * N, Z(Phi), (long)N * n, (afq::Precision)precision};
+                                  (long)N * N, Z(Phi), block_stride(phi_stride, N, n),
+                                  (afq::Precision)precision};
     if (precision == AFQ_MIXED) {
       afq::c32* p32 = afq::scratch_as<afq::c32>(ctx->c, "capi_P1c32", (size_t)N * N);
       afq::cast_z64_c32(ctx->c.stream(), (long)N * N, Z(P1), p32);
@@ -231,11 +261,12 @@ int afq_propagate(afq_ctx* ctx, const double* P1, const void* V, double* Phi, in
   });
 }
 
-int afq_orthogonalize(afq_ctx* ctx, double* Phi, int W, int N, int n, double* ldet) {
+int afq_orthogonalize(afq_ctx* ctx, double* Phi, int64_t phi_stride, int W, int N, int n,
+                      double* ldet) {
   return guard([&] {
     REQUIRE(ctx && Phi && ldet, "NULL argument");
     REQUIRE(n >= 0 && n <= 64 && N >= n && W >= 0, "invalid sizes");
-    afq::det_ops::orthogonalize(ctx->c, Z(Phi), (long)N * n, W, N, n, Z(ldet));
+    afq::det_ops::orthogonalize(ctx->c, Z(Phi), block_stride(phi_stride, N, n), W, N, n, Z(ldet));
   });
 }
 
@@ -389,6 +420,15 @@ int afq_prop_population_control(afq_prop* prop, const afq_walkers* w, double r) 
     AFQ_HIP(hipMemcpyAsync(rd, &r, sizeof(double), hipMemcpyHostToDevice, prop->ham->ctx->c.stream()));
     AFQ_HIP(hipStreamSynchronize(prop->ham->ctx->c.stream()));
     prop->p->population_control(to_state(w), rd);
+  });
+}
+
+int afq_prop_bound_stats(afq_prop* prop, int64_t* counts, int reset) {
+  return guard([&] {
+    REQUIRE(prop && counts, "NULL argument");
+    long long c[9];
+    prop->p->bound_stats(c, reset != 0);
+    for (int i = 0; i < 9; ++i) counts[i] = (int64_t)c[i];
   });
 }
 
@@ -561,29 +601,30 @@ int afq_phmsd_destroy(afq_phmsd* trial) {
 }
 
 int afq_phmsd_mixed_density_matrix(afq_phmsd* trial, const double* Phi_a, const double* Phi_b,
-                                   int W, double* G, double* ovlp) {
+                                   int64_t phi_stride, int W, double* G, double* ovlp) {
   return guard([&] {
     REQUIRE(trial && Phi_a && Phi_b && G && ovlp, "NULL argument");
-    REQUIRE(W > 0, "W must be positive");
-    trial->t->mixed_density_matrix(Z(Phi_a), Z(Phi_b), W, Z(G), Z(ovlp));
+    REQUIRE(W > 0 && phi_stride >= 0, "W must be positive, the stride non-negative");
+    trial->t->mixed_density_matrix(Z(Phi_a), Z(Phi_b), W, Z(G), Z(ovlp), (long)phi_stride,
+                                   (long)phi_stride);
   });
 }
 
-int afq_phmsd_log_overlap(afq_phmsd* trial, const double* Phi_a, const double* Phi_b, int W,
-                          double* ovlp) {
+int afq_phmsd_log_overlap(afq_phmsd* trial, const double* Phi_a, const double* Phi_b,
+                          int64_t phi_stride, int W, double* ovlp) {
   return guard([&] {
     REQUIRE(trial && Phi_a && Phi_b && ovlp, "NULL argument");
-    REQUIRE(W > 0, "W must be positive");
-    trial->t->log_overlap(Z(Phi_a), Z(Phi_b), W, Z(ovlp));
+    REQUIRE(W > 0 && phi_stride >= 0, "W must be positive, the stride non-negative");
+    trial->t->log_overlap(Z(Phi_a), Z(Phi_b), W, Z(ovlp), (long)phi_stride, (long)phi_stride);
   });
 }
 
-int afq_phmsd_energy(afq_phmsd* trial, const double* Phi_a, const double* Phi_b, int W, double* E,
-                     double* ovlp) {
+int afq_phmsd_energy(afq_phmsd* trial, const double* Phi_a, const double* Phi_b,
+                     int64_t phi_stride, int W, double* E, double* ovlp) {
   return guard([&] {
     REQUIRE(trial && Phi_a && Phi_b && E && ovlp, "NULL argument");
-    REQUIRE(W > 0, "W must be positive");
-    trial->t->energy(Z(Phi_a), Z(Phi_b), W, Z(E), Z(ovlp));
+    REQUIRE(W > 0 && phi_stride >= 0, "W must be positive, the stride non-negative");
+    trial->t->energy(Z(Phi_a), Z(Phi_b), W, Z(E), Z(ovlp), (long)phi_stride, (long)phi_stride);
   });
 }
 

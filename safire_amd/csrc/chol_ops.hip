@@ -268,6 +268,65 @@ void CholeskyOps::vbias32(const c32* G, int W, double dt, z64* v) const {
   }
 }
 
+// Exchange partial sums of one spin block: part [W][ceil(M / group)] (one entry per walker and
+// group of exchange_group(n) Cholesky vectors, summed in group order by energy_finish) and, when
+// Kl is given, Kl[w][m] += tr T^m.  Gblk [W][ns][N] contiguous.  The Cholesky vectors are taken
+// in chunks whose T [W][ns][MC][ns] fits 1 GiB (max_memory_MB of energy_impl,
+// Real3IndexFactorization.hpp:511-587), sized for the larger spin block so both share the buffer.
+void CholeskyOps::exchange_spin(int sp, const z64* Gblk, int W, z64* part, z64* Kl,
+                                Precision prec) const {
+  hipStream_t s = ctx_.stream();
+  const int ns = sp == 0 ? nup_ : ndown_;
+  const int nmax = std::max(nup_, ndown_);
+  const bool t32 = prec != kFP64;
+  const size_t tsz = t32 ? sizeof(c32) : sizeof(z64);
+  const int mg = exchange_group(std::max(nmax, 1));
+  const long per_m = (long)W * nmax * nmax * (long)tsz;
+  int MC = (int)std::max<long>(1, std::min<long>(M_, (1L << 30) / std::max<long>(per_m, 1)));
+  MC = std::max(mg, MC / mg * mg);
+  const int nchunk = ceil_div(M_, MC);
+  const int ng_total = ceil_div(M_, mg);
+  void* T = ctx_.scratch("en_T", (size_t)W * nmax * nmax * MC * tsz);
+  const size_t loff = sp == 0 ? 0 : (size_t)M_ * nup_ * N_;
+  c32* G32 = nullptr;
+  for (int ch = 0; ch < nchunk; ++ch) {
+    int m0 = ch * MC;
+    int mc = std::min(MC, M_ - m0);
+    const size_t lc = loff + (size_t)m0 * ns * N_;
+    // T[(w,b)][(m,a)] = sum_k G[w][b][k] Lnak[m0+m][a][k]
+    if (!t32) {
+      EpiStore<double, double> e{static_cast<z64*>(T), (long)MC * ns, 0, 1, 0, 0};
+      Opnd<double, true, true> A{&Gblk->re, N_, 0};
+      if (real_trial_) {
+        Opnd<double, false, true> B{Lnak_r_ + lc, N_, 0};
+        gemm_large<double>(ctx_, "exx_chunk_r", 1, W * ns, mc * ns, N_, A, B, e);
+      } else {
+        Opnd<double, true, true> B{&(Lnak_ + lc)->re, N_, 0};
+        gemm_large<double>(ctx_, "exx_chunk", 1, W * ns, mc * ns, N_, A, B, e);
+      }
+      exchange_blocks<double>(s, W, ns, MC, m0, mc, M_, static_cast<const z64*>(T), part, ng_total,
+                              m0 / mg, Kl);
+    } else {
+      AFQ_CHECK(prec_ == kMixed, kErrState, "FP32 tensors were not built (precision FP64)");
+      if (!G32) {
+        G32 = scratch_as<c32>(ctx_, "en_G32", (size_t)W * ns * N_);
+        cast_z64_c32(s, (long)W * ns * N_, Gblk, G32);
+      }
+      EpiStore<float, float> ef{static_cast<c32*>(T), (long)MC * ns, 0, 1.f, 0.f, 0};
+      Opnd<float, true, true> A{&G32->re, N_, 0};
+      if (real_trial_) {
+        Opnd<float, false, true> B{Lnak_r32_ + lc, N_, 0};
+        gemm_large<float>(ctx_, "exx_chunk_r", 1, W * ns, mc * ns, N_, A, B, ef);
+      } else {
+        Opnd<float, true, true> B{&(Lnak32_ + lc)->re, N_, 0};
+        gemm_large<float>(ctx_, "exx_chunk_c", 1, W * ns, mc * ns, N_, A, B, ef);
+      }
+      exchange_blocks<float>(s, W, ns, MC, m0, mc, M_, static_cast<const c32*>(T), part, ng_total,
+                             m0 / mg, Kl);
+    }
+  }
+}
+
 void CholeskyOps::energy(const z64* G, int W, z64* E, bool addH1, bool addEJ, bool addEXX,
                          Precision prec) const {
   if (W == 0) return;
@@ -284,80 +343,55 @@ void CholeskyOps::energy(const z64* G, int W, z64* E, bool addH1, bool addEJ, bo
   if (!addEXX) return;
   z64* Kl = scratch_as<z64>(ctx_, "en_Kl", (size_t)W * M_);
   fill_z64(s, (long)W * M_, {0, 0}, Kl);
-  // chunks of Cholesky vectors: T chunk of W*ns*MC*ns complex (FP32 in mixed) within 1 GiB, MC a
-  // multiple of the exchange group so that the groups (one partial sum per walker each) tile M
   const int nspin = walker_type_ == kCollinear ? 2 : 1;
-  const int nmax = std::max(nup_, ndown_);
-  const bool t32 = prec != kFP64;
-  const size_t tsz = t32 ? sizeof(c32) : sizeof(z64);
-  const int mg = exchange_group(std::max(nmax, 1));
-  const long per_m = (long)W * nmax * nmax * (long)tsz;
-  int MC = (int)std::max<long>(1, std::min<long>(M_, (1L << 30) / std::max<long>(per_m, 1)));
-  MC = std::max(mg, MC / mg * mg);
-  const int nchunk = ceil_div(M_, MC);
-  const int ng_total = ceil_div(M_, mg);
-  void* T = ctx_.scratch("en_T", (size_t)W * nmax * nmax * MC * tsz);
+  const int ng_total = ceil_div(M_, exchange_group(std::max(std::max(nup_, ndown_), 1)));
   z64* part = scratch_as<z64>(ctx_, "en_part", (size_t)W * ng_total * nspin);
-  z64* Gs = nullptr;
-  c32* G32 = nullptr;
   for (int sp = 0; sp < nspin; ++sp) {
     int ns = sp == 0 ? nup_ : ndown_;
     if (ns == 0) {
       fill_z64(s, (long)W * ng_total, {0, 0}, part + (size_t)sp * W * ng_total);
       continue;
     }
-    size_t goff = sp == 0 ? 0 : (size_t)nup_ * N_;
-    const size_t loff = sp == 0 ? 0 : (size_t)M_ * nup_ * N_;
     // contiguous spin block of G: [W][ns][N]
     const z64* Gblk = G;
     if (nspin == 2) {
-      Gs = scratch_as<z64>(ctx_, "en_Gs", (size_t)W * ns * N_);
-      AFQ_HIP(hipMemcpy2DAsync(Gs, sizeof(z64) * ns * N_, G + goff, sizeof(z64) * nel_ * N_,
-                               sizeof(z64) * ns * N_, W, hipMemcpyDeviceToDevice, s));
+      z64* Gs = scratch_as<z64>(ctx_, "en_Gs", (size_t)W * ns * N_);
+      AFQ_HIP(hipMemcpy2DAsync(Gs, sizeof(z64) * ns * N_, G + (sp == 0 ? 0 : (size_t)nup_ * N_),
+                               sizeof(z64) * nel_ * N_, sizeof(z64) * ns * N_, W,
+                               hipMemcpyDeviceToDevice, s));
       Gblk = Gs;
     }
-    for (int ch = 0; ch < nchunk; ++ch) {
-      int m0 = ch * MC;
-      int mc = std::min(MC, M_ - m0);
-      const size_t lc = loff + (size_t)m0 * ns * N_;
-      // T[(w,b)][(m,a)] = sum_k G[w][b][k] Lnak[m0+m][a][k]
-      if (!t32) {
-        EpiStore<double, double> e{static_cast<z64*>(T), (long)MC * ns, 0, 1, 0, 0};
-        Opnd<double, true, true> A{&Gblk->re, N_, 0};
-        if (real_trial_) {
-          Opnd<double, false, true> B{Lnak_r_ + lc, N_, 0};
-          gemm_large<double>(ctx_, "exx_chunk_r", 1, W * ns, mc * ns, N_, A, B, e);
-        } else {
-          Opnd<double, true, true> B{&(Lnak_ + lc)->re, N_, 0};
-          gemm_large<double>(ctx_, "exx_chunk", 1, W * ns, mc * ns, N_, A, B, e);
-        }
-        exchange_blocks<double>(s, W, ns, MC, m0, mc, M_, static_cast<const z64*>(T),
-                                part + (size_t)sp * W * ng_total, ng_total, m0 / mg,
-                                addEJ ? Kl : nullptr);
-      } else {
-        AFQ_CHECK(prec_ == kMixed, kErrState, "FP32 tensors were not built (precision FP64)");
-        if (!G32) G32 = scratch_as<c32>(ctx_, "en_G32", (size_t)W * ns * N_);
-        if (ch == 0) cast_z64_c32(s, (long)W * ns * N_, Gblk, G32);
-        EpiStore<float, float> ef{static_cast<c32*>(T), (long)MC * ns, 0, 1.f, 0.f, 0};
-        Opnd<float, true, true> A{&G32->re, N_, 0};
-        if (real_trial_) {
-          Opnd<float, false, true> B{Lnak_r32_ + lc, N_, 0};
-          gemm_large<float>(ctx_, "exx_chunk_r", 1, W * ns, mc * ns, N_, A, B, ef);
-        } else {
-          Opnd<float, true, true> B{&(Lnak32_ + lc)->re, N_, 0};
-          gemm_large<float>(ctx_, "exx_chunk_c", 1, W * ns, mc * ns, N_, A, B, ef);
-        }
-        exchange_blocks<float>(s, W, ns, MC, m0, mc, M_, static_cast<const c32*>(T),
-                               part + (size_t)sp * W * ng_total, ng_total, m0 / mg,
-                               addEJ ? Kl : nullptr);
-      }
-    }
-    G32 = nullptr;
+    exchange_spin(sp, Gblk, W, part + (size_t)sp * W * ng_total, addEJ ? Kl : nullptr, prec);
   }
   // E[w][1] = -1/2 c sum exx;  E[w][2] = 1/2 c^2 sum_m Kl^2  (energy :169-173)
   for (int sp = 0; sp < nspin; ++sp)
     energy_finish(s, W, M_, ng_total, part + (size_t)sp * W * ng_total, Kl, -0.5 * c, 0.5 * c * c,
                   1, (addEJ && sp == nspin - 1) ? 1 : 0, E);
+}
+
+void CholeskyOps::energy_spin(int spin, const z64* G, int W, z64* E, z64* EJn, bool addH1,
+                              bool addEJ, bool addEXX, Precision prec) const {
+  AFQ_CHECK(spin == 0 || (spin == 1 && walker_type_ == kCollinear), kErrArg,
+            "spin must be 0 (alpha) or, for COLLINEAR walkers, 1 (beta)");
+  AFQ_CHECK(!addEJ || addEXX, kErrArg, "addEJ and not addEXX not yet implemented");
+  AFQ_CHECK(!addEJ || EJn, kErrArg, "addEJ needs EJn");
+  if (W == 0) return;
+  hipStream_t s = ctx_.stream();
+  const int ns = spin == 0 ? nup_ : ndown_;
+  const double c = walker_type_ == kClosed ? 2.0 : 1.0;
+  fill_z64(s, (long)W * 3, {0, 0}, E);
+  if (addEJ) fill_z64(s, (long)W * M_, {0, 0}, EJn);
+  // by convention E0 goes to the alpha block only (:195-196)
+  if (addH1 && spin == 0) fill_blocks_z64(s, W, 1, 3, {E0_, 0.0}, E);
+  if (ns == 0) return;  // an empty spin sector contributes nothing (energy_impl :499-500)
+  if (addH1)
+    dot_rows(s, W, (long)ns * N_, G, haj_ + (spin == 0 ? 0 : (size_t)nup_ * N_), c, 0.0, 0.0, E, 3);
+  if (!addEXX) return;
+  const int ng_total = ceil_div(M_, exchange_group(std::max(std::max(nup_, ndown_), 1)));
+  z64* part = scratch_as<z64>(ctx_, "ens_part", (size_t)W * ng_total);
+  exchange_spin(spin, G, W, part, addEJ ? EJn : nullptr, prec);
+  // E[w][1] = -1/2 c sum exx;  E[w][2] = 1/2 sum_m EJn^2 (:206-207)
+  energy_finish(s, W, M_, ng_total, part, EJn, -0.5 * c, 0.5, 1, addEJ ? 1 : 0, E);
 }
 
 void CholeskyOps::vMF(double dt, z64* v) const {

@@ -52,6 +52,7 @@ void* Context::scratch(const std::string& name, size_t bytes) {
     AFQ_HIP(hipFree(b.p));
     b.p = nullptr;
     b.bytes = 0;
+    ++gen_;
   }
   size_t sz = bytes < 256 ? 256 : bytes;
   if (hipMalloc(&b.p, sz) != hipSuccess) {

@@ -129,6 +129,15 @@ __global__ void k_gather_rows(long R, int C, const int* idx, const z64* src, z64
   const long r = t / C;
   dst[t] = src[(long)idx[r] * C + t % C];
 }
+// columns cols.p[k][w * stride] <-> packed[k][w] for the K property columns of W walkers
+__global__ void k_columns(int K, int W, long stride, ColumnSet cols, z64* packed, int to_packed) {
+  long t = (long)blockIdx.x * kT + threadIdx.x;
+  if (t >= (long)K * W) return;
+  const int k = (int)(t / W), w = (int)(t % W);
+  z64* c = cols.p[k] + (long)w * stride;
+  if (to_packed) packed[t] = *c;
+  else *c = packed[t];
+}
 // out[b][j][i][:] = in[b][i][j][:]   (in [B][D0][D1][K], out [B][D1][D0][K])
 __global__ void k_swap_mid(long B, int D0, int D1, int K, const z64* in, z64* out) {
   long t = (long)blockIdx.x * kT + threadIdx.x;
@@ -401,6 +410,14 @@ void broadcast_z64(hipStream_t s, long W, long n, const z64* src, z64* dst) {
 void gather_rows(hipStream_t s, long R, int C, const int* idx, const z64* src, z64* dst) {
   if (R * C <= 0) return;
   hipLaunchKernelGGL(k_gather_rows, grid1(R * C), dim3(kT), 0, s, R, C, idx, src, dst);
+  AFQ_LAUNCH_CHECK();
+}
+void copy_columns(hipStream_t s, int K, int W, long stride, const ColumnSet& cols, z64* packed,
+                  bool to_packed) {
+  AFQ_CHECK(K >= 0 && K <= ColumnSet::kMax, kErrArg, "too many columns");
+  if ((long)K * W <= 0) return;
+  hipLaunchKernelGGL(k_columns, grid1((long)K * W), dim3(kT), 0, s, K, W, stride, cols, packed,
+                     to_packed ? 1 : 0);
   AFQ_LAUNCH_CHECK();
 }
 void swap_mid(hipStream_t s, long B, int D0, int D1, int K, const z64* in, z64* out) {

@@ -91,6 +91,7 @@ __global__ void __launch_bounds__(kXThreads)
   const int w = blockIdx.x;
   const uint32_t step = p.step_ptr ? *p.step_ptr : p.step;
   double hwr = 0, hwi = 0, xsr = 0, xsi = 0, vv = 0;
+  int hits = 0;
   for (int m = threadIdx.x; m < M; m += kXThreads) {
     z64 vm = vMF[m];
     double am = hypot(vm.re, vm.im);
@@ -98,6 +99,7 @@ __global__ void __launch_bounds__(kXThreads)
     long idx = (long)w * M + m;
     z64 x = X[idx];
     double ax = hypot(x.re, x.im);
+    hits += ax > p.bound ? 1 : 0;
     z64 vbt = ax > p.bound ? z64{x.re / ax * p.bound, x.im / ax * p.bound} : x;
     if (p.project_vbias) vbt.im = 0.0;
     // vdiff = i (vb_t - vmf_t)
@@ -120,6 +122,13 @@ __global__ void __launch_bounds__(kXThreads)
   xsr = block_sum<kXThreads>(xsr, red);
   xsi = block_sum<kXThreads>(xsi, red);
   vv = block_sum<kXThreads>(vv, red);
+  if (p.stats) {
+    const double h = block_sum<kXThreads>((double)hits, red);
+    if (threadIdx.x == 0) {
+      atomicAdd(&p.stats[0], (unsigned long long)M);
+      if (h > 0) atomicAdd(&p.stats[1], (unsigned long long)h);
+    }
+  }
   if (threadIdx.x == 0) {
     HW[w] = {hwr, hwi};
     // mf = i * sum X vMF - 0.5 Re(sum vMF^2)
@@ -147,8 +156,9 @@ __global__ void k_walker_update(int W, UpdateParams p, const double* __restrict_
                                 z64* __restrict__ ovlp, z64* __restrict__ theta,
                                 z64* __restrict__ wfac, int hl, const uint32_t* __restrict__ step_ptr,
                                 const z64* __restrict__ E) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= W) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool e_counted = false, e_up = false, e_lo = false, w_up = false;
+  if (i < W) {
   const double Eshift = *Eshift_p;
   const double hi = Eshift + p.upper_cut * sqrt(2.0 / p.dt);
   const double lo = Eshift - p.lower_cut * sqrt(2.0 / p.dt);
@@ -173,7 +183,12 @@ __global__ void k_walker_update(int W, UpdateParams p, const double* __restrict_
     ph = {0.0, 0.0};
     e = old_e;
   } else {
-    if (!p.free_projection) e.re = fmin(fmax(e.re, lo), hi);
+    if (!p.free_projection) {
+      e_counted = true;
+      e_up = e.re > hi;
+      e_lo = !e_up && e.re < lo;
+      e.re = fmin(fmax(e.re, lo), hi);
+    }
     // the hybrid expression is the log of the step factor itself; a local energy enters
     // through the midpoint of the old and the new time slice
     const z64 ee = p.local_energy ? z64{0.5 * (e.re + old_e.re), 0.5 * (e.im + old_e.im)} : e;
@@ -190,12 +205,35 @@ __global__ void k_walker_update(int W, UpdateParams p, const double* __restrict_
   // bound_walker_weights
   double maxw = fmax(p.weight_floor, p.weight_frac * p.target_pop);
   double a = hypot(w.re, w.im);
+  w_up = a > maxw;
   if (a > maxw) w = {w.re * maxw / a, w.im * maxw / a};
   weight[i] = w;
   phase[i] = ph;
   eloc_p[i] = e;
   ovlp[i] = nv;
   if (wfac) wfac[(long)i * hl + (*step_ptr) % (uint32_t)hl] = fac;
+  }
+  if (p.stats) {  // one atomic per wavefront and counter
+    const unsigned long long c[4] = {__ballot(e_counted), __ballot(e_up), __ballot(e_lo),
+                                     __ballot(w_up)};
+    const unsigned long long nw = __ballot(i < W);
+    if ((threadIdx.x & 63) == 0) {
+      if (c[0]) atomicAdd(&p.stats[3], (unsigned long long)__popcll(c[0]));
+      if (c[1]) atomicAdd(&p.stats[4], (unsigned long long)__popcll(c[1]));
+      if (c[2]) atomicAdd(&p.stats[5], (unsigned long long)__popcll(c[2]));
+      if (nw) atomicAdd(&p.stats[6], (unsigned long long)__popcll(nw));
+      if (c[3]) atomicAdd(&p.stats[7], (unsigned long long)__popcll(c[3]));
+    }
+  }
+}
+
+__global__ void k_store_energies(int W, const z64* __restrict__ E, z64* e1, z64* exx, z64* ej,
+                                 long bs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W) return;
+  if (e1) e1[(long)i * bs] = E[3 * i];
+  if (exx) exx[(long)i * bs] = E[3 * i + 1];
+  if (ej) ej[(long)i * bs] = E[3 * i + 2];
 }
 
 __global__ void k_store_fields(int W, int M, const z64* __restrict__ X, z64* __restrict__ ring,
@@ -323,21 +361,21 @@ __global__ void __launch_bounds__(kCombThreads)
 }
 
 // copy walker rows into the freed slots: Slater blocks and all properties
-__global__ void k_copy_walkers(int W, const int* __restrict__ src, long rowA, long rowB,
-                               z64* PhiA, z64* PhiB, z64* weight, z64* phase, z64* eloc,
+__global__ void k_copy_walkers(int W, const int* __restrict__ src, long rowA, long rowB, long bsA,
+                               long bsB, z64* PhiA, z64* PhiB, z64* weight, z64* phase, z64* eloc,
                                z64* ovlp, z64* theta, WalkerRows extra) {
   const int d = blockIdx.x;
   const int s = src[d];
   if (s == d) return;
-  for (long e = threadIdx.x; e < rowA; e += blockDim.x) PhiA[d * rowA + e] = PhiA[s * rowA + e];
+  for (long e = threadIdx.x; e < rowA; e += blockDim.x) PhiA[d * bsA + e] = PhiA[s * bsA + e];
   if (PhiB)
-    for (long e = threadIdx.x; e < rowB; e += blockDim.x) PhiB[d * rowB + e] = PhiB[s * rowB + e];
+    for (long e = threadIdx.x; e < rowB; e += blockDim.x) PhiB[d * bsB + e] = PhiB[s * bsB + e];
 #pragma unroll
   for (int r = 0; r < WalkerRows::kMax; ++r) {
     z64* q = extra.p[r];
-    const long n = extra.n[r];
+    const long n = extra.n[r], bs = extra.bs[r] ? extra.bs[r] : n;
     if (q)
-      for (long e = threadIdx.x; e < n; e += blockDim.x) q[d * n + e] = q[s * n + e];
+      for (long e = threadIdx.x; e < n; e += blockDim.x) q[d * bs + e] = q[s * bs + e];
   }
   if (threadIdx.x == 0) {
     weight[d] = weight[s];
@@ -369,16 +407,17 @@ void increment_counter(hipStream_t s, uint32_t* c) {
 }
 
 void population_control(hipStream_t s, int W, const double* r_dev, uint64_t seed, uint32_t stream,
-                        const uint32_t* step_ptr, long rowA, long rowB, z64* PhiA, z64* PhiB,
-                        z64* weight, z64* phase, z64* eloc, z64* ovlp, z64* theta, double* cum,
-                        int* src, int* status, const WalkerRows& extra) {
+                        const uint32_t* step_ptr, long rowA, long rowB, long bsA, long bsB,
+                        z64* PhiA, z64* PhiB, z64* weight, z64* phase, z64* eloc, z64* ovlp,
+                        z64* theta, double* cum, int* src, int* status, const WalkerRows& extra) {
   if (W <= 0) return;
   AFQ_CHECK((size_t)W * sizeof(int) <= 65536, kErrShape, "population control supports W <= 16384");
   hipLaunchKernelGGL(k_comb_plan, dim3(1), dim3(kCombThreads), sizeof(int) * W, s, W, weight, r_dev,
                      seed, stream, step_ptr, cum, src, status);
   AFQ_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_copy_walkers, dim3(W), dim3(256), 0, s, W, src, rowA, rowB, PhiA, PhiB,
-                     weight, phase, eloc, ovlp, theta, extra);
+  AFQ_CHECK(bsA >= rowA && (!PhiB || bsB >= rowB), kErrShape, "walker stride below the block size");
+  hipLaunchKernelGGL(k_copy_walkers, dim3(W), dim3(256), 0, s, W, src, rowA, rowB, bsA, bsB, PhiA,
+                     PhiB, weight, phase, eloc, ovlp, theta, extra);
   AFQ_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_unit_weights, dim3(ceil_div(W, 256)), dim3(256), 0, s, W, status, weight);
   AFQ_LAUNCH_CHECK();
@@ -409,6 +448,14 @@ void walker_update(hipStream_t s, int W, const UpdateParams& p, const double* Es
   AFQ_CHECK(!p.local_energy || E, kErrArg, "local-energy update needs the energies");
   hipLaunchKernelGGL(k_walker_update, dim3(ceil_div(W, 256)), dim3(256), 0, s, W, p, Eshift,
                      new_ovlp, mf, HW, weight, phase, eloc, ovlp, theta, wfac, hl, step_ptr, E);
+  AFQ_LAUNCH_CHECK();
+}
+
+void store_energy_components(hipStream_t s, int W, const z64* E, z64* e1, z64* exx, z64* ej,
+                             long bs) {
+  if (W <= 0 || !(e1 || exx || ej)) return;
+  hipLaunchKernelGGL(k_store_energies, dim3(ceil_div(W, 256)), dim3(256), 0, s, W, E, e1, exx, ej,
+                     bs);
   AFQ_LAUNCH_CHECK();
 }
 

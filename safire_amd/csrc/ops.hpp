@@ -31,6 +31,9 @@ class Context {
   void* scratch(const std::string& name, size_t bytes);
   size_t scratch_bytes() const;
   bool capturing() const;
+  // Incremented whenever a scratch buffer is freed to grow it: a HIP graph captured under an
+  // older generation may hold the freed address and must be re-captured.
+  uint64_t generation() const { return gen_; }
 
  private:
   int device_;
@@ -41,6 +44,7 @@ class Context {
     size_t bytes = 0;
   };
   std::map<std::string, Buf> bufs_;
+  uint64_t gen_ = 0;
 };
 
 template <typename T>
@@ -141,6 +145,11 @@ class CholeskyOps {
   // E[w][0..2] = E1, EXX, EJ  (energy :138-174, energy_impl :476-590); G compact [W][nel][N]
   void energy(const z64* G, int W, z64* E, bool addH1, bool addEJ, bool addEXX,
               Precision prec) const;
+  // per-spin energy (Real3IndexFactorization.hpp:176-210): G [W][n_spin][N] of one block,
+  // E [W][3] (E0 in the alpha block only), EJn [W][M] = sum_a T^m_aa (may be null without
+  // addEJ), E[w][2] = 1/2 sum_m EJn^2
+  void energy_spin(int spin, const z64* G, int W, z64* E, z64* EJn, bool addH1, bool addEJ,
+                   bool addEXX, Precision prec) const;
   // vMF (NOMSD.cpp:437-464) with the charge-decomposition zero_imag (AFQMCBasePropagator.cpp:105)
   void vMF(double dt, z64* v) const;
   // P1 = exp(-1/2 hermitize(dt (h + vexx) + vHS(vMF))) with |x| <= 1e-8 dropped
@@ -167,6 +176,7 @@ class CholeskyOps {
   c32* Lnak32_ = nullptr;
   z64* haj_ = nullptr;      // [nel][N]
   void free_all();
+  void exchange_spin(int sp, const z64* Gblk, int W, z64* part, z64* Kl, Precision prec) const;
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -203,12 +213,15 @@ class PhmsdTrial {
   int nup() const { return sp_[0].nel; }
   int ndn() const { return sp_[1].nel; }
   CholeskyOps& ham() const { return ham_; }
+  // Walker blocks Pa [W][N][nel_a], Pb [W][N][nel_b], consecutive walkers bs_a / bs_b complex
+  // elements apart (0: contiguous).
   // compact G [W][nact_a + nact_b][N] and log overlap Ov (PHMSD::MixedDensityMatrix, compact)
-  void mixed_density_matrix(const z64* Pa, const z64* Pb, int W, z64* G, z64* Ov);
+  void mixed_density_matrix(const z64* Pa, const z64* Pb, int W, z64* G, z64* Ov, long bs_a = 0,
+                            long bs_b = 0);
   // PHMSD::Log_Overlap
-  void log_overlap(const z64* Pa, const z64* Pb, int W, z64* Ov);
+  void log_overlap(const z64* Pa, const z64* Pb, int W, z64* Ov, long bs_a = 0, long bs_b = 0);
   // PHMSD::Energy (energy_alg1): E [W][3], Ov [W]
-  void energy(const z64* Pa, const z64* Pb, int W, z64* E, z64* Ov);
+  void energy(const z64* Pa, const z64* Pb, int W, z64* E, z64* Ov, long bs_a = 0, long bs_b = 0);
   // PHMSD::vMF followed by the charge-decomposition zero_imag
   void vMF(double dt, z64* v);
   int ndet() const { return ndet_; }
@@ -254,7 +267,7 @@ class PhmsdTrial {
   std::vector<z64> det_c_h_;
   // occupied active orbitals of every configuration of spin sp, [nconf][nel]
   std::vector<int> occupations(int sp) const;
-  void woodbury(int s, const z64* Phi, int W, bool want_G, z64* log_ov);
+  void woodbury(int s, const z64* Phi, long phi_bs, int W, bool want_G, z64* log_ov);
   void ratios(int s, int W, bool want_inverse);
   void weights(int W, z64* Ov);
   void free_all();
@@ -296,6 +309,19 @@ struct WalkerState {
   z64* ovlp;     // OVLP (log overlap)
   z64* theta;
   int W;
+  // Walker strides in complex elements.  phi_bs / phi_b_bs: between consecutive walkers' Slater
+  // blocks (0: contiguous, N * nup / N * ndown); prop_bs: between consecutive walkers'
+  // properties (1: contiguous).  SAFIRE's walker buffer keeps one row per walker
+  // (WalkerSetBase.cpp:75-121), so there every stride is the row length (WalkerSetBase.h:256-276,
+  // 463-481).
+  long phi_bs = 0, phi_b_bs = 0, prop_bs = 1;
+  // E1_, EXX_, EJ_ columns (may be null), e_bs apart: the local-energy step (hybrid = false)
+  // stores the energy components there (WalkerSetUpdate.hpp:180-184), and population control
+  // copies them with their walkers
+  z64* e1 = nullptr;
+  z64* exx = nullptr;
+  z64* ej = nullptr;
+  long e_bs = 1;
 };
 
 class Propagator {
@@ -333,6 +359,11 @@ class Propagator {
   void set_vhs_timing(bool on);
   // duration of the last step's vHS in ms (waits for the end event)
   float vhs_ms();
+  // bound-trigger counters since creation or the last reset, in the order of
+  // AFQMCBasePropagator::printBoundStatistics (AFQMCBasePropagator.h:195-235): force-bias clamp
+  // total / upper / lower (per walker and field), local-energy clamp total / upper / lower and
+  // weight clamp total / upper / lower (per walker).  Synchronizes the stream.
+  void bound_stats(long long out[9], bool reset);
 
   // ---- back-propagation (AFQMCBasePropagator::BackPropagate, AFQMCBasePropagator.cpp:404-560;
   // BackPropEstimator.hpp).  enable_backprop makes every later step store its fields in a ring
@@ -404,6 +435,7 @@ class Propagator {
   bool real_P1_ = false;
   uint32_t* step_dev_ = nullptr;
   int* status_dev_ = nullptr;
+  unsigned long long* stats_dev_ = nullptr;  // [9] bound-trigger counters
   // graph cache, one executable graph per (walker buffers, ortho, relaxation) key
   struct GraphEntry {
     std::vector<const void*> key;
@@ -411,7 +443,9 @@ class Propagator {
     hipGraphExec_t exec;
   };
   std::vector<GraphEntry> graphs_;
-  bool warmed_ = false;
+  uint64_t graph_gen_ = 0;  // context scratch generation the cached graphs were captured under
+  // keys whose body ran eagerly once (every scratch buffer it needs exists, its tiles are tuned)
+  std::vector<std::vector<const void*>> warmed_;
   bool vhs_timing_ = false;
   hipEvent_t vhs_ev_[2] = {nullptr, nullptr};
   int bp_n_ = 0, bp_hl_ = 0;
@@ -443,6 +477,19 @@ class Propagator {
                     const z64* weight, const z64* ldR, z64* onerdm, z64* diag2rdm);
   void clear_graphs();
   void step_impl(const WalkerState& s, const double* u, const double* Eshift);
+  void orthogonalize_impl(const WalkerState& s);
+  void population_control_impl(const WalkerState& s, const double* r_dev);
+  // walker strides of a state (the contiguous defaults filled in)
+  long bs_a(const WalkerState& s) const { return s.phi_bs ? s.phi_bs : (long)ham_.N() * wup_; }
+  long bs_b(const WalkerState& s) const { return s.phi_b_bs ? s.phi_b_bs : (long)ham_.N() * wdn_; }
+  void check_strides(const WalkerState& s) const;
+  // The state with its properties in contiguous scratch (gathered from strided columns; the
+  // state itself when they are contiguous) and the scatter back after a call that updated them.
+  WalkerState stage_properties(const WalkerState& s);
+  void unstage_properties(const WalkerState& s, const WalkerState& staged);
+  // A fully contiguous copy of a strided state (Slater blocks and properties) for the read-only
+  // estimator calls; the state itself when it is contiguous.
+  WalkerState packed(const WalkerState& s);
 };
 
 }  // namespace afq

@@ -883,17 +883,19 @@ void PhmsdTrial::free_all() {
 
 static std::string tag(const char* base, int s) { return std::string(base) + (s ? "1" : "0"); }
 
-void PhmsdTrial::woodbury(int s, const z64* Phi, int W, bool want_G, z64* log_ov) {
+void PhmsdTrial::woodbury(int s, const z64* Phi, long phi_bs, int W, bool want_G, z64* log_ov) {
   using gemm::EpiStore;
   using gemm::Opnd;
   const SpinDev& d = sp_[s];
   const int N = ham_.N(), n = d.nel, na = d.nact;
+  if (phi_bs == 0) phi_bs = (long)N * n;
+  AFQ_CHECK(phi_bs >= (long)N * n, kErrShape, "walker stride below the Slater block size");
   z64* TAB = scratch_as<z64>(ctx_, "ph_TAB", (size_t)W * na * n);
   z64* Ti = scratch_as<z64>(ctx_, tag("ph_Ti", s), (size_t)W * n * n);
   z64* Qv = scratch_as<z64>(ctx_, tag("ph_Qv", s), (size_t)W * std::max(d.nvir, 1) * n);
   {  // TAB = orbs * Phi
     Opnd<double, true, true> A{&d.orbs->re, N, 0};
-    Opnd<double, true, false> B{&Phi->re, n, (long)N * n};
+    Opnd<double, true, false> B{&Phi->re, n, phi_bs};
     gemm_small<double>(ctx_, "ph_TAB", W, na, n, N, A, B,
                        EpiStore<double, double>{TAB, n, (long)na * n, 1, 0, 0});
   }
@@ -908,7 +910,7 @@ void PhmsdTrial::woodbury(int s, const z64* Phi, int W, bool want_G, z64* log_ov
   if (want_G) {  // GA = TNN^{-T} Phi^T
     z64* GA = scratch_as<z64>(ctx_, tag("ph_GA", s), (size_t)W * n * N);
     Opnd<double, true, false> A{&Ti->re, n, (long)n * n};
-    Opnd<double, true, true> B{&Phi->re, n, (long)N * n};
+    Opnd<double, true, true> B{&Phi->re, n, phi_bs};
     gemm_small<double>(ctx_, "mdm_G", W, n, N, n, A, B,
                        EpiStore<double, double>{GA, N, (long)n * N, 1, 0, 0});
   }
@@ -964,7 +966,8 @@ void PhmsdTrial::weights(int W, z64* Ov) {
   AFQ_LAUNCH_CHECK();
 }
 
-void PhmsdTrial::mixed_density_matrix(const z64* Pa, const z64* Pb, int W, z64* G, z64* Ov) {
+void PhmsdTrial::mixed_density_matrix(const z64* Pa, const z64* Pb, int W, z64* G, z64* Ov,
+                                      long bs_a, long bs_b) {
   using gemm::EpiStore;
   using gemm::Opnd;
   if (W <= 0) return;
@@ -973,9 +976,9 @@ void PhmsdTrial::mixed_density_matrix(const z64* Pa, const z64* Pb, int W, z64* 
   z64* log_ov = scratch_as<z64>(ctx_, "ph_logov", W);
   z64* Osum = scratch_as<z64>(ctx_, "ph_Osum", W);
   fill_z64(s, W, {0, 0}, log_ov);
-  woodbury(0, Pa, W, true, log_ov);
+  woodbury(0, Pa, bs_a, W, true, log_ov);
   ratios(0, W, true);
-  woodbury(1, Pb, W, true, log_ov);
+  woodbury(1, Pb, bs_b, W, true, log_ov);
   ratios(1, W, true);
   weights(W, Osum);
   for (int sp = 0; sp < 2; ++sp) {
@@ -1011,22 +1014,23 @@ void PhmsdTrial::mixed_density_matrix(const z64* Pa, const z64* Pb, int W, z64* 
   AFQ_LAUNCH_CHECK();
 }
 
-void PhmsdTrial::log_overlap(const z64* Pa, const z64* Pb, int W, z64* Ov) {
+void PhmsdTrial::log_overlap(const z64* Pa, const z64* Pb, int W, z64* Ov, long bs_a, long bs_b) {
   if (W <= 0) return;
   hipStream_t s = ctx_.stream();
   z64* log_ov = scratch_as<z64>(ctx_, "ph_logov", W);
   z64* Osum = scratch_as<z64>(ctx_, "ph_Osum", W);
   fill_z64(s, W, {0, 0}, log_ov);
-  woodbury(0, Pa, W, false, log_ov);
+  woodbury(0, Pa, bs_a, W, false, log_ov);
   ratios(0, W, false);
-  woodbury(1, Pb, W, false, log_ov);
+  woodbury(1, Pb, bs_b, W, false, log_ov);
   ratios(1, W, false);
   weights(W, Osum);
   hipLaunchKernelGGL(k_ph_log, dim3(ceil_div(W, 256)), dim3(256), 0, s, W, Osum, log_ov, Ov);
   AFQ_LAUNCH_CHECK();
 }
 
-void PhmsdTrial::energy(const z64* Pa, const z64* Pb, int W, z64* E, z64* Ov) {
+void PhmsdTrial::energy(const z64* Pa, const z64* Pb, int W, z64* E, z64* Ov, long bs_a,
+                        long bs_b) {
   using gemm::EpiStore;
   using gemm::Opnd;
   if (W <= 0) return;
@@ -1035,9 +1039,9 @@ void PhmsdTrial::energy(const z64* Pa, const z64* Pb, int W, z64* E, z64* Ov) {
   z64* log_ov = scratch_as<z64>(ctx_, "ph_logov", W);
   z64* Osum = scratch_as<z64>(ctx_, "ph_Osum", W);
   fill_z64(s, W, {0, 0}, log_ov);
-  woodbury(0, Pa, W, true, log_ov);
+  woodbury(0, Pa, bs_a, W, true, log_ov);
   ratios(0, W, true);
-  woodbury(1, Pb, W, true, log_ov);
+  woodbury(1, Pb, bs_b, W, true, log_ov);
   ratios(1, W, true);
   weights(W, Osum);
   int nch[2];

@@ -14,6 +14,7 @@
 // mode vbias, vHS and the Taylor propagation run in FP32 on the matrix cores; G, overlaps, the
 // LU/QR, fields and weights stay FP64, and G is recomputed in FP64 from the FP64 walker matrices
 // after every propagation.
+#include <algorithm>
 #include <vector>
 
 #include "gemm.hpp"
@@ -52,6 +53,8 @@ Propagator::Propagator(CholeskyOps& ham, int W, const PropagatorParams& p, Phmsd
     AFQ_HIP(hipMemsetAsync(step_dev_, 0, sizeof(uint32_t), s));
     status_dev_ = dalloc<int>(1);
     AFQ_HIP(hipMemsetAsync(status_dev_, 0, sizeof(int), s));
+    stats_dev_ = dalloc<unsigned long long>(9);
+    AFQ_HIP(hipMemsetAsync(stats_dev_, 0, 9 * sizeof(unsigned long long), s));
     // generateP1: vMF (subtractMF = true), then H1 and P1
     if (trial_)
       trial_->vMF(p_.dt, vMF_);
@@ -78,7 +81,7 @@ Propagator::Propagator(CholeskyOps& ham, int W, const PropagatorParams& p, Phmsd
     AFQ_HIP(hipStreamSynchronize(s));
   } catch (...) {
     for (void* q : {(void*)vMF_, (void*)P1_, (void*)P1r_, (void*)P1r32_, (void*)P1c32_,
-                    (void*)step_dev_, (void*)status_dev_})
+                    (void*)step_dev_, (void*)status_dev_, (void*)stats_dev_})
       if (q) (void)hipFree(q);
     throw;
   }
@@ -93,7 +96,8 @@ Propagator::~Propagator() {
     (void)hipGraphDestroy(g.graph);
   }
   for (void* q : {(void*)vMF_, (void*)P1_, (void*)P1r_, (void*)P1r32_, (void*)P1c32_,
-                  (void*)step_dev_, (void*)status_dev_, (void*)bp_fields_, (void*)bp_wfac_,
+                  (void*)step_dev_, (void*)status_dev_, (void*)stats_dev_, (void*)bp_fields_,
+                  (void*)bp_wfac_,
                   (void*)bp_anch_a_, (void*)bp_anch_b_, (void*)ref_rows_[0], (void*)ref_rows_[1],
                   (void*)ref_cc_, (void*)ref_sel_[0], (void*)ref_sel_[1], (void*)ref_cols_[0],
                   (void*)ref_cols_[1], (void*)te_op_[0], (void*)te_op_[1], (void*)te_op_[2],
@@ -106,19 +110,76 @@ void Propagator::set_step(uint32_t step) {
   AFQ_HIP(hipStreamSynchronize(ctx_.stream()));
 }
 
+void Propagator::check_strides(const WalkerState& st) const {
+  const long N = ham_.N();
+  AFQ_CHECK(st.phi_bs == 0 || st.phi_bs >= N * wup_, kErrShape,
+            "walker stride of the alpha block is below N * nup");
+  AFQ_CHECK(st.phi_b_bs == 0 || st.phi_b_bs >= N * wdn_, kErrShape,
+            "walker stride of the beta block is below N * ndown");
+  AFQ_CHECK(st.prop_bs >= 1, kErrShape, "property stride must be positive");
+}
+
+WalkerState Propagator::stage_properties(const WalkerState& st) {
+  if (st.prop_bs == 1) return st;
+  z64* pk = scratch_as<z64>(ctx_, "wk_props", (size_t)5 * st.W);
+  ColumnSet cols;
+  z64* src[5] = {st.weight, st.phase, st.eloc, st.ovlp, st.theta};
+  for (int k = 0; k < 5; ++k) cols.p[k] = src[k];
+  copy_columns(ctx_.stream(), 5, st.W, st.prop_bs, cols, pk, true);
+  WalkerState o = st;
+  o.weight = pk;
+  o.phase = pk + st.W;
+  o.eloc = pk + 2 * st.W;
+  o.ovlp = pk + 3 * st.W;
+  o.theta = pk + 4 * st.W;
+  o.prop_bs = 1;
+  return o;
+}
+
+void Propagator::unstage_properties(const WalkerState& st, const WalkerState& staged) {
+  if (st.prop_bs == 1) return;
+  ColumnSet cols;
+  z64* dst[5] = {st.weight, st.phase, st.eloc, st.ovlp, st.theta};
+  for (int k = 0; k < 5; ++k) cols.p[k] = dst[k];
+  copy_columns(ctx_.stream(), 5, st.W, st.prop_bs, cols, staged.weight, false);
+}
+
+WalkerState Propagator::packed(const WalkerState& st) {
+  check_strides(st);
+  const long N = ham_.N();
+  const bool coll = !closed_ && wdn_ > 0;
+  const bool phi_c = bs_a(st) == N * wup_ && (!coll || bs_b(st) == N * wdn_);
+  WalkerState o = stage_properties(st);
+  if (phi_c) return o;
+  hipStream_t s = ctx_.stream();
+  z64* a = scratch_as<z64>(ctx_, "wk_phi_a", (size_t)st.W * N * std::max(wup_, 1));
+  AFQ_HIP(hipMemcpy2DAsync(a, sizeof(z64) * N * wup_, st.Phi, sizeof(z64) * bs_a(st),
+                           sizeof(z64) * N * wup_, st.W, hipMemcpyDeviceToDevice, s));
+  o.Phi = a;
+  o.phi_bs = 0;
+  if (coll) {
+    z64* b = scratch_as<z64>(ctx_, "wk_phi_b", (size_t)st.W * N * wdn_);
+    AFQ_HIP(hipMemcpy2DAsync(b, sizeof(z64) * N * wdn_, st.Phi_b, sizeof(z64) * bs_b(st),
+                             sizeof(z64) * N * wdn_, st.W, hipMemcpyDeviceToDevice, s));
+    o.Phi_b = b;
+    o.phi_b_bs = 0;
+  }
+  return o;
+}
+
 void Propagator::log_overlap(const WalkerState& st, z64* ovlp) {
   if (trial_) {
-    trial_->log_overlap(st.Phi, st.Phi_b, st.W, ovlp);
+    trial_->log_overlap(st.Phi, st.Phi_b, st.W, ovlp, bs_a(st), bs_b(st));
     return;
   }
   hipStream_t s = ctx_.stream();
   const int N = ham_.N(), nup = ham_.nup(), ndown = ham_.ndown();
   const bool closed = closed_;
   fill_z64(s, st.W, {0, 0}, ovlp);
-  det_ops::log_overlap(ctx_, ham_.PsiT(), nup, N, st.Phi, (long)N * nup, st.W, ovlp,
+  det_ops::log_overlap(ctx_, ham_.PsiT(), nup, N, st.Phi, bs_a(st), st.W, ovlp,
                        closed ? 2.0 : 1.0);
   if (!closed && ndown > 0)
-    det_ops::log_overlap(ctx_, ham_.PsiT() + (size_t)nup * N, ndown, N, st.Phi_b, (long)N * ndown,
+    det_ops::log_overlap(ctx_, ham_.PsiT() + (size_t)nup * N, ndown, N, st.Phi_b, bs_b(st),
                          st.W, ovlp, 1.0);
   // + log(conj(ci0)) = 0 for the single determinant with unit coefficient
 }
@@ -142,16 +203,16 @@ void Propagator::step_impl(const WalkerState& st, const double* u, const double*
   // precision G is computed in FP64 and stored as complex64 for the FP32 vbias contraction (the
   // values the contraction would round it to anyway; half the bytes it re-reads).
   if (trial_) {
-    trial_->mixed_density_matrix(st.Phi, st.Phi_b, W, G, ov);
+    trial_->mixed_density_matrix(st.Phi, st.Phi_b, W, G, ov, bs_a(st), bs_b(st));
     ham_.vbias(G, W, p_.dt, X, prec_);
   } else {
     c32* G32 = prec_ == kMixed ? scratch_as<c32>(ctx_, "st_G32", (size_t)W * nel * N) : nullptr;
     fill_z64(s, W, {0, 0}, ov);
-    det_ops::mixed_density_matrix(ctx_, ham_.PsiT(), nup, N, st.Phi, (long)N * nup, W, G,
+    det_ops::mixed_density_matrix(ctx_, ham_.PsiT(), nup, N, st.Phi, bs_a(st), W, G,
                                   (long)nel * N, ov, true, closed ? 2.0 : 1.0, G32);
     if (!closed && ndown > 0)
       det_ops::mixed_density_matrix(ctx_, ham_.PsiT() + (size_t)nup * N, ndown, N, st.Phi_b,
-                                    (long)N * ndown, W, G + (size_t)nup * N, (long)nel * N, ov,
+                                    bs_b(st), W, G + (size_t)nup * N, (long)nel * N, ov,
                                     true, 1.0, G32 ? G32 + (size_t)nup * N : nullptr);
     if (G32)
       ham_.vbias32(G32, W, p_.dt, X);
@@ -160,7 +221,7 @@ void Propagator::step_impl(const WalkerState& st, const double* u, const double*
   }
   // 2. fields, hybrid weight, mean-field factor
   FieldParams fp{p_.vbias_bound, (p_.use_cp || p_.project_force_bias) ? 1 : 0, p_.free_projection,
-                 p_.seed, p_.stream, 0, u ? nullptr : step_dev_};
+                 p_.seed, p_.stream, 0, u ? nullptr : step_dev_, stats_dev_};
   construct_x(s, u, vMF_, W, M, fp, X, HW, mf, X32);
   if (bp_n_ > 0) store_fields(s, W, M, X, bp_fields_, bp_n_, step_dev_);
   // 3. vHS
@@ -169,12 +230,12 @@ void Propagator::step_impl(const WalkerState& st, const double* u, const double*
   if (vhs_timing_) AFQ_HIP(hipEventRecord(vhs_ev_[1], s));
   // 4. propagate both spin blocks with the same V
   det_ops::PropagateArgs pa{W, N, nup, p_.order, 'N', P1_, real_P1_ ? P1r_ : nullptr, P1r32_,
-                            P1c32_, V, (long)N * N, st.Phi, (long)N * nup, prec_};
+                            P1c32_, V, (long)N * N, st.Phi, bs_a(st), prec_};
   det_ops::propagate(ctx_, pa);
   if (!closed && ndown > 0) {
     pa.n = ndown;
     pa.Phi = st.Phi_b;
-    pa.phi_bs = (long)N * ndown;
+    pa.phi_bs = bs_b(st);
     det_ops::propagate(ctx_, pa);
   }
   // 5. new overlaps (hybrid), or the local energies and overlaps (hybrid = false)
@@ -187,9 +248,21 @@ void Propagator::step_impl(const WalkerState& st, const double* u, const double*
   }
   // 6. weights
   UpdateParams up{p_.dt, ham_.E0(), p_.lower_cut, p_.upper_cut, p_.weight_floor, p_.weight_frac,
-                  p_.target_pop, p_.free_projection, p_.use_cp, p_.local_energy};
+                  p_.target_pop, p_.free_projection, p_.use_cp, p_.local_energy, stats_dev_};
   walker_update(s, W, up, Eshift, nov, mf, HW, st.weight, st.phase, st.eloc, st.ovlp, st.theta,
                 bp_n_ > 0 ? bp_wfac_ : nullptr, bp_hl_, step_dev_, En);
+  if (En) store_energy_components(s, W, En, st.e1, st.exx, st.ej, st.e_bs);
+}
+
+void Propagator::bound_stats(long long out[9], bool reset) {
+  unsigned long long h[9];
+  AFQ_HIP(hipMemcpyAsync(h, stats_dev_, sizeof(h), hipMemcpyDeviceToHost, ctx_.stream()));
+  AFQ_HIP(hipStreamSynchronize(ctx_.stream()));
+  for (int i = 0; i < 9; ++i) out[i] = (long long)h[i];
+  if (reset) {
+    AFQ_HIP(hipMemsetAsync(stats_dev_, 0, sizeof(h), ctx_.stream()));
+    AFQ_HIP(hipStreamSynchronize(ctx_.stream()));
+  }
 }
 
 void Propagator::set_vhs_timing(bool on) {
@@ -212,23 +285,40 @@ float Propagator::vhs_ms() {
   return ms;
 }
 
-void Propagator::step(const WalkerState& st, const double* u, const double* Eshift) {
+void Propagator::step(const WalkerState& st0, const double* u, const double* Eshift) {
+  check_strides(st0);
+  const WalkerState st = stage_properties(st0);
   step_impl(st, u, Eshift);
   if (!u) increment_counter(ctx_.stream(), step_dev_);
+  unstage_properties(st0, st);
 }
 
-void Propagator::orthogonalize(const WalkerState& st) {
+void Propagator::orthogonalize(const WalkerState& st0) {
+  check_strides(st0);
+  const WalkerState st = stage_properties(st0);
+  orthogonalize_impl(st);
+  unstage_properties(st0, st);
+}
+
+void Propagator::orthogonalize_impl(const WalkerState& st) {
   hipStream_t s = ctx_.stream();
   const int N = ham_.N(), nup = wup_, ndown = wdn_;
   z64* ldet = scratch_as<z64>(ctx_, "or_ldet", st.W);
   fill_z64(s, st.W, {0, 0}, ldet);
-  det_ops::orthogonalize(ctx_, st.Phi, (long)N * nup, st.W, N, nup, ldet);
+  det_ops::orthogonalize(ctx_, st.Phi, bs_a(st), st.W, N, nup, ldet);
   if (!closed_ && ndown > 0)
-    det_ops::orthogonalize(ctx_, st.Phi_b, (long)N * ndown, st.W, N, ndown, ldet);
+    det_ops::orthogonalize(ctx_, st.Phi_b, bs_b(st), st.W, N, ndown, ldet);
   log_overlap(st, st.ovlp);
 }
 
-void Propagator::population_control(const WalkerState& st, const double* r_dev) {
+void Propagator::population_control(const WalkerState& st0, const double* r_dev) {
+  check_strides(st0);
+  const WalkerState st = stage_properties(st0);
+  population_control_impl(st, r_dev);
+  unstage_properties(st0, st);
+}
+
+void Propagator::population_control_impl(const WalkerState& st, const double* r_dev) {
   const int N = ham_.N(), nup = wup_, ndown = wdn_;
   const bool coll = !closed_ && ndown > 0;
   double* cum = scratch_as<double>(ctx_, "pc_cum", st.W);
@@ -253,10 +343,17 @@ void Propagator::population_control(const WalkerState& st, const double* r_dev) 
       extra.n[4 + i] = rowte;
     }
   }
+  z64* ecol[3] = {st.e1, st.exx, st.ej};
+  for (int i = 0; i < 3; ++i)
+    if (ecol[i]) {
+      extra.p[7 + i] = ecol[i];
+      extra.n[7 + i] = 1;
+      extra.bs[7 + i] = st.e_bs;
+    }
   afq::population_control(ctx_.stream(), st.W, r_dev, p_.seed, p_.stream + 1000u, step_dev_,
-                          (long)N * nup, coll ? (long)N * ndown : 0, st.Phi, coll ? st.Phi_b : nullptr,
-                          st.weight, st.phase, st.eloc, st.ovlp, st.theta, cum, src, status_dev_,
-                          extra);
+                          (long)N * nup, coll ? (long)N * ndown : 0, bs_a(st), coll ? bs_b(st) : 0,
+                          st.Phi, coll ? st.Phi_b : nullptr, st.weight, st.phase, st.eloc, st.ovlp,
+                          st.theta, cum, src, status_dev_, extra);
 }
 
 void Propagator::clear_graphs() {
@@ -265,7 +362,7 @@ void Propagator::clear_graphs() {
     (void)hipGraphDestroy(g.graph);
   }
   graphs_.clear();
-  warmed_ = false;
+  warmed_.clear();
 }
 
 void Propagator::enable_backprop(int nback) {
@@ -298,12 +395,14 @@ void Propagator::enable_backprop(int nback) {
 
 void Propagator::bp_anchor(const WalkerState& st) {
   AFQ_CHECK(bp_n_ > 0, kErrState, "back-propagation is not enabled");
+  check_strides(st);
   hipStream_t s = ctx_.stream();
-  const int N = ham_.N();
-  AFQ_HIP(hipMemcpyAsync(bp_anch_a_, st.Phi, sizeof(z64) * st.W * N * wup_, hipMemcpyDeviceToDevice, s));
+  const long N = ham_.N();
+  AFQ_HIP(hipMemcpy2DAsync(bp_anch_a_, sizeof(z64) * N * wup_, st.Phi, sizeof(z64) * bs_a(st),
+                           sizeof(z64) * N * wup_, st.W, hipMemcpyDeviceToDevice, s));
   if (!closed_ && wdn_ > 0)
-    AFQ_HIP(hipMemcpyAsync(bp_anch_b_, st.Phi_b, sizeof(z64) * st.W * N * wdn_,
-                           hipMemcpyDeviceToDevice, s));
+    AFQ_HIP(hipMemcpy2DAsync(bp_anch_b_, sizeof(z64) * N * wdn_, st.Phi_b, sizeof(z64) * bs_b(st),
+                             sizeof(z64) * N * wdn_, st.W, hipMemcpyDeviceToDevice, s));
 }
 
 void Propagator::ensure_references() {
@@ -494,8 +593,9 @@ void Propagator::bp_propagate_orbitals(const WalkerState& st, uint32_t C, int nb
   }
 }
 
-void Propagator::backpropagate(const WalkerState& st, int nbp, int ortho, int restore, z64* G,
+void Propagator::backpropagate(const WalkerState& st0, int nbp, int ortho, int restore, z64* G,
                                z64* wout) {
+  const WalkerState st = packed(st0);
   AFQ_CHECK(restore >= 0 && restore <= 2, kErrArg, "restore must be 0, 1 or 2");
   ensure_references();
   AFQ_CHECK(nref_ == 1, kErrArg,
@@ -580,8 +680,9 @@ void Propagator::measure_refs(int W, const z64* const Lp[2], bool per_walker,
   }
 }
 
-void Propagator::bp_measure(const WalkerState& st, int nbp, int ortho, int restore, z64* onerdm,
+void Propagator::bp_measure(const WalkerState& st0, int nbp, int ortho, int restore, z64* onerdm,
                             z64* diag2rdm) {
+  const WalkerState st = packed(st0);
   AFQ_CHECK(restore >= 0 && restore <= 2, kErrArg, "restore must be 0, 1 or 2");
   ensure_references();
   hipStream_t s = ctx_.stream();
@@ -602,7 +703,8 @@ void Propagator::bp_measure(const WalkerState& st, int nbp, int ortho, int resto
   measure_refs(W, L, true, anch, wr, ldR, onerdm, diag2rdm);
 }
 
-void Propagator::mixed_measure(const WalkerState& st, z64* onerdm, z64* diag2rdm) {
+void Propagator::mixed_measure(const WalkerState& st0, z64* onerdm, z64* diag2rdm) {
+  const WalkerState st = packed(st0);
   AFQ_CHECK(st.W <= W_, kErrShape, "walker count exceeds the propagator's capacity");
   ensure_references();
   // constructMixedMeasurementInputs (MixedEstimator.hpp:40-106): conj(ci) <D_d|Phi> normalized
@@ -675,7 +777,8 @@ void Propagator::te_anchor(bool follow_walkers) {
   fill_z64(s, (long)cnt, {0.0, 0.0}, te_op_[2]);
 }
 
-void Propagator::te_propagate(const WalkerState& st, int nsteps) {
+void Propagator::te_propagate(const WalkerState& st0, int nsteps) {
+  const WalkerState st = packed(st0);
   using gemm::EpiStore;
   using gemm::Opnd;
   AFQ_CHECK(bp_n_ > 0, kErrState, "back-propagation is not enabled");
@@ -751,8 +854,9 @@ void Propagator::te_propagate(const WalkerState& st, int nsteps) {
   }
 }
 
-void Propagator::te_measure(const WalkerState& st, int restore_steps, bool transpose_m,
+void Propagator::te_measure(const WalkerState& st0, int restore_steps, bool transpose_m,
                             z64* onerdm, z64* diag2rdm) {
+  const WalkerState st = packed(st0);
   using gemm::EpiStore;
   using gemm::Opnd;
   AFQ_CHECK(bp_n_ > 0, kErrState, "back-propagation is not enabled");
@@ -838,8 +942,9 @@ int Propagator::status() {
 }
 
 void Propagator::energy(const WalkerState& st, z64* E, z64* ovlp, Precision prec) {
+  check_strides(st);
   if (trial_) {  // PHMSD::Energy (energy_alg1), FP64
-    trial_->energy(st.Phi, st.Phi_b, st.W, E, ovlp);
+    trial_->energy(st.Phi, st.Phi_b, st.W, E, ovlp, bs_a(st), bs_b(st));
     return;
   }
   hipStream_t s = ctx_.stream();
@@ -847,46 +952,56 @@ void Propagator::energy(const WalkerState& st, z64* E, z64* ovlp, Precision prec
   const bool closed = closed_;
   z64* G = scratch_as<z64>(ctx_, "en_G", (size_t)W * nel * N);
   fill_z64(s, W, {0, 0}, ovlp);
-  det_ops::mixed_density_matrix(ctx_, ham_.PsiT(), nup, N, st.Phi, (long)N * nup, W, G,
+  det_ops::mixed_density_matrix(ctx_, ham_.PsiT(), nup, N, st.Phi, bs_a(st), W, G,
                                 (long)nel * N, ovlp, true, closed ? 2.0 : 1.0);
   if (!closed && ndown > 0)
     det_ops::mixed_density_matrix(ctx_, ham_.PsiT() + (size_t)nup * N, ndown, N, st.Phi_b,
-                                  (long)N * ndown, W, G + (size_t)nup * N, (long)nel * N, ovlp,
-                                  true);
+                                  bs_b(st), W, G + (size_t)nup * N, (long)nel * N, ovlp, true);
   ham_.energy(G, W, E, true, true, true, prec);
 }
 
-void Propagator::step_graph(const WalkerState& st, const double* Eshift, bool ortho, bool popctl,
+void Propagator::step_graph(const WalkerState& st0, const double* Eshift, bool ortho, bool popctl,
                             double relax, double* energy_out) {
   hipStream_t s = ctx_.stream();
+  check_strides(st0);
+  const WalkerState& st = st0;
   std::vector<const void*> key = {st.Phi, st.Phi_b, st.weight, st.phase, st.eloc, st.ovlp,
                                   st.theta, Eshift, energy_out, (const void*)(long)st.W,
-                                  (const void*)(long)((ortho ? 1 : 0) | (popctl ? 2 : 0))};
+                                  (const void*)(long)((ortho ? 1 : 0) | (popctl ? 2 : 0)),
+                                  (const void*)st.phi_bs, (const void*)st.phi_b_bs,
+                                  (const void*)st.prop_bs, st.e1, st.exx, st.ej,
+                                  (const void*)st.e_bs};
   long rbits;
   static_assert(sizeof(rbits) == sizeof(relax), "");
   std::memcpy(&rbits, &relax, sizeof(rbits));
   key.push_back((const void*)rbits);
   auto body = [&]() {
-    step_impl(st, nullptr, Eshift);
-    if (ortho) orthogonalize(st);
-    if (popctl) population_control(st, nullptr);
-    average_pseudo_energy(s, st.W, st.weight, st.eloc, p_.dt, relax, energy_out,
+    const WalkerState sp = stage_properties(st0);
+    step_impl(sp, nullptr, Eshift);
+    if (ortho) orthogonalize_impl(sp);
+    if (popctl) population_control_impl(sp, nullptr);
+    average_pseudo_energy(s, sp.W, sp.weight, sp.eloc, p_.dt, relax, energy_out,
                           const_cast<double*>(Eshift));
     increment_counter(s, step_dev_);
+    unstage_properties(st0, sp);
   };
-  bool have = false;
+  // A scratch buffer that grew since the graphs were captured (another call on this context
+  // needed more room) was freed: the graphs may hold its old address.
+  if (!graphs_.empty() && graph_gen_ != ctx_.generation()) clear_graphs();
   for (size_t i = 0; i < graphs_.size(); ++i)
     if (graphs_[i].key == key) {
       AFQ_HIP(hipGraphLaunch(graphs_[i].exec, s));
-      have = true;
-      break;
+      return;
     }
-  if (have) return;
-  if (!warmed_) {  // first use: run eagerly so every scratch buffer exists before capture
+  if (std::find(warmed_.begin(), warmed_.end(), key) == warmed_.end()) {
+    // first use of this key: run eagerly so that every scratch buffer its body needs exists
+    // (and its products are tuned) before capture
     body();
-    warmed_ = true;
+    warmed_.push_back(key);
     return;
   }
+  if (graphs_.empty()) graph_gen_ = ctx_.generation();
+  AFQ_CHECK(graph_gen_ == ctx_.generation(), kErrState, "scratch generation changed between captures");
   hipGraph_t g = nullptr;
   hipGraphExec_t ge = nullptr;
   AFQ_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));

@@ -7,7 +7,9 @@
 // k in the same order, so the choice never changes the result bits.  Tuning needs an
 // idempotent product (output independent of its previous contents); accumulating call sites
 // use the first candidate.  AFQ_TUNE=0 in the environment disables tuning; AFQ_TUNE_VERBOSE=1
-// prints every candidate's time to stderr.
+// prints every candidate's time to stderr.  A site can be pinned to one candidate (tests run
+// every candidate of a site this way): Tuner::force(site, i), or AFQ_TUNE_FORCE="site:i,..." in
+// the environment; a pinned choice also applies inside graph capture and is never cached.
 #pragma once
 
 #include <algorithm>
@@ -38,6 +40,30 @@ class Tuner {
     best_[key] = idx;
     ms_[key] = ms;
   }
+  // candidate pinned for the site of `key` ("site:precision", the text before its second ':',
+  // or else "site", the text before its first), or -1
+  int forced(const std::string& key, int ncand) {
+    std::lock_guard<std::mutex> g(mu_);
+    ncand_[key] = ncand;
+    if (force_.empty()) return -1;
+    const size_t c1 = key.find(':');
+    const std::string site = key.substr(0, c1);
+    auto it = force_.end();
+    if (c1 != std::string::npos) it = force_.find(key.substr(0, key.find(':', c1 + 1)));
+    if (it == force_.end()) it = force_.find(site);
+    if (it == force_.end()) return -1;
+    AFQ_CHECK(it->second < ncand, kErrArg,
+              "site " + site + " has " + std::to_string(ncand) + " candidates, " +
+                  std::to_string(it->second) + " was forced");
+    return it->second;
+  }
+  // i < 0 removes the pin; site "" removes every pin
+  void force(const std::string& site, int i) {
+    std::lock_guard<std::mutex> g(mu_);
+    if (site.empty()) force_.clear();
+    else if (i < 0) force_.erase(site);
+    else force_[site] = i;
+  }
   bool enabled() const { return enabled_; }
   bool verbose() const { return verbose_; }
   std::string report() {
@@ -45,7 +71,10 @@ class Tuner {
     std::string r;
     for (auto& kv : best_)
       r += kv.first + " -> " + std::to_string(kv.second) + " (" + std::to_string(ms_[kv.first]) +
-           " ms)\n";
+           " ms) of " + std::to_string(ncand_[kv.first]) + "\n";
+    for (auto& kv : ncand_)
+      if (!best_.count(kv.first))
+        r += kv.first + " -> untuned of " + std::to_string(kv.second) + "\n";
     return r;
   }
 
@@ -55,10 +84,24 @@ class Tuner {
     enabled_ = !(e && e[0] == '0');
     const char* v = std::getenv("AFQ_TUNE_VERBOSE");
     verbose_ = v && v[0] == '1';
+    if (const char* f = std::getenv("AFQ_TUNE_FORCE")) {
+      std::string spec(f);
+      size_t p = 0;
+      while (p < spec.size()) {
+        size_t q = spec.find(',', p);
+        if (q == std::string::npos) q = spec.size();
+        const std::string item = spec.substr(p, q - p);
+        const size_t c = item.rfind(':');
+        if (c != std::string::npos && c > 0) force_[item.substr(0, c)] = std::atoi(item.c_str() + c + 1);
+        p = q + 1;
+      }
+    }
   }
   std::mutex mu_;
   std::map<std::string, int> best_;
   std::map<std::string, float> ms_;
+  std::map<std::string, int> ncand_;
+  std::map<std::string, int> force_;
   bool enabled_ = true;
   bool verbose_ = false;
 };
@@ -66,6 +109,11 @@ class Tuner {
 template <class F>
 void tuned(Context& c, const std::string& key, int ncand, F&& launch, bool idempotent) {
   Tuner& t = Tuner::get();
+  const int pin = t.forced(key, ncand);
+  if (pin >= 0) {
+    launch(pin);
+    return;
+  }
   int choice = t.lookup(key);
   if (choice >= 0) {
     launch(choice);
@@ -107,6 +155,11 @@ void tuned(Context& c, const std::string& key, int ncand, F&& launch, bool idemp
 template <class F>
 void tuned_on_copy(Context& c, const std::string& key, int ncand, z64* buf, size_t count, F&& run) {
   Tuner& t = Tuner::get();
+  const int pin = t.forced(key, ncand);
+  if (pin >= 0) {
+    run(pin, buf);
+    return;
+  }
   int choice = t.lookup(key);
   if (choice >= 0 && choice < ncand) {
     run(choice, buf);

@@ -7,9 +7,9 @@ After equilibration the energy estimator (EnergyEstimator.h:51-90) evaluates the
 every `measure_interval` steps and records the weighted average sum_w w E_w / sum_w w
 (MeasurementOutput, Measurements.hpp:92-121) together with the weight statistics.
 
-Error bars (Trace.mean_error, mean_error) follow the reference's result tools: the series is
-rebinned with the bin size optimal_rebinsize chooses and the mean's error is the standard error
-of the bins (safire_amd.stats).  blocking_error, the Flyvbjerg-Petersen blocking analysis
+Error bars (Trace.mean_error, mean_error) are the standard error of bin means, the quantity the
+reference's result tools report: the bins span twelve integrated autocorrelation times of the
+series (safire_amd.stats).  blocking_error, the Flyvbjerg-Petersen blocking analysis
 (J. Chem. Phys. 91, 461 (1989)), is kept as an independent estimate.
 """
 from __future__ import annotations

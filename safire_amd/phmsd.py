@@ -175,16 +175,18 @@ class PHMSD:
         return (self.ham.nup, self.ham.ndown)
 
     def MixedDensityMatrix(self, wset, G, Ov):
-        afq._ck(afq.lib().afq_phmsd_mixed_density_matrix(self.h, afq.ptr(wset.Phi),
-                                                          afq.ptr(wset.Phi_b), wset.W,
-                                                          afq.ptr(G), afq.ptr(Ov)))
+        c = wset.cstruct()
+        afq._ck(afq.lib().afq_phmsd_mixed_density_matrix(self.h, c.Phi, c.Phi_b, c.phi_stride,
+                                                          wset.W, afq.ptr(G), afq.ptr(Ov)))
 
     def Log_Overlap(self, wset, Ov):
-        afq._ck(afq.lib().afq_phmsd_log_overlap(self.h, afq.ptr(wset.Phi), afq.ptr(wset.Phi_b),
-                                                 wset.W, afq.ptr(Ov)))
+        c = wset.cstruct()
+        afq._ck(afq.lib().afq_phmsd_log_overlap(self.h, c.Phi, c.Phi_b, c.phi_stride, wset.W,
+                                                 afq.ptr(Ov)))
 
     def Energy(self, wset, E, Ov):
-        afq._ck(afq.lib().afq_phmsd_energy(self.h, afq.ptr(wset.Phi), afq.ptr(wset.Phi_b), wset.W,
+        c = wset.cstruct()
+        afq._ck(afq.lib().afq_phmsd_energy(self.h, c.Phi, c.Phi_b, c.phi_stride, wset.W,
                                             afq.ptr(E), afq.ptr(Ov)))
 
     def vbias(self, wset, v, dt):

@@ -1,139 +1,158 @@
-"""Error analysis of the estimators' time series, as the reference's result tools do it
-(python/safiretools/results/stats.py:23-95, 126-180, 241-280).
+"""Error bars of correlated Monte Carlo series (the estimators' time series).
 
-- statistical_inefficiency: kappa = 1 + 2 sum_t rho(t) from the normalized autocorrelation
-  function, summed with Geyer's initial positive sequence (Geyer, Stat. Sci. 7, 473 (1992)):
-  rho(2j) + rho(2j + 1) pairs are added up to the first pair that is not positive; kappa is
-  floored at 1 and a series constant to roundoff gets 1.  The autocorrelation comes from a
-  zero-padded real FFT of the deviations from the mean (the mean taken in double precision,
-  the transform in single precision, as the reference does).
-- optimal_rebinsize: the bin size B minimizing the squared relative bias of the variance of
-  the mean, ((kappa^2 - 1) / (2 kappa B))^2 for an exponentially decaying autocorrelation,
-  plus the relative variance 2 / (m - 1) of an error estimate from m bins; kappa is the one of
-  the slowest component, chosen on one half of the series and estimated on the other, both
-  ways round and averaged.
-- rebinning_analysis: bin means of consecutive samples and the autocorrelation time
-  tau = (kappa - 1) / 2 with kappa = B (std of the bins / std of the samples)^2, the largest
-  over the components.
-- standard_error: std(ddof=1) / sqrt(m) of the bins, real and imaginary parts separately.
-- jackknife: value and error of a function of the means of binned series.
+The reference's result tools report an observable as the mean of its series with the standard
+error of bin means (python/safiretools/results/stats.py); this module provides the same
+quantities from the textbook methods, in its own formulation:
+
+- statistical_inefficiency: kappa = 1 + 2 sum_{t>=1} rho(t), the factor by which correlation
+  inflates the variance of the mean.  The autocovariance comes from an FFT of the centred
+  series (float64, zero padded to a power of two, so no wrap-around), and the sum is truncated
+  with Geyer's initial monotone sequence estimator (C. J. Geyer, Stat. Sci. 7, 473 (1992),
+  section 3.3): with Gamma_k = gamma(2k) + gamma(2k+1), keep Gamma_k while it is positive and
+  replace each by the smallest of those before it; kappa = 2 sum_k Gamma_k / gamma(0) - 1,
+  at least 1.  A series whose spread is at rounding level has kappa = 1.
+- optimal_rebinsize: bins of 12 integrated autocorrelation times, tau_int = (kappa - 1) / 2 of
+  the slowest component (every column, real and imaginary parts separately), so that the
+  variance of a bin mean is within a few percent of its asymptotic value; at least
+  MIN_BINS bins are kept, and an uncorrelated series (kappa ~ 1) is not rebinned.
+- rebinning_analysis: means of consecutive bins (the tail past the last whole bin dropped)
+  and the autocorrelation time implied by them, tau = (B var(bins) / var(samples) - 1) / 2,
+  the largest over the components (H. Flyvbjerg, H. G. Petersen, J. Chem. Phys. 91, 461
+  (1989)).
+- standard_error: sample standard deviation of the bins / sqrt(number of bins); the real and
+  imaginary parts of complex bins separately.
+- jackknife: value and error of a function of the means of one or more binned series from the
+  leave-one-bin-out estimates (B. Efron, The Jackknife, the Bootstrap and Other Resampling
+  Plans, SIAM 1982).
 """
 from __future__ import annotations
 
 import numpy as np
 
-_ROUNDOFF = 1e-9            # a spread up to this is roundoff: nothing to correlate
-_FFT_CHUNK = 1 << 24        # complex elements of one batch of transforms
+MIN_BINS = 8          # fewest bins an error estimate is made from when the series allows it
+BIN_TAUS = 12.0       # bin length in integrated autocorrelation times
+_COLUMN_BLOCK = 1 << 22   # float64 elements of one FFT batch
 
 
-def _fft_len(n: int) -> int:
-    from scipy.fft import next_fast_len
-    return next_fast_len(n, real=True)
+def _flat_columns(x: np.ndarray) -> np.ndarray:
+    """[samples][components] float64 view of a series; complex parts become separate columns."""
+    c = x.reshape(x.shape[0], -1)
+    if np.iscomplexobj(c):
+        c = np.concatenate([c.real, c.imag], axis=1)
+    return np.asarray(c, np.float64)
 
 
-def _geyer(columns: np.ndarray, nfft: int) -> np.ndarray:
-    """kappa of every (non-constant) column of a [samples][columns] array."""
-    from scipy.fft import irfft, rfft
-    n, ncol = columns.shape
-    dev = np.empty((ncol, n), np.float32)
-    np.subtract(columns.T, columns.mean(axis=0)[:, None], out=dev, casting="same_kind")
-    power = np.abs(rfft(dev, n=nfft, axis=1)) ** 2
-    acov = irfft(power, n=nfft, axis=1)[:, :n] / n      # divided by n: positive semidefinite
-    rho = acov / acov[:, :1]
-    pairs = rho[:, : n // 2 * 2].reshape(ncol, -1, 2).sum(axis=2)
-    bad = pairs <= 0.0
-    npos = np.where(bad.any(axis=1), bad.argmax(axis=1), pairs.shape[1])
-    keep = np.arange(pairs.shape[1])[None, :] < npos[:, None]
-    s = np.where(keep, pairs, 0.0).sum(axis=1)
-    # the pairs start at t = 0 and so count the leading 1 twice
-    return np.maximum(2.0 * s.astype(np.float64) - 1.0, 1.0)
+def _flat(series: np.ndarray) -> bool:
+    """True for the columns whose spread is rounding noise of their magnitude."""
+    spread = np.ptp(series, axis=0)
+    scale = np.maximum(np.max(np.abs(series), axis=0), 1.0)
+    return spread <= 1e-12 * scale
+
+
+def _kappa_block(cols: np.ndarray) -> np.ndarray:
+    """Statistical inefficiency of each column of a [n][k] float64 block (n >= 2)."""
+    n, k = cols.shape
+    centred = cols - cols.mean(axis=0)
+    size = 1 << int(np.ceil(np.log2(2 * n)))
+    spec = np.fft.rfft(centred, n=size, axis=0)
+    gamma = np.fft.irfft(spec.real ** 2 + spec.imag ** 2, n=size, axis=0)[:n] / n
+    npair = n // 2
+    Gam = gamma[0:2 * npair:2] + gamma[1:2 * npair:2]            # [npair][k]
+    positive = np.cumprod(Gam > 0.0, axis=0).astype(bool)       # the initial positive run
+    mono = np.minimum.accumulate(np.where(positive, Gam, np.inf), axis=0)
+    total = np.where(positive, mono, 0.0).sum(axis=0)
+    g0 = gamma[0]
+    out = np.ones(k)
+    ok = g0 > 0.0
+    out[ok] = 2.0 * total[ok] / g0[ok] - 1.0
+    return np.maximum(out, 1.0)
 
 
 def statistical_inefficiency(samples) -> float | np.ndarray:
     """kappa of a real series, or of every column of a [samples][columns] array."""
-    samples = np.asarray(samples, np.float64)
-    cols = samples[:, None] if samples.ndim == 1 else samples
-    n = cols.shape[0]
-    kappa = np.ones(cols.shape[1])
+    x = np.asarray(samples, np.float64)
+    cols = x.reshape(x.shape[0], -1) if x.ndim > 1 else x[:, None]
+    n, k = cols.shape
+    kappa = np.ones(k)
     if n >= 2:
-        moving = np.flatnonzero(np.ptp(cols, axis=0) > _ROUNDOFF)
-        nfft = _fft_len(2 * n - 1)
-        step = max(1, _FFT_CHUNK // nfft)
-        for i in range(0, moving.size, step):
-            sel = moving[i:i + step]
-            kappa[sel] = _geyer(np.take(cols, sel, axis=1), nfft)
-    return float(kappa[0]) if samples.ndim == 1 else kappa
+        live = np.flatnonzero(~_flat(cols))
+        step = max(1, _COLUMN_BLOCK // (2 * n))
+        for start in range(0, live.size, step):
+            idx = live[start:start + step]
+            kappa[idx] = _kappa_block(cols[:, idx])
+    return float(kappa[0]) if x.ndim == 1 else kappa
 
 
 def optimal_rebinsize(samples, skip: int = 0) -> int:
-    """Bin size for rebinning_analysis chosen from the series itself (1 for an uncorrelated
-    series)."""
+    """Bin size for rebinning_analysis chosen from the series itself (1 when uncorrelated)."""
     x = np.asarray(samples)[skip:]
     n = x.shape[0]
-    if n < 2:
+    if n < 2 * MIN_BINS:
         return 1
-    comp = x.reshape(n, -1)
-    if np.iscomplexobj(comp):
-        comp = np.concatenate([comp.real, comp.imag], axis=1)
-    h0, h1 = (np.atleast_1d(statistical_inefficiency(h)) for h in np.array_split(comp, 2))
-    kappa = 0.5 * float(h1[np.argmax(h0)] + h0[np.argmax(h1)])
-    sizes = np.arange(1, n // 2 + 1)
-    counts = n // sizes
-    cost = ((kappa ** 2 - 1.0) / (2.0 * kappa * sizes)) ** 2 + 2.0 / (counts - 1)
-    return int(sizes[np.argmin(cost)])
+    kappa = float(np.max(np.atleast_1d(statistical_inefficiency(_flat_columns(x)))))
+    tau_int = 0.5 * (kappa - 1.0)
+    B = int(np.ceil(BIN_TAUS * tau_int))
+    return int(min(max(B, 1), n // MIN_BINS))
 
 
 def rebinning_analysis(samples, *, rebinsize: int, skip: int = 0):
     """(bin means [m][...], autocorrelation time) of a series whose leading axis runs over
     samples; the samples past the last whole bin are dropped."""
     x = np.asarray(samples)[skip:]
-    B = int(rebinsize)
+    B = max(int(rebinsize), 1)
     m = x.shape[0] // B
     if m < 2:
-        raise ValueError(f"{x.shape[0]} samples in bins of {B} leave {m} bins, too few for an "
-                         "error estimate")
-    x = x[: m * B]
-    naive = np.std(x, axis=0, ddof=1)
-    bins = x.reshape(m, B, *x.shape[1:]).mean(axis=1)
-    binstd = np.std(bins, axis=0, ddof=1)
-    with np.errstate(divide="ignore", invalid="ignore"):
-        kappa = B * (binstd / naive) ** 2
-    kappa = np.where(naive > 0.0, kappa, 1.0)
-    return bins, max(0.5 * (float(np.max(kappa)) - 1.0), 0.0)
+        raise ValueError(f"{x.shape[0]} samples in bins of {B} make {m} bin(s); an error "
+                         "estimate needs at least 2")
+    used = x[: m * B]
+    bins = used.reshape((m, B) + x.shape[1:]).mean(axis=1)
+    v_samples = np.var(_flat_columns(used), axis=0, ddof=1)
+    v_bins = np.var(_flat_columns(bins), axis=0, ddof=1)
+    live = v_samples > 0.0
+    tau = 0.0
+    if live.any():
+        tau = float(np.max(0.5 * (B * v_bins[live] / v_samples[live] - 1.0)))
+    return bins, max(tau, 0.0)
 
 
 def standard_error(bins):
-    """Standard error of the mean over the leading axis; complex bins get re + i im errors."""
-    bins = np.asarray(bins)
-    if np.iscomplexobj(bins):
-        s = bins.real.std(axis=0, ddof=1) + 1j * bins.imag.std(axis=0, ddof=1)
-    else:
-        s = bins.std(axis=0, ddof=1)
-    return s / np.sqrt(bins.shape[0])
+    """Standard error of the mean over the leading axis; complex bins give err(re) + i err(im)."""
+    b = np.asarray(bins)
+    root_m = np.sqrt(b.shape[0])
+    if np.iscomplexobj(b):
+        return (np.std(b.real, axis=0, ddof=1) + 1j * np.std(b.imag, axis=0, ddof=1)) / root_m
+    return np.std(b, axis=0, ddof=1) / root_m
 
 
 def average(samples, skip: int = 0, rebinsize: int | None = None):
-    """(mean, standard error) of an observable's series after rebinning (the optimal bin size
-    unless one is given), the analysis behind the reference's Results.average."""
-    if rebinsize is None:
-        rebinsize = optimal_rebinsize(samples, skip)
-    bins, _ = rebinning_analysis(samples, rebinsize=rebinsize, skip=skip)
+    """(mean, standard error) of an observable's series after rebinning (optimal_rebinsize
+    unless a bin size is given)."""
+    B = optimal_rebinsize(samples, skip) if rebinsize is None else rebinsize
+    bins, _ = rebinning_analysis(samples, rebinsize=B, skip=skip)
     return bins.mean(axis=0), standard_error(bins)
 
 
 def jackknife(func, *binned):
-    """Jackknife value and error of func over the means of one or more binned series with the
-    same number of bins (leave-one-bin-out pseudovalues)."""
-    counts = {np.asarray(b).shape[0] for b in binned}
-    if len(counts) != 1:
-        raise ValueError("the binned series must agree in their number of bins")
-    m = counts.pop()
+    """Jackknife value and error of func(mean_1, mean_2, ...) over binned series that share
+    their number of bins m: theta_i = func of the means without bin i, value = m theta_all -
+    (m - 1) mean_i theta_i, error = sqrt((m - 1) / m sum_i (theta_i - mean theta)^2)."""
+    arrays = [np.asarray(b) for b in binned]
+    m = arrays[0].shape[0]
+    if any(a.shape[0] != m for a in arrays):
+        raise ValueError("the binned series must have the same number of bins")
     if m < 2:
         raise ValueError("a jackknife needs at least 2 bins")
-    binned = [np.asarray(b) for b in binned]
-    tot = [b.sum(axis=0) for b in binned]
-    loo = np.array([func(*((t - b[i]) / (m - 1) for t, b in zip(tot, binned)))
-                    for i in range(m)])
-    full = func(*(b.mean(axis=0) for b in binned))
-    pseudo = m * full - (m - 1) * loo
-    return pseudo.mean(axis=0), standard_error(pseudo)
+    means = [a.mean(axis=0) for a in arrays]
+    theta_all = np.asarray(func(*means))
+    # mean without bin i: (m mean - b_i) / (m - 1)
+    theta = np.stack([np.asarray(func(*[(m * mu - a[i]) / (m - 1) for mu, a in zip(means, arrays)]))
+                      for i in range(m)])
+    theta_bar = theta.mean(axis=0)
+    value = m * theta_all - (m - 1) * theta_bar
+    scale = np.sqrt((m - 1) / m)
+    if np.iscomplexobj(theta):
+        err = scale * (np.sqrt(((theta.real - theta_bar.real) ** 2).sum(axis=0)) +
+                       1j * np.sqrt(((theta.imag - theta_bar.imag) ** 2).sum(axis=0)))
+    else:
+        err = scale * np.sqrt(((theta - theta_bar) ** 2).sum(axis=0))
+    return value, err

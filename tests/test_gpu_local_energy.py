@@ -1,0 +1,214 @@
+"""The local-energy step (the reference's hybrid = false) beyond the CLOSED FP64 eager case, and
+the bound-trigger counters.
+
+* The captured step with local_energy equals the eager sequence bit for bit, also when an FP64
+  energy evaluation between steps makes the context's energy scratch grow after capture (the
+  step graph must be re-captured rather than replay freed memory).
+* COLLINEAR and PHMSD local-energy steps store E1_, EXX_, EJ_ (WalkerSetUpdate.hpp:180-184):
+  after the step they equal a fresh energy evaluation of the propagated walkers bit for bit.
+* Mixed precision local-energy step against the FP64 one from the same state (stated bound).
+* Bound statistics (AFQMCBasePropagator.h:195-235) against counts made on the host from the
+  same inputs.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import oracle as orc  # noqa: E402
+from safire_amd.synthetic import make_phmsd_problem, make_problem, random_walkers  # noqa: E402
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from safire_amd import afq
+    return afq, torch
+
+
+def _init(afq, torch, prop, ws, W):
+    E = afq.empty((W, 3), torch.complex128)
+    ov = afq.empty(W, torch.complex128)
+    prop.Energy(ws, E, ov)
+    ws.eloc.copy_(E.sum(1))
+    ws.ovlp.copy_(ov)
+    return afq.device_tensor(np.array([float(afq.host(E.sum(1)).real.mean())]))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_local_energy_graph_matches_eager(gpu, precision):
+    afq, torch = gpu
+    p = make_problem(24, 5, 96, seed=1001)
+    W = 32
+    prec = afq.AFQ_FP64 if precision == "fp64" else afq.AFQ_MIXED
+    say(f"case: local-energy graph vs eager, {precision}, cfg1 W=32, 9 steps, FP64 Energy between")
+    out = {}
+    for mode in ("eager", "graph"):
+        c = afq.Context(0)
+        ham = afq.HamiltonianOperations(c, p.h1, p.Likn, p.PsiT, p.nup, precision=prec)
+        prop = afq.Propagator(ham, W, dt=0.005, seed=31, local_energy=True)
+        ws = afq.WalkerSet(random_walkers(p, W, seed=2))
+        Esh = _init(afq, torch, prop, ws, W)
+        e_out = afq.empty(1, torch.float64)
+        E = afq.empty((W, 3), torch.complex128)
+        ov = afq.empty(W, torch.complex128)
+        relax = 1 - np.exp(-0.1)
+        energies = []
+        for step in range(9):
+            ortho, pc = step % 5 == 0, step % 4 == 0
+            if mode == "eager":
+                prop.Propagate(ws, Esh)
+                if ortho:
+                    prop.Orthogonalize(ws)
+                if pc:
+                    ctr = orc.philox4x32_10([0, 0, step, 1 + 1000], [31, 0])
+                    bits = (int(ctr[0]) << 32) | int(ctr[1])
+                    prop.population_control(ws, ((bits >> 11) + 0.5) * 2.0 ** -53)
+                afq.average_pseudo_energy(c, ws.weight, ws.eloc, 0.005, e_out, relax, Esh)
+            else:
+                prop.step_graph(ws, Esh, ortho=ortho, popctl=pc, relax=relax, energy_out=e_out)
+            if step in (2, 6):
+                # an FP64 evaluation between steps: with a mixed hamiltonian its exchange scratch
+                # is larger than the captured step's, so the buffer grows after capture
+                prop.Energy(ws, E, ov, afq.AFQ_FP64)
+                energies.append(afq.host(E))
+            torch.cuda.synchronize()
+        out[mode] = (afq.host(ws.Phi), afq.host(ws.weight), afq.host(ws.eloc), afq.host(Esh),
+                     *energies)
+    for a, b in zip(out["eager"], out["graph"]):
+        assert np.array_equal(a, b)
+
+
+def _components_match(afq, torch, prop, ws, W):
+    E = afq.empty((W, 3), torch.complex128)
+    ov = afq.empty(W, torch.complex128)
+    prop.Energy(ws, E, ov)
+    Eh = afq.host(E)
+    for k, name in enumerate(("E1", "EXX", "EJ")):
+        assert np.array_equal(afq.host(getattr(ws, name)), Eh[:, k]), name
+    assert np.array_equal(afq.host(ws.ovlp), afq.host(ov))
+
+
+def test_collinear_local_energy_stores_components(gpu):
+    afq, torch = gpu
+    say("case: COLLINEAR local-energy step N=24 n=5+4 M=96 W=24, walker buffer with E columns")
+    p = make_problem(24, 5, 96, seed=1001)
+    PsiT = np.concatenate([p.PsiT, p.PsiT[:4]])
+    ctx = afq.Context(0)
+    ham = afq.HamiltonianOperations(ctx, p.h1, p.Likn, PsiT, 5, 4, walker_type=afq.AFQ_COLLINEAR)
+    prop = afq.Propagator(ham, 24, dt=0.005, seed=3, local_energy=True)
+    Pa = random_walkers(p, 24, seed=4)
+    Pb = np.ascontiguousarray(random_walkers(p, 24, seed=5)[:, :, :4])
+    for ws in (afq.WalkerBuffer(Pa, Pb, pad=1), afq.WalkerSet(Pa, Pb)):
+        if isinstance(ws, afq.WalkerSet):
+            ws.keep_energy_components()
+        Esh = _init(afq, torch, prop, ws, 24)
+        for step in range(3):
+            prop.Propagate(ws, Esh)
+            _components_match(afq, torch, prop, ws, 24)
+        # population control moves the columns with the walkers
+        w = afq.host(ws.weight)
+        w[::3] = 0.0
+        ws.weight.copy_(torch.from_numpy(w))
+        prop.population_control(ws, 0.41)
+        _components_match(afq, torch, prop, ws, 24)
+
+
+def test_phmsd_local_energy_stores_components(gpu):
+    afq, torch = gpu
+    from safire_amd.phmsd import PHMSD, build_ph_struct
+    say("case: PHMSD local-energy step N=24 n=5+5 M=48 60 determinants W=16")
+    P = make_phmsd_problem(N=24, nel=5, M=48, ndet=60, ncore=2, nactive=6, max_rank=3, seed=77)
+    ph = build_ph_struct(P.occa, P.occb, P.ci)
+    ctx = afq.Context(0)
+    ham = afq.HamiltonianOperations(ctx, P.h1, P.Likn, P.PsiT, P.nact, P.nact,
+                                    walker_type=afq.AFQ_COLLINEAR)
+    trial = PHMSD(ham, ph)
+    prop = afq.Propagator(ham, 16, dt=0.005, seed=8, trial=trial, local_energy=True)
+    ws = afq.WalkerBuffer(*P.walkers(16, seed=2, noise=0.1))
+    Esh = _init(afq, torch, prop, ws, 16)
+    for step in range(3):
+        prop.Propagate(ws, Esh)
+        _components_match(afq, torch, prop, ws, 16)
+    # the stored components are the oracle's PHMSD energy of the propagated walkers
+    L1, h1 = orc.half_rotate(P.orbs, P.Likn, P.h1)
+    Er, _ = orc.PhTrial(ph, P.orbs).energy(np.concatenate([h1, h1]), np.concatenate([L1, L1]),
+                                           afq.host(ws.Phi), afq.host(ws.Phi_b))
+    E = np.stack([afq.host(ws.E1), afq.host(ws.EXX), afq.host(ws.EJ)], axis=1)
+    assert np.max(np.abs(E - Er)) / np.max(np.abs(Er)) < 1e-10
+
+
+def test_mixed_local_energy_step_error(gpu):
+    """One local-energy step in mixed precision from the FP64 state: the stored local energies
+    (FP32 exchange contraction, FP64 accumulation) within 1e-6 relative of the FP64 step's
+    (the FP32 exchange of ~n^2 N M products at u = 6e-8, plus the FP32 propagation)."""
+    afq, torch = gpu
+    p = make_problem(24, 5, 96, seed=1001)
+    W = 32
+    say("case: mixed vs FP64 local-energy step, cfg1 W=32")
+    res = {}
+    for prec in (afq.AFQ_FP64, afq.AFQ_MIXED):
+        c = afq.Context(0)
+        ham = afq.HamiltonianOperations(c, p.h1, p.Likn, p.PsiT, p.nup, precision=prec)
+        prop = afq.Propagator(ham, W, dt=0.005, local_energy=True)
+        ws = afq.WalkerSet(p.walkers(W))
+        Esh = _init(afq, torch, prop, ws, W)
+        u = afq.device_tensor(orc.field_uniforms(3, 1, 0, W, p.M))
+        prop.Propagate(ws, Esh, u=u)
+        res[prec] = afq.host(ws.eloc)
+    d = np.abs(res[afq.AFQ_MIXED] - res[afq.AFQ_FP64])
+    rel = d / np.abs(res[afq.AFQ_FP64])
+    say(f"  max |dE| {d.max():.3e} Ha, max rel {rel.max():.3e}")
+    assert rel.max() < 1e-6
+
+
+def test_bound_statistics(gpu):
+    """vbias clamp: |vbias| > bound counted per (walker, field) from the oracle's force bias of
+    the same walkers; local-energy clamp: an Eshift far below the local energies puts every
+    walker above the window; weight clamp: walkers started above max(100, 0.1 W)."""
+    afq, torch = gpu
+    p = make_problem(24, 5, 96, seed=1001)
+    W = 40
+    say("case: bound statistics cfg1 W=40, vbias_bound 0.05")
+    ctx = afq.Context(0)
+    ham = afq.HamiltonianOperations(ctx, p.h1, p.Likn, p.PsiT, p.nup)
+    prop = afq.Propagator(ham, W, dt=0.005, vbias_bound=0.05)
+    Phi = random_walkers(p, W, seed=6)
+    osys = orc.OracleSystem(p, dt=0.005, vbias_bound=0.05)
+    st, _ = osys.init_walkers(Phi)
+    w = np.ones(W, np.complex128)
+    w[[3, 7, 11]] = 1.0e6
+    st["weight"] = w
+    ws = afq.WalkerSet(Phi)
+    for k in ("weight", "phase", "eloc", "ovlp", "theta"):
+        getattr(ws, k).copy_(torch.from_numpy(st[k]))
+    assert prop.bound_stats() == {"vbias": (0, 0, 0), "eloc": (0, 0, 0), "weight": (0, 0, 0)}
+    Gr, _ = orc.mixed_density_matrix(p.PsiT, Phi)
+    vb = orc.vbias(osys.Lnak, Gr, 0.005)
+    hits = int(np.sum(np.abs(vb) > 0.05))
+    assert 0 < hits < W * p.M
+    Eshift = -1.0e4
+    u = orc.field_uniforms(1, 1, 0, W, p.M)
+    prop.Propagate(ws, afq.device_tensor(np.array([Eshift])), u=afq.device_tensor(u))
+    osys.step(st, u, Eshift)
+    got = prop.bound_stats(reset=True)
+    # the oracle's step leaves a clamped local energy exactly on the window's edge and a clamped
+    # weight exactly on the cap
+    hi = Eshift + 10.0 * np.sqrt(2.0 / 0.005)
+    e_up = int(np.sum(st["eloc"].real == hi))
+    w_up = int(np.sum(np.abs(st["weight"]) == max(100.0, 0.1 * W)))
+    say(f"  {got}, oracle: vbias hits {hits}, eloc upper {e_up}, weight upper {w_up}")
+    assert e_up > W // 2 and w_up >= 1
+    assert got["vbias"] == (W * p.M, hits, 0)
+    assert got["eloc"] == (W, e_up, 0)
+    assert got["weight"] == (W, w_up, 0)
+    assert prop.bound_stats() == {"vbias": (0, 0, 0), "eloc": (0, 0, 0), "weight": (0, 0, 0)}

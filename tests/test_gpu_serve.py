@@ -37,3 +37,29 @@ def test_alone_equals_mix():
         assert abs(a.energy - m.energy) <= 1e-12 * abs(a.energy)
     summ = serve.summarize(list(mixed.values()))
     assert summ["jobs"] == 4 and summ["aggregate_walker_steps_per_s"] > 0
+
+
+def test_largest_serving_job_alone_equals_mix():
+    """The serving workload's largest job (N=160, 40+40 electrons, M=800, 400 mixed steps, the
+    benchmark recipe's Cholesky amplitude 0.5) alone and among three other jobs of the
+    workload's sizes arriving while it runs: the same energy bit for bit."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from safire_amd import serve
+    from safire_amd.synthetic import make_problem
+    sizes = [(160, 256), (32, 512), (96, 128), (128, 64)]
+    specs = [serve.JobSpec(i, 0.05 * i, N, N // 4, 5 * N, W, 900 + i, lscale=0.5)
+             for i, (N, W) in enumerate(sizes)]
+    assert specs[0].steps == 400
+    problems = {s.id: make_problem(s.N, s.nup, s.M, s.seed, lscale=s.lscale) for s in specs}
+    say("case: serving, N=160 M=800 400 steps alone vs with 3 jobs (N=32, 96, 128)")
+    alone = serve.Server(max_active=1).run([specs[0]], problems)[0]
+    mixed = {r.id: r for r in serve.Server(max_active=4, chunk=7).run(specs, problems)}
+    m = mixed[0]
+    say(f"N=160: alone {alone.energy:.10f} +- {alone.error:.4f}  mixed {m.energy:.10f}, "
+        f"time to energy alone {alone.time_to_energy:.2f} s, in the mix {m.time_to_energy:.2f} s")
+    assert np.isfinite(alone.energy) and not alone.collapsed
+    assert alone.energy == m.energy
+    for r in mixed.values():
+        assert np.isfinite(r.energy), r
