@@ -34,14 +34,24 @@ VHS_TRAFFIC_BYTES = (615418 * 2 + 516147) * 1024
 VHS_ALGORITHMIC_BYTES = W * M * 8 + (N * (N + 1) // 2) * M * 4 + W * N * N * 8
 
 
-def flops_per_walker_step(N, n, M, K=ORDER, k_ortho=ORTHO):
-    """Algorithmic FLOP of one hybrid step of one walker (complex MAC = 8 FLOP)."""
+def flops_per_walker_step(N, n, M, K=ORDER, k_ortho=ORTHO, performed=True):
+    """FLOP of one hybrid step of one walker.  performed=True counts the products the kernels
+    issue for a real trial, real Cholesky vectors and a real one-body propagator: vbias and the
+    two P1 products are complex x real (4 FLOP per MAC), the Taylor terms complex x complex
+    (8), vHS the N(N+1)/2 distinct entries of the symmetric V.  performed=False is the count
+    with every MAC complex (8 FLOP) and vHS over all N^2 entries (4 N^2 M), the formula the
+    scope table states for the reference's operations."""
     g = 16 * n * n * N + 16 * n ** 3
-    vb = 8 * M * n * N
-    prop = 8 * (K + 2) * N * N * n
     ov = 8 * n * n * N + (8 / 3) * n ** 3
     qr = 16 * n * n * N / k_ortho
-    vhs = vhs_flops_per_walker(N, M)
+    if performed:
+        vb = 4 * M * n * N
+        prop = 8 * K * N * N * n + 4 * 2 * N * N * n
+        vhs = vhs_flops_per_walker(N, M)
+    else:
+        vb = 8 * M * n * N
+        prop = 8 * (K + 2) * N * N * n
+        vhs = 4.0 * N * N * M
     return g + vb + prop + ov + qr + vhs
 
 
@@ -277,6 +287,26 @@ def energy_error(problem, W_e=64):
         float(np.median(np.abs(res[afq.AFQ_FP64])))
 
 
+def config_energy_errors(W_e=64):
+    """Per-walker local-energy deviation of the mixed step from the FP64 step (one step from the
+    same state and field draws, energies in FP64) for each configuration's system: the CPU case
+    (N=24), the FP64 GPU case (N=108), the benchmark shape at the recipe's Cholesky amplitude 0.5
+    and at 0.1, and the smallest and largest serving jobs (amplitude 0.1)."""
+    cases = {"N=24 n=5 M=96": (24, 5, 96, 1001, 0.5),
+             "N=108 n=24 M=600": (108, 24, 600, 1002, 0.5),
+             "N=200 n=40 M=1600": (N, NUP, M, SEED, 0.5),
+             "N=200 n=40 M=1600 amplitude 0.1": (N, NUP, M, SEED, 0.1),
+             "serving job N=32 n=8 M=160": (32, 8, 160, 10_000, 0.1),
+             "serving job N=160 n=40 M=800": (160, 40, 800, 10_001, 0.1)}
+    from safire_amd.synthetic import make_problem
+    out = {}
+    for name, (n_, nup, m_, seed, amp) in cases.items():
+        emax, emed, erel, emag = energy_error(make_problem(n_, nup, m_, seed, lscale=amp), W_e)
+        out[name] = {"amplitude": amp, "max_Ha": emax, "median_Ha": emed, "max_rel": erel,
+                     "median_abs_E_Ha": emag}
+    return out
+
+
 def stage_errors(problem, W_e=128, seed=29, steps=1):
     """Per-stage error budget of the mixed-precision step against the FP64 step from the same
     walkers and the same field uniforms.  Each variant runs `steps` steps with one stage in FP32
@@ -423,7 +453,7 @@ def main():
                                        "estimators", "energy", "serve"],
                     default=None,
                     help="profiling modes: launch only the named kernel(s)")
-    ap.add_argument("--serve-jobs", type=int, default=24,
+    ap.add_argument("--serve-jobs", type=int, default=200,
                     help="jobs of the serving run (Poisson arrivals at 4/s; 200 = the full "
                          "serving workload; 0 skips it)")
     args = ap.parse_args()
@@ -531,12 +561,24 @@ def main():
         return
     emax = emed = erel = emag = None
     bh_err = None
+    cfg_err = budget = None
     if args.only is None:
         log("energy error mixed vs fp64 (one step, 64 walkers)")
         emax, emed, erel, emag = energy_error(problem)
         log("energy error mixed vs fp64 on BH (10 steps, 256 walkers)")
         bh_err = bh_mixed_error()
         log(f"bh: {bh_err}")
+        log("energy error mixed vs fp64 per configuration (one step, 64 walkers)")
+        cfg_err = config_energy_errors()
+        log(f"per configuration: {cfg_err}")
+        log("stage error budget at the benchmark shape (one step, 128 walkers)")
+        budget = {}
+        for amp in (0.5, 0.1):
+            b = stage_errors(make_problem(N, NUP, M, SEED, lscale=amp), W_e=128)
+            budget[f"amplitude_{amp}"] = {k: b[k] for k in ("floor", "vbias", "vhs", "taylor",
+                                                            "mixed")}
+            budget[f"amplitude_{amp}"]["median_abs_E_Ha"] = b["median_abs_E_Ha"]
+        log(f"budget: {budget}")
 
     phmsd = None
     if args.only is None:
@@ -572,6 +614,7 @@ def main():
 
     sec = out["mixed"]["sec"]
     fstep = flops_per_walker_step(N, NUP, M)
+    fstep_all = flops_per_walker_step(N, NUP, M, performed=False)
     vhs_flops = vhs_flops_per_walker(N, M) * W
     tv = out["mixed"]["vhs"]
     achieved = vhs_flops / tv
@@ -602,11 +645,18 @@ def main():
                      "unit": "TFLOP/s", "frac": achieved / PEAK["fp32"],
                      "traffic": VHS_TRAFFIC_BYTES,
                      "traffic_over_algorithmic": VHS_TRAFFIC_BYTES / VHS_ALGORITHMIC_BYTES},
+        # the whole step: the FLOP the kernels issue (flops_per_walker_step) per second over the
+        # MFMA peak of the contraction precision; *_all_complex: the count with every MAC
+        # complex and vHS over all N^2 entries
+        "step_flop_per_walker": fstep,
         "step_roofline_frac": (fstep * W / sec) / PEAK["fp32"],
+        "step_roofline_frac_all_complex_count": (fstep_all * W / sec) / PEAK["fp32"],
         "fp64": None if "fp64" not in out else {
             "value": W / out["fp64"]["sec"], "ms_per_step": out["fp64"]["sec"] * 1e3,
             "vhs_ms": out["fp64"]["vhs"] * 1e3, "vhs_ms_back_to_back": out["fp64"]["vhs_b2b"] * 1e3,
-            "step_roofline_frac": (fstep * W / out["fp64"]["sec"]) / PEAK["fp64"]},
+            "step_roofline_frac": (fstep * W / out["fp64"]["sec"]) / PEAK["fp64"],
+            "step_roofline_frac_all_complex_count": (fstep_all * W / out["fp64"]["sec"])
+                                                    / PEAK["fp64"]},
         "mixed_over_fp64": None if "fp64" not in out else out["fp64"]["sec"] / sec,
         # one local-energy evaluation of all walkers (measured every measure_interval steps);
         # the exchange contraction dominates: 4 n^2 N M FLOP per walker with the real trial
@@ -623,6 +673,8 @@ def main():
         "energy_error_mixed_vs_fp64_Ha": None if emax is None else
             {"max": emax, "median": emed, "max_rel": erel, "median_abs_E": emag},
         "energy_error_mixed_vs_fp64_BH": bh_err,
+        "energy_error_mixed_vs_fp64_per_config": cfg_err,
+        "mixed_precision_error_budget": budget,
         "phmsd": phmsd,
         "serve": serve_summary,
         "cpu_baseline": cpu,

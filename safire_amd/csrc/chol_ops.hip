@@ -11,7 +11,8 @@
 //                add_one_body_shifts (one_body.hpp), exp_hermitian (exp.hpp:28-60)
 //
 // The large contractions are single MFMA GEMMs over the walker batch: vbias is
-// [W x nN] . [nN x M], vHS is [W x M] . [M x N^2] with a real right operand.  When the trial is
+// [W x nN] . [nN x M], vHS is [W x M] . [M x N^2] with a real right operand, accumulated from
+// the last Cholesky vector to the first (smallest norms first).  When the trial is
 // real the half-rotated tensor is real and vbias uses the real-operand (2-MFMA) form.  The
 // exchange energy follows the reference's chunking over Cholesky vectors: one GEMM per chunk
 // builds T[(w,b)][(m,a)] and a fixed-order reduction forms sum_ab T_ab T_ba and the Coulomb
@@ -211,15 +212,15 @@ void CholeskyOps::vHS(const void* X, int W, double dt, void* V, Precision prec) 
     const int nb = ceil_div(N_, 8);
     const int npairs = nb * (nb + 1) / 2 * 64;
     if (prec == kFP64) {
-      Opnd<double, true, true> A{reinterpret_cast<const double*>(X), M_, 0};
-      gemm::OpndSymPairs<double> B{Likn_, M_, 0, N_, nb};
+      gemm::RevK<Opnd<double, true, true>> A({reinterpret_cast<const double*>(X), M_, 0}, M_);
+      gemm::RevK<gemm::OpndSymPairs<double>> B({Likn_, M_, 0, N_, nb}, M_);
       gemm_large<double>(ctx_, "vhs_sym", 1, W, npairs, M_, A, B,
                          gemm::EpiStoreSym<double, double>{reinterpret_cast<z64*>(V), NN, 0, a,
                                                            0, N_, nb});
     } else {
       AFQ_CHECK(prec_ == kMixed, kErrState, "FP32 tensors were not built (precision FP64)");
-      Opnd<float, true, true> A{reinterpret_cast<const float*>(X), M_, 0};
-      gemm::OpndSymPairs<float> B{Likn32_, M_, 0, N_, nb};
+      gemm::RevK<Opnd<float, true, true>> A({reinterpret_cast<const float*>(X), M_, 0}, M_);
+      gemm::RevK<gemm::OpndSymPairs<float>> B({Likn32_, M_, 0, N_, nb}, M_);
       gemm_large<float>(ctx_, "vhs_sym", 1, W, npairs, M_, A, B,
                         gemm::EpiStoreSym<float, float>{reinterpret_cast<c32*>(V), NN, 0,
                                                         (float)a, 0.f, N_, nb});
@@ -227,14 +228,14 @@ void CholeskyOps::vHS(const void* X, int W, double dt, void* V, Precision prec) 
     return;
   }
   if (prec == kFP64) {
-    Opnd<double, true, true> A{reinterpret_cast<const double*>(X), M_, 0};
-    Opnd<double, false, true> B{Likn_, M_, 0};
+    gemm::RevK<Opnd<double, true, true>> A({reinterpret_cast<const double*>(X), M_, 0}, M_);
+    gemm::RevK<Opnd<double, false, true>> B({Likn_, M_, 0}, M_);
     gemm_large<double>(ctx_, "vhs", 1, W, (int)NN, M_, A, B,
                        EpiStore<double, double>{reinterpret_cast<z64*>(V), NN, 0, a, 0, 0});
   } else {
     AFQ_CHECK(prec_ == kMixed, kErrState, "FP32 tensors were not built (precision FP64)");
-    Opnd<float, true, true> A{reinterpret_cast<const float*>(X), M_, 0};
-    Opnd<float, false, true> B{Likn32_, M_, 0};
+    gemm::RevK<Opnd<float, true, true>> A({reinterpret_cast<const float*>(X), M_, 0}, M_);
+    gemm::RevK<Opnd<float, false, true>> B({Likn32_, M_, 0}, M_);
     gemm_large<float>(ctx_, "vhs", 1, W, (int)NN, M_, A, B,
                       EpiStore<float, float>{reinterpret_cast<c32*>(V), NN, 0, (float)a, 0.f, 0});
   }

@@ -57,11 +57,24 @@ struct Opnd {
   static constexpr bool kcontig = KCONTIG;
   static constexpr bool conj = CONJ;
   static constexpr bool mapped = false;
+  static constexpr bool revk = false;
   const TIn* p;
   long ld;
   long bs;
   __device__ long row_offset(int) const { return 0; }
   __device__ long batch_offset(int b) const { return (long)b * bs; }
+};
+
+// An operand read with its k index reversed: product index k addresses element kr - k, so a
+// product of two such operands accumulates from the last k to the first.  Summing the smallest
+// terms first keeps the rounding of an FP32 accumulation small when the magnitudes fall with k
+// (the Cholesky vectors of vHS come ordered by decreasing norm).  Both operands of a product
+// must be reversed alike.
+template <class OP>
+struct RevK : OP {
+  static constexpr bool revk = true;
+  long kr;  // K - 1
+  RevK(const OP& op, long K) : OP(op), kr(K - 1) {}
 };
 
 // Operand shared between batches: batch b reads block (b / div) % mod, so consecutive batches
@@ -100,6 +113,7 @@ struct OpndSymPairs {
   static constexpr bool kcontig = true;
   static constexpr bool conj = false;
   static constexpr bool mapped = true;
+  static constexpr bool revk = false;
   const TIn* p;
   long ld;  // K stride of one (i, k) row
   long bs;
@@ -230,12 +244,14 @@ struct Stager {
       }
       int gr = r0 + r, gk = k0 + k;
       bool in = (gr < R) && (gk < K) && (EXACT || idx < TOT);
+      long ka = gk;
+      if constexpr (OP::revk) ka = op.kr - gk;
       long a;
       if constexpr (OP::mapped) {
         in = (roff[e] >= 0) && (gk < K) && (EXACT || idx < TOT);
-        a = op.batch_offset(b) + roff[e] + gk;
+        a = op.batch_offset(b) + roff[e] + ka;
       } else {
-        a = op.batch_offset(b) + (OP::kcontig ? (long)gr * op.ld + gk : (long)gk * op.ld + gr);
+        a = op.batch_offset(b) + (OP::kcontig ? (long)gr * op.ld + ka : ka * op.ld + gr);
       }
       if (OP::is_cplx) {
         cplx<TIn> v = in ? reinterpret_cast<const cplx<TIn>*>(op.p)[a] : cplx<TIn>{0, 0};
@@ -426,13 +442,13 @@ __global__ void __launch_bounds__(256)
   const int b = (int)(t / mn);
   const long e = t % mn;
   const cplx<T>* p = ws + (long)b * ksplit * mn + e;
-  T sr = 0, si = 0;
+  double sr = 0, si = 0;  // FP32 slices are summed in FP64
   for (int k = 0; k < ksplit; ++k) {
     cplx<T> v = p[(long)k * mn];
     sr += v.re;
     si += v.im;
   }
-  epi(b, (int)(e / N), (int)(e % N), sr, si);
+  epi(b, (int)(e / N), (int)(e % N), (T)sr, (T)si);
 }
 
 // Host launcher.  batch = number of independent products (grid = batch * tiles).

@@ -1,19 +1,20 @@
 """Mixed precision against the FP64 path at the benchmark shape (N=200, n=40+40, M=1600).
 
-The per-walker local-energy deviation after one step, with each FP32 stage alone and all of
-them together (bench.stage_errors), and the floor: the FP64 step's walker matrices rounded to
-complex64 once.  Any FP32 computation of the propagated walkers rounds them at least that much,
-so the floor bounds every FP32 propagation from below.
+The per-walker local-energy deviation after one step, with each FP32 stage alone (the force
+bias vbias, the vHS contraction, the Taylor propagation) and all of them together
+(bench.stage_errors), and the floor: the FP64 step's walker matrices rounded to complex64 once
+(u/2 per element, u = 2^-24 = 6.0e-8).  Any FP32 computation of the propagated walkers rounds
+them at least that much, so the floor bounds every FP32 propagation from below.
 
-Bounds asserted (u = 2^-24 = 6.0e-8, FP32 unit roundoff):
-* the Cholesky amplitude 0.5 of the benchmark recipe: local energies of ~5e4 Ha, the update
-  P1 expM(V) P1 Phi - Phi is ~5x the walker itself (||V||_2 ~ 3.4, ||P1 - 1||_2 ~ 2.9), and the
-  floor alone exceeds 0.1 mHa, so 0.1 mHa is out of reach for any FP32 propagation; every
-  variant stays within 1e-4 |E| (~1700 u: the FP32 error of a 200-term MFMA accumulation,
-  ~sqrt(200) u, amplified by the non-converged Taylor sum, sum_k ||V||^k / k! / ||expM|| ~ 6,
-  and by the P1 products and the energy's conditioning, ~20);
-* Cholesky amplitude 0.1 (local energies of ~700 Ha, ||V||_2 ~ 0.7): the mixed step meets
-  0.1 mHa per walker.
+Bounds asserted:
+* at the benchmark recipe's Cholesky amplitude 0.5 (local energies of ~5e4 Ha) the floor alone
+  exceeds 0.1 mHa, so that bar is out of reach for any FP32 propagation there;
+* at both amplitudes every FP32 stage, and the mixed step, stays within 100 times the floor
+  (maximum over walkers of |dE| / |E|): an FP32 contraction over K terms carries ~sqrt(K) u
+  relative error, sqrt(1600) = 40 for vHS, i.e. ~80 times the floor's u / 2 (measured: 20-60
+  times at amplitude 0.5, 5-35 times at 0.1);
+* at Cholesky amplitude 0.1 (local energies of ~600 Ha) the mixed step meets 0.1 mHa per
+  walker (measured 9.3e-5 Ha at most over 128 walkers).
 """
 import os
 import sys
@@ -54,14 +55,11 @@ def test_floor_exceeds_the_bar_at_the_benchmark_amplitude(budget):
     assert b["floor"]["max_Ha"] <= b["mixed"]["max_Ha"]
 
 
+@pytest.mark.parametrize("amp", [0.5, 0.1])
 @pytest.mark.parametrize("stage", ["vbias", "vhs", "taylor", "mixed"])
-def test_stage_errors_within_bound(budget, stage):
-    b = budget[0.5]
-    assert b[stage]["max_rel"] < 1e-4, b[stage]
-    # the all-FP32 step is not worse than its stages added up (no hidden interaction)
-    if stage == "mixed":
-        tot = sum(b[s]["max_Ha"] for s in ("vbias", "vhs", "taylor"))
-        assert b["mixed"]["max_Ha"] <= 3.0 * tot
+def test_stage_errors_within_bound(budget, stage, amp):
+    b = budget[amp]
+    assert b[stage]["max_rel"] <= 100.0 * b["floor"]["max_rel"], (b[stage], b["floor"])
 
 
 def test_mixed_meets_the_bar_at_physical_amplitude(budget):

@@ -1,7 +1,8 @@
 """Parity of the HIP walker step with the FP64 oracle, through the C ABI.
 
-FP64 bar: every array agrees with the oracle to 1e-10 relative, measured norm-wise per array
-(max |gpu - oracle| / max |oracle|), on the same seeded inputs.  Integer-exact outputs (the
+FP64 bar: every array agrees with the oracle to 1e-10 relative, measured per walker (each
+walker's max |gpu - oracle| over its own max |oracle|, floored at 1e-6 of the array's) as well as
+norm-wise per array, on the same seeded inputs.  Integer-exact outputs (the
 Philox uniforms) are compared bit for bit.  Mixed precision: per-walker local energies within
 0.1 mHa of the FP64 path from the same walker state.
 """
@@ -37,7 +38,10 @@ def gpu():
 
 
 def rel(a, b):
-    """max |a - b| / max |b| over the finite entries; NaN/inf positions must agree."""
+    """Relative error over the finite entries (NaN/inf positions must agree): the larger of
+    the norm-wise max |a - b| / max |b| and the per-walker one, max over the leading (walker)
+    axis of max |a_w - b_w| / max |b_w|, where a walker's scale is floored at 1e-6 max |b| so
+    that entries that are zero by construction do not turn rounding into a relative error."""
     a = np.asarray(a)
     b = np.asarray(b)
     assert a.shape == b.shape, (a.shape, b.shape)
@@ -46,7 +50,14 @@ def rel(a, b):
     if not fb.any():
         return 0.0
     d = np.max(np.abs(b[fb]))
-    return float(np.max(np.abs(a[fb] - b[fb])) / (d if d > 0 else 1.0))
+    d = d if d > 0 else 1.0
+    diff = np.abs(np.where(fb, a, 0) - np.where(fb, b, 0))
+    mag = np.abs(np.where(fb, b, 0))
+    if a.ndim == 0:
+        return float(diff / d)
+    dw = diff.reshape(len(diff), -1).max(axis=1)
+    sw = np.maximum(mag.reshape(len(mag), -1).max(axis=1), 1e-6 * d)
+    return float(max(dw.max() / d, np.max(dw / sw)))
 
 
 def cplx(a):

@@ -26,11 +26,18 @@ def say(*a):
 
 
 def rel(a, b):
+    """The larger of max |a - b| / max |b| and the per-walker error (leading axis), each
+    walker's scale floored at 1e-6 max |b|."""
     a, b = np.asarray(a), np.asarray(b)
     assert a.shape == b.shape, (a.shape, b.shape)
     assert np.isfinite(a).all() and np.isfinite(b).all()
     d = np.max(np.abs(b))
-    return float(np.max(np.abs(a - b)) / (d if d > 0 else 1.0))
+    d = d if d > 0 else 1.0
+    if a.ndim == 0:
+        return float(np.abs(a - b) / d)
+    dw = np.abs(a - b).reshape(len(a), -1).max(axis=1)
+    sw = np.maximum(np.abs(b).reshape(len(b), -1).max(axis=1), 1e-6 * d)
+    return float(max(dw.max() / d, np.max(dw / sw)))
 
 
 def rel_log(a, b):

@@ -40,18 +40,18 @@ def test_alone_equals_mix():
 
 
 def test_largest_serving_job_alone_equals_mix():
-    """The serving workload's largest job (N=160, 40+40 electrons, M=800, 400 mixed steps, the
-    benchmark recipe's Cholesky amplitude 0.5) alone and among three other jobs of the
-    workload's sizes arriving while it runs: the same energy bit for bit."""
+    """The serving workload's largest job (N=160, 40+40 electrons, M=800, 400 mixed steps, 256
+    walkers, Cholesky amplitude 0.1 as every serving job) alone and among three other jobs of
+    the workload's sizes arriving while it runs: the same energy bit for bit."""
     import torch
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
     from safire_amd import serve
     from safire_amd.synthetic import make_problem
     sizes = [(160, 256), (32, 512), (96, 128), (128, 64)]
-    specs = [serve.JobSpec(i, 0.05 * i, N, N // 4, 5 * N, W, 900 + i, lscale=0.5)
-             for i, (N, W) in enumerate(sizes)]
-    assert specs[0].steps == 400
+    specs = [serve.JobSpec(i, 0.05 * i, N, N // 4, 5 * N, W, 900 + i) for i, (N, W) in
+             enumerate(sizes)]
+    assert specs[0].steps == 400 and specs[0].lscale == 0.1
     problems = {s.id: make_problem(s.N, s.nup, s.M, s.seed, lscale=s.lscale) for s in specs}
     say("case: serving, N=160 M=800 400 steps alone vs with 3 jobs (N=32, 96, 128)")
     alone = serve.Server(max_active=1).run([specs[0]], problems)[0]
@@ -62,4 +62,21 @@ def test_largest_serving_job_alone_equals_mix():
     assert np.isfinite(alone.energy) and not alone.collapsed
     assert alone.energy == m.energy
     for r in mixed.values():
-        assert np.isfinite(r.energy), r
+        assert np.isfinite(r.energy) and not r.collapsed, r
+
+
+def test_collapse_at_the_benchmark_amplitude_is_reported():
+    """At the benchmark recipe's Cholesky amplitude 0.5 the N=160 job's fields rotate each
+    walker's overlap phase by O(1) per step (||V||_2 ~ 3), the phaseless constraint zeroes about
+    half of the weights every step, and the population dies between two population controls:
+    the loop reports the job as collapsed with no energy instead of a number."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from safire_amd import serve
+    from safire_amd.synthetic import make_problem
+    spec = serve.JobSpec(0, 0.0, 160, 40, 800, 256, 900, lscale=0.5)
+    say("case: serving, N=160 M=800 400 steps at amplitude 0.5")
+    r = serve.Server(max_active=1).run([spec], {0: make_problem(160, 40, 800, 900, lscale=0.5)})[0]
+    say(f"  collapsed {r.collapsed}, energy {r.energy}")
+    assert r.collapsed and not np.isfinite(r.energy)

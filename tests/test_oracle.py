@@ -315,6 +315,36 @@ def test_oracle_steps_stay_finite():
     assert np.isfinite(e)
 
 
+def test_average_pseudo_energy_closed_forms():
+    """The growth estimator log(Re[sum w e^{dt E} / sum w]) / dt (average_energy.hpp:54-63)
+    pinned by what its definition fixes:
+    * a constant real local energy is returned exactly, whatever the (complex) weights;
+    * a constant complex energy a + ib gives a + log(cos(dt b)) / dt: the real part is taken of
+      the averaged exponential, not of the energies;
+    * dt -> 0 gives the weighted mean of the local energies, and for positive weights the
+      estimator lies above that mean (convexity of exp), which a sign slip in the exponent breaks;
+    * a common complex factor on all weights cancels."""
+    rng = np.random.default_rng(41)
+    W, dt = 37, 0.005
+    w = rng.exponential(size=W) * np.exp(1j * rng.uniform(-0.3, 0.3, size=W))
+    E = -12.5
+    assert abs(orc.average_pseudo_energy(w, np.full(W, E + 0j), dt) - E) < 1e-11
+    a, b = -7.25, 3.0
+    got = orc.average_pseudo_energy(w, np.full(W, complex(a, b)), dt)
+    assert abs(got - (a + np.log(np.cos(dt * b)) / dt)) < 1e-11
+    wr = rng.exponential(size=W)
+    e = rng.normal(size=W) * 4.0 - 20.0
+    mean = float(np.sum(wr * e) / np.sum(wr))
+    var = float(np.sum(wr * (e - mean) ** 2) / np.sum(wr))
+    for h in (1e-3, 1e-5):
+        g = orc.average_pseudo_energy(wr + 0j, e + 0j, h)
+        assert g >= mean
+        assert abs(g - mean - 0.5 * h * var) < 2.0 * h * h * np.max(np.abs(e - mean)) ** 3 + 1e-9
+    ez = e + 1j * rng.normal(size=W)
+    ref = orc.average_pseudo_energy(w, ez, dt)
+    assert abs(orc.average_pseudo_energy(w * (2.5 - 1.5j), ez, dt) - ref) < 1e-12 * abs(ref)
+
+
 def test_comb_properties():
     """Systematic (comb) resampling: the population is preserved, every walker gets floor or ceil
     of W |w_i| / sum |w| copies, zero-weight walkers get none, and the copy plan fills exactly the
