@@ -65,7 +65,7 @@ typedef struct afq_phmsd afq_phmsd;
 /* Version of this ABI (major * 100 + minor).  2.00: afq_prop_params gained local_energy and
  * afq_prop_te_anchor its follow_walkers argument.  3.00: walker strides (afq_walkers.phi_stride
  * and prop_stride, the phi_stride argument of the Slater-determinant and PHMSD entry points) and
- * afq_tune_force. */
+ * afq_tune_force.  3.01: afq_mixed_density_matrix_noalloc. */
 int afq_abi_version(void);
 /* Text of the last error on this thread ("" when none). */
 const char* afq_last_error(void);
@@ -135,6 +135,18 @@ int afq_one_body_propagator(afq_ham* ham, const double* vMF, double dt, double* 
  * (determinants_impl.hpp:33-75).  compact != 0: G [W][n][N]; else full G [W][N][N]. */
 int afq_mixed_density_matrix(afq_ctx* ctx, const double* PsiT, int n, int N, const double* Phi,
                              int64_t phi_stride, int W, double* G, double* ovlp, int compact);
+/* afq_mixed_density_matrix without any allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and results; the workspace, W n n
+ * complex (compact) or W n (n + N) complex (full G), comes from a fixed per-GPU array of
+ * AFQ_NOALLOC_SCRATCH_BYTES that the library holds until the process ends; larger calls fail
+ * with AFQ_ERR_SHAPE.  No tile is timed (the tuner's cached choice or its first candidate runs:
+ * the same bits).  Every call on one GPU shares that array, so calls that overlap on one GPU
+ * (two streams or two host threads) race. */
+#define AFQ_NOALLOC_SCRATCH_BYTES (32u << 20)
+int afq_mixed_density_matrix_noalloc(afq_ctx* ctx, const double* PsiT, int n, int N,
+                                     const double* Phi, int64_t phi_stride, int W, double* G,
+                                     double* ovlp, int compact);
 /* det_ops::Log_Overlap (density_matrix.hpp:431-447): ovlp += log det(PsiT Phi). */
 int afq_log_overlap(afq_ctx* ctx, const double* PsiT, int n, int N, const double* Phi,
                     int64_t phi_stride, int W, double* ovlp);

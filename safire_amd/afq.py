@@ -73,6 +73,7 @@ _SIGS = {
     "afq_vmf": [_P, _D, _P],
     "afq_one_body_propagator": [_P, _P, _D, _P],
     "afq_mixed_density_matrix": [_P, _P, _I, _I, _P, ctypes.c_int64, _I, _P, _P, _I],
+    "afq_mixed_density_matrix_noalloc": [_P, _P, _I, _I, _P, ctypes.c_int64, _I, _P, _P, _I],
     "afq_log_overlap": [_P, _P, _I, _I, _P, ctypes.c_int64, _I, _P],
     "afq_inverse_logdet": [_P, _P, _I, _I, _P, _P],
     "afq_propagate": [_P, _P, _P, _P, ctypes.c_int64, _I, _I, _I, _I, ctypes.c_char, _I],
@@ -120,7 +121,8 @@ _SIGS = {
 _lib = None
 
 
-ABI_VERSION = 300   # afq_abi_version() of the library these bindings describe
+ABI_VERSION = 301   # afq_abi_version() of the library these bindings describe
+NOALLOC_SCRATCH_BYTES = 32 << 20   # AFQ_NOALLOC_SCRATCH_BYTES
 
 
 def lib():
@@ -277,6 +279,16 @@ class det_ops:  # noqa: N801  (namespace mirroring sfqmc::afqmc::det_ops)
         pp, bs = _blocks(Phi)
         _ck(lib().afq_mixed_density_matrix(ctx.h, ptr(PsiT), n, N, pp, bs, W, ptr(G), ptr(ovlp),
                                            int(compact)))
+
+    @staticmethod
+    def MixedDensityMatrixNoAlloc(ctx, PsiT, Phi, G, ovlp, compact=True):
+        """MixedDensityMatrix through the allocation-free entry point (fixed per-GPU workspace
+        of NOALLOC_SCRATCH_BYTES; larger calls raise)."""
+        n, N = PsiT.shape
+        W = Phi.shape[0]
+        pp, bs = _blocks(Phi)
+        _ck(lib().afq_mixed_density_matrix_noalloc(ctx.h, ptr(PsiT), n, N, pp, bs, W, ptr(G),
+                                                   ptr(ovlp), int(compact)))
 
     @staticmethod
     def Log_Overlap(ctx, PsiT, Phi, ovlp):

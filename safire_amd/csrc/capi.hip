@@ -1,5 +1,6 @@
 // C ABI (include/afq.h) over the device classes.  Exceptions never cross the boundary: each
 // entry point converts afq::Error into its status code and keeps the message per thread.
+#include <mutex>
 #include <string>
 
 #include "../../include/afq.h"
@@ -25,7 +26,41 @@ struct afq_prop {
   afq::Propagator* p;
 };
 
+// Workspace of the allocation-free entry points.  Each GPU holds its own copy until the process
+// ends; host code never uses this name as an address (it is a host-side stand-in), only the
+// device address looked up per device when a context is created.
+static __device__ __align__(256) unsigned char scratch[AFQ_NOALLOC_SCRATCH_BYTES];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Looks up this device's copy of `scratch` once; the device must be current.
+void register_scratch(int device) {
+  AFQ_CHECK(device >= 0 && device < kMaxDevices, afq::kErrArg, "device index out of range");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void* p = nullptr;
+  AFQ_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  AFQ_CHECK(p != nullptr, afq::kErrHip, "no device address for the allocation-free workspace");
+  g_scratch_ptr[device] = p;
+}
+
+void* device_scratch(int device) {
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  void* p = (device >= 0 && device < kMaxDevices) ? g_scratch_ptr[device] : nullptr;
+  AFQ_CHECK(p != nullptr, afq::kErrState, "allocation-free workspace not set up on this device");
+  return p;
+}
+
+// Sets the context's allocation-free mode for one call.
+struct NoAllocScope {
+  afq::Context& c;
+  explicit NoAllocScope(afq::Context& ctx) : c(ctx) { c.set_no_alloc(true); }
+  ~NoAllocScope() { c.set_no_alloc(false); }
+};
 
 thread_local std::string g_err;
 
@@ -83,7 +118,7 @@ void check_walkers(const afq_prop* p, const afq_walkers* w) {
 
 extern "C" {
 
-int afq_abi_version(void) { return 300; }
+int afq_abi_version(void) { return 301; }
 
 const char* afq_last_error(void) { return g_err.c_str(); }
 
@@ -93,7 +128,14 @@ int afq_ctx_create(int device, void* stream, afq_ctx** out) {
     int n = 0;
     AFQ_HIP(hipGetDeviceCount(&n));
     AFQ_CHECK(device >= 0 && device < n, afq::kErrArg, "no such device");
-    *out = new afq_ctx(device, reinterpret_cast<hipStream_t>(stream));
+    auto* c = new afq_ctx(device, reinterpret_cast<hipStream_t>(stream));  // makes device current
+    try {
+      register_scratch(device);
+    } catch (...) {
+      delete c;
+      throw;
+    }
+    *out = c;
   });
 }
 
@@ -221,6 +263,27 @@ int afq_mixed_density_matrix(afq_ctx* ctx, const double* PsiT, int n, int N, con
     long gbs = compact ? (long)n * N : (long)N * N;
     afq::det_ops::mixed_density_matrix(ctx->c, Z(PsiT), n, N, Z(Phi), block_stride(phi_stride, N, n),
                                        W, Z(G), gbs, Z(ovlp), compact != 0);
+  });
+}
+
+int afq_mixed_density_matrix_noalloc(afq_ctx* ctx, const double* PsiT, int n, int N,
+                                     const double* Phi, int64_t phi_stride, int W, double* G,
+                                     double* ovlp, int compact) {
+  return guard([&] {
+    REQUIRE(ctx && PsiT && Phi && G && ovlp, "NULL argument");
+    REQUIRE(n >= 0 && n <= 64 && N >= n && W >= 0, "invalid sizes");
+    const size_t nT = (size_t)W * n * n, nGc = compact ? 0 : (size_t)W * n * N;
+    const size_t need = (nT + nGc) * sizeof(z64);
+    AFQ_CHECK(need <= AFQ_NOALLOC_SCRATCH_BYTES, afq::kErrShape,
+              "allocation-free density matrix needs " + std::to_string(need) +
+                  " bytes of workspace, above its limit of " +
+                  std::to_string((size_t)AFQ_NOALLOC_SCRATCH_BYTES));
+    const long bs = block_stride(phi_stride, N, n);
+    z64* ws = static_cast<z64*>(device_scratch(ctx->c.device()));
+    long gbs = compact ? (long)n * N : (long)N * N;
+    NoAllocScope na(ctx->c);
+    afq::det_ops::mixed_density_matrix(ctx->c, Z(PsiT), n, N, Z(Phi), bs, W, Z(G), gbs, Z(ovlp),
+                                       compact != 0, 1.0, nullptr, ws, compact ? nullptr : ws + nT);
   });
 }
 

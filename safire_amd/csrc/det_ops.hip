@@ -43,8 +43,11 @@ bool Context::capturing() const {
 }
 
 void* Context::scratch(const std::string& name, size_t bytes) {
+  auto it = bufs_.find(name);
+  if (it != bufs_.end() && it->second.bytes >= bytes && it->second.p) return it->second.p;
+  AFQ_CHECK(!no_alloc_, kErrState,
+            "scratch buffer '" + name + "' requested by an allocation-free call");
   Buf& b = bufs_[name];
-  if (b.bytes >= bytes && b.p) return b.p;
   AFQ_CHECK(!capturing(), kErrState,
             "scratch buffer '" + name + "' would grow during graph capture");
   if (b.p) {
@@ -90,8 +93,10 @@ void log_overlap(Context& c, const z64* PsiT, int n, int N, const z64* Phi, long
 
 void mixed_density_matrix(Context& c, const z64* PsiT, int n, int N, const z64* Phi, long phi_bs,
                           int W, z64* G, long g_bs, z64* ovlp, bool compact, double scale,
-                          c32* G32) {
+                          c32* G32, z64* ws_T, z64* ws_Gc) {
   AFQ_CHECK(!G32 || compact, kErrArg, "the complex64 copy is for the compact density matrix");
+  AFQ_CHECK(ws_T ? (compact || ws_Gc) : !ws_Gc, kErrArg,
+            "workspace: T, and Gc for the full density matrix, or neither");
   if (W == 0) return;
   hipStream_t s = c.stream();
   if (n == 0) {
@@ -99,7 +104,7 @@ void mixed_density_matrix(Context& c, const z64* PsiT, int n, int N, const z64* 
     if (!compact) fill_z64(s, (long)W * g_bs, {0, 0}, G);
     return;
   }
-  z64* T = scratch_as<z64>(c, "mdm_T", (size_t)W * n * n);
+  z64* T = ws_T ? ws_T : scratch_as<z64>(c, "mdm_T", (size_t)W * n * n);
   {
     Opnd<double, true, true> A{&PsiT->re, N, 0};
     Opnd<double, true, false> B{&Phi->re, n, phi_bs};
@@ -107,7 +112,7 @@ void mixed_density_matrix(Context& c, const z64* PsiT, int n, int N, const z64* 
   }
   lu_inverse_logdet(s, T, (long)n * n, W, n, T, (long)n * n, ovlp, scale, {0, 0}, true);
   // compact G[a][k] = sum_b Tinv[b][a] Phi[k][b]    (product<'T','T'>(TNN, B, C))
-  z64* Gc = compact ? G : scratch_as<z64>(c, "mdm_Gc", (size_t)W * n * N);
+  z64* Gc = compact ? G : ws_Gc ? ws_Gc : scratch_as<z64>(c, "mdm_Gc", (size_t)W * n * N);
   long gc_bs = compact ? g_bs : (long)n * N;
   {
     Opnd<double, true, false> A{&T->re, n, (long)n * n};

@@ -34,11 +34,16 @@ class Context {
   // Incremented whenever a scratch buffer is freed to grow it: a HIP graph captured under an
   // older generation may hold the freed address and must be re-captured.
   uint64_t generation() const { return gen_; }
+  // While set, scratch() refuses to allocate and the tuner times nothing (it takes the cached
+  // choice or the first candidate, which give the same bits): the allocation-free entry points.
+  bool no_alloc() const { return no_alloc_; }
+  void set_no_alloc(bool on) { no_alloc_ = on; }
 
  private:
   int device_;
   hipStream_t stream_;
   bool own_stream_;
+  bool no_alloc_ = false;
   struct Buf {
     void* p = nullptr;
     size_t bytes = 0;
@@ -62,10 +67,11 @@ void log_overlap(Context& c, const z64* PsiT, int n, int N, const z64* Phi, long
                  z64* ovlp, double scale = 1.0, z64 add = {0, 0});
 // MixedDensityMatrix: compact G [W][n][N] or full G [W][N][N]; ovlp accumulated as in
 // log_overlap.  G32 (compact only): G is computed in FP64 and stored rounded to complex64 at
-// G32 (same layout) instead of G.
+// G32 (same layout) instead of G.  ws_T [W][n][n] and, for the full G, ws_Gc [W][n][N]: the
+// caller's workspace in place of the context's scratch buffers (both or neither).
 void mixed_density_matrix(Context& c, const z64* PsiT, int n, int N, const z64* Phi, long phi_bs,
                           int W, z64* G, long g_bs, z64* ovlp, bool compact, double scale = 1.0,
-                          c32* G32 = nullptr);
+                          c32* G32 = nullptr, z64* ws_T = nullptr, z64* ws_Gc = nullptr);
 // inverse_logdet (density_matrix.hpp:82-111)
 void inverse_logdet(Context& c, const z64* A, int nb, int n, z64* ovlp, z64* Ainv);
 // Full mixed density matrices with a left state per walker (the 3-d MixedDensityMatrix of

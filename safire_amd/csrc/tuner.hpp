@@ -1,7 +1,8 @@
 // Tile-shape autotuner for the MFMA GEMM engine.
 //
 // Each call site offers a few tile configurations of the same product.  The first time a
-// (site, precision, shape) key is seen outside graph capture, every candidate is run once and
+// (site, precision, shape) key is seen outside graph capture and outside an allocation-free
+// call (which take the cached choice, else the first candidate), every candidate is run once and
 // then timed twice with HIP events on the context's stream; the fastest is cached for the
 // process and used from then on (also inside captured graphs).  All candidates accumulate over
 // k in the same order, so the choice never changes the result bits.  Tuning needs an
@@ -119,7 +120,7 @@ void tuned(Context& c, const std::string& key, int ncand, F&& launch, bool idemp
     launch(choice);
     return;
   }
-  if (ncand <= 1 || !idempotent || !t.enabled() || c.capturing()) {
+  if (ncand <= 1 || !idempotent || !t.enabled() || c.no_alloc() || c.capturing()) {
     launch(0);
     return;
   }
@@ -165,7 +166,7 @@ void tuned_on_copy(Context& c, const std::string& key, int ncand, z64* buf, size
     run(choice, buf);
     return;
   }
-  if (ncand <= 1 || !t.enabled() || c.capturing()) {
+  if (ncand <= 1 || !t.enabled() || c.no_alloc() || c.capturing()) {
     run(0, buf);
     return;
   }
