@@ -59,13 +59,15 @@ typedef struct afq_ctx afq_ctx;
 typedef struct afq_ham afq_ham;
 typedef struct afq_prop afq_prop;
 typedef struct afq_phmsd afq_phmsd;
+typedef struct afq_nomsd afq_nomsd;
 
 /* ---- context ------------------------------------------------------------------------------ */
 
 /* Version of this ABI (major * 100 + minor).  2.00: afq_prop_params gained local_energy and
  * afq_prop_te_anchor its follow_walkers argument.  3.00: walker strides (afq_walkers.phi_stride
  * and prop_stride, the phi_stride argument of the Slater-determinant and PHMSD entry points) and
- * afq_tune_force.  3.01: afq_mixed_density_matrix_noalloc. */
+ * afq_tune_force.  3.01: afq_mixed_density_matrix_noalloc.  3.02: multi-determinant NOMSD
+ * (afq_nomsd_*, afq_prop_create_nomsd) and afq_vbias_full. */
 int afq_abi_version(void);
 /* Text of the last error on this thread ("" when none). */
 const char* afq_last_error(void);
@@ -240,6 +242,10 @@ int afq_prop_create(afq_ham* ham, int W, const afq_prop_params* p, afq_prop** ou
 /* The same propagator over a PHMSD trial (Wavefunction<PHMSD>, COLLINEAR): vMF from the trial
  * density, G for vbias, overlaps and energies from the multi-determinant expansion. */
 int afq_prop_create_phmsd(afq_phmsd* trial, int W, const afq_prop_params* p, afq_prop** out);
+/* As afq_prop_create, with a multi-determinant NOMSD trial (ndet > 1): force bias and energy
+ * through the determinant sum, vMF from the trial's full density matrix, and the back-propagated
+ * references over every determinant (NOMSD.cpp:39-65, 159-217, 437-521). */
+int afq_prop_create_nomsd(afq_nomsd* trial, int W, const afq_prop_params* p, afq_prop** out);
 int afq_prop_destroy(afq_prop* prop);
 /* Device views: vMF complex [M], P1 complex [N][N]. */
 int afq_prop_views(afq_prop* prop, const double** vMF, const double** P1);
@@ -415,6 +421,42 @@ int afq_phmsd_energy(afq_phmsd* trial, const double* Phi_a, const double* Phi_b,
                      int64_t phi_stride, int W, double* E, double* ovlp);
 /* PHMSD::vMF (PHMSD.cpp:1329-1573) + zero_imag (AFQMCBasePropagator.cpp:105-108): vMF [M]. */
 int afq_phmsd_vmf(afq_phmsd* trial, double dt, double* vMF);
+
+/* ---- multi-determinant NOMSD (NOMSD.hpp:54-100 with ndet > 1) ----
+ * A non-orthogonal expansion sum_d c_d |D_d>: PsiT [ndet][nel][N] complex (device; determinant 0
+ * must equal the PsiT `ham` was created with, whose half-rotated tensors it shares), ci [ndet]
+ * complex (HOST).  Each further determinant gets its own half-rotated Lnak and haj.  CLOSED or
+ * COLLINEAR (Phi_b NULL for CLOSED).  Full density matrices G are [W][nspin][N][N]. */
+int afq_nomsd_create(afq_ham* ham, int ndet, const double* PsiT, const double* ci,
+                     afq_nomsd** out);
+int afq_nomsd_destroy(afq_nomsd* trial);
+/* NOMSD::Log_Overlap (NOMSD.cpp:280-333): ovlp [W] = log sum_d conj(c_d) <D_d|Phi>. */
+int afq_nomsd_log_overlap(afq_nomsd* trial, const double* Phi_a, const double* Phi_b,
+                          int64_t phi_stride, int W, double* ovlp);
+/* NOMSD::MixedDensityMatrix, full (NOMSD.cpp:219-278): G = sum_d w_d G_d / sum_d w_d with
+ * w_d = conj(c_d) exp(ov_d - log_m[w]); log_m [W] is the scale (the walkers' stored log
+ * overlap); ovlp [W] = sum_d w_d + log_m, the value the reference leaves in Ov. */
+int afq_nomsd_mixed_density_matrix(afq_nomsd* trial, const double* Phi_a, const double* Phi_b,
+                                   int64_t phi_stride, int W, const double* log_m, double* G,
+                                   double* ovlp);
+/* NOMSD::vbias with ndet > 1 (NOMSD.cpp:39-65): the full mixed density matrix, then
+ * afq_vbias_full; v [W][M]. */
+int afq_nomsd_vbias(afq_nomsd* trial, const double* Phi_a, const double* Phi_b,
+                    int64_t phi_stride, int W, const double* log_m, double dt, int precision,
+                    double* v);
+/* NOMSD::Energy with ndet > 1 (NOMSD.cpp:159-217): per-determinant local energies from each
+ * determinant's half-rotated tensors, combined with weights conj(c_d) <D_d|Phi> / <Psi_T|Phi>;
+ * E [W][3] = (E1 + E0, EXX, EJ), ovlp [W] the log overlap. */
+int afq_nomsd_energy(afq_nomsd* trial, const double* Phi_a, const double* Phi_b,
+                     int64_t phi_stride, int W, int precision, double* E, double* ovlp);
+/* NOMSD::vMF with ndet > 1 (NOMSD.cpp:437-521) + zero_imag (AFQMCBasePropagator.cpp:105-108):
+ * vMF [M] from the trial's own full density matrix. */
+int afq_nomsd_vmf(afq_nomsd* trial, double dt, double* vMF);
+/* Real3IndexFactorization::vbias of full density matrices (the ndet > 1 branch,
+ * Real3IndexFactorization.hpp:395-423): G [W][nspin][N][N],
+ * v[w][m] = sqrt(dt) c sum_s sum_ik G[w][s][i][k] Likn[i][k][m] (c = 2 CLOSED / 1 COLLINEAR);
+ * AFQ_MIXED contracts in FP32. */
+int afq_vbias_full(afq_ham* ham, const double* G, int W, double dt, int precision, double* v);
 
 #ifdef __cplusplus
 }

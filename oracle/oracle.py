@@ -554,3 +554,137 @@ def msd_brute(orbs, occa, occb, ci, haj, Lnak, Phi_a, Phi_b, E0=0.0):
     lib().orc_msd_brute(N, nup, ndn, nact, nact, _p(orbs), _p(orbs), len(ci), _p(occa), _p(occb),
                         _p(ci), _p(haj), _p(Lnak), M, E0, _p(Pa), _p(Pb), W, _p(G), _p(E), _p(ov))
     return G, E, ov
+
+
+# --- multi-determinant NOMSD trial (NOMSD.cpp:39-61, 159-333, 338-520) ------------------------
+# A trial of ndet determinants, each given per spin block in PsiT form: dets[d] = (Pa [nup][N],
+# Pb [ndown][N] or None for CLOSED).  Everything below combines the single-determinant oracle
+# functions above, in the order the reference combines its determinants.
+
+def _log_conj_c(c):
+    """log(conj(c)), with |c| < 1e-12 replaced by 1e-12 (NOMSD.cpp:316)."""
+    return np.log(1e-12 + 0j) if abs(c) < 1e-12 else np.log(np.conj(c))
+
+
+def msd_det_log_overlaps(dets, ci, Phi_a, Phi_b=None):
+    """[ndet][W] log <D_d|Phi> + log conj(c_d): CLOSED doubles the alpha block (NOMSD.cpp:305-322)."""
+    W = Phi_a.shape[0]
+    out = np.zeros((len(dets), W), np.complex128)
+    for d, (Pa, Pb) in enumerate(dets):
+        ov = log_overlap(Pa, Phi_a)
+        ov = 2.0 * ov if Pb is None else ov + log_overlap(Pb, Phi_b)
+        out[d] = ov + _log_conj_c(ci[d])
+    return out
+
+
+def msd_log_overlap(dets, ci, Phi_a, Phi_b=None):
+    """NOMSD::Log_Overlap with ndet > 1: log sum_d exp(l_d) about the largest real part
+    (NOMSD.cpp:324-331)."""
+    lo = msd_det_log_overlaps(dets, ci, Phi_a, Phi_b)
+    lm = lo[np.argmax(lo.real, axis=0), np.arange(lo.shape[1])]
+    return lm + np.log(np.sum(np.exp(lo - lm), axis=0))
+
+
+def msd_mixed_density_matrix(dets, ci, Phi_a, Phi_b, log_m):
+    """NOMSD::MixedDensityMatrix with ndet > 1 (NOMSD.cpp:240-277): full G [W][nspin*N*N] =
+    sum_d w_d G_d / sum_d w_d with w_d = conj(c_d) exp(ov_d - log_m), ov_d the determinant's
+    log overlap (alpha doubled for CLOSED), log_m the walkers' OVLP; Ov = sum_d w_d + log_m, as
+    the reference leaves it."""
+    W, N, _ = Phi_a.shape
+    nspin = 1 if dets[0][1] is None else 2
+    G = np.zeros((W, nspin * N * N), np.complex128)
+    Ov = np.zeros(W, np.complex128)
+    for d, (Pa, Pb) in enumerate(dets):
+        Ga, ov = mixed_density_matrix(Pa, Phi_a, compact=False)
+        blocks = [Ga.reshape(W, N * N)]
+        if Pb is None:
+            ov = 2.0 * ov
+        else:
+            Gb, ovb = mixed_density_matrix(Pb, Phi_b, compact=False)
+            ov = ov + ovb
+            blocks.append(Gb.reshape(W, N * N))
+        wd = np.conj(ci[d]) * np.exp(ov - log_m)
+        Ov += wd
+        G += wd[:, None] * np.concatenate(blocks, axis=1)
+    G /= Ov[:, None]
+    return G, Ov + log_m
+
+
+def vbias_full(Likn, G, dt, c):
+    """Real3IndexFactorization::vbias for a full G (ndet > 1, :395-423): v[w][m] =
+    sqrt(dt) c sum_s sum_ik G[w][s][i][k] L[i][k][m]."""
+    Likn = _c(Likn, np.float64)
+    N, _, M = Likn.shape
+    G = _c(G, np.complex128)
+    W = G.shape[0]
+    nspin = G.shape[1] // (N * N)
+    Lm = np.ascontiguousarray(np.transpose(Likn, (2, 0, 1)).reshape(M, 1, N * N))
+    Lm = np.ascontiguousarray(np.repeat(Lm, nspin, axis=1).reshape(M, nspin * N, N))
+    return vbias(Lm.astype(np.complex128), G.reshape(W, nspin * N, N), dt, c)
+
+
+def msd_half_rotated(dets, Likn, h1):
+    """Per-determinant half-rotated tensors (RealDenseHamiltonian.cpp:141-163): for determinant
+    d, Lnak_d (the alpha block [M][nup][N], then for COLLINEAR the beta block [M][ndown][N],
+    the layout energy() takes) and haj_d [nel][N] (alpha rows then beta)."""
+    out = []
+    for Pa, Pb in dets:
+        La, ha = half_rotate(Pa, Likn, h1)
+        if Pb is None:
+            out.append((La, ha))
+        else:
+            Lb, hb = half_rotate(Pb, Likn, h1)
+            out.append((np.concatenate([La.ravel(), Lb.ravel()]), np.concatenate([ha, hb])))
+    return out
+
+
+def msd_energy(dets, ci, rot, Phi_a, Phi_b=None, E0=0.0):
+    """NOMSD::Energy with ndet > 1 (NOMSD.cpp:185-216): each determinant's local energy from
+    its compact G and its half-rotated tensors, combined with w_d = conj(c_d) exp(ov_d - max_d
+    Re ov_d); E [W][3], Ov = log sum_d conj(c_d) exp(ov_d - log_m) + log_m."""
+    W = Phi_a.shape[0]
+    closed = dets[0][1] is None
+    nup = dets[0][0].shape[0]
+    ndn = 0 if closed else dets[0][1].shape[0]
+    ovs, Es = [], []
+    for (Pa, Pb), (Lnak, haj) in zip(dets, rot):
+        Ga, ov = mixed_density_matrix(Pa, Phi_a, compact=True)
+        if closed:
+            ov, G = 2.0 * ov, Ga
+        else:
+            Gb, ovb = mixed_density_matrix(Pb, Phi_b, compact=True)
+            ov, G = ov + ovb, np.concatenate([Ga, Gb], axis=1)
+        ovs.append(ov)
+        Es.append(energy(haj, Lnak, G, nup, ndn, closed=closed, E0=E0))
+    ovs = np.array(ovs)
+    lm = ovs[np.argmax(ovs.real, axis=0), np.arange(W)]
+    wd = np.conj(np.asarray(ci))[:, None] * np.exp(ovs - lm)
+    E = np.einsum("dw,dwk->wk", wd, np.array(Es)) / wd.sum(0)[:, None]
+    return E, np.log(wd.sum(0)) + lm
+
+
+def msd_vmf(dets, ci, Likn, dt):
+    """NOMSD::vMF with ndet > 1 (NOMSD.cpp:438-516) and zero_imag: the trial density
+    sum_{p <= q} Re(conj(c_q) c_p e^{ov_qp}) G_qp / sum of the same weights, ov_qp and G_qp the
+    overlap (alpha block only doubled nowhere) and full density matrix of determinant q against
+    determinant p as the walker; v = vbias of that full G, imaginary part dropped."""
+    N = dets[0][0].shape[1]
+    closed = dets[0][1] is None
+    nspin = 1 if closed else 2
+    Gs = np.zeros((1, nspin * N * N), np.complex128)
+    Os = 0.0
+    for p in range(len(dets)):
+        for q in range(p, len(dets)):
+            Ga, ov = mixed_density_matrix(dets[q][0], np.conj(dets[p][0].T)[None], compact=False)
+            blocks = [Ga.reshape(1, N * N)]
+            if not closed:
+                Gb, ovb = mixed_density_matrix(dets[q][1], np.conj(dets[p][1].T)[None],
+                                               compact=False)
+                ov = ov + ovb
+                blocks.append(Gb.reshape(1, N * N))
+            scl = float((np.conj(ci[q]) * ci[p] * np.exp(ov[0])).real)
+            Os += scl
+            Gs += scl * np.concatenate(blocks, axis=1)
+    Gs /= Os
+    v = vbias_full(Likn, Gs, dt, 2.0 if closed else 1.0)[0]
+    return v.real + 0j

@@ -116,12 +116,21 @@ _SIGS = {
     "afq_phmsd_log_overlap": [_P, _P, _P, ctypes.c_int64, _I, _P],
     "afq_phmsd_energy": [_P, _P, _P, ctypes.c_int64, _I, _P, _P],
     "afq_phmsd_vmf": [_P, _D, _P],
+    "afq_prop_create_nomsd": [_P, _I, ctypes.POINTER(PropParams), ctypes.POINTER(_P)],
+    "afq_nomsd_create": [_P, _I, _P, _P, ctypes.POINTER(_P)],
+    "afq_nomsd_destroy": [_P],
+    "afq_nomsd_log_overlap": [_P, _P, _P, ctypes.c_int64, _I, _P],
+    "afq_nomsd_mixed_density_matrix": [_P, _P, _P, ctypes.c_int64, _I, _P, _P, _P],
+    "afq_nomsd_vbias": [_P, _P, _P, ctypes.c_int64, _I, _P, _D, _I, _P],
+    "afq_nomsd_energy": [_P, _P, _P, ctypes.c_int64, _I, _I, _P, _P],
+    "afq_nomsd_vmf": [_P, _D, _P],
+    "afq_vbias_full": [_P, _P, _I, _D, _I, _P],
 }
 
 _lib = None
 
 
-ABI_VERSION = 301   # afq_abi_version() of the library these bindings describe
+ABI_VERSION = 302   # afq_abi_version() of the library these bindings describe
 NOALLOC_SCRATCH_BYTES = 32 << 20   # AFQ_NOALLOC_SCRATCH_BYTES
 
 
@@ -400,6 +409,12 @@ class HamiltonianOperations:
         _ck(lib().afq_vhs(self.h, ptr(X), W, dt, self.precision if precision is None else precision,
                           ptr(V)))
 
+    def vbias_full(self, G, v, dt, precision=None):
+        """vbias of full density matrices G [W][nspin N N] (the ndet > 1 branch)."""
+        W = G.shape[0]
+        _ck(lib().afq_vbias_full(self.h, ptr(G), W, dt,
+                                 self.precision if precision is None else precision, ptr(v)))
+
     def energy(self, E, G, addH1=True, addEJ=True, addEXX=True, precision=None):
         W = G.shape[0]
         _ck(lib().afq_energy(self.h, ptr(G), W, int(addH1), int(addEJ), int(addEXX),
@@ -569,8 +584,9 @@ class WalkerBuffer:
 
 
 class Propagator:
-    """AFQMCBasePropagator over the hamiltonian's single-determinant trial, or over a PHMSD
-    trial (safire_amd.phmsd.PHMSD) built on the same hamiltonian."""
+    """AFQMCBasePropagator over the hamiltonian's single-determinant trial, or over a
+    multi-determinant trial built on the same hamiltonian: PHMSD (safire_amd.phmsd.PHMSD) or
+    NOMSD (safire_amd.nomsd.NOMSD)."""
 
     def __init__(self, ham: HamiltonianOperations, W, dt=0.005, order=6, vbias_bound=50.0,
                  seed=0, stream=1, target_pop=0.0, free_projection=False, trial=None,
@@ -588,7 +604,9 @@ class Propagator:
         else:
             if trial.ham is not ham:
                 raise AfqError("the trial was built over another hamiltonian")
-            _ck(lib().afq_prop_create_phmsd(trial.h, W, ctypes.byref(p), ctypes.byref(h)))
+            create = (lib().afq_prop_create_nomsd if getattr(trial, "kind", "") == "nomsd"
+                      else lib().afq_prop_create_phmsd)
+            _ck(create(trial.h, W, ctypes.byref(p), ctypes.byref(h)))
         self.h = h
         v, P = _P(), _P()
         _ck(lib().afq_prop_views(h, ctypes.byref(v), ctypes.byref(P)))

@@ -134,12 +134,21 @@ void exchange_reduce(hipStream_t s, int W, int n, int MC, int m0, int M, const z
 // Cholesky vectors per workgroup of exchange_blocks for n electrons
 int exchange_group(int n);
 // exchange / Coulomb partial sums of a chunk of mcount Cholesky vectors starting at m0, from
-// T [W][n][MC][n] (complex TT): exx_part[w][g0 + g] for the chunk's groups g of exchange_group(n)
-// vectors (ng_total partials per walker), Kl[w][m0 + m] += tr T^m (Kl may be null)
+// T [W][n][MC][n] (complex TT): exx_part[w][g0 + g] for the chunk's groups g of mg vectors
+// (ng_total partials per walker; both spin blocks of a walker use the same mg, the group size
+// of the larger block, so their partials line up), Kl[w][m0 + m] += tr T^m (Kl may be null)
 template <typename TT>
-void exchange_blocks(hipStream_t s, int W, int n, int MC, int m0, int mcount, int M,
+void exchange_blocks(hipStream_t s, int W, int n, int mg, int MC, int m0, int mcount, int M,
                      const cplx<TT>* T, z64* exx_part, int ng_total, int g0, z64* Kl);
 void energy_finish(hipStream_t s, int W, int M, int nchunk, const z64* exx_part, const z64* Kl,
                    double cx, double cj, int addEXX, int addEJ, z64* E);
+// Fused FP32 exchange of a real half-rotated tensor (exchange.hip): T^m = G_w L^m^T formed and
+// reduced on chip.  G [W][ns][N] complex64, L [M][ns][N] real; part [W][nparts] exchange
+// partials (nparts = exx_fused_parts(M)), Kl[w][m] += tr T^m (Kl may be null).
+bool exx_fused_supported(int ns, int N, int M);
+int exx_fused_vpw();
+int exx_fused_parts(int M);
+void exchange_fused(hipStream_t s, int W, int ns, int N, int M, const c32* G, const float* L,
+                    z64* part, int nparts, z64* Kl);
 
 }  // namespace afq

@@ -21,6 +21,10 @@ struct afq_phmsd {
   afq_ham* ham;
   afq::PhmsdTrial* t;
 };
+struct afq_nomsd {
+  afq_ham* ham;
+  afq::NomsdTrial* t;
+};
 struct afq_prop {
   afq_ham* ham;
   afq::Propagator* p;
@@ -114,11 +118,31 @@ void check_walkers(const afq_prop* p, const afq_walkers* w) {
     REQUIRE(w->Phi_b != nullptr, "COLLINEAR walkers need Phi_b");
 }
 
+void check_precision(int precision) {
+  REQUIRE(precision == AFQ_FP64 || precision == AFQ_MIXED, "precision must be AFQ_FP64 or AFQ_MIXED");
+}
+
+// the two spin blocks' batch strides, and Phi_b present when the walkers are COLLINEAR
+struct NomsdBlocks {
+  long bs_a, bs_b;
+};
+NomsdBlocks nomsd_blocks(const afq_nomsd* trial, const double* Phi_a, const double* Phi_b,
+                         int64_t phi_stride, int W) {
+  REQUIRE(trial && Phi_a, "NULL argument");
+  REQUIRE(W > 0 && phi_stride >= 0, "W must be positive, the stride non-negative");
+  const afq::CholeskyOps& h = *trial->ham->ops;
+  const bool col = h.walker_type() == afq::kCollinear;
+  if (col && h.ndown() > 0) REQUIRE(Phi_b != nullptr, "COLLINEAR walkers need Phi_b");
+  return {block_stride(phi_stride, h.N(), h.nup()),
+          col ? block_stride(phi_stride, h.N(), h.ndown()) : 0};
+}
+
+
 }  // namespace
 
 extern "C" {
 
-int afq_abi_version(void) { return 301; }
+int afq_abi_version(void) { return 302; }
 
 const char* afq_last_error(void) { return g_err.c_str(); }
 
@@ -418,6 +442,20 @@ int afq_prop_create_phmsd(afq_phmsd* trial, int W, const afq_prop_params* p, afq
   });
 }
 
+int afq_prop_create_nomsd(afq_nomsd* trial, int W, const afq_prop_params* p, afq_prop** out) {
+  return guard([&] {
+    REQUIRE(trial && p && out, "NULL argument");
+    auto* pr = new afq_prop{trial->ham, nullptr};
+    try {
+      pr->p = new afq::Propagator(*trial->ham->ops, W, to_params(p), trial->t);
+    } catch (...) {
+      delete pr;
+      throw;
+    }
+    *out = pr;
+  });
+}
+
 int afq_prop_destroy(afq_prop* prop) {
   return guard([&] {
     if (!prop) return;
@@ -695,6 +733,91 @@ int afq_phmsd_vmf(afq_phmsd* trial, double dt, double* vMF) {
   return guard([&] {
     REQUIRE(trial && vMF, "NULL argument");
     trial->t->vMF(dt, Z(vMF));
+  });
+}
+
+/* ---- multi-determinant NOMSD ---- */
+
+int afq_nomsd_create(afq_ham* ham, int ndet, const double* PsiT, const double* ci,
+                     afq_nomsd** out) {
+  return guard([&] {
+    REQUIRE(ham && PsiT && ci && out, "NULL argument");
+    REQUIRE(ndet >= 1, "ndet must be positive");
+    auto* t = new afq_nomsd{ham, nullptr};
+    try {
+      t->t = new afq::NomsdTrial(*ham->ops, ndet, Z(PsiT), Z(ci));
+    } catch (...) {
+      delete t;
+      throw;
+    }
+    *out = t;
+  });
+}
+
+int afq_nomsd_destroy(afq_nomsd* trial) {
+  return guard([&] {
+    if (!trial) return;
+    delete trial->t;
+    delete trial;
+  });
+}
+
+int afq_nomsd_log_overlap(afq_nomsd* trial, const double* Phi_a, const double* Phi_b,
+                          int64_t phi_stride, int W, double* ovlp) {
+  return guard([&] {
+    const NomsdBlocks b = nomsd_blocks(trial, Phi_a, Phi_b, phi_stride, W);
+    REQUIRE(ovlp, "NULL argument");
+    trial->t->log_overlap(Z(Phi_a), Z(Phi_b), W, Z(ovlp), b.bs_a, b.bs_b);
+  });
+}
+
+int afq_nomsd_mixed_density_matrix(afq_nomsd* trial, const double* Phi_a, const double* Phi_b,
+                                   int64_t phi_stride, int W, const double* log_m, double* G,
+                                   double* ovlp) {
+  return guard([&] {
+    const NomsdBlocks b = nomsd_blocks(trial, Phi_a, Phi_b, phi_stride, W);
+    REQUIRE(log_m && G && ovlp, "NULL argument");
+    trial->t->mixed_density_matrix(Z(Phi_a), Z(Phi_b), W, Z(log_m), Z(G), Z(ovlp), b.bs_a,
+                                   b.bs_b);
+  });
+}
+
+int afq_nomsd_vbias(afq_nomsd* trial, const double* Phi_a, const double* Phi_b,
+                    int64_t phi_stride, int W, const double* log_m, double dt, int precision,
+                    double* v) {
+  return guard([&] {
+    const NomsdBlocks b = nomsd_blocks(trial, Phi_a, Phi_b, phi_stride, W);
+    REQUIRE(log_m && v, "NULL argument");
+    check_precision(precision);
+    trial->t->vbias(Z(Phi_a), Z(Phi_b), W, Z(log_m), dt, Z(v), (afq::Precision)precision,
+                    b.bs_a, b.bs_b);
+  });
+}
+
+int afq_nomsd_energy(afq_nomsd* trial, const double* Phi_a, const double* Phi_b,
+                     int64_t phi_stride, int W, int precision, double* E, double* ovlp) {
+  return guard([&] {
+    const NomsdBlocks b = nomsd_blocks(trial, Phi_a, Phi_b, phi_stride, W);
+    REQUIRE(E && ovlp, "NULL argument");
+    check_precision(precision);
+    trial->t->energy(Z(Phi_a), Z(Phi_b), W, Z(E), Z(ovlp), (afq::Precision)precision, b.bs_a,
+                     b.bs_b);
+  });
+}
+
+int afq_nomsd_vmf(afq_nomsd* trial, double dt, double* vMF) {
+  return guard([&] {
+    REQUIRE(trial && vMF, "NULL argument");
+    trial->t->vMF(dt, Z(vMF));
+  });
+}
+
+int afq_vbias_full(afq_ham* ham, const double* G, int W, double dt, int precision, double* v) {
+  return guard([&] {
+    REQUIRE(ham && G && v, "NULL argument");
+    REQUIRE(W >= 0, "W");
+    check_precision(precision);
+    ham->ops->vbias_full(Z(G), W, dt, Z(v), (afq::Precision)precision);
   });
 }
 

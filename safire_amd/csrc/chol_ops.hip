@@ -20,6 +20,7 @@
 // polynomial on the device instead of the reference's host eigendecomposition; both agree to
 // rounding for the Hermitian H1.
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "gemm.hpp"
@@ -70,73 +71,107 @@ CholeskyOps::CholeskyOps(Context& c, int walker_type, int N, int nup, int ndown,
   hipStream_t s = c.stream();
   const int nel_ = nel();
   try {
-    PsiT_ = dalloc<z64>((size_t)nel_ * N);
     Likn_ = dalloc<double>((size_t)N * N * M);
     h1_ = dalloc<double>((size_t)N * N);
     vexx_ = dalloc<double>((size_t)N * N);
-    Lnak_ = dalloc<z64>((size_t)M * nel_ * N);
-    haj_ = dalloc<z64>((size_t)nel_ * N);
-    AFQ_HIP(hipMemcpyAsync(PsiT_, PsiT, sizeof(z64) * nel_ * N, hipMemcpyDeviceToDevice, s));
     AFQ_HIP(hipMemcpyAsync(Likn_, Likn, sizeof(double) * N * N * M, hipMemcpyDeviceToDevice, s));
     AFQ_HIP(hipMemcpyAsync(h1_, h1, sizeof(double) * N * N, hipMemcpyDeviceToDevice, s));
-
     sym_L_ = symmetric_cholesky(s, N, M, Likn_);
-    // real trial?  (NOMSD.hpp:70-86 makes the same test on the host)
-    std::vector<z64> hp((size_t)nel_ * N);
-    AFQ_HIP(hipMemcpyAsync(hp.data(), PsiT_, sizeof(z64) * hp.size(), hipMemcpyDeviceToHost, s));
-    AFQ_HIP(hipStreamSynchronize(s));
-    real_trial_ = true;
-    for (auto& v : hp)
-      if (v.im != 0.0) { real_trial_ = false; break; }
-
-    // half rotation per spin block: C[a][(k,m)] = sum_i PsiT[a][i] Likn[i][(k,m)], then
-    // permuted to Lnak[m][a][k]
-    z64* tmp = nullptr;
-    int nmax = std::max(nup_, ndown_);
-    if (nmax > 0) tmp = scratch_as<z64>(c, "setup_tmp", (size_t)nmax * N * M);
-    for (int sp = 0; sp < (walker_type_ == kCollinear ? 2 : 1); ++sp) {
-      int ns = sp == 0 ? nup_ : ndown_;
-      if (ns == 0) continue;
-      const z64* Ps = PsiT_ + (sp == 0 ? 0 : (size_t)nup_ * N);
-      z64* Ls = Lnak_ + (sp == 0 ? 0 : (size_t)M * nup_ * N);
-      Opnd<double, true, true> A{&Ps->re, N, 0};
-      Opnd<double, false, false> B{Likn_, (long)N * M, 0};
-      launch_large<double>(s, 1, ns, N * M, N, A, B,
-                           EpiStore<double, double>{tmp, (long)N * M, 0, 1, 0, 0});
-      permute_akm_mak(s, ns, N, M, tmp, Ls);
-    }
-    // haj = PsiT h1   (half_rotate_hamiltonian, one_body.hpp)
-    {
-      Opnd<double, true, true> A{&PsiT_->re, N, 0};
-      Opnd<double, false, false> B{h1_, N, 0};
-      launch_small<double>(s, 1, nel_, N, N, A, B, EpiStore<double, double>{haj_, N, 0, 1, 0, 0});
-    }
     // vexx[i][l] = -1/2 sum_{(j,m)} Likn[i][(j,m)] Likn[l][(j,m)]
     {
       Opnd<double, false, true> A{Likn_, (long)N * M, 0};
       Opnd<double, false, true> B{Likn_, (long)N * M, 0};
       launch_large<double>(s, 1, N, N, N * M, A, B, EpiStoreReal<double>{vexx_, N, 0, -0.5});
     }
-    if (real_trial_) {
-      Lnak_r_ = dalloc<double>((size_t)M * nel_ * N);
-      real_part(s, (long)M * nel_ * N, Lnak_, Lnak_r_);
-    }
     if (prec_ == kMixed) {
       Likn32_ = dalloc<float>((size_t)N * N * M);
       cast_f64_f32(s, (long)N * N * M, Likn_, Likn32_);
-      if (real_trial_) {
-        Lnak_r32_ = dalloc<float>((size_t)M * nel_ * N);
-        real_part_f32(s, (long)M * nel_ * N, Lnak_, Lnak_r32_);
-      } else {
-        Lnak32_ = dalloc<c32>((size_t)M * nel_ * N);
-        cast_z64_c32(s, (long)M * nel_ * N, Lnak_, Lnak32_);
-      }
     }
+    Rotated own;
+    half_rotate_into(PsiT, own);
+    PsiT_ = own.PsiT;
+    Lnak_ = own.Lnak;
+    haj_ = own.haj;
+    Lnak_r_ = own.Lnak_r;
+    Lnak_r32_ = own.Lnak_r32;
+    Lnak32_ = own.Lnak32;
+    real_trial_ = own.real;
     AFQ_HIP(hipStreamSynchronize(s));
   } catch (...) {
     free_all();
     throw;
   }
+}
+
+// The half-rotated tensors of one determinant (RealDenseHamiltonian.cpp:141-163): per spin
+// block C[a][(k,m)] = sum_i PsiT[a][i] Likn[i][(k,m)], permuted to Lnak[m][a][k]; haj = PsiT h1
+// (half_rotate_hamiltonian, one_body.hpp); the real and FP32 copies the contractions read.
+void CholeskyOps::half_rotate_into(const z64* PsiT, Rotated& r) const {
+  hipStream_t s = ctx_.stream();
+  const int N = N_, M = M_, nel_ = nel();
+  try {
+    r.PsiT = dalloc<z64>((size_t)nel_ * N);
+    r.Lnak = dalloc<z64>((size_t)M * nel_ * N);
+    r.haj = dalloc<z64>((size_t)nel_ * N);
+    AFQ_HIP(hipMemcpyAsync(r.PsiT, PsiT, sizeof(z64) * nel_ * N, hipMemcpyDeviceToDevice, s));
+    // real determinant?  (NOMSD.hpp:70-86 makes the same test on the host)
+    std::vector<z64> hp((size_t)nel_ * N);
+    AFQ_HIP(hipMemcpyAsync(hp.data(), r.PsiT, sizeof(z64) * hp.size(), hipMemcpyDeviceToHost, s));
+    AFQ_HIP(hipStreamSynchronize(s));
+    r.real = true;
+    for (auto& v : hp)
+      if (v.im != 0.0) { r.real = false; break; }
+    z64* tmp = nullptr;
+    const int nmax = std::max(nup_, ndown_);
+    if (nmax > 0) tmp = scratch_as<z64>(ctx_, "setup_tmp", (size_t)nmax * N * M);
+    for (int sp = 0; sp < (walker_type_ == kCollinear ? 2 : 1); ++sp) {
+      const int ns = sp == 0 ? nup_ : ndown_;
+      if (ns == 0) continue;
+      const z64* Ps = r.PsiT + (sp == 0 ? 0 : (size_t)nup_ * N);
+      z64* Ls = r.Lnak + (sp == 0 ? 0 : (size_t)M * nup_ * N);
+      Opnd<double, true, true> A{&Ps->re, N, 0};
+      Opnd<double, false, false> B{Likn_, (long)N * M, 0};
+      launch_large<double>(s, 1, ns, N * M, N, A, B,
+                           EpiStore<double, double>{tmp, (long)N * M, 0, 1, 0, 0});
+      permute_akm_mak(s, ns, N, M, tmp, Ls);
+    }
+    {
+      Opnd<double, true, true> A{&r.PsiT->re, N, 0};
+      Opnd<double, false, false> B{h1_, N, 0};
+      launch_small<double>(s, 1, nel_, N, N, A, B, EpiStore<double, double>{r.haj, N, 0, 1, 0, 0});
+    }
+    if (r.real) {
+      r.Lnak_r = dalloc<double>((size_t)M * nel_ * N);
+      real_part(s, (long)M * nel_ * N, r.Lnak, r.Lnak_r);
+    }
+    if (prec_ == kMixed) {
+      if (r.real) {
+        r.Lnak_r32 = dalloc<float>((size_t)M * nel_ * N);
+        real_part_f32(s, (long)M * nel_ * N, r.Lnak, r.Lnak_r32);
+      } else {
+        r.Lnak32 = dalloc<c32>((size_t)M * nel_ * N);
+        cast_z64_c32(s, (long)M * nel_ * N, r.Lnak, r.Lnak32);
+      }
+    }
+  } catch (...) {
+    release(r);
+    throw;
+  }
+}
+
+Rotated CholeskyOps::rotate(const z64* PsiT) const {
+  Rotated r;
+  half_rotate_into(PsiT, r);
+  AFQ_HIP(hipStreamSynchronize(ctx_.stream()));
+  return r;
+}
+
+void CholeskyOps::release(Rotated& r) const {
+  (void)hipStreamSynchronize(ctx_.stream());
+  for (void* p : {(void*)r.PsiT, (void*)r.Lnak, (void*)r.haj, (void*)r.Lnak_r, (void*)r.Lnak_r32,
+                  (void*)r.Lnak32})
+    if (p) (void)hipFree(p);
+  r = Rotated{};
 }
 
 void CholeskyOps::free_all() {
@@ -269,16 +304,57 @@ void CholeskyOps::vbias32(const c32* G, int W, double dt, z64* v) const {
   }
 }
 
+void CholeskyOps::vbias_full(const z64* G, int W, double dt, z64* v, Precision prec) const {
+  if (W == 0) return;
+  const double a = std::sqrt(dt) * (walker_type_ == kClosed ? 2.0 : 1.0);
+  const int nspin = walker_type_ == kCollinear ? 2 : 1;
+  const long NN = (long)N_ * N_;
+  // v[w][m] (+)= a sum_{(i,k)} G[w][s][(i,k)] Likn[(i,k)][m], one product per spin block
+  for (int sp = 0; sp < nspin; ++sp) {
+    Opnd<double, true, true> A{&(G + sp * NN)->re, nspin * NN, 0};
+    if (prec == kFP64) {
+      Opnd<double, false, false> B{Likn_, M_, 0};
+      gemm_large_splitk<double>(ctx_, "vbias_full", W, M_, (int)NN, A, B,
+                                EpiStore<double, double>{v, M_, 0, a, 0, sp}, sp == 0);
+    } else {
+      AFQ_CHECK(prec_ == kMixed, kErrState, "FP32 tensors were not built (precision FP64)");
+      Opnd<float, false, false> B{Likn32_, M_, 0};
+      gemm_large_splitk<float>(ctx_, "vbias_full32", W, M_, (int)NN, A, B,
+                               EpiStore<float, double>{v, M_, 0, (float)a, 0.f, sp}, sp == 0);
+    }
+  }
+}
+
 // Exchange partial sums of one spin block: part [W][ceil(M / group)] (one entry per walker and
 // group of exchange_group(n) Cholesky vectors, summed in group order by energy_finish) and, when
 // Kl is given, Kl[w][m] += tr T^m.  Gblk [W][ns][N] contiguous.  The Cholesky vectors are taken
 // in chunks whose T [W][ns][MC][ns] fits 1 GiB (max_memory_MB of energy_impl,
 // Real3IndexFactorization.hpp:511-587), sized for the larger spin block so both share the buffer.
+bool CholeskyOps::exchange_fused_path(Precision prec, bool real) const {
+  const char* e = std::getenv("AFQ_EXX_FUSED");  // "0": the GEMM + reduction path
+  return prec != kFP64 && real && exx_fused_supported(std::max(nup_, ndown_), N_, M_) &&
+         !(e && e[0] == '0');
+}
+
+int CholeskyOps::exchange_parts(Precision prec, bool real) const {
+  return exchange_fused_path(prec, real)
+             ? exx_fused_parts(M_)
+             : ceil_div(M_, exchange_group(std::max(std::max(nup_, ndown_), 1)));
+}
+
 void CholeskyOps::exchange_spin(int sp, const z64* Gblk, int W, z64* part, z64* Kl,
-                                Precision prec) const {
+                                Precision prec, const Rotated& t) const {
   hipStream_t s = ctx_.stream();
   const int ns = sp == 0 ? nup_ : ndown_;
   const int nmax = std::max(nup_, ndown_);
+  if (exchange_fused_path(prec, t.real)) {
+    AFQ_CHECK(prec_ == kMixed, kErrState, "FP32 tensors were not built (precision FP64)");
+    c32* G32 = scratch_as<c32>(ctx_, "en_G32", (size_t)W * ns * N_);
+    cast_z64_c32(s, (long)W * ns * N_, Gblk, G32);
+    const size_t loff = sp == 0 ? 0 : (size_t)M_ * nup_ * N_;
+    exchange_fused(s, W, ns, N_, M_, G32, t.Lnak_r32 + loff, part, exx_fused_parts(M_), Kl);
+    return;
+  }
   const bool t32 = prec != kFP64;
   const size_t tsz = t32 ? sizeof(c32) : sizeof(z64);
   const int mg = exchange_group(std::max(nmax, 1));
@@ -298,14 +374,14 @@ void CholeskyOps::exchange_spin(int sp, const z64* Gblk, int W, z64* part, z64* 
     if (!t32) {
       EpiStore<double, double> e{static_cast<z64*>(T), (long)MC * ns, 0, 1, 0, 0};
       Opnd<double, true, true> A{&Gblk->re, N_, 0};
-      if (real_trial_) {
-        Opnd<double, false, true> B{Lnak_r_ + lc, N_, 0};
+      if (t.real) {
+        Opnd<double, false, true> B{t.Lnak_r + lc, N_, 0};
         gemm_large<double>(ctx_, "exx_chunk_r", 1, W * ns, mc * ns, N_, A, B, e);
       } else {
-        Opnd<double, true, true> B{&(Lnak_ + lc)->re, N_, 0};
+        Opnd<double, true, true> B{&(t.Lnak + lc)->re, N_, 0};
         gemm_large<double>(ctx_, "exx_chunk", 1, W * ns, mc * ns, N_, A, B, e);
       }
-      exchange_blocks<double>(s, W, ns, MC, m0, mc, M_, static_cast<const z64*>(T), part, ng_total,
+      exchange_blocks<double>(s, W, ns, mg, MC, m0, mc, M_, static_cast<const z64*>(T), part, ng_total,
                               m0 / mg, Kl);
     } else {
       AFQ_CHECK(prec_ == kMixed, kErrState, "FP32 tensors were not built (precision FP64)");
@@ -315,23 +391,24 @@ void CholeskyOps::exchange_spin(int sp, const z64* Gblk, int W, z64* part, z64* 
       }
       EpiStore<float, float> ef{static_cast<c32*>(T), (long)MC * ns, 0, 1.f, 0.f, 0};
       Opnd<float, true, true> A{&G32->re, N_, 0};
-      if (real_trial_) {
-        Opnd<float, false, true> B{Lnak_r32_ + lc, N_, 0};
+      if (t.real) {
+        Opnd<float, false, true> B{t.Lnak_r32 + lc, N_, 0};
         gemm_large<float>(ctx_, "exx_chunk_r", 1, W * ns, mc * ns, N_, A, B, ef);
       } else {
-        Opnd<float, true, true> B{&(Lnak32_ + lc)->re, N_, 0};
+        Opnd<float, true, true> B{&(t.Lnak32 + lc)->re, N_, 0};
         gemm_large<float>(ctx_, "exx_chunk_c", 1, W * ns, mc * ns, N_, A, B, ef);
       }
-      exchange_blocks<float>(s, W, ns, MC, m0, mc, M_, static_cast<const c32*>(T), part, ng_total,
+      exchange_blocks<float>(s, W, ns, mg, MC, m0, mc, M_, static_cast<const c32*>(T), part, ng_total,
                              m0 / mg, Kl);
     }
   }
 }
 
 void CholeskyOps::energy(const z64* G, int W, z64* E, bool addH1, bool addEJ, bool addEXX,
-                         Precision prec) const {
+                         Precision prec, const Rotated* det) const {
   if (W == 0) return;
   AFQ_CHECK(!addEJ || addEXX, kErrArg, "addEJ and not addEXX not yet implemented");
+  const Rotated t = det ? *det : self();
   hipStream_t s = ctx_.stream();
   const int nel_ = nel();
   const double c = walker_type_ == kClosed ? 2.0 : 1.0;
@@ -339,13 +416,13 @@ void CholeskyOps::energy(const z64* G, int W, z64* E, bool addH1, bool addEJ, bo
   fill_z64(s, (long)W * 3, {0, 0}, E);
   if (addH1) {
     // E[w][0] = E0 + c sum G haj over both spin blocks (G rows and haj rows share the order)
-    dot_rows(s, W, (long)nel_ * N_, G, haj_, c, 0.0, E0_, E, 3);
+    dot_rows(s, W, (long)nel_ * N_, G, t.haj, c, 0.0, E0_, E, 3);
   }
   if (!addEXX) return;
   z64* Kl = scratch_as<z64>(ctx_, "en_Kl", (size_t)W * M_);
   fill_z64(s, (long)W * M_, {0, 0}, Kl);
   const int nspin = walker_type_ == kCollinear ? 2 : 1;
-  const int ng_total = ceil_div(M_, exchange_group(std::max(std::max(nup_, ndown_), 1)));
+  const int ng_total = exchange_parts(prec, t.real);
   z64* part = scratch_as<z64>(ctx_, "en_part", (size_t)W * ng_total * nspin);
   for (int sp = 0; sp < nspin; ++sp) {
     int ns = sp == 0 ? nup_ : ndown_;
@@ -362,7 +439,7 @@ void CholeskyOps::energy(const z64* G, int W, z64* E, bool addH1, bool addEJ, bo
                                hipMemcpyDeviceToDevice, s));
       Gblk = Gs;
     }
-    exchange_spin(sp, Gblk, W, part + (size_t)sp * W * ng_total, addEJ ? Kl : nullptr, prec);
+    exchange_spin(sp, Gblk, W, part + (size_t)sp * W * ng_total, addEJ ? Kl : nullptr, prec, t);
   }
   // E[w][1] = -1/2 c sum exx;  E[w][2] = 1/2 c^2 sum_m Kl^2  (energy :169-173)
   for (int sp = 0; sp < nspin; ++sp)
@@ -388,9 +465,9 @@ void CholeskyOps::energy_spin(int spin, const z64* G, int W, z64* E, z64* EJn, b
   if (addH1)
     dot_rows(s, W, (long)ns * N_, G, haj_ + (spin == 0 ? 0 : (size_t)nup_ * N_), c, 0.0, 0.0, E, 3);
   if (!addEXX) return;
-  const int ng_total = ceil_div(M_, exchange_group(std::max(std::max(nup_, ndown_), 1)));
+  const int ng_total = exchange_parts(prec, real_trial_);
   z64* part = scratch_as<z64>(ctx_, "ens_part", (size_t)W * ng_total);
-  exchange_spin(spin, G, W, part, addEJ ? EJn : nullptr, prec);
+  exchange_spin(spin, G, W, part, addEJ ? EJn : nullptr, prec, self());
   // E[w][1] = -1/2 c sum exx;  E[w][2] = 1/2 sum_m EJn^2 (:206-207)
   energy_finish(s, W, M_, ng_total, part, EJn, -0.5 * c, 0.5, 1, addEJ ? 1 : 0, E);
 }

@@ -11,6 +11,8 @@
 // of Householder: with R's diagonal positive and real (the reference rescales the Householder
 // Q by sign(Re R_ii)), Q is unique, so both produce the same Q to rounding while the
 // GEMM-shaped Gram/solve steps keep the work on the matrix cores.
+#include <cstdlib>
+
 #include "gemm.hpp"
 #include "ops.hpp"
 #include "tuner.hpp"
@@ -285,12 +287,15 @@ __device__ inline void read_bfrag(T (&br)[4], T (&bi)[4], const T* Br, const T* 
 }
 
 // One 16-wide k group of acc[s][t] += op(A)[stripe s rows][k group] * B[k group][column tile t],
-// op(A) = conj(A) when CONJ.
-template <typename T, bool CPLX, int SPW, int CB, bool CONJ = false>
+// op(A) = conj(A) when CONJ.  M3 (complex A): the three real products of Gauss's method,
+// p1 = Ar Br, p2 = Ai Bi, p3 = (Ar +- Ai)(Br + Bi), in (ar, ai, a3); stripes_times_b combines
+// them.  Otherwise four products straight into the real and imaginary parts (ar, ai).
+template <typename T, bool CPLX, int SPW, int CB, bool CONJ = false, bool M3 = false>
 __device__ inline void group_mfma(const AFrag<T, CPLX> (&f)[SPW], int g, int NS, int NW, int wave,
                                   int fr, int fk, const T* Br, const T* Bi, int LDK,
                                   typename gemm::Acc<T>::v4 (&ar)[SPW][CB],
-                                  typename gemm::Acc<T>::v4 (&ai)[SPW][CB]) {
+                                  typename gemm::Acc<T>::v4 (&ai)[SPW][CB],
+                                  typename gemm::Acc<T>::v4 (&a3)[SPW][CB]) {
   using gemm::Acc;
 #pragma unroll
   for (int t = 0; t < CB; ++t) {
@@ -301,6 +306,13 @@ __device__ inline void group_mfma(const AFrag<T, CPLX> (&f)[SPW], int g, int NS,
       if (wave + s * NW >= NS) continue;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
+        if constexpr (CPLX && M3) {
+          const T as = CONJ ? f[s].re[j] - f[s].im[j] : f[s].re[j] + f[s].im[j];
+          ar[s][t] = Acc<T>::mfma(f[s].re[j], br[j], ar[s][t]);
+          ai[s][t] = Acc<T>::mfma(f[s].im[j], bi[j], ai[s][t]);
+          a3[s][t] = Acc<T>::mfma(as, br[j] + bi[j], a3[s][t]);
+          continue;
+        }
         ar[s][t] = Acc<T>::mfma(f[s].re[j], br[j], ar[s][t]);
         ai[s][t] = Acc<T>::mfma(f[s].re[j], bi[j], ai[s][t]);
         if constexpr (CPLX && !CONJ) {
@@ -325,33 +337,52 @@ __device__ inline void load_group(AFrag<T, CPLX> (&f)[SPW], const T* __restrict_
 }
 
 // acc = A[stripe rows][0:N] * B[0:N][tiles] for the wave's stripes (complex B in LDS). The A
-// fragments of the next two k groups are in flight while a group is multiplied.
-template <typename T, bool CPLX, int SPW, int CB, bool CONJ = false>
+// fragments of the next two k groups are in flight while a group is multiplied.  M3: complex
+// products by Gauss's three-multiplication method (three MFMAs per complex block instead of four).
+template <typename T, bool CPLX, int SPW, int CB, bool CONJ = false, bool M3 = false>
 __device__ inline void stripes_times_b(const T* __restrict__ A, long lda, bool vec, int N, int NS,
                                        int NW, int wave, int fr, int fk, const T* Br,
                                        const T* Bi, int LDK,
                                        typename gemm::Acc<T>::v4 (&ar)[SPW][CB],
                                        typename gemm::Acc<T>::v4 (&ai)[SPW][CB]) {
   using V4 = typename gemm::Acc<T>::v4;
+  constexpr bool G3 = CPLX && M3;
+  V4 a3[SPW][CB];
 #pragma unroll
   for (int s = 0; s < SPW; ++s)
 #pragma unroll
     for (int t = 0; t < CB; ++t) {
       ar[s][t] = V4{0, 0, 0, 0};
       ai[s][t] = V4{0, 0, 0, 0};
+      a3[s][t] = V4{0, 0, 0, 0};
     }
   AFrag<T, CPLX> f0[SPW], f1[SPW], f2[SPW];
   load_group(f0, A, lda, vec, N, NS, NW, wave, fr, fk, 0);
   load_group(f1, A, lda, vec, N, NS, NW, wave, fr, fk, 1);
   for (int g = 0; g < NS; g += 3) {
     load_group(f2, A, lda, vec, N, NS, NW, wave, fr, fk, g + 2);
-    group_mfma<T, CPLX, SPW, CB, CONJ>(f0, g, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai);
+    group_mfma<T, CPLX, SPW, CB, CONJ, G3>(f0, g, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai, a3);
     if (g + 1 >= NS) break;
     load_group(f0, A, lda, vec, N, NS, NW, wave, fr, fk, g + 3);
-    group_mfma<T, CPLX, SPW, CB, CONJ>(f1, g + 1, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai);
+    group_mfma<T, CPLX, SPW, CB, CONJ, G3>(f1, g + 1, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai, a3);
     if (g + 2 >= NS) break;
     load_group(f1, A, lda, vec, N, NS, NW, wave, fr, fk, g + 4);
-    group_mfma<T, CPLX, SPW, CB, CONJ>(f2, g + 2, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai);
+    group_mfma<T, CPLX, SPW, CB, CONJ, G3>(f2, g + 2, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai, a3);
+  }
+  if constexpr (G3) {
+    // p1 = Ar Br, p2 = Ai Bi, p3 = (Ar +- Ai)(Br + Bi):
+    //   A B:        re = p1 - p2, im = p3 - p1 - p2
+    //   conj(A) B:  re = p1 + p2, im = p3 - p1 + p2
+#pragma unroll
+    for (int s = 0; s < SPW; ++s)
+#pragma unroll
+      for (int t = 0; t < CB; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const T p1 = ar[s][t][r], p2 = ai[s][t][r], p3 = a3[s][t][r];
+          ar[s][t][r] = CONJ ? p1 + p2 : p1 - p2;
+          ai[s][t][r] = CONJ ? (p3 - p1) + p2 : (p3 - p1) - p2;
+        }
   }
 }
 
@@ -387,7 +418,7 @@ __device__ inline void write_b(T* Br, T* Bi, int LDK, int NS, int NW, int wave, 
 
 // One workgroup per (walker, CB 16-column tiles); NW wavefronts.  CONJV: the V products use
 // conj(V), which is V^H for the complex-symmetric V of vHS (the adjoint propagator, sgn = -1).
-template <typename T, bool P1C, int SPW, int CB, bool CONJV = false>
+template <typename T, bool P1C, int SPW, int CB, bool CONJV, bool M3>
 __global__ void __launch_bounds__(1024)
     k_prop_taylor(int W, int N, int n, int order, double sgn, const T* __restrict__ P1, int vecP,
                   const T* __restrict__ V, long v_bs, int vecV, z64* __restrict__ Phi, long phi_bs,
@@ -430,8 +461,8 @@ __global__ void __launch_bounds__(1024)
   write_b<T, SPW, CB>(Br, Bi, LDK, NS, NW, wave, fr, fk, ar, ai);
   __syncthreads();
   for (int k = 1; k <= order; ++k) {
-    stripes_times_b<T, true, SPW, CB, CONJV>(Vw, N, vecV, N, NS, NW, wave, fr, fk, Br, Bi, LDK, ar,
-                                             ai);
+    stripes_times_b<T, true, SPW, CB, CONJV, M3>(Vw, N, vecV, N, NS, NW, wave, fr, fk, Br, Bi,
+                                                 LDK, ar, ai);
     // T_k = (i sgn / k) V T_{k-1}; S += T_k
     const T fi = (T)(sgn / (double)k);
 #pragma unroll
@@ -482,11 +513,11 @@ inline int max_lds_per_block() {
   return v;
 }
 
-template <typename T, bool P1C, int SPW, int CB, bool CONJV = false>
+template <typename T, bool P1C, int SPW, int CB, bool CONJV, bool M3>
 void launch_fused(hipStream_t s, const PropagateArgs& a, const FusedShape& f, const T* P1, int vecP,
                   const T* V, int vecV, z64* Phi) {
   const size_t lds = 2 * 16 * CB * (size_t)f.LDK * sizeof(T);
-  auto kern = k_prop_taylor<T, P1C, SPW, CB, CONJV>;
+  auto kern = k_prop_taylor<T, P1C, SPW, CB, CONJV, M3>;
   if (lds > 64 * 1024)
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -529,17 +560,24 @@ bool propagate_fused(Context& c, const PropagateArgs& a, const T* P1, bool p1c, 
                    (reinterpret_cast<uintptr_t>(V) % 16) == 0;
   hipStream_t s = c.stream();
   const T* Vt = reinterpret_cast<const T*>(V);
+  // The V products by Gauss's three-multiplication method (three MFMAs per complex block
+  // instead of four): always in FP64; in FP32 only when AFQ_PROP_M3=1 (the four-product form
+  // keeps the FP32 rounding of the terms closer to the FP64 path's)
+  const char* m3e = std::getenv("AFQ_PROP_M3");
+  const bool m3 = sizeof(T) == 8 || (m3e && m3e[0] == '1');
   auto run = [&](int i, z64* Phi) {
     const int cb = cbs[i];
     constexpr int SPW2 = sizeof(T) == 4 ? 2 : 1;
     if constexpr (sizeof(T) == 8) {
       if (adj) {  // FP64 takes one stripe per wave and one column tile
-        if (p1c) launch_fused<T, true, 1, 1, true>(s, a, f, P1, vecP, Vt, vecV, Phi);
-        else launch_fused<T, false, 1, 1, true>(s, a, f, P1, vecP, Vt, vecV, Phi);
+        if (p1c) launch_fused<T, true, 1, 1, true, true>(s, a, f, P1, vecP, Vt, vecV, Phi);
+        else launch_fused<T, false, 1, 1, true, true>(s, a, f, P1, vecP, Vt, vecV, Phi);
         return;
       }
     }
-#define AFQ_PT(P1C, SPW, CB) launch_fused<T, P1C, SPW, CB>(s, a, f, P1, vecP, Vt, vecV, Phi)
+#define AFQ_PT(P1C, SPW, CB)                                                       \
+  (m3 ? launch_fused<T, P1C, SPW, CB, false, true>(s, a, f, P1, vecP, Vt, vecV, Phi) \
+      : launch_fused<T, P1C, SPW, CB, false, sizeof(T) == 8>(s, a, f, P1, vecP, Vt, vecV, Phi))
     if (f.spw == 1) {
       if constexpr (sizeof(T) == 4) {
         if (p1c) {

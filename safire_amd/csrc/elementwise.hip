@@ -457,10 +457,10 @@ int exchange_group(int n) {
 }
 
 template <typename TT>
-void exchange_blocks(hipStream_t s, int W, int n, int MC, int m0, int mcount, int M,
+void exchange_blocks(hipStream_t s, int W, int n, int mg, int MC, int m0, int mcount, int M,
                      const cplx<TT>* T, z64* exx_part, int ng_total, int g0, z64* Kl) {
   if (W <= 0 || mcount <= 0) return;
-  const int mg = exchange_group(n);
+  AFQ_CHECK(mg >= 1 && m0 % mg == 0, kErrArg, "exchange group size");
   const int ng = ceil_div(mcount, mg);
   const size_t lds = 2 * sizeof(TT) * (size_t)n * (mg * n + 1);
   AFQ_CHECK(lds <= 160 * 1024 && g0 + ng <= ng_total, kErrShape, "exchange block layout");
@@ -471,10 +471,10 @@ void exchange_blocks(hipStream_t s, int W, int n, int MC, int m0, int mcount, in
                      M, T, exx_part, ng_total, g0, Kl);
   AFQ_LAUNCH_CHECK();
 }
-template void exchange_blocks<double>(hipStream_t, int, int, int, int, int, int, const z64*, z64*,
-                                      int, int, z64*);
-template void exchange_blocks<float>(hipStream_t, int, int, int, int, int, int, const c32*, z64*,
-                                     int, int, z64*);
+template void exchange_blocks<double>(hipStream_t, int, int, int, int, int, int, int, const z64*,
+                                      z64*, int, int, z64*);
+template void exchange_blocks<float>(hipStream_t, int, int, int, int, int, int, int, const c32*,
+                                     z64*, int, int, z64*);
 
 void energy_finish(hipStream_t s, int W, int M, int nchunk, const z64* exx_part, const z64* Kl,
                    double cx, double cj, int addEXX, int addEJ, z64* E) {

@@ -116,6 +116,18 @@ void orthogonalize(Context& c, z64* Phi, long phi_bs, int W, int N, int n, z64* 
 // ---------------------------------------------------------------------------------------------
 // Cholesky (real 3-index) hamiltonian operations
 // ---------------------------------------------------------------------------------------------
+// Half-rotated tensors of one trial determinant (RealDenseHamiltonian.cpp:141-163,
+// half_rotate_hamiltonian): device buffers owned by whoever built them.
+struct Rotated {
+  z64* PsiT = nullptr;        // [nel][N]
+  z64* Lnak = nullptr;        // [M][nup][N], then [M][ndown][N] for COLLINEAR
+  z64* haj = nullptr;         // [nel][N]
+  double* Lnak_r = nullptr;   // its real part, when the determinant is real
+  float* Lnak_r32 = nullptr;  // FP32 real part (mixed precision, real determinant)
+  c32* Lnak32 = nullptr;      // FP32 copy (mixed precision, complex determinant)
+  bool real = false;
+};
+
 class CholeskyOps {
  public:
   // Builds the half-rotated tensors on the device from the bare h1, Likn and the trial
@@ -148,14 +160,27 @@ class CholeskyOps {
   void vbias32(const c32* G, int W, double dt, z64* v) const;
   // V[w][i][k] = sqrt(dt) sum_m L[i][k][m] X[w][m]   (vHS, :282-326); V is z64 (FP64) or c32
   void vHS(const void* X, int W, double dt, void* V, Precision prec) const;
-  // E[w][0..2] = E1, EXX, EJ  (energy :138-174, energy_impl :476-590); G compact [W][nel][N]
+  // E[w][0..2] = E1, EXX, EJ  (energy :138-174, energy_impl :476-590); G compact [W][nel][N].
+  // det: the half-rotated tensors of another determinant of a multi-determinant trial (the
+  // reference's idet); null: the hamiltonian's own.
   void energy(const z64* G, int W, z64* E, bool addH1, bool addEJ, bool addEXX,
-              Precision prec) const;
+              Precision prec, const Rotated* det = nullptr) const;
   // per-spin energy (Real3IndexFactorization.hpp:176-210): G [W][n_spin][N] of one block,
   // E [W][3] (E0 in the alpha block only), EJn [W][M] = sum_a T^m_aa (may be null without
   // addEJ), E[w][2] = 1/2 sum_m EJn^2
   void energy_spin(int spin, const z64* G, int W, z64* E, z64* EJn, bool addH1, bool addEJ,
                    bool addEXX, Precision prec) const;
+  // vbias of full density matrices (the ndet > 1 branch, Real3IndexFactorization.hpp:395-423):
+  // G [W][nspin][N][N], v[w][m] = sqrt(dt) c sum_s sum_ik G[w][s][i][k] L[i][k][m]
+  void vbias_full(const z64* G, int W, double dt, z64* v, Precision prec) const;
+  // the half-rotated tensors of determinant PsiT [nel][N] (device), in the precision of this
+  // hamiltonian; free them with release()
+  Rotated rotate(const z64* PsiT) const;
+  void release(Rotated& r) const;
+  // the hamiltonian's own determinant
+  Rotated self() const {
+    return Rotated{PsiT_, Lnak_, haj_, Lnak_r_, Lnak_r32_, Lnak32_, real_trial_};
+  }
   // vMF (NOMSD.cpp:437-464) with the charge-decomposition zero_imag (AFQMCBasePropagator.cpp:105)
   void vMF(double dt, z64* v) const;
   // P1 = exp(-1/2 hermitize(dt (h + vexx) + vHS(vMF))) with |x| <= 1e-8 dropped
@@ -182,7 +207,61 @@ class CholeskyOps {
   c32* Lnak32_ = nullptr;
   z64* haj_ = nullptr;      // [nel][N]
   void free_all();
-  void exchange_spin(int sp, const z64* Gblk, int W, z64* part, z64* Kl, Precision prec) const;
+  void exchange_spin(int sp, const z64* Gblk, int W, z64* part, z64* Kl, Precision prec,
+                     const Rotated& t) const;
+  void half_rotate_into(const z64* PsiT, Rotated& r) const;
+  // the fused on-chip exchange applies (FP32, real half-rotated tensor, n <= 40)
+  bool exchange_fused_path(Precision prec, bool real = true) const;
+  // exchange partials per walker and spin block of the chosen path
+  int exchange_parts(Precision prec, bool real = true) const;
+};
+
+// ---------------------------------------------------------------------------------------------
+// Multi-determinant NOMSD trial (NOMSD.cpp:39-61, 159-333, 338-520): ndet general determinants
+// with coefficients ci; determinant 0 is the hamiltonian's own PsiT.
+// ---------------------------------------------------------------------------------------------
+class NomsdTrial {
+ public:
+  // PsiT [ndet][nel][N] (device; determinant 0 equal to the hamiltonian's PsiT), ci [ndet] (host)
+  NomsdTrial(CholeskyOps& ham, int ndet, const z64* PsiT, const z64* ci);
+  ~NomsdTrial();
+  int ndet() const { return ndet_; }
+  CholeskyOps& ham() const { return ham_; }
+  // walker blocks Pa [W][N][nup], Pb [W][N][ndown] (COLLINEAR), bs_a / bs_b complex elements
+  // apart (0: contiguous)
+  // NOMSD::Log_Overlap: log sum_d conj(c_d) <D_d|Phi> about the largest term (NOMSD.cpp:295-331)
+  void log_overlap(const z64* Pa, const z64* Pb, int W, z64* Ov, long bs_a = 0, long bs_b = 0);
+  // NOMSD::MixedDensityMatrix, full G [W][nspin][N][N] = sum_d w_d G_d / sum_d w_d with
+  // w_d = conj(c_d) exp(ov_d - log_m) (log_m [W]: the walkers' OVLP); Ov = sum_d w_d + log_m,
+  // as the reference leaves it (NOMSD.cpp:240-277)
+  void mixed_density_matrix(const z64* Pa, const z64* Pb, int W, const z64* log_m, z64* G,
+                            z64* Ov, long bs_a = 0, long bs_b = 0);
+  // NOMSD::vbias with ndet > 1 (NOMSD.cpp:39-61): the full mixed density matrix, then the
+  // full-G contraction (Real3IndexFactorization.hpp:395-423); v [W][M]
+  void vbias(const z64* Pa, const z64* Pb, int W, const z64* log_m, double dt, z64* v,
+             Precision prec, long bs_a = 0, long bs_b = 0);
+  // NOMSD::Energy with ndet > 1 (NOMSD.cpp:185-216): E [W][3], Ov [W]
+  void energy(const z64* Pa, const z64* Pb, int W, z64* E, z64* Ov, Precision prec,
+              long bs_a = 0, long bs_b = 0);
+  // NOMSD::vMF with ndet > 1 (NOMSD.cpp:438-516) and zero_imag: v [M]
+  void vMF(double dt, z64* v);
+  // the determinants as references (NOMSD::getReferences): rows_s [ndet][nel_s][N] in PsiT
+  // form, cc [ndet] = conj(ci); device buffers of those sizes
+  void references(z64* rows0, z64* rows1, z64* cc);
+
+ private:
+  CholeskyOps& ham_;
+  Context& ctx_;
+  int ndet_ = 0;
+  std::vector<Rotated> dets_;   // [0]: the hamiltonian's own (not released)
+  std::vector<z64> ci_h_;
+  z64* cc_ = nullptr;           // device [ndet] conj(ci)
+  z64* lcc_ = nullptr;          // device [ndet] log conj(ci), |ci| < 1e-12 as 1e-12
+  z64* gmf_ = nullptr;          // device [nspin][N][N] trial density of vMF
+  // determinant d against the walkers: log overlaps o [W] (alpha doubled when CLOSED) and, when
+  // G is given, the density matrices (compact or full)
+  void det_block(int d, const z64* Pa, const z64* Pb, int W, long bs_a, long bs_b, z64* o,
+                 z64* G, bool compact);
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -334,7 +413,13 @@ class Propagator {
  public:
   // trial == nullptr: single determinant (the hamiltonian's PsiT rows, NOMSD); otherwise the
   // PHMSD trial built over `ham` supplies G, overlaps, energies and vMF.
-  Propagator(CholeskyOps& ham, int W, const PropagatorParams& p, PhmsdTrial* trial = nullptr);
+  Propagator(CholeskyOps& ham, int W, const PropagatorParams& p, PhmsdTrial* trial = nullptr)
+      : Propagator(ham, W, p, trial, nullptr) {}
+  // a multi-determinant NOMSD trial over `ham` supplies G, overlaps, energies and vMF
+  Propagator(CholeskyOps& ham, int W, const PropagatorParams& p, NomsdTrial* msd)
+      : Propagator(ham, W, p, nullptr, msd) {}
+  Propagator(CholeskyOps& ham, int W, const PropagatorParams& p, PhmsdTrial* trial,
+             NomsdTrial* msd);
   ~Propagator();
   const z64* vMF() const { return vMF_; }
   const z64* P1() const { return P1_; }
@@ -428,6 +513,7 @@ class Propagator {
   CholeskyOps& ham_;
   Context& ctx_;
   PhmsdTrial* trial_;
+  NomsdTrial* msd_;
   int W_;
   int wup_, wdn_;   // electrons per spin of the walkers (columns of Phi, Phi_b)
   bool closed_;

@@ -35,13 +35,16 @@ T* dalloc(size_t count) {
 
 }  // namespace
 
-Propagator::Propagator(CholeskyOps& ham, int W, const PropagatorParams& p, PhmsdTrial* trial)
-    : ham_(ham), ctx_(ham.ctx()), trial_(trial), W_(W),
+Propagator::Propagator(CholeskyOps& ham, int W, const PropagatorParams& p, PhmsdTrial* trial,
+                       NomsdTrial* msd)
+    : ham_(ham), ctx_(ham.ctx()), trial_(trial), msd_(msd), W_(W),
       wup_(trial ? trial->nup() : ham.nup()),
       wdn_(trial ? trial->ndn() : ham.ndown()),
       closed_(ham.walker_type() == kClosed), p_(p), prec_((Precision)ham.precision()) {
   AFQ_CHECK(W > 0, kErrArg, "W must be positive");
   AFQ_CHECK(!trial || &trial->ham() == &ham, kErrArg, "the trial belongs to another hamiltonian");
+  AFQ_CHECK(!msd || &msd->ham() == &ham, kErrArg, "the trial belongs to another hamiltonian");
+  AFQ_CHECK(!(trial && msd), kErrArg, "one trial");
   AFQ_CHECK(p.order >= 0, kErrArg, "order");
   if (p_.target_pop <= 0) p_.target_pop = W;
   hipStream_t s = ctx_.stream();
@@ -58,6 +61,8 @@ Propagator::Propagator(CholeskyOps& ham, int W, const PropagatorParams& p, Phmsd
     // generateP1: vMF (subtractMF = true), then H1 and P1
     if (trial_)
       trial_->vMF(p_.dt, vMF_);
+    else if (msd_)
+      msd_->vMF(p_.dt, vMF_);
     else
       ham.vMF(p_.dt, vMF_);
     ham.one_body_propagator(vMF_, p_.dt, P1_);
@@ -172,6 +177,10 @@ void Propagator::log_overlap(const WalkerState& st, z64* ovlp) {
     trial_->log_overlap(st.Phi, st.Phi_b, st.W, ovlp, bs_a(st), bs_b(st));
     return;
   }
+  if (msd_) {
+    msd_->log_overlap(st.Phi, st.Phi_b, st.W, ovlp, bs_a(st), bs_b(st));
+    return;
+  }
   hipStream_t s = ctx_.stream();
   const int N = ham_.N(), nup = ham_.nup(), ndown = ham_.ndown();
   const bool closed = closed_;
@@ -205,6 +214,9 @@ void Propagator::step_impl(const WalkerState& st, const double* u, const double*
   if (trial_) {
     trial_->mixed_density_matrix(st.Phi, st.Phi_b, W, G, ov, bs_a(st), bs_b(st));
     ham_.vbias(G, W, p_.dt, X, prec_);
+  } else if (msd_) {
+    // full mixed density matrix of the expansion, scaled by the walkers' OVLP (NOMSD.cpp:251)
+    msd_->vbias(st.Phi, st.Phi_b, W, st.ovlp, p_.dt, X, prec_, bs_a(st), bs_b(st));
   } else {
     c32* G32 = prec_ == kMixed ? scratch_as<c32>(ctx_, "st_G32", (size_t)W * nel * N) : nullptr;
     fill_z64(s, W, {0, 0}, ov);
@@ -409,7 +421,7 @@ void Propagator::ensure_references() {
   if (nref_ > 0) return;
   hipStream_t s = ctx_.stream();
   const int N = ham_.N();
-  const int R = trial_ ? trial_->ndet() : 1;
+  const int R = trial_ ? trial_->ndet() : msd_ ? msd_->ndet() : 1;
   const int nsl[2] = {wup_, wdn_};
   z64* buf[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   try {
@@ -431,6 +443,10 @@ void Propagator::ensure_references() {
                                hipMemcpyHostToDevice, s));
         AFQ_HIP(hipStreamSynchronize(s));
       }
+    } else if (msd_) {
+      // NOMSD with several determinants: each one a reference with conj(ci)
+      msd_->references(buf[0], buf[1], buf[2]);
+      AFQ_HIP(hipStreamSynchronize(s));
     } else {
       // NOMSD: the single determinant, coefficient 1
       for (int sp = 0; sp < 2; ++sp)
@@ -485,15 +501,18 @@ void Propagator::bp_propagate_refs(const WalkerState& st, int nbp, int ortho, z6
     bp_propagate_orbitals(st, C, nbp, ortho, Lp, ldR);
     return;
   }
-  // a single reference: L_s [W][N][n_s] starts from the trial's orbitals in every walker
-  AFQ_CHECK(R == 1, kErrState, "several references are back-propagated through their orbitals");
+  // general references (one determinant, or the determinants of a NOMSD expansion): L_s
+  // [R][W][N][n_s] starts from each reference's orbitals in every walker, and each reference
+  // is propagated and orthogonalized as a batch of its own with the step's V
   for (int sp = 0; sp < nsp; ++sp) {
     Lp[sp] = nullptr;
     if (nsl[sp] == 0) continue;  // a spin without electrons (a polarized COLLINEAR set)
-    Lp[sp] = scratch_as<z64>(ctx_, sp == 0 ? "bp_L0" : "bp_L1", (size_t)W * N * nsl[sp]);
-    broadcast_z64(s, W, (long)N * nsl[sp], ref_cols_[sp], Lp[sp]);
+    Lp[sp] = scratch_as<z64>(ctx_, sp == 0 ? "bp_L0" : "bp_L1", (size_t)R * W * N * nsl[sp]);
+    for (int d = 0; d < R; ++d)
+      broadcast_z64(s, W, (long)N * nsl[sp], ref_cols_[sp] + (size_t)d * N * nsl[sp],
+                    Lp[sp] + (size_t)d * W * N * nsl[sp]);
   }
-  fill_z64(s, W, {0, 0}, ldR);
+  fill_z64(s, (long)R * W, {0, 0}, ldR);
   z64* X = scratch_as<z64>(ctx_, "bp_X", (size_t)W * M);
   z64* V = scratch_as<z64>(ctx_, "bp_V", (size_t)W * N * N);
   for (int ni = nbp - 1; ni >= 0; --ni) {
@@ -502,16 +521,18 @@ void Propagator::bp_propagate_refs(const WalkerState& st, int nbp, int ortho, z6
     gather_fields(s, W, M, bp_fields_, bp_n_, slot, X);
     ham_.vHS(X, W, p_.dt, V, kFP64);
     const bool qr = ni == 0 || ni % ortho == 0;
-    for (int sp = 0; sp < nsp; ++sp) {
-      const int n = nsl[sp];
-      if (n == 0) continue;
-      det_ops::PropagateArgs pa{W, N, n, p_.order, 'H', P1_, real_P1_ ? P1r_ : nullptr,
-                                nullptr, nullptr, V, (long)N * N, Lp[sp], (long)N * n, kFP64};
-      pa.v_symmetric = true;  // vHS writes both triangles from the same sums
-      det_ops::propagate(ctx_, pa);
-      // logdetR accumulates log det R of both spins (AFQMCBasePropagator.cpp:473-478)
-      if (qr) det_ops::orthogonalize(ctx_, Lp[sp], (long)N * n, W, N, n, ldR);
-    }
+    for (int d = 0; d < R; ++d)
+      for (int sp = 0; sp < nsp; ++sp) {
+        const int n = nsl[sp];
+        if (n == 0) continue;
+        z64* L = Lp[sp] + (size_t)d * W * N * n;
+        det_ops::PropagateArgs pa{W, N, n, p_.order, 'H', P1_, real_P1_ ? P1r_ : nullptr,
+                                  nullptr, nullptr, V, (long)N * N, L, (long)N * n, kFP64};
+        pa.v_symmetric = true;  // vHS writes both triangles from the same sums
+        det_ops::propagate(ctx_, pa);
+        // logdetR accumulates log det R of both spins (AFQMCBasePropagator.cpp:473-478)
+        if (qr) det_ops::orthogonalize(ctx_, L, (long)N * n, W, N, n, ldR + (size_t)d * W);
+      }
   }
 }
 
@@ -945,6 +966,10 @@ void Propagator::energy(const WalkerState& st, z64* E, z64* ovlp, Precision prec
   check_strides(st);
   if (trial_) {  // PHMSD::Energy (energy_alg1), FP64
     trial_->energy(st.Phi, st.Phi_b, st.W, E, ovlp, bs_a(st), bs_b(st));
+    return;
+  }
+  if (msd_) {  // NOMSD::Energy with ndet > 1
+    msd_->energy(st.Phi, st.Phi_b, st.W, E, ovlp, prec, bs_a(st), bs_b(st));
     return;
   }
   hipStream_t s = ctx_.stream();
