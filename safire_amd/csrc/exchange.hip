@@ -12,9 +12,11 @@
 //   * The next k group's fragments are loaded while this group's MFMAs run (a full group loads
 //     unconditionally from clamped rows, so the compiler's wait lands after the MFMAs).
 //   * The pairing T_ab T_ba crosses lanes, so the accumulators go through LDS: the columns of
-//     one vector at a time ([2 nsp][nsp + 1] per wavefront), then each lane adds its share of
-//     the products in FP64.  Each lane keeps its own FP64 sums over all its tiles; a fixed-order
-//     wavefront reduction at the end gives one exchange partial per (walker, wavefront).
+//     one vector at a time, column-major ([nsp][2 nsp + 8] per wavefront), then each lane adds
+//     its share of the products in FP64 over 8 x 8 blocks of (b, a), using the symmetry of the
+//     sum to visit only the blocks on and above the diagonal.  Each lane keeps its own FP64
+//     sums over all its tiles; a fixed-order wavefront reduction at the end gives one exchange
+//     partial per (walker, wavefront).
 //   * Coulomb traces sum_a T^m_aa go to Kl[w][m] (added, so both spin blocks of a COLLINEAR
 //     walker accumulate), each (w, m) owned by exactly one wavefront.
 // A workgroup (four wavefronts) takes one walker pair and a run of 8 VPW Cholesky vectors;
@@ -134,17 +136,20 @@ __global__ void __launch_bounds__(64 * kXWaves)
     k_exx_fused(int W, int ns, int N, int M, const c32* __restrict__ G,
                 const float* __restrict__ L, int vpw, int npairs, int nchunk, z64* __restrict__ part,
                 int nparts, z64* __restrict__ Kl) {
-  constexpr int NSP = 8 * NB, R = 2 * NSP, LD = NSP + 1;
-  extern __shared__ __align__(16) float smem_x[];  // [kXWaves][2][R * LD]
+  // LDS pitch of a column of T: the rows of both walkers plus 8, which keeps the 8 x 8 lane
+  // blocks of the exchange reads on distinct banks
+  constexpr int NSP = 8 * NB, R = 2 * NSP, LDC = R + 8;
+  extern __shared__ __align__(16) float smem_x[];  // [kXWaves][2][NSP * LDC]
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int fr = lane & 15, fk = lane >> 4;
+  const int xb = lane & 7, xa = lane >> 3;
   const long total = gridDim.x, xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
   const long q = xcd * (total / 8) + min(xcd, total % 8) + slot;
   const int pair = (int)(q % npairs), chunk = (int)(q / npairs);
   const int w0 = 2 * pair;
   const int NGF = N / 16;  // full k groups
-  float* tre = smem_x + wave * 2 * R * LD;
-  float* tim = tre + R * LD;
+  float* tre = smem_x + wave * 2 * NSP * LDC;
+  float* tim = tre + NSP * LDC;
   double xr[2] = {0.0, 0.0}, xi[2] = {0.0, 0.0};
 
   for (int vp = 0; vp < vpw; ++vp) {
@@ -181,7 +186,9 @@ __global__ void __launch_bounds__(64 * kXWaves)
       x_load<NB, VEC, false>(cur, gb + NGF * GS, lb + NGF * LS, o, N - 16 * NGF - 4 * fk);
       x_mma<NB>(cur, ar, ai);
     }
-    // one Cholesky vector (tile columns mj NSP .. mj NSP + NSP - 1) at a time through LDS
+    // one Cholesky vector (tile columns mj NSP .. mj NSP + NSP - 1) at a time through LDS,
+    // column-major: T[row][c] at c LDC + row, so each lane stores its four accumulator rows as
+    // one 16-byte write
 #pragma unroll
     for (int mj = 0; mj < 2; ++mj) {
       // the previous pass's reads of this wavefront's LDS are complete before it is rewritten
@@ -193,13 +200,10 @@ __global__ void __launch_bounds__(64 * kXWaves)
         const int c = 16 * jb + fr - mj * NSP;
         if (c < 0 || c >= NSP) continue;
 #pragma unroll
-        for (int i = 0; i < NB; ++i)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int row = 16 * i + 4 * fk + r;
-            tre[row * LD + c] = ar[i][jb][r];
-            tim[row * LD + c] = ai[i][jb][r];
-          }
+        for (int i = 0; i < NB; ++i) {
+          *reinterpret_cast<f32x4*>(tre + c * LDC + 16 * i + 4 * fk) = ar[i][jb];
+          *reinterpret_cast<f32x4*>(tim + c * LDC + 16 * i + 4 * fk) = ai[i][jb];
+        }
       }
       // the stores are complete (LDS operations of a wavefront complete in order) before any
       // lane reads another lane's entries
@@ -211,21 +215,27 @@ __global__ void __launch_bounds__(64 * kXWaves)
 #pragma unroll
       for (int wi = 0; wi < 2; ++wi) {
         if (w0 + wi >= W) continue;
-        const float* ur = tre + wi * NSP * LD;
-        const float* ui = tim + wi * NSP * LD;
-        // exchange: sum_ab T_ba T_ab, entries (b, a) = (t / ns, t % ns) by lane
-        for (int t = lane; t < ns * ns; t += 64) {
-          const int b = t / ns, a = t % ns;
-          const double pr = ur[b * LD + a], pi = ui[b * LD + a];
-          const double qr = ur[a * LD + b], qi = ui[a * LD + b];
-          xr[wi] += pr * qr - pi * qi;
-          xi[wi] += pr * qi + pi * qr;
-        }
+        const float* ur = tre + wi * NSP;
+        const float* ui = tim + wi * NSP;
+        // exchange: sum_ab T_ab T_ba over 8 x 8 blocks (ib, ia) of (b, a), lane (lb, la) taking
+        // b = 8 ib + lb, a = 8 ia + la; the sum is symmetric in (a, b), so the blocks above the
+        // diagonal count twice and those below are skipped
+#pragma unroll
+        for (int ib = 0; ib < NB; ++ib)
+#pragma unroll
+          for (int ia = ib; ia < NB; ++ia) {
+            const int b = 8 * ib + xb, a = 8 * ia + xa;
+            const double pr = ur[a * LDC + b], pi = ui[a * LDC + b];  // T_ba
+            const double qr = ur[b * LDC + a], qi = ui[b * LDC + a];  // T_ab
+            const double wgt = (b < ns && a < ns) ? (ia == ib ? 1.0 : 2.0) : 0.0;
+            xr[wi] += wgt * (pr * qr - pi * qi);
+            xi[wi] += wgt * (pr * qi + pi * qr);
+          }
         // Coulomb trace of this (walker, vector)
         double kr = 0.0, ki = 0.0;
         if (lane < ns) {
-          kr = ur[lane * LD + lane];
-          ki = ui[lane * LD + lane];
+          kr = ur[lane * LDC + lane];
+          ki = ui[lane * LDC + lane];
         }
         kr = wave_sum(kr);
         ki = wave_sum(ki);
@@ -254,7 +264,7 @@ void launch_exx(hipStream_t s, int W, int ns, int N, int M, const c32* G, const 
   const long blocks = (long)npairs * nchunk;
   AFQ_CHECK(blocks < (1L << 31), kErrShape, "grid too large");
   constexpr int NSP = 8 * NB;
-  const size_t lds = sizeof(float) * kXWaves * 2 * (2 * NSP) * (NSP + 1);
+  const size_t lds = sizeof(float) * kXWaves * 2 * NSP * (2 * NSP + 8);
   const bool vec = (N % 4) == 0 && (reinterpret_cast<uintptr_t>(G) % 16) == 0 &&
                    (reinterpret_cast<uintptr_t>(L) % 16) == 0;
   auto kern = vec ? k_exx_fused<NB, true> : k_exx_fused<NB, false>;
