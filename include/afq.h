@@ -67,7 +67,7 @@ typedef struct afq_nomsd afq_nomsd;
  * afq_prop_te_anchor its follow_walkers argument.  3.00: walker strides (afq_walkers.phi_stride
  * and prop_stride, the phi_stride argument of the Slater-determinant and PHMSD entry points) and
  * afq_tune_force.  3.01: afq_mixed_density_matrix_noalloc.  3.02: multi-determinant NOMSD
- * (afq_nomsd_*, afq_prop_create_nomsd) and afq_vbias_full. */
+ * (afq_nomsd_*, afq_prop_create_nomsd), afq_vbias_full and afq_ham_create_spin. */
 int afq_abi_version(void);
 /* Text of the last error on this thread ("" when none). */
 const char* afq_last_error(void);
@@ -97,6 +97,15 @@ int afq_tune_force(const char* site, int candidate);
 int afq_ham_create(afq_ctx* ctx, int walker_type, int N, int nup, int ndown, int M,
                    const double* Likn, const double* h1, const double* PsiT, double E0,
                    int precision, afq_ham** out);
+/* As afq_ham_create, with spin-dependent integrals (RealDenseHamiltonian.cpp:86-120, hcore
+ * [nspin_in_H1][NMO][NMO] and DenseFactorized/L [nspin_in_H2][NMO][NMO][ncv]): Likn
+ * [nspin_L][N][N][M], h1 [nspin_h1][N][N], each 1 or 2 (alpha block, then beta; 2 only for
+ * COLLINEAR walkers).  Spin s uses block s % nspin: its half-rotated tensors, vexx, vHS block
+ * and one-body propagator.  With nspin_L = 2, afq_vhs writes V [2][W][N][N]; with either 2,
+ * afq_one_body_propagator writes P1 [2][N][N]. */
+int afq_ham_create_spin(afq_ctx* ctx, int walker_type, int N, int nup, int ndown, int M,
+                        const double* Likn, int nspin_L, const double* h1, int nspin_h1,
+                        const double* PsiT, double E0, int precision, afq_ham** out);
 int afq_ham_destroy(afq_ham* ham);
 /* Device views of the half-rotated tensors: Lnak complex [M][nup][N] (then [M][ndown][N] for
  * COLLINEAR), haj complex [nel][N], vexx double [N][N]. number_of_cholesky_vectors() is M. */
@@ -107,7 +116,8 @@ int afq_ham_tensors(afq_ham* ham, const double** Lnak, const double** haj, const
  * precision AFQ_MIXED contracts in FP32 (FP32 MFMA, FP32 accumulation) and writes FP64 v. */
 int afq_vbias(afq_ham* ham, const double* G, int W, double dt, int precision, double* v);
 /* Real3IndexFactorization::vHS (Real3IndexFactorization.hpp:282-326):
- * V[w][i][k] = sqrt(dt) * sum_m L[i][k][m] X[w][m].  FP64: X, V complex128.
+ * V[s][w][i][k] = sqrt(dt) * sum_m L_s[i][k][m] X[w][m] over the Cholesky tensor's spin blocks
+ * s (one unless created with nspin_L = 2).  FP64: X, V complex128.
  * AFQ_MIXED: X, V complex64 (FP32 MFMA). */
 int afq_vhs(afq_ham* ham, const void* X, int W, double dt, int precision, void* V);
 /* Real3IndexFactorization::energy (Real3IndexFactorization.hpp:138-174, energy_impl :476-590):
@@ -127,7 +137,8 @@ int afq_energy_spin(afq_ham* ham, int spin, const double* G, int W, int addH1, i
 int afq_vmf(afq_ham* ham, double dt, double* vMF);
 /* generateP1 + getOneBodyPropagatorMatrix (AFQMCBasePropagator.cpp:43-214,
  * Real3IndexFactorization.hpp:99-124): P1 = exp(-1/2 hermitize(dt (h1 + vexx) + vHS(vMF))),
- * entries with |x| <= 1e-8 set to zero (to_csr, csr_utils.hpp:43-66).  P1 complex [N][N]. */
+ * entries with |x| <= 1e-8 set to zero (to_csr, csr_utils.hpp:43-66).  P1 complex [N][N], or
+ * [2][N][N] (alpha, beta) for spin-dependent integrals. */
 int afq_one_body_propagator(afq_ham* ham, const double* vMF, double dt, double* P1);
 
 /* ---- Slater-determinant operations (det_ops) --------------------------------------------- */

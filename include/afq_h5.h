@@ -32,8 +32,9 @@ const char* afq_h5_last_error(void);
 /* /Hamiltonian extents: nspin of hcore, orbitals N, Cholesky vectors M, nspin of L. */
 int afq_h5_hamiltonian_shape(const char* path, int* nspin, int* N, int* M, int* nspin_L);
 
-/* hcore [nspin][N][N] and L as [N][N][M] (the file's [nspin_L = 1][1][N][1][N][M]); E0 from the
- * attributes, with nelec the total electron count of the walkers (used by the Madelung term). */
+/* hcore [nspin][N][N] and L as [nspin_L][N][N][M] (the file's [nspin_L][1][N][1][N][M], nspin_L
+ * 1 or nspin); E0 from the attributes, with nelec the total electron count of the walkers (used
+ * by the Madelung term). */
 int afq_h5_read_hamiltonian(const char* path, int nelec, double* hcore, double* L, double* E0);
 
 /* /Wavefunction: kind (AFQ_H5_NOMSD / AFQ_H5_PHMSD), walker type of spin_type, orbitals N,

@@ -125,6 +125,7 @@ _SIGS = {
     "afq_nomsd_energy": [_P, _P, _P, ctypes.c_int64, _I, _I, _P, _P],
     "afq_nomsd_vmf": [_P, _D, _P],
     "afq_vbias_full": [_P, _P, _I, _D, _I, _P],
+    "afq_ham_create_spin": [_P, _I, _I, _I, _I, _I, _P, _I, _P, _I, _P, _D, _I, ctypes.POINTER(_P)],
 }
 
 _lib = None
@@ -352,12 +353,17 @@ def average_pseudo_energy(ctx, weight, eloc, dt, out, relax=0.0, Eshift=None):
 # ---------------------------------------------------------------------------------------------
 
 class HamiltonianOperations:
+    """Real3IndexFactorization over h1 [N][N] and Likn [N][N][M], or spin-dependent integrals
+    h1 [2][N][N] and/or Likn [2][N][N][M] (alpha, beta; COLLINEAR walkers)."""
+
     def __init__(self, ctx, h1, Likn, PsiT, nup, ndown=0, walker_type=AFQ_CLOSED, E0=0.0,
                  precision=AFQ_FP64):
         torch = _torch()
         self.ctx = ctx
-        N = h1.shape[0]
-        M = Likn.shape[2]
+        self.nspin_h1 = h1.shape[0] if h1.ndim == 3 else 1
+        self.nspin_L = Likn.shape[0] if Likn.ndim == 4 else 1
+        N = h1.shape[-1]
+        M = Likn.shape[-1]
         self.N, self.M, self.nup, self.ndown = N, M, nup, (ndown if walker_type == AFQ_COLLINEAR else 0)
         self.walker_type = walker_type
         self.precision = precision
@@ -366,8 +372,13 @@ class HamiltonianOperations:
         dh = device_tensor(h1, np.float64)
         dP = device_tensor(PsiT, np.complex128)
         h = _P()
-        _ck(lib().afq_ham_create(ctx.h, walker_type, N, nup, ndown, M, ptr(dL), ptr(dh), ptr(dP),
-                                 E0, precision, ctypes.byref(h)))
+        if self.nspin_h1 == 1 and self.nspin_L == 1:
+            _ck(lib().afq_ham_create(ctx.h, walker_type, N, nup, ndown, M, ptr(dL), ptr(dh),
+                                     ptr(dP), E0, precision, ctypes.byref(h)))
+        else:
+            _ck(lib().afq_ham_create_spin(ctx.h, walker_type, N, nup, ndown, M, ptr(dL),
+                                          self.nspin_L, ptr(dh), self.nspin_h1, ptr(dP), E0,
+                                          precision, ctypes.byref(h)))
         ctx.synchronize()
         del dL, dh, dP
         self.h = h
@@ -620,8 +631,11 @@ class Propagator:
         return out
 
     def P1(self):
+        """P1 [N][N], or [2][N][N] (alpha, beta) for spin-dependent integrals."""
         torch = _torch()
-        out = torch.empty((self.ham.N, self.ham.N), dtype=torch.complex128, device="cuda")
+        np1 = 2 if max(self.ham.nspin_L, self.ham.nspin_h1) > 1 else 1
+        shape = (self.ham.N, self.ham.N) if np1 == 1 else (2, self.ham.N, self.ham.N)
+        out = torch.empty(shape, dtype=torch.complex128, device="cuda")
         self.ctx.synchronize()
         _copy_d2d(out.data_ptr(), self._p1, out.numel() * 16)
         return out

@@ -86,10 +86,11 @@ void average_pseudo_energy(hipStream_t s, int W, const z64* weight, const z64* e
 
 // ---- estimators.hip (rdm_measure, which needs the context's scratch, is declared in ops.hpp) ---
 // Reference coefficients of a multi-reference measurement (MixedEstimator.hpp:61-103,
-// BackPropEstimator.hpp:62-120): x[d][w] = cc[d] exp(lo[d][w] + conj(ad[w][d])) (ad may be
-// null), coeff[d][w] = x[d][w] / sum_d' x[d'][w].  lo [R][W] log overlaps, cc [R] conj(ci).
+// BackPropEstimator.hpp:62-120): x[d][w] = cc[d] exp(scale (lo[d][w] + conj(ad[w][d]))) (ad
+// may be null), coeff[d][w] = x[d][w] / sum_d' x[d'][w].  lo [R][W] log overlaps, cc [R]
+// conj(ci); scale 2 doubles the alpha block of CLOSED walkers where the estimator does.
 void reference_coefficients(hipStream_t s, int W, int R, const z64* lo, const z64* cc,
-                            const z64* ad, z64* coeff);
+                            const z64* ad, z64* coeff, double scale = 1.0);
 
 // ---- elementwise.hip -------------------------------------------------------------------------
 void scale_add_cplx(hipStream_t s, long n, const z64* x, double ar, double ai, z64* y);  // y = a x

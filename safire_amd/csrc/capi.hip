@@ -212,6 +212,23 @@ int afq_ham_create(afq_ctx* ctx, int walker_type, int N, int nup, int ndown, int
   });
 }
 
+int afq_ham_create_spin(afq_ctx* ctx, int walker_type, int N, int nup, int ndown, int M,
+                        const double* Likn, int nspin_L, const double* h1, int nspin_h1,
+                        const double* PsiT, double E0, int precision, afq_ham** out) {
+  return guard([&] {
+    REQUIRE(ctx && out && Likn && h1 && PsiT, "NULL argument");
+    auto* h = new afq_ham{ctx, nullptr};
+    try {
+      h->ops = new afq::CholeskyOps(ctx->c, walker_type, N, nup, ndown, M, Likn, h1, Z(PsiT), E0,
+                                    precision, nspin_L, nspin_h1);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
 int afq_ham_destroy(afq_ham* ham) {
   return guard([&] {
     if (!ham) return;

@@ -59,33 +59,42 @@ T* dalloc(size_t count) {
 
 CholeskyOps::CholeskyOps(Context& c, int walker_type, int N, int nup, int ndown, int M,
                          const double* Likn, const double* h1, const z64* PsiT, double E0,
-                         int precision)
+                         int precision, int nspin_L, int nspin_h1)
     : ctx_(c), walker_type_(walker_type), N_(N), nup_(nup), ndown_(ndown), M_(M), prec_(precision),
-      E0_(E0) {
+      nl_(nspin_L), nh_(nspin_h1), E0_(E0) {
   AFQ_CHECK(walker_type == kClosed || walker_type == kCollinear, kErrArg, "walker_type");
   AFQ_CHECK(N > 0 && M > 0 && nup >= 0 && ndown >= 0 && nup <= N && ndown <= N, kErrShape,
             "invalid sizes");
   AFQ_CHECK(nup <= 64 && ndown <= 64, kErrShape, "electrons per spin must be <= 64");
   AFQ_CHECK(precision == kFP64 || precision == kMixed, kErrArg, "precision");
+  AFQ_CHECK((nspin_L == 1 || nspin_L == 2) && (nspin_h1 == 1 || nspin_h1 == 2), kErrArg,
+            "the Cholesky tensor and h1 have 1 or 2 spin blocks");
+  // CLOSED walkers take spin-independent integrals (walkerDimsAreConvertible)
+  AFQ_CHECK(walker_type == kCollinear || (nspin_L == 1 && nspin_h1 == 1), kErrArg,
+            "spin-dependent integrals need COLLINEAR walkers");
   if (walker_type == kClosed) ndown_ = 0;
   hipStream_t s = c.stream();
   const int nel_ = nel();
   try {
-    Likn_ = dalloc<double>((size_t)N * N * M);
-    h1_ = dalloc<double>((size_t)N * N);
-    vexx_ = dalloc<double>((size_t)N * N);
-    AFQ_HIP(hipMemcpyAsync(Likn_, Likn, sizeof(double) * N * N * M, hipMemcpyDeviceToDevice, s));
-    AFQ_HIP(hipMemcpyAsync(h1_, h1, sizeof(double) * N * N, hipMemcpyDeviceToDevice, s));
-    sym_L_ = symmetric_cholesky(s, N, M, Likn_);
-    // vexx[i][l] = -1/2 sum_{(j,m)} Likn[i][(j,m)] Likn[l][(j,m)]
-    {
-      Opnd<double, false, true> A{Likn_, (long)N * M, 0};
-      Opnd<double, false, true> B{Likn_, (long)N * M, 0};
-      launch_large<double>(s, 1, N, N, N * M, A, B, EpiStoreReal<double>{vexx_, N, 0, -0.5});
+    const size_t NNM = (size_t)N * N * M;
+    Likn_ = dalloc<double>(nl_ * NNM);
+    h1_ = dalloc<double>((size_t)nh_ * N * N);
+    vexx_ = dalloc<double>((size_t)nl_ * N * N);
+    AFQ_HIP(hipMemcpyAsync(Likn_, Likn, sizeof(double) * nl_ * NNM, hipMemcpyDeviceToDevice, s));
+    AFQ_HIP(hipMemcpyAsync(h1_, h1, sizeof(double) * nh_ * N * N, hipMemcpyDeviceToDevice, s));
+    sym_L_ = true;
+    for (int l = 0; l < nl_; ++l) sym_L_ = symmetric_cholesky(s, N, M, Likn_ + l * NNM) && sym_L_;
+    // vexx_l[i][k] = -1/2 sum_{(j,m)} Likn_l[i][(j,m)] Likn_l[k][(j,m)]
+    // (RealDenseHamiltonian.cpp:165-181, one block per spin block of L)
+    for (int l = 0; l < nl_; ++l) {
+      Opnd<double, false, true> A{Likn_ + l * NNM, (long)N * M, 0};
+      Opnd<double, false, true> B{Likn_ + l * NNM, (long)N * M, 0};
+      launch_large<double>(s, 1, N, N, N * M, A, B,
+                           EpiStoreReal<double>{vexx_ + (size_t)l * N * N, N, 0, -0.5});
     }
     if (prec_ == kMixed) {
-      Likn32_ = dalloc<float>((size_t)N * N * M);
-      cast_f64_f32(s, (long)N * N * M, Likn_, Likn32_);
+      Likn32_ = dalloc<float>(nl_ * NNM);
+      cast_f64_f32(s, (long)(nl_ * NNM), Likn_, Likn32_);
     }
     Rotated own;
     half_rotate_into(PsiT, own);
@@ -130,15 +139,26 @@ void CholeskyOps::half_rotate_into(const z64* PsiT, Rotated& r) const {
       const z64* Ps = r.PsiT + (sp == 0 ? 0 : (size_t)nup_ * N);
       z64* Ls = r.Lnak + (sp == 0 ? 0 : (size_t)M * nup_ * N);
       Opnd<double, true, true> A{&Ps->re, N, 0};
-      Opnd<double, false, false> B{Likn_, (long)N * M, 0};
+      Opnd<double, false, false> B{Likn_ + (size_t)(sp % nl_) * N * N * M, (long)N * M, 0};
       launch_large<double>(s, 1, ns, N * M, N, A, B,
                            EpiStore<double, double>{tmp, (long)N * M, 0, 1, 0, 0});
       permute_akm_mak(s, ns, N, M, tmp, Ls);
     }
-    {
+    if (nh_ == 1) {
       Opnd<double, true, true> A{&r.PsiT->re, N, 0};
       Opnd<double, false, false> B{h1_, N, 0};
       launch_small<double>(s, 1, nel_, N, N, A, B, EpiStore<double, double>{r.haj, N, 0, 1, 0, 0});
+    } else {
+      // haj of each spin block from its own h1 (half_rotate_hamiltonian, one_body.hpp:123-160)
+      for (int sp = 0; sp < 2; ++sp) {
+        const int ns = sp == 0 ? nup_ : ndown_;
+        if (ns == 0) continue;
+        const size_t off = sp == 0 ? 0 : (size_t)nup_ * N;
+        Opnd<double, true, true> A{&(r.PsiT + off)->re, N, 0};
+        Opnd<double, false, false> B{h1_ + (size_t)sp * N * N, N, 0};
+        launch_small<double>(s, 1, ns, N, N, A, B,
+                             EpiStore<double, double>{r.haj + off, N, 0, 1, 0, 0});
+      }
     }
     if (r.real) {
       r.Lnak_r = dalloc<double>((size_t)M * nel_ * N);
@@ -238,41 +258,47 @@ void CholeskyOps::vbias(const z64* G, int W, double dt, z64* v, Precision prec) 
 
 void CholeskyOps::vHS(const void* X, int W, double dt, void* V, Precision prec) const {
   if (W == 0) return;
-  hipStream_t s = ctx_.stream();
   const double a = std::sqrt(dt);
   const long NN = (long)N_ * N_;
-  if (sym_L_) {
-    // V[w] is complex symmetric: compute the 8x8 blocks of the upper triangle of (i, k) pairs
-    // and write each value to (i, k) and (k, i)
-    const int nb = ceil_div(N_, 8);
-    const int npairs = nb * (nb + 1) / 2 * 64;
+  const size_t esz = prec == kFP64 ? sizeof(z64) : sizeof(c32);
+  for (int l = 0; l < nl_; ++l) {
+    // block l of V [nl][W][N][N] from block l of L
+    char* Vl = reinterpret_cast<char*>(V) + (size_t)l * W * NN * esz;
+    const size_t loff = (size_t)l * NN * M_;
+    if (sym_L_) {
+      // V[w] is complex symmetric: compute the 8x8 blocks of the upper triangle of (i, k) pairs
+      // and write each value to (i, k) and (k, i)
+      const int nb = ceil_div(N_, 8);
+      const int npairs = nb * (nb + 1) / 2 * 64;
+      if (prec == kFP64) {
+        gemm::RevK<Opnd<double, true, true>> A({reinterpret_cast<const double*>(X), M_, 0}, M_);
+        gemm::RevK<gemm::OpndSymPairs<double>> B({Likn_ + loff, M_, 0, N_, nb}, M_);
+        gemm_large<double>(ctx_, "vhs_sym", 1, W, npairs, M_, A, B,
+                           gemm::EpiStoreSym<double, double>{reinterpret_cast<z64*>(Vl), NN, 0, a,
+                                                             0, N_, nb});
+      } else {
+        AFQ_CHECK(prec_ == kMixed, kErrState, "FP32 tensors were not built (precision FP64)");
+        gemm::RevK<Opnd<float, true, true>> A({reinterpret_cast<const float*>(X), M_, 0}, M_);
+        gemm::RevK<gemm::OpndSymPairs<float>> B({Likn32_ + loff, M_, 0, N_, nb}, M_);
+        gemm_large<float>(ctx_, "vhs_sym", 1, W, npairs, M_, A, B,
+                          gemm::EpiStoreSym<float, float>{reinterpret_cast<c32*>(Vl), NN, 0,
+                                                          (float)a, 0.f, N_, nb});
+      }
+      continue;
+    }
     if (prec == kFP64) {
       gemm::RevK<Opnd<double, true, true>> A({reinterpret_cast<const double*>(X), M_, 0}, M_);
-      gemm::RevK<gemm::OpndSymPairs<double>> B({Likn_, M_, 0, N_, nb}, M_);
-      gemm_large<double>(ctx_, "vhs_sym", 1, W, npairs, M_, A, B,
-                         gemm::EpiStoreSym<double, double>{reinterpret_cast<z64*>(V), NN, 0, a,
-                                                           0, N_, nb});
+      gemm::RevK<Opnd<double, false, true>> B({Likn_ + loff, M_, 0}, M_);
+      gemm_large<double>(ctx_, "vhs", 1, W, (int)NN, M_, A, B,
+                         EpiStore<double, double>{reinterpret_cast<z64*>(Vl), NN, 0, a, 0, 0});
     } else {
       AFQ_CHECK(prec_ == kMixed, kErrState, "FP32 tensors were not built (precision FP64)");
       gemm::RevK<Opnd<float, true, true>> A({reinterpret_cast<const float*>(X), M_, 0}, M_);
-      gemm::RevK<gemm::OpndSymPairs<float>> B({Likn32_, M_, 0, N_, nb}, M_);
-      gemm_large<float>(ctx_, "vhs_sym", 1, W, npairs, M_, A, B,
-                        gemm::EpiStoreSym<float, float>{reinterpret_cast<c32*>(V), NN, 0,
-                                                        (float)a, 0.f, N_, nb});
+      gemm::RevK<Opnd<float, false, true>> B({Likn32_ + loff, M_, 0}, M_);
+      gemm_large<float>(ctx_, "vhs", 1, W, (int)NN, M_, A, B,
+                        EpiStore<float, float>{reinterpret_cast<c32*>(Vl), NN, 0, (float)a, 0.f,
+                                               0});
     }
-    return;
-  }
-  if (prec == kFP64) {
-    gemm::RevK<Opnd<double, true, true>> A({reinterpret_cast<const double*>(X), M_, 0}, M_);
-    gemm::RevK<Opnd<double, false, true>> B({Likn_, M_, 0}, M_);
-    gemm_large<double>(ctx_, "vhs", 1, W, (int)NN, M_, A, B,
-                       EpiStore<double, double>{reinterpret_cast<z64*>(V), NN, 0, a, 0, 0});
-  } else {
-    AFQ_CHECK(prec_ == kMixed, kErrState, "FP32 tensors were not built (precision FP64)");
-    gemm::RevK<Opnd<float, true, true>> A({reinterpret_cast<const float*>(X), M_, 0}, M_);
-    gemm::RevK<Opnd<float, false, true>> B({Likn32_, M_, 0}, M_);
-    gemm_large<float>(ctx_, "vhs", 1, W, (int)NN, M_, A, B,
-                      EpiStore<float, float>{reinterpret_cast<c32*>(V), NN, 0, (float)a, 0.f, 0});
   }
 }
 
@@ -312,13 +338,14 @@ void CholeskyOps::vbias_full(const z64* G, int W, double dt, z64* v, Precision p
   // v[w][m] (+)= a sum_{(i,k)} G[w][s][(i,k)] Likn[(i,k)][m], one product per spin block
   for (int sp = 0; sp < nspin; ++sp) {
     Opnd<double, true, true> A{&(G + sp * NN)->re, nspin * NN, 0};
+    const size_t loff = (size_t)(sp % nl_) * NN * M_;
     if (prec == kFP64) {
-      Opnd<double, false, false> B{Likn_, M_, 0};
+      Opnd<double, false, false> B{Likn_ + loff, M_, 0};
       gemm_large_splitk<double>(ctx_, "vbias_full", W, M_, (int)NN, A, B,
                                 EpiStore<double, double>{v, M_, 0, a, 0, sp}, sp == 0);
     } else {
       AFQ_CHECK(prec_ == kMixed, kErrState, "FP32 tensors were not built (precision FP64)");
-      Opnd<float, false, false> B{Likn32_, M_, 0};
+      Opnd<float, false, false> B{Likn32_ + loff, M_, 0};
       gemm_large_splitk<float>(ctx_, "vbias_full32", W, M_, (int)NN, A, B,
                                EpiStore<float, double>{v, M_, 0, (float)a, 0.f, sp}, sp == 0);
     }
@@ -495,44 +522,49 @@ void CholeskyOps::one_body_propagator(const z64* vMF, double dt, z64* P1, bool i
   hipStream_t s = ctx_.stream();
   const int N = N_;
   const long NN = (long)N * N;
-  z64* shift = scratch_as<z64>(ctx_, "p1_shift", NN);
+  z64* shift = scratch_as<z64>(ctx_, "p1_shift", (size_t)nl_ * NN);
   z64* H = scratch_as<z64>(ctx_, "p1_H", NN);
   z64* E = scratch_as<z64>(ctx_, "p1_E", NN);
   z64* F = scratch_as<z64>(ctx_, "p1_F", NN);
-  // meanfield shift = vHS(vMF) (includes sqrt(dt)); H1 = dt h + dt vexx + shift
+  // meanfield shift = vHS(vMF) (includes sqrt(dt)), one block per spin block of L
   vHS(vMF, 1, dt, shift, kFP64);
-  add_one_body(s, N, dt, h1_, vexx_, shift, H);
-  hermitize_scale(s, N, inverse ? 0.5 : -0.5, H);  // hermitize(H1); H1 *= -1/2 (+1/2: inverse)
-  // scaling and squaring: ||H||_1 on the host (setup path)
-  std::vector<z64> hH(NN);
-  AFQ_HIP(hipMemcpyAsync(hH.data(), H, sizeof(z64) * NN, hipMemcpyDeviceToHost, s));
-  AFQ_HIP(hipStreamSynchronize(s));
-  double norm1 = 0.0;
-  for (int j = 0; j < N; ++j) {
-    double cs = 0.0;
-    for (int i = 0; i < N; ++i) cs += std::hypot(hH[(size_t)i * N + j].re, hH[(size_t)i * N + j].im);
-    norm1 = std::max(norm1, cs);
+  for (int sp = 0; sp < p1_spins(); ++sp) {
+    // H1_s = dt h_s + dt vexx_s + shift_s (add_one_body_shifts, one_body.hpp:65-85)
+    add_one_body(s, N, dt, h1_ + (size_t)(sp % nh_) * NN, vexx_ + (size_t)(sp % nl_) * NN,
+                 shift + (size_t)(sp % nl_) * NN, H);
+    hermitize_scale(s, N, inverse ? 0.5 : -0.5, H);  // hermitize(H1); H1 *= -1/2 (+1/2: inverse)
+    // scaling and squaring: ||H||_1 on the host (setup path)
+    std::vector<z64> hH(NN);
+    AFQ_HIP(hipMemcpyAsync(hH.data(), H, sizeof(z64) * NN, hipMemcpyDeviceToHost, s));
+    AFQ_HIP(hipStreamSynchronize(s));
+    double norm1 = 0.0;
+    for (int j = 0; j < N; ++j) {
+      double cs = 0.0;
+      for (int i = 0; i < N; ++i) cs += std::hypot(hH[(size_t)i * N + j].re, hH[(size_t)i * N + j].im);
+      norm1 = std::max(norm1, cs);
+    }
+    int sq = 0;
+    while (norm1 / std::ldexp(1.0, sq) > 0.25) ++sq;
+    const double scl = std::ldexp(1.0, -sq);
+    // Horner: E = I; for k = 20..1: E = I + (B E) / k with B = H / 2^sq
+    identity(s, N, E);
+    for (int k = 20; k >= 1; --k) {
+      identity(s, N, F);
+      Opnd<double, true, true> A{&H->re, N, 0};
+      Opnd<double, true, false> B{&E->re, N, 0};
+      launch_small<double>(s, 1, N, N, N, A, B, EpiStore<double, double>{F, N, 0, scl / k, 0, 1});
+      std::swap(E, F);
+    }
+    for (int i = 0; i < sq; ++i) {
+      Opnd<double, true, true> A{&E->re, N, 0};
+      Opnd<double, true, false> B{&E->re, N, 0};
+      launch_small<double>(s, 1, N, N, N, A, B, EpiStore<double, double>{F, N, 0, 1, 0, 0});
+      std::swap(E, F);
+    }
+    z64* Ps = P1 + (size_t)sp * NN;
+    AFQ_HIP(hipMemcpyAsync(Ps, E, sizeof(z64) * NN, hipMemcpyDeviceToDevice, s));
+    truncate_small(s, NN, 1e-8, Ps);  // to_csr(P1, 1e-8) (csr_utils.hpp:43-66)
   }
-  int sq = 0;
-  while (norm1 / std::ldexp(1.0, sq) > 0.25) ++sq;
-  const double scl = std::ldexp(1.0, -sq);
-  // Horner: E = I; for k = 20..1: E = I + (B E) / k with B = H / 2^sq
-  identity(s, N, E);
-  for (int k = 20; k >= 1; --k) {
-    identity(s, N, F);
-    Opnd<double, true, true> A{&H->re, N, 0};
-    Opnd<double, true, false> B{&E->re, N, 0};
-    launch_small<double>(s, 1, N, N, N, A, B, EpiStore<double, double>{F, N, 0, scl / k, 0, 1});
-    std::swap(E, F);
-  }
-  for (int i = 0; i < sq; ++i) {
-    Opnd<double, true, true> A{&E->re, N, 0};
-    Opnd<double, true, false> B{&E->re, N, 0};
-    launch_small<double>(s, 1, N, N, N, A, B, EpiStore<double, double>{F, N, 0, 1, 0, 0});
-    std::swap(E, F);
-  }
-  AFQ_HIP(hipMemcpyAsync(P1, E, sizeof(z64) * NN, hipMemcpyDeviceToDevice, s));
-  truncate_small(s, NN, 1e-8, P1);  // to_csr(P1, 1e-8) (csr_utils.hpp:43-66)
 }
 
 }  // namespace afq

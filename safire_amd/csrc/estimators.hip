@@ -121,7 +121,7 @@ __global__ void k_rdm_finish(long n_one, long n_all, int WB, const z64* __restri
 // one thread per walker; the largest real part is taken out before exponentiating, which
 // cancels in the ratio and keeps every term finite
 __global__ void k_ref_coeff(int W, int R, const z64* __restrict__ lo, const z64* __restrict__ cc,
-                            const z64* __restrict__ ad, z64* __restrict__ coeff) {
+                            const z64* __restrict__ ad, z64* __restrict__ coeff, double scale) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= W) return;
   auto expo = [&](int d) {
@@ -130,7 +130,7 @@ __global__ void k_ref_coeff(int W, int R, const z64* __restrict__ lo, const z64*
       const z64 a = ad[(long)w * R + d];
       e = {e.re + a.re, e.im - a.im};
     }
-    return e;
+    return z64{scale * e.re, scale * e.im};
   };
   double m = -HUGE_VAL;
   for (int d = 0; d < R; ++d) m = fmax(m, expo(d).re);
@@ -148,9 +148,10 @@ __global__ void k_ref_coeff(int W, int R, const z64* __restrict__ lo, const z64*
 }  // namespace
 
 void reference_coefficients(hipStream_t s, int W, int R, const z64* lo, const z64* cc,
-                            const z64* ad, z64* coeff) {
+                            const z64* ad, z64* coeff, double scale) {
   AFQ_CHECK(W > 0 && R > 0, kErrShape, "reference coefficients shape");
-  hipLaunchKernelGGL(k_ref_coeff, dim3(ceil_div(W, 256)), dim3(256), 0, s, W, R, lo, cc, ad, coeff);
+  hipLaunchKernelGGL(k_ref_coeff, dim3(ceil_div(W, 256)), dim3(256), 0, s, W, R, lo, cc, ad, coeff,
+                     scale);
   AFQ_LAUNCH_CHECK();
 }
 

@@ -477,7 +477,10 @@ int afq_h5_read_hamiltonian(const char* path, int nelec, double* hcore, double* 
   return guarded([&] {
     int nspin = 0, N = 0, M = 0, nsl = 0;
     if (afq_h5_hamiltonian_shape(path, &nspin, &N, &M, &nsl) != 0) throw Fail(kErrShape, g_err);
-    if (nsl != 1) throw Fail(kErrShape, "spin-dependent Cholesky vectors are not supported");
+    // L is shared by every spin (nspin_L = 1) or has hcore's spin blocks
+    // (RealDenseHamiltonian.cpp:98-111)
+    if (nsl != 1 && nsl != nspin)
+      throw Fail(kErrShape, "DenseFactorized/L must have 1 spin block or as many as hcore");
     Id f = open_file(path);
     Id hg = open_group(f, "Hamiltonian");
     if (hcore) {

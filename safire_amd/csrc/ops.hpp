@@ -132,9 +132,12 @@ class CholeskyOps {
  public:
   // Builds the half-rotated tensors on the device from the bare h1, Likn and the trial
   // (RealDenseHamiltonian::getHamiltonianOperations, RealDenseHamiltonian.cpp:52-186).
-  // Likn [N][N][M] real, h1 [N][N] real, PsiT [nup+ndown][N] complex (alpha rows then beta).
+  // Likn [nspin_L][N][N][M] real, h1 [nspin_h1][N][N] real (nspin 2: alpha then beta, COLLINEAR
+  // only; spin s reads block s % nspin, RealDenseHamiltonian.cpp:86-120), PsiT [nup+ndown][N]
+  // complex (alpha rows then beta).
   CholeskyOps(Context& c, int walker_type, int N, int nup, int ndown, int M, const double* Likn,
-              const double* h1, const z64* PsiT, double E0, int precision);
+              const double* h1, const z64* PsiT, double E0, int precision, int nspin_L = 1,
+              int nspin_h1 = 1);
   ~CholeskyOps();
 
   int N() const { return N_; }
@@ -151,6 +154,11 @@ class CholeskyOps {
   const double* vexx() const { return vexx_; }
   bool real_trial() const { return real_trial_; }
   bool symmetric_L() const { return sym_L_; }
+  // spin blocks of the Cholesky tensor and of h1; vHS writes nspin_L() blocks [W][N][N] and the
+  // one-body propagator has p1_spins() blocks [N][N] (2 when either depends on spin)
+  int nspin_L() const { return nl_; }
+  int nspin_h1() const { return nh_; }
+  int p1_spins() const { return (nl_ > 1 || nh_ > 1) ? 2 : 1; }
 
   // v[w][m] = sqrt(dt) c sum_{a,k} G[w][a][k] Lnak[m][a][k]   (vbias, :328-428)
   // prec == kMixed runs the contraction in FP32 and writes FP64 v.
@@ -158,7 +166,8 @@ class CholeskyOps {
   // vbias of a complex64 copy of G (FP32 MFMA; the same values as vbias(G, ..., kMixed), which
   // rounds G to FP32 while staging it)
   void vbias32(const c32* G, int W, double dt, z64* v) const;
-  // V[w][i][k] = sqrt(dt) sum_m L[i][k][m] X[w][m]   (vHS, :282-326); V is z64 (FP64) or c32
+  // V[s][w][i][k] = sqrt(dt) sum_m L_s[i][k][m] X[w][m] for the nspin_L() blocks s (vHS,
+  // :282-326, [nstot][nwalk][NMO][NMO]); V is z64 (FP64) or c32
   void vHS(const void* X, int W, double dt, void* V, Precision prec) const;
   // E[w][0..2] = E1, EXX, EJ  (energy :138-174, energy_impl :476-590); G compact [W][nel][N].
   // det: the half-rotated tensors of another determinant of a multi-determinant trial (the
@@ -183,8 +192,9 @@ class CholeskyOps {
   }
   // vMF (NOMSD.cpp:437-464) with the charge-decomposition zero_imag (AFQMCBasePropagator.cpp:105)
   void vMF(double dt, z64* v) const;
-  // P1 = exp(-1/2 hermitize(dt (h + vexx) + vHS(vMF))) with |x| <= 1e-8 dropped
-  // (generateP1, AFQMCBasePropagator.cpp:43-214). P1 [N][N] complex.
+  // P1_s = exp(-1/2 hermitize(dt (h_s + vexx_s) + vHS_s(vMF))) with |x| <= 1e-8 dropped
+  // (generateP1, AFQMCBasePropagator.cpp:43-214; add_one_body_shifts, one_body.hpp:65-85).
+  // P1 [p1_spins()][N][N] complex.
   // inverse: exp(+1/2 H1) instead, the P1x_inv of generateP1(true) (:118-123)
   void one_body_propagator(const z64* vMF, double dt, z64* P1, bool inverse = false) const;
 
@@ -193,14 +203,15 @@ class CholeskyOps {
  private:
   Context& ctx_;
   int walker_type_, N_, nup_, ndown_, M_, prec_;
+  int nl_ = 1, nh_ = 1;  // spin blocks of Likn and h1
   double E0_;
   bool real_trial_ = false;
   bool sym_L_ = false;    // every L^m symmetric: vHS computes one triangle and mirrors it
   z64* PsiT_ = nullptr;   // [nel][N]
-  double* Likn_ = nullptr;  // [N][N][M]
+  double* Likn_ = nullptr;  // [nl][N][N][M]
   float* Likn32_ = nullptr;
-  double* h1_ = nullptr;    // [N][N]
-  double* vexx_ = nullptr;  // [N][N]
+  double* h1_ = nullptr;    // [nh][N][N]
+  double* vexx_ = nullptr;  // [nl][N][N]
   z64* Lnak_ = nullptr;     // [M][nel][N]: alpha block [M][nup][N] then beta [M][ndown][N]
   double* Lnak_r_ = nullptr;  // real part when the trial is real
   float* Lnak_r32_ = nullptr;
@@ -422,6 +433,7 @@ class Propagator {
              NomsdTrial* msd);
   ~Propagator();
   const z64* vMF() const { return vMF_; }
+  // P1 [p1_spins][N][N]
   const z64* P1() const { return P1_; }
   // One step (AFQMCBasePropagator::Propagate, AFQMCBasePropagator.cpp:289-392). Uniforms u
   // [W][M] injected when given, else drawn from Philox with the step counter held on the device.
@@ -520,11 +532,15 @@ class Propagator {
   PropagatorParams p_;
   Precision prec_;
   z64* vMF_ = nullptr;
-  z64* P1_ = nullptr;
+  z64* P1_ = nullptr;       // [np][N][N], np = ham.p1_spins()
   double* P1r_ = nullptr;
   float* P1r32_ = nullptr;
   c32* P1c32_ = nullptr;
   bool real_P1_ = false;
+  // the propagation of spin block sp: its P1 block (the inverse one with `inverse`) and its V
+  // block (V [nspin_L][W][N][N] in precision prec)
+  det_ops::PropagateArgs spin_args(int sp, int W, int n, char ta, const void* V, Precision prec,
+                                   z64* Phi, long phi_bs, bool inverse = false) const;
   uint32_t* step_dev_ = nullptr;
   int* status_dev_ = nullptr;
   unsigned long long* stats_dev_ = nullptr;  // [9] bound-trigger counters
@@ -565,8 +581,10 @@ class Propagator {
   void bp_propagate_refs(const WalkerState& s, int nbp, int ortho, z64* Lp[2], z64* ldR);
   void bp_propagate_orbitals(const WalkerState& s, uint32_t C, int nbp, int ortho, z64* Lp[2],
                              z64* ldR);
+  // ov_scale: 2 where the estimator doubles the CLOSED overlaps (back-propagation), else 1
   void measure_refs(int W, const z64* const Lp[2], bool per_walker, const z64* const Phi[2],
-                    const z64* weight, const z64* ldR, z64* onerdm, z64* diag2rdm);
+                    const z64* weight, const z64* ldR, z64* onerdm, z64* diag2rdm,
+                    double ov_scale);
   void clear_graphs();
   void step_impl(const WalkerState& s, const double* u, const double* Eshift);
   void orthogonalize_impl(const WalkerState& s);

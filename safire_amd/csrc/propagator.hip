@@ -49,9 +49,10 @@ Propagator::Propagator(CholeskyOps& ham, int W, const PropagatorParams& p, Phmsd
   if (p_.target_pop <= 0) p_.target_pop = W;
   hipStream_t s = ctx_.stream();
   const int N = ham.N(), M = ham.M();
+  const long NP = (long)ham.p1_spins() * N * N;  // every one-body propagator block
   try {
     vMF_ = dalloc<z64>(M);
-    P1_ = dalloc<z64>((size_t)N * N);
+    P1_ = dalloc<z64>(NP);
     step_dev_ = dalloc<uint32_t>(1);
     AFQ_HIP(hipMemsetAsync(step_dev_, 0, sizeof(uint32_t), s));
     status_dev_ = dalloc<int>(1);
@@ -66,22 +67,22 @@ Propagator::Propagator(CholeskyOps& ham, int W, const PropagatorParams& p, Phmsd
     else
       ham.vMF(p_.dt, vMF_);
     ham.one_body_propagator(vMF_, p_.dt, P1_);
-    std::vector<z64> h((size_t)N * N);
+    std::vector<z64> h(NP);
     AFQ_HIP(hipMemcpyAsync(h.data(), P1_, sizeof(z64) * h.size(), hipMemcpyDeviceToHost, s));
     AFQ_HIP(hipStreamSynchronize(s));
     real_P1_ = true;
     for (auto& v : h)
       if (v.im != 0.0) { real_P1_ = false; break; }
     if (real_P1_) {
-      P1r_ = dalloc<double>((size_t)N * N);
-      real_part(s, (long)N * N, P1_, P1r_);
+      P1r_ = dalloc<double>(NP);
+      real_part(s, NP, P1_, P1r_);
       if (prec_ == kMixed) {
-        P1r32_ = dalloc<float>((size_t)N * N);
-        real_part_f32(s, (long)N * N, P1_, P1r32_);
+        P1r32_ = dalloc<float>(NP);
+        real_part_f32(s, NP, P1_, P1r32_);
       }
     } else if (prec_ == kMixed) {
-      P1c32_ = dalloc<c32>((size_t)N * N);
-      cast_z64_c32(s, (long)N * N, P1_, P1c32_);
+      P1c32_ = dalloc<c32>(NP);
+      cast_z64_c32(s, NP, P1_, P1c32_);
     }
     AFQ_HIP(hipStreamSynchronize(s));
   } catch (...) {
@@ -206,7 +207,8 @@ void Propagator::step_impl(const WalkerState& st, const double* u, const double*
   z64* HW = scratch_as<z64>(ctx_, "st_HW", W);
   z64* mf = scratch_as<z64>(ctx_, "st_mf", W);
   z64* nov = scratch_as<z64>(ctx_, "st_nov", W);
-  void* V = ctx_.scratch("st_V", (size_t)W * N * N * (prec_ == kMixed ? sizeof(c32) : sizeof(z64)));
+  void* V = ctx_.scratch("st_V", (size_t)ham_.nspin_L() * W * N * N *
+                                     (prec_ == kMixed ? sizeof(c32) : sizeof(z64)));
 
   // 1. G and vbias (NOMSD::vbias, NOMSD.cpp:39-61; PHMSD::vbias, PHMSD.cpp:53-72).  In mixed
   // precision G is computed in FP64 and stored as complex64 for the FP32 vbias contraction (the
@@ -240,14 +242,12 @@ void Propagator::step_impl(const WalkerState& st, const double* u, const double*
   if (vhs_timing_) AFQ_HIP(hipEventRecord(vhs_ev_[0], s));
   ham_.vHS(prec_ == kMixed ? (const void*)X32 : (const void*)X, W, p_.dt, V, prec_);
   if (vhs_timing_) AFQ_HIP(hipEventRecord(vhs_ev_[1], s));
-  // 4. propagate both spin blocks with the same V
-  det_ops::PropagateArgs pa{W, N, nup, p_.order, 'N', P1_, real_P1_ ? P1r_ : nullptr, P1r32_,
-                            P1c32_, V, (long)N * N, st.Phi, bs_a(st), prec_};
+  // 4. propagate both spin blocks, each with its P1 and V blocks (one shared V and P1 for a
+  // spin-independent hamiltonian)
+  det_ops::PropagateArgs pa = spin_args(0, W, nup, 'N', V, prec_, st.Phi, bs_a(st));
   det_ops::propagate(ctx_, pa);
   if (!closed && ndown > 0) {
-    pa.n = ndown;
-    pa.Phi = st.Phi_b;
-    pa.phi_bs = bs_b(st);
+    pa = spin_args(1, W, ndown, 'N', V, prec_, st.Phi_b, bs_b(st));
     det_ops::propagate(ctx_, pa);
   }
   // 5. new overlaps (hybrid), or the local energies and overlaps (hybrid = false)
@@ -264,6 +264,24 @@ void Propagator::step_impl(const WalkerState& st, const double* u, const double*
   walker_update(s, W, up, Eshift, nov, mf, HW, st.weight, st.phase, st.eloc, st.ovlp, st.theta,
                 bp_n_ > 0 ? bp_wfac_ : nullptr, bp_hl_, step_dev_, En);
   if (En) store_energy_components(s, W, En, st.e1, st.exx, st.ej, st.e_bs);
+}
+
+det_ops::PropagateArgs Propagator::spin_args(int sp, int W, int n, char ta, const void* V,
+                                             Precision prec, z64* Phi, long phi_bs,
+                                             bool inverse) const {
+  const int N = ham_.N();
+  const long NN = (long)N * N;
+  const long p1 = (long)(sp % ham_.p1_spins()) * NN;
+  const size_t esz = prec == kFP64 ? sizeof(z64) : sizeof(c32);
+  const void* Vs = reinterpret_cast<const char*>(V) + (size_t)(sp % ham_.nspin_L()) * W * NN * esz;
+  const z64* P = inverse ? P1inv_ : P1_;
+  const double* Pr = inverse ? P1invr_ : P1r_;
+  det_ops::PropagateArgs pa{W, N, n, p_.order, ta, P + p1, real_P1_ ? Pr + p1 : nullptr,
+                            (prec != kFP64 && !inverse && P1r32_) ? P1r32_ + p1 : nullptr,
+                            (prec != kFP64 && !inverse && P1c32_) ? P1c32_ + p1 : nullptr, Vs,
+                            NN, Phi, phi_bs,
+                            prec};
+  return pa;
 }
 
 void Propagator::bound_stats(long long out[9], bool reset) {
@@ -512,9 +530,12 @@ void Propagator::bp_propagate_refs(const WalkerState& st, int nbp, int ortho, z6
       broadcast_z64(s, W, (long)N * nsl[sp], ref_cols_[sp] + (size_t)d * N * nsl[sp],
                     Lp[sp] + (size_t)d * W * N * nsl[sp]);
   }
-  fill_z64(s, (long)R * W, {0, 0}, ldR);
+  // log det R accumulates reference-major [R][W] (each reference's batch contiguous) and is
+  // handed back walker-major, ldR [W][R], the layout the reference coefficients read
+  z64* ldRT = R > 1 ? scratch_as<z64>(ctx_, "bp_ldRT", (size_t)R * W) : ldR;
+  fill_z64(s, (long)R * W, {0, 0}, ldRT);
   z64* X = scratch_as<z64>(ctx_, "bp_X", (size_t)W * M);
-  z64* V = scratch_as<z64>(ctx_, "bp_V", (size_t)W * N * N);
+  z64* V = scratch_as<z64>(ctx_, "bp_V", (size_t)ham_.nspin_L() * W * N * N);
   for (int ni = nbp - 1; ni >= 0; --ni) {
     const long back = (long)nbp - ni;  // 1 = the newest field
     const int slot = (int)((((long)C - back) % bp_n_ + bp_n_) % bp_n_);
@@ -526,14 +547,14 @@ void Propagator::bp_propagate_refs(const WalkerState& st, int nbp, int ortho, z6
         const int n = nsl[sp];
         if (n == 0) continue;
         z64* L = Lp[sp] + (size_t)d * W * N * n;
-        det_ops::PropagateArgs pa{W, N, n, p_.order, 'H', P1_, real_P1_ ? P1r_ : nullptr,
-                                  nullptr, nullptr, V, (long)N * N, L, (long)N * n, kFP64};
+        det_ops::PropagateArgs pa = spin_args(sp, W, n, 'H', V, kFP64, L, (long)N * n);
         pa.v_symmetric = true;  // vHS writes both triangles from the same sums
         det_ops::propagate(ctx_, pa);
         // logdetR accumulates log det R of both spins (AFQMCBasePropagator.cpp:473-478)
-        if (qr) det_ops::orthogonalize(ctx_, L, (long)N * n, W, N, n, ldR + (size_t)d * W);
+        if (qr) det_ops::orthogonalize(ctx_, L, (long)N * n, W, N, n, ldRT + (size_t)d * W);
       }
   }
+  if (R > 1) swap_mid(s, 1, R, W, 1, ldRT, ldR);
 }
 
 void Propagator::bp_propagate_orbitals(const WalkerState& st, uint32_t C, int nbp, int ortho,
@@ -564,7 +585,7 @@ void Propagator::bp_propagate_orbitals(const WalkerState& st, uint32_t C, int nb
   fill_z64(s, (long)W * R, {0, 0}, ldR);
   z64* ld0 = scratch_as<z64>(ctx_, "bpo_ld", W);
   z64* X = scratch_as<z64>(ctx_, "bp_X", (size_t)W * M);
-  z64* V = scratch_as<z64>(ctx_, "bp_V", (size_t)W * N * N);
+  z64* V = scratch_as<z64>(ctx_, "bp_V", (size_t)ham_.nspin_L() * W * N * N);
   for (int ni = nbp - 1; ni >= 0; --ni) {
     const long back = (long)nbp - ni;
     const int slot = (int)((((long)C - back) % bp_n_ + bp_n_) % bp_n_);
@@ -574,8 +595,7 @@ void Propagator::bp_propagate_orbitals(const WalkerState& st, uint32_t C, int nb
     for (int sp = 0; sp < 2; ++sp) {
       const int na = trial_->nact(sp), n = nsl[sp];
       const int Rn = R * n;
-      det_ops::PropagateArgs pa{W, N, na, p_.order, 'H', P1_, real_P1_ ? P1r_ : nullptr,
-                                nullptr, nullptr, V, (long)N * N, O[sp], (long)N * na, kFP64};
+      det_ops::PropagateArgs pa = spin_args(sp, W, na, 'H', V, kFP64, O[sp], (long)N * na);
       pa.v_symmetric = true;
       det_ops::propagate(ctx_, pa);
       if (!qr) continue;
@@ -660,7 +680,7 @@ int reference_chunk(int R, int W, int nsp, int N, int n) {
 // (per_walker) or [R][N][n_s] shared by the walkers.
 void Propagator::measure_refs(int W, const z64* const Lp[2], bool per_walker,
                               const z64* const Phi[2], const z64* weight, const z64* ldR,
-                              z64* onerdm, z64* diag2rdm) {
+                              z64* onerdm, z64* diag2rdm, double ov_scale) {
   hipStream_t s = ctx_.stream();
   const int N = ham_.N(), R = nref_;
   const bool coll = !closed_;
@@ -683,7 +703,7 @@ void Propagator::measure_refs(int W, const z64* const Lp[2], bool per_walker,
                                               (long)N * nsl[sp], W, B, nullptr, 0,
                                               lo + (size_t)d0 * W);
   }
-  reference_coefficients(s, W, R, lo, ref_cc_, ldR, cf);
+  reference_coefficients(s, W, R, lo, ref_cc_, ldR, cf, ov_scale);
   z64* G = scratch_as<z64>(ctx_, "ms_G", (size_t)RC * W * nsp * N * N);
   // a spin without electrons keeps a zero density matrix
   if (nsl[0] == 0 || (nsp == 2 && nsl[1] == 0))
@@ -718,10 +738,11 @@ void Propagator::bp_measure(const WalkerState& st0, int nbp, int ortho, int rest
   z64* wr = scratch_as<z64>(ctx_, "bpm_w", W);
   restore_weights(s, W, st.weight, bp_wfac_, bp_hl_, C, restore * nbp, wr);
   // reference coefficients conj(ci) <L_d|anchor> e^{conj(logdetR)} (constructBPMeasurementInputs,
-  // BackPropEstimator.hpp:62-104); the back-propagated references are [R][W][N][n]
+  // BackPropEstimator.hpp:62-104, which doubles both for CLOSED walkers); the back-propagated
+  // references are [R][W][N][n]
   const z64* L[2] = {Lp[0], Lp[1]};
   const z64* anch[2] = {bp_anch_a_, bp_anch_b_};
-  measure_refs(W, L, true, anch, wr, ldR, onerdm, diag2rdm);
+  measure_refs(W, L, true, anch, wr, ldR, onerdm, diag2rdm, closed_ ? 2.0 : 1.0);
 }
 
 void Propagator::mixed_measure(const WalkerState& st0, z64* onerdm, z64* diag2rdm) {
@@ -729,10 +750,11 @@ void Propagator::mixed_measure(const WalkerState& st0, z64* onerdm, z64* diag2rd
   AFQ_CHECK(st.W <= W_, kErrShape, "walker count exceeds the propagator's capacity");
   ensure_references();
   // constructMixedMeasurementInputs (MixedEstimator.hpp:40-106): conj(ci) <D_d|Phi> normalized
-  // over the references of each walker (the reference's shift by OVLP cancels in the ratio)
+  // over the references of each walker (the reference's shift by OVLP cancels in the ratio); its
+  // overlaps take the alpha block once, also for CLOSED walkers
   const z64* L[2] = {ref_cols_[0], ref_cols_[1]};
   const z64* Phi[2] = {st.Phi, st.Phi_b};
-  measure_refs(st.W, L, false, Phi, st.weight, nullptr, onerdm, diag2rdm);
+  measure_refs(st.W, L, false, Phi, st.weight, nullptr, onerdm, diag2rdm, 1.0);
 }
 
 // ---- time-evolved operators ------------------------------------------------------------------
@@ -741,14 +763,15 @@ const z64* Propagator::P1_inverse() {
   if (P1inv_) return P1inv_;
   hipStream_t s = ctx_.stream();
   const int N = ham_.N();
-  z64* p = dalloc<z64>((size_t)N * N);
+  const long NP = (long)ham_.p1_spins() * N * N;
+  z64* p = dalloc<z64>(NP);
   double* pr = nullptr;
   try {
     ham_.one_body_propagator(vMF_, p_.dt, p, true);
     // H1 real <=> P1 real <=> its inverse real
     if (real_P1_) {
-      pr = dalloc<double>((size_t)N * N);
-      real_part(s, (long)N * N, p, pr);
+      pr = dalloc<double>(NP);
+      real_part(s, NP, p, pr);
     }
     AFQ_HIP(hipStreamSynchronize(s));
   } catch (...) {
@@ -817,7 +840,8 @@ void Propagator::te_propagate(const WalkerState& st0, int nsteps) {
   AFQ_HIP(hipMemcpyAsync(&C, step_dev_, sizeof(C), hipMemcpyDeviceToHost, s));
   AFQ_HIP(hipStreamSynchronize(s));
   z64* Xf = scratch_as<z64>(ctx_, "bp_X", (size_t)W * M);
-  z64* V = scratch_as<z64>(ctx_, "bp_V", (size_t)W * NN);
+  const int nl = ham_.nspin_L();
+  z64* V = scratch_as<z64>(ctx_, "bp_V", (size_t)nl * W * NN);
   z64 *Xo = te_op_[0], *Yo = te_op_[1], *Mo = te_op_[2];
   for (int i = 0; i < nsteps; ++i) {  // oldest first
     const long back = (long)nsteps - i;
@@ -825,15 +849,13 @@ void Propagator::te_propagate(const WalkerState& st0, int nsteps) {
     gather_fields(s, W, M, bp_fields_, bp_n_, slot, Xf);
     ham_.vHS(Xf, W, p_.dt, V, kFP64);
     for (int sp = 0; sp < nsp; ++sp) {
-      det_ops::PropagateArgs pa{W, N, N, p_.order, 'N', P1_, real_P1_ ? P1r_ : nullptr,
-                                nullptr, nullptr, V, NN, Xo + sp * NN, obs, kFP64};
+      det_ops::PropagateArgs pa = spin_args(sp, W, N, 'N', V, kFP64, Xo + sp * NN, obs);
       det_ops::propagate(ctx_, pa);
     }
     // B^{-H} = P1inv expM(-V)^H P1inv: the adjoint propagator of -V with the inverse P1
-    scale_add_cplx(s, (long)W * NN, V, -1.0, 0.0, V);
+    scale_add_cplx(s, (long)nl * W * NN, V, -1.0, 0.0, V);
     for (int sp = 0; sp < nsp; ++sp) {
-      det_ops::PropagateArgs pa{W, N, N, p_.order, 'H', P1inv_, real_P1_ ? P1invr_ : nullptr,
-                                nullptr, nullptr, V, NN, Yo + sp * NN, obs, kFP64};
+      det_ops::PropagateArgs pa = spin_args(sp, W, N, 'H', V, kFP64, Yo + sp * NN, obs, true);
       pa.v_symmetric = true;
       det_ops::propagate(ctx_, pa);
     }
@@ -912,7 +934,9 @@ void Propagator::te_measure(const WalkerState& st0, int restore_steps, bool tran
                                               (long)N * nsl[sp], W, nsl[sp], N, Phi[sp],
                                               (long)N * nsl[sp], W, std::min(RC, R - d0) * W,
                                               nullptr, 0, lo + (size_t)d0 * W);
-  reference_coefficients(s, W, R, lo, ref_cc_, nullptr, cf);
+  // constructTimeEvolvedMeasurementInputs doubles the CLOSED overlaps
+  // (TimeEvolvedBPEstimator.hpp:85-95)
+  reference_coefficients(s, W, R, lo, ref_cc_, nullptr, cf, closed_ ? 2.0 : 1.0);
   z64* Gc = scratch_as<z64>(ctx_, "te_Gc", (size_t)W * nmax * N);
   z64* GY = scratch_as<z64>(ctx_, "te_GY", (size_t)W * nmax * N);
   z64* XR = scratch_as<z64>(ctx_, "te_XR", (size_t)W * nmax * N);

@@ -68,16 +68,16 @@ def _p(a):
 @dataclasses.dataclass
 class Hamiltonian:
     h1: np.ndarray        # [nspin][N][N] real
-    Likn: np.ndarray      # [N][N][M] real
+    Likn: np.ndarray      # [N][N][M] real, or [2][N][N][M] with spin-dependent vectors
     E0: float
 
     @property
     def N(self):
-        return self.Likn.shape[0]
+        return self.Likn.shape[-2]
 
     @property
     def M(self):
-        return self.Likn.shape[2]
+        return self.Likn.shape[-1]
 
 
 def read_hamiltonian(path: str, nelec: int) -> Hamiltonian:
@@ -87,7 +87,7 @@ def read_hamiltonian(path: str, nelec: int) -> Hamiltonian:
     _ck(lib().afq_h5_hamiltonian_shape(b, ctypes.byref(ns), ctypes.byref(N), ctypes.byref(M),
                                        ctypes.byref(nsl)))
     h1 = np.empty((ns.value, N.value, N.value))
-    L = np.empty((N.value, N.value, M.value))
+    L = np.empty(((nsl.value,) if nsl.value > 1 else ()) + (N.value, N.value, M.value))
     E0 = ctypes.c_double()
     _ck(lib().afq_h5_read_hamiltonian(b, nelec, _p(h1), _p(L), ctypes.byref(E0)))
     return Hamiltonian(h1=h1, Likn=L, E0=E0.value)
@@ -136,8 +136,8 @@ def read_wavefunction(path: str) -> Wavefunction:
 @dataclasses.dataclass
 class System:
     """Inputs of HamiltonianOperations (and of a PHMSD trial) from one pair of files."""
-    h1: np.ndarray          # [N][N] real (spin-independent one-body term)
-    Likn: np.ndarray        # [N][N][M] real
+    h1: np.ndarray          # [N][N] real, or [2][N][N] (alpha, beta) when spin-dependent
+    Likn: np.ndarray        # [N][N][M] real, or [2][N][N][M] when spin-dependent
     E0: float
     PsiT: np.ndarray        # [nup (+ ndown)][N] trial rows (COLLINEAR: up rows, then down)
     nup: int
@@ -147,29 +147,31 @@ class System:
     nact: int = 0
     occa: np.ndarray | None = None   # PHMSD: sorted active occupations [ndet][nup] ...
     occb: np.ndarray | None = None   # ... [ndet][ndown]
-    ci: np.ndarray | None = None     # coefficients of the sorted strings
+    ci: np.ndarray | None = None     # PHMSD: coefficients of the sorted strings; NOMSD: ci
+    dets: np.ndarray | None = None   # NOMSD with several determinants: [ndet][nup (+ ndown)][N]
 
     def walkers(self, W: int):
-        """W copies of the trial determinant (Phi = PsiT^H per spin block)."""
-        up = np.ascontiguousarray(self.PsiT[:self.nup].conj().T)
+        """W copies of the trial determinant (Phi = PsiT^H per spin block); for several NOMSD
+        determinants the one of largest |ci| (NOMSD::initial_guess, NOMSD.cpp:131-153)."""
+        rows = self.PsiT if self.dets is None else self.dets[int(np.argmax(np.abs(self.ci)))]
+        up = np.ascontiguousarray(rows[:self.nup].conj().T)
         Pa = np.broadcast_to(up, (W,) + up.shape).copy()
         if self.walker_type == CLOSED:
             return Pa, None
-        dn = np.ascontiguousarray(self.PsiT[self.nup:self.nup + self.ndown].conj().T)
+        dn = np.ascontiguousarray(rows[self.nup:self.nup + self.ndown].conj().T)
         return Pa, np.broadcast_to(dn, (W,) + dn.shape).copy()
 
 
 def _one_body(h: Hamiltonian) -> np.ndarray:
-    h1 = h.h1[0]
-    for s in range(1, h.h1.shape[0]):
-        if not np.array_equal(h.h1[s], h1):
-            raise H5Error("spin-dependent one-body Hamiltonians are not supported")
-    return np.ascontiguousarray(h1)
+    """hcore as [N][N] when its spin blocks are equal, else [2][N][N] (alpha, beta)."""
+    if all(np.array_equal(h.h1[s], h.h1[0]) for s in range(1, h.h1.shape[0])):
+        return np.ascontiguousarray(h.h1[0])
+    return np.ascontiguousarray(h.h1)
 
 
 def load_system(ham_path: str, wfn_path: str, walker_type: int | None = None) -> System:
-    """A NOMSD (single determinant) or PHMSD trial with its Hamiltonian.  walker_type defaults to
-    the trial's spin_type; a CLOSED single determinant may be run as COLLINEAR walkers
+    """A NOMSD (one or several determinants) or PHMSD trial with its Hamiltonian.  walker_type
+    defaults to the trial's spin_type; a CLOSED trial may be run as COLLINEAR walkers
     (readWfn.h:129-140 duplicates the orbitals)."""
     wf = read_wavefunction(wfn_path)
     wt = wf.walker_type if walker_type is None else walker_type
@@ -180,19 +182,23 @@ def load_system(ham_path: str, wfn_path: str, walker_type: int | None = None) ->
     if ham.N != wf.N:
         raise H5Error(f"Hamiltonian has {ham.N} orbitals, the trial {wf.N}")
     h1 = _one_body(ham)
+    if wt == CLOSED and (h1.ndim == 3 or ham.Likn.ndim == 4):
+        raise H5Error("spin-dependent integrals need COLLINEAR walkers")
     if wf.kind == NOMSD:
-        if len(wf.ci) != 1:
-            raise H5Error("NOMSD trials with more than one determinant are not supported")
-        if wf.walker_type == CLOSED and wt == CLOSED:
-            PsiT = wf.psit[0]
-        elif wf.walker_type == CLOSED and wt == COLLINEAR:
-            PsiT = np.concatenate([wf.psit[0], wf.psit[0]])
-        elif wf.walker_type == COLLINEAR and wt == COLLINEAR:
-            PsiT = np.concatenate([wf.psit[0], wf.psit[1]])
-        else:
+        # PsiT_k: determinant k (CLOSED), or k = 2 d + spin (COLLINEAR) (readWfn.h:90-140)
+        def rows(d):
+            if wf.walker_type == CLOSED and wt == CLOSED:
+                return wf.psit[d]
+            if wf.walker_type == CLOSED and wt == COLLINEAR:
+                return np.concatenate([wf.psit[d], wf.psit[d]])
+            if wf.walker_type == COLLINEAR and wt == COLLINEAR:
+                return np.concatenate([wf.psit[2 * d], wf.psit[2 * d + 1]])
             raise H5Error("a COLLINEAR trial cannot drive CLOSED walkers")
-        return System(h1=h1, Likn=ham.Likn, E0=ham.E0, PsiT=np.ascontiguousarray(PsiT),
-                      nup=wf.nup, ndown=wf.ndown if wt == COLLINEAR else 0, walker_type=wt)
+        ndet = len(wf.ci)
+        dets = np.ascontiguousarray(np.stack([rows(d) for d in range(ndet)])) if ndet > 1 else None
+        return System(h1=h1, Likn=ham.Likn, E0=ham.E0, PsiT=np.ascontiguousarray(rows(0)),
+                      nup=wf.nup, ndown=wf.ndown if wt == COLLINEAR else 0, walker_type=wt,
+                      ci=wf.ci.copy(), dets=dets)
     return _phmsd_system(ham, h1, wf)
 
 

@@ -3,7 +3,7 @@ tests/golden/<molecule>/ and extracts their statistical reference results
 (tests/functional/statistical_references/<molecule>/<hamiltonian>/<trial>/<walkers>/results.h5:
 the run parameters from `input_file`, mean / error of Energy, its components and Overlap, and of
 the mixed and back-propagated OneRDM and DiagonalTwoRDM) with h5dump:
-  BH  19 orbitals, 78 Cholesky vectors, RHF / UHF / CASCI-PHMSD trials
+  BH  19 orbitals, 78 Cholesky vectors, RHF / UHF / CASCI-PHMSD / CASCI-NOMSD trials
   N2  28 orbitals, 134 Cholesky vectors, a 50-determinant CAS trial (PHMSD)
   Li  14 orbitals, 70 Cholesky vectors, a fully polarized ROHF trial (3 + 0 electrons)
 Run where the reference's sources are; GPU runs only read the copies.
@@ -22,12 +22,16 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 MOLECULES = {
     "BH": {
         "inputs": ["afqmc_H_rhf_closed.h5", "afqmc_H_rhf_collinear.h5", "afqmc_rhf_nomsd.h5",
-                   "afqmc_uhf_nomsd.h5", "afqmc_casci_rhf_phmsd.h5", "afqmc_casci_uhf_phmsd.h5"],
+                   "afqmc_uhf_nomsd.h5", "afqmc_casci_rhf_phmsd.h5", "afqmc_casci_uhf_phmsd.h5",
+                   "afqmc_casci_rhf_nomsd.h5", "afqmc_casci_uhf_nomsd.h5"],
         "cases": [("dense_rhf_basis_closed", "rhf_nomsd", "closed"),
                   ("dense_rhf_basis_closed", "rhf_nomsd", "collinear"),
                   ("dense_rhf_basis_closed", "uhf_nomsd", "collinear"),
                   ("dense_rhf_basis_closed", "rcasci_rhf_phmsd", "collinear"),
-                  ("dense_rhf_basis_closed", "rcasci_uhf_phmsd", "collinear")],
+                  ("dense_rhf_basis_closed", "rcasci_uhf_phmsd", "collinear"),
+                  ("dense_rhf_basis_closed", "rcasci_rhf_nomsd", "closed"),
+                  ("dense_rhf_basis_closed", "rcasci_rhf_nomsd", "collinear"),
+                  ("dense_rhf_basis_closed", "rcasci_uhf_nomsd", "collinear")],
     },
     "N2": {
         "inputs": ["cas_basis_hamil.h5", "cas_wfn.h5"],

@@ -5,8 +5,10 @@ electrons, a 50-determinant CAS trial) and Li (14 orbitals, 70 vectors, a fully 
 trial with 3+0 electrons).
 
 - Deterministic: the local energy at the BH RHF trial equals the closed-form Hartree-Fock energy
-  of the integrals (1e-10), and the PHMSD (CASCI, 44 determinants) overlap, density matrix and
-  local energy of perturbed walkers equal the explicit sum over determinants of the oracle.
+  of the integrals (1e-10), the PHMSD (CASCI, 44 determinants) overlap, density matrix and
+  local energy of perturbed walkers equal the explicit sum over determinants of the oracle, and
+  the NOMSD form of the UHF CASCI trial (34 non-orthogonal determinants) gives the overlaps and
+  local energies of its PHMSD form.
 - Statistical: seeded hybrid AFQMC runs at the reference's parameters (dt = 0.01, 1536 walkers,
   3000 equilibration steps, measurements, population control and orthogonalization every 10
   steps, the reference's estimator blocks) agree with the reference's statistical results
@@ -106,9 +108,59 @@ def test_bh_phmsd_vs_explicit_sum(gpu):
     assert np.abs(afq.host(ov2) - afq.host(ov)).max() < 1e-12
 
 
+def test_bh_casci_nomsd_equals_phmsd_form(gpu):
+    """The reference holds the UHF CASCI trial of BH twice: as 34 non-orthogonal determinants
+    (NOMSD) and as particle-hole excitations (PHMSD).  The device NOMSD trial's log overlap,
+    mixed density matrix, vbias and local energy on perturbed walkers equal the oracle's PHMSD
+    determinant sum (overlaps up to the common normalisation, 1e-10)."""
+    afq, ctx, torch = gpu
+    from safire_amd.nomsd import NOMSD
+    say("case: BH CASCI UHF NOMSD (34 determinants) vs its PHMSD form")
+    sn = h5inputs.load_system(os.path.join(G, "afqmc_H_rhf_closed.h5"),
+                              os.path.join(G, "afqmc_casci_uhf_nomsd.h5"))
+    sp = h5inputs.load_system(os.path.join(G, "afqmc_H_rhf_closed.h5"),
+                              os.path.join(G, "afqmc_casci_uhf_phmsd.h5"))
+    ham = afq.HamiltonianOperations(ctx, sn.h1, sn.Likn, sn.PsiT, 3, 3,
+                                    walker_type=afq.AFQ_COLLINEAR, E0=sn.E0)
+    trial = NOMSD(ham, afq.device_tensor(sn.dets, np.complex128), sn.ci)
+    W = 6
+    rng = np.random.default_rng(11)
+    Pa = sn.PsiT[:3].conj().T[None] + 0.2 * (rng.standard_normal((W, 19, 3)) +
+                                             1j * rng.standard_normal((W, 19, 3)))
+    Pb = sn.PsiT[3:].conj().T[None] + 0.2 * (rng.standard_normal((W, 19, 3)) +
+                                             1j * rng.standard_normal((W, 19, 3)))
+    ws = afq.WalkerSet(Pa, Pb)
+    orbs = sp.PsiT[:sp.nact]
+    L1, h1 = orc.half_rotate(orbs, sp.Likn, sp.h1)
+    Gb, Eb, ovb = orc.msd_brute(orbs, sp.occa, sp.occb, sp.ci, np.concatenate([h1, h1]),
+                                np.concatenate([L1, L1]), Pa, Pb, E0=sp.E0)
+    ov = afq.empty(W, torch.complex128)
+    trial.Log_Overlap(ws, ov)
+    eo = np.abs(np.exp(afq.host(ov) - ovb) - 1).max()
+    E = afq.empty((W, 3), torch.complex128)
+    ov2 = afq.empty(W, torch.complex128)
+    trial.Energy(ws, E, ov2)
+    eE = np.abs(afq.host(E) - Eb).max() / np.abs(Eb).max()
+    # full density matrix: the PHMSD compact G rotated back to the orbital basis
+    ws.ovlp.copy_(ov)
+    Gd = afq.empty((W, 2 * 19 * 19), torch.complex128)
+    Ov = afq.empty(W, torch.complex128)
+    trial.MixedDensityMatrix(ws, Gd, Ov)
+    Gfull = np.einsum("ai,wsak->wsik", orbs, Gb.reshape(W, 2, sp.nact, 19)).reshape(W, -1)
+    eG = np.abs(afq.host(Gd) - Gfull).max() / np.abs(Gfull).max()
+    v = afq.empty((W, sn.Likn.shape[2]), torch.complex128)
+    trial.vbias(ws, v, 0.01)
+    vr = orc.vbias_full(sn.Likn, Gfull, 0.01, 1.0)
+    ev = np.abs(afq.host(v) - vr).max() / np.abs(vr).max()
+    say(f"  overlap {eo:.2e}  E {eE:.2e}  G {eG:.2e}  vbias {ev:.2e}")
+    assert max(eo, eE, eG, ev) < 1e-10
+    trial.close()
+
+
 CASES = [("BH", "rhf_nomsd/closed", "fp64"), ("BH", "rhf_nomsd/closed", "mixed"),
          ("BH", "uhf_nomsd/collinear", "fp64"), ("BH", "rcasci_rhf_phmsd/collinear", "fp64"),
-         ("BH", "rcasci_uhf_phmsd/collinear", "fp64"), ("N2", "cas_wfn_collinear/collinear", "fp64"),
+         ("BH", "rcasci_uhf_phmsd/collinear", "fp64"), ("BH", "rcasci_rhf_nomsd/closed", "fp64"),
+         ("BH", "rcasci_uhf_nomsd/collinear", "fp64"), ("N2", "cas_wfn_collinear/collinear", "fp64"),
          ("Li", "rohf_wfn_polarized/collinear", "fp64")]
 SCALARS = ("Energy", "OnebodyEnergy", "ExchangeEnergy", "CoulombEnergy", "Overlap")
 
@@ -128,6 +180,9 @@ def run_case(afq, ctx, molecule, case, prec, measured_steps):
     if s.ph is None:
         ham = afq.HamiltonianOperations(ctx, s.h1, s.Likn, s.PsiT, s.nup, s.ndown, walker_type=wt,
                                         E0=s.E0, precision=p)
+        if s.dets is not None:
+            from safire_amd.nomsd import NOMSD
+            trial = NOMSD(ham, afq.device_tensor(s.dets, np.complex128), s.ci)
         Pa, Pb = s.walkers(W)
     else:
         ham = afq.HamiltonianOperations(ctx, s.h1, s.Likn, s.PsiT, s.nact, s.nact,

@@ -307,3 +307,92 @@ def test_nomsd_rejects_mismatched_first_determinant(gpu):
     rows = np.stack([cs.dets[1][0], cs.dets[0][0]])
     with pytest.raises(afq.AfqError):
         NOMSD(ham, afq.device_tensor(rows, np.complex128), cs.ci)
+
+
+@pytest.mark.parametrize("closed", [True, False])
+def test_nomsd_measurements_vs_oracle(gpu, closed):
+    """Every determinant is a reference of the mixed and back-propagated OneRDM /
+    DiagonalTwoRDM (NOMSD::getReferences, NOMSD.cpp:103-129): the device measurements equal the
+    oracle's sums over the references of density matrices weighted by conj(c_d) times the
+    overlap (times e^{conj(log det R)} after back-propagation).  CLOSED walkers take the alpha
+    overlap once in the mixed estimator and doubled, with log det R, in the back-propagated one
+    (MixedEstimator.hpp:61-75, BackPropEstimator.hpp:62-81)."""
+    afq, ctx, torch = gpu
+    W, seed, nback, nbp, ortho, dt = 8, 41, 5, 4, 2, 0.005
+    cs = make_case(closed, ndet=4)
+    say(f"case: NOMSD measurements closed={closed} ndet=4 W={W}, window {nbp}, QR every {ortho}")
+    ham, trial = build(afq, ctx, torch, cs)
+    prop = afq.Propagator(ham, W, dt=dt, seed=seed, trial=trial)
+    assert prop.number_of_references() == len(cs.ci)
+    prop.enable_backprop(nback)
+    ws = afq.WalkerSet(*cs.walkers(W, seed=3, noise=0.3))
+    E = afq.empty((W, 3), torch.complex128)
+    ov = afq.empty(W, torch.complex128)
+    prop.Energy(ws, E, ov)
+    ws.eloc.copy_(E.sum(1))
+    ws.ovlp.copy_(ov)
+    Esh = afq.device_tensor(np.array([float(afq.host(E.sum(1)).real.mean())]))
+    prop.set_step(0)
+    prop.Propagate(ws, Esh)
+    prop.bp_anchor(ws)
+    anchors = [afq.host(ws.Phi)] + ([] if closed else [afq.host(ws.Phi_b)])
+    for _ in range(nbp):
+        prop.Propagate(ws, Esh)
+    step = 1 + nbp
+    Ps = [afq.host(ws.Phi)] + ([] if closed else [afq.host(ws.Phi_b)])
+    w = afq.host(ws.weight)
+    refs = [[a] + ([] if closed else [b]) for a, b in cs.dets]
+
+    def measure(Gs, los, weights, scale):
+        lo = scale * np.array(los)
+        x = np.conj(cs.ci)[:, None] * np.exp(lo - lo.real.max(0))
+        cf = x / x.sum(0)
+        one = two = 0
+        for d, G in enumerate(Gs):
+            o, t = orc.rdm_measure(G, weights, closed, cf[d])
+            one, two = one + o, two + t
+        return one, two
+
+    # mixed estimator
+    Gs, los = [], []
+    for r in refs:
+        blocks = [orc.mixed_density_matrix(r[sp], Ps[sp], compact=False) for sp in range(len(r))]
+        Gs.append(np.stack([g for g, _ in blocks], axis=1))
+        los.append(sum(o for _, o in blocks))
+    m1, m2 = measure(Gs, los, w, 1.0)
+    d1, d2 = prop.mixed_measure(ws)
+    e1, e2 = rel(afq.host(d1), m1), rel(afq.host(d2), m2)
+    say(f"  mixed: OneRDM {e1:.2e}  DiagonalTwoRDM {e2:.2e}")
+    # back-propagation estimator
+    fields, facs = prop.bp_rings(W)
+    P1 = afq.host(prop.P1())
+    Vs = [orc.vhs(cs.p.Likn, fields[:, (step - (nbp - ni)) % nback], dt) for ni in range(nbp)]
+    Gs, los = [], []
+    for r in refs:
+        Ls = [np.broadcast_to(x.conj().T, (W,) + x.conj().T.shape).copy() for x in r]
+        ldR = np.zeros(W, np.complex128)
+        for ni in range(nbp - 1, -1, -1):
+            for sp in range(len(r)):
+                Ls[sp] = orc.propagate(P1, Vs[ni], Ls[sp], 6, "H")
+                if ni == 0 or ni % ortho == 0:
+                    Ls[sp], ld = orc.orthogonalize(Ls[sp])
+                    ldR += ld
+        G = np.empty((W, len(r), cs.p.N, cs.p.N), np.complex128)
+        lo = np.conj(ldR).copy()
+        for sp in range(len(r)):
+            for k in range(W):
+                A, B = Ls[sp][k], anchors[sp][k]
+                T = A.conj().T @ B
+                G[k, sp] = (B @ np.linalg.solve(T, A.conj().T)).T
+                sg, la = np.linalg.slogdet(T)
+                lo[k] += np.log(sg) + la
+        Gs.append(G)
+        los.append(lo)
+    wr = w * np.prod([facs[:, (step - 1 - k) % (3 * nback)] for k in range(nbp)], axis=0)
+    b1, b2 = measure(Gs, los, wr, 2.0 if closed else 1.0)
+    d1, d2 = prop.bp_measure(ws, nbp, ortho=ortho, restore=1)
+    e3, e4 = rel(afq.host(d1), b1), rel(afq.host(d2), b2)
+    say(f"  back-propagated: OneRDM {e3:.2e}  DiagonalTwoRDM {e4:.2e}")
+    prop.close()
+    trial.close()
+    assert max(e1, e2, e3, e4) < TOL

@@ -91,3 +91,45 @@ def test_phmsd_structure(wfn, ndet):
 def test_missing_file_is_an_error():
     with pytest.raises(h5inputs.H5Error):
         h5inputs.read_wavefunction(os.path.join(G, "does_not_exist.h5"))
+
+
+def test_casci_nomsd_loads_every_determinant():
+    for wfn, wt, rows in (("afqmc_casci_rhf_nomsd.h5", h5inputs.CLOSED, 3),
+                          ("afqmc_casci_uhf_nomsd.h5", h5inputs.COLLINEAR, 6)):
+        s = h5inputs.load_system(os.path.join(G, "afqmc_H_rhf_closed.h5"), os.path.join(G, wfn))
+        assert s.walker_type == wt and s.dets.shape == (34, rows, 19) and len(s.ci) == 34
+        assert np.array_equal(s.dets[0], s.PsiT)
+    # a CLOSED expansion run as COLLINEAR walkers repeats each determinant's orbitals per spin
+    c = h5inputs.load_system(os.path.join(G, "afqmc_H_rhf_closed.h5"),
+                             os.path.join(G, "afqmc_casci_rhf_nomsd.h5"),
+                             walker_type=h5inputs.COLLINEAR)
+    assert c.dets.shape == (34, 6, 19) and np.array_equal(c.dets[:, :3], c.dets[:, 3:])
+
+
+def test_casci_uhf_nomsd_is_the_casci_uhf_phmsd_state():
+    """The reference's UHF CASCI trial is held twice, as 34 non-orthogonal determinants
+    (PsiT_{2d + spin}) and as 34 particle-hole excitations: the NOMSD restatement (determinant
+    sum of log overlaps, per-determinant half-rotated energies) and the pinned PHMSD one agree on
+    generic walkers, which fixes the determinant order, the spin interleaving and the weights."""
+    sn = h5inputs.load_system(os.path.join(G, "afqmc_H_rhf_closed.h5"),
+                              os.path.join(G, "afqmc_casci_uhf_nomsd.h5"))
+    sp = h5inputs.load_system(os.path.join(G, "afqmc_H_rhf_closed.h5"),
+                              os.path.join(G, "afqmc_casci_uhf_phmsd.h5"))
+    dets = [(d[:3], d[3:]) for d in sn.dets]
+    rng = np.random.default_rng(1)
+    W = 4
+
+    def walkers(rows):
+        return np.stack([np.linalg.qr(rows.conj().T + 0.3 * (rng.normal(size=(19, 3)) +
+                                                              1j * rng.normal(size=(19, 3))))[0]
+                         for _ in range(W)])
+    Pa, Pb = walkers(sn.PsiT[:3]), walkers(sn.PsiT[3:])
+    ovn = orc.msd_log_overlap(dets, sn.ci, Pa, Pb)
+    orbs = sp.PsiT[:sp.nact]
+    L1, h1 = orc.half_rotate(orbs, sp.Likn, sp.h1)
+    _, Eb, ovb = orc.msd_brute(orbs, sp.occa, sp.occb, sp.ci, np.concatenate([h1, h1]),
+                               np.concatenate([L1, L1]), Pa, Pb, E0=sp.E0)
+    assert np.abs(np.exp(ovn - ovb) - 1).max() < 1e-12
+    En, _ = orc.msd_energy(dets, sn.ci, orc.msd_half_rotated(dets, sn.Likn, sn.h1), Pa, Pb,
+                           E0=sn.E0)
+    assert np.abs(En - Eb).max() < 1e-10 * np.abs(Eb).max()
