@@ -23,6 +23,7 @@
 #include <cstdlib>
 #include <vector>
 
+#include "crgemm.hpp"
 #include "gemm.hpp"
 #include "ops.hpp"
 #include "tuner.hpp"
@@ -258,6 +259,7 @@ void CholeskyOps::vbias(const z64* G, int W, double dt, z64* v, Precision prec) 
 
 void CholeskyOps::vHS(const void* X, int W, double dt, void* V, Precision prec) const {
   if (W == 0) return;
+  hipStream_t s = ctx_.stream();
   const double a = std::sqrt(dt);
   const long NN = (long)N_ * N_;
   const size_t esz = prec == kFP64 ? sizeof(z64) : sizeof(c32);
@@ -278,11 +280,19 @@ void CholeskyOps::vHS(const void* X, int W, double dt, void* V, Precision prec) 
                                                              0, N_, nb});
       } else {
         AFQ_CHECK(prec_ == kMixed, kErrState, "FP32 tensors were not built (precision FP64)");
-        gemm::RevK<Opnd<float, true, true>> A({reinterpret_cast<const float*>(X), M_, 0}, M_);
-        gemm::RevK<gemm::OpndSymPairs<float>> B({Likn32_ + loff, M_, 0, N_, nb}, M_);
-        gemm_large<float>(ctx_, "vhs_sym", 1, W, npairs, M_, A, B,
-                          gemm::EpiStoreSym<float, float>{reinterpret_cast<c32*>(Vl), NN, 0,
-                                                          (float)a, 0.f, N_, nb});
+        gemm::EpiStoreSym<float, float> e{reinterpret_cast<c32*>(Vl), NN, 0, (float)a, 0.f, N_, nb};
+        const c32* X32 = reinterpret_cast<const c32*>(X);
+        if (crg_path() && crg::crg_supported(X32, M_, W, Likn32_ + loff, M_, NN)) {
+          // register-tiled: 64 walkers x 112 pairs per wavefront, k groups from the last
+          // Cholesky vector to the first
+          crg::launch_crgemm<4, 7, true>(s, W, npairs, M_, X32, M_, Likn32_ + loff,
+                                         crg::RowsSymPairs{M_, N_, nb}, 1, ceil_div(M_, 16) * 16,
+                                         e, nullptr);
+        } else {
+          gemm::RevK<Opnd<float, true, true>> A({reinterpret_cast<const float*>(X), M_, 0}, M_);
+          gemm::RevK<gemm::OpndSymPairs<float>> B({Likn32_ + loff, M_, 0, N_, nb}, M_);
+          gemm_large<float>(ctx_, "vhs_sym", 1, W, npairs, M_, A, B, e);
+        }
       }
       continue;
     }
@@ -357,6 +367,13 @@ void CholeskyOps::vbias_full(const z64* G, int W, double dt, z64* v, Precision p
 // Kl is given, Kl[w][m] += tr T^m.  Gblk [W][ns][N] contiguous.  The Cholesky vectors are taken
 // in chunks whose T [W][ns][MC][ns] fits 1 GiB (max_memory_MB of energy_impl,
 // Real3IndexFactorization.hpp:511-587), sized for the larger spin block so both share the buffer.
+// The register-tiled FP32 products (crgemm.hpp) for vHS and vbias; AFQ_CRGEMM=0 selects the
+// LDS-staged GEMM engine instead.
+bool CholeskyOps::crg_path() {
+  const char* e = std::getenv("AFQ_CRGEMM");
+  return !(e && e[0] == '0');
+}
+
 bool CholeskyOps::exchange_fused_path(Precision prec, bool real) const {
   const char* e = std::getenv("AFQ_EXX_FUSED");  // "0": the GEMM + reduction path
   return prec != kFP64 && real && exx_fused_supported(std::max(nup_, ndown_), N_, M_) &&

@@ -1,0 +1,239 @@
+// Register-tiled FP32 product of a complex operand and a real operand, both K-contiguous:
+//
+//   C(m, n) = sum_k A(m, k) * B(n, k)      A complex64 [M][lda], B real [N rows][K]
+//
+// the shape of vHS (A = the fields X [W][M], B = the Cholesky vectors' symmetric pairs) and of
+// vbias (A = the walkers' G, B = the half-rotated Cholesky vectors).  Each wavefront owns a
+// 16 RB x 16 CB tile of C in MFMA accumulators (v_mfma_f32_16x16x4f32, 2 MFMAs per block:
+// re(A) B and im(A) B) and streams its operands from memory straight into MFMA fragments: lane
+// (fr, fk) of k group g feeds MFMA step j with k = 16 g + 4 fk + j, four consecutive k values
+// per lane, so each fragment load is one 16-byte word (two for complex A).  There is no LDS and
+// no barrier: per k group a wavefront issues 2 RB + CB loads and 8 RB CB MFMAs, the next group's
+// loads issued before the current group's MFMAs (full groups load unconditionally, from rows
+// clamped into range, so the compiler waits for them only after the MFMAs).
+//
+// Rows past M and columns past N (or symmetric pairs outside the N x N space) read a clamped,
+// valid row and are never stored.  Operands are addressed by 32-bit byte offsets from uniform
+// bases (crg_supported checks the sizes).  With REVK the k groups run from last to first (the
+// ragged group first), so an accumulation over Cholesky vectors ordered by decreasing norm adds
+// the small terms first.  Split-K slices write FP32 partials that kreduce_splitk sums in slice
+// order; every result depends on the shape only.
+#pragma once
+
+#include "gemm.hpp"
+
+namespace afq {
+namespace crg {
+
+using gemm::f32x4;
+
+constexpr int kWaves = 4;
+
+// B rows: plain rows of a [rows][ldb] array
+struct RowsPlain {
+  long ldb;  // elements between rows
+  __device__ inline bool row(int n, int N, long& off) const {
+    off = (long)n * ldb;
+    return n < N;
+  }
+};
+
+// B rows: the symmetric pairs (i, k) of a [N*N][ldb] array, in the order of gemm::sym_pair
+struct RowsSymPairs {
+  long ldb;
+  int Nn, nb;
+  __device__ inline bool row(int n, int, long& off) const {
+    int i, k;
+    const bool ok = gemm::sym_pair(n, nb, Nn, i, k);
+    off = ok ? ((long)i * Nn + k) * ldb : 0;
+    return ok;
+  }
+};
+
+template <int RB, int CB>
+struct Frag {
+  float ar[RB][4], ai[RB][4];  // A rows: re and im of four k values
+  float b[CB][4];              // B rows
+};
+
+template <int RB, int CB>
+struct Ofs {
+  uint32_t a[RB], b[CB];  // byte offsets of the lane's rows at its first k value
+};
+
+// the fragments of one k group; ab / bb: the operands' bases advanced to the group's first k.
+// FULL: unconditional loads; else rem = K - (the lane's first k) masks k >= K to zero.
+template <int RB, int CB, bool FULL>
+__device__ inline void load(Frag<RB, CB>& f, const char* ab, const char* bb, const Ofs<RB, CB>& o,
+                            int rem) {
+  const float4 z = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int i = 0; i < RB; ++i) {
+    float4 x = z, y = z;
+    if (FULL || rem >= 4) {
+      x = *reinterpret_cast<const float4*>(ab + o.a[i]);
+      y = *reinterpret_cast<const float4*>(ab + o.a[i] + 16);
+    } else {
+      float2 t[4] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (rem > j) t[j] = *reinterpret_cast<const float2*>(ab + o.a[i] + 8 * j);
+      x = float4{t[0].x, t[0].y, t[1].x, t[1].y};
+      y = float4{t[2].x, t[2].y, t[3].x, t[3].y};
+    }
+    f.ar[i][0] = x.x; f.ai[i][0] = x.y; f.ar[i][1] = x.z; f.ai[i][1] = x.w;
+    f.ar[i][2] = y.x; f.ai[i][2] = y.y; f.ar[i][3] = y.z; f.ai[i][3] = y.w;
+  }
+#pragma unroll
+  for (int jb = 0; jb < CB; ++jb) {
+    float4 x = z;
+    if (FULL || rem >= 4) {
+      x = *reinterpret_cast<const float4*>(bb + o.b[jb]);
+    } else {
+      float t[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (rem > j) t[j] = *reinterpret_cast<const float*>(bb + o.b[jb] + 4 * j);
+      x = float4{t[0], t[1], t[2], t[3]};
+    }
+    f.b[jb][0] = x.x; f.b[jb][1] = x.y; f.b[jb][2] = x.z; f.b[jb][3] = x.w;
+  }
+}
+
+template <int RB, int CB>
+__device__ inline void mma(const Frag<RB, CB>& f, f32x4 (&cr)[RB][CB], f32x4 (&ci)[RB][CB]) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+#pragma unroll
+      for (int jb = 0; jb < CB; ++jb) {
+        cr[i][jb] = __builtin_amdgcn_mfma_f32_16x16x4f32(f.ar[i][j], f.b[jb][j], cr[i][jb], 0, 0, 0);
+        ci[i][jb] = __builtin_amdgcn_mfma_f32_16x16x4f32(f.ai[i][j], f.b[jb][j], ci[i][jb], 0, 0, 0);
+      }
+}
+
+// One wavefront per tile (tile m fastest, then n, then the K slice); the workgroups are
+// renumbered so that each XCD takes a contiguous run of tiles, i.e. shares the B rows of a few
+// column tiles through its L2.
+template <int RB, int CB, bool REVK, class BROWS, class EPI>
+__global__ void __launch_bounds__(64 * kWaves)
+    k_crgemm(int M, int N, int K, const c32* __restrict__ A, long lda,
+             const float* __restrict__ B, BROWS brows, int tm, int tn, int ksplit, int kchunk,
+             EPI epi) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int fr = lane & 15, fk = lane >> 4;
+  const long total = gridDim.x, xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
+  const long wg = xcd * (total / 8) + min(xcd, total % 8) + slot;
+  const long q = wg * kWaves + wave;
+  if (q >= (long)tm * tn * ksplit) return;  // uniform per wavefront; no barrier in the kernel
+  const int im = (int)(q % tm), in = (int)((q / tm) % tn), ks = (int)(q / ((long)tm * tn));
+  const int m0 = 16 * RB * im, n0 = 16 * CB * in;
+  const int k0 = ks * kchunk, kn = min(K - k0, kchunk);  // this slice: k0 .. k0 + kn - 1
+  Ofs<RB, CB> o;
+#pragma unroll
+  for (int i = 0; i < RB; ++i) {
+    const int r = min(m0 + 16 * i + fr, M - 1);
+    o.a[i] = (uint32_t)(((long)r * lda + k0 + 4 * fk) * (long)sizeof(c32));
+  }
+#pragma unroll
+  for (int jb = 0; jb < CB; ++jb) {
+    long off;
+    const int c = n0 + 16 * jb + fr;
+    if (!brows.row(min(c, N - 1), N, off)) brows.row(0, N, off);  // a valid row stands in
+    o.b[jb] = (uint32_t)((off + k0 + 4 * fk) * (long)sizeof(float));
+  }
+  f32x4 cr[RB][CB], ci[RB][CB];
+#pragma unroll
+  for (int i = 0; i < RB; ++i)
+#pragma unroll
+    for (int jb = 0; jb < CB; ++jb) {
+      cr[i][jb] = f32x4{0.f, 0.f, 0.f, 0.f};
+      ci[i][jb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+  const char* ab = reinterpret_cast<const char*>(A);
+  const char* bb = reinterpret_cast<const char*>(B);
+  constexpr long AS = 16 * sizeof(c32), BS = 16 * sizeof(float);  // bytes per k group
+  const int NGF = kn / 16;                                      // full k groups of the slice
+  const bool tail = NGF * 16 < kn;
+  Frag<RB, CB> cur, nxt;
+  if (REVK && tail) {  // the ragged (last) group first
+    load<RB, CB, false>(cur, ab + NGF * AS, bb + NGF * BS, o, kn - 16 * NGF - 4 * fk);
+    mma<RB, CB>(cur, cr, ci);
+  }
+  if (NGF > 0) {
+    // group order: 0 .. NGF - 1, or NGF - 1 .. 0 with REVK
+    auto gid = [&](int t) { return REVK ? NGF - 1 - t : t; };
+    load<RB, CB, true>(cur, ab + gid(0) * AS, bb + gid(0) * BS, o, 0);
+    for (int t = 0; t < NGF; ++t) {
+      const int gn = gid(min(t + 1, NGF - 1));  // the last group reloads itself
+      load<RB, CB, true>(nxt, ab + gn * AS, bb + gn * BS, o, 0);
+      // the prefetch stays ahead of the MFMAs and the copy behind them
+      __builtin_amdgcn_sched_barrier(0);
+      mma<RB, CB>(cur, cr, ci);
+      __builtin_amdgcn_sched_barrier(0);
+      cur = nxt;
+    }
+  }
+  if (!REVK && tail) {
+    load<RB, CB, false>(cur, ab + NGF * AS, bb + NGF * BS, o, kn - 16 * NGF - 4 * fk);
+    mma<RB, CB>(cur, cr, ci);
+  }
+#pragma unroll
+  for (int i = 0; i < RB; ++i)
+#pragma unroll
+    for (int jb = 0; jb < CB; ++jb) {
+      const int n = n0 + 16 * jb + fr;
+      if (n >= N) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + 16 * i + 4 * fk + r;
+        if (m < M) epi(ks, m, n, cr[i][jb][r], ci[i][jb][r]);
+      }
+    }
+}
+
+// The kernel's requirements: 32-bit byte offsets reach every row of A and B, and the fragment
+// loads are 16-byte aligned (A rows of an even number of complex elements, B rows of a multiple
+// of 4 floats, the bases 16-byte aligned).
+inline bool crg_supported(const void* A, long lda, long a_rows, const void* B, long ldb,
+                          long b_rows) {
+  return a_rows * lda * (long)sizeof(c32) < (1L << 31) &&
+         b_rows * ldb * (long)sizeof(float) < (1L << 31) && lda % 2 == 0 && ldb % 4 == 0 &&
+         reinterpret_cast<uintptr_t>(A) % 16 == 0 && reinterpret_cast<uintptr_t>(B) % 16 == 0;
+}
+
+// C = epi(A B^T) over M x N with K cut into ksplit slices of kchunk (a multiple of 16); more than
+// one slice writes FP32 partials into ws [ksplit][M][N] and sums them in slice order (in FP64)
+// before the epilogue.
+template <int RB, int CB, bool REVK, class BROWS, class EPI>
+void launch_crgemm(hipStream_t s, int M, int N, int K, const c32* A, long lda, const float* B,
+                   const BROWS& brows, int ksplit, int kchunk, const EPI& epi, cplx<float>* ws) {
+  if (M <= 0 || N <= 0) return;
+  AFQ_CHECK(K > 0 && kchunk % 16 == 0 && (long)kchunk * ksplit >= K &&
+                (ksplit == 1 || (long)kchunk * (ksplit - 1) < K),
+            kErrArg, "bad K slicing");
+  const int tm = ceil_div(M, 16 * RB), tn = ceil_div(N, 16 * CB);
+  const long tiles = (long)tm * tn * ksplit;
+  const long blocks = ceil_div(tiles, (long)kWaves);
+  AFQ_CHECK(blocks < (1L << 31), kErrShape, "grid too large");
+  if (ksplit == 1) {
+    hipLaunchKernelGGL((k_crgemm<RB, CB, REVK, BROWS, EPI>), dim3((unsigned)blocks),
+                       dim3(64 * kWaves), 0, s, M, N, K, A, lda, B, brows, tm, tn, 1, kchunk, epi);
+    AFQ_LAUNCH_CHECK();
+    return;
+  }
+  AFQ_CHECK(ws != nullptr, kErrArg, "split-K workspace");
+  gemm::EpiPartial<float> ep{ws, (long)M * N, N};
+  hipLaunchKernelGGL((k_crgemm<RB, CB, REVK, BROWS, gemm::EpiPartial<float>>),
+                     dim3((unsigned)blocks), dim3(64 * kWaves), 0, s, M, N, K, A, lda, B, brows, tm,
+                     tn, ksplit, kchunk, ep);
+  AFQ_LAUNCH_CHECK();
+  const long tot = (long)M * N;
+  hipLaunchKernelGGL((gemm::kreduce_splitk<float, EPI>), dim3((unsigned)ceil_div(tot, 256L)),
+                     dim3(256), 0, s, 1, M, N, ksplit, ws, epi);
+  AFQ_LAUNCH_CHECK();
+}
+
+}  // namespace crg
+}  // namespace afq

@@ -221,6 +221,7 @@ class CholeskyOps {
   void exchange_spin(int sp, const z64* Gblk, int W, z64* part, z64* Kl, Precision prec,
                      const Rotated& t) const;
   void half_rotate_into(const z64* PsiT, Rotated& r) const;
+  static bool crg_path();
   // the fused on-chip exchange applies (FP32, real half-rotated tensor, n <= 40)
   bool exchange_fused_path(Precision prec, bool real = true) const;
   // exchange partials per walker and spin block of the chosen path

@@ -195,7 +195,9 @@ def test_every_tile_candidate_is_bit_identical(bench, precision):
     prefix = "f32" if prec == afq.AFQ_MIXED else "f64"
     big = [s for s in sites if s.startswith(("vhs", "vbias", "exx", "prop_fused"))]
     say(f"  sites: {sites}")
-    assert any(s.startswith("vhs_sym:" + prefix) for s in big)
+    # the FP32 symmetric vHS runs the register-tiled kernel, which has no tile candidates
+    if prec == afq.AFQ_FP64:
+        assert any(s.startswith("vhs_sym:" + prefix) for s in big)
     assert any(s.startswith("prop_fused:" + prefix) for s in big)
     assert any(s.startswith("exx_chunk") for s in big)
     runs = 0
