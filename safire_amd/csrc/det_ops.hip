@@ -226,12 +226,16 @@ struct AFrag {
   T im[CPLX ? 4 : 1];
 };
 
-// A[row][k .. k + 3] (complex elements interleaved); rows >= rows or k >= K read as zero.
-// vec: the rows of A are 16-byte aligned, so in-range fragments use vector loads.
-template <typename T, bool CPLX>
+// A[row][k .. k + 3] (complex elements interleaved).  FULL (k + 3 < K): unconditional loads,
+// rows >= rows read row rows - 1 (their products land in output rows that write_b zeroes and
+// the final store skips), so the compiler counts the loads in flight and the next groups' loads
+// overlap this group's MFMAs.  Otherwise rows >= rows or k >= K read as zero.  VEC: the rows of
+// A are 16-byte aligned, so fragments use vector loads.
+template <typename T, bool CPLX, bool VEC, bool FULL>
 __device__ inline void load_afrag(AFrag<T, CPLX>& f, const T* __restrict__ A, long lda, int row,
-                                  int rows, int k, int K, bool vec) {
-  if (vec && row < rows && k + 3 < K) {
+                                  int rows, int k, int K) {
+  if (FULL) row = min(row, rows - 1);
+  if (VEC && (FULL || (row < rows && k + 3 < K))) {
     if constexpr (CPLX && sizeof(T) == 4) {
       const float4* p = reinterpret_cast<const float4*>(A + 2 * ((long)row * lda + k));
       const float4 x = p[0], y = p[1];
@@ -256,7 +260,7 @@ __device__ inline void load_afrag(AFrag<T, CPLX>& f, const T* __restrict__ A, lo
   } else {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const bool in = row < rows && k + j < K;
+      const bool in = FULL || (row < rows && k + j < K);
       const long e = (long)row * lda + k + j;
       if constexpr (CPLX) {
         f.re[j] = in ? A[2 * e] : (T)0;
@@ -327,20 +331,22 @@ __device__ inline void group_mfma(const AFrag<T, CPLX> (&f)[SPW], int g, int NS,
   }
 }
 
-template <typename T, bool CPLX, int SPW>
+// a stripe past the last one loads the last one (its products are never used), so every load
+// of a group is issued unconditionally
+template <typename T, bool CPLX, int SPW, bool VEC, bool FULL>
 __device__ inline void load_group(AFrag<T, CPLX> (&f)[SPW], const T* __restrict__ A, long lda,
-                                  bool vec, int N, int NS, int NW, int wave, int fr, int fk, int g) {
-  if (g >= NS) return;
+                                  int N, int NS, int NW, int wave, int fr, int fk, int g) {
 #pragma unroll
   for (int s = 0; s < SPW; ++s)
-    if (wave + s * NW < NS) load_afrag(f[s], A, lda, (wave + s * NW) * 16 + fr, N, 16 * g + 4 * fk, N, vec);
+    load_afrag<T, CPLX, VEC, FULL>(f[s], A, lda, min(wave + s * NW, NS - 1) * 16 + fr, N,
+                                   16 * g + 4 * fk, N);
 }
 
 // acc = A[stripe rows][0:N] * B[0:N][tiles] for the wave's stripes (complex B in LDS). The A
 // fragments of the next two k groups are in flight while a group is multiplied.  M3: complex
 // products by Gauss's three-multiplication method (three MFMAs per complex block instead of four).
-template <typename T, bool CPLX, int SPW, int CB, bool CONJ = false, bool M3 = false>
-__device__ inline void stripes_times_b(const T* __restrict__ A, long lda, bool vec, int N, int NS,
+template <typename T, bool CPLX, int SPW, int CB, bool VEC, bool CONJ = false, bool M3 = false>
+__device__ inline void stripes_times_b(const T* __restrict__ A, long lda, int N, int NS,
                                        int NW, int wave, int fr, int fk, const T* Br,
                                        const T* Bi, int LDK,
                                        typename gemm::Acc<T>::v4 (&ar)[SPW][CB],
@@ -357,17 +363,33 @@ __device__ inline void stripes_times_b(const T* __restrict__ A, long lda, bool v
       a3[s][t] = V4{0, 0, 0, 0};
     }
   AFrag<T, CPLX> f0[SPW], f1[SPW], f2[SPW];
-  load_group(f0, A, lda, vec, N, NS, NW, wave, fr, fk, 0);
-  load_group(f1, A, lda, vec, N, NS, NW, wave, fr, fk, 1);
-  for (int g = 0; g < NS; g += 3) {
-    load_group(f2, A, lda, vec, N, NS, NW, wave, fr, fk, g + 2);
-    group_mfma<T, CPLX, SPW, CB, CONJ, G3>(f0, g, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai, a3);
-    if (g + 1 >= NS) break;
-    load_group(f0, A, lda, vec, N, NS, NW, wave, fr, fk, g + 3);
-    group_mfma<T, CPLX, SPW, CB, CONJ, G3>(f1, g + 1, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai, a3);
-    if (g + 2 >= NS) break;
-    load_group(f1, A, lda, vec, N, NS, NW, wave, fr, fk, g + 4);
-    group_mfma<T, CPLX, SPW, CB, CONJ, G3>(f2, g + 2, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai, a3);
+  // the full k groups: two groups' loads in flight while one is multiplied (prefetches past the
+  // last full group reload it rather than branching around the load); the scheduling barriers
+  // keep each group's loads ahead of the previous group's MFMAs
+  const int NGF = N / 16;
+  if (NGF > 0) {
+    load_group<T, CPLX, SPW, VEC, true>(f0, A, lda, N, NS, NW, wave, fr, fk, 0);
+    load_group<T, CPLX, SPW, VEC, true>(f1, A, lda, N, NS, NW, wave, fr, fk, min(1, NGF - 1));
+    for (int g = 0; g < NGF; g += 3) {
+      load_group<T, CPLX, SPW, VEC, true>(f2, A, lda, N, NS, NW, wave, fr, fk, min(g + 2, NGF - 1));
+      __builtin_amdgcn_sched_barrier(0);
+      group_mfma<T, CPLX, SPW, CB, CONJ, G3>(f0, g, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai, a3);
+      __builtin_amdgcn_sched_barrier(0);
+      if (g + 1 >= NGF) break;
+      load_group<T, CPLX, SPW, VEC, true>(f0, A, lda, N, NS, NW, wave, fr, fk, min(g + 3, NGF - 1));
+      __builtin_amdgcn_sched_barrier(0);
+      group_mfma<T, CPLX, SPW, CB, CONJ, G3>(f1, g + 1, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai, a3);
+      __builtin_amdgcn_sched_barrier(0);
+      if (g + 2 >= NGF) break;
+      load_group<T, CPLX, SPW, VEC, true>(f1, A, lda, N, NS, NW, wave, fr, fk, min(g + 4, NGF - 1));
+      __builtin_amdgcn_sched_barrier(0);
+      group_mfma<T, CPLX, SPW, CB, CONJ, G3>(f2, g + 2, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai, a3);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  if (NGF < NS) {  // the ragged last k group, masked
+    load_group<T, CPLX, SPW, VEC, false>(f0, A, lda, N, NS, NW, wave, fr, fk, NGF);
+    group_mfma<T, CPLX, SPW, CB, CONJ, G3>(f0, NGF, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai, a3);
   }
   if constexpr (G3) {
     // p1 = Ar Br, p2 = Ai Bi, p3 = (Ar +- Ai)(Br + Bi):
@@ -386,10 +408,11 @@ __device__ inline void stripes_times_b(const T* __restrict__ A, long lda, bool v
   }
 }
 
-// B[column][row] <- the wave's stripes of (xr, xi)
+// B[column][row] <- the wave's stripes of (xr, xi); rows >= N (products of clamped A rows) are
+// stored as zero, so they contribute nothing to the next product
 template <typename T, int SPW, int CB>
-__device__ inline void write_b(T* Br, T* Bi, int LDK, int NS, int NW, int wave, int fr, int fk,
-                               const typename gemm::Acc<T>::v4 (&xr)[SPW][CB],
+__device__ inline void write_b(T* Br, T* Bi, int LDK, int N, int NS, int NW, int wave, int fr,
+                               int fk, const typename gemm::Acc<T>::v4 (&xr)[SPW][CB],
                                const typename gemm::Acc<T>::v4 (&xi)[SPW][CB]) {
 #pragma unroll
   for (int s = 0; s < SPW; ++s) {
@@ -400,16 +423,24 @@ __device__ inline void write_b(T* Br, T* Bi, int LDK, int NS, int NW, int wave, 
       const int off = (16 * t + fr) * LDK + r0;
       if constexpr (sizeof(T) == 4) {
         // FP32 accumulator register r holds row 4 (lane / 16) + r: four consecutive rows
-        *reinterpret_cast<float4*>(Br + off + 4 * fk) =
-            float4{xr[s][t][0], xr[s][t][1], xr[s][t][2], xr[s][t][3]};
-        *reinterpret_cast<float4*>(Bi + off + 4 * fk) =
-            float4{xi[s][t][0], xi[s][t][1], xi[s][t][2], xi[s][t][3]};
+        const int rb = r0 + 4 * fk;
+        float4 vr = float4{xr[s][t][0], xr[s][t][1], xr[s][t][2], xr[s][t][3]};
+        float4 vi = float4{xi[s][t][0], xi[s][t][1], xi[s][t][2], xi[s][t][3]};
+        if (rb + 3 >= N) {
+          if (rb >= N) vr.x = vi.x = 0.f;
+          if (rb + 1 >= N) vr.y = vi.y = 0.f;
+          if (rb + 2 >= N) vr.z = vi.z = 0.f;
+          vr.w = vi.w = 0.f;
+        }
+        *reinterpret_cast<float4*>(Br + off + 4 * fk) = vr;
+        *reinterpret_cast<float4*>(Bi + off + 4 * fk) = vi;
       } else {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int row = gemm::Acc<T>::row(fk, r);
-          Br[off + row] = xr[s][t][r];
-          Bi[off + row] = xi[s][t][r];
+          const bool in = r0 + row < N;
+          Br[off + row] = in ? xr[s][t][r] : (T)0;
+          Bi[off + row] = in ? xi[s][t][r] : (T)0;
         }
       }
     }
@@ -418,10 +449,10 @@ __device__ inline void write_b(T* Br, T* Bi, int LDK, int NS, int NW, int wave, 
 
 // One workgroup per (walker, CB 16-column tiles); NW wavefronts.  CONJV: the V products use
 // conj(V), which is V^H for the complex-symmetric V of vHS (the adjoint propagator, sgn = -1).
-template <typename T, bool P1C, int SPW, int CB, bool CONJV, bool M3>
+template <typename T, bool P1C, int SPW, int CB, bool CONJV, bool M3, bool VEC>
 __global__ void __launch_bounds__(1024)
-    k_prop_taylor(int W, int N, int n, int order, double sgn, const T* __restrict__ P1, int vecP,
-                  const T* __restrict__ V, long v_bs, int vecV, z64* __restrict__ Phi, long phi_bs,
+    k_prop_taylor(int W, int N, int n, int order, double sgn, const T* __restrict__ P1,
+                  const T* __restrict__ V, long v_bs, z64* __restrict__ Phi, long phi_bs,
                   int NW, int LDK) {
   using V4 = typename gemm::Acc<T>::v4;
   constexpr int BC = 16 * CB;
@@ -434,7 +465,9 @@ __global__ void __launch_bounds__(1024)
   const long bid = blockIdx.x, xcd = bid % 8, slot = bid / 8;
   const long q = xcd * (total / 8) + min(xcd, total % 8) + slot;
   const int w = (int)(q / nct), c0 = (int)(q % nct) * BC;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // the wavefront index is uniform across the wavefront: held in a scalar register, the stripe
+  // tests below are scalar branches rather than lane masks
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int fr = lane & 15, fk = lane >> 4;
   z64* Pw = Phi + (long)w * phi_bs;
   const T* Vw = V + 2 * (long)w * v_bs;
@@ -449,7 +482,7 @@ __global__ void __launch_bounds__(1024)
 
   V4 ar[SPW][CB], ai[SPW][CB], sr[SPW][CB], si[SPW][CB];
   // T_0 = P1 Phi, S = T_0
-  stripes_times_b<T, P1C, SPW, CB>(P1, N, vecP, N, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai);
+  stripes_times_b<T, P1C, SPW, CB, VEC>(P1, N, N, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai);
 #pragma unroll
   for (int s = 0; s < SPW; ++s)
 #pragma unroll
@@ -458,10 +491,10 @@ __global__ void __launch_bounds__(1024)
       si[s][t] = ai[s][t];
     }
   __syncthreads();
-  write_b<T, SPW, CB>(Br, Bi, LDK, NS, NW, wave, fr, fk, ar, ai);
+  write_b<T, SPW, CB>(Br, Bi, LDK, N, NS, NW, wave, fr, fk, ar, ai);
   __syncthreads();
   for (int k = 1; k <= order; ++k) {
-    stripes_times_b<T, true, SPW, CB, CONJV, M3>(Vw, N, vecV, N, NS, NW, wave, fr, fk, Br, Bi,
+    stripes_times_b<T, true, SPW, CB, VEC, CONJV, M3>(Vw, N, N, NS, NW, wave, fr, fk, Br, Bi,
                                                  LDK, ar, ai);
     // T_k = (i sgn / k) V T_{k-1}; S += T_k
     const T fi = (T)(sgn / (double)k);
@@ -478,13 +511,13 @@ __global__ void __launch_bounds__(1024)
           si[s][t][r] = si[s][t][r] + ti;
         }
     __syncthreads();
-    write_b<T, SPW, CB>(Br, Bi, LDK, NS, NW, wave, fr, fk, ar, ai);
+    write_b<T, SPW, CB>(Br, Bi, LDK, N, NS, NW, wave, fr, fk, ar, ai);
     __syncthreads();
   }
-  write_b<T, SPW, CB>(Br, Bi, LDK, NS, NW, wave, fr, fk, sr, si);
+  write_b<T, SPW, CB>(Br, Bi, LDK, N, NS, NW, wave, fr, fk, sr, si);
   __syncthreads();
   // Phi = P1 S
-  stripes_times_b<T, P1C, SPW, CB>(P1, N, vecP, N, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai);
+  stripes_times_b<T, P1C, SPW, CB, VEC>(P1, N, N, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai);
 #pragma unroll
   for (int s = 0; s < SPW; ++s) {
     if (wave + s * NW >= NS) continue;
@@ -517,13 +550,14 @@ template <typename T, bool P1C, int SPW, int CB, bool CONJV, bool M3>
 void launch_fused(hipStream_t s, const PropagateArgs& a, const FusedShape& f, const T* P1, int vecP,
                   const T* V, int vecV, z64* Phi) {
   const size_t lds = 2 * 16 * CB * (size_t)f.LDK * sizeof(T);
-  auto kern = k_prop_taylor<T, P1C, SPW, CB, CONJV, M3>;
+  auto kern = (vecP && vecV) ? k_prop_taylor<T, P1C, SPW, CB, CONJV, M3, true>
+                             : k_prop_taylor<T, P1C, SPW, CB, CONJV, M3, false>;
   if (lds > 64 * 1024)
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   const long blocks = (long)a.W * ceil_div(a.n, 16 * CB);
   hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(64 * f.NW), lds, s, a.W, a.N, a.n, a.order,
-                     CONJV ? -1.0 : 1.0, P1, vecP, V, a.v_bs, vecV, Phi, a.phi_bs, f.NW, f.LDK);
+                     CONJV ? -1.0 : 1.0, P1, V, a.v_bs, Phi, a.phi_bs, f.NW, f.LDK);
   AFQ_LAUNCH_CHECK();
 }
 
