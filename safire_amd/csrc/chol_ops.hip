@@ -235,7 +235,15 @@ void CholeskyOps::vbias(const z64* G, int W, double dt, z64* v, Precision prec) 
       // A(w, (a,k)) = G[w][off + a][k]: rows of length nel*N, K = ns*N contiguous
       Opnd<double, true, true> A{&(G + goff)->re, (long)nel_ * N_, 0};
       EpiStore<double, double> e{v, M_, 0, a, 0, acc};
-      if (real_trial_) {
+      if (real_trial_ && crg_path() &&
+          crg::crg_supported<double>(G + goff, (long)nel_ * N_, W, Lnak_r_ + loff, K, M_)) {
+        // register-tiled, K cut into slices so that the 64 x 48 tiles fill the machine
+        int ks, kc;
+        crg::crg_split(W, M_, (int)K, 4, 3, ks, kc);
+        z64* ws = ks > 1 ? scratch_as<z64>(ctx_, "crg_ws64", (size_t)ks * W * M_) : nullptr;
+        crg::launch_crgemm<double, 4, 3, false>(s, W, M_, (int)K, G + goff, (long)nel_ * N_,
+                                                Lnak_r_ + loff, crg::RowsPlain{K}, ks, kc, e, ws);
+      } else if (real_trial_) {
         Opnd<double, false, true> B{Lnak_r_ + loff, K, 0};
         gemm_large_splitk<double>(ctx_, "vbias_r", W, M_, (int)K, A, B, e, acc == 0);
       } else {
@@ -273,21 +281,29 @@ void CholeskyOps::vHS(const void* X, int W, double dt, void* V, Precision prec) 
       const int nb = ceil_div(N_, 8);
       const int npairs = nb * (nb + 1) / 2 * 64;
       if (prec == kFP64) {
-        gemm::RevK<Opnd<double, true, true>> A({reinterpret_cast<const double*>(X), M_, 0}, M_);
-        gemm::RevK<gemm::OpndSymPairs<double>> B({Likn_ + loff, M_, 0, N_, nb}, M_);
-        gemm_large<double>(ctx_, "vhs_sym", 1, W, npairs, M_, A, B,
-                           gemm::EpiStoreSym<double, double>{reinterpret_cast<z64*>(Vl), NN, 0, a,
-                                                             0, N_, nb});
+        gemm::EpiStoreSym<double, double> e{reinterpret_cast<z64*>(Vl), NN, 0, a, 0, N_, nb};
+        const z64* X64 = reinterpret_cast<const z64*>(X);
+        if (crg_path() && crg::crg_supported<double>(X64, M_, W, Likn_ + loff, M_, NN)) {
+          // register-tiled: 64 walkers x 48 pairs per wavefront (the tile count fills the
+          // machine's last round of wavefront slots), k from the last Cholesky vector
+          crg::launch_crgemm<double, 4, 3, true>(s, W, npairs, M_, X64, M_, Likn_ + loff,
+                                                 crg::RowsSymPairs{M_, N_, nb}, 1,
+                                                 ceil_div(M_, 16) * 16, e, nullptr);
+        } else {
+          gemm::RevK<Opnd<double, true, true>> A({reinterpret_cast<const double*>(X), M_, 0}, M_);
+          gemm::RevK<gemm::OpndSymPairs<double>> B({Likn_ + loff, M_, 0, N_, nb}, M_);
+          gemm_large<double>(ctx_, "vhs_sym", 1, W, npairs, M_, A, B, e);
+        }
       } else {
         AFQ_CHECK(prec_ == kMixed, kErrState, "FP32 tensors were not built (precision FP64)");
         gemm::EpiStoreSym<float, float> e{reinterpret_cast<c32*>(Vl), NN, 0, (float)a, 0.f, N_, nb};
         const c32* X32 = reinterpret_cast<const c32*>(X);
-        if (crg_path() && crg::crg_supported(X32, M_, W, Likn32_ + loff, M_, NN)) {
+        if (crg_path() && crg::crg_supported<float>(X32, M_, W, Likn32_ + loff, M_, NN)) {
           // register-tiled: 64 walkers x 112 pairs per wavefront, k groups from the last
           // Cholesky vector to the first
-          crg::launch_crgemm<4, 7, true>(s, W, npairs, M_, X32, M_, Likn32_ + loff,
-                                         crg::RowsSymPairs{M_, N_, nb}, 1, ceil_div(M_, 16) * 16,
-                                         e, nullptr);
+          crg::launch_crgemm<float, 4, 7, true>(s, W, npairs, M_, X32, M_, Likn32_ + loff,
+                                                crg::RowsSymPairs{M_, N_, nb}, 1,
+                                                ceil_div(M_, 16) * 16, e, nullptr);
         } else {
           gemm::RevK<Opnd<float, true, true>> A({reinterpret_cast<const float*>(X), M_, 0}, M_);
           gemm::RevK<gemm::OpndSymPairs<float>> B({Likn32_ + loff, M_, 0, N_, nb}, M_);
@@ -330,7 +346,16 @@ void CholeskyOps::vbias32(const c32* G, int W, double dt, z64* v) const {
     const long K = (long)ns * N_;
     Opnd<float, true, true> A{&(G + goff)->re, (long)nel_ * N_, 0};
     EpiStore<float, double> e{v, M_, 0, a, 0.f, acc};
-    if (real_trial_) {
+    if (real_trial_ && crg_path() &&
+        crg::crg_supported<float>(G + goff, (long)nel_ * N_, W, Lnak_r32_ + loff, K, M_)) {
+      // register-tiled, K cut into slices so that the 64 x 112 tiles fill the machine
+      int ks, kc;
+      crg::crg_split(W, M_, (int)K, 4, 7, ks, kc);
+      cplx<float>* ws =
+          ks > 1 ? scratch_as<cplx<float>>(ctx_, "crg_ws32", (size_t)ks * W * M_) : nullptr;
+      crg::launch_crgemm<float, 4, 7, false>(s, W, M_, (int)K, G + goff, (long)nel_ * N_,
+                                             Lnak_r32_ + loff, crg::RowsPlain{K}, ks, kc, e, ws);
+    } else if (real_trial_) {
       Opnd<float, false, true> B{Lnak_r32_ + loff, K, 0};
       gemm_large_splitk<float>(ctx_, "vbias32_r", W, M_, (int)K, A, B, e, acc == 0);
     } else {
@@ -367,8 +392,8 @@ void CholeskyOps::vbias_full(const z64* G, int W, double dt, z64* v, Precision p
 // Kl is given, Kl[w][m] += tr T^m.  Gblk [W][ns][N] contiguous.  The Cholesky vectors are taken
 // in chunks whose T [W][ns][MC][ns] fits 1 GiB (max_memory_MB of energy_impl,
 // Real3IndexFactorization.hpp:511-587), sized for the larger spin block so both share the buffer.
-// The register-tiled FP32 products (crgemm.hpp) for vHS and vbias; AFQ_CRGEMM=0 selects the
-// LDS-staged GEMM engine instead.
+// The register-tiled products of crgemm.hpp (FP32 and FP64) for vHS and vbias; AFQ_CRGEMM=0
+// selects the LDS-staged GEMM engine instead.
 bool CholeskyOps::crg_path() {
   const char* e = std::getenv("AFQ_CRGEMM");
   return !(e && e[0] == '0');

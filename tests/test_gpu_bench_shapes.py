@@ -195,9 +195,7 @@ def test_every_tile_candidate_is_bit_identical(bench, precision):
     prefix = "f32" if prec == afq.AFQ_MIXED else "f64"
     big = [s for s in sites if s.startswith(("vhs", "vbias", "exx", "prop_fused"))]
     say(f"  sites: {sites}")
-    # the FP32 symmetric vHS runs the register-tiled kernel, which has no tile candidates
-    if prec == afq.AFQ_FP64:
-        assert any(s.startswith("vhs_sym:" + prefix) for s in big)
+    # vHS and vbias run the register-tiled kernel (crgemm.hpp), which has no tile candidates
     assert any(s.startswith("prop_fused:" + prefix) for s in big)
     assert any(s.startswith("exx_chunk") for s in big)
     runs = 0
@@ -223,8 +221,10 @@ def test_every_tile_candidate_is_bit_identical(bench, precision):
         assert per_walker_rel(ref[4][SAMPLE], Er) < TOL
 
 
-def test_forced_candidate_out_of_range_is_an_error(bench):
+def test_forced_candidate_out_of_range_is_an_error(bench, monkeypatch):
     afq, torch, p, osys, Phi, st_s, Esh = bench
+    # the tuned site is the LDS-staged GEMM engine's vHS; the register-tiled product is untuned
+    monkeypatch.setenv("AFQ_CRGEMM", "0")
     from safire_amd.afq import AfqError
     ctx = afq.Context(0)
     ham = afq.HamiltonianOperations(ctx, p.h1, p.Likn, p.PsiT, p.nup, precision=afq.AFQ_FP64)
