@@ -61,6 +61,36 @@ def vhs_flops_per_walker(N, M):
     return 2.0 * N * (N + 1) * M
 
 
+def phmsd_flops_per_walker_step(N, M, spins, ndet, real_L=True, K=ORDER, k_ortho=ORTHO):
+    """FLOP of one hybrid step of one COLLINEAR walker with the particle-hole trial, counted
+    from the drawn excitation list.  spins: per spin (n electrons, nact active orbitals, ranks of
+    the unique excited configurations).  Per spin, the density matrix (PHMSD::MixedDensityMatrix)
+    costs the Woodbury set-up TAB = orbs Phi (nact n N complex MACs), the LU and inverse of its
+    n x n reference block (n^3), QQ0 (nvir n^2) and GA (n^2 N); per excitation of rank r the
+    LU of its r x r block, that block's inverse (r^2 (r - 1)) and its compact R rows (r^2 nact);
+    the R assembly (r nact per excitation) and G = R^T GA (nact n N).  The step's closing
+    Log_Overlap repeats the Woodbury set-up and the excitation LUs.  The spin-coupled weights
+    take one complex MAC per determinant and spin, in both calls.  vbias contracts the nact x N
+    G with the half-rotated Cholesky vectors (complex x real with a real trial: 4 FLOP per MAC),
+    vHS, the Taylor propagation and the QR share (every k_ortho steps) are the single
+    determinant's.  Complex MACs count 8 FLOP."""
+    def lu(r):
+        return sum((r - 1 - k) ** 2 for k in range(r))
+    f = vhs_flops_per_walker(N, M) + 2 * 2 * 8 * ndet
+    for n, na, ranks in spins:
+        ranks = [int(r) for r in ranks if r > 0]
+        nvir = na - n
+        wood = 8 * (na * n * N + n ** 3 + nvir * n * n)
+        mdm = (wood + 8 * n * n * N + 8 * sum(lu(r) + r * r * (r - 1) + r * r * na + r * na
+                                              for r in ranks) + 8 * na * n * N)
+        ovl = wood + 8 * sum(lu(r) for r in ranks)
+        vb = (4 if real_L else 8) * M * na * N
+        prop = 8 * K * N * N * n + 4 * 2 * N * N * n
+        qr = 16 * n * n * N / k_ortho
+        f += mdm + ovl + vb + prop + qr
+    return f
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -442,10 +472,16 @@ def phmsd_bench(warmup, steps, energy_reps=5):
     torch.cuda.synchronize()
     te = (time.perf_counter() - t0) / energy_reps
     Emean = float(afq.host(E.sum(1)).real.mean())
+    fstep = phmsd_flops_per_walker_step(
+        P.N, P.M, [(P.nup, P.nact, ph.rank[0]), (P.ndn, P.nact, ph.rank[1])], ph.ndet,
+        real_L=not np.iscomplexobj(P.orbs) or not np.any(np.imag(P.orbs)))
     return {"workload": "N=64, 10+10 electrons, M=320, 2000 determinants (ranks <= 4, "
                         f"{ph.nconf[0]}+{ph.nconf[1]} unique configurations), 256 walkers, FP64",
             "value": Wp / sec, "unit": "walker-steps/s", "ms_per_step": sec * 1e3,
-            "energy_ms": te * 1e3, "mean_local_energy": Emean}
+            "energy_ms": te * 1e3, "mean_local_energy": Emean,
+            # FLOP of the step from the drawn excitation list, over the FP64 MFMA peak
+            "step_flop_per_walker": fstep,
+            "step_roofline_frac": fstep * Wp / sec / PEAK["fp64"]}
 
 
 def main():
@@ -643,7 +679,9 @@ def main():
         # achieved: the vHS launch's average duration inside the replayed step graph (HIP
         # events on the library's stream); the same kernel launched back to back, alone, is
         # reported beside it
-        "roofline": {"bound": "mfma", "kernel": "vHS GEMM (FP32 MFMA, [W x M] . [M x N^2])",
+        "roofline": {"bound": "mfma",
+                     "kernel": "vHS: k_crgemm, register-tiled FP32 MFMA, X [W x M] . the "
+                               "N(N+1)/2 symmetric pairs of L [M], V mirrored",
                      "achieved": achieved / 1e12, "peak": PEAK["fp32"] / 1e12,
                      "vhs_ms_in_step": tv * 1e3,
                      "vhs_ms_back_to_back": out["mixed"]["vhs_b2b"] * 1e3,

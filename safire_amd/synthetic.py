@@ -36,11 +36,11 @@ class Workload:
 
 # The four fixed workloads; the serving loop draws its own (see safire_amd.serve).
 WORKLOADS = {
-    "cfg1": Workload("cfg1: FP64 N=24 n=5+5 M=96 W=64", 24, 5, 96, 64, 100, seed=1001),
-    "cfg2": Workload("cfg2: FP64 N=108 n=24+24 M=600 W=512", 108, 24, 600, 512, 200, seed=1002),
-    "cfg3": Workload("cfg3: mixed N=200 n=40+40 M=1600 W=1024", 200, 40, 1600, 1024, 300, seed=1003),
+    "fp64_N24": Workload("FP64 N=24 n=5+5 M=96 W=64", 24, 5, 96, 64, 100, seed=1001),
+    "fp64_N108": Workload("FP64 N=108 n=24+24 M=600 W=512", 108, 24, 600, 512, 200, seed=1002),
+    "mixed_N200": Workload("mixed N=200 n=40+40 M=1600 W=1024", 200, 40, 1600, 1024, 300, seed=1003),
     # COLLINEAR, particle-hole multi-determinant trial of 2000 determinants (make_phmsd_problem)
-    "cfg4": Workload("cfg4: PHMSD N=64 n=10+10 M=320 W=256 ndet=2000", 64, 10, 320, 256, 100,
+    "phmsd_N64": Workload("PHMSD N=64 n=10+10 M=320 W=256 ndet=2000", 64, 10, 320, 256, 100,
                      seed=1004),
 }
 

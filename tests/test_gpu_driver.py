@@ -62,7 +62,7 @@ def test_driver_matches_oracle_trajectory():
     # weaker interactions (lscale 0.1) keep a phaseless population alive for long runs
     p = make_problem(24, 5, 96, seed=1001, lscale=0.1)
     W, steps, seed = 32, 31, 4242
-    say(f"case: driver vs oracle driver cfg1 W={W} steps={steps}")
+    say(f"case: driver vs oracle driver N=24 n=5 M=96 W={W} steps={steps}")
     ref, esh = oracle_driver(p, W, steps, seed)
     tr = device_driver(p, W, steps, seed, equilibration=10, ortho_interval=5,
                        popctl_interval=10, measure_interval=5)
@@ -83,7 +83,7 @@ def test_mixed_trace_within_error_bar():
     from safire_amd import afq
     p = make_problem(24, 5, 96, seed=1001, lscale=0.1)
     W, steps, seed = 256, 400, 777
-    say(f"case: mixed vs FP64 energy trace cfg1 W={W} steps={steps}")
+    say(f"case: mixed vs FP64 energy trace N=24 n=5 M=96 W={W} steps={steps}")
     res = {}
     for prec in (afq.AFQ_FP64, afq.AFQ_MIXED):
         tr = device_driver(p, W, steps, seed, precision=prec, equilibration=50, ortho_interval=5,

@@ -50,7 +50,7 @@ def test_local_energy_graph_matches_eager(gpu, precision):
     p = make_problem(24, 5, 96, seed=1001)
     W = 32
     prec = afq.AFQ_FP64 if precision == "fp64" else afq.AFQ_MIXED
-    say(f"case: local-energy graph vs eager, {precision}, cfg1 W=32, 9 steps, FP64 Energy between")
+    say(f"case: local-energy graph vs eager, {precision}, N=24 n=5 M=96 W=32, 9 steps, FP64 Energy between")
     out = {}
     for mode in ("eager", "graph"):
         c = afq.Context(0)
@@ -154,7 +154,7 @@ def test_mixed_local_energy_step_error(gpu):
     afq, torch = gpu
     p = make_problem(24, 5, 96, seed=1001)
     W = 32
-    say("case: mixed vs FP64 local-energy step, cfg1 W=32")
+    say("case: mixed vs FP64 local-energy step, N=24 n=5 M=96 W=32")
     res = {}
     for prec in (afq.AFQ_FP64, afq.AFQ_MIXED):
         c = afq.Context(0)
@@ -178,7 +178,7 @@ def test_bound_statistics(gpu):
     afq, torch = gpu
     p = make_problem(24, 5, 96, seed=1001)
     W = 40
-    say("case: bound statistics cfg1 W=40, vbias_bound 0.05")
+    say("case: bound statistics N=24 n=5 M=96 W=40, vbias_bound 0.05")
     ctx = afq.Context(0)
     ham = afq.HamiltonianOperations(ctx, p.h1, p.Likn, p.PsiT, p.nup)
     prop = afq.Propagator(ham, W, dt=0.005, vbias_bound=0.05)

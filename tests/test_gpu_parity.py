@@ -187,7 +187,7 @@ def test_orthogonalize_vs_oracle(gpu, N, n, W):
 
 # ---- hamiltonian operations ------------------------------------------------------------------
 
-@pytest.fixture(scope="module", params=[(24, 5, 96), (108, 24, 600)], ids=["cfg1", "cfg2"])
+@pytest.fixture(scope="module", params=[(24, 5, 96), (108, 24, 600)], ids=["N24_n5_M96", "N108_n24_M600"])
 def system(request, gpu):
     afq, ctx, torch = gpu
     N, n, M = request.param
@@ -370,10 +370,10 @@ def _run_pair(afq, torch, p, W, steps, seed=11, ortho_every=5, precision=0, loca
         yield step, ws, st, ctx
 
 
-def test_step_vs_oracle_cfg1(gpu):
+def test_step_vs_oracle_N24(gpu):
     afq, ctx, torch = gpu
     p = make_problem(24, 5, 96, seed=1001)
-    say("case: full step cfg1 N=24 n=5 M=96 W=64, 10 steps, QR at 0 and 5")
+    say("case: full step N=24 n=5 M=96 W=64, 10 steps, QR at 0 and 5")
     for step, ws, st, c in _run_pair(afq, torch, p, 64, 10):
         assert rel(afq.host(ws.Phi), st["Phi"]) < TOL, step
         for k in ("ovlp", "eloc", "weight", "phase", "theta"):
@@ -385,17 +385,17 @@ def test_local_energy_step_vs_oracle(gpu):
     Log_Overlap) drives the weights with the midpoint rule."""
     afq, ctx, torch = gpu
     p = make_problem(24, 5, 96, seed=1001)
-    say("case: local-energy step cfg1 N=24 n=5 M=96 W=32, 6 steps, QR at 0 and 5")
+    say("case: local-energy step N=24 n=5 M=96 W=32, 6 steps, QR at 0 and 5")
     for step, ws, st, c in _run_pair(afq, torch, p, 32, 6, local_energy=True):
         assert rel(afq.host(ws.Phi), st["Phi"]) < TOL, step
         for k in ("ovlp", "eloc", "weight", "phase", "theta"):
             assert rel(afq.host(getattr(ws, k)), st[k]) < TOL, (step, k)
 
 
-def test_step_vs_oracle_cfg2(gpu):
+def test_step_vs_oracle_N108(gpu):
     afq, ctx, torch = gpu
     p = make_problem(108, 24, 600, seed=1002)
-    say("case: full step cfg2 N=108 n=24 M=600 W=16, 2 steps")
+    say("case: full step N=108 n=24 M=600 W=16, 2 steps")
     for step, ws, st, c in _run_pair(afq, torch, p, 16, 2):
         assert rel(afq.host(ws.Phi), st["Phi"]) < TOL, step
         for k in ("ovlp", "eloc", "weight", "phase"):
@@ -445,7 +445,7 @@ def test_graph_replay_matches_eager(gpu):
     afq, ctx, torch = gpu
     p = make_problem(24, 5, 96, seed=1001)
     W = 32
-    say("case: hipGraph replay vs eager, cfg1 W=32, 7 steps")
+    say("case: hipGraph replay vs eager, N=24 n=5 M=96 W=32, 7 steps")
     out = {}
     for mode in ("eager", "graph"):
         c = afq.Context(0)
@@ -513,7 +513,7 @@ def test_graph_with_population_control_matches_eager(gpu):
     afq, ctx, torch = gpu
     p = make_problem(24, 5, 96, seed=1001)
     W = 40
-    say("case: graph step + QR + comb vs eager, cfg1 W=40, 12 steps")
+    say("case: graph step + QR + comb vs eager, N=24 n=5 M=96 W=40, 12 steps")
     out = {}
     seed = 99
     for mode in ("eager", "graph"):

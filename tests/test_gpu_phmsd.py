@@ -73,13 +73,13 @@ def setup_case(afq, ctx, **kw):
 
 SMALL = dict(N=24, nel=6, M=40, ndet=60, ncore=2, nactive=8, max_rank=4, seed=21)
 RANK6 = dict(N=28, nel=8, M=32, ndet=80, ncore=1, nactive=14, max_rank=6, seed=22)
-CFG4 = dict()   # make_phmsd_problem defaults: N=64, 10+10 electrons, M=320, 2000 determinants
+PH2000 = dict()   # make_phmsd_problem defaults: N=64, 10+10 electrons, M=320, 2000 determinants
 
 
-@pytest.mark.parametrize("case,W", [("small", 16), ("rank6", 8), ("cfg4", 12)])
+@pytest.mark.parametrize("case,W", [("small", 16), ("rank6", 8), ("ph2000", 12)])
 def test_phmsd_trial_ops(gpu, case, W):
     afq, ctx, torch = gpu
-    kw = {"small": SMALL, "rank6": RANK6, "cfg4": CFG4}[case]
+    kw = {"small": SMALL, "rank6": RANK6, "ph2000": PH2000}[case]
     say(f"case: PHMSD trial ops {case} W={W}")
     P, ph, ham, trial, ot, t = setup_case(afq, ctx, **kw)
     say("  nconf", ph.nconf, "ndet", ph.ndet, "max rank", ph.max_rank(0), ph.max_rank(1))
@@ -111,8 +111,8 @@ def test_phmsd_trial_ops(gpu, case, W):
 
 def test_phmsd_vmf(gpu):
     afq, ctx, torch = gpu
-    say("case: PHMSD vMF small + cfg4")
-    for kw in (SMALL, CFG4):
+    say("case: PHMSD vMF small + 2000 determinants")
+    for kw in (SMALL, PH2000):
         P, ph, ham, trial, ot, t = setup_case(afq, ctx, **kw)
         v = afq.empty(P.M, torch.complex128)
         trial.vMF(v, 0.005)
@@ -181,7 +181,7 @@ def test_phmsd_propagation_matches_oracle(gpu):
 def test_phmsd_graph_equals_eager(gpu):
     afq, ctx, torch = gpu
     W = 32
-    say(f"case: PHMSD graph replay vs eager cfg4-shaped small W={W}")
+    say(f"case: PHMSD graph replay vs eager N=64 2000-determinant-shaped small W={W}")
     P, ph, ham, trial, ot, t = setup_case(afq, ctx, **SMALL)
     Pa, Pb = P.walkers(W, seed=3, noise=0.2)
     out = []
