@@ -27,10 +27,11 @@ DT, ORDER, ORTHO = 0.005, 6, 5
 SEED = 1003
 PEAK = {"fp32": 157.3e12, "fp64": 78.6e12}
 HBM_PEAK = 8.0e12
-# HBM bytes of one FP32 vHS launch at this workload: FETCH_SIZE x 2 + WRITE_SIZE x 1 (the
-# factors that make a 1 GiB copy read and write 1 GiB), rocprofv3 --pmc of `bench.py --only vhs`
-# (profiles/vhs_mixed_cfg3_pmc_traffic.md).  Algorithmic: X 13.1 MB + L 128.6 MB + V 327.7 MB.
-VHS_TRAFFIC_BYTES = (615418 * 2 + 516147) * 1024
+# HBM bytes of one FP32 vHS launch (k_crgemm over the symmetric pairs) at this workload:
+# FETCH_SIZE x 2 + WRITE_SIZE x 1 in KiB (the factors that make a 1 GiB copy read and write
+# 1 GiB), rocprofv3 --pmc of `bench.py --only step`
+# (profiles/step_N200_n40_M1600_W1024_pmc.md).  Algorithmic: X 13.1 MB + L 128.6 MB + V 327.7 MB.
+VHS_TRAFFIC_BYTES = (226327 * 2 + 562803) * 1024
 VHS_ALGORITHMIC_BYTES = W * M * 8 + (N * (N + 1) // 2) * M * 4 + W * N * N * 8
 
 
