@@ -235,11 +235,12 @@ void CholeskyOps::vbias(const z64* G, int W, double dt, z64* v, Precision prec) 
       // A(w, (a,k)) = G[w][off + a][k]: rows of length nel*N, K = ns*N contiguous
       Opnd<double, true, true> A{&(G + goff)->re, (long)nel_ * N_, 0};
       EpiStore<double, double> e{v, M_, 0, a, 0, acc};
+      int ks = 0, kc = 0;
       if (real_trial_ && crg_path() &&
-          crg::crg_supported<double>(G + goff, (long)nel_ * N_, W, Lnak_r_ + loff, K, M_)) {
-        // register-tiled, K cut into slices so that the 64 x 48 tiles fill the machine
-        int ks, kc;
+          crg::crg_supported<double>(G + goff, (long)nel_ * N_, W, Lnak_r_ + loff, K, M_))
         crg::crg_split(W, M_, (int)K, 4, 3, ks, kc);
+      if (ks > 0 && crg::crg_fills(W, M_, 4, 3, ks)) {
+        // register-tiled, K cut into slices so that the 64 x 48 tiles fill the machine
         z64* ws = ks > 1 ? scratch_as<z64>(ctx_, "crg_ws64", (size_t)ks * W * M_) : nullptr;
         crg::launch_crgemm<double, 4, 3, false>(s, W, M_, (int)K, G + goff, (long)nel_ * N_,
                                                 Lnak_r_ + loff, crg::RowsPlain{K}, ks, kc, e, ws);
@@ -283,7 +284,8 @@ void CholeskyOps::vHS(const void* X, int W, double dt, void* V, Precision prec) 
       if (prec == kFP64) {
         gemm::EpiStoreSym<double, double> e{reinterpret_cast<z64*>(Vl), NN, 0, a, 0, N_, nb};
         const z64* X64 = reinterpret_cast<const z64*>(X);
-        if (crg_path() && crg::crg_supported<double>(X64, M_, W, Likn_ + loff, M_, NN)) {
+        if (crg_path() && crg::crg_supported<double>(X64, M_, W, Likn_ + loff, M_, NN) &&
+            crg::crg_fills(W, npairs, 4, 3)) {
           // register-tiled: 64 walkers x 48 pairs per wavefront (the tile count fills the
           // machine's last round of wavefront slots), k from the last Cholesky vector
           crg::launch_crgemm<double, 4, 3, true>(s, W, npairs, M_, X64, M_, Likn_ + loff,
@@ -298,7 +300,8 @@ void CholeskyOps::vHS(const void* X, int W, double dt, void* V, Precision prec) 
         AFQ_CHECK(prec_ == kMixed, kErrState, "FP32 tensors were not built (precision FP64)");
         gemm::EpiStoreSym<float, float> e{reinterpret_cast<c32*>(Vl), NN, 0, (float)a, 0.f, N_, nb};
         const c32* X32 = reinterpret_cast<const c32*>(X);
-        if (crg_path() && crg::crg_supported<float>(X32, M_, W, Likn32_ + loff, M_, NN)) {
+        if (crg_path() && crg::crg_supported<float>(X32, M_, W, Likn32_ + loff, M_, NN) &&
+            crg::crg_fills(W, npairs, 4, 7)) {
           // register-tiled: 64 walkers x 112 pairs per wavefront, k groups from the last
           // Cholesky vector to the first
           crg::launch_crgemm<float, 4, 7, true>(s, W, npairs, M_, X32, M_, Likn32_ + loff,
@@ -346,11 +349,12 @@ void CholeskyOps::vbias32(const c32* G, int W, double dt, z64* v) const {
     const long K = (long)ns * N_;
     Opnd<float, true, true> A{&(G + goff)->re, (long)nel_ * N_, 0};
     EpiStore<float, double> e{v, M_, 0, a, 0.f, acc};
+    int ks = 0, kc = 0;
     if (real_trial_ && crg_path() &&
-        crg::crg_supported<float>(G + goff, (long)nel_ * N_, W, Lnak_r32_ + loff, K, M_)) {
-      // register-tiled, K cut into slices so that the 64 x 112 tiles fill the machine
-      int ks, kc;
+        crg::crg_supported<float>(G + goff, (long)nel_ * N_, W, Lnak_r32_ + loff, K, M_))
       crg::crg_split(W, M_, (int)K, 4, 7, ks, kc);
+    if (ks > 0 && crg::crg_fills(W, M_, 4, 7, ks)) {
+      // register-tiled, K cut into slices so that the 64 x 112 tiles fill the machine
       cplx<float>* ws =
           ks > 1 ? scratch_as<cplx<float>>(ctx_, "crg_ws32", (size_t)ks * W * M_) : nullptr;
       crg::launch_crgemm<float, 4, 7, false>(s, W, M_, (int)K, G + goff, (long)nel_ * N_,

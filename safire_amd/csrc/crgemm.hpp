@@ -248,6 +248,15 @@ inline void crg_split(int M, int N, int K, int RB, int CB, int& ksplit, int& kch
   ksplit = ceil_div(K, kchunk);
 }
 
+// Whether the register-tiled kernel suits a product whose M x N outputs, in ksplit K slices,
+// form the 16 RB x 16 CB wavefront tiles: below half a round of the machine's 1024 wavefront
+// slots its 64-row tiles leave most SIMDs idle and the LDS-staged engine's smaller tuned tiles
+// are faster (vHS at N = 200, M = 1600 and 128 walkers: 360 tiles).
+inline bool crg_fills(int M, int N, int RB, int CB, int ksplit = 1) {
+  constexpr long kMinTiles = 512;
+  return (long)ceil_div(M, 16 * RB) * ceil_div(N, 16 * CB) * ksplit >= kMinTiles;
+}
+
 // C = epi(A B^T) over M x N with K cut into ksplit slices of kchunk (a multiple of 16); more than
 // one slice writes partials into ws [ksplit][M][N] and sums them in slice order (in FP64)
 // before the epilogue.

@@ -447,39 +447,16 @@ __device__ inline void write_b(T* Br, T* Bi, int LDK, int N, int NS, int NW, int
   }
 }
 
-// One workgroup per (walker, CB 16-column tiles); NW wavefronts.  CONJV: the V products use
-// conj(V), which is V^H for the complex-symmetric V of vHS (the adjoint propagator, sgn = -1).
+// One wavefront's share of the propagation of one walker: the output stripes wave + s NW
+// (s < SPW) against the CB 16-column tiles of the B operand at (Br, Bi) (LDS, [column][row],
+// pitch LDK), which hold walker columns c0, c0 + 1, ...  Every wavefront of the workgroup passes
+// the same sequence of barriers.  CONJV: the V products use conj(V), which is V^H for the
+// complex-symmetric V of vHS (the adjoint propagator, sgn = -1).
 template <typename T, bool P1C, int SPW, int CB, bool CONJV, bool M3, bool VEC>
-__global__ void __launch_bounds__(1024)
-    k_prop_taylor(int W, int N, int n, int order, double sgn, const T* __restrict__ P1,
-                  const T* __restrict__ V, long v_bs, z64* __restrict__ Phi, long phi_bs,
-                  int NW, int LDK) {
+__device__ __attribute__((always_inline)) inline void prop_share(int N, int n, int order, double sgn, const T* __restrict__ P1,
+                                  const T* __restrict__ Vw, z64* __restrict__ Pw, int c0, int NS,
+                                  int NW, int wave, int fr, int fk, T* Br, T* Bi, int LDK) {
   using V4 = typename gemm::Acc<T>::v4;
-  constexpr int BC = 16 * CB;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* Br = reinterpret_cast<T*>(smem_raw);
-  T* Bi = Br + BC * LDK;
-  const int nct = (n + BC - 1) / BC;
-  const int NS = (N + 15) / 16;
-  const long total = (long)W * nct;
-  const long bid = blockIdx.x, xcd = bid % 8, slot = bid / 8;
-  const long q = xcd * (total / 8) + min(xcd, total % 8) + slot;
-  const int w = (int)(q / nct), c0 = (int)(q % nct) * BC;
-  // the wavefront index is uniform across the wavefront: held in a scalar register, the stripe
-  // tests below are scalar branches rather than lane masks
-  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int fr = lane & 15, fk = lane >> 4;
-  z64* Pw = Phi + (long)w * phi_bs;
-  const T* Vw = V + 2 * (long)w * v_bs;
-
-  for (int t = tid; t < BC * NS * 16; t += blockDim.x) {
-    const int cc = t % BC, k = t / BC;
-    const z64 v = (k < N && c0 + cc < n) ? Pw[(long)k * n + c0 + cc] : z64{0.0, 0.0};
-    Br[cc * LDK + k] = (T)v.re;
-    Bi[cc * LDK + k] = (T)v.im;
-  }
-  __syncthreads();
-
   V4 ar[SPW][CB], ai[SPW][CB], sr[SPW][CB], si[SPW][CB];
   // T_0 = P1 Phi, S = T_0
   stripes_times_b<T, P1C, SPW, CB, VEC>(P1, N, N, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai);
@@ -531,6 +508,52 @@ __global__ void __launch_bounds__(1024)
   }
 }
 
+// One workgroup per (walker, CB 16-column tiles).  Without TAIL, NW wavefronts own the stripes
+// (SPW each).  TAIL (SPW = 1): NW, a multiple of 4, wavefronts own the first NW stripes with all
+// CB tiles, and one more wavefront per (stripe, tile) of the remaining NS - NW stripes owns that
+// single tile, so the four SIMDs carry nearly equal shares (N = 200, CB = 3: 10, 10, 10 and 9
+// stripe-tiles instead of 12, 9, 9 and 9).  Every output tile is multiplied in the same order
+// either way, so the variants give the same bits.
+template <typename T, bool P1C, int SPW, int CB, bool CONJV, bool M3, bool VEC, bool TAIL>
+__global__ void __launch_bounds__(1024)
+    k_prop_taylor(int W, int N, int n, int order, double sgn, const T* __restrict__ P1,
+                  const T* __restrict__ V, long v_bs, z64* __restrict__ Phi, long phi_bs,
+                  int NW, int LDK) {
+  constexpr int BC = 16 * CB;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* Br = reinterpret_cast<T*>(smem_raw);
+  T* Bi = Br + BC * LDK;
+  const int nct = (n + BC - 1) / BC;
+  const int NS = (N + 15) / 16;
+  const long total = (long)W * nct;
+  const long bid = blockIdx.x, xcd = bid % 8, slot = bid / 8;
+  const long q = xcd * (total / 8) + min(xcd, total % 8) + slot;
+  const int w = (int)(q / nct), c0 = (int)(q % nct) * BC;
+  // the wavefront index is uniform across the wavefront: held in a scalar register, the stripe
+  // tests below are scalar branches rather than lane masks
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int fr = lane & 15, fk = lane >> 4;
+  z64* Pw = Phi + (long)w * phi_bs;
+  const T* Vw = V + 2 * (long)w * v_bs;
+
+  for (int t = tid; t < BC * NS * 16; t += blockDim.x) {
+    const int cc = t % BC, k = t / BC;
+    const z64 v = (k < N && c0 + cc < n) ? Pw[(long)k * n + c0 + cc] : z64{0.0, 0.0};
+    Br[cc * LDK + k] = (T)v.re;
+    Bi[cc * LDK + k] = (T)v.im;
+  }
+  __syncthreads();
+  if (!TAIL || wave < NW) {
+    prop_share<T, P1C, SPW, CB, CONJV, M3, VEC>(N, n, order, sgn, P1, Vw, Pw, c0, NS, NW, wave,
+                                                fr, fk, Br, Bi, LDK);
+  } else if constexpr (TAIL) {
+    const int u = wave - NW, st = NW + u / CB, tt = u % CB;
+    prop_share<T, P1C, 1, 1, CONJV, M3, VEC>(N, n, order, sgn, P1, Vw, Pw, c0 + 16 * tt, NS, NW,
+                                             st, fr, fk, Br + 16 * tt * LDK, Bi + 16 * tt * LDK,
+                                             LDK);
+  }
+}
+
 struct FusedShape {
   int NW, spw, LDK;
 };
@@ -546,24 +569,30 @@ inline int max_lds_per_block() {
   return v;
 }
 
-template <typename T, bool P1C, int SPW, int CB, bool CONJV, bool M3>
+// TAIL: the first NS rounded down to a multiple of 4 stripes take one wavefront each, the rest
+// one wavefront per (stripe, tile)
+template <typename T, bool P1C, int SPW, int CB, bool CONJV, bool M3, bool TAIL = false>
 void launch_fused(hipStream_t s, const PropagateArgs& a, const FusedShape& f, const T* P1, int vecP,
                   const T* V, int vecV, z64* Phi) {
   const size_t lds = 2 * 16 * CB * (size_t)f.LDK * sizeof(T);
-  auto kern = (vecP && vecV) ? k_prop_taylor<T, P1C, SPW, CB, CONJV, M3, true>
-                             : k_prop_taylor<T, P1C, SPW, CB, CONJV, M3, false>;
+  auto kern = (vecP && vecV) ? k_prop_taylor<T, P1C, SPW, CB, CONJV, M3, true, TAIL>
+                             : k_prop_taylor<T, P1C, SPW, CB, CONJV, M3, false, TAIL>;
   if (lds > 64 * 1024)
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  const int NS = ceil_div(a.N, 16);
+  const int nw = TAIL ? (NS & ~3) : f.NW;
+  const int waves = TAIL ? nw + (NS - nw) * CB : f.NW;
   const long blocks = (long)a.W * ceil_div(a.n, 16 * CB);
-  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(64 * f.NW), lds, s, a.W, a.N, a.n, a.order,
-                     CONJV ? -1.0 : 1.0, P1, V, a.v_bs, Phi, a.phi_bs, f.NW, f.LDK);
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(64 * waves), lds, s, a.W, a.N, a.n,
+                     a.order, CONJV ? -1.0 : 1.0, P1, V, a.v_bs, Phi, a.phi_bs, nw, f.LDK);
   AFQ_LAUNCH_CHECK();
 }
 
 // Launches the fused kernel when the shape fits it (returns false otherwise). Candidates differ
-// in the number of 16-column tiles per workgroup (CB); they multiply in the same order, so the
-// tuner (timed on a scratch copy of the walkers) never changes the result.
+// in the number of 16-column tiles per workgroup (CB) and in the tail arrangement; they multiply
+// every output tile in the same order, so the tuner (timed on a scratch copy of the walkers)
+// never changes the result.
 template <typename T>
 bool propagate_fused(Context& c, const PropagateArgs& a, const T* P1, bool p1c, const void* V) {
   // the adjoint needs V^H; the kernel reads V by rows, so it takes V^H = conj(V) for a
@@ -581,10 +610,22 @@ bool propagate_fused(Context& c, const PropagateArgs& a, const T* P1, bool p1c, 
   const size_t lds1 = 2 * 16 * (size_t)f.LDK * sizeof(T);
   // wider column tiles spill in FP64 and with two stripes per wave
   const int maxcb = sizeof(T) == 4 && f.spw == 1 ? 3 : 1;
+  struct Cand {
+    int cb;
+    bool tail;
+  };
+  Cand cands[5];
   int ncand = 0;
-  int cbs[3];
+  auto fits = [&](int cb) {
+    return lds1 * cb <= (size_t)max_lds_per_block() && (cb == 1 || 16 * (cb - 1) < n);
+  };
   for (int cb = 1; cb <= maxcb; ++cb)
-    if (lds1 * cb <= (size_t)max_lds_per_block() && (cb == 1 || 16 * (cb - 1) < n)) cbs[ncand++] = cb;
+    if (fits(cb)) cands[ncand++] = {cb, false};
+  // FP32 with one stripe per wave and a stripe count that is not a multiple of 4: the tail
+  // arrangement, which spreads the last stripes' tiles over the SIMDs
+  if (sizeof(T) == 4 && f.spw == 1 && NS > 4 && NS % 4 != 0)
+    for (int cb = 2; cb <= maxcb; ++cb)
+      if (fits(cb) && (NS & ~3) + (NS % 4) * cb <= 16) cands[ncand++] = {cb, true};
   if (ncand == 0) return false;
   AFQ_CHECK((long)W * ceil_div(n, 16) < (1L << 31), kErrShape, "grid too large");
   const size_t esz = sizeof(T) * (p1c ? 2 : 1);
@@ -600,7 +641,8 @@ bool propagate_fused(Context& c, const PropagateArgs& a, const T* P1, bool p1c, 
   const char* m3e = std::getenv("AFQ_PROP_M3");
   const bool m3 = sizeof(T) == 8 || (m3e && m3e[0] == '1');
   auto run = [&](int i, z64* Phi) {
-    const int cb = cbs[i];
+    const int cb = cands[i].cb;
+    const bool tail = cands[i].tail;
     constexpr int SPW2 = sizeof(T) == 4 ? 2 : 1;
     if constexpr (sizeof(T) == 8) {
       if (adj) {  // FP64 takes one stripe per wave and one column tile
@@ -609,27 +651,27 @@ bool propagate_fused(Context& c, const PropagateArgs& a, const T* P1, bool p1c, 
         return;
       }
     }
-#define AFQ_PT(P1C, SPW, CB)                                                       \
-  (m3 ? launch_fused<T, P1C, SPW, CB, false, true>(s, a, f, P1, vecP, Vt, vecV, Phi) \
-      : launch_fused<T, P1C, SPW, CB, false, sizeof(T) == 8>(s, a, f, P1, vecP, Vt, vecV, Phi))
+#define AFQ_PT(P1C, SPW, CB, TL)                                                         \
+  (m3 ? launch_fused<T, P1C, SPW, CB, false, true, TL>(s, a, f, P1, vecP, Vt, vecV, Phi) \
+      : launch_fused<T, P1C, SPW, CB, false, sizeof(T) == 8, TL>(s, a, f, P1, vecP, Vt, vecV, Phi))
     if (f.spw == 1) {
       if constexpr (sizeof(T) == 4) {
         if (p1c) {
-          if (cb == 1) AFQ_PT(true, 1, 1);
-          else if (cb == 2) AFQ_PT(true, 1, 2);
-          else AFQ_PT(true, 1, 3);
+          if (cb == 1) AFQ_PT(true, 1, 1, false);
+          else if (cb == 2) tail ? AFQ_PT(true, 1, 2, true) : AFQ_PT(true, 1, 2, false);
+          else tail ? AFQ_PT(true, 1, 3, true) : AFQ_PT(true, 1, 3, false);
         } else {
-          if (cb == 1) AFQ_PT(false, 1, 1);
-          else if (cb == 2) AFQ_PT(false, 1, 2);
-          else AFQ_PT(false, 1, 3);
+          if (cb == 1) AFQ_PT(false, 1, 1, false);
+          else if (cb == 2) tail ? AFQ_PT(false, 1, 2, true) : AFQ_PT(false, 1, 2, false);
+          else tail ? AFQ_PT(false, 1, 3, true) : AFQ_PT(false, 1, 3, false);
         }
       } else {
-        if (p1c) AFQ_PT(true, 1, 1);
-        else AFQ_PT(false, 1, 1);
+        if (p1c) AFQ_PT(true, 1, 1, false);
+        else AFQ_PT(false, 1, 1, false);
       }
     } else {
-      if (p1c) AFQ_PT(true, SPW2, 1);
-      else AFQ_PT(false, SPW2, 1);
+      if (p1c) AFQ_PT(true, SPW2, 1, false);
+      else AFQ_PT(false, SPW2, 1, false);
     }
 #undef AFQ_PT
   };
