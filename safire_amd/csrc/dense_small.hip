@@ -33,7 +33,8 @@ __device__ inline void clog(double re, double im, double& lr, double& li) {
 // to the lowest position (getrf's first maximum), and the row at position j trades positions
 // with it. Gauss-Jordan then eliminates column j from every other row with the pivot row read
 // from the pivot lane's registers (readlane), so the only dynamically indexed register is
-// column j. The log det
+// column j. Without the inverse, columns <= j are dead after step j and are not updated. The
+// log det
 // takes log(+-U_jj + DBL_MIN) with the sign flipped exactly on getrf's swapped steps; the
 // in-place result Y gives A^{-1}[k][c] = Y[p_k][s_c] (p_k = pivot row of step k, s_c = step at
 // which row c pivoted), applied while storing.
@@ -44,11 +45,11 @@ __device__ inline double rdl(double v, int src) {
   return __hiloint2double(hi, lo);
 }
 
-template <int NMAX>
+template <int NMAX, bool INV>
 __global__ void __launch_bounds__(64)
     k_lu_rows(const z64* __restrict__ A, long a_bs, int n, z64* __restrict__ Ainv, long i_bs,
               z64* __restrict__ ovlp, double ov_scale, double ov_add_re, double ov_add_im,
-              int want_inverse, int no_pivot) {
+              int no_pivot) {
   __shared__ int pivrow[64];
   const int lane = threadIdx.x;
   const long b = blockIdx.x;
@@ -128,6 +129,7 @@ __global__ void __launch_bounds__(64)
       // LDS round trip on the step's critical path)
 #pragma unroll
       for (int c = 0; c < NMAX; ++c) {
+        if (!INV && c <= j) continue;  // uniform: dead columns of the log det alone
         const double pr = rdl(xr[c], p), pim = rdl(xi[c], p);
         // x - h p as four fused multiply-adds
         double nr = fma(hi, pim, fma(-hr, pr, xr[c]));
@@ -154,7 +156,7 @@ __global__ void __launch_bounds__(64)
       ovlp[b] = {ov_scale * re + ov_add_re, ov_scale * im + ov_add_im};
     }
   }
-  if (!want_inverse) return;
+  if constexpr (!INV) return;
   __syncthreads();
   z64* Ib = Ainv + b * i_bs;
   if (lane < n && mystep < n) {
@@ -294,9 +296,15 @@ void lu_inverse_logdet(hipStream_t s, const z64* A, long a_bs, int nb, int n, z6
   AFQ_CHECK(n >= 1 && n <= 64, kErrShape, "n must be in [1, 64]");
   if (nb <= 0) return;
   const int wi = want_inverse ? 1 : 0;
-#define AFQ_LU_ROWS(NM, NP)                                                                    \
-  hipLaunchKernelGGL(k_lu_rows<NM>, dim3(nb), dim3(64), 0, s, A, a_bs, n, Ainv, i_bs, ovlp,    \
-                     ov_scale, ov_add.re, ov_add.im, wi, NP)
+#define AFQ_LU_ROWS(NM, NP)                                                                   \
+  do {                                                                                        \
+    if (wi)                                                                                   \
+      hipLaunchKernelGGL((k_lu_rows<NM, true>), dim3(nb), dim3(64), 0, s, A, a_bs, n, Ainv,   \
+                         i_bs, ovlp, ov_scale, ov_add.re, ov_add.im, NP);                     \
+    else                                                                                      \
+      hipLaunchKernelGGL((k_lu_rows<NM, false>), dim3(nb), dim3(64), 0, s, A, a_bs, n, Ainv,  \
+                         i_bs, ovlp, ov_scale, ov_add.re, ov_add.im, NP);                     \
+  } while (0)
   if (n <= 8)
     AFQ_LU_ROWS(8, 0);
   else if (n <= 16)
@@ -326,23 +334,23 @@ void chol_upper_inverse(hipStream_t s, const z64* S, int nb, int n, z64* Rinv, z
   AFQ_LAUNCH_CHECK();
   const long nn = (long)n * n;
   if (n <= 8)
-    hipLaunchKernelGGL(k_lu_rows<8>, dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet, 1.0,
-                       0.0, 0.0, 1, 1);
+    hipLaunchKernelGGL((k_lu_rows<8, true>), dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet,
+                       1.0, 0.0, 0.0, 1);
   else if (n <= 16)
-    hipLaunchKernelGGL(k_lu_rows<16>, dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet, 1.0,
-                       0.0, 0.0, 1, 1);
+    hipLaunchKernelGGL((k_lu_rows<16, true>), dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet,
+                       1.0, 0.0, 0.0, 1);
   else if (n <= 32)
-    hipLaunchKernelGGL(k_lu_rows<32>, dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet, 1.0,
-                       0.0, 0.0, 1, 1);
+    hipLaunchKernelGGL((k_lu_rows<32, true>), dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet,
+                       1.0, 0.0, 0.0, 1);
   else if (n <= 40)
-    hipLaunchKernelGGL(k_lu_rows<40>, dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet, 1.0,
-                       0.0, 0.0, 1, 1);
+    hipLaunchKernelGGL((k_lu_rows<40, true>), dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet,
+                       1.0, 0.0, 0.0, 1);
   else if (n <= 48)
-    hipLaunchKernelGGL(k_lu_rows<48>, dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet, 1.0,
-                       0.0, 0.0, 1, 1);
+    hipLaunchKernelGGL((k_lu_rows<48, true>), dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet,
+                       1.0, 0.0, 0.0, 1);
   else
-    hipLaunchKernelGGL(k_lu_rows<64>, dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet, 1.0,
-                       0.0, 0.0, 1, 1);
+    hipLaunchKernelGGL((k_lu_rows<64, true>), dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet,
+                       1.0, 0.0, 0.0, 1);
   AFQ_LAUNCH_CHECK();
 }
 

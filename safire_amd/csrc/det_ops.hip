@@ -16,6 +16,7 @@
 #include "gemm.hpp"
 #include "ops.hpp"
 #include "tuner.hpp"
+#include "zgemm_rt.hpp"
 
 namespace afq {
 
@@ -81,6 +82,19 @@ using gemm::EpiStore2;
 using gemm::EpiTaylor;
 using gemm::Opnd;
 
+// A small batched complex FP64 product: the register-tiled kernel (its blockings tuned per
+// shape) where it applies, else the LDS-staged engine.
+template <class OA, class OB, class EPI>
+void zgemm_small(Context& c, const char* site, int W, int M, int N, int K, const OA& A,
+                 const OB& B, const EPI& e) {
+  if (!zrt::supported(M, N, K, A, B)) {
+    gemm_small<double>(c, site, W, M, N, K, A, B, e);
+    return;
+  }
+  hipStream_t s = c.stream();
+  tuned(c, tune_key((std::string(site) + "_rt").c_str(), sizeof(double), W, M, N, K),
+        zrt::kCandidates, [&](int i) { zrt::launch(s, W, M, N, K, A, B, e, i); }, true);
+}
 
 void log_overlap(Context& c, const z64* PsiT, int n, int N, const z64* Phi, long phi_bs, int W,
                  z64* ovlp, double scale, z64 add) {
@@ -89,7 +103,8 @@ void log_overlap(Context& c, const z64* PsiT, int n, int N, const z64* Phi, long
   z64* T = scratch_as<z64>(c, "lo_T", (size_t)W * n * n);
   Opnd<double, true, true> A{&PsiT->re, N, 0};
   Opnd<double, true, false> B{&Phi->re, n, phi_bs};
-  gemm_small<double>(c, "lo_T", W, n, n, N, A, B, EpiStore<double, double>{T, n, (long)n * n, 1, 0, 0});
+  const EpiStore<double, double> e{T, n, (long)n * n, 1, 0, 0};
+  zgemm_small(c, "lo_T", W, n, n, N, A, B, e);
   lu_inverse_logdet(s, T, (long)n * n, W, n, nullptr, 0, ovlp, scale, add, false);
 }
 
@@ -110,7 +125,8 @@ void mixed_density_matrix(Context& c, const z64* PsiT, int n, int N, const z64* 
   {
     Opnd<double, true, true> A{&PsiT->re, N, 0};
     Opnd<double, true, false> B{&Phi->re, n, phi_bs};
-    gemm_small<double>(c, "mdm_T", W, n, n, N, A, B, EpiStore<double, double>{T, n, (long)n * n, 1, 0, 0});
+    const EpiStore<double, double> e{T, n, (long)n * n, 1, 0, 0};
+    zgemm_small(c, "mdm_T", W, n, n, N, A, B, e);
   }
   lu_inverse_logdet(s, T, (long)n * n, W, n, T, (long)n * n, ovlp, scale, {0, 0}, true);
   // compact G[a][k] = sum_b Tinv[b][a] Phi[k][b]    (product<'T','T'>(TNN, B, C))
@@ -119,10 +135,13 @@ void mixed_density_matrix(Context& c, const z64* PsiT, int n, int N, const z64* 
   {
     Opnd<double, true, false> A{&T->re, n, (long)n * n};
     Opnd<double, true, true> B{&Phi->re, n, phi_bs};
-    if (G32)
-      gemm_small<double>(c, "mdm_G", W, n, N, n, A, B, EpiStore<double, float>{G32, N, g_bs, 1, 0, 0});
-    else
-      gemm_small<double>(c, "mdm_G", W, n, N, n, A, B, EpiStore<double, double>{Gc, N, gc_bs, 1, 0, 0});
+    if (G32) {
+      const EpiStore<double, float> e{G32, N, g_bs, 1, 0, 0};
+      zgemm_small(c, "mdm_G", W, n, N, n, A, B, e);
+    } else {
+      const EpiStore<double, double> e{Gc, N, gc_bs, 1, 0, 0};
+      zgemm_small(c, "mdm_G", W, n, N, n, A, B, e);
+    }
   }
   if (!compact) {
     // G[i][k] = sum_a PsiT[a][i] Gc[a][k]           (product<'T'>(A, TNM, C))
