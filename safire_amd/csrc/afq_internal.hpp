@@ -6,8 +6,20 @@
 namespace afq {
 
 // ---- dense_small.hip -------------------------------------------------------------------------
+// Factors kept between calls on the same walkers: the matrices T [nb][n][n] they belong to (NaN
+// until set), their inverses [nb][n][n] and raw log dets [nb].
+struct LuKeep {
+  z64* T;
+  z64* Tinv;
+  z64* ldet;
+};
+// n <= 48.  keep, !reuse: A is keep->T; the inverse goes to keep->Tinv and the raw log det to keep->ldet.
+// keep, reuse: matrices equal bit for bit to keep->T take the kept log det (nothing recomputed,
+// nothing written but ovlp); the others are copied into keep->T and factorized into the kept
+// buffers.  Either way ovlp is updated as without keep and Ainv must be keep->Tinv.
 void lu_inverse_logdet(hipStream_t s, const z64* A, long a_bs, int nb, int n, z64* Ainv, long i_bs,
-                       z64* ovlp, double ov_scale, z64 ov_add, bool want_inverse);
+                       z64* ovlp, double ov_scale, z64 ov_add, bool want_inverse,
+                       const LuKeep* keep = nullptr, bool reuse = false);
 void chol_upper_inverse(hipStream_t s, const z64* S, int nb, int n, z64* Rinv, z64* ldet);
 
 // ---- fields.hip ------------------------------------------------------------------------------

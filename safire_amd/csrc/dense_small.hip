@@ -45,11 +45,29 @@ __device__ inline double rdl(double v, int src) {
   return __hiloint2double(hi, lo);
 }
 
+// ovlp[b] = scale * wrap(ovlp[b] + (lr, li)) + add, the imaginary part wrapped into [-pi, pi]
+__device__ inline void ov_update(z64* ovlp, long b, double lr, double li, double ov_scale,
+                                 double ov_add_re, double ov_add_im) {
+  z64 o = ovlp[b];
+  double re = o.re + lr, im = o.im + li;
+  if (im > kPi) {
+    while (im > kPi) im -= 2 * kPi;
+  } else if (im < -kPi) {
+    while (im < -kPi) im += 2 * kPi;
+  }
+  ovlp[b] = {ov_scale * re + ov_add_re, ov_scale * im + ov_add_im};
+}
+
+// keep_ldet != nullptr: the raw log det sum (before the update of ovlp) of matrix b is stored
+// there.  reuse (INV only): keepT holds the matrix whose inverse (at Ainv) and raw log det (at
+// keep_ldet) were kept; when matrix b matches it bit for bit (no NaN), the kept values are
+// exact for it: ovlp is updated from keep_ldet and nothing else is written.  Otherwise keepT
+// takes matrix b and the factors are recomputed into Ainv and keep_ldet.
 template <int NMAX, bool INV>
 __global__ void __launch_bounds__(64)
     k_lu_rows(const z64* __restrict__ A, long a_bs, int n, z64* __restrict__ Ainv, long i_bs,
               z64* __restrict__ ovlp, double ov_scale, double ov_add_re, double ov_add_im,
-              int no_pivot) {
+              int no_pivot, z64* __restrict__ keepT, z64* __restrict__ keep_ldet, int reuse) {
   __shared__ int pivrow[64];
   const int lane = threadIdx.x;
   const long b = blockIdx.x;
@@ -60,6 +78,34 @@ __global__ void __launch_bounds__(64)
     z64 v = (lane < n && c < n) ? Ab[(long)lane * n + c] : z64{0.0, 0.0};
     xr[c] = v.re;
     xi[c] = v.im;
+  }
+  // (rows of 64 columns fill the register file: no kept factors there)
+  if constexpr (INV && NMAX <= 48) {
+    if (reuse) {
+      z64* Kb = keepT + b * a_bs;
+      bool same = true;
+      if (lane < n) {
+#pragma unroll
+        for (int c = 0; c < NMAX; ++c) {
+          const z64 k = c < n ? Kb[(long)lane * n + c] : z64{xr[c], xi[c]};
+          same = same & (xr[c] == xr[c]) & (xi[c] == xi[c]) &
+                 (__double_as_longlong(k.re) == __double_as_longlong(xr[c])) &
+                 (__double_as_longlong(k.im) == __double_as_longlong(xi[c]));
+        }
+      }
+      if (__ballot(!same) == 0) {  // uniform
+        if (ovlp != nullptr && lane == 0) {
+          const z64 l = keep_ldet[b];
+          ov_update(ovlp, b, l.re, l.im, ov_scale, ov_add_re, ov_add_im);
+        }
+        return;
+      }
+      if (lane < n) {
+#pragma unroll
+        for (int c = 0; c < NMAX; ++c)
+          if (c < n) Kb[(long)lane * n + c] = z64{xr[c], xi[c]};
+      }
+    }
   }
   bool used = lane >= n;
   int pos = lane;         // position of my row in getrf's order
@@ -140,20 +186,14 @@ __global__ void __launch_bounds__(64)
       }
     }
   }
-  if (ovlp != nullptr) {
+  if (ovlp != nullptr || keep_ldet != nullptr) {
     double lr = 0.0, li = 0.0;
     if (lane < n) clog(ur + 2.2250738585072014e-308, ui, lr, li);
     lr = wave_sum(lr);
     li = wave_sum(li);
     if (lane == 0) {
-      z64 o = ovlp[b];
-      double re = o.re + lr, im = o.im + li;
-      if (im > kPi) {
-        while (im > kPi) im -= 2 * kPi;
-      } else if (im < -kPi) {
-        while (im < -kPi) im += 2 * kPi;
-      }
-      ovlp[b] = {ov_scale * re + ov_add_re, ov_scale * im + ov_add_im};
+      if (keep_ldet != nullptr) keep_ldet[b] = {lr, li};
+      if (ovlp != nullptr) ov_update(ovlp, b, lr, li, ov_scale, ov_add_re, ov_add_im);
     }
   }
   if constexpr (!INV) return;
@@ -292,18 +332,25 @@ __global__ void __launch_bounds__(64)
 }  // namespace
 
 void lu_inverse_logdet(hipStream_t s, const z64* A, long a_bs, int nb, int n, z64* Ainv,
-                       long i_bs, z64* ovlp, double ov_scale, z64 ov_add, bool want_inverse) {
+                       long i_bs, z64* ovlp, double ov_scale, z64 ov_add, bool want_inverse,
+                       const LuKeep* keep, bool reuse) {
   AFQ_CHECK(n >= 1 && n <= 64, kErrShape, "n must be in [1, 64]");
+  AFQ_CHECK(!keep || (want_inverse && n <= 48 && Ainv == keep->Tinv && a_bs == (long)n * n &&
+                      i_bs == (long)n * n && (reuse || A == keep->T)),
+            kErrArg, "kept factors: n <= 48, the inverse in the kept buffer, [nb][n][n] layouts");
   if (nb <= 0) return;
   const int wi = want_inverse ? 1 : 0;
+  z64* kT = keep && reuse ? keep->T : nullptr;
+  z64* kl = keep ? keep->ldet : nullptr;
+  const int ru = keep && reuse ? 1 : 0;
 #define AFQ_LU_ROWS(NM, NP)                                                                   \
   do {                                                                                        \
     if (wi)                                                                                   \
       hipLaunchKernelGGL((k_lu_rows<NM, true>), dim3(nb), dim3(64), 0, s, A, a_bs, n, Ainv,   \
-                         i_bs, ovlp, ov_scale, ov_add.re, ov_add.im, NP);                     \
+                         i_bs, ovlp, ov_scale, ov_add.re, ov_add.im, NP, kT, kl, ru);         \
     else                                                                                      \
       hipLaunchKernelGGL((k_lu_rows<NM, false>), dim3(nb), dim3(64), 0, s, A, a_bs, n, Ainv,  \
-                         i_bs, ovlp, ov_scale, ov_add.re, ov_add.im, NP);                     \
+                         i_bs, ovlp, ov_scale, ov_add.re, ov_add.im, NP, kT, kl, ru);         \
   } while (0)
   if (n <= 8)
     AFQ_LU_ROWS(8, 0);
@@ -335,22 +382,22 @@ void chol_upper_inverse(hipStream_t s, const z64* S, int nb, int n, z64* Rinv, z
   const long nn = (long)n * n;
   if (n <= 8)
     hipLaunchKernelGGL((k_lu_rows<8, true>), dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet,
-                       1.0, 0.0, 0.0, 1);
+                       1.0, 0.0, 0.0, 1, nullptr, nullptr, 0);
   else if (n <= 16)
     hipLaunchKernelGGL((k_lu_rows<16, true>), dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet,
-                       1.0, 0.0, 0.0, 1);
+                       1.0, 0.0, 0.0, 1, nullptr, nullptr, 0);
   else if (n <= 32)
     hipLaunchKernelGGL((k_lu_rows<32, true>), dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet,
-                       1.0, 0.0, 0.0, 1);
+                       1.0, 0.0, 0.0, 1, nullptr, nullptr, 0);
   else if (n <= 40)
     hipLaunchKernelGGL((k_lu_rows<40, true>), dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet,
-                       1.0, 0.0, 0.0, 1);
+                       1.0, 0.0, 0.0, 1, nullptr, nullptr, 0);
   else if (n <= 48)
     hipLaunchKernelGGL((k_lu_rows<48, true>), dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet,
-                       1.0, 0.0, 0.0, 1);
+                       1.0, 0.0, 0.0, 1, nullptr, nullptr, 0);
   else
     hipLaunchKernelGGL((k_lu_rows<64, true>), dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet,
-                       1.0, 0.0, 0.0, 1);
+                       1.0, 0.0, 0.0, 1, nullptr, nullptr, 0);
   AFQ_LAUNCH_CHECK();
 }
 

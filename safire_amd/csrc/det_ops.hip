@@ -97,20 +97,24 @@ void zgemm_small(Context& c, const char* site, int W, int M, int N, int K, const
 }
 
 void log_overlap(Context& c, const z64* PsiT, int n, int N, const z64* Phi, long phi_bs, int W,
-                 z64* ovlp, double scale, z64 add) {
+                 z64* ovlp, double scale, z64 add, const LuKeep* keep) {
   if (n == 0 || W == 0) return;
   hipStream_t s = c.stream();
-  z64* T = scratch_as<z64>(c, "lo_T", (size_t)W * n * n);
+  z64* T = keep ? keep->T : scratch_as<z64>(c, "lo_T", (size_t)W * n * n);
   Opnd<double, true, true> A{&PsiT->re, N, 0};
   Opnd<double, true, false> B{&Phi->re, n, phi_bs};
   const EpiStore<double, double> e{T, n, (long)n * n, 1, 0, 0};
   zgemm_small(c, "lo_T", W, n, n, N, A, B, e);
-  lu_inverse_logdet(s, T, (long)n * n, W, n, nullptr, 0, ovlp, scale, add, false);
+  if (keep)
+    lu_inverse_logdet(s, T, (long)n * n, W, n, keep->Tinv, (long)n * n, ovlp, scale, add, true,
+                      keep, false);
+  else
+    lu_inverse_logdet(s, T, (long)n * n, W, n, nullptr, 0, ovlp, scale, add, false);
 }
 
 void mixed_density_matrix(Context& c, const z64* PsiT, int n, int N, const z64* Phi, long phi_bs,
                           int W, z64* G, long g_bs, z64* ovlp, bool compact, double scale,
-                          c32* G32, z64* ws_T, z64* ws_Gc) {
+                          c32* G32, z64* ws_T, z64* ws_Gc, const LuKeep* keep) {
   AFQ_CHECK(!G32 || compact, kErrArg, "the complex64 copy is for the compact density matrix");
   AFQ_CHECK(ws_T ? (compact || ws_Gc) : !ws_Gc, kErrArg,
             "workspace: T, and Gc for the full density matrix, or neither");
@@ -128,12 +132,14 @@ void mixed_density_matrix(Context& c, const z64* PsiT, int n, int N, const z64* 
     const EpiStore<double, double> e{T, n, (long)n * n, 1, 0, 0};
     zgemm_small(c, "mdm_T", W, n, n, N, A, B, e);
   }
-  lu_inverse_logdet(s, T, (long)n * n, W, n, T, (long)n * n, ovlp, scale, {0, 0}, true);
+  z64* Ti = keep ? keep->Tinv : T;
+  lu_inverse_logdet(s, T, (long)n * n, W, n, Ti, (long)n * n, ovlp, scale, {0, 0}, true, keep,
+                    true);
   // compact G[a][k] = sum_b Tinv[b][a] Phi[k][b]    (product<'T','T'>(TNN, B, C))
   z64* Gc = compact ? G : ws_Gc ? ws_Gc : scratch_as<z64>(c, "mdm_Gc", (size_t)W * n * N);
   long gc_bs = compact ? g_bs : (long)n * N;
   {
-    Opnd<double, true, false> A{&T->re, n, (long)n * n};
+    Opnd<double, true, false> A{&Ti->re, n, (long)n * n};
     Opnd<double, true, true> B{&Phi->re, n, phi_bs};
     if (G32) {
       const EpiStore<double, float> e{G32, N, g_bs, 1, 0, 0};
@@ -472,9 +478,10 @@ __device__ inline void write_b(T* Br, T* Bi, int LDK, int N, int NS, int NW, int
 // the same sequence of barriers.  CONJV: the V products use conj(V), which is V^H for the
 // complex-symmetric V of vHS (the adjoint propagator, sgn = -1).
 template <typename T, bool P1C, int SPW, int CB, bool CONJV, bool M3, bool VEC>
-__device__ __attribute__((always_inline)) inline void prop_share(int N, int n, int order, double sgn, const T* __restrict__ P1,
-                                  const T* __restrict__ Vw, z64* __restrict__ Pw, int c0, int NS,
-                                  int NW, int wave, int fr, int fk, T* Br, T* Bi, int LDK) {
+__device__ __attribute__((always_inline)) inline void prop_share(
+    int N, int n, int order, double sgn, const T* __restrict__ P1, const T* __restrict__ Vw,
+    z64* __restrict__ Pw, int c0, int NS, int NW, int wave, int fr, int fk, T* Br, T* Bi, int LDK,
+    int e) {
   using V4 = typename gemm::Acc<T>::v4;
   V4 ar[SPW][CB], ai[SPW][CB], sr[SPW][CB], si[SPW][CB];
   // T_0 = P1 Phi, S = T_0
@@ -512,8 +519,9 @@ __device__ __attribute__((always_inline)) inline void prop_share(int N, int n, i
   }
   write_b<T, SPW, CB>(Br, Bi, LDK, N, NS, NW, wave, fr, fk, sr, si);
   __syncthreads();
-  // Phi = P1 S
+  // Phi = P1 S, scaled back by 2^e
   stripes_times_b<T, P1C, SPW, CB, VEC>(P1, N, N, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai);
+  const double sb = e == 0 ? 1.0 : ldexp(1.0, e);
 #pragma unroll
   for (int s = 0; s < SPW; ++s) {
     if (wave + s * NW >= NS) continue;
@@ -522,7 +530,8 @@ __device__ __attribute__((always_inline)) inline void prop_share(int N, int n, i
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = (wave + s * NW) * 16 + gemm::Acc<T>::row(fk, r), col = c0 + 16 * t + fr;
-        if (row < N && col < n) Pw[(long)row * n + col] = z64{(double)ar[s][t][r], (double)ai[s][t][r]};
+        if (row < N && col < n)
+          Pw[(long)row * n + col] = z64{(double)ar[s][t][r] * sb, (double)ai[s][t][r] * sb};
       }
   }
 }
@@ -555,21 +564,59 @@ __global__ void __launch_bounds__(1024)
   z64* Pw = Phi + (long)w * phi_bs;
   const T* Vw = V + 2 * (long)w * v_bs;
 
-  for (int t = tid; t < BC * NS * 16; t += blockDim.x) {
-    const int cc = t % BC, k = t / BC;
-    const z64 v = (k < N && c0 + cc < n) ? Pw[(long)k * n + c0 + cc] : z64{0.0, 0.0};
-    Br[cc * LDK + k] = (T)v.re;
-    Bi[cc * LDK + k] = (T)v.im;
+  int e = 0;  // binary scale of the tile (uniform)
+  if constexpr (sizeof(T) == 4) {
+    // FP32: the tile enters scaled by 2^-e, e the binary exponent of its largest entry, and
+    // leaves scaled back by 2^e.  The products are linear in the tile and power-of-two scaling
+    // is exact, so the results are the unscaled ones wherever those stay in FP32's range; walker
+    // matrices that grow between orthogonalizations past FP32's range no longer overflow.
+    constexpr int KMAX = 4 * CB * SPW;  // >= BC NS 16 / blockDim (checked at launch)
+    __shared__ double wmx[16];
+    z64 vals[KMAX];
+    double mx = 0.0;
+#pragma unroll
+    for (int i = 0; i < KMAX; ++i) {
+      const int t = tid + i * (int)blockDim.x;
+      const int cc = t % BC, k = t / BC;
+      const z64 v = (t < BC * NS * 16 && k < N && c0 + cc < n) ? Pw[(long)k * n + c0 + cc]
+                                                              : z64{0.0, 0.0};
+      vals[i] = v;
+      mx = fmax(mx, fmax(fabs(v.re), fabs(v.im)));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+    if (lane == 0) wmx[wave] = mx;
+    __syncthreads();
+    mx = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) mx = fmax(mx, wmx[w]);
+    if (mx > 0.0 && mx <= 1.7976931348623157e308) e = __builtin_amdgcn_readfirstlane(ilogb(mx));
+    const double sc = ldexp(1.0, -e);
+#pragma unroll
+    for (int i = 0; i < KMAX; ++i) {
+      const int t = tid + i * (int)blockDim.x;
+      if (t < BC * NS * 16) {
+        const int cc = t % BC, k = t / BC;
+        Br[cc * LDK + k] = (T)(vals[i].re * sc);
+        Bi[cc * LDK + k] = (T)(vals[i].im * sc);
+      }
+    }
+  } else {
+    for (int t = tid; t < BC * NS * 16; t += blockDim.x) {
+      const int cc = t % BC, k = t / BC;
+      const z64 v = (k < N && c0 + cc < n) ? Pw[(long)k * n + c0 + cc] : z64{0.0, 0.0};
+      Br[cc * LDK + k] = (T)v.re;
+      Bi[cc * LDK + k] = (T)v.im;
+    }
   }
   __syncthreads();
   if (!TAIL || wave < NW) {
     prop_share<T, P1C, SPW, CB, CONJV, M3, VEC>(N, n, order, sgn, P1, Vw, Pw, c0, NS, NW, wave,
-                                                fr, fk, Br, Bi, LDK);
+                                                fr, fk, Br, Bi, LDK, e);
   } else if constexpr (TAIL) {
     const int u = wave - NW, st = NW + u / CB, tt = u % CB;
     prop_share<T, P1C, 1, 1, CONJV, M3, VEC>(N, n, order, sgn, P1, Vw, Pw, c0 + 16 * tt, NS, NW,
                                              st, fr, fk, Br + 16 * tt * LDK, Bi + 16 * tt * LDK,
-                                             LDK);
+                                             LDK, e);
   }
 }
 
@@ -602,6 +649,9 @@ void launch_fused(hipStream_t s, const PropagateArgs& a, const FusedShape& f, co
   const int NS = ceil_div(a.N, 16);
   const int nw = TAIL ? (NS & ~3) : f.NW;
   const int waves = TAIL ? nw + (NS - nw) * CB : f.NW;
+  // the FP32 tile load keeps 4 CB SPW entries per thread in registers
+  AFQ_CHECK(sizeof(T) == 8 || 16L * CB * NS * 16 <= 4L * CB * SPW * 64 * waves, kErrShape,
+            "propagation tile larger than the FP32 load covers");
   const long blocks = (long)a.W * ceil_div(a.n, 16 * CB);
   hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(64 * waves), lds, s, a.W, a.N, a.n,
                      a.order, CONJV ? -1.0 : 1.0, P1, V, a.v_bs, Phi, a.phi_bs, nw, f.LDK);

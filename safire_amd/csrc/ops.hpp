@@ -63,15 +63,20 @@ T* scratch_as(Context& c, const std::string& name, size_t count) {
 namespace det_ops {
 
 // T = PsiT * Phi_w; ovlp[w] = scale * wrap(ovlp[w] + log det T) + add; optionally T^{-1} in Tinv.
+// keep: T, its inverse and raw log det are kept for a later mixed_density_matrix of the same
+// walkers (LuKeep, dense_small.hip).
 void log_overlap(Context& c, const z64* PsiT, int n, int N, const z64* Phi, long phi_bs, int W,
-                 z64* ovlp, double scale = 1.0, z64 add = {0, 0});
+                 z64* ovlp, double scale = 1.0, z64 add = {0, 0}, const LuKeep* keep = nullptr);
 // MixedDensityMatrix: compact G [W][n][N] or full G [W][N][N]; ovlp accumulated as in
 // log_overlap.  G32 (compact only): G is computed in FP64 and stored rounded to complex64 at
 // G32 (same layout) instead of G.  ws_T [W][n][n] and, for the full G, ws_Gc [W][n][N]: the
 // caller's workspace in place of the context's scratch buffers (both or neither).
+// keep: walkers whose T equals the kept T bit for bit take the kept inverse and log det (the
+// same bits the factorization would give); the others are factorized into the kept buffers.
 void mixed_density_matrix(Context& c, const z64* PsiT, int n, int N, const z64* Phi, long phi_bs,
                           int W, z64* G, long g_bs, z64* ovlp, bool compact, double scale = 1.0,
-                          c32* G32 = nullptr, z64* ws_T = nullptr, z64* ws_Gc = nullptr);
+                          c32* G32 = nullptr, z64* ws_T = nullptr, z64* ws_Gc = nullptr,
+                          const LuKeep* keep = nullptr);
 // inverse_logdet (density_matrix.hpp:82-111)
 void inverse_logdet(Context& c, const z64* A, int nb, int n, z64* ovlp, z64* Ainv);
 // Full mixed density matrices with a left state per walker (the 3-d MixedDensityMatrix of
@@ -538,6 +543,13 @@ class Propagator {
   float* P1r32_ = nullptr;
   c32* P1c32_ = nullptr;
   bool real_P1_ = false;
+  // single determinant: T = PsiT Phi of each spin block, its inverse and raw log det as the
+  // end-of-step overlap leaves them, for the next step's density matrix (which takes them for
+  // the walkers whose T comes out bit for bit the same, and recomputes the others)
+  LuKeep keep_[2] = {};
+  const LuKeep* keep(int sp, int W) const {
+    return keep_[sp].T && W <= W_ ? &keep_[sp] : nullptr;
+  }
   // the propagation of spin block sp: its P1 block (the inverse one with `inverse`) and its V
   // block (V [nspin_L][W][N][N] in precision prec)
   det_ops::PropagateArgs spin_args(int sp, int W, int n, char ta, const void* V, Precision prec,

@@ -15,6 +15,8 @@
 // LU/QR, fields and weights stay FP64, and G is recomputed in FP64 from the FP64 walker matrices
 // after every propagation.
 #include <algorithm>
+#include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "gemm.hpp"
@@ -84,10 +86,26 @@ Propagator::Propagator(CholeskyOps& ham, int W, const PropagatorParams& p, Phmsd
       P1c32_ = dalloc<c32>(NP);
       cast_z64_c32(s, NP, P1_, P1c32_);
     }
+    // kept overlap factors (AFQ_LU_KEEP=0 switches them off): T starts as NaN, which matches
+    // nothing
+    const char* ke = std::getenv("AFQ_LU_KEEP");
+    if (!trial_ && !msd_ && !(ke && ke[0] == '0')) {
+      for (int sp = 0; sp < (closed_ ? 1 : 2); ++sp) {
+        const int n = sp == 0 ? wup_ : wdn_;
+        if (n < 1 || n > 48) continue;
+        const size_t nn = (size_t)W * n * n;
+        keep_[sp].T = dalloc<z64>(nn);
+        keep_[sp].Tinv = dalloc<z64>(nn);
+        keep_[sp].ldet = dalloc<z64>(W);
+        fill_z64(s, (long)nn, {std::nan(""), std::nan("")}, keep_[sp].T);
+      }
+    }
     AFQ_HIP(hipStreamSynchronize(s));
   } catch (...) {
     for (void* q : {(void*)vMF_, (void*)P1_, (void*)P1r_, (void*)P1r32_, (void*)P1c32_,
-                    (void*)step_dev_, (void*)status_dev_, (void*)stats_dev_})
+                    (void*)step_dev_, (void*)status_dev_, (void*)stats_dev_, (void*)keep_[0].T,
+                    (void*)keep_[0].Tinv, (void*)keep_[0].ldet, (void*)keep_[1].T,
+                    (void*)keep_[1].Tinv, (void*)keep_[1].ldet})
       if (q) (void)hipFree(q);
     throw;
   }
@@ -107,7 +125,9 @@ Propagator::~Propagator() {
                   (void*)bp_anch_a_, (void*)bp_anch_b_, (void*)ref_rows_[0], (void*)ref_rows_[1],
                   (void*)ref_cc_, (void*)ref_sel_[0], (void*)ref_sel_[1], (void*)ref_cols_[0],
                   (void*)ref_cols_[1], (void*)te_op_[0], (void*)te_op_[1], (void*)te_op_[2],
-                  (void*)P1inv_, (void*)P1invr_})
+                  (void*)P1inv_, (void*)P1invr_, (void*)keep_[0].T, (void*)keep_[0].Tinv,
+                  (void*)keep_[0].ldet, (void*)keep_[1].T, (void*)keep_[1].Tinv,
+                  (void*)keep_[1].ldet})
     if (q) (void)hipFree(q);
 }
 
@@ -187,10 +207,10 @@ void Propagator::log_overlap(const WalkerState& st, z64* ovlp) {
   const bool closed = closed_;
   fill_z64(s, st.W, {0, 0}, ovlp);
   det_ops::log_overlap(ctx_, ham_.PsiT(), nup, N, st.Phi, bs_a(st), st.W, ovlp,
-                       closed ? 2.0 : 1.0);
+                       closed ? 2.0 : 1.0, {0, 0}, keep(0, st.W));
   if (!closed && ndown > 0)
     det_ops::log_overlap(ctx_, ham_.PsiT() + (size_t)nup * N, ndown, N, st.Phi_b, bs_b(st),
-                         st.W, ovlp, 1.0);
+                         st.W, ovlp, 1.0, {0, 0}, keep(1, st.W));
   // + log(conj(ci0)) = 0 for the single determinant with unit coefficient
 }
 
@@ -223,11 +243,13 @@ void Propagator::step_impl(const WalkerState& st, const double* u, const double*
     c32* G32 = prec_ == kMixed ? scratch_as<c32>(ctx_, "st_G32", (size_t)W * nel * N) : nullptr;
     fill_z64(s, W, {0, 0}, ov);
     det_ops::mixed_density_matrix(ctx_, ham_.PsiT(), nup, N, st.Phi, bs_a(st), W, G,
-                                  (long)nel * N, ov, true, closed ? 2.0 : 1.0, G32);
+                                  (long)nel * N, ov, true, closed ? 2.0 : 1.0, G32, nullptr,
+                                  nullptr, keep(0, W));
     if (!closed && ndown > 0)
       det_ops::mixed_density_matrix(ctx_, ham_.PsiT() + (size_t)nup * N, ndown, N, st.Phi_b,
                                     bs_b(st), W, G + (size_t)nup * N, (long)nel * N, ov,
-                                    true, 1.0, G32 ? G32 + (size_t)nup * N : nullptr);
+                                    true, 1.0, G32 ? G32 + (size_t)nup * N : nullptr, nullptr,
+                                    nullptr, keep(1, W));
     if (G32)
       ham_.vbias32(G32, W, p_.dt, X);
     else
