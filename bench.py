@@ -2,10 +2,14 @@
 
 Workload (the metric's configuration): N=200 orbitals, 40+40 electrons (closed shell), M=1600
 Cholesky vectors, 1024 walkers, dt=0.005, 6-term Taylor propagator, QR every 5 steps, synthetic
-hamiltonian (safire_amd.synthetic, seed 1003).  A step is one hybrid propagation of every walker
-(G, vbias, fields, vHS, P1 exp(iV) P1, overlap, weights) plus the growth estimator and Eshift
-relaxation, with the QR every fifth step, replayed from a HIP graph.  `value` is the mixed
-precision path (FP32 MFMA contractions, FP64 G/LU/QR/weights); the FP64 path is timed beside it.
+hamiltonian (safire_amd.synthetic, seed 1003) with Cholesky amplitude 0.1.  A step is one hybrid
+propagation of every walker (G, vbias, fields, vHS, P1 exp(iV) P1, overlap, weights) plus the
+growth estimator and Eshift relaxation, with the QR every fifth step, replayed from a HIP graph.
+`value` is the mixed precision path (FP32 MFMA contractions, FP64 G/LU/QR/weights); the FP64
+path is timed beside it.  At the generator's default amplitude 0.5 the walkers' weights all
+reach zero within about four steps (the FP64 oracle shows the same) and the matrices overflow,
+so the timed steps would run on dead walkers and the mixed-vs-FP64 condition cannot be met;
+those timings are reported beside the headline (`recipe_amplitude_0.5`).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu] [--only MODE]
 """
@@ -25,6 +29,8 @@ sys.path.insert(0, ROOT)
 N, NUP, M, W = 200, 40, 1600, 1024
 DT, ORDER, ORTHO = 0.005, 6, 5
 SEED = 1003
+AMPLITUDE = 0.1          # Cholesky amplitude of the headline workload
+RECIPE_AMPLITUDE = 0.5   # the generator's default
 PEAK = {"fp32": 157.3e12, "fp64": 78.6e12}
 HBM_PEAK = 8.0e12
 # HBM bytes of one FP32 vHS launch (k_crgemm over the symmetric pairs) at this workload:
@@ -439,6 +445,40 @@ def cpu_baseline(problem, budget_s=12.0, warmup=3):
                       f"{len(rates)} timed steps ({t_tot:.1f} s) after {warmup} untimed steps"}
 
 
+def walker_state(ws):
+    """Finiteness and survival of a walker set: the fraction of walkers whose matrices are
+    finite and whose weights are finite and non-zero."""
+    import torch
+    from safire_amd import afq
+    phi_ok = torch.isfinite(torch.view_as_real(ws.Phi)).flatten(1).all(1)
+    w = ws.weight
+    w_ok = torch.isfinite(torch.view_as_real(w)).all(-1) & (w.abs() > 0)
+    return {"finite_matrices": float(phi_ok.double().mean()),
+            "alive_finite_weights": float((phi_ok & w_ok).double().mean())}
+
+
+def recipe_workload(warmup, steps, amplitude=RECIPE_AMPLITUDE):
+    """The benchmark shapes at the generator's default Cholesky amplitude 0.5: mixed and FP64
+    step times and the state of the walkers after the timed steps.  The propagator's norm grows
+    the walker matrices by 1e5-1e9 per step and every walker's weight reaches zero within about
+    four steps (the FP64 oracle shows the same), so these steps run on dead, overflowing walkers;
+    the one-step mixed-vs-FP64 energy error from the initial walkers is in
+    `energy_error_mixed_vs_fp64_per_config`."""
+    from safire_amd.synthetic import make_problem
+    import torch
+    pr = make_problem(N, NUP, M, SEED, lscale=amplitude)
+    res = {"amplitude": amplitude}
+    from safire_amd import afq
+    for prec, name in ((afq.AFQ_MIXED, "mixed"), (afq.AFQ_FP64, "fp64")):
+        ctx, ham, prop, ws, Esh, eout = gpu_setup(pr, prec, W)
+        sec = time_steps(prop, ws, Esh, eout, warmup, steps)
+        res[name] = {"ms_per_step": sec * 1e3, "value": W / sec, "unit": "walker-steps/s",
+                     "walkers_after_timed_steps": walker_state(ws)}
+        del ctx, ham, prop, ws
+        torch.cuda.empty_cache()
+    return res
+
+
 def phmsd_bench(warmup, steps, energy_reps=5):
     """The multi-determinant workload: N=64, 10+10 electrons, M=320, 2000 particle-hole
     determinants (ranks <= 4, 15 active orbitals per spin), 256 COLLINEAR walkers in FP64.
@@ -493,7 +533,7 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--only", choices=["vhs", "vhs64", "copy", "step", "step64", "phmsd", "vendor",
-                                       "estimators", "energy", "serve"],
+                                       "estimators", "energy", "serve", "recipe"],
                     default=None,
                     help="profiling modes: launch only the named kernel(s)")
     ap.add_argument("--serve-jobs", type=int, default=200,
@@ -509,7 +549,7 @@ def main():
     from safire_amd import afq
     from safire_amd.synthetic import make_problem
     log(f"generating workload N={N} n={NUP} M={M} W={W}")
-    problem = make_problem(N, NUP, M, SEED)
+    problem = make_problem(N, NUP, M, SEED, lscale=AMPLITUDE)
 
     if args.only in ("vhs", "vhs64"):
         prec = afq.AFQ_MIXED if args.only == "vhs" else afq.AFQ_FP64
@@ -565,7 +605,7 @@ def main():
 
     out = {}
     for prec, name in ((afq.AFQ_MIXED, "mixed"), (afq.AFQ_FP64, "fp64")):
-        if args.only == "serve":
+        if args.only in ("serve", "recipe"):
             break
         if (name == "fp64" and args.skip_fp64) or (name == "mixed" and args.only == "step64"):
             continue
@@ -573,10 +613,12 @@ def main():
         ctx, ham, prop, ws, Esh, eout = gpu_setup(problem, prec, W)
         log(f"[{name}] timing {args.steps} steps after {args.warmup} warmup")
         sec = time_steps(prop, ws, Esh, eout, args.warmup, args.steps)
+        state = walker_state(ws)
         tvs = time_vhs_in_step(prop, ws, Esh, eout)
         tv = time_vhs(ham, prec, W)
         te = time_energy(prop, ws, prec) if args.only is None else None
-        out[name] = dict(sec=sec, vhs=tvs, vhs_b2b=tv, energy=te)
+        out[name] = dict(sec=sec, vhs=tvs, vhs_b2b=tv, energy=te, state=state)
+        log(f"[{name}] walkers after the timed steps: {state}")
         log(f"[{name}] {sec * 1e3:.2f} ms/step, {W / sec:.4g} walker-steps/s, vHS {tvs * 1e3:.3f} ms "
             f"in the step ({tv * 1e3:.3f} ms back to back)"
             + (f", energy {te * 1e3:.2f} ms" if te else ""))
@@ -597,6 +639,14 @@ def main():
         log(f"[mixed] W={Ws}: {ss * 1e3:.3f} ms/step, {Ws / ss:.4g} walker-steps/s")
         del ctx, ham, prop, ws
         torch.cuda.empty_cache()
+
+    recipe = None
+    if args.only in (None, "recipe"):
+        log(f"recipe amplitude {RECIPE_AMPLITUDE}: timing")
+        recipe = recipe_workload(args.warmup, args.steps)
+        log(f"recipe amplitude {RECIPE_AMPLITUDE}: {recipe}")
+        if args.only == "recipe":
+            return
 
     if args.only != "serve":
         log("autotuned tiles:\n" + afq.tuning_report())
@@ -676,6 +726,7 @@ def main():
         "data": "synthetic (seeded hamiltonian, trial, fields)",
         "config": {"workload": "N=200 orbitals, 40+40 electrons, M=1600 Cholesky vectors, "
                                "1024 walkers, dt=0.005, taylor 6, QR every 5 steps",
+                   "cholesky_amplitude": AMPLITUDE,
                    "precision": "mixed", "graph": "one HIP graph per step"},
         # achieved: the vHS launch's average duration inside the replayed step graph (HIP
         # events on the library's stream); the same kernel launched back to back, alone, is
@@ -715,6 +766,10 @@ def main():
                        W / (out[name]["sec"] + out[name]["energy"] / 10)}
             for name in out if out[name].get("energy")},
         "shard_one_of_8": shard,
+        # the state of the walkers after the timed steps, and the same shapes at the generator's
+        # default Cholesky amplitude 0.5 (walkers dead within about four steps)
+        "walkers_after_timed_steps": {k: out[k]["state"] for k in out},
+        "recipe_amplitude_0.5": recipe,
         "energy_error_mixed_vs_fp64_Ha": None if emax is None else
             {"max": emax, "median": emed, "max_rel": erel, "median_abs_E": emag},
         "energy_error_mixed_vs_fp64_BH": bh_err,

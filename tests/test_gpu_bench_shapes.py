@@ -1,8 +1,9 @@
 """Parity at the benchmark's own shapes, through the C ABI.
 
 The benchmark workload is N=200 orbitals, 40+40 electrons (CLOSED), M=1600 Cholesky vectors,
-1024 walkers (safire_amd.synthetic seed 1003, the hamiltonian bench.py times).  The autotuner
-picks tiles per shape bucket, so these tests run at exactly that shape:
+1024 walkers (safire_amd.synthetic seed 1003, here at the generator's default Cholesky amplitude;
+the benchmark times the same shapes, hence the same kernels and tiles, at amplitude 0.1).  The
+autotuner picks tiles per shape bucket, so these tests run at exactly that shape:
 
 * the FP64 step of all 1024 walkers (tuned tiles, as the benchmark runs it) against the oracle
   on eight sampled walkers, 1e-10 relative per walker;

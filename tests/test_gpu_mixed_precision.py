@@ -7,14 +7,14 @@ bias vbias, the vHS contraction, the Taylor propagation) and all of them togethe
 them at least that much, so the floor bounds every FP32 propagation from below.
 
 Bounds asserted:
-* at the benchmark recipe's Cholesky amplitude 0.5 (local energies of ~5e4 Ha) the floor alone
+* at the generator's default Cholesky amplitude 0.5 (local energies of ~5e4 Ha) the floor alone
   exceeds 0.1 mHa, so that bar is out of reach for any FP32 propagation there;
 * at both amplitudes every FP32 stage, and the mixed step, stays within 100 times the floor
   (maximum over walkers of |dE| / |E|): an FP32 contraction over K terms carries ~sqrt(K) u
   relative error, sqrt(1600) = 40 for vHS, i.e. ~80 times the floor's u / 2 (measured: 20-60
   times at amplitude 0.5, 5-35 times at 0.1);
-* at Cholesky amplitude 0.1 (local energies of ~600 Ha) the mixed step meets 0.1 mHa per
-  walker (measured 9.3e-5 Ha at most over 128 walkers).
+* at Cholesky amplitude 0.1 (local energies of ~600 Ha; the benchmark's workload) the mixed
+  step meets 0.1 mHa per walker (measured 9.3e-5 Ha at most over 128 walkers).
 """
 import os
 import sys
@@ -49,7 +49,7 @@ def budget():
     return out
 
 
-def test_floor_exceeds_the_bar_at_the_benchmark_amplitude(budget):
+def test_floor_exceeds_the_bar_at_the_default_amplitude(budget):
     b = budget[0.5]
     assert b["floor"]["max_Ha"] > 1e-4
     assert b["floor"]["max_Ha"] <= b["mixed"]["max_Ha"]
