@@ -16,7 +16,6 @@
 #include "gemm.hpp"
 #include "ops.hpp"
 #include "tuner.hpp"
-#include "zgemm_rt.hpp"
 
 namespace afq {
 
@@ -82,20 +81,6 @@ using gemm::EpiStore2;
 using gemm::EpiTaylor;
 using gemm::Opnd;
 
-// A small batched complex FP64 product: the register-tiled kernel (its blockings tuned per
-// shape) where it applies, else the LDS-staged engine.
-template <class OA, class OB, class EPI>
-void zgemm_small(Context& c, const char* site, int W, int M, int N, int K, const OA& A,
-                 const OB& B, const EPI& e) {
-  if (!zrt::supported(M, N, K, A, B)) {
-    gemm_small<double>(c, site, W, M, N, K, A, B, e);
-    return;
-  }
-  hipStream_t s = c.stream();
-  tuned(c, tune_key((std::string(site) + "_rt").c_str(), sizeof(double), W, M, N, K),
-        zrt::kCandidates, [&](int i) { zrt::launch(s, W, M, N, K, A, B, e, i); }, true);
-}
-
 void log_overlap(Context& c, const z64* PsiT, int n, int N, const z64* Phi, long phi_bs, int W,
                  z64* ovlp, double scale, z64 add, const LuKeep* keep) {
   if (n == 0 || W == 0) return;
@@ -153,7 +138,7 @@ void mixed_density_matrix(Context& c, const z64* PsiT, int n, int N, const z64* 
     // G[i][k] = sum_a PsiT[a][i] Gc[a][k]           (product<'T'>(A, TNM, C))
     Opnd<double, true, false> A{&PsiT->re, N, 0};
     Opnd<double, true, false> B{&Gc->re, N, (long)n * N};
-    gemm_small<double>(c, "mdm_full", W, N, N, n, A, B, EpiStore<double, double>{G, N, g_bs, 1, 0, 0});
+    zgemm_small(c, "mdm_full", W, N, N, n, A, B, EpiStore<double, double>{G, N, g_bs, 1, 0, 0});
   }
 }
 
@@ -166,19 +151,19 @@ void mixed_density_matrix_left(Context& c, const z64* L, long l_bs, int n, int N
   {  // T[a][b] = sum_k conj(L[k][a]) Phi[k][b]
     Opnd<double, true, false, true> A{&L->re, n, l_bs};
     Opnd<double, true, false> B{&Phi->re, n, phi_bs};
-    gemm_small<double>(c, "mdl_T", W, n, n, N, A, B, EpiStore<double, double>{T, n, (long)n * n, 1, 0, 0});
+    zgemm_small(c, "mdl_T", W, n, n, N, A, B, EpiStore<double, double>{T, n, (long)n * n, 1, 0, 0});
   }
   lu_inverse_logdet(s, T, (long)n * n, W, n, T, (long)n * n, ovlp, 1.0, {0, 0}, true);
   z64* Gc = scratch_as<z64>(c, "mdl_Gc", (size_t)W * n * N);
   {  // Gc[a][k] = sum_b Tinv[b][a] Phi[k][b]
     Opnd<double, true, false> A{&T->re, n, (long)n * n};
     Opnd<double, true, true> B{&Phi->re, n, phi_bs};
-    gemm_small<double>(c, "mdm_G", W, n, N, n, A, B, EpiStore<double, double>{Gc, N, (long)n * N, 1, 0, 0});
+    zgemm_small(c, "mdm_G", W, n, N, n, A, B, EpiStore<double, double>{Gc, N, (long)n * N, 1, 0, 0});
   }
   {  // G[i][k] = sum_a conj(L[i][a]) Gc[a][k]
     Opnd<double, true, true, true> A{&L->re, n, l_bs};
     Opnd<double, true, false> B{&Gc->re, N, (long)n * N};
-    gemm_small<double>(c, "mdl_full", W, N, N, n, A, B, EpiStore<double, double>{G, N, g_bs, 1, 0, 0});
+    zgemm_small(c, "mdl_full", W, N, N, n, A, B, EpiStore<double, double>{G, N, g_bs, 1, 0, 0});
   }
 }
 
@@ -220,7 +205,7 @@ void log_overlap_left(Context& c, const z64* L, long l_bs, int n, int N, const z
   z64* T = scratch_as<z64>(c, "lol_T", (size_t)W * n * n);
   Opnd<double, true, false, true> A{&L->re, n, l_bs};
   Opnd<double, true, false> B{&Phi->re, n, phi_bs};
-  gemm_small<double>(c, "mdl_T", W, n, n, N, A, B, EpiStore<double, double>{T, n, (long)n * n, 1, 0, 0});
+  zgemm_small(c, "mdl_T", W, n, n, N, A, B, EpiStore<double, double>{T, n, (long)n * n, 1, 0, 0});
   lu_inverse_logdet(c.stream(), T, (long)n * n, W, n, nullptr, 0, ovlp, 1.0, {0, 0}, false);
 }
 
@@ -851,7 +836,7 @@ void orthogonalize(Context& c, z64* Phi, long phi_bs, int W, int N, int n, z64* 
     {  // S = X^H X
       Opnd<double, true, false, true> A{&X->re, n, xbs};
       Opnd<double, true, false> B{&X->re, n, xbs};
-      gemm_small<double>(c, "qr_gram", W, n, n, N, A, B, EpiStore<double, double>{S, n, (long)n * n, 1, 0, 0});
+      zgemm_small(c, "qr_gram", W, n, n, N, A, B, EpiStore<double, double>{S, n, (long)n * n, 1, 0, 0});
     }
     chol_upper_inverse(s, S, W, n, Ri, ldet);
     {  // Y = X R^{-1}

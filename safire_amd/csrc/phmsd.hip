@@ -896,7 +896,7 @@ void PhmsdTrial::woodbury(int s, const z64* Phi, long phi_bs, int W, bool want_G
   {  // TAB = orbs * Phi
     Opnd<double, true, true> A{&d.orbs->re, N, 0};
     Opnd<double, true, false> B{&Phi->re, n, phi_bs};
-    gemm_small<double>(ctx_, "ph_TAB", W, na, n, N, A, B,
+    zgemm_small(ctx_, "ph_TAB", W, na, n, N, A, B,
                        EpiStore<double, double>{TAB, n, (long)na * n, 1, 0, 0});
   }
   lu_inverse_logdet(ctx_.stream(), TAB, (long)na * n, W, n, Ti, (long)n * n, log_ov, 1.0, {0, 0},
@@ -904,14 +904,14 @@ void PhmsdTrial::woodbury(int s, const z64* Phi, long phi_bs, int W, bool want_G
   if (d.nvir > 0) {  // QQ0 particle rows = TAB[nel:] * TNN^{-1}
     Opnd<double, true, true> A{&(TAB + (size_t)n * n)->re, n, (long)na * n};
     Opnd<double, true, false> B{&Ti->re, n, (long)n * n};
-    gemm_small<double>(ctx_, "ph_QQ0", W, d.nvir, n, n, A, B,
+    zgemm_small(ctx_, "ph_QQ0", W, d.nvir, n, n, A, B,
                        EpiStore<double, double>{Qv, n, (long)d.nvir * n, 1, 0, 0});
   }
   if (want_G) {  // GA = TNN^{-T} Phi^T
     z64* GA = scratch_as<z64>(ctx_, tag("ph_GA", s), (size_t)W * n * N);
     Opnd<double, true, false> A{&Ti->re, n, (long)n * n};
     Opnd<double, true, true> B{&Phi->re, n, phi_bs};
-    gemm_small<double>(ctx_, "mdm_G", W, n, N, n, A, B,
+    zgemm_small(ctx_, "mdm_G", W, n, N, n, A, B,
                        EpiStore<double, double>{GA, N, (long)n * N, 1, 0, 0});
   }
 }
@@ -1006,7 +1006,7 @@ void PhmsdTrial::mixed_density_matrix(const z64* Pa, const z64* Pb, int W, z64* 
     const z64* GA = scratch_as<z64>(ctx_, tag("ph_GA", sp), (size_t)W * d.nel * N);
     Opnd<double, true, false> A{&R->re, d.nact, (long)d.nel * d.nact};
     Opnd<double, true, false> B{&GA->re, N, (long)d.nel * N};
-    gemm_small<double>(ctx_, "ph_G", W, d.nact, N, d.nel, A, B,
+    zgemm_small(ctx_, "ph_G", W, d.nact, N, d.nel, A, B,
                        EpiStore<double, double>{G + (sp == 0 ? 0 : (size_t)sp_[0].nact * N), N,
                                                 (long)ntot * N, 1, 0, 0});
   }

@@ -21,6 +21,7 @@
 #include <string>
 
 #include "gemm.hpp"
+#include "zgemm_rt.hpp"
 #include "ops.hpp"
 
 namespace afq {
@@ -327,6 +328,20 @@ void gemm_small(Context& c, const char* site, int batch, int M, int N, int K, co
       default: gemm::launch<T, 48, 48, 16, 1, 1>(s, batch, M, N, K, A, B, e); break;
     }
   }, idempotent);
+}
+
+// A small batched complex FP64 product: the register-tiled kernel (its blockings tuned per
+// shape) where it applies, else the LDS-staged engine.
+template <class OA, class OB, class EPI>
+void zgemm_small(Context& c, const char* site, int W, int M, int N, int K, const OA& A,
+                 const OB& B, const EPI& e) {
+  if (!zrt::supported(M, N, K, A, B)) {
+    gemm_small<double>(c, site, W, M, N, K, A, B, e);
+    return;
+  }
+  hipStream_t s = c.stream();
+  tuned(c, tune_key((std::string(site) + "_rt").c_str(), sizeof(double), W, M, N, K),
+        zrt::kCandidates, [&](int i) { zrt::launch(s, W, M, N, K, A, B, e, i); }, true);
 }
 
 }  // namespace afq
