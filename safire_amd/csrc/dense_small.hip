@@ -63,11 +63,13 @@ __device__ inline void ov_update(z64* ovlp, long b, double lr, double li, double
 // keep_ldet) were kept; when matrix b matches it bit for bit (no NaN), the kept values are
 // exact for it: ovlp is updated from keep_ldet and nothing else is written.  Otherwise keepT
 // takes matrix b and the factors are recomputed into Ainv and keep_ldet.
-template <int NMAX, bool INV>
+// NOPIV: no pivoting, for upper triangular matrices only (the Cholesky factor's inverse): row j
+// is zero left of column j at step j, so columns < j are not updated.
+template <int NMAX, bool INV, bool NOPIV = false>
 __global__ void __launch_bounds__(64)
     k_lu_rows(const z64* __restrict__ A, long a_bs, int n, z64* __restrict__ Ainv, long i_bs,
               z64* __restrict__ ovlp, double ov_scale, double ov_add_re, double ov_add_im,
-              int no_pivot, z64* __restrict__ keepT, z64* __restrict__ keep_ldet, int reuse) {
+              z64* __restrict__ keepT, z64* __restrict__ keep_ldet, int reuse) {
   __shared__ int pivrow[64];
   const int lane = threadIdx.x;
   const long b = blockIdx.x;
@@ -80,7 +82,7 @@ __global__ void __launch_bounds__(64)
     xi[c] = v.im;
   }
   // (rows of 64 columns fill the register file: no kept factors there)
-  if constexpr (INV && NMAX <= 48) {
+  if constexpr (INV && !NOPIV && NMAX <= 48) {
     if (reuse) {
       z64* Kb = keepT + b * a_bs;
       bool same = true;
@@ -133,7 +135,7 @@ __global__ void __launch_bounds__(64)
       double mag = fabs(fr) + fabs(fi);
       double best = used ? -1.0 : (mag == mag ? mag : -0.5);
       int bpos = used ? 1 << 20 : pos;
-      if (no_pivot) {
+      if (NOPIV) {
         best = (pos == j) ? 1.0 : -1.0;
       }
 #pragma unroll
@@ -176,6 +178,7 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
       for (int c = 0; c < NMAX; ++c) {
         if (!INV && c <= j) continue;  // uniform: dead columns of the log det alone
+        if (NOPIV && c < j) continue;  // uniform: zero in the pivot row
         const double pr = rdl(xr[c], p), pim = rdl(xi[c], p);
         // x - h p as four fused multiply-adds
         double nr = fma(hi, pim, fma(-hr, pr, xr[c]));
@@ -343,27 +346,27 @@ void lu_inverse_logdet(hipStream_t s, const z64* A, long a_bs, int nb, int n, z6
   z64* kT = keep && reuse ? keep->T : nullptr;
   z64* kl = keep ? keep->ldet : nullptr;
   const int ru = keep && reuse ? 1 : 0;
-#define AFQ_LU_ROWS(NM, NP)                                                                   \
+#define AFQ_LU_ROWS(NM)                                                                       \
   do {                                                                                        \
     if (wi)                                                                                   \
       hipLaunchKernelGGL((k_lu_rows<NM, true>), dim3(nb), dim3(64), 0, s, A, a_bs, n, Ainv,   \
-                         i_bs, ovlp, ov_scale, ov_add.re, ov_add.im, NP, kT, kl, ru);         \
+                         i_bs, ovlp, ov_scale, ov_add.re, ov_add.im, kT, kl, ru);             \
     else                                                                                      \
       hipLaunchKernelGGL((k_lu_rows<NM, false>), dim3(nb), dim3(64), 0, s, A, a_bs, n, Ainv,  \
-                         i_bs, ovlp, ov_scale, ov_add.re, ov_add.im, NP, kT, kl, ru);         \
+                         i_bs, ovlp, ov_scale, ov_add.re, ov_add.im, kT, kl, ru);             \
   } while (0)
   if (n <= 8)
-    AFQ_LU_ROWS(8, 0);
+    AFQ_LU_ROWS(8);
   else if (n <= 16)
-    AFQ_LU_ROWS(16, 0);
+    AFQ_LU_ROWS(16);
   else if (n <= 32)
-    AFQ_LU_ROWS(32, 0);
+    AFQ_LU_ROWS(32);
   else if (n <= 40)
-    AFQ_LU_ROWS(40, 0);
+    AFQ_LU_ROWS(40);
   else if (n <= 48)
-    AFQ_LU_ROWS(48, 0);
+    AFQ_LU_ROWS(48);
   else
-    AFQ_LU_ROWS(64, 0);
+    AFQ_LU_ROWS(64);
   AFQ_LAUNCH_CHECK();
 }
 
@@ -381,23 +384,23 @@ void chol_upper_inverse(hipStream_t s, const z64* S, int nb, int n, z64* Rinv, z
   AFQ_LAUNCH_CHECK();
   const long nn = (long)n * n;
   if (n <= 8)
-    hipLaunchKernelGGL((k_lu_rows<8, true>), dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet,
-                       1.0, 0.0, 0.0, 1, nullptr, nullptr, 0);
+    hipLaunchKernelGGL((k_lu_rows<8, true, true>), dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn,
+                       ldet, 1.0, 0.0, 0.0, nullptr, nullptr, 0);
   else if (n <= 16)
-    hipLaunchKernelGGL((k_lu_rows<16, true>), dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet,
-                       1.0, 0.0, 0.0, 1, nullptr, nullptr, 0);
+    hipLaunchKernelGGL((k_lu_rows<16, true, true>), dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn,
+                       ldet, 1.0, 0.0, 0.0, nullptr, nullptr, 0);
   else if (n <= 32)
-    hipLaunchKernelGGL((k_lu_rows<32, true>), dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet,
-                       1.0, 0.0, 0.0, 1, nullptr, nullptr, 0);
+    hipLaunchKernelGGL((k_lu_rows<32, true, true>), dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn,
+                       ldet, 1.0, 0.0, 0.0, nullptr, nullptr, 0);
   else if (n <= 40)
-    hipLaunchKernelGGL((k_lu_rows<40, true>), dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet,
-                       1.0, 0.0, 0.0, 1, nullptr, nullptr, 0);
+    hipLaunchKernelGGL((k_lu_rows<40, true, true>), dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn,
+                       ldet, 1.0, 0.0, 0.0, nullptr, nullptr, 0);
   else if (n <= 48)
-    hipLaunchKernelGGL((k_lu_rows<48, true>), dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet,
-                       1.0, 0.0, 0.0, 1, nullptr, nullptr, 0);
+    hipLaunchKernelGGL((k_lu_rows<48, true, true>), dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn,
+                       ldet, 1.0, 0.0, 0.0, nullptr, nullptr, 0);
   else
-    hipLaunchKernelGGL((k_lu_rows<64, true>), dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn, ldet,
-                       1.0, 0.0, 0.0, 1, nullptr, nullptr, 0);
+    hipLaunchKernelGGL((k_lu_rows<64, true, true>), dim3(nb), dim3(64), 0, s, Rinv, nn, n, Rinv, nn,
+                       ldet, 1.0, 0.0, 0.0, nullptr, nullptr, 0);
   AFQ_LAUNCH_CHECK();
 }
 
