@@ -842,7 +842,7 @@ void orthogonalize(Context& c, z64* Phi, long phi_bs, int W, int N, int n, z64* 
     {  // Y = X R^{-1}
       Opnd<double, true, true> A{&X->re, n, xbs};
       Opnd<double, true, false> B{&Ri->re, n, (long)n * n};
-      gemm_tall<double>(c, "qr_Q", W, N, n, n, A, B, EpiStore<double, double>{Y, n, ybs, 1, 0, 0});
+      zgemm_small(c, "qr_Q", W, N, n, n, A, B, EpiStore<double, double>{Y, n, ybs, 1, 0, 0});
     }
   }
 }
