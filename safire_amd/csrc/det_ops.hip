@@ -355,12 +355,15 @@ __device__ inline void load_group(AFrag<T, CPLX> (&f)[SPW], const T* __restrict_
 // acc = A[stripe rows][0:N] * B[0:N][tiles] for the wave's stripes (complex B in LDS). The A
 // fragments of the next two k groups are in flight while a group is multiplied.  M3: complex
 // products by Gauss's three-multiplication method (three MFMAs per complex block instead of four).
+// The full k groups [gb, ge) (ge < 0: all of them), then the ragged last group if `ragged`: a
+// part of the product for a split of k over several wavefronts.
 template <typename T, bool CPLX, int SPW, int CB, bool VEC, bool CONJ = false, bool M3 = false>
 __device__ inline void stripes_times_b(const T* __restrict__ A, long lda, int N, int NS,
                                        int NW, int wave, int fr, int fk, const T* Br,
                                        const T* Bi, int LDK,
                                        typename gemm::Acc<T>::v4 (&ar)[SPW][CB],
-                                       typename gemm::Acc<T>::v4 (&ai)[SPW][CB]) {
+                                       typename gemm::Acc<T>::v4 (&ai)[SPW][CB], int gb = 0,
+                                       int ge = -1, bool ragged = true) {
   using V4 = typename gemm::Acc<T>::v4;
   constexpr bool G3 = CPLX && M3;
   V4 a3[SPW][CB];
@@ -377,27 +380,28 @@ __device__ inline void stripes_times_b(const T* __restrict__ A, long lda, int N,
   // last full group reload it rather than branching around the load); the scheduling barriers
   // keep each group's loads ahead of the previous group's MFMAs
   const int NGF = N / 16;
-  if (NGF > 0) {
-    load_group<T, CPLX, SPW, VEC, true>(f0, A, lda, N, NS, NW, wave, fr, fk, 0);
-    load_group<T, CPLX, SPW, VEC, true>(f1, A, lda, N, NS, NW, wave, fr, fk, min(1, NGF - 1));
-    for (int g = 0; g < NGF; g += 3) {
-      load_group<T, CPLX, SPW, VEC, true>(f2, A, lda, N, NS, NW, wave, fr, fk, min(g + 2, NGF - 1));
+  const int g1 = ge < 0 ? NGF : ge;
+  if (g1 > gb) {
+    load_group<T, CPLX, SPW, VEC, true>(f0, A, lda, N, NS, NW, wave, fr, fk, gb);
+    load_group<T, CPLX, SPW, VEC, true>(f1, A, lda, N, NS, NW, wave, fr, fk, min(gb + 1, g1 - 1));
+    for (int g = gb; g < g1; g += 3) {
+      load_group<T, CPLX, SPW, VEC, true>(f2, A, lda, N, NS, NW, wave, fr, fk, min(g + 2, g1 - 1));
       __builtin_amdgcn_sched_barrier(0);
       group_mfma<T, CPLX, SPW, CB, CONJ, G3>(f0, g, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai, a3);
       __builtin_amdgcn_sched_barrier(0);
-      if (g + 1 >= NGF) break;
-      load_group<T, CPLX, SPW, VEC, true>(f0, A, lda, N, NS, NW, wave, fr, fk, min(g + 3, NGF - 1));
+      if (g + 1 >= g1) break;
+      load_group<T, CPLX, SPW, VEC, true>(f0, A, lda, N, NS, NW, wave, fr, fk, min(g + 3, g1 - 1));
       __builtin_amdgcn_sched_barrier(0);
       group_mfma<T, CPLX, SPW, CB, CONJ, G3>(f1, g + 1, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai, a3);
       __builtin_amdgcn_sched_barrier(0);
-      if (g + 2 >= NGF) break;
-      load_group<T, CPLX, SPW, VEC, true>(f1, A, lda, N, NS, NW, wave, fr, fk, min(g + 4, NGF - 1));
+      if (g + 2 >= g1) break;
+      load_group<T, CPLX, SPW, VEC, true>(f1, A, lda, N, NS, NW, wave, fr, fk, min(g + 4, g1 - 1));
       __builtin_amdgcn_sched_barrier(0);
       group_mfma<T, CPLX, SPW, CB, CONJ, G3>(f2, g + 2, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai, a3);
       __builtin_amdgcn_sched_barrier(0);
     }
   }
-  if (NGF < NS) {  // the ragged last k group, masked
+  if (ragged && NGF < NS) {  // the ragged last k group, masked
     load_group<T, CPLX, SPW, VEC, false>(f0, A, lda, N, NS, NW, wave, fr, fk, NGF);
     group_mfma<T, CPLX, SPW, CB, CONJ, G3>(f0, NGF, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai, a3);
   }
@@ -462,7 +466,7 @@ __device__ inline void write_b(T* Br, T* Bi, int LDK, int N, int NS, int NW, int
 // pitch LDK), which hold walker columns c0, c0 + 1, ...  Every wavefront of the workgroup passes
 // the same sequence of barriers.  CONJV: the V products use conj(V), which is V^H for the
 // complex-symmetric V of vHS (the adjoint propagator, sgn = -1).
-template <typename T, bool P1C, int SPW, int CB, bool CONJV, bool M3, bool VEC>
+template <typename T, bool P1C, int SPW, int CB, bool CONJV, bool M3, bool VEC, bool XB = false>
 __device__ __attribute__((always_inline)) inline void prop_share(
     int N, int n, int order, double sgn, const T* __restrict__ P1, const T* __restrict__ Vw,
     z64* __restrict__ Pw, int c0, int NS, int NW, int wave, int fr, int fk, T* Br, T* Bi, int LDK,
@@ -506,6 +510,7 @@ __device__ __attribute__((always_inline)) inline void prop_share(
   __syncthreads();
   // Phi = P1 S, scaled back by 2^e
   stripes_times_b<T, P1C, SPW, CB, VEC>(P1, N, N, NS, NW, wave, fr, fk, Br, Bi, LDK, ar, ai);
+  if constexpr (XB) __syncthreads();  // the split wavefronts' last partials (prop_share_split)
   const double sb = e == 0 ? 1.0 : ldexp(1.0, e);
 #pragma unroll
   for (int s = 0; s < SPW; ++s) {
@@ -521,13 +526,99 @@ __device__ __attribute__((always_inline)) inline void prop_share(
   }
 }
 
+// One of the P wavefronts that share stripe st (one 16-column tile): part `part` multiplies the
+// full k groups [part NGF / P, (part + 1) NGF / P) (the last part also the ragged group) and
+// leaves its partial product in red [tail wavefront][lane]; after the barrier the owner (part 0)
+// sums the P partials in part order and carries the stripe's share of the recurrence (the running
+// sum, the LDS operand and the final store).  The barrier sequence is prop_share's with XB.
+template <typename T, bool P1C, bool CONJV, bool M3, bool VEC>
+__device__ __attribute__((always_inline)) inline void prop_share_split(
+    int N, int n, int order, double sgn, const T* __restrict__ P1, const T* __restrict__ Vw,
+    z64* __restrict__ Pw, int c0, int NS, int st, int part, int P, int tw, int fr, int fk, T* Br,
+    T* Bi, int LDK, int e, typename gemm::Acc<T>::v4* red) {
+  using V4 = typename gemm::Acc<T>::v4;
+  const int lane = fr + 16 * fk;
+  const int NGF = N / 16, gb = part * NGF / P, ge = (part + 1) * NGF / P;
+  const bool rag = part == P - 1;
+  const bool owner = part == 0;
+  V4 ar[1][1], ai[1][1], sr[1][1], si[1][1];
+  // partials -> LDS; after the barrier the owner sums them into (ar, ai)
+  auto put = [&]() {
+    red[(2 * tw) * 64 + lane] = ar[0][0];
+    red[(2 * tw + 1) * 64 + lane] = ai[0][0];
+  };
+  auto gather = [&]() {
+    const int t0 = tw - part;
+    ar[0][0] = red[(2 * t0) * 64 + lane];
+    ai[0][0] = red[(2 * t0 + 1) * 64 + lane];
+    for (int q = 1; q < P; ++q) {
+      ar[0][0] += red[(2 * (t0 + q)) * 64 + lane];
+      ai[0][0] += red[(2 * (t0 + q) + 1) * 64 + lane];
+    }
+  };
+  // T_0 = P1 Phi, S = T_0
+  stripes_times_b<T, P1C, 1, 1, VEC>(P1, N, N, NS, NS, st, fr, fk, Br, Bi, LDK, ar, ai, gb, ge,
+                                      rag);
+  put();
+  __syncthreads();
+  if (owner) {
+    gather();
+    sr[0][0] = ar[0][0];
+    si[0][0] = ai[0][0];
+    write_b<T, 1, 1>(Br, Bi, LDK, N, NS, NS, st, fr, fk, ar, ai);
+  }
+  __syncthreads();
+  for (int k = 1; k <= order; ++k) {
+    stripes_times_b<T, true, 1, 1, VEC, CONJV, M3>(Vw, N, N, NS, NS, st, fr, fk, Br, Bi, LDK, ar,
+                                                   ai, gb, ge, rag);
+    put();
+    __syncthreads();
+    if (owner) {
+      gather();
+      const T fi = (T)(sgn / (double)k);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const T tr = -(fi * ai[0][0][r]), ti = fi * ar[0][0][r];
+        ar[0][0][r] = tr;
+        ai[0][0][r] = ti;
+        sr[0][0][r] = sr[0][0][r] + tr;
+        si[0][0][r] = si[0][0][r] + ti;
+      }
+      write_b<T, 1, 1>(Br, Bi, LDK, N, NS, NS, st, fr, fk, ar, ai);
+    }
+    __syncthreads();
+  }
+  if (owner) write_b<T, 1, 1>(Br, Bi, LDK, N, NS, NS, st, fr, fk, sr, si);
+  __syncthreads();
+  // Phi = P1 S, scaled back by 2^e
+  stripes_times_b<T, P1C, 1, 1, VEC>(P1, N, N, NS, NS, st, fr, fk, Br, Bi, LDK, ar, ai, gb, ge,
+                                      rag);
+  put();
+  __syncthreads();
+  if (owner) {
+    gather();
+    const double sb = e == 0 ? 1.0 : ldexp(1.0, e);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = st * 16 + gemm::Acc<T>::row(fk, r), col = c0 + fr;
+      if (row < N && col < n)
+        Pw[(long)row * n + col] = z64{(double)ar[0][0][r] * sb, (double)ai[0][0][r] * sb};
+    }
+  }
+}
+
 // One workgroup per (walker, CB 16-column tiles).  Without TAIL, NW wavefronts own the stripes
 // (SPW each).  TAIL (SPW = 1): NW, a multiple of 4, wavefronts own the first NW stripes with all
 // CB tiles, and one more wavefront per (stripe, tile) of the remaining NS - NW stripes owns that
 // single tile, so the four SIMDs carry nearly equal shares (N = 200, CB = 3: 10, 10, 10 and 9
 // stripe-tiles instead of 12, 9, 9 and 9).  Every output tile is multiplied in the same order
 // either way, so the variants give the same bits.
-template <typename T, bool P1C, int SPW, int CB, bool CONJV, bool M3, bool VEC, bool TAIL>
+// KT (CB = 1, SPW = 1): NW (a multiple of 4) wavefronts own the first NW stripes, and the
+// remaining R = NS - NW (1 or 2) stripes are each split over 4 / R more wavefronts by k
+// (prop_share_split), so the four SIMDs carry equal shares.  The split stripes sum their k range
+// in parts, so their bits differ from the unsplit kernel's.
+template <typename T, bool P1C, int SPW, int CB, bool CONJV, bool M3, bool VEC, bool TAIL,
+          bool KT = false>
 __global__ void __launch_bounds__(1024)
     k_prop_taylor(int W, int N, int n, int order, double sgn, const T* __restrict__ P1,
                   const T* __restrict__ V, long v_bs, z64* __restrict__ Phi, long phi_bs,
@@ -594,9 +685,14 @@ __global__ void __launch_bounds__(1024)
     }
   }
   __syncthreads();
-  if (!TAIL || wave < NW) {
-    prop_share<T, P1C, SPW, CB, CONJV, M3, VEC>(N, n, order, sgn, P1, Vw, Pw, c0, NS, NW, wave,
-                                                fr, fk, Br, Bi, LDK, e);
+  if ((!TAIL && !KT) || wave < NW) {
+    prop_share<T, P1C, SPW, CB, CONJV, M3, VEC, KT>(N, n, order, sgn, P1, Vw, Pw, c0, NS, NW,
+                                                    wave, fr, fk, Br, Bi, LDK, e);
+  } else if constexpr (KT) {
+    const int tw = wave - NW, P = 4 / (NS - NW);
+    auto* red = reinterpret_cast<typename gemm::Acc<T>::v4*>(Bi + BC * LDK);
+    prop_share_split<T, P1C, CONJV, M3, VEC>(N, n, order, sgn, P1, Vw, Pw, c0, NS, NW + tw / P,
+                                             tw % P, P, tw, fr, fk, Br, Bi, LDK, e, red);
   } else if constexpr (TAIL) {
     const int u = wave - NW, st = NW + u / CB, tt = u % CB;
     prop_share<T, P1C, 1, 1, CONJV, M3, VEC>(N, n, order, sgn, P1, Vw, Pw, c0 + 16 * tt, NS, NW,
@@ -622,18 +718,25 @@ inline int max_lds_per_block() {
 
 // TAIL: the first NS rounded down to a multiple of 4 stripes take one wavefront each, the rest
 // one wavefront per (stripe, tile)
-template <typename T, bool P1C, int SPW, int CB, bool CONJV, bool M3, bool TAIL = false>
+// KT: the first NS rounded down to a multiple of 4 stripes take one wavefront each and four more
+// wavefronts split the rest by k (16 KB of LDS for their partials)
+template <typename T, bool P1C, int SPW, int CB, bool CONJV, bool M3, bool TAIL = false,
+          bool KT = false>
 void launch_fused(hipStream_t s, const PropagateArgs& a, const FusedShape& f, const T* P1, int vecP,
                   const T* V, int vecV, z64* Phi) {
-  const size_t lds = 2 * 16 * CB * (size_t)f.LDK * sizeof(T);
-  auto kern = (vecP && vecV) ? k_prop_taylor<T, P1C, SPW, CB, CONJV, M3, true, TAIL>
-                             : k_prop_taylor<T, P1C, SPW, CB, CONJV, M3, false, TAIL>;
+  static_assert(!KT || (CB == 1 && SPW == 1 && !TAIL), "the split tail takes one tile per wave");
+  const size_t lds = 2 * 16 * CB * (size_t)f.LDK * sizeof(T) +
+                     (KT ? 4 * 2 * 64 * sizeof(typename gemm::Acc<T>::v4) : 0);
+  auto kern = (vecP && vecV) ? k_prop_taylor<T, P1C, SPW, CB, CONJV, M3, true, TAIL, KT>
+                             : k_prop_taylor<T, P1C, SPW, CB, CONJV, M3, false, TAIL, KT>;
   if (lds > 64 * 1024)
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   const int NS = ceil_div(a.N, 16);
-  const int nw = TAIL ? (NS & ~3) : f.NW;
-  const int waves = TAIL ? nw + (NS - nw) * CB : f.NW;
+  const int nw = (TAIL || KT) ? (NS & ~3) : f.NW;
+  const int waves = TAIL ? nw + (NS - nw) * CB : KT ? nw + 4 : f.NW;
+  AFQ_CHECK(!KT || ((NS - nw == 1 || NS - nw == 2) && waves <= 16 && nw >= 4), kErrShape,
+            "split tail shape");
   // the FP32 tile load keeps 4 CB SPW entries per thread in registers
   AFQ_CHECK(sizeof(T) == 8 || 16L * CB * NS * 16 <= 4L * CB * SPW * 64 * waves, kErrShape,
             "propagation tile larger than the FP32 load covers");
@@ -694,6 +797,13 @@ bool propagate_fused(Context& c, const PropagateArgs& a, const T* P1, bool p1c, 
   // keeps the FP32 rounding of the terms closer to the FP64 path's)
   const char* m3e = std::getenv("AFQ_PROP_M3");
   const bool m3 = sizeof(T) == 8 || (m3e && m3e[0] == '1');
+  // FP64 (one stripe and one column tile per wavefront): split the stripes past a multiple of 4
+  // by k when 1 or 2 remain (AFQ_PROP_SPLIT=0 keeps one wavefront per stripe)
+  const char* kte = std::getenv("AFQ_PROP_SPLIT");
+  const int rem = NS % 4;
+  const bool kt = sizeof(T) == 8 && f.spw == 1 && (rem == 1 || rem == 2) && NS >= 5 &&
+                  (NS & ~3) + 4 <= 16 && !(kte && kte[0] == '0') &&
+                  lds1 + 4 * 2 * 64 * 4 * sizeof(T) <= (size_t)max_lds_per_block();
   auto run = [&](int i, z64* Phi) {
     const int cb = cands[i].cb;
     const bool tail = cands[i].tail;
@@ -719,6 +829,9 @@ bool propagate_fused(Context& c, const PropagateArgs& a, const T* P1, bool p1c, 
           else if (cb == 2) tail ? AFQ_PT(false, 1, 2, true) : AFQ_PT(false, 1, 2, false);
           else tail ? AFQ_PT(false, 1, 3, true) : AFQ_PT(false, 1, 3, false);
         }
+      } else if (kt) {  // FP64: the stripes past a multiple of 4 split by k over 4 wavefronts
+        if (p1c) launch_fused<T, true, 1, 1, false, true, false, true>(s, a, f, P1, vecP, Vt, vecV, Phi);
+        else launch_fused<T, false, 1, 1, false, true, false, true>(s, a, f, P1, vecP, Vt, vecV, Phi);
       } else {
         if (p1c) AFQ_PT(true, 1, 1, false);
         else AFQ_PT(false, 1, 1, false);
