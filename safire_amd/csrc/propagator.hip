@@ -319,8 +319,10 @@ void Propagator::bound_stats(long long out[9], bool reset) {
 
 void Propagator::set_vhs_timing(bool on) {
   if (on && !vhs_ev_[0]) {
-    AFQ_HIP(hipEventCreate(&vhs_ev_[0]));
-    AFQ_HIP(hipEventCreate(&vhs_ev_[1]));
+    // no system-scope fence at the records: the fence's cache write-back would be timed with
+    // the kernel
+    AFQ_HIP(hipEventCreateWithFlags(&vhs_ev_[0], hipEventDisableSystemFence));
+    AFQ_HIP(hipEventCreateWithFlags(&vhs_ev_[1], hipEventDisableSystemFence));
   }
   if (on != vhs_timing_) {
     AFQ_HIP(hipStreamSynchronize(ctx_.stream()));
