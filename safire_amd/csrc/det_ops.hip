@@ -531,7 +531,7 @@ __device__ __attribute__((always_inline)) inline void prop_share(
 // leaves its partial product in red [tail wavefront][lane]; after the barrier the owner (part 0)
 // sums the P partials in part order and carries the stripe's share of the recurrence (the running
 // sum, the LDS operand and the final store).  The barrier sequence is prop_share's with XB.
-template <typename T, bool P1C, bool CONJV, bool M3, bool VEC>
+template <typename T, bool P1C, int CB, bool CONJV, bool M3, bool VEC>
 __device__ __attribute__((always_inline)) inline void prop_share_split(
     int N, int n, int order, double sgn, const T* __restrict__ P1, const T* __restrict__ Vw,
     z64* __restrict__ Pw, int c0, int NS, int st, int part, int P, int tw, int fr, int fk, T* Br,
@@ -541,69 +541,97 @@ __device__ __attribute__((always_inline)) inline void prop_share_split(
   const int NGF = N / 16, gb = part * NGF / P, ge = (part + 1) * NGF / P;
   const bool rag = part == P - 1;
   const bool owner = part == 0;
-  V4 ar[1][1], ai[1][1], sr[1][1], si[1][1];
-  // partials -> LDS; after the barrier the owner sums them into (ar, ai)
+  V4 ar[1][CB], ai[1][CB];
+  // red [tail wavefront][tile][re, im][lane], then the owner's running sum S [tile][re, im][lane]
+  // (in LDS rather than registers, which the products need)
+  auto slot = [&](int w, int t, int c) { return ((w * CB + t) * 2 + c) * 64 + lane; };
+  const int own = tw / P;  // the owner's index among the split stripes (0 or 1)
+  auto sslot = [&](int t, int c) { return ((4 * CB + own * CB + t) * 2 + c) * 64 + lane; };
   auto put = [&]() {
-    red[(2 * tw) * 64 + lane] = ar[0][0];
-    red[(2 * tw + 1) * 64 + lane] = ai[0][0];
+#pragma unroll
+    for (int t = 0; t < CB; ++t) {
+      red[slot(tw, t, 0)] = ar[0][t];
+      red[slot(tw, t, 1)] = ai[0][t];
+    }
   };
-  auto gather = [&]() {
+  auto gather = [&]() {  // the owner sums the P partials in part order
     const int t0 = tw - part;
-    ar[0][0] = red[(2 * t0) * 64 + lane];
-    ai[0][0] = red[(2 * t0 + 1) * 64 + lane];
-    for (int q = 1; q < P; ++q) {
-      ar[0][0] += red[(2 * (t0 + q)) * 64 + lane];
-      ai[0][0] += red[(2 * (t0 + q) + 1) * 64 + lane];
+#pragma unroll
+    for (int t = 0; t < CB; ++t) {
+      ar[0][t] = red[slot(t0, t, 0)];
+      ai[0][t] = red[slot(t0, t, 1)];
+      for (int q = 1; q < P; ++q) {
+        ar[0][t] += red[slot(t0 + q, t, 0)];
+        ai[0][t] += red[slot(t0 + q, t, 1)];
+      }
     }
   };
   // T_0 = P1 Phi, S = T_0
-  stripes_times_b<T, P1C, 1, 1, VEC>(P1, N, N, NS, NS, st, fr, fk, Br, Bi, LDK, ar, ai, gb, ge,
-                                      rag);
+  stripes_times_b<T, P1C, 1, CB, VEC>(P1, N, N, NS, NS, st, fr, fk, Br, Bi, LDK, ar, ai, gb, ge,
+                                       rag);
   put();
   __syncthreads();
   if (owner) {
     gather();
-    sr[0][0] = ar[0][0];
-    si[0][0] = ai[0][0];
-    write_b<T, 1, 1>(Br, Bi, LDK, N, NS, NS, st, fr, fk, ar, ai);
+#pragma unroll
+    for (int t = 0; t < CB; ++t) {
+      red[sslot(t, 0)] = ar[0][t];
+      red[sslot(t, 1)] = ai[0][t];
+    }
+    write_b<T, 1, CB>(Br, Bi, LDK, N, NS, NS, st, fr, fk, ar, ai);
   }
   __syncthreads();
   for (int k = 1; k <= order; ++k) {
-    stripes_times_b<T, true, 1, 1, VEC, CONJV, M3>(Vw, N, N, NS, NS, st, fr, fk, Br, Bi, LDK, ar,
-                                                   ai, gb, ge, rag);
+    stripes_times_b<T, true, 1, CB, VEC, CONJV, M3>(Vw, N, N, NS, NS, st, fr, fk, Br, Bi, LDK, ar,
+                                                    ai, gb, ge, rag);
     put();
     __syncthreads();
     if (owner) {
       gather();
       const T fi = (T)(sgn / (double)k);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const T tr = -(fi * ai[0][0][r]), ti = fi * ar[0][0][r];
-        ar[0][0][r] = tr;
-        ai[0][0][r] = ti;
-        sr[0][0][r] = sr[0][0][r] + tr;
-        si[0][0][r] = si[0][0][r] + ti;
+      for (int t = 0; t < CB; ++t) {
+        V4 sr = red[sslot(t, 0)], si = red[sslot(t, 1)];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const T tr = -(fi * ai[0][t][r]), ti = fi * ar[0][t][r];
+          ar[0][t][r] = tr;
+          ai[0][t][r] = ti;
+          sr[r] = sr[r] + tr;
+          si[r] = si[r] + ti;
+        }
+        red[sslot(t, 0)] = sr;
+        red[sslot(t, 1)] = si;
       }
-      write_b<T, 1, 1>(Br, Bi, LDK, N, NS, NS, st, fr, fk, ar, ai);
+      write_b<T, 1, CB>(Br, Bi, LDK, N, NS, NS, st, fr, fk, ar, ai);
     }
     __syncthreads();
   }
-  if (owner) write_b<T, 1, 1>(Br, Bi, LDK, N, NS, NS, st, fr, fk, sr, si);
+  if (owner) {
+#pragma unroll
+    for (int t = 0; t < CB; ++t) {
+      ar[0][t] = red[sslot(t, 0)];
+      ai[0][t] = red[sslot(t, 1)];
+    }
+    write_b<T, 1, CB>(Br, Bi, LDK, N, NS, NS, st, fr, fk, ar, ai);
+  }
   __syncthreads();
   // Phi = P1 S, scaled back by 2^e
-  stripes_times_b<T, P1C, 1, 1, VEC>(P1, N, N, NS, NS, st, fr, fk, Br, Bi, LDK, ar, ai, gb, ge,
-                                      rag);
+  stripes_times_b<T, P1C, 1, CB, VEC>(P1, N, N, NS, NS, st, fr, fk, Br, Bi, LDK, ar, ai, gb, ge,
+                                       rag);
   put();
   __syncthreads();
   if (owner) {
     gather();
     const double sb = e == 0 ? 1.0 : ldexp(1.0, e);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int row = st * 16 + gemm::Acc<T>::row(fk, r), col = c0 + fr;
-      if (row < N && col < n)
-        Pw[(long)row * n + col] = z64{(double)ar[0][0][r] * sb, (double)ai[0][0][r] * sb};
-    }
+    for (int t = 0; t < CB; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = st * 16 + gemm::Acc<T>::row(fk, r), col = c0 + 16 * t + fr;
+        if (row < N && col < n)
+          Pw[(long)row * n + col] = z64{(double)ar[0][t][r] * sb, (double)ai[0][t][r] * sb};
+      }
   }
 }
 
@@ -613,7 +641,7 @@ __device__ __attribute__((always_inline)) inline void prop_share_split(
 // single tile, so the four SIMDs carry nearly equal shares (N = 200, CB = 3: 10, 10, 10 and 9
 // stripe-tiles instead of 12, 9, 9 and 9).  Every output tile is multiplied in the same order
 // either way, so the variants give the same bits.
-// KT (CB = 1, SPW = 1): NW (a multiple of 4) wavefronts own the first NW stripes, and the
+// KT (SPW = 1): NW (a multiple of 4) wavefronts own the first NW stripes, and the
 // remaining R = NS - NW (1 or 2) stripes are each split over 4 / R more wavefronts by k
 // (prop_share_split), so the four SIMDs carry equal shares.  The split stripes sum their k range
 // in parts, so their bits differ from the unsplit kernel's.
@@ -691,8 +719,9 @@ __global__ void __launch_bounds__(1024)
   } else if constexpr (KT) {
     const int tw = wave - NW, P = 4 / (NS - NW);
     auto* red = reinterpret_cast<typename gemm::Acc<T>::v4*>(Bi + BC * LDK);
-    prop_share_split<T, P1C, CONJV, M3, VEC>(N, n, order, sgn, P1, Vw, Pw, c0, NS, NW + tw / P,
-                                             tw % P, P, tw, fr, fk, Br, Bi, LDK, e, red);
+    prop_share_split<T, P1C, CB, CONJV, M3, VEC>(N, n, order, sgn, P1, Vw, Pw, c0, NS,
+                                                 NW + tw / P, tw % P, P, tw, fr, fk, Br, Bi, LDK,
+                                                 e, red);
   } else if constexpr (TAIL) {
     const int u = wave - NW, st = NW + u / CB, tt = u % CB;
     prop_share<T, P1C, 1, 1, CONJV, M3, VEC>(N, n, order, sgn, P1, Vw, Pw, c0 + 16 * tt, NS, NW,
@@ -718,15 +747,21 @@ inline int max_lds_per_block() {
 
 // TAIL: the first NS rounded down to a multiple of 4 stripes take one wavefront each, the rest
 // one wavefront per (stripe, tile)
+// LDS for the split wavefronts: partials (4 wavefronts x CB tiles x (re, im) x 64 lanes) and the
+// running sums of the (at most 2) owners (CB tiles x (re, im) x 64 lanes each)
+template <typename T>
+constexpr size_t split_lds(int cb) {
+  return (size_t)6 * cb * 2 * 64 * sizeof(typename gemm::Acc<T>::v4);
+}
+
 // KT: the first NS rounded down to a multiple of 4 stripes take one wavefront each and four more
-// wavefronts split the rest by k (16 KB of LDS for their partials)
+// wavefronts split the rest by k
 template <typename T, bool P1C, int SPW, int CB, bool CONJV, bool M3, bool TAIL = false,
           bool KT = false>
 void launch_fused(hipStream_t s, const PropagateArgs& a, const FusedShape& f, const T* P1, int vecP,
                   const T* V, int vecV, z64* Phi) {
-  static_assert(!KT || (CB == 1 && SPW == 1 && !TAIL), "the split tail takes one tile per wave");
-  const size_t lds = 2 * 16 * CB * (size_t)f.LDK * sizeof(T) +
-                     (KT ? 4 * 2 * 64 * sizeof(typename gemm::Acc<T>::v4) : 0);
+  static_assert(!KT || (SPW == 1 && !TAIL), "the split tail takes one stripe per wave");
+  const size_t lds = 2 * 16 * CB * (size_t)f.LDK * sizeof(T) + (KT ? split_lds<T>(CB) : 0);
   auto kern = (vecP && vecV) ? k_prop_taylor<T, P1C, SPW, CB, CONJV, M3, true, TAIL, KT>
                              : k_prop_taylor<T, P1C, SPW, CB, CONJV, M3, false, TAIL, KT>;
   if (lds > 64 * 1024)
@@ -769,20 +804,36 @@ bool propagate_fused(Context& c, const PropagateArgs& a, const T* P1, bool p1c, 
   const int maxcb = sizeof(T) == 4 && f.spw == 1 ? 3 : 1;
   struct Cand {
     int cb;
-    bool tail;
+    bool tail, split;
   };
   Cand cands[5];
   int ncand = 0;
   auto fits = [&](int cb) {
     return lds1 * cb <= (size_t)max_lds_per_block() && (cb == 1 || 16 * (cb - 1) < n);
   };
-  for (int cb = 1; cb <= maxcb; ++cb)
-    if (fits(cb)) cands[ncand++] = {cb, false};
-  // FP32 with one stripe per wave and a stripe count that is not a multiple of 4: the tail
-  // arrangement, which spreads the last stripes' tiles over the SIMDs
-  if (sizeof(T) == 4 && f.spw == 1 && NS > 4 && NS % 4 != 0)
-    for (int cb = 2; cb <= maxcb; ++cb)
-      if (fits(cb) && (NS & ~3) + (NS % 4) * cb <= 16) cands[ncand++] = {cb, true};
+  // FP64, one stripe per wave and 1 or 2 stripes past a multiple of 4: those split by k over four
+  // more wavefronts (AFQ_PROP_SPLIT=0 keeps one wavefront per stripe).  FP32 balances the SIMDs
+  // by the tail arrangement instead, which measured the same as the split there (0.995 vs
+  // 0.997 ms at N = 200, n = 40) without the extra barrier.  Every split candidate sums the split
+  // stripes in the same parts, so they share their bits.
+  const char* kte = std::getenv("AFQ_PROP_SPLIT");
+  const int rem = NS % 4;
+  const bool kt0 = sizeof(T) == 8 && !adj && f.spw == 1 && (rem == 1 || rem == 2) && NS >= 5 &&
+                   (NS & ~3) + 4 <= 16 && !(kte && kte[0] == '0');
+  if (kt0)
+    for (int cb = 1; cb <= maxcb; ++cb)
+      if (fits(cb) && lds1 * cb + split_lds<T>(cb) <= (size_t)max_lds_per_block())
+        cands[ncand++] = {cb, false, true};
+  const bool kt = ncand > 0;
+  if (!kt) {
+    for (int cb = 1; cb <= maxcb; ++cb)
+      if (fits(cb)) cands[ncand++] = {cb, false, false};
+    // FP32 with one stripe per wave and a stripe count that is not a multiple of 4: the tail
+    // arrangement, which spreads the last stripes' tiles over the SIMDs
+    if (sizeof(T) == 4 && f.spw == 1 && NS > 4 && NS % 4 != 0)
+      for (int cb = 2; cb <= maxcb; ++cb)
+        if (fits(cb) && (NS & ~3) + (NS % 4) * cb <= 16) cands[ncand++] = {cb, true, false};
+  }
   if (ncand == 0) return false;
   AFQ_CHECK((long)W * ceil_div(n, 16) < (1L << 31), kErrShape, "grid too large");
   const size_t esz = sizeof(T) * (p1c ? 2 : 1);
@@ -797,16 +848,9 @@ bool propagate_fused(Context& c, const PropagateArgs& a, const T* P1, bool p1c, 
   // keeps the FP32 rounding of the terms closer to the FP64 path's)
   const char* m3e = std::getenv("AFQ_PROP_M3");
   const bool m3 = sizeof(T) == 8 || (m3e && m3e[0] == '1');
-  // FP64 (one stripe and one column tile per wavefront): split the stripes past a multiple of 4
-  // by k when 1 or 2 remain (AFQ_PROP_SPLIT=0 keeps one wavefront per stripe)
-  const char* kte = std::getenv("AFQ_PROP_SPLIT");
-  const int rem = NS % 4;
-  const bool kt = sizeof(T) == 8 && f.spw == 1 && (rem == 1 || rem == 2) && NS >= 5 &&
-                  (NS & ~3) + 4 <= 16 && !(kte && kte[0] == '0') &&
-                  lds1 + 4 * 2 * 64 * 4 * sizeof(T) <= (size_t)max_lds_per_block();
   auto run = [&](int i, z64* Phi) {
     const int cb = cands[i].cb;
-    const bool tail = cands[i].tail;
+    const bool tail = cands[i].tail, split = cands[i].split;
     constexpr int SPW2 = sizeof(T) == 4 ? 2 : 1;
     if constexpr (sizeof(T) == 8) {
       if (adj) {  // FP64 takes one stripe per wave and one column tile
@@ -818,7 +862,26 @@ bool propagate_fused(Context& c, const PropagateArgs& a, const T* P1, bool p1c, 
 #define AFQ_PT(P1C, SPW, CB, TL)                                                         \
   (m3 ? launch_fused<T, P1C, SPW, CB, false, true, TL>(s, a, f, P1, vecP, Vt, vecV, Phi) \
       : launch_fused<T, P1C, SPW, CB, false, sizeof(T) == 8, TL>(s, a, f, P1, vecP, Vt, vecV, Phi))
-    if (f.spw == 1) {
+#define AFQ_PS(P1C, CB)                                                                          \
+  (m3 ? launch_fused<T, P1C, 1, CB, false, true, false, true>(s, a, f, P1, vecP, Vt, vecV, Phi) \
+      : launch_fused<T, P1C, 1, CB, false, sizeof(T) == 8, false, true>(s, a, f, P1, vecP, Vt,    \
+                                                                       vecV, Phi))
+    if (split) {
+      if constexpr (sizeof(T) == 4) {
+        if (p1c) {
+          if (cb == 1) AFQ_PS(true, 1);
+          else if (cb == 2) AFQ_PS(true, 2);
+          else AFQ_PS(true, 3);
+        } else {
+          if (cb == 1) AFQ_PS(false, 1);
+          else if (cb == 2) AFQ_PS(false, 2);
+          else AFQ_PS(false, 3);
+        }
+      } else {
+        if (p1c) AFQ_PS(true, 1);
+        else AFQ_PS(false, 1);
+      }
+    } else if (f.spw == 1) {
       if constexpr (sizeof(T) == 4) {
         if (p1c) {
           if (cb == 1) AFQ_PT(true, 1, 1, false);
@@ -829,9 +892,6 @@ bool propagate_fused(Context& c, const PropagateArgs& a, const T* P1, bool p1c, 
           else if (cb == 2) tail ? AFQ_PT(false, 1, 2, true) : AFQ_PT(false, 1, 2, false);
           else tail ? AFQ_PT(false, 1, 3, true) : AFQ_PT(false, 1, 3, false);
         }
-      } else if (kt) {  // FP64: the stripes past a multiple of 4 split by k over 4 wavefronts
-        if (p1c) launch_fused<T, true, 1, 1, false, true, false, true>(s, a, f, P1, vecP, Vt, vecV, Phi);
-        else launch_fused<T, false, 1, 1, false, true, false, true>(s, a, f, P1, vecP, Vt, vecV, Phi);
       } else {
         if (p1c) AFQ_PT(true, 1, 1, false);
         else AFQ_PT(false, 1, 1, false);
@@ -841,6 +901,7 @@ bool propagate_fused(Context& c, const PropagateArgs& a, const T* P1, bool p1c, 
       else AFQ_PT(false, SPW2, 1, false);
     }
 #undef AFQ_PT
+#undef AFQ_PS
   };
   tuned_on_copy(c, tune_key(adj ? "prop_fused_adj" : "prop_fused", sizeof(T), W, N, n, N), ncand,
                 a.Phi, (size_t)W * a.phi_bs, run);
