@@ -192,9 +192,12 @@ __global__ void __launch_bounds__(64 * kWaves)
     for (int t = 0; t < NGF; ++t) {
       const int gn = gid(min(t + 1, NGF - 1));  // the last group reloads itself
       load<T, RB, CB, true>(nxt, ab + gn * AS, bb + gn * BS, o, 0);
-      // the prefetch stays ahead of the MFMAs and the copy behind them
+      // the prefetch stays ahead of the MFMAs and the copy behind them; the wavefront in its
+      // MFMA phase issues ahead of those still loading
       __builtin_amdgcn_sched_barrier(0);
+      __builtin_amdgcn_s_setprio(1);
       mma<T, RB, CB>(cur, cr, ci);
+      __builtin_amdgcn_s_setprio(0);
       __builtin_amdgcn_sched_barrier(0);
       cur = nxt;
     }

@@ -311,6 +311,8 @@ __device__ inline void group_mfma(const AFrag<T, CPLX> (&f)[SPW], int g, int NS,
                                   typename gemm::Acc<T>::v4 (&ai)[SPW][CB],
                                   typename gemm::Acc<T>::v4 (&a3)[SPW][CB]) {
   using gemm::Acc;
+  // the wavefront in its MFMA phase issues ahead of those still loading
+  __builtin_amdgcn_s_setprio(1);
 #pragma unroll
   for (int t = 0; t < CB; ++t) {
     T br[4], bi[4];
@@ -339,6 +341,7 @@ __device__ inline void group_mfma(const AFrag<T, CPLX> (&f)[SPW], int g, int NS,
       }
     }
   }
+  __builtin_amdgcn_s_setprio(0);
 }
 
 // a stripe past the last one loads the last one (its products are never used), so every load

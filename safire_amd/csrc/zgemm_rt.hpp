@@ -58,9 +58,11 @@ __device__ inline void load_step(Frag<RB, CB>& f, const OA& A, long ab, const in
   }
 }
 
+// (the wavefront in its MFMA phase issues ahead of those still loading)
 template <int RB, int CB>
 __device__ inline void mma_step(const Frag<RB, CB>& f, f64x4 (&cr)[RB][CB], f64x4 (&ci)[RB][CB]) {
   using Acc = gemm::Acc<double>;
+  __builtin_amdgcn_s_setprio(1);
 #pragma unroll
   for (int j = 0; j < CB; ++j)
 #pragma unroll
@@ -70,6 +72,7 @@ __device__ inline void mma_step(const Frag<RB, CB>& f, f64x4 (&cr)[RB][CB], f64x
       ci[i][j] = Acc::mfma(f.ar[i], f.bi[j], ci[i][j]);
       ci[i][j] = Acc::mfma(f.ai[i], f.br[j], ci[i][j]);
     }
+  __builtin_amdgcn_s_setprio(0);
 }
 
 template <int RB, int CB, class OA, class OB, class EPI>
