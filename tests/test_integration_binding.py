@@ -1,5 +1,5 @@
-"""The SAFIRE-side binding shown in INTEGRATION.md (tests/integration/hip_binding.hpp) compiles
-against include/afq.h, links with libafq_hip.so, and maps a walker buffer in the reference's
+"""The SAFIRE-side binding (tests/integration/hip_binding.hpp: the adapter a SAFIRE build adds
+to call this library) compiles against include/afq.h, links with libafq_hip.so, and maps a walker buffer in the reference's
 row layout onto afq_walkers (walker strides = the row, each property at its data_displ
 column).  Host only: the program never calls into the GPU library."""
 import os
